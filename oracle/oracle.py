@@ -1,0 +1,243 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so built from stokes_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg, always as the checker or the timed CPU baseline,
+never as part of the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(
+            os.path.join(HERE, "stokes_oracle.c")):
+        subprocess.check_call(["make", "-s", "-C", HERE, "-B" if force else "all"])
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        dp = C.POINTER(C.c_double)
+        fp = C.POINTER(C.c_float)
+        ip = C.POINTER(C.c_int)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_int]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_set_vonly.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_knobs.argtypes = [C.c_void_p] + [C.c_double] * 7 + [C.c_int] + [C.c_double] * 3
+        L.orc_level_info.argtypes = [C.c_void_p, C.c_int, ip]
+        L.orc_grid.restype = dp
+        L.orc_grid.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_array.restype = C.c_void_p
+        L.orc_array.argtypes = [C.c_void_p, C.c_char_p]
+        L.orc_bc_line.argtypes = [C.c_void_p] + [C.c_int] * 7 + [C.c_float, C.c_float] + [C.c_int] * 3
+        L.orc_ronurecalc.argtypes = [C.c_void_p, C.c_long, fp, fp, fp, fp, fp, C.c_double]
+        L.orc_ronurestrict.argtypes = [C.c_void_p, C.c_int]
+        L.orc_assemble.argtypes = [C.c_void_p]
+        L.orc_get_row.argtypes = [C.c_void_p, C.c_long, ip, dp, dp]
+        L.orc_gausmat4.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_residual.argtypes = [C.c_void_p, C.c_int]
+        L.orc_sweep_tiles.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_bc_pass.argtypes = [C.c_void_p, C.c_int, dp]
+        L.orc_vprestrict.argtypes = [C.c_void_p, C.c_int]
+        L.orc_vpprolong.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_coarse_factor.argtypes = [C.c_void_p]
+        L.orc_coarse_solve.argtypes = [C.c_void_p]
+        L.orc_dense_lu.argtypes = [C.c_int, dp, ip]
+        L.orc_dense_lu_solve.argtypes = [C.c_int, dp, ip, dp]
+        L.orc_vcycle.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_relres.restype = C.c_double
+        L.orc_relres.argtypes = [C.c_void_p, dp]
+        L.orc_solve.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_double, C.c_int, dp]
+        L.orc_band_solve.argtypes = [C.c_void_p, dp, C.POINTER(C.c_long)]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class Oracle:
+    """One hierarchy + assembled operator held by the C oracle."""
+
+    def __init__(self, gx, gy, gz, L, periodic=(0, 0, 0)):
+        self.L = L
+        self._g = [np.ascontiguousarray(a, dtype=np.float64) for a in (gx, gy, gz)]
+        self.h = lib().orc_create(len(gx), len(gy), len(gz), L, _dp(self._g[0]), _dp(self._g[1]),
+                                  _dp(self._g[2]), *periodic)
+        if not self.h:
+            raise ValueError("bad hierarchy")
+        self.info = []
+        for l in range(L + 1):
+            buf = np.zeros(6, dtype=np.int32)
+            lib().orc_level_info(self.h, l, _ip(buf))
+            self.info.append(tuple(int(v) for v in buf))
+        self.ntot = self.info[L][5] + self.info[L][4]
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().orc_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def knobs(self, p0koef=0.1, p1koef=0.0, p2koef=0.1, v0koef=1.0, v1koef=1.0, nubeg=1.0,
+              nuend=1e4, viscmod=0, gravity=(0.0, 0.0, 0.0)):
+        lib().orc_set_knobs(self.h, p0koef, p1koef, p2koef, v0koef, v1koef, nubeg, nuend, viscmod,
+                            *gravity)
+
+    def grid(self, l, axis):
+        n = self.info[l][axis]
+        p = lib().orc_grid(self.h, l, axis)
+        return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+    def arr(self, name):
+        sizes = {"nu": self.ntot, "ro": self.ntot, "nd": self.ntot, "nxx": self.info[0][4],
+                 "nxy": self.info[0][4], "nxz": self.info[0][4], "nyz": self.info[0][4],
+                 "bondm": 4 * self.ntot, "sol0": 4 * self.ntot, "sol1": 4 * self.ntot,
+                 "fre0": 4 * self.ntot, "fre1": 4 * self.ntot, "val1": 57 * self.ntot,
+                 "lin0": 57 * self.ntot, "num0": 4 * self.ntot, "pos0": 4 * self.ntot}
+        types = {"bondm": C.c_int, "lin0": C.c_int, "num0": C.c_int, "pos0": C.c_int,
+                 "nu": C.c_float, "ro": C.c_float, "nd": C.c_float, "nxx": C.c_float,
+                 "nxy": C.c_float, "nxz": C.c_float, "nyz": C.c_float}
+        p = lib().orc_array(self.h, name.encode())
+        if name in ("bondv1", "bondn1"):
+            nb = self.nbond
+            if name == "bondv1":
+                return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), shape=(2 * (nb + 1),))
+            return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_int)), shape=(nb + 1,))
+        t = types.get(name, C.c_double)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(t)), shape=(sizes[name],))
+
+    @property
+    def nbond(self):
+        p = lib().orc_array(self.h, b"nbond")
+        return C.cast(p, C.POINTER(C.c_int))[0]
+
+    def bc_lines(self, lines):
+        for ln in lines:
+            lib().orc_bc_line(self.h, *[int(v) for v in ln[:7]], float(ln[7]), float(ln[8]),
+                              *[int(v) for v in ln[9:12]])
+
+    def ronurecalc(self, mx, my, mz, mnu, mro, nukoef=0.0):
+        a = [np.ascontiguousarray(v, dtype=np.float32) for v in (mx, my, mz, mnu, mro)]
+        rc = lib().orc_ronurecalc(self.h, len(a[0]), *[_fp(v) for v in a], nukoef)
+        if rc:
+            raise RuntimeError("ronurecalc failed")
+
+    def ronurestrict_all(self):
+        for l in range(1, self.L + 1):
+            lib().orc_ronurestrict(self.h, l)
+
+    def ronurestrict(self, l):
+        lib().orc_ronurestrict(self.h, l)
+
+    def assemble(self):
+        lib().orc_assemble(self.h)
+
+    def row(self, r):
+        idx = np.zeros(64, dtype=np.int32)
+        val = np.zeros(64, dtype=np.float64)
+        rhs = np.zeros(1, dtype=np.float64)
+        n = lib().orc_get_row(self.h, r, _ip(idx), _dp(val), _dp(rhs))
+        return idx[:n].copy(), val[:n].copy(), float(rhs[0])
+
+    def gausmat4(self, am, l):
+        lib().orc_gausmat4(self.h, am, l)
+
+    def residual(self, l):
+        lib().orc_residual(self.h, l)
+
+    def sweep_tiles(self, am, l, ty=64, tx=4, tz=4):
+        lib().orc_sweep_tiles(self.h, am, l, ty, tx, tz)
+
+    def vprestrict(self, l):
+        lib().orc_vprestrict(self.h, l)
+
+    def vpprolong(self, l, ynreset=0):
+        lib().orc_vpprolong(self.h, l, ynreset)
+
+    def coarse_factor(self):
+        return lib().orc_coarse_factor(self.h)
+
+    def coarse_solve(self):
+        return lib().orc_coarse_solve(self.h)
+
+    def vcycle(self, pre, post, mode=0, tile=(64, 4, 4)):
+        p = np.ascontiguousarray(pre, dtype=np.int32)
+        q = np.ascontiguousarray(post, dtype=np.int32)
+        lib().orc_vcycle(self.h, _ip(p), _ip(q), mode, *tile)
+
+    def relres(self):
+        b = np.zeros(3)
+        r = lib().orc_relres(self.h, _dp(b))
+        return r, b
+
+    def solve(self, pre, post, mode=0, tol=1e-8, maxcyc=200):
+        p = np.ascontiguousarray(pre, dtype=np.int32)
+        q = np.ascontiguousarray(post, dtype=np.int32)
+        hist = np.zeros(maxcyc + 1)
+        it = lib().orc_solve(self.h, _ip(p), _ip(q), mode, tol, maxcyc, _dp(hist))
+        return it, hist[: abs(it) + 1]
+
+    def band_solve(self):
+        x = np.zeros(4 * self.info[0][4])
+        bw = C.c_long(0)
+        rc = lib().orc_band_solve(self.h, _dp(x), C.byref(bw))
+        return rc, x, bw.value
+
+
+def dense_lu(a):
+    a = np.array(a, dtype=np.float64, order="C")
+    n = a.shape[0]
+    piv = np.zeros(n, dtype=np.int32)
+    rc = lib().orc_dense_lu(n, _dp(a), _ip(piv))
+    return rc, a, piv
+
+
+def dense_lu_solve(lu, piv, b):
+    b = np.array(b, dtype=np.float64)
+    lib().orc_dense_lu_solve(lu.shape[0], _dp(lu), _ip(piv), _dp(b))
+    return b
+
+
+def from_problem(pb, L=None):
+    """Oracle instance for a setups.Problem: hierarchy, knobs, BCs, markers,
+    restricted properties, assembled operator and factored coarsest level."""
+    L = pb.levels if L is None else L
+    o = Oracle(pb.gx, pb.gy, pb.gz, L)
+    k = pb.knobs
+    o.knobs(k["p0koef"], k["p1koef"], k["p2koef"], k["v0koef"], k["v1koef"], k["nubeg"], k["nuend"],
+            k["viscmod"], pb.gravity)
+    o.bc_lines(pb.bc)
+    o.ronurecalc(pb.mx, pb.my, pb.mz, pb.mnu, pb.mro)
+    o.ronurestrict_all()
+    o.assemble()
+    rc = o.coarse_factor()
+    if rc:
+        raise RuntimeError(f"coarse LU failed rc={rc}")
+    return o

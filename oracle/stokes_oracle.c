@@ -1,0 +1,1413 @@
+/*
+ * stokes_oracle.c -- CPU restatement of the reference Stokes multigrid path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the oracle that the HIP library is
+ * checked against and the CPU baseline that bench.py times; the product
+ * library never links it.  Each function names the reference file:line it
+ * restates.  Arithmetic order and storage types follow the reference's
+ * single-thread (OMP_NUM_THREADS=1) execution so that results can be compared
+ * exactly where the reference itself is deterministic.
+ */
+#include "stokes_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ABSV(a) ((a) >= 0 ? (a) : -(a))
+#define NID(o, l, m1, m2, m3) ((o)->mgp[l] + (long)(m3) * (o)->mgxy[l] + (long)(m1) * (o)->mgy[l] + (m2))
+
+/* ------------------------------------------------------------------------- */
+/* Hierarchy: parallel/load3mg.c:2011-2195 (gridcheck)                        */
+/* ------------------------------------------------------------------------- */
+orc_t *orc_create(int nx, int ny, int nz, int L, const double *gx, const double *gy,
+                  const double *gz, int xper, int yper, int zper) {
+  if (L < 0 || L >= ORC_MAXLVL || nx < 5 || ny < 5 || nz < 5) return NULL;
+  orc_t *o = (orc_t *)calloc(1, sizeof(orc_t));
+  o->L = L;
+  o->xper = xper;
+  o->yper = yper;
+  o->zper = zper;
+  o->mgn[0] = (long)nx * ny * nz;
+  o->mgp[0] = 0;
+  o->mgs[0] = 1;
+  o->mgx[0] = nx;
+  o->mgy[0] = ny;
+  o->mgz[0] = nz;
+  o->mgxy[0] = nx * ny;
+  /* level sizes, load3mg.c:2064-2075 */
+  for (int l = 1; l <= L; l++) {
+    o->mgp[l] = o->mgp[l - 1] + o->mgn[l - 1];
+    o->mgs[l] = o->mgs[l - 1] * 2;
+    o->mgx[l] = ((o->mgx[l - 1] - 5) / 2) + 5;
+    o->mgy[l] = ((o->mgy[l - 1] - 5) / 2) + 5;
+    o->mgz[l] = ((o->mgz[l - 1] - 5) / 2) + 5;
+    o->mgxy[l] = o->mgx[l] * o->mgy[l];
+    o->mgn[l] = (long)o->mgx[l] * o->mgy[l] * o->mgz[l];
+  }
+  /* coarse coordinates keep the 3 outer lines per side, load3mg.c:2078-2190 */
+  const double *g[3] = {gx, gy, gz};
+  int nf[3] = {nx, ny, nz};
+  for (int l = 0; l <= L; l++) {
+    int nl[3] = {o->mgx[l], o->mgy[l], o->mgz[l]};
+    double **dst[3] = {&o->mggx[l], &o->mggy[l], &o->mggz[l]};
+    for (int a = 0; a < 3; a++) {
+      double *c = (double *)calloc(nl[a], sizeof(double));
+      c[0] = g[a][0];
+      c[1] = g[a][1];
+      c[2] = g[a][2];
+      c[nl[a] - 1] = g[a][nf[a] - 1];
+      c[nl[a] - 2] = g[a][nf[a] - 2];
+      c[nl[a] - 3] = g[a][nf[a] - 3];
+      for (int n2 = 3; n2 < nl[a] - 3; n2++) c[n2] = g[a][(n2 - 2) * o->mgs[l] + 2];
+      *dst[a] = c;
+    }
+  }
+  o->xsize = gx[nx - 1] - gx[0];
+  o->ysize = gy[ny - 1] - gy[0];
+  o->zsize = gz[nz - 1] - gz[0];
+  long nt = o->mgp[L] + o->mgn[L];
+  o->ntot = nt;
+  o->nu = (float *)calloc(nt, sizeof(float));
+  o->ro = (float *)calloc(nt, sizeof(float));
+  o->nd = (float *)calloc(nt, sizeof(float));
+  o->nxx = (float *)calloc(o->mgn[0], sizeof(float));
+  o->nxy = (float *)calloc(o->mgn[0], sizeof(float));
+  o->nxz = (float *)calloc(o->mgn[0], sizeof(float));
+  o->nyz = (float *)calloc(o->mgn[0], sizeof(float));
+  o->bondm = (int *)calloc(4 * nt, sizeof(int));
+  o->capbond = 1024;
+  o->bondv1 = (float *)calloc(2 * (o->capbond + 1), sizeof(float));
+  o->bondn1 = (int *)calloc(o->capbond + 1, sizeof(int));
+  o->sol0 = (double *)calloc(4 * nt, sizeof(double));
+  o->sol1 = (double *)calloc(4 * nt, sizeof(double));
+  o->fre0 = (double *)calloc(4 * nt, sizeof(double));
+  o->fre1 = (double *)calloc(4 * nt, sizeof(double));
+  o->tmp = (double *)calloc(4 * nt, sizeof(double));
+  o->val1 = (double *)calloc(57 * nt, sizeof(double));
+  o->lin0 = (int *)calloc(57 * nt, sizeof(int));
+  o->num0 = (int *)calloc(4 * nt, sizeof(int));
+  o->pos0 = (int *)calloc(4 * nt, sizeof(int));
+  /* defaults: shipped effective knobs (parallel/mode.t3c:87-95, per-file overrides) */
+  orc_set_knobs(o, 0.10, 0.0, 0.10, 1.0, 1.0, 1.0, 1e4, 0, 0.0, 0.0, 0.0);
+  return o;
+}
+
+void orc_destroy(orc_t *o) {
+  if (!o) return;
+  for (int l = 0; l <= o->L; l++) {
+    free(o->mggx[l]);
+    free(o->mggy[l]);
+    free(o->mggz[l]);
+  }
+  free(o->nu); free(o->ro); free(o->nd);
+  free(o->nxx); free(o->nxy); free(o->nxz); free(o->nyz);
+  free(o->bondm); free(o->bondv1); free(o->bondn1);
+  free(o->sol0); free(o->sol1); free(o->fre0); free(o->fre1); free(o->tmp);
+  free(o->val1); free(o->lin0); free(o->num0); free(o->pos0);
+  free(o->lu); free(o->piv);
+  free(o);
+}
+
+void orc_set_vonly(orc_t *o, int flag) { o->vonly = flag; }
+
+void orc_set_knobs(orc_t *o, double p0, double p1, double p2, double v0, double v1,
+                   double nubeg, double nuend, int viscmod, double gxk, double gyk, double gzk) {
+  o->p0koef = p0; o->p1koef = p1; o->p2koef = p2; o->v0koef = v0; o->v1koef = v1;
+  o->nubeg = nubeg; o->nuend = nuend; o->viscmod = viscmod;
+  o->gxk = gxk; o->gyk = gyk; o->gzk = gzk;
+}
+
+int orc_level_info(const orc_t *o, int l, int *out6) {
+  if (l < 0 || l > o->L) return -1;
+  out6[0] = o->mgx[l]; out6[1] = o->mgy[l]; out6[2] = o->mgz[l];
+  out6[3] = o->mgxy[l]; out6[4] = (int)o->mgn[l]; out6[5] = (int)o->mgp[l];
+  return 0;
+}
+
+double *orc_grid(orc_t *o, int l, int axis) {
+  if (l < 0 || l > o->L) return NULL;
+  return axis == 0 ? o->mggx[l] : axis == 1 ? o->mggy[l] : o->mggz[l];
+}
+
+void *orc_array(orc_t *o, const char *n) {
+  if (!strcmp(n, "nu")) return o->nu;
+  if (!strcmp(n, "ro")) return o->ro;
+  if (!strcmp(n, "nd")) return o->nd;
+  if (!strcmp(n, "nxx")) return o->nxx;
+  if (!strcmp(n, "nxy")) return o->nxy;
+  if (!strcmp(n, "nxz")) return o->nxz;
+  if (!strcmp(n, "nyz")) return o->nyz;
+  if (!strcmp(n, "bondm")) return o->bondm;
+  if (!strcmp(n, "bondv1")) return o->bondv1;
+  if (!strcmp(n, "bondn1")) return o->bondn1;
+  if (!strcmp(n, "sol0")) return o->sol0;
+  if (!strcmp(n, "sol1")) return o->sol1;
+  if (!strcmp(n, "fre0")) return o->fre0;
+  if (!strcmp(n, "fre1")) return o->fre1;
+  if (!strcmp(n, "val1")) return o->val1;
+  if (!strcmp(n, "lin0")) return o->lin0;
+  if (!strcmp(n, "num0")) return o->num0;
+  if (!strcmp(n, "pos0")) return o->pos0;
+  if (!strcmp(n, "nbond")) return &o->nbond;
+  return NULL;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Boundary equations on every level: parallel/in3mg.c:234-390                */
+/* var: 0 P, 1 vx, 2 vy, 3 vz.  Fine-grid index ranges are mapped to each     */
+/* level keeping the 3 outer lines (in3mg.c:307-324); |shift|>1 = periodic    */
+/* (in3mg.c:339-341).  Later equations overwrite bondm of earlier ones.      */
+/* ------------------------------------------------------------------------- */
+static int lvl_index(int m, int nfine, int nlev, long mgs) {
+  int r = (m - 2) / (int)mgs + 2;
+  if (m < 3) r = m;
+  if (m > nfine - 4) r = nlev - (nfine - m);
+  return r;
+}
+
+int orc_bc_line(orc_t *o, int var, int m10, int m11, int m20, int m21, int m30, int m31,
+                float cnst, float koef, int dm1, int dm2, int dm3) {
+  if (var < 0 || var > 3) return -1;
+  for (int l = 0; l <= o->L; l++) {
+    int m1min = lvl_index(m10, o->mgx[0], o->mgx[l], o->mgs[l]);
+    int m1max = lvl_index(m11, o->mgx[0], o->mgx[l], o->mgs[l]);
+    int m2min = lvl_index(m20, o->mgy[0], o->mgy[l], o->mgs[l]);
+    int m2max = lvl_index(m21, o->mgy[0], o->mgy[l], o->mgs[l]);
+    int m3min = lvl_index(m30, o->mgz[0], o->mgz[l], o->mgs[l]);
+    int m3max = lvl_index(m31, o->mgz[0], o->mgz[l], o->mgs[l]);
+    for (int m3 = m3min; m3 <= m3max; m3++)
+      for (int m1 = m1min; m1 <= m1max; m1++)
+        for (int m2 = m2min; m2 <= m2max; m2++) {
+          long m4 = NID(o, l, m1, m2, m3);
+          int sx = dm1, sy = dm2, sz = dm3;
+          if (ABSV(dm1) > 1) sx = -(o->mgx[l] - 1);
+          if (ABSV(dm2) > 1) sy = -(o->mgy[l] - 1);
+          if (ABSV(dm3) > 1) sz = -(o->mgz[l] - 1);
+          long m6 = m4 + (long)sz * o->mgxy[l] + (long)sx * o->mgy[l] + sy;
+          if (o->nbond + 1 > o->capbond) {
+            int nc = o->capbond * 2;
+            o->bondv1 = (float *)realloc(o->bondv1, 2 * (size_t)(nc + 1) * sizeof(float));
+            o->bondn1 = (int *)realloc(o->bondn1, (size_t)(nc + 1) * sizeof(int));
+            o->capbond = nc;
+          }
+          long m5 = m4 * 4 + var;
+          o->nbond++;
+          o->bondm[m5] = o->nbond;
+          o->bondv1[2 * o->nbond + 0] = cnst;
+          if (dm1 || dm2 || dm3) {
+            o->bondv1[2 * o->nbond + 1] = koef;
+            o->bondn1[o->nbond] = (int)(m6 * 4 + var + 1);
+          } else {
+            o->bondv1[2 * o->nbond + 1] = 0;
+            o->bondn1[o->nbond] = 0;
+          }
+        }
+  }
+  return o->nbond;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Marker -> node averaging: parallel/mark3mg.c:630-1132 (ronurecalc)         */
+/* Marker material values are given per marker (the outputs of viscalc and   */
+/* dencalc1, mark3mg.c:1418-1498).  Cell search follows m1serch/m2serch/      */
+/* m3serch (mark3mg.c:1138-1205) via prcalc1 (mark3mg.c:1285-1304).           */
+/* Non-periodic boxes only (the periodic folds of mark3mg.c:893-955 are not  */
+/* restated here).                                                            */
+/* ------------------------------------------------------------------------- */
+static long serch(const double *g, int n, double x) {
+  long a = 0, b = n - 1, m;
+  do {
+    m = (a + b) / 2;
+    if (g[m] > x) b = m; else a = m;
+  } while ((b - a) > 1);
+  if (a > n - 2) a = n - 2;
+  return a;
+}
+
+int orc_ronurecalc(orc_t *o, long nmark, const float *mx, const float *my, const float *mz,
+                   const float *mnu_in, const float *mro_in, double nukoef) {
+  if (o->xper || o->yper || o->zper) return -1;
+  const int nx = o->mgx[0], ny = o->mgy[0], nz = o->mgz[0];
+  const long nn = o->mgn[0];
+  const double *gx = o->mggx[0], *gy = o->mggy[0], *gz = o->mggz[0];
+  const double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
+  /* accumulators: FP32 property sums (Nu,Ro,Nxx.. are float**), FP64 weights */
+  float *anu = (float *)calloc(nn, sizeof(float)), *aro = (float *)calloc(nn, sizeof(float));
+  float *axx = (float *)calloc(nn, sizeof(float)), *axy = (float *)calloc(nn, sizeof(float));
+  float *axz = (float *)calloc(nn, sizeof(float)), *ayz = (float *)calloc(nn, sizeof(float));
+  double *w = (double *)calloc(6 * nn, sizeof(double));
+  double *wro = w, *wnu = w + nn, *wxx = w + 2 * nn, *wxy = w + 3 * nn, *wxz = w + 4 * nn,
+         *wyz = w + 5 * nn;
+  const int vm = o->viscmod;
+  for (long mm1 = 0; mm1 < nmark; mm1++) {
+    if (!(mx[mm1] > 0 && my[mm1] > 0 && mz[mm1] > 0 && mx[mm1] < xs && my[mm1] < ys &&
+          mz[mm1] < zs))
+      continue;
+    double x = mx[mm1], y = my[mm1], z = mz[mm1];
+    long m10 = serch(gx, nx, x), m20 = serch(gy, ny, y), m30 = serch(gz, nz, z);
+    double mnu = mnu_in[mm1];
+    double mro = mro_in[mm1];
+    if (mnu < o->nubeg) mnu = o->nubeg;
+    if (mnu > o->nuend) mnu = o->nuend;
+    double dx = gx[m10 + 1] - gx[m10], dy = gy[m20 + 1] - gy[m20], dz = gz[m30 + 1] - gz[m30];
+    for (long m3 = m30; m3 <= m30 + 1; m3++) {
+      double ddz = (gz[m3] - z) / dz;
+      ddz = 1.0 - ABSV(ddz);
+      for (long m1 = m10; m1 <= m10 + 1; m1++) {
+        double ddx = (gx[m1] - x) / dx;
+        ddx = 1.0 - ABSV(ddx);
+        for (long m2 = m20; m2 <= m20 + 1; m2++) {
+          double ddy = (gy[m2] - y) / dy;
+          ddy = 1.0 - ABSV(ddy);
+          double swt = ddx * ddy * ddz, swt1, ival2 = 0;
+          long m4 = m3 * (long)nx * ny + m1 * ny + m2;
+          if (ddx > nukoef && ddy > nukoef && ddz > nukoef) {
+            if (vm == 0) ival2 = mnu * swt;
+            if (vm == 1) ival2 = log(mnu) * swt;
+            if (vm == 2) ival2 = 1.0 / mnu * swt;
+            anu[m4] += ival2;
+            wnu[m4] += swt;
+          }
+          if (m1 > m10 && m2 > m20 && m3 > m30) {
+            swt1 = 8.0 * (0.5 - ABSV(ddx - 0.5)) * (0.5 - ABSV(ddy - 0.5)) * (0.5 - ABSV(ddz - 0.5));
+            if (vm == 0) ival2 = mnu * swt1;
+            if (vm == 1) ival2 = log(mnu) * swt1;
+            if (vm == 2) ival2 = 1.0 / mnu * swt1;
+            axx[m4] += ival2;
+            wxx[m4] += swt1;
+          }
+          if (m3 == m30 && ddx > 0.5 && ddy > 0.5) {
+            swt1 = 8.0 * (ddx - 0.5) * (ddy - 0.5) * (0.5 - ABSV(ddz - 0.5));
+            if (vm == 0) ival2 = mnu * swt1;
+            if (vm == 1) ival2 = log(mnu) * swt1;
+            if (vm == 2) ival2 = 1.0 / mnu * swt1;
+            axy[m4] += ival2;
+            wxy[m4] += swt1;
+          }
+          if (m2 == m20 && ddx > 0.5 && ddz > 0.5) {
+            swt1 = 8.0 * (ddx - 0.5) * (0.5 - ABSV(ddy - 0.5)) * (ddz - 0.5);
+            if (vm == 0) ival2 = mnu * swt1;
+            if (vm == 1) ival2 = log(mnu) * swt1;
+            if (vm == 2) ival2 = 1.0 / mnu * swt1;
+            axz[m4] += ival2;
+            wxz[m4] += swt1;
+          }
+          if (m1 == m10 && ddy > 0.5 && ddz > 0.5) {
+            swt1 = 8.0 * (0.5 - ABSV(ddx - 0.5)) * (ddy - 0.5) * (ddz - 0.5);
+            if (vm == 0) ival2 = mnu * swt1;
+            if (vm == 1) ival2 = log(mnu) * swt1;
+            if (vm == 2) ival2 = 1.0 / mnu * swt1;
+            ayz[m4] += ival2;
+            wyz[m4] += swt1;
+          }
+          if (ddx > 0.5 && ddy > 0.5 && ddz > 0.5) {
+            swt1 = 8.0 * (ddx - 0.5) * (ddy - 0.5) * (ddz - 0.5);
+            aro[m4] += mro * swt1;
+            wro[m4] += swt1;
+          }
+        }
+      }
+    }
+  }
+  /* normalise, back-transform, clamp: mark3mg.c:962-1031 */
+  for (long m3 = 0; m3 < nn; m3++) {
+    if (wnu[m3] > 0) {
+      o->nu[m3] = anu[m3] / wnu[m3];
+      if (vm == 1) o->nu[m3] = exp(o->nu[m3]);
+      if (vm == 2) o->nu[m3] = 1.0 / o->nu[m3];
+    }
+    if (wxx[m3] > 0) {
+      o->nxx[m3] = axx[m3] / wxx[m3];
+      if (vm == 1) o->nxx[m3] = exp(o->nxx[m3]);
+      if (vm == 2) o->nxx[m3] = 1.0 / o->nxx[m3];
+    }
+    if (wxy[m3] > 0) {
+      o->nxy[m3] = axy[m3] / wxy[m3];
+      if (vm == 1) o->nxy[m3] = exp(o->nxy[m3]);
+      if (vm == 2) o->nxy[m3] = 1.0 / o->nxy[m3];
+    }
+    if (wxz[m3] > 0) {
+      o->nxz[m3] = axz[m3] / wxz[m3];
+      if (vm == 1) o->nxz[m3] = exp(o->nxz[m3]);
+      if (vm == 2) o->nxz[m3] = 1.0 / o->nxz[m3];
+    }
+    if (wyz[m3] > 0) {
+      o->nyz[m3] = ayz[m3] / wyz[m3];
+      if (vm == 1) o->nyz[m3] = exp(o->nyz[m3]);
+      if (vm == 2) o->nyz[m3] = 1.0 / o->nyz[m3];
+    }
+    if (wro[m3] > 0) o->ro[m3] = aro[m3] / wro[m3];
+    if (o->nu[m3] < o->nubeg) o->nu[m3] = o->nubeg;
+    if (o->nu[m3] > o->nuend) o->nu[m3] = o->nuend;
+    if (o->nxx[m3] < o->nubeg) o->nxx[m3] = o->nubeg;
+    if (o->nxx[m3] > o->nuend) o->nxx[m3] = o->nuend;
+    if (o->nxy[m3] < o->nubeg) o->nxy[m3] = o->nubeg;
+    if (o->nxy[m3] > o->nuend) o->nxy[m3] = o->nuend;
+    if (o->nxz[m3] < o->nubeg) o->nxz[m3] = o->nubeg;
+    if (o->nxz[m3] > o->nuend) o->nxz[m3] = o->nuend;
+    if (o->nyz[m3] < o->nubeg) o->nyz[m3] = o->nubeg;
+    if (o->nyz[m3] > o->nuend) o->nyz[m3] = o->nuend;
+  }
+  free(anu); free(aro); free(axx); free(axy); free(axz); free(ayz); free(w);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Viscosity/density restriction: parallel/move3mg.c:3387-3528                */
+/* ------------------------------------------------------------------------- */
+void orc_ronurestrict(orc_t *o, int l) {
+  const int f = l - 1;
+  const long nc = o->mgn[l], pc = o->mgp[l];
+  float *Nu = (float *)calloc(nc, sizeof(float)), *Ro = (float *)calloc(nc, sizeof(float));
+  double *S1 = (double *)calloc(nc, sizeof(double));
+  const double *cx = o->mggx[l], *cy = o->mggy[l], *cz = o->mggz[l];
+  const double *fx = o->mggx[f], *fy = o->mggy[f], *fz = o->mggz[f];
+  for (int m3 = 0; m3 < o->mgz[f]; m3++)
+    for (int m1 = 0; m1 < o->mgx[f]; m1++)
+      for (int m2 = 0; m2 < o->mgy[f]; m2++) {
+        long m4 = NID(o, f, m1, m2, m3);
+        int m10min = (m1 - 2) / 2 + 2;
+        if (m1 < 3) m10min = m1;
+        if (m1 > o->mgx[f] - 4) m10min = o->mgx[l] - (o->mgx[f] - m1);
+        if (m10min > o->mgx[l] - 2) m10min = o->mgx[l] - 2;
+        int m20min = (m2 - 2) / 2 + 2;
+        if (m2 < 3) m20min = m2;
+        if (m2 > o->mgy[f] - 4) m20min = o->mgy[l] - (o->mgy[f] - m2);
+        if (m20min > o->mgy[l] - 2) m20min = o->mgy[l] - 2;
+        int m30min = (m3 - 2) / 2 + 2;
+        if (m3 < 3) m30min = m3;
+        if (m3 > o->mgz[f] - 4) m30min = o->mgz[l] - (o->mgz[f] - m3);
+        if (m30min > o->mgz[l] - 2) m30min = o->mgz[l] - 2;
+        double dx = cx[m10min + 1] - cx[m10min];
+        double dy = cy[m20min + 1] - cy[m20min];
+        double dz = cz[m30min + 1] - cz[m30min];
+        for (int m30 = m30min; m30 <= m30min + 1; m30++) {
+          double ddz = (cz[m30] - fz[m3]) / dz;
+          ddz = 1.0 - ABSV(ddz);
+          for (int m10 = m10min; m10 <= m10min + 1; m10++) {
+            double ddx = (cx[m10] - fx[m1]) / dx;
+            ddx = 1.0 - ABSV(ddx);
+            for (int m20 = m20min; m20 <= m20min + 1; m20++) {
+              double ddy = (cy[m20] - fy[m2]) / dy;
+              ddy = 1.0 - ABSV(ddy);
+              long m40 = NID(o, l, m10, m20, m30) - pc;
+              double swt = ddx * ddy * ddz, ival2 = 0;
+              if (o->viscmod == 0) ival2 = o->nu[m4] * swt;
+              if (o->viscmod == 1) ival2 = log(o->nu[m4]) * swt;
+              if (o->viscmod == 2) ival2 = 1.0 / o->nu[m4] * swt;
+              Nu[m40] += ival2;
+              ival2 = o->ro[m4] * swt;
+              Ro[m40] += ival2;
+              S1[m40] += swt;
+            }
+          }
+        }
+      }
+  for (long m = 0; m < nc; m++) {
+    o->nu[pc + m] = 0;
+    o->ro[pc + m] = 0;
+    o->nu[pc + m] += Nu[m];
+    o->ro[pc + m] += Ro[m];
+    o->nu[pc + m] /= S1[m];
+    if (o->viscmod == 1) o->nu[pc + m] = exp(o->nu[pc + m]);
+    if (o->viscmod == 2) o->nu[pc + m] = 1.0 / o->nu[pc + m];
+    o->ro[pc + m] /= S1[m];
+  }
+  free(Nu); free(Ro); free(S1);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Stress-point stencils, coefficient mode: parallel/move3mg.c:3-788          */
+/* Each appends (index, coefficient) pairs at un1[un1[0]+1..] and advances    */
+/* un1[0]; ynval weights the contribution.                                   */
+/* ------------------------------------------------------------------------- */
+static double cellvisc(const orc_t *o, const long *v) {
+  const float *nu = o->nu;
+  double e = 0;
+  if (o->viscmod == 0) e = (nu[v[0]] + nu[v[1]] + nu[v[2]] + nu[v[3]] + nu[v[4]] + nu[v[5]] + nu[v[6]] + nu[v[7]]) / 8.0;
+  if (o->viscmod == 1)
+    e = exp((log(nu[v[0]]) + log(nu[v[1]]) + log(nu[v[2]]) + log(nu[v[3]]) + log(nu[v[4]]) + log(nu[v[5]]) + log(nu[v[6]]) + log(nu[v[7]])) / 8.0);
+  if (o->viscmod == 2)
+    e = 1.0 / ((1.0 / nu[v[0]] + 1.0 / nu[v[1]] + 1.0 / nu[v[2]] + 1.0 / nu[v[3]] + 1.0 / nu[v[4]] + 1.0 / nu[v[5]] + 1.0 / nu[v[6]] + 1.0 / nu[v[7]]) / 8.0);
+  return e;
+}
+
+static double edgevisc(const orc_t *o, long a, long b) {
+  const float *nu = o->nu;
+  double e = 0;
+  if (o->viscmod == 0) e = (nu[a] + nu[b]) / 2.0;
+  if (o->viscmod == 1) e = exp((log(nu[a]) + log(nu[b])) / 2.0);
+  if (o->viscmod == 2) e = 1.0 / ((1.0 / nu[a] + 1.0 / nu[b]) / 2.0);
+  return e;
+}
+
+static double clampnu(const orc_t *o, double e) {
+  if (e > o->nuend) e = o->nuend;
+  if (e < o->nubeg) e = o->nubeg;
+  return e;
+}
+
+/* cell node octet around cell (m1,m2,m3), with periodic wraps (move3mg.c:27-34) */
+static void cell_octet(const orc_t *o, int l, long m1, long m2, long m3, long *v) {
+  v[0] = o->mgp[l] + (m3 - 1) * o->mgxy[l] + (m1 - 1) * o->mgy[l] + (m2 - 1);
+  v[1] = v[0] + 1;
+  v[2] = v[0] + o->mgy[l];
+  v[3] = v[2] + 1;
+  v[4] = v[0] + o->mgxy[l];
+  v[5] = v[4] + 1;
+  v[6] = v[4] + o->mgy[l];
+  v[7] = v[6] + 1;
+  if (o->xper && m1 == o->mgx[l] - 1) v[2] = o->mgp[l] + (m3 - 1) * o->mgxy[l] + m2 - 1;
+  if (o->yper && m2 == o->mgy[l] - 1) v[1] = o->mgp[l] + (m3 - 1) * o->mgxy[l] + (m1 - 1) * o->mgy[l];
+  if (o->zper && m3 == o->mgz[l] - 1) v[4] = o->mgp[l] + (m1 - 1) * o->mgy[l] + m2 - 1;
+}
+
+/* Sxx/Syy/Szz, move3mg.c:3-380 (axis 0/1/2 selects which strain leads) */
+static void snn_coef(const orc_t *o, int axis, long m1, long m2, long m3, double w, int l,
+                     long *un1, double *ui1) {
+  double xkf = 1.0 / (o->mggx[l][m1] - o->mggx[l][m1 - 1]);
+  double ykf = 1.0 / (o->mggy[l][m2] - o->mggy[l][m2 - 1]);
+  double zkf = 1.0 / (o->mggz[l][m3] - o->mggz[l][m3 - 1]);
+  long v[8];
+  cell_octet(o, l, m1, m2, m3, v);
+  double ne = (l == 0) ? (double)o->nxx[v[7]] : cellvisc(o, v);
+  ne = clampnu(o, ne);
+  long k = un1[0];
+  double cx = axis == 0 ? 4.0 / 3.0 : -2.0 / 3.0;
+  double cy = axis == 1 ? 4.0 / 3.0 : -2.0 / 3.0;
+  double cz = axis == 2 ? 4.0 / 3.0 : -2.0 / 3.0;
+  /* the reference writes the signs literally per stencil; keep its factor order */
+  if (axis == 0) {
+    un1[k + 1] = v[0] * 4 + 1; ui1[k + 1] = -w * ne * 4.0 / 3.0 * xkf;
+    un1[k + 2] = v[2] * 4 + 1; ui1[k + 2] = +w * ne * 4.0 / 3.0 * xkf;
+    un1[k + 3] = v[0] * 4 + 2; ui1[k + 3] = +w * ne * 2.0 / 3.0 * ykf;
+    un1[k + 4] = v[1] * 4 + 2; ui1[k + 4] = -w * ne * 2.0 / 3.0 * ykf;
+    un1[k + 5] = v[0] * 4 + 3; ui1[k + 5] = +w * ne * 2.0 / 3.0 * zkf;
+    un1[k + 6] = v[4] * 4 + 3; ui1[k + 6] = -w * ne * 2.0 / 3.0 * zkf;
+  } else if (axis == 1) {
+    un1[k + 1] = v[0] * 4 + 1; ui1[k + 1] = +w * ne * 2.0 / 3.0 * xkf;
+    un1[k + 2] = v[2] * 4 + 1; ui1[k + 2] = -w * ne * 2.0 / 3.0 * xkf;
+    un1[k + 3] = v[0] * 4 + 2; ui1[k + 3] = -w * ne * 4.0 / 3.0 * ykf;
+    un1[k + 4] = v[1] * 4 + 2; ui1[k + 4] = +w * ne * 4.0 / 3.0 * ykf;
+    un1[k + 5] = v[0] * 4 + 3; ui1[k + 5] = +w * ne * 2.0 / 3.0 * zkf;
+    un1[k + 6] = v[4] * 4 + 3; ui1[k + 6] = -w * ne * 2.0 / 3.0 * zkf;
+  } else {
+    un1[k + 1] = v[0] * 4 + 1; ui1[k + 1] = +w * ne * 2.0 / 3.0 * xkf;
+    un1[k + 2] = v[2] * 4 + 1; ui1[k + 2] = -w * ne * 2.0 / 3.0 * xkf;
+    un1[k + 3] = v[0] * 4 + 2; ui1[k + 3] = +w * ne * 2.0 / 3.0 * ykf;
+    un1[k + 4] = v[1] * 4 + 2; ui1[k + 4] = -w * ne * 2.0 / 3.0 * ykf;
+    un1[k + 5] = v[0] * 4 + 3; ui1[k + 5] = -w * ne * 4.0 / 3.0 * zkf;
+    un1[k + 6] = v[4] * 4 + 3; ui1[k + 6] = +w * ne * 4.0 / 3.0 * zkf;
+  }
+  (void)cx; (void)cy; (void)cz;
+  un1[0] += 6;
+}
+
+/* Sxy, move3mg.c:388-512 */
+static void sxy_coef(const orc_t *o, long m1, long m2, long m3, double w, int l, long *un1,
+                     double *ui1) {
+  const int X = o->mgx[l], Y = o->mgy[l];
+  const double *gx = o->mggx[l], *gy = o->mggy[l];
+  double xkf, ykf;
+  if (o->xper && m1 == 0) xkf = 1.0 / (gx[1] - gx[0] + gx[X - 1] - gx[X - 2]);
+  else xkf = 1.0 / (gx[m1 + 1] - gx[m1 - 1]);
+  if (o->yper && m2 == 0) ykf = 1.0 / (gy[1] - gy[0] + gy[Y - 1] - gy[Y - 2]);
+  else ykf = 1.0 / (gy[m2 + 1] - gy[m2 - 1]);
+  long p = o->mgp[l], xy = o->mgxy[l];
+  long v[8];
+  v[0] = p + m3 * xy + (m1 - 1) * Y + (m2 - 1);
+  v[1] = v[0] + 1; v[2] = v[0] + Y; v[3] = v[2] + 1;
+  v[4] = v[0] + xy; v[5] = v[4] + 1; v[6] = v[4] + Y; v[7] = v[6] + 1;
+  if (o->xper && m1 == 0) {
+    v[1] = p + m3 * xy + (X - 2) * Y + m2;
+    v[2] = p + m3 * xy + (m2 - 1); v[3] = v[2] + 1; v[7] = v[3] + xy;
+  }
+  if (o->yper && m2 == 0) {
+    v[2] = p + m3 * xy + m1 * Y + Y - 2;
+    v[1] = p + m3 * xy + (m1 - 1) * Y; v[3] = v[1] + Y; v[7] = v[3] + xy;
+  }
+  if (o->xper && m1 == 0 && o->yper && m2 == 0) {
+    v[1] = p + m3 * xy + (X - 2) * Y;
+    v[2] = p + m3 * xy + Y - 2; v[3] = p + m3 * xy; v[7] = v[3] + xy;
+  }
+  double ne = (l == 0) ? (double)o->nxy[v[3]] : edgevisc(o, v[3], v[7]);
+  ne = clampnu(o, ne);
+  long k = un1[0];
+  un1[k + 1] = v[2] * 4 + 1; ui1[k + 1] = -w * ne * 2.0 * ykf;
+  un1[k + 2] = v[3] * 4 + 1; ui1[k + 2] = +w * ne * 2.0 * ykf;
+  un1[k + 3] = v[1] * 4 + 2; ui1[k + 3] = -w * ne * 2.0 * xkf;
+  un1[k + 4] = v[3] * 4 + 2; ui1[k + 4] = +w * ne * 2.0 * xkf;
+  un1[0] += 4;
+}
+
+/* Sxz, move3mg.c:520-651 */
+static void sxz_coef(const orc_t *o, long m1, long m2, long m3, double w, int l, long *un1,
+                     double *ui1) {
+  const int X = o->mgx[l], Y = o->mgy[l], Z = o->mgz[l];
+  const double *gx = o->mggx[l], *gz = o->mggz[l];
+  double xkf, zkf;
+  if (o->xper && m1 == 0) xkf = 1.0 / (gx[1] - gx[0] + gx[X - 1] - gx[X - 2]);
+  else xkf = 1.0 / (gx[m1 + 1] - gx[m1 - 1]);
+  if (o->zper && m3 == 0) zkf = 1.0 / (gz[1] - gz[0] + gz[Z - 1] - gz[Z - 2]);
+  else zkf = 1.0 / (gz[m3 + 1] - gz[m3 - 1]);
+  long p = o->mgp[l], xy = o->mgxy[l];
+  long v[8];
+  v[0] = p + (m3 - 1) * xy + (m1 - 1) * Y + m2;
+  v[1] = v[0] + 1; v[2] = v[0] + Y; v[3] = v[2] + 1;
+  v[4] = v[0] + xy; v[5] = v[4] + 1; v[6] = v[4] + Y; v[7] = v[6] + 1;
+  if (o->xper && m1 == 0) {
+    v[4] = p + m3 * xy + (X - 2) * Y + m2;
+    v[2] = p + (m3 - 1) * xy + m2; v[6] = v[2] + xy; v[7] = v[6] + 1;
+  }
+  if (o->zper && m3 == 0) {
+    v[2] = p + (Z - 2) * xy + m1 * Y + m2;
+    v[4] = p + (m1 - 1) * Y + m2; v[6] = v[4] + Y; v[7] = v[6] + 1;
+  }
+  if (o->xper && m1 == 0 && o->zper && m3 == 0) {
+    v[2] = p + (Z - 2) * xy + m2;
+    v[4] = p + (X - 2) * Y + m2;
+    v[6] = p + m2; v[7] = v[6] + 1;
+  }
+  double ne = (l == 0) ? (double)o->nxz[v[6]] : edgevisc(o, v[6], v[7]);
+  ne = clampnu(o, ne);
+  long k = un1[0];
+  un1[k + 1] = v[2] * 4 + 1; ui1[k + 1] = -w * ne * 2.0 * zkf;
+  un1[k + 2] = v[6] * 4 + 1; ui1[k + 2] = +w * ne * 2.0 * zkf;
+  un1[k + 3] = v[4] * 4 + 3; ui1[k + 3] = -w * ne * 2.0 * xkf;
+  un1[k + 4] = v[6] * 4 + 3; ui1[k + 4] = +w * ne * 2.0 * xkf;
+  un1[0] += 4;
+}
+
+/* Syz, move3mg.c:659-788 */
+static void syz_coef(const orc_t *o, long m1, long m2, long m3, double w, int l, long *un1,
+                     double *ui1) {
+  const int Y = o->mgy[l], Z = o->mgz[l];
+  const double *gy = o->mggy[l], *gz = o->mggz[l];
+  double ykf, zkf;
+  if (o->yper && m2 == 0) ykf = 1.0 / (gy[1] - gy[0] + gy[Y - 1] - gy[Y - 2]);
+  else ykf = 1.0 / (gy[m2 + 1] - gy[m2 - 1]);
+  if (o->zper && m3 == 0) zkf = 1.0 / (gz[1] - gz[0] + gz[Z - 1] - gz[Z - 2]);
+  else zkf = 1.0 / (gz[m3 + 1] - gz[m3 - 1]);
+  long p = o->mgp[l], xy = o->mgxy[l];
+  long v[8];
+  v[0] = p + (m3 - 1) * xy + m1 * Y + (m2 - 1);
+  v[1] = v[0] + 1; v[2] = v[0] + Y; v[3] = v[2] + 1;
+  v[4] = v[0] + xy; v[5] = v[4] + 1; v[6] = v[4] + Y; v[7] = v[6] + 1;
+  if (o->yper && m2 == 0) {
+    v[1] = p + (m3 - 1) * xy + m1 * Y; v[5] = v[1] + xy; v[7] = v[5] + Y;
+    v[4] = v[5] + Y - 2;
+  }
+  if (o->zper && m3 == 0) {
+    v[4] = p + m1 * Y + (m2 - 1); v[5] = v[4] + 1; v[7] = v[5] + Y;
+    v[1] = v[5] + (Z - 2) * xy;
+  }
+  if (o->yper && m2 == 0 && o->zper && m3 == 0) {
+    v[5] = p + m1 * Y; v[7] = v[5] + Y;
+    v[1] = v[5] + (Z - 2) * xy;
+    v[4] = v[5] + Y - 2;
+  }
+  double ne = (l == 0) ? (double)o->nyz[v[5]] : edgevisc(o, v[5], v[7]);
+  ne = clampnu(o, ne);
+  long k = un1[0];
+  un1[k + 1] = v[1] * 4 + 2; ui1[k + 1] = -w * ne * 2.0 * zkf;
+  un1[k + 2] = v[5] * 4 + 2; ui1[k + 2] = +w * ne * 2.0 * zkf;
+  un1[k + 3] = v[4] * 4 + 3; ui1[k + 3] = -w * ne * 2.0 * ykf;
+  un1[k + 4] = v[5] * 4 + 3; ui1[k + 4] = +w * ne * 2.0 * ykf;
+  un1[0] += 4;
+}
+
+/* grad-div contribution, move3mg.c:1580-1674.  Its level-0 nxx branch is    */
+/* dead (mgi==-10), so the 8-node nu average is used on every level.         */
+static void continadd_coef(const orc_t *o, long m1, long m2, long m3, double w, int l,
+                           long *un1, double *ui1) {
+  double xkf = 1.0 / (o->mggx[l][m1] - o->mggx[l][m1 - 1]);
+  double ykf = 1.0 / (o->mggy[l][m2] - o->mggy[l][m2 - 1]);
+  double zkf = 1.0 / (o->mggz[l][m3] - o->mggz[l][m3 - 1]);
+  long v[8];
+  cell_octet(o, l, m1, m2, m3, v);
+  double ne = clampnu(o, cellvisc(o, v));
+  long k = un1[0];
+  ui1[0] += 0.0 * w; /* rightc = 0 */
+  un1[k + 1] = v[0] * 4 + 1; ui1[k + 1] = -xkf * w * ne;
+  un1[k + 2] = v[2] * 4 + 1; ui1[k + 2] = +xkf * w * ne;
+  un1[k + 3] = v[0] * 4 + 2; ui1[k + 3] = -ykf * w * ne;
+  un1[k + 4] = v[1] * 4 + 2; ui1[k + 4] = +ykf * w * ne;
+  un1[k + 5] = v[0] * 4 + 3; ui1[k + 5] = -zkf * w * ne;
+  un1[k + 6] = v[4] * 4 + 3; ui1[k + 6] = +zkf * w * ne;
+  un1[0] += 6;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Momentum rows: x move3mg.c:796-1003, y 1011-1223, z 1231-1447              */
+/* Slot layout (1..17) and merge order follow the reference exactly: the     */
+/* assembled ELL rows and the coarsest CSR depend on it.                     */
+/* ------------------------------------------------------------------------- */
+#define SETS(s, k) (un1[s] = un1[k], ui1[s] = ui1[k])
+#define ADDS(s, k) (ui1[s] += ui1[k])
+
+static void xstokes_row(const orc_t *o, long m1, long m2, long m3, int l, long *un1, double *ui1) {
+  const int X = o->mgx[l], Y = o->mgy[l], Z = o->mgz[l];
+  const double *gx = o->mggx[l], *gy = o->mggy[l], *gz = o->mggz[l];
+  double xkf = 2.0 / (gx[m1 + 1] - gx[m1 - 1]);
+  if (o->xper && m1 == 0) xkf = 2.0 / (gx[1] - gx[0] + gx[X - 1] - gx[X - 2]);
+  double ykf = 1.0 / (gy[m2 + 1] - gy[m2]);
+  double zkf = 1.0 / (gz[m3 + 1] - gz[m3]);
+  long v[8];
+  v[0] = o->mgp[l] + m3 * o->mgxy[l] + m1 * Y + m2;
+  v[1] = v[0] + 1; v[2] = v[0] + Y; v[3] = v[2] + 1;
+  v[4] = v[0] + o->mgxy[l]; v[5] = v[4] + 1; v[6] = v[4] + Y; v[7] = v[6] + 1;
+  if (o->xper && m1 == 0) v[5] = o->mgp[l] + (m3 + 1) * o->mgxy[l] + (X - 1) * Y + m2 + 1;
+  const float *ro = o->ro;
+  ui1[0] = -o->gxk * (ro[v[0]] + ro[v[1]] + ro[v[4]] + ro[v[5]]) / 4.0;
+  un1[0] = 17;
+  un1[16] = v[5] * 4 + 0; ui1[16] = +xkf;
+  un1[17] = v[7] * 4 + 0; ui1[17] = -xkf;
+  if (o->xper && m1 == 0) snn_coef(o, 0, X - 1, m2 + 1, m3 + 1, -xkf, l, un1, ui1);
+  else snn_coef(o, 0, m1, m2 + 1, m3 + 1, -xkf, l, un1, ui1);
+  SETS(2, 18); SETS(1, 19); SETS(8, 20); SETS(9, 21); SETS(12, 22); SETS(13, 23);
+  un1[0] = 17;
+  snn_coef(o, 0, m1 + 1, m2 + 1, m3 + 1, xkf, l, un1, ui1);
+  ADDS(1, 18); SETS(3, 19); SETS(10, 20); SETS(11, 21); SETS(14, 22); SETS(15, 23);
+  un1[0] = 17;
+  sxy_coef(o, m1, m2, m3, -ykf, l, un1, ui1);
+  ADDS(1, 19); SETS(4, 18); ADDS(8, 20); ADDS(10, 21);
+  un1[0] = 17;
+  if (o->yper && m2 == Y - 2) sxy_coef(o, m1, 0, m3, ykf, l, un1, ui1);
+  else sxy_coef(o, m1, m2 + 1, m3, ykf, l, un1, ui1);
+  ADDS(1, 18); SETS(5, 19); ADDS(9, 20); ADDS(11, 21);
+  un1[0] = 17;
+  sxz_coef(o, m1, m2, m3, -zkf, l, un1, ui1);
+  ADDS(1, 19); SETS(6, 18); ADDS(12, 20); ADDS(14, 21);
+  un1[0] = 17;
+  if (o->zper && m3 == Z - 2) sxz_coef(o, m1, m2, 0, zkf, l, un1, ui1);
+  else sxz_coef(o, m1, m2, m3 + 1, zkf, l, un1, ui1);
+  ADDS(1, 18); SETS(7, 19); ADDS(13, 20); ADDS(15, 21);
+  if (o->p2koef != 0) {
+    un1[0] = 17;
+    if (o->xper && m1 == 0) continadd_coef(o, X - 1, m2 + 1, m3 + 1, -xkf * o->p2koef, l, un1, ui1);
+    else continadd_coef(o, m1, m2 + 1, m3 + 1, -xkf * o->p2koef, l, un1, ui1);
+    ADDS(2, 18); ADDS(1, 19); ADDS(8, 20); ADDS(9, 21); ADDS(12, 22); ADDS(13, 23);
+    un1[0] = 17;
+    continadd_coef(o, m1 + 1, m2 + 1, m3 + 1, xkf * o->p2koef, l, un1, ui1);
+    ADDS(1, 18); ADDS(3, 19); ADDS(10, 20); ADDS(11, 21); ADDS(14, 22); ADDS(15, 23);
+  }
+  un1[0] = 17;
+}
+
+static void ystokes_row(const orc_t *o, long m1, long m2, long m3, int l, long *un1, double *ui1) {
+  const int X = o->mgx[l], Y = o->mgy[l], Z = o->mgz[l];
+  const double *gx = o->mggx[l], *gy = o->mggy[l], *gz = o->mggz[l];
+  double xkf = 1.0 / (gx[m1 + 1] - gx[m1]);
+  double ykf = 2.0 / (gy[m2 + 1] - gy[m2 - 1]);
+  if (o->yper && m2 == 0) ykf = 2.0 / (gy[1] - gy[0] + gy[Y - 1] - gy[Y - 2]);
+  double zkf = 1.0 / (gz[m3 + 1] - gz[m3]);
+  long v[8];
+  v[0] = o->mgp[l] + m3 * o->mgxy[l] + m1 * Y + m2;
+  v[1] = v[0] + 1; v[2] = v[0] + Y; v[3] = v[2] + 1;
+  v[4] = v[0] + o->mgxy[l]; v[5] = v[4] + 1; v[6] = v[4] + Y; v[7] = v[6] + 1;
+  if (o->yper && m2 == 0) v[6] = o->mgp[l] + (m3 + 1) * o->mgxy[l] + (m1 + 1) * Y + Y - 1;
+  const float *ro = o->ro;
+  ui1[0] = -o->gyk * (ro[v[0]] + ro[v[2]] + ro[v[4]] + ro[v[6]]) / 4.0;
+  un1[0] = 17;
+  un1[16] = v[6] * 4 + 0; ui1[16] = +ykf;
+  un1[17] = v[7] * 4 + 0; ui1[17] = -ykf;
+  if (o->yper && m2 == 0) snn_coef(o, 1, m1 + 1, Y - 1, m3 + 1, -ykf, l, un1, ui1);
+  else snn_coef(o, 1, m1 + 1, m2, m3 + 1, -ykf, l, un1, ui1);
+  SETS(8, 18); SETS(10, 19); SETS(4, 20); SETS(1, 21); SETS(12, 22); SETS(14, 23);
+  un1[0] = 17;
+  snn_coef(o, 1, m1 + 1, m2 + 1, m3 + 1, ykf, l, un1, ui1);
+  SETS(9, 18); SETS(11, 19); ADDS(1, 20); SETS(5, 21); SETS(13, 22); SETS(15, 23);
+  un1[0] = 17;
+  sxy_coef(o, m1, m2, m3, -xkf, l, un1, ui1);
+  ADDS(8, 18); ADDS(9, 19); SETS(2, 20); ADDS(1, 21);
+  un1[0] = 17;
+  if (o->xper && m1 == X - 2) sxy_coef(o, 0, m2, m3, xkf, l, un1, ui1);
+  else sxy_coef(o, m1 + 1, m2, m3, xkf, l, un1, ui1);
+  ADDS(10, 18); ADDS(11, 19); ADDS(1, 20); SETS(3, 21);
+  un1[0] = 17;
+  syz_coef(o, m1, m2, m3, -zkf, l, un1, ui1);
+  SETS(6, 18); ADDS(1, 19); ADDS(12, 20); ADDS(13, 21);
+  un1[0] = 17;
+  if (o->zper && m3 == Z - 2) syz_coef(o, m1, m2, 0, zkf, l, un1, ui1);
+  else syz_coef(o, m1, m2, m3 + 1, zkf, l, un1, ui1);
+  ADDS(1, 18); SETS(7, 19); ADDS(14, 20); ADDS(15, 21);
+  if (o->p2koef != 0) {
+    un1[0] = 17;
+    if (o->yper && m2 == 0) continadd_coef(o, m1 + 1, Y - 1, m3 + 1, -ykf * o->p2koef, l, un1, ui1);
+    else continadd_coef(o, m1 + 1, m2, m3 + 1, -ykf * o->p2koef, l, un1, ui1);
+    ADDS(8, 18); ADDS(10, 19); ADDS(4, 20); ADDS(1, 21); ADDS(12, 22); ADDS(14, 23);
+    un1[0] = 17;
+    continadd_coef(o, m1 + 1, m2 + 1, m3 + 1, ykf * o->p2koef, l, un1, ui1);
+    ADDS(9, 18); ADDS(11, 19); ADDS(1, 20); ADDS(5, 21); ADDS(13, 22); ADDS(15, 23);
+  }
+  un1[0] = 17;
+}
+
+static void zstokes_row(const orc_t *o, long m1, long m2, long m3, int l, long *un1, double *ui1) {
+  const int X = o->mgx[l], Y = o->mgy[l], Z = o->mgz[l];
+  const double *gx = o->mggx[l], *gy = o->mggy[l], *gz = o->mggz[l];
+  double xkf = 1.0 / (gx[m1 + 1] - gx[m1]);
+  double ykf = 1.0 / (gy[m2 + 1] - gy[m2]);
+  double zkf = 2.0 / (gz[m3 + 1] - gz[m3 - 1]);
+  if (o->zper && m3 == 0) zkf = 2.0 / (gz[1] - gz[0] + gz[Z - 1] - gz[Z - 2]);
+  long v[8];
+  v[0] = o->mgp[l] + m3 * o->mgxy[l] + m1 * Y + m2;
+  v[1] = v[0] + 1; v[2] = v[0] + Y; v[3] = v[2] + 1;
+  v[4] = v[0] + o->mgxy[l]; v[5] = v[4] + 1; v[6] = v[4] + Y; v[7] = v[6] + 1;
+  if (o->zper && m3 == 0) v[3] = o->mgp[l] + (Z - 1) * o->mgxy[l] + (m1 + 1) * Y + m2 + 1;
+  const float *ro = o->ro;
+  ui1[0] = -o->gzk * (ro[v[0]] + ro[v[1]] + ro[v[2]] + ro[v[3]]) / 4.0;
+  un1[0] = 17;
+  un1[16] = v[3] * 4 + 0; ui1[16] = +zkf;
+  un1[17] = v[7] * 4 + 0; ui1[17] = -zkf;
+  if (o->zper && m3 == 0) snn_coef(o, 2, m1 + 1, m2 + 1, Z - 1, -zkf, l, un1, ui1);
+  else snn_coef(o, 2, m1 + 1, m2 + 1, m3, -zkf, l, un1, ui1);
+  SETS(12, 18); SETS(14, 19); SETS(8, 20); SETS(9, 21); SETS(6, 22); SETS(1, 23);
+  un1[0] = 17;
+  snn_coef(o, 2, m1 + 1, m2 + 1, m3 + 1, zkf, l, un1, ui1);
+  SETS(13, 18); SETS(15, 19); SETS(10, 20); SETS(11, 21); ADDS(1, 22); SETS(7, 23);
+  un1[0] = 17;
+  sxz_coef(o, m1, m2, m3, -xkf, l, un1, ui1);
+  ADDS(12, 18); ADDS(13, 19); SETS(2, 20); ADDS(1, 21);
+  un1[0] = 17;
+  if (o->xper && m1 == X - 2) sxz_coef(o, 0, m2, m3, xkf, l, un1, ui1);
+  else sxz_coef(o, m1 + 1, m2, m3, xkf, l, un1, ui1);
+  ADDS(14, 18); ADDS(15, 19); ADDS(1, 20); SETS(3, 21);
+  un1[0] = 17;
+  syz_coef(o, m1, m2, m3, -ykf, l, un1, ui1);
+  ADDS(8, 18); ADDS(10, 19); SETS(4, 20); ADDS(1, 21);
+  un1[0] = 17;
+  if (o->yper && m2 == Y - 2) syz_coef(o, m1, 0, m3, ykf, l, un1, ui1);
+  else syz_coef(o, m1, m2 + 1, m3, ykf, l, un1, ui1);
+  ADDS(9, 18); ADDS(11, 19); ADDS(1, 20); SETS(5, 21);
+  if (o->p2koef != 0) {
+    un1[0] = 17;
+    if (o->zper && m3 == 0) continadd_coef(o, m1 + 1, m2 + 1, Z - 1, -zkf * o->p2koef, l, un1, ui1);
+    else continadd_coef(o, m1 + 1, m2 + 1, m3, -zkf * o->p2koef, l, un1, ui1);
+    ADDS(12, 18); ADDS(14, 19); ADDS(8, 20); ADDS(9, 21); ADDS(6, 22); ADDS(1, 23);
+    un1[0] = 17;
+    continadd_coef(o, m1 + 1, m2 + 1, m3 + 1, zkf * o->p2koef, l, un1, ui1);
+    ADDS(13, 18); ADDS(15, 19); ADDS(10, 20); ADDS(11, 21); ADDS(1, 22); ADDS(7, 23);
+  }
+  un1[0] = 17;
+}
+
+/* continuity row, move3mg.c:1455-1573; side effect nd[cell] = cell viscosity */
+static void contin_row(orc_t *o, long m1, long m2, long m3, int l, long *un1, double *ui1) {
+  double xkf = 1.0 / (o->mggx[l][m1] - o->mggx[l][m1 - 1]);
+  double ykf = 1.0 / (o->mggy[l][m2] - o->mggy[l][m2 - 1]);
+  double zkf = 1.0 / (o->mggz[l][m3] - o->mggz[l][m3 - 1]);
+  long v[8];
+  cell_octet(o, l, m1, m2, m3, v);
+  double ne = (l == 0) ? (double)o->nxx[v[7]] : cellvisc(o, v);
+  ne = clampnu(o, ne);
+  ui1[0] = 0;
+  un1[0] = 0;
+  o->nd[v[7]] = ne;
+  un1[1] = v[0] * 4 + 1; ui1[1] = -xkf;
+  un1[2] = v[2] * 4 + 1; ui1[2] = +xkf;
+  un1[3] = v[0] * 4 + 2; ui1[3] = -ykf;
+  un1[4] = v[1] * 4 + 2; ui1[4] = +ykf;
+  un1[5] = v[0] * 4 + 3; ui1[5] = -zkf;
+  un1[6] = v[4] * 4 + 3; ui1[6] = +zkf;
+  un1[0] = 6;
+}
+
+/* boundary row x_r - k x_j = c, move3mg.c:1682-1729 */
+static void bc_row(const orc_t *o, long r, long *un1, double *ui1) {
+  int b = o->bondm[r];
+  un1[0] = 1;
+  ui1[0] = o->bondv1[2 * b + 0];
+  un1[1] = r;
+  ui1[1] = 1.0;
+  if (o->bondn1[b]) {
+    un1[0] += 1;
+    un1[un1[0]] = o->bondn1[b] - 1;
+    ui1[un1[0]] = -o->bondv1[2 * b + 1];
+  }
+}
+
+/* one row of level l at node (m1,m2,m3), component c.  Returns 0 if the      */
+/* unknown is a ghost (no row), 1 for an equation row, 2 for a boundary row.  */
+/* Row selection follows move3mg.c:2372-2663.                                */
+int orc_row(orc_t *o, int l, int m1, int m2, int m3, int c, long *un1, double *ui1) {
+  long n = NID(o, l, m1, m2, m3);
+  long r = n * 4 + c;
+  const int X = o->mgx[l], Y = o->mgy[l], Z = o->mgz[l];
+  int exists;
+  if (c == 0) exists = (m1 && m2 && m3);
+  else if (c == 1) exists = (m2 < Y - 1 && m3 < Z - 1);
+  else if (c == 2) exists = (m1 < X - 1 && m3 < Z - 1);
+  else exists = (m1 < X - 1 && m2 < Y - 1);
+  if (!exists) {
+    un1[0] = 0;
+    ui1[0] = 0;
+    return 0;
+  }
+  if (o->bondm[r]) {
+    bc_row(o, r, un1, ui1);
+    return 2;
+  }
+  if (c == 0) contin_row(o, m1, m2, m3, l, un1, ui1);
+  else if (c == 1) xstokes_row(o, m1, m2, m3, l, un1, ui1);
+  else if (c == 2) ystokes_row(o, m1, m2, m3, l, un1, ui1);
+  else zstokes_row(o, m1, m2, m3, l, un1, ui1);
+  return 1;
+}
+
+/* gausmat2 row store, parallel/gaus3mg_MG.c:311-365 */
+static void gausmat2_store(orc_t *o, long mcmax, long pos0cur1, const long *un1, const double *ui1) {
+  o->fre1[mcmax] = ui1[0];
+  o->pos0[mcmax] = (int)pos0cur1;
+  o->num0[mcmax] = 0;
+  for (long m2 = 1; m2 <= un1[0]; m2++) {
+    o->lin0[pos0cur1] = (int)un1[m2];
+    o->val1[pos0cur1] = ui1[m2];
+    pos0cur1++;
+    o->num0[mcmax]++;
+  }
+}
+
+/* assembly of all levels (move3mg.c:2329-2663); ghost unknowns get no row */
+void orc_assemble(orc_t *o) {
+  static const int slot[4] = {0, 6, 23, 40};
+  long un1[ORC_MAXPOS];
+  double ui1[ORC_MAXPOS];
+  memset(o->num0, 0, sizeof(int) * 4 * o->ntot);
+  for (int l = 0; l <= o->L; l++)
+    for (int m3 = 0; m3 < o->mgz[l]; m3++)
+      for (int m1 = 0; m1 < o->mgx[l]; m1++)
+        for (int m2 = 0; m2 < o->mgy[l]; m2++) {
+          long n = NID(o, l, m1, m2, m3);
+          for (int c = 0; c < 4; c++) {
+            memset(un1, 0, sizeof(un1));
+            memset(ui1, 0, sizeof(ui1));
+            int k = orc_row(o, l, m1, m2, m3, c, un1, ui1);
+            if (k == 0) {
+              o->pos0[n * 4 + c] = (int)(n * 57 + slot[c]);
+              o->num0[n * 4 + c] = 0;
+              o->fre1[n * 4 + c] = 0;
+              continue;
+            }
+            gausmat2_store(o, n * 4 + c, n * 57 + slot[c], un1, ui1);
+          }
+        }
+}
+
+int orc_get_row(const orc_t *o, long r, int *idx, double *val, double *rhs) {
+  int n = o->num0[r];
+  for (int k = 0; k < n; k++) {
+    idx[k] = o->lin0[o->pos0[r] + k];
+    val[k] = o->val1[o->pos0[r] + k];
+  }
+  *rhs = o->fre1[r];
+  return n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Smoother + residual: parallel/move3mg.c:3855-3960 (gausmat4), one thread   */
+/* ------------------------------------------------------------------------- */
+static inline void row_update(orc_t *o, long m1, double *x) {
+  double ival = -o->fre1[m1];
+  double ival1 = o->val1[o->pos0[m1]];
+  for (int m2 = 0; m2 < o->num0[m1]; m2++)
+    ival += o->val1[o->pos0[m1] + m2] * x[o->lin0[o->pos0[m1] + m2]];
+  if (!(m1 & 3)) {
+    if (!o->bondm[m1]) x[m1] -= ival * o->nd[m1 / 4] * o->p0koef;
+    else x[m1] -= ival;
+  } else {
+    if (!o->bondm[m1]) x[m1] -= ival / ival1 * o->v0koef;
+    else x[m1] -= ival;
+  }
+}
+
+void orc_residual(orc_t *o, int l) {
+  long mcmin = o->mgp[l] * 4, mcmax = (o->mgp[l] + o->mgn[l]) * 4 - 1;
+  for (long m1 = mcmin; m1 <= mcmax; m1++)
+    if (o->num0[m1]) {
+      double ival = o->fre1[m1];
+      for (int m2 = 0; m2 < o->num0[m1]; m2++)
+        ival -= o->val1[o->pos0[m1] + m2] * o->sol0[o->lin0[o->pos0[m1] + m2]];
+      o->fre0[m1] = ival;
+    }
+}
+
+void orc_gausmat4(orc_t *o, int am, int l) {
+  for (int s = 0; s < am; s++)
+    for (long m30 = 0; m30 < o->mgn[l]; m30++)
+      for (int m40 = 0; m40 < 4; m40++) {
+        long m1 = (o->mgp[l] + m30) * 4 + m40;
+        if (o->vonly && m40 == 0) continue;
+        if (o->num0[m1]) row_update(o, m1, o->sol0);
+      }
+  orc_residual(o, l);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Coloured tile-hybrid sweep: the same gausmat4 row updates in the order the */
+/* GPU smoother uses.  Nodes are grouped in ty x tx x tz tiles (y fastest).   */
+/* Inside a tile the phases are vx(red, black), vy(red, black), vz(red,       */
+/* black), P, colour = (m1+m2+m3)&1; unknowns of other tiles are read from    */
+/* the state at the start of the sweep (block-Jacobi between tiles).          */
+/* Boundary rows are left out of the tile phases and updated afterwards in    */
+/* dependency order (orc_bc_pass).                                           */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  long mgp;
+  int Y, XY;
+  int y0, x0, z0, ty, tx, tz;
+} tilectx_t;
+
+static inline int in_tile(const tilectx_t *t, long u) {
+  long n = u / 4 - t->mgp;
+  if (n < 0) return 0;
+  int m3 = (int)(n / t->XY);
+  int rem = (int)(n - (long)m3 * t->XY);
+  int m1 = rem / t->Y, m2 = rem - m1 * t->Y;
+  return m2 >= t->y0 && m2 < t->y0 + t->ty && m1 >= t->x0 && m1 < t->x0 + t->tx && m3 >= t->z0 &&
+         m3 < t->z0 + t->tz;
+}
+
+static void tile_row_update(orc_t *o, long r, double *cur, const double *old, const tilectx_t *t) {
+  double ival = -o->fre1[r];
+  double ival1 = o->val1[o->pos0[r]];
+  for (int m2 = 0; m2 < o->num0[r]; m2++) {
+    long u = o->lin0[o->pos0[r] + m2];
+    double xu = in_tile(t, u) ? cur[u] : old[u];
+    ival += o->val1[o->pos0[r] + m2] * xu;
+  }
+  if (!(r & 3)) cur[r] -= ival * o->nd[r / 4] * o->p0koef;
+  else cur[r] -= ival / ival1 * o->v0koef;
+}
+
+static int bc_depth(const orc_t *o, long r, int lim) {
+  int d = 0;
+  while (d <= lim) {
+    int b = o->bondm[r];
+    if (!b || !o->bondn1[b]) return d;
+    long j = o->bondn1[b] - 1;
+    if (!o->bondm[j] || !o->num0[j]) return d;
+    r = j;
+    d++;
+  }
+  return -1;
+}
+
+void orc_bc_pass(orc_t *o, int l, double *x) {
+  long a = o->mgp[l] * 4, e = (o->mgp[l] + o->mgn[l]) * 4;
+  int maxd = 0;
+  for (long r = a; r < e; r++)
+    if (o->num0[r] && o->bondm[r]) {
+      int d = bc_depth(o, r, 64);
+      if (d > maxd) maxd = d;
+    }
+  for (int d = 0; d <= maxd; d++)
+    for (long r = a; r < e; r++)
+      if (o->num0[r] && o->bondm[r] && !(o->vonly && !(r & 3)) && bc_depth(o, r, 64) == d) {
+        double ival = -o->fre1[r];
+        for (int m2 = 0; m2 < o->num0[r]; m2++)
+          ival += o->val1[o->pos0[r] + m2] * x[o->lin0[o->pos0[r] + m2]];
+        x[r] -= ival;
+      }
+}
+
+void orc_sweep_tiles(orc_t *o, int am, int l, int ty, int tx, int tz) {
+  const int X = o->mgx[l], Y = o->mgy[l], Z = o->mgz[l];
+  long a = o->mgp[l] * 4, e = (o->mgp[l] + o->mgn[l]) * 4;
+  double *old = o->tmp;
+  double *cur = o->sol0;
+  static const int comp[4] = {1, 2, 3, 0};
+  for (int s = 0; s < am; s++) {
+    memcpy(old + a, cur + a, sizeof(double) * (e - a));
+    tilectx_t t;
+    t.mgp = o->mgp[l]; t.Y = Y; t.XY = o->mgxy[l]; t.ty = ty; t.tx = tx; t.tz = tz;
+    for (t.z0 = 0; t.z0 < Z; t.z0 += tz)
+      for (t.x0 = 0; t.x0 < X; t.x0 += tx)
+        for (t.y0 = 0; t.y0 < Y; t.y0 += ty)
+          for (int ph = 0; ph < (o->vonly ? 6 : 7); ph++) {
+            int c = comp[ph / 2];
+            int colour = ph & 1;
+            for (int m3 = t.z0; m3 < t.z0 + tz && m3 < Z; m3++)
+              for (int m1 = t.x0; m1 < t.x0 + tx && m1 < X; m1++)
+                for (int m2 = t.y0; m2 < t.y0 + ty && m2 < Y; m2++) {
+                  if (c != 0 && ((m1 + m2 + m3) & 1) != colour) continue;
+                  long r = NID(o, l, m1, m2, m3) * 4 + c;
+                  if (!o->num0[r] || o->bondm[r]) continue;
+                  tile_row_update(o, r, cur, old, &t);
+                }
+          }
+    orc_bc_pass(o, l, cur);
+  }
+  orc_residual(o, l);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Residual restriction: parallel/move3mg.c:3966-4354 (one thread)            */
+/* Per-axis rules: 'N' = staggered on grid lines (vx in x), 'C' = cell centre */
+/* with lower index (vx in y/z), 'P' = pressure cell (upper-corner index).    */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  int lo;
+  double d;
+} axmap_t;
+
+static void ax_rule(char kind, int m, int nf, int nc, const double *gf, const double *gc, int *lo_out,
+                    double *d_out) {
+  int lo = (m - 2) / 2 + 2;
+  if (m < 3) lo = m;
+  if (m > nf - 4) lo = nc - (nf - m);
+  if (kind == 'N') {
+    if (lo > nc - 2) lo = nc - 2;
+    *d_out = gc[lo + 1] - gc[lo];
+  } else if (kind == 'C') {
+    if (lo > nc - 3) lo = nc - 3;
+    if ((gf[m] + gf[m + 1]) / 2.0 < (gc[lo] + gc[lo + 1]) / 2.0 && lo > 0) lo -= 1;
+    *d_out = (gc[lo + 2] - gc[lo]) / 2.0;
+  } else {
+    if (lo > nc - 2) lo = nc - 2;
+    if ((gf[m] + gf[m - 1]) / 2.0 < (gc[lo] + gc[lo - 1]) / 2.0 && lo > 1) lo -= 1;
+    *d_out = (gc[lo + 1] - gc[lo - 1]) / 2.0;
+  }
+  *lo_out = lo;
+}
+
+static double ax_w(char kind, int cc, int lo, int m, const double *gf, const double *gc, double d) {
+  double dd;
+  if (kind == 'N') dd = (gc[cc] - gf[m]) / d;
+  else if (kind == 'C') dd = (gc[cc] + gc[cc + 1] - gf[m] - gf[m + 1]) / 2.0 / d;
+  else dd = (gc[cc] + gc[cc - 1] - gf[m] - gf[m - 1]) / 2.0 / d;
+  if (cc == lo) dd = -dd;
+  return 1.0 - dd;
+}
+
+static const char KIND[4][3] = {{'P', 'P', 'P'}, {'N', 'C', 'C'}, {'C', 'N', 'C'}, {'C', 'C', 'N'}};
+
+static int comp_exists(const orc_t *o, int l, int m1, int m2, int m3, int c) {
+  if (c == 0) return m1 && m2 && m3;
+  if (c == 1) return m2 < o->mgy[l] - 1 && m3 < o->mgz[l] - 1;
+  if (c == 2) return m1 < o->mgx[l] - 1 && m3 < o->mgz[l] - 1;
+  return m1 < o->mgx[l] - 1 && m2 < o->mgy[l] - 1;
+}
+
+void orc_vprestrict(orc_t *o, int l) {
+  const int f = l - 1;
+  long a = o->mgp[l] * 4, e = (o->mgp[l] + o->mgn[l]) * 4;
+  double *S0 = (double *)calloc(e - a, sizeof(double)), *S1 = (double *)calloc(e - a, sizeof(double));
+  for (long m = a; m < e; m++) { o->fre1[m] = 0; o->sol1[m] = 0; }
+  const double *gf[3] = {o->mggx[f], o->mggy[f], o->mggz[f]};
+  const double *gc[3] = {o->mggx[l], o->mggy[l], o->mggz[l]};
+  const int nfa[3] = {o->mgx[f], o->mgy[f], o->mgz[f]}, nca[3] = {o->mgx[l], o->mgy[l], o->mgz[l]};
+  for (int m3 = 0; m3 < o->mgz[f]; m3++)
+    for (int m1 = 0; m1 < o->mgx[f]; m1++)
+      for (int m2 = 0; m2 < o->mgy[f]; m2++) {
+        long m4 = NID(o, f, m1, m2, m3);
+        static const int order[4] = {1, 2, 3, 0};
+        for (int q = 0; q < 4; q++) {
+          int c = order[q];
+          if (!comp_exists(o, f, m1, m2, m3, c) || o->bondm[m4 * 4 + c]) continue;
+          int mm[3] = {m1, m2, m3}, lo[3];
+          double d[3];
+          for (int ax = 0; ax < 3; ax++)
+            ax_rule(KIND[c][ax], mm[ax], nfa[ax], nca[ax], gf[ax], gc[ax], &lo[ax], &d[ax]);
+          for (int m30 = lo[2]; m30 <= lo[2] + 1; m30++) {
+            double ddz = ax_w(KIND[c][2], m30, lo[2], m3, gf[2], gc[2], d[2]);
+            for (int m10 = lo[0]; m10 <= lo[0] + 1; m10++) {
+              double ddx = ax_w(KIND[c][0], m10, lo[0], m1, gf[0], gc[0], d[0]);
+              for (int m20 = lo[1]; m20 <= lo[1] + 1; m20++) {
+                double ddy = ax_w(KIND[c][1], m20, lo[1], m2, gf[1], gc[1], d[1]);
+                long m40 = NID(o, l, m10, m20, m30);
+                if (o->bondm[m40 * 4 + c]) continue;
+                double swt = ddx * ddy * ddz;
+                double ival2;
+                if (c == 0) ival2 = o->fre0[m4 * 4] * o->nd[m4] / o->nd[m40] * swt;
+                else ival2 = o->fre0[m4 * 4 + c] * swt;
+                S0[m40 * 4 + c - a] += ival2;
+                S1[m40 * 4 + c - a] += swt;
+              }
+            }
+          }
+        }
+      }
+  for (long m = a; m < e; m++) {
+    o->fre1[m] += S0[m - a];
+    o->sol1[m] += S1[m - a];
+  }
+  for (long m = a; m < e; m++) {
+    if (o->sol1[m]) {
+      o->fre1[m] /= o->sol1[m];
+      o->sol1[m] = 0;
+    }
+    o->sol0[m] = 0;
+  }
+  free(S0);
+  free(S1);
+}
+
+/* correction prolongation: parallel/move3mg.c:3535-3850 */
+void orc_vpprolong(orc_t *o, int l, int ynreset) {
+  const int f = l - 1;
+  const double *gf[3] = {o->mggx[f], o->mggy[f], o->mggz[f]};
+  const double *gc[3] = {o->mggx[l], o->mggy[l], o->mggz[l]};
+  const int nfa[3] = {o->mgx[f], o->mgy[f], o->mgz[f]}, nca[3] = {o->mgx[l], o->mgy[l], o->mgz[l]};
+  static const int order[4] = {1, 2, 3, 0};
+  for (int m3 = 0; m3 < o->mgz[f]; m3++)
+    for (int m1 = 0; m1 < o->mgx[f]; m1++)
+      for (int m2 = 0; m2 < o->mgy[f]; m2++) {
+        long m4 = NID(o, f, m1, m2, m3);
+        for (int q = 0; q < 4; q++) {
+          int c = order[q];
+          if (!comp_exists(o, f, m1, m2, m3, c)) continue;
+          if (o->vonly && c == 0) continue;
+          int mm[3] = {m1, m2, m3}, lo[3];
+          double d[3];
+          for (int ax = 0; ax < 3; ax++)
+            ax_rule(KIND[c][ax], mm[ax], nfa[ax], nca[ax], gf[ax], gc[ax], &lo[ax], &d[ax]);
+          double k = (c == 0) ? o->p1koef : o->v1koef;
+          if (ynreset) o->sol0[m4 * 4 + c] = 0;
+          for (int m30 = lo[2]; m30 <= lo[2] + 1; m30++) {
+            double ddz = ax_w(KIND[c][2], m30, lo[2], m3, gf[2], gc[2], d[2]);
+            for (int m10 = lo[0]; m10 <= lo[0] + 1; m10++) {
+              double ddx = ax_w(KIND[c][0], m10, lo[0], m1, gf[0], gc[0], d[0]);
+              for (int m20 = lo[1]; m20 <= lo[1] + 1; m20++) {
+                double ddy = ax_w(KIND[c][1], m20, lo[1], m2, gf[1], gc[1], d[1]);
+                long m40 = NID(o, l, m10, m20, m30);
+                double swt = ddx * ddy * ddz;
+                o->sol0[m4 * 4 + c] += o->sol0[m40 * 4 + c] * swt * k;
+              }
+            }
+          }
+        }
+      }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Coarsest-level direct solve.  The reference hands the coarsest CSR system */
+/* to MKL PARDISO (parallel/gaus3mg_MG.c:94-249); this is a dense LU with     */
+/* partial pivoting (row interchanges, first maximal |a| wins), on the same  */
+/* assembled rows; ghost unknowns get identity rows (move3mg.c:2417-2425).    */
+/* The solution is stored at the hierarchy offset 4*(mgp[L]+local)+c.          */
+/* ------------------------------------------------------------------------- */
+int orc_dense_lu(int n, double *a, int *piv) {
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    double best = fabs(a[(long)k * n + k]);
+    for (int i = k + 1; i < n; i++) {
+      double v = fabs(a[(long)i * n + k]);
+      if (v > best) { best = v; p = i; }
+    }
+    piv[k] = p;
+    if (best == 0.0) return k + 1;
+    if (p != k)
+      for (int j = 0; j < n; j++) {
+        double t = a[(long)k * n + j];
+        a[(long)k * n + j] = a[(long)p * n + j];
+        a[(long)p * n + j] = t;
+      }
+    double inv = a[(long)k * n + k];
+    for (int i = k + 1; i < n; i++) a[(long)i * n + k] /= inv;
+    for (int i = k + 1; i < n; i++) {
+      double lik = a[(long)i * n + k];
+      if (lik == 0.0) continue;
+      double *ri = a + (long)i * n, *rk = a + (long)k * n;
+      for (int j = k + 1; j < n; j++) ri[j] -= lik * rk[j];
+    }
+  }
+  return 0;
+}
+
+void orc_dense_lu_solve(int n, const double *lu, const int *piv, double *b) {
+  for (int k = 0; k < n; k++)
+    if (piv[k] != k) { double t = b[k]; b[k] = b[piv[k]]; b[piv[k]] = t; }
+  for (int i = 0; i < n; i++) {
+    double s = b[i];
+    for (int j = 0; j < i; j++) s -= lu[(long)i * n + j] * b[j];
+    b[i] = s;
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double s = b[i];
+    for (int j = i + 1; j < n; j++) s -= lu[(long)i * n + j] * b[j];
+    b[i] = s / lu[(long)i * n + i];
+  }
+}
+
+int orc_coarse_factor(orc_t *o) {
+  int L = o->L;
+  int n = (int)(4 * o->mgn[L]);
+  long base = 4 * o->mgp[L];
+  free(o->lu);
+  free(o->piv);
+  o->nc = n;
+  o->lu = (double *)calloc((size_t)n * n, sizeof(double));
+  o->piv = (int *)calloc(n, sizeof(int));
+  for (int i = 0; i < n; i++) {
+    long r = base + i;
+    if (!o->num0[r] || (o->vonly && !(r & 3))) {
+      o->lu[(long)i * n + i] = 1.0;
+      continue;
+    }
+    for (int k = 0; k < o->num0[r]; k++) {
+      long c = o->lin0[o->pos0[r] + k] - base;
+      if (c < 0 || c >= n) return -2;
+      o->lu[(long)i * n + c] += o->val1[o->pos0[r] + k];
+    }
+  }
+  return orc_dense_lu(n, o->lu, o->piv);
+}
+
+int orc_coarse_solve(orc_t *o) {
+  int L = o->L;
+  long base = 4 * o->mgp[L];
+  int n = o->nc;
+  double *b = (double *)malloc(sizeof(double) * n);
+  for (int i = 0; i < n; i++)
+    b[i] = (o->num0[base + i] && !(o->vonly && !((base + i) & 3))) ? o->fre1[base + i] : 0.0;
+  orc_dense_lu_solve(n, o->lu, o->piv, b);
+  for (int i = 0; i < n; i++) o->sol0[base + i] = b[i];
+  free(b);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* V-cycle (move3mg.c:2710-2792 with the coarse solve every cycle on the     */
+/* restricted residual, as sketched at pardiso/move3mg.c:2718,2756-2763).     */
+/* mode 0: lexicographic gausmat4; mode 1: coloured tile sweep.               */
+/* ------------------------------------------------------------------------- */
+static void smooth(orc_t *o, int am, int l, int mode, int ty, int tx, int tz) {
+  if (am <= 0) { orc_residual(o, l); return; }
+  if (mode == 0) orc_gausmat4(o, am, l);
+  else orc_sweep_tiles(o, am, l, ty, tx, tz);
+}
+
+void orc_vcycle(orc_t *o, const int *pre, const int *post, int mode, int ty, int tx, int tz) {
+  int L = o->L;
+  if (L == 0) {
+    smooth(o, pre[0], 0, mode, ty, tx, tz);
+    return;
+  }
+  for (int l = 0; l < L; l++) {
+    smooth(o, pre[l], l, mode, ty, tx, tz);
+    orc_vprestrict(o, l + 1);
+  }
+  orc_coarse_solve(o);
+  orc_vpprolong(o, L, 0);
+  for (int l = L - 1; l > 0; l--) {
+    if (post[l] > 0) smooth(o, post[l], l, mode, ty, tx, tz);
+    orc_vpprolong(o, l, 0);
+  }
+  if (post[0] > 0) smooth(o, post[0], 0, mode, ty, tx, tz);
+}
+
+/* Relative residual on the finest level: ||W r|| / ||W b||, W = 1 on        */
+/* momentum and boundary rows, W = eta_cell / h_cell on continuity rows       */
+/* (h_cell = mean cell edge).  blocks3 = {||r_mom||, ||W r_cont||, ||r_bc||}. */
+double orc_relres(orc_t *o, double *blocks3) {
+  double sr = 0, sb = 0, bm = 0, bc = 0, bb = 0;
+  const double *gx = o->mggx[0], *gy = o->mggy[0], *gz = o->mggz[0];
+  for (int m3 = 0; m3 < o->mgz[0]; m3++)
+    for (int m1 = 0; m1 < o->mgx[0]; m1++)
+      for (int m2 = 0; m2 < o->mgy[0]; m2++) {
+        long n = NID(o, 0, m1, m2, m3);
+        for (int c = 0; c < 4; c++) {
+          long r = n * 4 + c;
+          if (!o->num0[r]) continue;
+          double res = o->fre1[r];
+          for (int k = 0; k < o->num0[r]; k++) res -= o->val1[o->pos0[r] + k] * o->sol0[o->lin0[o->pos0[r] + k]];
+          double w = 1.0;
+          if (c == 0 && !o->bondm[r]) {
+            double h = ((gx[m1] - gx[m1 - 1]) + (gy[m2] - gy[m2 - 1]) + (gz[m3] - gz[m3 - 1])) / 3.0;
+            w = o->nd[n] / h;
+          }
+          sr += (w * res) * (w * res);
+          sb += (w * o->fre1[r]) * (w * o->fre1[r]);
+          if (o->bondm[r]) bb += res * res;
+          else if (c == 0) bc += (w * res) * (w * res);
+          else bm += res * res;
+        }
+      }
+  if (blocks3) { blocks3[0] = sqrt(bm); blocks3[1] = sqrt(bc); blocks3[2] = sqrt(bb); }
+  return sb > 0 ? sqrt(sr) / sqrt(sb) : sqrt(sr);
+}
+
+int orc_solve(orc_t *o, const int *pre, const int *post, int mode, double tol, int maxcyc,
+              double *hist) {
+  double r = orc_relres(o, NULL);
+  if (hist) hist[0] = r;
+  int it = 0;
+  while (r > tol && it < maxcyc) {
+    orc_vcycle(o, pre, post, mode, 64, 4, 4);
+    it++;
+    r = orc_relres(o, NULL);
+    if (hist) hist[it] = r;
+    if (!(r == r)) return -it;
+  }
+  return it;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Banded direct solve of the finest assembled system: restatement of the    */
+/* frontal banded Gaussian elimination gausmat3 (parallel/gausjomp.c:70-254), */
+/* single thread.  Rows are processed in order: the row is gathered into a   */
+/* dense buffer, entries left of the diagonal are eliminated with the already */
+/* normalised rows (gausjomp.c:150-183), the row is normalised and its right  */
+/* band (width rightnum = max(col-row)) stored (gausjomp.c:196-204); then     */
+/* back substitution (gausjomp.c:219-250).  No pivoting, as in the reference. */
+/* Returns 0; -1 if the band store exceeds the reference's MAXMAT=1e9         */
+/* (parallel/head3mg.c:13); -2 on allocation failure; -3 singular line.       */
+/* ------------------------------------------------------------------------- */
+int orc_band_solve(orc_t *o, double *x, long *band_out) {
+  long n = 4 * o->mgn[0];
+  long rightn = 0;
+  for (long r = 0; r < n; r++)
+    for (int k = 0; k < o->num0[r]; k++) {
+      long d = (long)o->lin0[o->pos0[r] + k] - r;
+      if (d > rightn) rightn = d;
+    }
+  if (band_out) *band_out = rightn;
+  if ((double)n * (double)rightn > 1e9) return -1;
+  double *U = (double *)calloc((size_t)n * (rightn ? rightn : 1), sizeof(double));
+  double *buf = (double *)calloc((size_t)(n + rightn + 1), sizeof(double));
+  double *f = (double *)malloc(sizeof(double) * n);
+  if (!U || !buf || !f) { free(U); free(buf); free(f); return -2; }
+  for (long r = 0; r < n; r++) f[r] = o->fre1[r];
+  for (long m1 = 0; m1 < n; m1++) {
+    if (!o->num0[m1]) continue;
+    long m6 = m1;
+    for (int k = 0; k < o->num0[m1]; k++) {
+      long m4 = o->lin0[o->pos0[m1] + k];
+      buf[m4] += o->val1[o->pos0[m1] + k];
+      if (m4 < m6) m6 = m4;
+    }
+    for (long m2 = m6; m2 < m1; m2++)
+      if (buf[m2]) {
+        double ival = buf[m2];
+        buf[m2] = 0;
+        if (!o->num0[m2]) { free(U); free(buf); free(f); return -3; }
+        const double *u = U + m2 * rightn;
+        for (long m3 = m2 + 1; m3 <= m2 + rightn; m3++) buf[m3] -= ival * u[m3 - m2 - 1];
+        f[m1] -= f[m2] * ival;
+      }
+    double ival = buf[m1];
+    buf[m1] = 0;
+    if (!ival) { free(U); free(buf); free(f); return -3; }
+    double *u = U + m1 * rightn;
+    for (long m2 = m1 + 1; m2 <= m1 + rightn; m2++) {
+      u[m2 - m1 - 1] = buf[m2] / ival;
+      buf[m2] = 0;
+    }
+    f[m1] /= ival;
+  }
+  for (long m1 = n - 1; m1 >= 0; m1--) {
+    if (o->num0[m1]) {
+      const double *u = U + m1 * rightn;
+      double ival = f[m1];
+      for (long m2 = m1 + 1; m2 < m1 + 1 + rightn && m2 < n; m2++) ival -= u[m2 - m1 - 1] * x[m2];
+      x[m1] = ival;
+    } else {
+      x[m1] = 0;
+    }
+  }
+  free(U); free(buf); free(f);
+  return 0;
+}

@@ -1,0 +1,121 @@
+/*
+ * stokes_oracle.h -- CPU restatement of the reference's 3D variable-viscosity
+ * Stokes multigrid solve (marker-in-cell staggered grid), used ONLY as test
+ * infrastructure: the checker that the HIP library is compared against, and
+ * the timed CPU baseline.  Nothing in the product library links this file.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the reference tree).  The restatement keeps the reference's storage types
+ * (FP32 node properties and boundary tables, FP64 solver vectors and matrix
+ * rows) so that single-threaded results can be compared bit-for-bit where the
+ * reference's arithmetic order is sequential.
+ *
+ * Layout (parallel/head3mg.c:59-113, parallel/load3mg.c:2011-2320):
+ *   level l has mgx[l] x mgy[l] x mgz[l] nodes, node index
+ *     n = mgp[l] + m3*mgxy[l] + m1*mgy[l] + m2     (y fastest)
+ *   unknown r = 4*n + c, c = 0:P 1:vx 2:vy 3:vz, levels concatenated.
+ *   Assembled rows live in a 57-slot-per-node ELL store
+ *     pos0[4n+c] = 57n + {0,6,23,40}[c], num0 = entry count.
+ */
+#ifndef STOKES_ORACLE_H
+#define STOKES_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAXLVL 12
+#define ORC_MAXPOS 64
+
+typedef struct orc_s {
+  int L; /* index of the coarsest level (reference: multinum) */
+  int mgx[ORC_MAXLVL], mgy[ORC_MAXLVL], mgz[ORC_MAXLVL], mgxy[ORC_MAXLVL];
+  long mgn[ORC_MAXLVL], mgp[ORC_MAXLVL], mgs[ORC_MAXLVL];
+  double *mggx[ORC_MAXLVL], *mggy[ORC_MAXLVL], *mggz[ORC_MAXLVL];
+  int xper, yper, zper;
+  double xsize, ysize, zsize;
+  double gxk, gyk, gzk; /* GXKOEF, GYKOEF, GZKOEF */
+  double p0koef, p1koef, p2koef, v0koef, v1koef, nubeg, nuend;
+  int viscmod;
+  long ntot; /* nodes over all levels */
+  /* node properties (FP32, like the reference) */
+  float *nu, *ro, *nd;          /* ntot */
+  float *nxx, *nxy, *nxz, *nyz; /* mgn[0] */
+  /* boundary equations  x_r = bondv1[b][0] + bondv1[b][1] * x_{bondn1[b]-1} */
+  int *bondm; /* 4*ntot, 0 = no boundary equation */
+  int nbond, capbond;
+  float *bondv1; /* 2*(capbond+1) */
+  int *bondn1;   /* capbond+1 */
+  /* solver vectors (FP64) */
+  double *sol0, *sol1, *fre0, *fre1; /* 4*ntot */
+  /* assembled rows */
+  double *val1; /* 57*ntot */
+  int *lin0;    /* 57*ntot */
+  int *num0, *pos0; /* 4*ntot */
+  /* coarsest dense LU */
+  int nc;
+  double *lu;
+  int *piv;
+  /* velocity-block mode: smoothers skip pressure rows, the coarsest LU
+     treats pressure unknowns as fixed (identity rows), prolongation skips P */
+  int vonly;
+  /* scratch */
+  double *tmp;
+} orc_t;
+
+/* hierarchy (parallel/load3mg.c:2011-2195) */
+orc_t *orc_create(int nx, int ny, int nz, int L, const double *gx, const double *gy,
+                  const double *gz, int xper, int yper, int zper);
+void orc_destroy(orc_t *o);
+void orc_set_vonly(orc_t *o, int flag);
+void orc_set_knobs(orc_t *o, double p0, double p1, double p2, double v0, double v1,
+                   double nubeg, double nuend, int viscmod, double gxk, double gyk, double gzk);
+int orc_level_info(const orc_t *o, int l, int *out6); /* mgx,mgy,mgz,mgxy,mgn,mgp */
+double *orc_grid(orc_t *o, int l, int axis);
+void *orc_array(orc_t *o, const char *name);
+
+/* boundary-equation grammar projected on every level (parallel/in3mg.c:234-390) */
+int orc_bc_line(orc_t *o, int var, int m10, int m11, int m20, int m21, int m30, int m31,
+                float cnst, float koef, int dm1, int dm2, int dm3);
+
+/* marker -> node averaging (parallel/mark3mg.c:630-1132) */
+int orc_ronurecalc(orc_t *o, long nmark, const float *mx, const float *my, const float *mz,
+                   const float *mnu, const float *mro, double nukoef);
+/* viscosity/density restriction (parallel/move3mg.c:3387-3528) */
+void orc_ronurestrict(orc_t *o, int l);
+
+/* row assembly (parallel/move3mg.c:796-1787 + 2329-2663, parallel/gaus3mg_MG.c:311-365) */
+int orc_row(orc_t *o, int l, int m1, int m2, int m3, int c, long *un1, double *ui1);
+void orc_assemble(orc_t *o);
+int orc_get_row(const orc_t *o, long r, int *idx, double *val, double *rhs);
+
+/* smoother + residual (parallel/move3mg.c:3855-3960) */
+void orc_gausmat4(orc_t *o, int am, int l);
+void orc_residual(orc_t *o, int l);
+/* coloured tile-hybrid sweep of the same row updates (the GPU ordering) */
+void orc_sweep_tiles(orc_t *o, int am, int l, int ty, int tx, int tz);
+void orc_bc_pass(orc_t *o, int l, double *x);
+
+/* transfers (parallel/move3mg.c:3535-3850, 3966-4354) */
+void orc_vprestrict(orc_t *o, int l);
+void orc_vpprolong(orc_t *o, int l, int ynreset);
+
+/* coarsest direct solve (replaces parallel/gaus3mg_MG.c:94-249) */
+int orc_coarse_factor(orc_t *o);
+int orc_coarse_solve(orc_t *o);
+int orc_dense_lu(int n, double *a, int *piv);
+void orc_dense_lu_solve(int n, const double *lu, const int *piv, double *b);
+
+/* V-cycle (canonical shape of mg/move3mgold.c:2486-2600, fixes for K1/K2) */
+void orc_vcycle(orc_t *o, const int *pre, const int *post, int mode, int ty, int tx, int tz);
+double orc_relres(orc_t *o, double *blocks3);
+int orc_solve(orc_t *o, const int *pre, const int *post, int mode, double tol, int maxcyc,
+              double *relres_hist);
+
+/* banded direct solver restatement (parallel/gausjomp.c:70-254) */
+int orc_band_solve(orc_t *o, double *x, long *band_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
