@@ -1,0 +1,165 @@
+/*
+ * pardiso2_amd.h -- C ABI of the MI355X (gfx950) variable-viscosity Stokes
+ * multigrid library.
+ *
+ * Two surfaces:
+ *
+ *  1. The reference's own entry points (same names, same C signatures, same
+ *     argument meaning), acting on the context selected with stk_bind().
+ *     Each declaration cites the reference interface it replaces.  Where the
+ *     reference reads global arrays (sol0, fre0, fre1, nd, bondm, ...) the
+ *     library uses the bound context's copies of them, laid out exactly like
+ *     the reference's (node-major, y fastest, 4 unknowns P,vx,vy,vz per node,
+ *     levels concatenated at mgp[l]).  Accessors stk_vec_set/get move those
+ *     arrays between host and device.
+ *
+ *  2. A context API (stk_*) for the matrix-free GPU solve: build the level
+ *     hierarchy, load boundary equations and node properties, prepare the
+ *     coarse operators, and solve to a relative residual.
+ *
+ * All pointers are host pointers unless a name says _device.  Functions
+ * return 0 on success and a negative STK_E* code on failure; nothing calls
+ * exit() (the reference's fatal paths, e.g. parallel/gaus3mg_MG.c:173-204,
+ * become status codes).
+ */
+#ifndef PARDISO2_AMD_H
+#define PARDISO2_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STK_OK 0
+#define STK_EINVAL -1      /* invalid argument or unsupported configuration */
+#define STK_ENOMEM -2      /* device allocation failed */
+#define STK_EHIP -3        /* HIP runtime error */
+#define STK_ESINGULAR -4   /* coarsest-level matrix singular */
+#define STK_ENOCONV -5     /* solve did not reach the tolerance */
+#define STK_ENAN -6        /* NaN/Inf in the iteration */
+#define STK_ESTATE -7      /* call order violated (e.g. solve before prepare) */
+#define STK_ENOCTX -8      /* reference-named entry point without stk_bind() */
+
+typedef struct stk_ctx stk_ctx;
+
+/* Solver knobs with the reference's meaning (parallel/mode.t3c:84-96,
+ * parallel/load3mg.c:116-131).  gx/gy/gz are GXKOEF/GYKOEF/GZKOEF. */
+typedef struct {
+  double p0koef, p1koef, p2koef, v0koef, v1koef;
+  double nubeg, nuend;
+  int viscmod;
+  double gx, gy, gz;
+} stk_knobs;
+
+typedef struct {
+  int precision;     /* 0: FP32 V-cycle inside FP64 outer iteration; 1: FP64 throughout */
+  int restart;       /* GCR direction history length (1..64) */
+  int pre[12];       /* pre-smoothing sweeps per level */
+  int post[12];      /* post-smoothing sweeps per level */
+  int tile[3];       /* smoother tile (y, x, z) in nodes; 0 = default */
+  double omega;      /* pressure block scale of the preconditioner */
+  double tol;        /* target relative residual ||W r|| / ||W b|| */
+  int maxit;         /* outer iteration cap */
+  int verbose;
+} stk_solver_opts;
+
+typedef struct {
+  int iterations;
+  int status;
+  double relres;     /* final relative residual, recomputed from b - A x in FP64 */
+  double relres0;
+  double seconds;    /* wall time of the solve on the device stream */
+  double blocks[3];  /* ||r_momentum||, ||W r_continuity||, ||r_boundary|| */
+  double hist[256];  /* relative residual after each outer iteration */
+} stk_stats;
+
+/* ---- context API ------------------------------------------------------ */
+
+/* Level hierarchy from the fine grid lines (restates gridcheck,
+ * parallel/load3mg.c:2011-2195): mg[l] = (mg[l-1]-5)/2+5, coarse lines keep
+ * the three outer lines per side.  levels = coarsest level index (multinum). */
+int stk_create(stk_ctx **ctx, int nx, int ny, int nz, int levels, const double *gx,
+               const double *gy, const double *gz, int xperiodic, int yperiodic, int zperiodic);
+void stk_destroy(stk_ctx *ctx);
+/* mgx, mgy, mgz, mgxy, mgn, mgp of level l */
+int stk_level_info(const stk_ctx *ctx, int level, int out6[6]);
+int stk_set_knobs(stk_ctx *ctx, const stk_knobs *k);
+void stk_default_opts(stk_solver_opts *o, int levels);
+
+/* One boundary-equation line of the reference grammar, projected on every
+ * level (restates parallel/in3mg.c:234-390).  var: 0 P, 1 vx, 2 vy, 3 vz. */
+int stk_bc_line(stk_ctx *ctx, int var, int m10, int m11, int m20, int m21, int m30, int m31,
+                float cnst, float koef, int dm1, int dm2, int dm3);
+/* Boundary tables in the reference's own layout (parallel/head3mg.c:110-112):
+ * bondm[4*ntot] (0 = none), bondv1[2*(nbond+1)], bondn1[nbond+1] (1-based). */
+int stk_set_bc_tables(stk_ctx *ctx, const int *bondm, int nbond, const float *bondv1,
+                      const int *bondn1);
+int stk_get_bc_tables(const stk_ctx *ctx, int *bondm, int *nbond, float *bondv1, int *bondn1);
+
+/* Finest-level node properties (parallel/head3mg.c:94-96): nu, ro at nodes,
+ * nxx at cell centres (upper-corner node index), nxy/nxz/nyz on edges. */
+int stk_set_fields(stk_ctx *ctx, const float *nu, const float *ro, const float *nxx,
+                   const float *nxy, const float *nxz, const float *nyz);
+/* Marker -> node averaging on the device (restates ronurecalc,
+ * parallel/mark3mg.c:630-1132, for per-marker viscosity and density). */
+int stk_markers_to_nodes(stk_ctx *ctx, long nmark, const float *mx, const float *my,
+                         const float *mz, const float *mnu, const float *mro, double nukoef);
+/* node arrays over all levels: nu, ro (ntot), nd (ntot), level-0 nxx..nyz */
+int stk_get_fields(stk_ctx *ctx, float *nu, float *ro, float *nd, float *nxx, float *nxy,
+                   float *nxz, float *nyz);
+
+/* Restrict nu/ro to all levels, build stencil viscosities, right-hand side
+ * and the coarsest-level inverse.  Must follow any field/BC/knob change. */
+int stk_prepare(stk_ctx *ctx);
+
+/* Finest-level solution in the reference's sol0 layout (4*mgn[0] doubles). */
+int stk_set_solution(stk_ctx *ctx, const double *sol0_level0);
+int stk_get_solution(stk_ctx *ctx, double *sol0_level0);
+/* Solve A x = b on the finest level from the current solution. */
+int stk_solve(stk_ctx *ctx, const stk_solver_opts *opts, stk_stats *stats);
+/* Relative residual of the current finest solution (FP64, device). */
+int stk_relres(stk_ctx *ctx, double *relres, double blocks[3]);
+/* Shift P so that P(px,py,pz) = pinit (parallel/move3mg.c:2835-2856). */
+int stk_pressure_gauge(stk_ctx *ctx, int px, int py, int pz, double pinit);
+
+/* ---- reference arrays on the context (host <-> device) ---------------- */
+#define STK_SOL0 0
+#define STK_FRE1 1
+#define STK_FRE0 2
+int stk_vec_set(stk_ctx *ctx, int which, int level, const double *host);
+int stk_vec_get(stk_ctx *ctx, int which, int level, double *host);
+/* y = A x on a level (FP64, matrix-free), x/y host arrays of 4*mgn[l] */
+int stk_apply(stk_ctx *ctx, int level, const double *x, double *y);
+/* Smoother sweeps in tile-coloured order on the context's sol0/fre1 of one
+ * level, FP64, all rows (pressure Uzawa step included), then fre0. */
+int stk_smooth(stk_ctx *ctx, int level, int sweeps, int ty, int tx, int tz);
+int stk_coarse_solve(stk_ctx *ctx);
+
+/* ---- reference entry points (act on the bound context) ---------------- */
+int stk_bind(stk_ctx *ctx);
+
+/* parallel/gaus3mg_MG.c:2 -- clear an rm x (rm+1) dense matrix */
+void matclear(int rm, double *adr);
+/* parallel/gaus3mg_MG.c:21 -- dense Gauss elimination, no pivoting */
+void gausmat(int rm, double *adr, double *adr1);
+/* parallel/gaus3mg_MG.c:311 -- store one assembled row (un1/ui1 format) */
+int gausmat2(int am, long mcmax, long pos0cur1, long *un1, double *ui1);
+/* parallel/gaus3mg_MG.c:64 -- am=1: add one coarsest-level CSR row;
+ * am=0: solve the coarsest system (device dense inverse) and store the
+ * solution at sol0[4*(mgp[L]+node)+c] of the bound context. */
+int pardisomat(int am, long mcmax, long mcmax2, long pos0cur1, int tid, long *un1, double *ui1);
+/* parallel/move3mg.c:3855 (prototype parallel/head3mg.c:350-377) -- am
+ * smoother sweeps on level mgi, then fre0 = fre1 - A sol0 on [mcmin,mcmax] */
+int gausmat4(int am, long mcmax, long mcmin, int mgi);
+/* parallel/move3mg.c:3966 -- residual restriction to level mgi */
+void vprestrict(int mgi);
+/* parallel/move3mg.c:3535 -- correction prolongation from level mgi */
+void vpprolong(int mgi, int ynreset);
+/* parallel/move3mg.c:3387 -- nu/ro restriction to level mgi */
+void ronurestrict(int mgi);
+/* parallel/move3mg.c:1791 -- full Stokes solve of the bound context */
+void vpiterate(int m0, int f0);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,270 @@
+"""ctypes binding of libpardiso2_amd.so (the C ABI of include/pardiso2_amd.h).
+
+The library must exist: there is no CPU fallback.  Build it with
+``python pardiso2_amd/build.py`` (or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libpardiso2_amd.so")
+
+ERRORS = {0: "ok", -1: "invalid argument", -2: "device out of memory", -3: "HIP error",
+          -4: "coarsest matrix singular", -5: "not converged", -6: "NaN/Inf",
+          -7: "call order", -8: "no bound context"}
+
+EXPORTS = [
+    "stk_create", "stk_destroy", "stk_level_info", "stk_set_knobs", "stk_default_opts", "stk_bc_line",
+    "stk_set_bc_tables", "stk_get_bc_tables", "stk_set_fields", "stk_markers_to_nodes", "stk_get_fields",
+    "stk_prepare", "stk_set_solution", "stk_get_solution", "stk_solve", "stk_relres",
+    "stk_pressure_gauge", "stk_vec_set", "stk_vec_get", "stk_apply", "stk_smooth", "stk_coarse_solve",
+    "stk_bind", "matclear", "gausmat", "gausmat2", "pardisomat", "gausmat4", "vprestrict", "vpprolong",
+    "ronurestrict", "vpiterate",
+]
+
+
+class Knobs(C.Structure):
+    _fields_ = [("p0koef", C.c_double), ("p1koef", C.c_double), ("p2koef", C.c_double),
+                ("v0koef", C.c_double), ("v1koef", C.c_double), ("nubeg", C.c_double),
+                ("nuend", C.c_double), ("viscmod", C.c_int), ("gx", C.c_double), ("gy", C.c_double),
+                ("gz", C.c_double)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("restart", C.c_int), ("pre", C.c_int * 12),
+                ("post", C.c_int * 12), ("tile", C.c_int * 3), ("omega", C.c_double),
+                ("tol", C.c_double), ("maxit", C.c_int), ("verbose", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("status", C.c_int), ("relres", C.c_double),
+                ("relres0", C.c_double), ("seconds", C.c_double), ("blocks", C.c_double * 3),
+                ("hist", C.c_double * 256)]
+
+
+_lib = None
+
+
+class StkError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise StkError(f"{LIBPATH} is missing: build it with python pardiso2_amd/build.py")
+        L = C.CDLL(LIBPATH)
+        vp, dp, fp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_int)
+        L.stk_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp, C.c_int,
+                                 C.c_int, C.c_int]
+        L.stk_destroy.argtypes = [vp]
+        L.stk_destroy.restype = None
+        L.stk_level_info.argtypes = [vp, C.c_int, ip]
+        L.stk_set_knobs.argtypes = [vp, C.POINTER(Knobs)]
+        L.stk_default_opts.argtypes = [C.POINTER(Opts), C.c_int]
+        L.stk_default_opts.restype = None
+        L.stk_bc_line.argtypes = [vp] + [C.c_int] * 7 + [C.c_float, C.c_float] + [C.c_int] * 3
+        L.stk_set_bc_tables.argtypes = [vp, ip, C.c_int, fp, ip]
+        L.stk_get_bc_tables.argtypes = [vp, ip, ip, fp, ip]
+        L.stk_set_fields.argtypes = [vp] + [fp] * 6
+        L.stk_markers_to_nodes.argtypes = [vp, C.c_long] + [fp] * 5 + [C.c_double]
+        L.stk_get_fields.argtypes = [vp] + [fp] * 7
+        L.stk_prepare.argtypes = [vp]
+        L.stk_set_solution.argtypes = [vp, dp]
+        L.stk_get_solution.argtypes = [vp, dp]
+        L.stk_solve.argtypes = [vp, C.POINTER(Opts), C.POINTER(Stats)]
+        L.stk_relres.argtypes = [vp, dp, dp]
+        L.stk_pressure_gauge.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.stk_vec_set.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.stk_vec_get.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.stk_apply.argtypes = [vp, C.c_int, dp, dp]
+        L.stk_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.stk_coarse_solve.argtypes = [vp]
+        L.stk_bind.argtypes = [vp]
+        L.matclear.argtypes = [C.c_int, dp]
+        L.matclear.restype = None
+        L.gausmat.argtypes = [C.c_int, dp, dp]
+        L.gausmat.restype = None
+        L.gausmat2.argtypes = [C.c_int, C.c_long, C.c_long, C.POINTER(C.c_long), dp]
+        L.pardisomat.argtypes = [C.c_int, C.c_long, C.c_long, C.c_long, C.c_int, C.POINTER(C.c_long), dp]
+        L.gausmat4.argtypes = [C.c_int, C.c_long, C.c_long, C.c_int]
+        for nm in ("vprestrict", "ronurestrict"):
+            getattr(L, nm).argtypes = [C.c_int]
+            getattr(L, nm).restype = None
+        L.vpprolong.argtypes = [C.c_int, C.c_int]
+        L.vpprolong.restype = None
+        L.vpiterate.argtypes = [C.c_int, C.c_int]
+        L.vpiterate.restype = None
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        raise StkError(f"{what}: {ERRORS.get(rc, rc)} ({rc})")
+    return rc
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def default_opts(levels: int, **kw) -> Opts:
+    o = Opts()
+    lib().stk_default_opts(C.byref(o), levels)
+    for k, v in kw.items():
+        if k in ("pre", "post"):
+            arr = getattr(o, k)
+            for i, x in enumerate(v):
+                arr[i] = int(x)
+        elif k == "tile":
+            for i, x in enumerate(v):
+                o.tile[i] = int(x)
+        else:
+            setattr(o, k, v)
+    return o
+
+
+class Stokes:
+    """One Stokes problem resident on the GPU (a stk_ctx)."""
+
+    def __init__(self, gx, gy, gz, levels: int):
+        self.levels = levels
+        self._g = [np.ascontiguousarray(a, dtype=np.float64) for a in (gx, gy, gz)]
+        h = C.c_void_p()
+        check(lib().stk_create(C.byref(h), len(gx), len(gy), len(gz), levels, _dp(self._g[0]),
+                               _dp(self._g[1]), _dp(self._g[2]), 0, 0, 0), "stk_create")
+        self.h = h
+        self.info = []
+        for l in range(levels + 1):
+            b = np.zeros(6, dtype=np.int32)
+            check(lib().stk_level_info(self.h, l, _ip(b)), "stk_level_info")
+            self.info.append(tuple(int(v) for v in b))
+        self.ntot = self.info[-1][5] + self.info[-1][4]
+        self.n0 = self.info[0][4]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().stk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def knobs(self, p0koef=0.1, p1koef=0.0, p2koef=0.1, v0koef=1.0, v1koef=1.0, nubeg=1.0, nuend=1e4,
+              viscmod=0, gravity=(0.0, 0.0, 0.0)):
+        k = Knobs(p0koef, p1koef, p2koef, v0koef, v1koef, nubeg, nuend, viscmod, *gravity)
+        check(lib().stk_set_knobs(self.h, C.byref(k)), "stk_set_knobs")
+
+    def bc_lines(self, lines):
+        for ln in lines:
+            check(lib().stk_bc_line(self.h, *[int(v) for v in ln[:7]], float(ln[7]), float(ln[8]),
+                                    *[int(v) for v in ln[9:12]]), "stk_bc_line")
+
+    def set_fields(self, nu, ro, nxx, nxy, nxz, nyz):
+        a = [np.ascontiguousarray(v[: self.n0], dtype=np.float32) for v in (nu, ro, nxx, nxy, nxz, nyz)]
+        check(lib().stk_set_fields(self.h, *[_fp(v) for v in a]), "stk_set_fields")
+
+    def markers_to_nodes(self, mx, my, mz, mnu, mro, nukoef=0.0):
+        a = [np.ascontiguousarray(v, dtype=np.float32) for v in (mx, my, mz, mnu, mro)]
+        check(lib().stk_markers_to_nodes(self.h, len(a[0]), *[_fp(v) for v in a], nukoef),
+              "stk_markers_to_nodes")
+
+    def fields(self):
+        nu = np.zeros(self.ntot, np.float32)
+        ro = np.zeros(self.ntot, np.float32)
+        nd = np.zeros(self.ntot, np.float32)
+        f = [np.zeros(self.n0, np.float32) for _ in range(4)]
+        check(lib().stk_get_fields(self.h, _fp(nu), _fp(ro), _fp(nd), *[_fp(v) for v in f]), "stk_get_fields")
+        return dict(nu=nu, ro=ro, nd=nd, nxx=f[0], nxy=f[1], nxz=f[2], nyz=f[3])
+
+    def prepare(self):
+        check(lib().stk_prepare(self.h), "stk_prepare")
+
+    def set_solution(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        check(lib().stk_set_solution(self.h, _dp(x)), "stk_set_solution")
+
+    def solution(self):
+        x = np.zeros(4 * self.n0)
+        check(lib().stk_get_solution(self.h, _dp(x)), "stk_get_solution")
+        return x
+
+    def solve(self, opts: Opts = None, raise_on_fail=False, **kw):
+        o = opts if opts is not None else default_opts(self.levels, **kw)
+        s = Stats()
+        rc = lib().stk_solve(self.h, C.byref(o), C.byref(s))
+        if raise_on_fail:
+            check(rc, "stk_solve")
+        elif rc not in (0, -5, -6):
+            check(rc, "stk_solve")
+        return s
+
+    def relres(self):
+        r = C.c_double()
+        b = np.zeros(3)
+        check(lib().stk_relres(self.h, C.byref(r), _dp(b)), "stk_relres")
+        return r.value, b
+
+    def pressure_gauge(self, px, py, pz, pinit=0.0):
+        check(lib().stk_pressure_gauge(self.h, px, py, pz, pinit), "stk_pressure_gauge")
+
+    def vec_set(self, which, level, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.size == 4 * self.info[level][4]
+        check(lib().stk_vec_set(self.h, which, level, _dp(x)), "stk_vec_set")
+
+    def vec_get(self, which, level):
+        x = np.zeros(4 * self.info[level][4])
+        check(lib().stk_vec_get(self.h, which, level, _dp(x)), "stk_vec_get")
+        return x
+
+    def apply(self, level, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        check(lib().stk_apply(self.h, level, _dp(x), _dp(y)), "stk_apply")
+        return y
+
+    def smooth(self, level, sweeps, tile=(32, 4, 4)):
+        check(lib().stk_smooth(self.h, level, sweeps, *tile), "stk_smooth")
+
+    def coarse_solve(self):
+        check(lib().stk_coarse_solve(self.h), "stk_coarse_solve")
+
+    def bind(self):
+        check(lib().stk_bind(self.h), "stk_bind")
+
+
+SOL0, FRE1, FRE0 = 0, 1, 2
+
+
+def from_problem(pb, markers=True, knobs=None):
+    """Stokes context for a setups.Problem: hierarchy, knobs, boundary lines,
+    node fields from the markers (on the device), prepared operators."""
+    s = Stokes(pb.gx, pb.gy, pb.gz, pb.levels)
+    k = dict(pb.knobs)
+    if knobs:
+        k.update(knobs)
+    s.knobs(k["p0koef"], k["p1koef"], k["p2koef"], k["v0koef"], k["v1koef"], k["nubeg"], k["nuend"],
+            k["viscmod"], pb.gravity)
+    s.bc_lines(pb.bc)
+    if markers:
+        s.markers_to_nodes(pb.mx, pb.my, pb.mz, pb.mnu, pb.mro)
+    s.prepare()
+    return s

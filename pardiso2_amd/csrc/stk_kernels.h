@@ -1,0 +1,698 @@
+// stk_kernels.h -- HIP kernels of the Stokes multigrid library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "stk_rows.h"
+
+namespace stk {
+
+template <typename T> struct V4;
+template <> struct V4<float> { using t = float4; };
+template <> struct V4<double> { using t = double4; };
+
+__device__ __forceinline__ void decode(long n, int X, int Y, int XY, int &m1, int &m2, int &m3) {
+  m3 = (int)(n / XY);
+  int rem = (int)(n - (long)m3 * XY);
+  m1 = rem / Y;
+  m2 = rem - m1 * Y;
+}
+
+// ---------------------------------------------------------------------------
+// Stencil viscosities of one level (nueff of the stress/continuity stencils,
+// parallel/move3mg.c:36-50, 432-447, 567-580, 708-721, 1609-1627), clamped
+// to [nubeg, nuend].  Level 0 reads nxx/nxy/nxz/nyz, coarse levels average
+// node nu by viscmod; the grad-div cell value always averages nu.
+// Sums of FP32 node values are formed in FP32 as in the reference.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double clampnu(double e, double nb, double ne) {
+  if (e > ne) e = ne;
+  if (e < nb) e = nb;
+  return e;
+}
+
+__device__ __forceinline__ double mean8(const float *nu, long v0, int Y, int XY, int vm) {
+  long v[8];
+  v[0] = v0; v[1] = v0 + 1; v[2] = v0 + Y; v[3] = v[2] + 1;
+  v[4] = v0 + XY; v[5] = v[4] + 1; v[6] = v[4] + Y; v[7] = v[6] + 1;
+  if (vm == 1) {
+    double s = log((double)nu[v[0]]);
+    for (int k = 1; k < 8; k++) s = s + log((double)nu[v[k]]);
+    return exp(s / 8.0);
+  }
+  if (vm == 2) {
+    double s = 1.0 / nu[v[0]];
+    for (int k = 1; k < 8; k++) s = s + 1.0 / nu[v[k]];
+    return 1.0 / (s / 8.0);
+  }
+  float s = nu[v[0]];
+  for (int k = 1; k < 8; k++) s = s + nu[v[k]];
+  return s / 8.0;
+}
+
+__device__ __forceinline__ double mean2(const float *nu, long a, long b, int vm) {
+  if (vm == 1) return exp((log((double)nu[a]) + log((double)nu[b])) / 2.0);
+  if (vm == 2) return 1.0 / ((1.0 / nu[a] + 1.0 / nu[b]) / 2.0);
+  return (nu[a] + nu[b]) / 2.0;
+}
+
+__global__ void k_visc_prep(int X, int Y, int Z, int level0, const float *__restrict__ nu,
+                            const float *__restrict__ nxx, const float *__restrict__ nxy,
+                            const float *__restrict__ nxz, const float *__restrict__ nyz,
+                            double nubeg, double nuend, int vm, double *e64c, double *e64xy,
+                            double *e64xz, double *e64yz, double *e64d, float *e32c, float *e32xy,
+                            float *e32xz, float *e32yz, float *e32d) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XY = X * Y;
+  if (n >= (long)XY * Z) return;
+  int m1, m2, m3;
+  decode(n, X, Y, XY, m1, m2, m3);
+  double ec = nubeg, ed = nubeg, exy = nubeg, exz = nubeg, eyz = nubeg;
+  if (m1 >= 1 && m2 >= 1 && m3 >= 1) {
+    long v0 = n - XY - Y - 1;
+    ed = clampnu(mean8(nu, v0, Y, XY, vm), nubeg, nuend);
+    ec = level0 ? clampnu((double)nxx[n], nubeg, nuend) : ed;
+  }
+  if (level0) {
+    exy = clampnu((double)nxy[n], nubeg, nuend);
+    exz = clampnu((double)nxz[n], nubeg, nuend);
+    eyz = clampnu((double)nyz[n], nubeg, nuend);
+  } else {
+    if (m3 < Z - 1) exy = clampnu(mean2(nu, n, n + XY, vm), nubeg, nuend);
+    if (m2 < Y - 1) exz = clampnu(mean2(nu, n, n + 1, vm), nubeg, nuend);
+    if (m1 < X - 1) eyz = clampnu(mean2(nu, n, n + Y, vm), nubeg, nuend);
+  }
+  e64c[n] = ec; e64xy[n] = exy; e64xz[n] = exz; e64yz[n] = eyz; e64d[n] = ed;
+  e32c[n] = (float)ec; e32xy[n] = (float)exy; e32xz[n] = (float)exz; e32yz[n] = (float)eyz;
+  e32d[n] = (float)ed;
+}
+
+// ---------------------------------------------------------------------------
+// Finest-level right-hand side: gravity terms of the momentum rows
+// (move3mg.c:848, 1061, 1281), zero for continuity, the constant of
+// boundary equations (move3mg.c:1710-1712).
+// ---------------------------------------------------------------------------
+__global__ void k_rhs(int X, int Y, int Z, const unsigned char *__restrict__ flags,
+                      const float *__restrict__ ro, double gx, double gy, double gz, double *b) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XY = X * Y;
+  if (n >= (long)XY * Z) return;
+  unsigned char f = flags[n];
+  double *bn = b + 4 * n;
+  if (kind_of(f, 0) != KIND_BC) bn[0] = 0.0;
+  if (kind_of(f, 1) == KIND_EQ) bn[1] = -gx * (ro[n] + ro[n + 1] + ro[n + XY] + ro[n + XY + 1]) / 4.0;
+  else if (kind_of(f, 1) == KIND_NONE) bn[1] = 0.0;
+  if (kind_of(f, 2) == KIND_EQ) bn[2] = -gy * (ro[n] + ro[n + Y] + ro[n + XY] + ro[n + XY + Y]) / 4.0;
+  else if (kind_of(f, 2) == KIND_NONE) bn[2] = 0.0;
+  if (kind_of(f, 3) == KIND_EQ) bn[3] = -gz * (ro[n] + ro[n + 1] + ro[n + Y] + ro[n + Y + 1]) / 4.0;
+  else if (kind_of(f, 3) == KIND_NONE) bn[3] = 0.0;
+}
+
+__global__ void k_bc_rhs(int nbc, const int *__restrict__ r, const double *__restrict__ c, double *b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbc) b[r[i]] = c[i];
+}
+
+// ---------------------------------------------------------------------------
+// Row operator on equation rows: RES -> y = b - A x (gausmat4's residual
+// order, move3mg.c:3937-3958), otherwise y = A x.  withp=0 leaves pressure
+// rows at zero (velocity block).  Boundary rows are done by k_bc_rowop.
+// ---------------------------------------------------------------------------
+template <typename T, bool RES>
+__global__ void k_rowop(Geo<T> g, const unsigned char *__restrict__ flags, const T *__restrict__ x,
+                        const T *__restrict__ b, T *__restrict__ y, int withp) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long n = u >> 2;
+  if (n >= (long)g.XY * g.Z) return;
+  int c = (int)(u & 3);
+  int k = kind_of(flags[n], c);
+  if (k != KIND_EQ || (!withp && c == 0)) {
+    if (k == KIND_NONE || (!withp && c == 0)) y[u] = T(0);
+    return;
+  }
+  int m1, m2, m3;
+  decode(n, g.X, g.Y, g.XY, m1, m2, m3);
+  T s[18];
+  int ns = row_coef<T>(g, c, m1, m2, m3, s);
+  T acc = RES ? b[u] : T(0);
+#pragma unroll
+  for (int q = 1; q <= 17; q++) {
+    if (q > ns) break;
+    long uq = 4 * (n + (long)kSlot[c][q][2] * g.XY + (long)kSlot[c][q][0] * g.Y + kSlot[c][q][1]) + kSlot[c][q][3];
+    if (RES) acc -= s[q] * x[uq];
+    else acc += s[q] * x[uq];
+  }
+  y[u] = acc;
+}
+
+// boundary rows  x_r - k x_j = b_r  (bonderr, move3mg.c:1682-1729)
+template <typename T, bool RES>
+__global__ void k_bc_rowop(int nbc, const int *__restrict__ br, const int *__restrict__ bj,
+                           const double *__restrict__ negk, const T *__restrict__ x,
+                           const T *__restrict__ b, T *__restrict__ y, int withp) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbc) return;
+  int r = br[i];
+  if (!withp && (r & 3) == 0) { y[r] = T(0); return; }
+  T acc;
+  if (RES) {
+    acc = b[r];
+    acc -= T(1) * x[r];
+    if (bj[i] >= 0) acc -= (T)negk[i] * x[bj[i]];
+  } else {
+    acc = T(0);
+    acc += T(1) * x[r];
+    if (bj[i] >= 0) acc += (T)negk[i] * x[bj[i]];
+  }
+  y[r] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Smoother: one sweep of the reference's row updates (gausmat4,
+// move3mg.c:3893-3923) in tile-coloured order.  A block owns a ty*tx*tz node
+// tile, stages it with a one-node halo in LDS from xold, and updates in
+// phases vx(red,black), vy(red,black), vz(red,black)[, P], red/black by
+// (m1+m2+m3)&1; halo values stay those of xold (block-Jacobi across tiles),
+// so the sweep is deterministic.  Boundary rows are skipped here and done
+// by k_bc_update afterwards.  The tile is written to xnew.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *__restrict__ flags,
+                                                const T *__restrict__ xold, T *__restrict__ xnew,
+                                                const T *__restrict__ b, int ty, int tx, int tz,
+                                                int ntx, int nty, int withp, T v0koef, T p0koef) {
+  extern __shared__ unsigned char smem_raw[];
+  T *sx = reinterpret_cast<T *>(smem_raw);
+  const int hy = ty + 2, hx = tx + 2, hz = tz + 2;
+  const int bt = blockIdx.x;
+  const int by = bt % nty, bx = (bt / nty) % ntx, bz = bt / (nty * ntx);
+  const int y0 = by * ty, x0 = bx * tx, z0 = bz * tz;
+  const int nh = hy * hx * hz;
+  // stage tile + halo
+  for (int i = threadIdx.x; i < nh; i += blockDim.x) {
+    int ly = i % hy, lx = (i / hy) % hx, lz = i / (hy * hx);
+    int m2 = y0 + ly - 1, m1 = x0 + lx - 1, m3 = z0 + lz - 1;
+    T v0 = T(0), v1 = T(0), v2 = T(0), v3 = T(0);
+    if (m1 >= 0 && m1 < g.X && m2 >= 0 && m2 < g.Y && m3 >= 0 && m3 < g.Z) {
+      const T *p = xold + 4 * g.nid(m1, m2, m3);
+      v0 = p[0]; v1 = p[1]; v2 = p[2]; v3 = p[3];
+    }
+    sx[4 * i + 0] = v0; sx[4 * i + 1] = v1; sx[4 * i + 2] = v2; sx[4 * i + 3] = v3;
+  }
+  __syncthreads();
+  const int nphase = withp ? 7 : 6;
+  const int hty = (ty + 1) / 2;
+  for (int ph = 0; ph < nphase; ph++) {
+    const int c = (ph < 6) ? 1 + ph / 2 : 0;
+    const int colour = ph & 1;
+    const int ncand = (c == 0) ? ty * tx * tz : hty * tx * tz;
+    for (int i = threadIdx.x; i < ncand; i += blockDim.x) {
+      int ly, lx, lz;
+      if (c == 0) {
+        ly = i % ty; lx = (i / ty) % tx; lz = i / (ty * tx);
+      } else {
+        int kk = i % hty; lx = (i / hty) % tx; lz = i / (hty * tx);
+        int par = (colour - (x0 + lx) - (z0 + lz) - y0) & 1;
+        ly = 2 * kk + par;
+        if (ly >= ty) continue;
+      }
+      int m1 = x0 + lx, m2 = y0 + ly, m3 = z0 + lz;
+      if (m1 >= g.X || m2 >= g.Y || m3 >= g.Z) continue;
+      long n = g.nid(m1, m2, m3);
+      if (kind_of(flags[n], c) != KIND_EQ) continue;
+      T s[18];
+      int ns = row_coef<T>(g, c, m1, m2, m3, s);
+      T ival = -b[4 * n + c];
+      const int base = ((lz + 1) * hx + (lx + 1)) * hy + (ly + 1);
+#pragma unroll
+      for (int q = 1; q <= 17; q++) {
+        if (q > ns) break;
+        int li = base + (kSlot[c][q][2] * hx + kSlot[c][q][0]) * hy + kSlot[c][q][1];
+        ival += s[q] * sx[4 * li + kSlot[c][q][3]];
+      }
+      if (c == 0) sx[4 * base] -= ival * g.ec[n] * p0koef;
+      else sx[4 * base + c] -= ival / s[1] * v0koef;
+    }
+    __syncthreads();
+  }
+  // write the tile interior
+  const int nt = ty * tx * tz;
+  for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+    int ly = i % ty, lx = (i / ty) % tx, lz = i / (ty * tx);
+    int m1 = x0 + lx, m2 = y0 + ly, m3 = z0 + lz;
+    if (m1 >= g.X || m2 >= g.Y || m3 >= g.Z) continue;
+    int li = ((lz + 1) * hx + (lx + 1)) * hy + (ly + 1);
+    T *p = xnew + 4 * g.nid(m1, m2, m3);
+    p[0] = sx[4 * li + 0]; p[1] = sx[4 * li + 1]; p[2] = sx[4 * li + 2]; p[3] = sx[4 * li + 3];
+  }
+}
+
+// boundary-row update x_r -= (x_r - k x_j - b_r), one dependency depth
+// mode 0: all boundary rows, 1: velocity rows only, 2: pressure rows only
+template <typename T>
+__global__ void k_bc_update(int beg, int end, const int *__restrict__ br, const int *__restrict__ bj,
+                            const double *__restrict__ negk, const T *__restrict__ b, T *x, int mode) {
+  int i = beg + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= end) return;
+  int r = br[i];
+  if ((mode == 1 && (r & 3) == 0) || (mode == 2 && (r & 3) != 0)) return;
+  T ival = -b[r];
+  ival += T(1) * x[r];
+  if (bj[i] >= 0) ival += (T)negk[i] * x[bj[i]];
+  x[r] -= ival;
+}
+
+// ---------------------------------------------------------------------------
+// Transfers in gather form.  Per axis and staggering kind a fine index i has
+// a coarse base lo(i) and weights for lo and lo+1 (the clamped index rules of
+// vprestrict/vpprolong, move3mg.c:3563-3576, 4015-4028, ...).  For the
+// restriction each coarse index keeps the list of (fine index, weight) pairs
+// in increasing fine order, so the sums run in the reference's
+// single-thread scatter order (fine nodes z, x, y).
+// ---------------------------------------------------------------------------
+struct Ptrs16 {
+  const double *p[16];
+};
+
+struct AxisTab {
+  const int *lo;        // [nf]
+  const double *w64;    // [2*nf] weights for lo, lo+1
+  const float *w32;
+  const int *cbeg;      // [nc+1]
+  const int *cfi;       // fine index of each entry
+  const double *cw64;   // weight of each entry
+  const float *cw32;
+};
+
+struct CompTab {
+  AxisTab ax[3];  // x, y, z
+};
+
+template <typename T> __device__ __forceinline__ T tw(const double *a, const float *b, long i);
+template <> __device__ __forceinline__ double tw<double>(const double *a, const float *, long i) { return a[i]; }
+template <> __device__ __forceinline__ float tw<float>(const double *, const float *b, long i) { return b[i]; }
+
+// vprestrict: coarse b from fine residual r (parallel/move3mg.c:3966-4354)
+template <typename T>
+__global__ void k_restrict(int Xf, int Yf, int Zf, int Xc, int Yc, int Zc, CompTab t0, CompTab t1,
+                           CompTab t2, CompTab t3, const unsigned char *__restrict__ ff,
+                           const unsigned char *__restrict__ fc, const T *__restrict__ rf,
+                           const float *__restrict__ ndf, const float *__restrict__ ndc, T *bc, T *xc,
+                           int withp) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long n = u >> 2;
+  const int XYc = Xc * Yc, XYf = Xf * Yf;
+  if (n >= (long)XYc * Zc) return;
+  int c = (int)(u & 3);
+  xc[u] = T(0);
+  int kc = kind_of(fc[n], c);
+  if (kc != KIND_EQ || (!withp && c == 0)) { bc[u] = T(0); return; }
+  int cx, cy, cz;
+  decode(n, Xc, Yc, XYc, cx, cy, cz);
+  const CompTab &t = (c == 0) ? t0 : (c == 1) ? t1 : (c == 2) ? t2 : t3;
+  T acc = T(0), ws = T(0);
+  for (int a = t.ax[2].cbeg[cz]; a < t.ax[2].cbeg[cz + 1]; a++) {
+    int k = t.ax[2].cfi[a];
+    T wz = tw<T>(t.ax[2].cw64, t.ax[2].cw32, a);
+    for (int bq = t.ax[0].cbeg[cx]; bq < t.ax[0].cbeg[cx + 1]; bq++) {
+      int i = t.ax[0].cfi[bq];
+      T wx = tw<T>(t.ax[0].cw64, t.ax[0].cw32, bq);
+      for (int e = t.ax[1].cbeg[cy]; e < t.ax[1].cbeg[cy + 1]; e++) {
+        int j = t.ax[1].cfi[e];
+        long nf = (long)k * XYf + (long)i * Yf + j;
+        if (kind_of(ff[nf], c) != KIND_EQ) continue;
+        T wy = tw<T>(t.ax[1].cw64, t.ax[1].cw32, e);
+        T swt = wx * wy * wz;
+        T iv;
+        if (c == 0) iv = rf[4 * nf] * (T)ndf[nf] / (T)ndc[n] * swt;
+        else iv = rf[4 * nf + c] * swt;
+        acc += iv;
+        ws += swt;
+      }
+    }
+  }
+  bc[u] = (ws != T(0)) ? acc / ws : acc;
+}
+
+// vpprolong: fine x += sum coarse x * w * koef (parallel/move3mg.c:3535-3850)
+template <typename T>
+__global__ void k_prolong(int Xf, int Yf, int Zf, int Xc, int Yc, int Zc, CompTab t0, CompTab t1,
+                          CompTab t2, CompTab t3, const unsigned char *__restrict__ ff,
+                          const T *__restrict__ xc, T *xf, T pk, T vk, int withp, int ynreset) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long n = u >> 2;
+  const int XYc = Xc * Yc, XYf = Xf * Yf;
+  if (n >= (long)XYf * Zf) return;
+  int c = (int)(u & 3);
+  if (kind_of(ff[n], c) == KIND_NONE || (!withp && c == 0)) return;
+  int m1, m2, m3;
+  decode(n, Xf, Yf, XYf, m1, m2, m3);
+  const CompTab &t = (c == 0) ? t0 : (c == 1) ? t1 : (c == 2) ? t2 : t3;
+  const T k = (c == 0) ? pk : vk;
+  int lx = t.ax[0].lo[m1], ly = t.ax[1].lo[m2], lz = t.ax[2].lo[m3];
+  T v = ynreset ? T(0) : xf[u];
+  for (int dz = 0; dz < 2; dz++) {
+    T wz = tw<T>(t.ax[2].w64, t.ax[2].w32, 2 * m3 + dz);
+    for (int dx = 0; dx < 2; dx++) {
+      T wx = tw<T>(t.ax[0].w64, t.ax[0].w32, 2 * m1 + dx);
+      for (int dy = 0; dy < 2; dy++) {
+        T wy = tw<T>(t.ax[1].w64, t.ax[1].w32, 2 * m2 + dy);
+        long nc = (long)(lz + dz) * XYc + (long)(lx + dx) * Yc + (ly + dy);
+        T swt = wx * wy * wz;
+        v += xc[4 * nc + c] * swt * k;
+      }
+    }
+  }
+  xf[u] = v;
+}
+
+// ronurestrict: coarse nu/ro from fine nodes (parallel/move3mg.c:3387-3528),
+// FP32 accumulators and FP64 weights as in the reference
+__global__ void k_ronurestrict(int Xf, int Yf, int Xc, int Yc, int Zc, CompTab t,
+                               const float *__restrict__ nuf, const float *__restrict__ rof,
+                               float *nuc, float *roc, int vm) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XYc = Xc * Yc, XYf = Xf * Yf;
+  if (n >= (long)XYc * Zc) return;
+  int cx, cy, cz;
+  decode(n, Xc, Yc, XYc, cx, cy, cz);
+  float anu = 0.f, aro = 0.f;
+  double ws = 0.0;
+  for (int a = t.ax[2].cbeg[cz]; a < t.ax[2].cbeg[cz + 1]; a++) {
+    int k = t.ax[2].cfi[a];
+    double wz = t.ax[2].cw64[a];
+    for (int bq = t.ax[0].cbeg[cx]; bq < t.ax[0].cbeg[cx + 1]; bq++) {
+      int i = t.ax[0].cfi[bq];
+      double wx = t.ax[0].cw64[bq];
+      for (int e = t.ax[1].cbeg[cy]; e < t.ax[1].cbeg[cy + 1]; e++) {
+        int j = t.ax[1].cfi[e];
+        double wy = t.ax[1].cw64[e];
+        long nf = (long)k * XYf + (long)i * Yf + j;
+        double swt = wx * wy * wz, iv = 0.0;
+        if (vm == 0) iv = nuf[nf] * swt;
+        if (vm == 1) iv = log((double)nuf[nf]) * swt;
+        if (vm == 2) iv = 1.0 / nuf[nf] * swt;
+        anu = (float)((double)anu + iv);
+        iv = rof[nf] * swt;
+        aro = (float)((double)aro + iv);
+        ws += swt;
+      }
+    }
+  }
+  float nv = (float)((double)anu / ws);
+  if (vm == 1) nv = (float)exp((double)nv);
+  if (vm == 2) nv = (float)(1.0 / nv);
+  nuc[n] = nv;
+  roc[n] = (float)((double)aro / ws);
+}
+
+// ---------------------------------------------------------------------------
+// Coarsest level: dense operator rows (velocity block or all rows), then a
+// Gauss-Jordan inverse with partial pivoting (first maximal |a| wins).
+// ---------------------------------------------------------------------------
+__global__ void k_dense_rows(Geo<double> g, const unsigned char *__restrict__ flags, int nbc,
+                             const int *__restrict__ br, const int *__restrict__ bj,
+                             const double *__restrict__ negk, double *A, int n4, int withp) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u >= n4) return;
+  long n = u >> 2;
+  int c = (int)(u & 3);
+  double *row = A + u * (long)n4;
+  int k = kind_of(flags[n], c);
+  if (k == KIND_NONE || (!withp && c == 0)) { row[u] += 1.0; return; }
+  if (k == KIND_BC) {
+    for (int i = 0; i < nbc; i++)
+      if (br[i] == u) {
+        row[u] += 1.0;
+        if (bj[i] >= 0) row[bj[i]] += negk[i];
+        break;
+      }
+    return;
+  }
+  int m1, m2, m3;
+  decode(n, g.X, g.Y, g.XY, m1, m2, m3);
+  double s[18];
+  int ns = row_coef<double>(g, c, m1, m2, m3, s);
+  for (int q = 1; q <= ns; q++) {
+    long uq = 4 * (n + (long)kSlot[c][q][2] * g.XY + (long)kSlot[c][q][0] * g.Y + kSlot[c][q][1]) + kSlot[c][q][3];
+    row[uq] += s[q];
+  }
+}
+
+// pivot search, swap, normalise of step k on [A | B]; saves column k factors
+__global__ void __launch_bounds__(1024) k_gj_pivot(double *A, double *B, int n, int k, double *fac,
+                                                   int *sing) {
+  __shared__ double bv[1024];
+  __shared__ int bi[1024];
+  double best = -1.0;
+  int bidx = n;
+  for (int i = k + threadIdx.x; i < n; i += blockDim.x) {
+    double v = fabs(A[(long)i * n + k]);
+    if (v > best || (v == best && i < bidx)) { best = v; bidx = i; }
+  }
+  bv[threadIdx.x] = best;
+  bi[threadIdx.x] = bidx;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      double v = bv[threadIdx.x + s];
+      int j = bi[threadIdx.x + s];
+      if (v > bv[threadIdx.x] || (v == bv[threadIdx.x] && j < bi[threadIdx.x])) {
+        bv[threadIdx.x] = v;
+        bi[threadIdx.x] = j;
+      }
+    }
+    __syncthreads();
+  }
+  int p = bi[0];
+  if (bv[0] == 0.0) {
+    if (threadIdx.x == 0) *sing = k + 1;
+    return;
+  }
+  if (p != k)
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      double t = A[(long)k * n + j]; A[(long)k * n + j] = A[(long)p * n + j]; A[(long)p * n + j] = t;
+      t = B[(long)k * n + j]; B[(long)k * n + j] = B[(long)p * n + j]; B[(long)p * n + j] = t;
+    }
+  __syncthreads();
+  double piv = A[(long)k * n + k];
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    A[(long)k * n + j] /= piv;
+    B[(long)k * n + j] /= piv;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) fac[i] = (i == k) ? 0.0 : A[(long)i * n + k];
+}
+
+__global__ void k_gj_elim(double *A, double *B, int n, int k, const double *__restrict__ fac) {
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int i = (int)(t / n), j = (int)(t % n);
+  if (i >= n) return;
+  double f = fac[i];
+  if (f == 0.0) return;
+  if (j > k) A[(long)i * n + j] -= f * A[(long)k * n + j];
+  B[(long)i * n + j] -= f * B[(long)k * n + j];
+  if (j == k) A[(long)i * n + k] = 0.0;
+}
+
+// y = M x for a dense row-major n x n matrix, one wavefront per row
+template <typename TM, typename T>
+__global__ void k_gemv(const TM *__restrict__ M, const T *__restrict__ x, T *y, int n) {
+  int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const TM *mr = M + (long)row * n;
+  double acc = 0.0;
+  for (int j = lane; j < n; j += 64) acc += (double)mr[j] * (double)x[j];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) y[row] = (T)acc;
+}
+
+// ---------------------------------------------------------------------------
+// Preconditioner pieces and vector kernels of the FP64 outer iteration.
+// ---------------------------------------------------------------------------
+// z_p = omega * eta_cell * r_p on continuity rows, zero elsewhere
+__global__ void k_pscale(long n4, const unsigned char *__restrict__ flags, const double *__restrict__ ec,
+                         const double *__restrict__ r, double *z, double omega) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u >= n4) return;
+  long n = u >> 2;
+  int c = (int)(u & 3);
+  z[u] = (c == 0 && kind_of(flags[n], 0) == KIND_EQ) ? omega * ec[n] * r[u] : 0.0;
+}
+
+// velocity right-hand side r_v - G z_p of the preconditioner (momentum rows
+// have P only in slots 16/17), boundary rows take r; stored in type T
+template <typename T>
+__global__ void k_vrhs(Geo<double> g, const unsigned char *__restrict__ flags, const double *__restrict__ r,
+                       const double *__restrict__ zp, T *bv) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long n = u >> 2;
+  if (n >= (long)g.XY * g.Z) return;
+  int c = (int)(u & 3);
+  int k = kind_of(flags[n], c);
+  if (c == 0 || k == KIND_NONE) { bv[u] = T(0); return; }
+  if (k == KIND_BC) { bv[u] = (T)r[u]; return; }
+  int m1, m2, m3;
+  decode(n, g.X, g.Y, g.XY, m1, m2, m3);
+  double kf = (c == 1) ? 2.0 * g.iBx[m1] : (c == 2) ? 2.0 * g.iBy[m2] : 2.0 * g.iBz[m3];
+  double acc = r[u];
+  for (int q = 16; q <= 17; q++) {
+    long uq = 4 * (n + (long)kSlot[c][q][2] * g.XY + (long)kSlot[c][q][0] * g.Y + kSlot[c][q][1]);
+    acc -= (q == 16 ? kf : -kf) * zp[uq];
+  }
+  bv[u] = (T)acc;
+}
+
+// z_v = x (velocity unknowns), pressure entries of z kept
+template <typename T>
+__global__ void k_combine(long n4, const T *__restrict__ xv, double *z) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u >= n4 || (u & 3) == 0) return;
+  z[u] = (double)xv[u];
+}
+
+// partial sums of w2*a*b over blocks, for k vectors b_i = B[i*n4 ..]
+__global__ void __launch_bounds__(256) k_multidot(long n4, const double *__restrict__ w2,
+                                                  const double *__restrict__ a, Ptrs16 Bs,
+                                                  int k, double *partial) {
+  __shared__ double red[256];
+  double acc[16];
+  for (int i = 0; i < 16; i++) acc[i] = 0.0;
+  for (long u = blockIdx.x * (long)blockDim.x + threadIdx.x; u < n4; u += (long)gridDim.x * blockDim.x) {
+    double wa = w2[u] * a[u];
+    for (int i = 0; i < k; i++) acc[i] += wa * Bs.p[i][u];
+  }
+  for (int i = 0; i < k; i++) {
+    red[threadIdx.x] = acc[i];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(long)i * gridDim.x + blockIdx.x] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_reduce_rows(const double *__restrict__ partial, int nb, int k, double *out) {
+  __shared__ double red[256];
+  int i = blockIdx.x;
+  if (i >= k) return;
+  double acc = 0.0;
+  for (int j = threadIdx.x; j < nb; j += blockDim.x) acc += partial[(long)i * nb + j];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[i] = red[0];
+}
+
+// a -= sum_i c_i A_i ; b -= sum_i c_i B_i
+__global__ void k_multiaxpy(long n4, int k, const double *__restrict__ coef, Ptrs16 As, Ptrs16 Bs,
+                            double *a, double *b) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u >= n4) return;
+  double va = a[u], vb = b[u];
+  for (int i = 0; i < k; i++) {
+    double c = coef[i];
+    va -= c * As.p[i][u];
+    vb -= c * Bs.p[i][u];
+  }
+  a[u] = va;
+  b[u] = vb;
+}
+
+__global__ void k_scale2(long n4, const double *__restrict__ sp, double *a, double *b) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u >= n4) return;
+  double s = *sp;
+  a[u] *= s;
+  b[u] *= s;
+}
+
+// GCR scalars from <q,q>, <q,r>: s = 1/sqrt(qq), alpha = qr * s
+__global__ void k_gcr_scalars(const double *qqr, double *s_alpha) {
+  double s = 1.0 / sqrt(qqr[0]);
+  s_alpha[0] = s;
+  s_alpha[1] = qqr[1] * s;
+}
+
+// x += alpha z ; r -= alpha q
+__global__ void k_update(long n4, const double *__restrict__ alpha, const double *__restrict__ z,
+                         const double *__restrict__ q, double *x, double *r) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u >= n4) return;
+  double a = *alpha;
+  x[u] += a * z[u];
+  r[u] -= a * q[u];
+}
+
+__global__ void k_sub(long n4, const double *__restrict__ a, const double *__restrict__ b, double *c) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u < n4) c[u] = a[u] - b[u];
+}
+
+// residual blocks: sum w2 r^2 over momentum / continuity / boundary rows and w2 b^2
+__global__ void __launch_bounds__(256) k_blocks(long n4, const unsigned char *__restrict__ flags,
+                                                const double *__restrict__ w2, const double *__restrict__ r,
+                                                const double *__restrict__ b, double *partial) {
+  __shared__ double red[256];
+  double acc[4] = {0, 0, 0, 0};
+  for (long u = blockIdx.x * (long)blockDim.x + threadIdx.x; u < n4; u += (long)gridDim.x * blockDim.x) {
+    int c = (int)(u & 3);
+    int k = kind_of(flags[u >> 2], c);
+    double rr = w2[u] * r[u] * r[u];
+    if (k == KIND_BC) acc[2] += rr;
+    else if (k == KIND_EQ) { if (c == 0) acc[1] += rr; else acc[0] += rr; }
+    acc[3] += w2[u] * b[u] * b[u];
+  }
+  for (int i = 0; i < 4; i++) {
+    red[threadIdx.x] = acc[i];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(long)i * gridDim.x + blockIdx.x] = red[0];
+    __syncthreads();
+  }
+}
+
+// W^2 of the relative-residual norm: 1 on momentum and boundary rows,
+// (eta_cell / h_cell)^2 on continuity rows, 0 on ghost unknowns
+__global__ void k_weights(int X, int Y, int Z, const unsigned char *__restrict__ flags,
+                          const double *__restrict__ ec, const double *__restrict__ gx,
+                          const double *__restrict__ gy, const double *__restrict__ gz, double *w2) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XY = X * Y;
+  if (n >= (long)XY * Z) return;
+  int m1, m2, m3;
+  decode(n, X, Y, XY, m1, m2, m3);
+  unsigned char f = flags[n];
+  for (int c = 0; c < 4; c++) {
+    int k = kind_of(f, c);
+    double w = (k == KIND_NONE) ? 0.0 : 1.0;
+    if (c == 0 && k == KIND_EQ) {
+      double h = ((gx[m1] - gx[m1 - 1]) + (gy[m2] - gy[m2 - 1]) + (gz[m3] - gz[m3 - 1])) / 3.0;
+      w = ec[n] / h;
+    }
+    w2[4 * n + c] = w * w;
+  }
+}
+
+template <typename TS, typename TD>
+__global__ void k_convert(long n, const TS *__restrict__ a, TD *b) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u < n) b[u] = (TD)a[u];
+}
+
+__global__ void k_pshift(long nn, const unsigned char *__restrict__ flags, double *x, const double *ref,
+                         double pinit) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  if (kind_of(flags[n], 0) != KIND_NONE) x[4 * n] += pinit - *ref;
+}
+
+}  // namespace stk

@@ -1,0 +1,1243 @@
+// stk_lib.hip -- host side of the MI355X Stokes multigrid library: level
+// hierarchy, boundary tables, transfer tables, multigrid cycle, FP64 outer
+// iteration and the C ABI declared in include/pardiso2_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../../include/pardiso2_amd.h"
+#include "stk_kernels.h"
+#include "stk_markers.h"
+
+#include <hipcub/hipcub.hpp>
+
+using namespace stk;
+
+#define MAXLV 12
+
+namespace {
+
+#define HIPCHK(x)                                  \
+  do {                                             \
+    hipError_t e_ = (x);                           \
+    if (e_ != hipSuccess) {                        \
+      fprintf(stderr, "pardiso2_amd: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+      return STK_EHIP;                             \
+    }                                              \
+  } while (0)
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) return STK_ENOMEM;
+  return STK_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+inline unsigned blocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// one axis of a transfer: fine->coarse index rule and gather lists
+struct AxisHost {
+  std::vector<int> lo;
+  std::vector<double> w;  // 2 per fine index
+  std::vector<int> cbeg, cfi;
+  std::vector<double> cw;
+};
+
+struct AxisDev {
+  int *lo = nullptr, *cbeg = nullptr, *cfi = nullptr;
+  double *w64 = nullptr, *cw64 = nullptr;
+  float *w32 = nullptr, *cw32 = nullptr;
+  AxisTab tab() const { return AxisTab{lo, w64, w32, cbeg, cfi, cw64, cw32}; }
+  void release() {
+    dfree(lo); dfree(cbeg); dfree(cfi); dfree(w64); dfree(cw64); dfree(w32); dfree(cw32);
+  }
+};
+
+struct Level {
+  int X = 0, Y = 0, Z = 0, XY = 0;
+  long n = 0, off = 0;
+  std::vector<double> g[3];
+  // device geometry
+  double *iA64[3] = {}, *iB64[3] = {}, *g64[3] = {};
+  float *iA32[3] = {}, *iB32[3] = {};
+  double *e64[5] = {};
+  float *e32[5] = {};
+  unsigned char *flags = nullptr;
+  std::vector<unsigned char> hflags;
+  // boundary rows, sorted by dependency depth
+  int nbc = 0;
+  std::vector<int> bcoff;
+  int *bc_r = nullptr, *bc_j = nullptr;
+  double *bc_negk = nullptr, *bc_c = nullptr;
+  std::vector<int> hbc_r, hbc_j;
+  std::vector<double> hbc_negk, hbc_c;
+  // vectors
+  float *x32[2] = {}, *b32 = nullptr, *r32 = nullptr;
+  double *x64[2] = {}, *b64 = nullptr, *r64 = nullptr;
+  int cur32 = 0, cur64 = 0;
+  // reference arrays (compat): sol0, fre1, fre0
+  double *ref[3] = {};
+  // transfer from level-1 (kinds N, C, P, R per axis)
+  AxisDev ax[4][3];
+  template <typename T> Geo<T> geo(double p2) const;
+};
+
+template <>
+Geo<double> Level::geo<double>(double p2) const {
+  Geo<double> g;
+  g.X = X; g.Y = Y; g.Z = Z; g.XY = XY;
+  g.iAx = iA64[0]; g.iBx = iB64[0]; g.iAy = iA64[1]; g.iBy = iB64[1]; g.iAz = iA64[2]; g.iBz = iB64[2];
+  g.ec = e64[0]; g.exy = e64[1]; g.exz = e64[2]; g.eyz = e64[3]; g.ed = e64[4];
+  g.p2 = p2;
+  return g;
+}
+template <>
+Geo<float> Level::geo<float>(double p2) const {
+  Geo<float> g;
+  g.X = X; g.Y = Y; g.Z = Z; g.XY = XY;
+  g.iAx = iA32[0]; g.iBx = iB32[0]; g.iAy = iA32[1]; g.iBy = iB32[1]; g.iAz = iA32[2]; g.iBz = iB32[2];
+  g.ec = e32[0]; g.exy = e32[1]; g.exz = e32[2]; g.eyz = e32[3]; g.ed = e32[4];
+  g.p2 = (float)p2;
+  return g;
+}
+
+}  // namespace
+
+struct stk_ctx {
+  int L = 0;
+  int per[3] = {0, 0, 0};
+  Level lv[MAXLV];
+  long ntot = 0;
+  stk_knobs kn{};
+  // boundary tables in the reference layout
+  std::vector<int> bondm;  // 4*ntot
+  std::vector<float> bondv1;  // 2*(nbond+1)
+  std::vector<int> bondn1;    // nbond+1
+  // fields
+  std::vector<float> hnu, hro, hnxx, hnxy, hnxz, hnyz;  // level 0 (nu/ro over all levels after prepare)
+  float *nu = nullptr, *ro = nullptr;                  // all levels
+  float *nxx = nullptr, *nxy = nullptr, *nxz = nullptr, *nyz = nullptr;
+  float *nd = nullptr;                                 // all levels (cell viscosity)
+  bool fields_set = false, prepared = false;
+  // coarsest inverse
+  int nc = 0;
+  double *ainv64 = nullptr, *ainvP64 = nullptr;  // velocity block / all rows
+  float *ainv32 = nullptr;
+  // outer iteration vectors (finest level)
+  double *X = nullptr, *B = nullptr, *R = nullptr, *W2 = nullptr, *T1 = nullptr;
+  std::vector<double *> Zs, Qs;
+  double *red = nullptr, *scal = nullptr;
+  int nred = 1024;
+  hipStream_t st = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // compat row store (gausmat2 / pardisomat)
+  std::vector<double> hval1, hfre1;
+  std::vector<int> hlin0, hnum0, hpos0;
+  std::vector<double> ca;           // coarsest CSR values (dense accumulation)
+  std::vector<double> cb;
+};
+
+namespace {
+
+stk_ctx *g_bound = nullptr;
+
+// ---------------------------------------------------------------------------
+// index rules (parallel/move3mg.c:3563-3576 etc.) and weights
+// kind 0 'N' grid-line staggering, 1 'C' cell centre, 2 'P' pressure cell,
+// 3 'R' node restriction of ronurestrict (move3mg.c:3433-3444)
+// ---------------------------------------------------------------------------
+bool axis_rule(int kind, int m, int nf, int nc, const std::vector<double> &gf,
+               const std::vector<double> &gc, int &lo, double &w0, double &w1) {
+  if (kind == 1 && m > nf - 2) return false;
+  if (kind == 2 && m < 1) return false;
+  lo = (m - 2) / 2 + 2;
+  if (m < 3) lo = m;
+  if (m > nf - 4) lo = nc - (nf - m);
+  double d, dd0, dd1;
+  if (kind == 0 || kind == 3) {
+    if (lo > nc - 2) lo = nc - 2;
+    d = gc[lo + 1] - gc[lo];
+    dd0 = (gc[lo] - gf[m]) / d;
+    dd1 = (gc[lo + 1] - gf[m]) / d;
+    if (kind == 3) {
+      w0 = 1.0 - std::fabs(dd0);
+      w1 = 1.0 - std::fabs(dd1);
+      return true;
+    }
+  } else if (kind == 1) {
+    if (lo > nc - 3) lo = nc - 3;
+    if ((gf[m] + gf[m + 1]) / 2.0 < (gc[lo] + gc[lo + 1]) / 2.0 && lo > 0) lo -= 1;
+    d = (gc[lo + 2] - gc[lo]) / 2.0;
+    dd0 = (gc[lo] + gc[lo + 1] - gf[m] - gf[m + 1]) / 2.0 / d;
+    dd1 = (gc[lo + 1] + gc[lo + 2] - gf[m] - gf[m + 1]) / 2.0 / d;
+  } else {
+    if (lo > nc - 2) lo = nc - 2;
+    if ((gf[m] + gf[m - 1]) / 2.0 < (gc[lo] + gc[lo - 1]) / 2.0 && lo > 1) lo -= 1;
+    d = (gc[lo + 1] - gc[lo - 1]) / 2.0;
+    dd0 = (gc[lo] + gc[lo - 1] - gf[m] - gf[m - 1]) / 2.0 / d;
+    dd1 = (gc[lo + 1] + gc[lo] - gf[m] - gf[m - 1]) / 2.0 / d;
+  }
+  w0 = 1.0 - (-dd0);
+  w1 = 1.0 - dd1;
+  return true;
+}
+
+AxisHost build_axis(int kind, int nf, int nc, const std::vector<double> &gf, const std::vector<double> &gc) {
+  AxisHost a;
+  a.lo.assign(nf, 0);
+  a.w.assign(2 * nf, 0.0);
+  std::vector<char> ok(nf, 0);
+  for (int m = 0; m < nf; m++) {
+    int lo;
+    double w0, w1;
+    if (!axis_rule(kind, m, nf, nc, gf, gc, lo, w0, w1)) continue;
+    ok[m] = 1;
+    a.lo[m] = lo;
+    a.w[2 * m] = w0;
+    a.w[2 * m + 1] = w1;
+  }
+  a.cbeg.assign(nc + 1, 0);
+  for (int C = 0; C < nc; C++) {
+    a.cbeg[C] = (int)a.cfi.size();
+    for (int m = 0; m < nf; m++) {
+      if (!ok[m]) continue;
+      if (a.lo[m] == C) { a.cfi.push_back(m); a.cw.push_back(a.w[2 * m]); }
+      else if (a.lo[m] + 1 == C) { a.cfi.push_back(m); a.cw.push_back(a.w[2 * m + 1]); }
+    }
+  }
+  a.cbeg[nc] = (int)a.cfi.size();
+  return a;
+}
+
+template <typename T>
+int upload(T **d, const std::vector<T> &h) {
+  int rc = dalloc(d, h.size());
+  if (rc) return rc;
+  if (!h.empty() && hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+    return STK_EHIP;
+  return STK_OK;
+}
+
+int upload_axis(AxisDev &d, const AxisHost &h) {
+  d.release();
+  std::vector<float> w32(h.w.begin(), h.w.end()), cw32(h.cw.begin(), h.cw.end());
+  int rc = 0;
+  rc |= upload(&d.lo, h.lo);
+  rc |= upload(&d.w64, h.w);
+  rc |= upload(&d.w32, w32);
+  rc |= upload(&d.cbeg, h.cbeg);
+  rc |= upload(&d.cfi, h.cfi);
+  rc |= upload(&d.cw64, h.cw);
+  rc |= upload(&d.cw32, cw32);
+  return rc ? STK_ENOMEM : STK_OK;
+}
+
+int lvl_index(int m, int nfine, int nlev, long mgs) {
+  int r = (m - 2) / (int)mgs + 2;
+  if (m < 3) r = m;
+  if (m > nfine - 4) r = nlev - (nfine - m);
+  return r;
+}
+
+bool comp_exists(const Level &L, int m1, int m2, int m3, int c) {
+  if (c == 0) return m1 && m2 && m3;
+  if (c == 1) return m2 < L.Y - 1 && m3 < L.Z - 1;
+  if (c == 2) return m1 < L.X - 1 && m3 < L.Z - 1;
+  return m1 < L.X - 1 && m2 < L.Y - 1;
+}
+
+// equation rows must keep their stencil inside the grid (the reference has
+// no guard either: boundary equations cover those unknowns)
+bool eq_inside(const Level &L, int m1, int m2, int m3, int c) {
+  if (c == 0) return true;
+  int lo1 = 1, lo2 = 1, lo3 = 1;
+  int h1 = L.X - 3, h2 = L.Y - 3, h3 = L.Z - 3;
+  if (c == 1) h1 = L.X - 2;
+  if (c == 2) h2 = L.Y - 2;
+  if (c == 3) h3 = L.Z - 2;
+  return m1 >= lo1 && m1 <= h1 && m2 >= lo2 && m2 <= h2 && m3 >= lo3 && m3 <= h3;
+}
+
+template <typename T> T *lvx(Level &L);
+template <> float *lvx<float>(Level &L) { return L.x32[L.cur32]; }
+template <> double *lvx<double>(Level &L) { return L.x64[L.cur64]; }
+template <typename T> T *lvxo(Level &L);
+template <> float *lvxo<float>(Level &L) { return L.x32[1 - L.cur32]; }
+template <> double *lvxo<double>(Level &L) { return L.x64[1 - L.cur64]; }
+template <typename T> void lvflip(Level &L);
+template <> void lvflip<float>(Level &L) { L.cur32 ^= 1; }
+template <> void lvflip<double>(Level &L) { L.cur64 ^= 1; }
+template <typename T> T *lvb(Level &L);
+template <> float *lvb<float>(Level &L) { return L.b32; }
+template <> double *lvb<double>(Level &L) { return L.b64; }
+template <typename T> T *lvr(Level &L);
+template <> float *lvr<float>(Level &L) { return L.r32; }
+template <> double *lvr<double>(Level &L) { return L.r64; }
+
+int ensure_vectors(stk_ctx *c, bool want32, bool want64) {
+  for (int l = 0; l <= c->L; l++) {
+    Level &L = c->lv[l];
+    size_t n4 = 4 * (size_t)L.n;
+    if (want32 && !L.b32) {
+      if (dalloc(&L.x32[0], n4) || dalloc(&L.x32[1], n4) || dalloc(&L.b32, n4) || dalloc(&L.r32, n4))
+        return STK_ENOMEM;
+      hipMemset(L.x32[0], 0, n4 * 4); hipMemset(L.x32[1], 0, n4 * 4);
+      hipMemset(L.b32, 0, n4 * 4); hipMemset(L.r32, 0, n4 * 4);
+    }
+    if (want64 && !L.b64) {
+      if (dalloc(&L.x64[0], n4) || dalloc(&L.x64[1], n4) || dalloc(&L.b64, n4) || dalloc(&L.r64, n4))
+        return STK_ENOMEM;
+      hipMemset(L.x64[0], 0, n4 * 8); hipMemset(L.x64[1], 0, n4 * 8);
+      hipMemset(L.b64, 0, n4 * 8); hipMemset(L.r64, 0, n4 * 8);
+    }
+  }
+  return STK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// multigrid building blocks
+// ---------------------------------------------------------------------------
+const int kTileDefault[3] = {32, 4, 4};
+
+template <typename T>
+void bc_passes(stk_ctx *c, Level &L, const T *b, T *x, int mode) {
+  for (size_t d = 0; d + 1 < L.bcoff.size(); d++) {
+    int beg = L.bcoff[d], end = L.bcoff[d + 1];
+    if (end > beg)
+      hipLaunchKernelGGL(k_bc_update<T>, dim3(blocks(end - beg, 256)), dim3(256), 0, c->st, beg, end, L.bc_r,
+                         L.bc_j, L.bc_negk, b, x, mode);
+  }
+}
+
+template <typename T>
+void smooth(stk_ctx *c, int l, int sweeps, const int *tile, int withp) {
+  Level &L = c->lv[l];
+  int ty = tile[0], tx = tile[1], tz = tile[2];
+  int nty = (L.Y + ty - 1) / ty, ntx = (L.X + tx - 1) / tx, ntz = (L.Z + tz - 1) / tz;
+  size_t lds = (size_t)(ty + 2) * (tx + 2) * (tz + 2) * 4 * sizeof(T);
+  Geo<T> g = L.geo<T>(c->kn.p2koef);
+  for (int s = 0; s < sweeps; s++) {
+    hipLaunchKernelGGL(k_smooth<T>, dim3(nty * ntx * ntz), dim3(256), lds, c->st, g, L.flags, lvx<T>(L),
+                       lvxo<T>(L), lvb<T>(L), ty, tx, tz, ntx, nty, withp, (T)c->kn.v0koef, (T)c->kn.p0koef);
+    lvflip<T>(L);
+    bc_passes<T>(c, L, lvb<T>(L), lvx<T>(L), withp ? 0 : 1);
+  }
+}
+
+template <typename T, bool RES>
+void rowop(stk_ctx *c, int l, const T *x, const T *b, T *y, int withp) {
+  Level &L = c->lv[l];
+  Geo<T> g = L.geo<T>(c->kn.p2koef);
+  hipLaunchKernelGGL((k_rowop<T, RES>), dim3(blocks(4 * L.n, 256)), dim3(256), 0, c->st, g, L.flags, x, b, y, withp);
+  if (L.nbc)
+    hipLaunchKernelGGL((k_bc_rowop<T, RES>), dim3(blocks(L.nbc, 256)), dim3(256), 0, c->st, L.nbc, L.bc_r,
+                       L.bc_j, L.bc_negk, x, b, y, withp);
+}
+
+CompTab comptab(const Level &L, int comp) {
+  static const int K[4][3] = {{2, 2, 2}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}};
+  CompTab t;
+  for (int a = 0; a < 3; a++) t.ax[a] = L.ax[K[comp][a]][a].tab();
+  return t;
+}
+
+template <typename T>
+void restrict_to(stk_ctx *c, int l, const T *rf, T *bcoarse, T *xcoarse, int withp) {
+  Level &F = c->lv[l - 1], &C = c->lv[l];
+  hipLaunchKernelGGL(k_restrict<T>, dim3(blocks(4 * C.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, C.X, C.Y, C.Z,
+                     comptab(C, 0), comptab(C, 1), comptab(C, 2), comptab(C, 3), F.flags, C.flags, rf,
+                     c->nd + F.off, c->nd + C.off, bcoarse, xcoarse, withp);
+}
+
+template <typename T>
+void prolong_from(stk_ctx *c, int l, const T *xcoarse, T *xf, int withp, int ynreset) {
+  Level &F = c->lv[l - 1], &C = c->lv[l];
+  hipLaunchKernelGGL(k_prolong<T>, dim3(blocks(4 * F.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, C.X, C.Y, C.Z,
+                     comptab(C, 0), comptab(C, 1), comptab(C, 2), comptab(C, 3), F.flags, xcoarse, xf,
+                     (T)c->kn.p1koef, (T)c->kn.v1koef, withp, ynreset);
+}
+
+template <typename T> const T *ainv(stk_ctx *c);
+template <> const float *ainv<float>(stk_ctx *c) { return c->ainv32; }
+template <> const double *ainv<double>(stk_ctx *c) { return c->ainv64; }
+
+// velocity-block V-cycle; level-0 right-hand side already in lvb<T>(lv[0])
+template <typename T>
+void vcycle_v(stk_ctx *c, const stk_solver_opts *o, const int *tile) {
+  const int L = c->L;
+  for (int l = 0; l < L; l++) {
+    Level &V = c->lv[l];
+    if (l == 0) hipMemsetAsync(lvx<T>(V), 0, 4 * V.n * sizeof(T), c->st);
+    smooth<T>(c, l, o->pre[l], tile, 0);
+    rowop<T, true>(c, l, lvx<T>(V), lvb<T>(V), lvr<T>(V), 0);
+    restrict_to<T>(c, l + 1, lvr<T>(V), lvb<T>(c->lv[l + 1]), lvx<T>(c->lv[l + 1]), 0);
+  }
+  Level &C = c->lv[L];
+  int n4 = (int)(4 * C.n);
+  hipLaunchKernelGGL((k_gemv<T, T>), dim3(blocks(n4, 4)), dim3(256), 0, c->st, ainv<T>(c), lvb<T>(C), lvx<T>(C), n4);
+  for (int l = L - 1; l >= 0; l--) {
+    prolong_from<T>(c, l + 1, lvx<T>(c->lv[l + 1]), lvx<T>(c->lv[l]), 0, 0);
+    smooth<T>(c, l, o->post[l], tile, 0);
+  }
+}
+
+// z = M^{-1} r : pressure by eta scaling, velocity by one V-cycle
+template <typename T>
+void precond(stk_ctx *c, const stk_solver_opts *o, const double *r, double *z, const int *tile) {
+  Level &F = c->lv[0];
+  long n4 = 4 * F.n;
+  hipLaunchKernelGGL(k_pscale, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, F.flags, F.e64[0], r, z, o->omega);
+  bc_passes<double>(c, F, r, z, 2);
+  Geo<double> g = F.geo<double>(c->kn.p2koef);
+  hipLaunchKernelGGL(k_vrhs<T>, dim3(blocks(n4, 256)), dim3(256), 0, c->st, g, F.flags, r, z, lvb<T>(F));
+  vcycle_v<T>(c, o, tile);
+  hipLaunchKernelGGL(k_combine<T>, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, lvx<T>(F), z);
+}
+
+// weighted dots: out[i] = <a, bs[i]>_W for k vectors (device result)
+void multidot(stk_ctx *c, const double *a, const std::vector<const double *> &bs, double *out) {
+  long n4 = 4 * c->lv[0].n;
+  int k = (int)bs.size();
+  for (int i0 = 0; i0 < k; i0 += 16) {
+    int kk = std::min(16, k - i0);
+    Ptrs16 P;
+    for (int i = 0; i < 16; i++) P.p[i] = (i < kk) ? bs[i0 + i] : nullptr;
+    hipLaunchKernelGGL(k_multidot, dim3(c->nred), dim3(256), 0, c->st, n4, c->W2, a, P, kk, c->red);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(kk), dim3(256), 0, c->st, c->red, c->nred, kk, out + i0);
+  }
+}
+
+void multiaxpy(stk_ctx *c, int k, const double *coef, double *a, double *b) {
+  long n4 = 4 * c->lv[0].n;
+  for (int i0 = 0; i0 < k; i0 += 16) {
+    int kk = std::min(16, k - i0);
+    Ptrs16 A, B;
+    for (int i = 0; i < 16; i++) {
+      A.p[i] = (i < kk) ? c->Qs[i0 + i] : nullptr;
+      B.p[i] = (i < kk) ? c->Zs[i0 + i] : nullptr;
+    }
+    hipLaunchKernelGGL(k_multiaxpy, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, kk, coef + i0, A, B, a, b);
+  }
+}
+
+int blocks_norms(stk_ctx *c, const double *r, const double *b, double out4[4]) {
+  long n4 = 4 * c->lv[0].n;
+  hipLaunchKernelGGL(k_blocks, dim3(c->nred), dim3(256), 0, c->st, n4, c->lv[0].flags, c->W2, r, b, c->red);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(4), dim3(256), 0, c->st, c->red, c->nred, 4, c->scal + 8);
+  HIPCHK(hipMemcpyAsync(out4, c->scal + 8, 4 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return STK_OK;
+}
+
+int true_residual(stk_ctx *c) {
+  rowop<double, true>(c, 0, c->X, c->B, c->R, 1);
+  return STK_OK;
+}
+
+int ensure_outer(stk_ctx *c, int m) {
+  long n4 = 4 * c->lv[0].n;
+  if (!c->X) {
+    if (dalloc(&c->X, n4) || dalloc(&c->B, n4) || dalloc(&c->R, n4) || dalloc(&c->W2, n4) || dalloc(&c->T1, n4) ||
+        dalloc(&c->red, (size_t)c->nred * 16) || dalloc(&c->scal, 128))
+      return STK_ENOMEM;
+    hipMemset(c->X, 0, n4 * 8);
+  }
+  while ((int)c->Zs.size() < m) {
+    double *z, *q;
+    if (dalloc(&z, n4) || dalloc(&q, n4)) return STK_ENOMEM;
+    c->Zs.push_back(z);
+    c->Qs.push_back(q);
+  }
+  return STK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// coarsest-level inverse (Gauss-Jordan with partial pivoting, FP64)
+// ---------------------------------------------------------------------------
+int coarse_inverse(stk_ctx *c, int withp, double **out) {
+  Level &C = c->lv[c->L];
+  int n = (int)(4 * C.n);
+  double *A, *Bm, *fac;
+  int *sing;
+  if (dalloc(&A, (size_t)n * n) || dalloc(&Bm, (size_t)n * n) || dalloc(&fac, n) || dalloc(&sing, 1))
+    return STK_ENOMEM;
+  HIPCHK(hipMemsetAsync(A, 0, (size_t)n * n * 8, c->st));
+  HIPCHK(hipMemsetAsync(sing, 0, 4, c->st));
+  std::vector<double> I((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) I[(size_t)i * n + i] = 1.0;
+  HIPCHK(hipMemcpyAsync(Bm, I.data(), (size_t)n * n * 8, hipMemcpyHostToDevice, c->st));
+  Geo<double> g = C.geo<double>(c->kn.p2koef);
+  hipLaunchKernelGGL(k_dense_rows, dim3(blocks(n, 128)), dim3(128), 0, c->st, g, C.flags, C.nbc, C.bc_r, C.bc_j,
+                     C.bc_negk, A, n, withp);
+  for (int k = 0; k < n; k++) {
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, c->st, A, Bm, n, k, fac, sing);
+    hipLaunchKernelGGL(k_gj_elim, dim3(blocks((long)n * n, 256)), dim3(256), 0, c->st, A, Bm, n, k, fac);
+  }
+  int hs = 0;
+  HIPCHK(hipMemcpyAsync(&hs, sing, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  dfree(A); dfree(fac); dfree(sing);
+  if (hs) { dfree(Bm); return STK_ESINGULAR; }
+  dfree(*out);
+  *out = Bm;
+  return STK_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *gx, const double *gy,
+               const double *gz, int xper, int yper, int zper) {
+  if (!out || levels < 0 || levels >= MAXLV || nx < 5 || ny < 5 || nz < 5 || !gx || !gy || !gz)
+    return STK_EINVAL;
+  if (xper || yper || zper) return STK_EINVAL;  // periodic boxes: not supported by the device path
+  stk_ctx *c = new stk_ctx();
+  c->L = levels;
+  const double *g0[3] = {gx, gy, gz};
+  int nf[3] = {nx, ny, nz};
+  long off = 0;
+  long mgs = 1;
+  int dims[3] = {nx, ny, nz};
+  for (int l = 0; l <= levels; l++) {
+    Level &L = c->lv[l];
+    if (l > 0) {
+      mgs *= 2;
+      for (int a = 0; a < 3; a++) dims[a] = (dims[a] - 5) / 2 + 5;
+    }
+    L.X = dims[0]; L.Y = dims[1]; L.Z = dims[2]; L.XY = L.X * L.Y;
+    L.n = (long)L.XY * L.Z;
+    L.off = off;
+    off += L.n;
+    // coarse lines keep 3 outer lines per side (load3mg.c:2078-2190)
+    for (int a = 0; a < 3; a++) {
+      int nl = dims[a];
+      std::vector<double> &gg = L.g[a];
+      gg.assign(nl, 0.0);
+      gg[0] = g0[a][0]; gg[1] = g0[a][1]; gg[2] = g0[a][2];
+      gg[nl - 1] = g0[a][nf[a] - 1]; gg[nl - 2] = g0[a][nf[a] - 2]; gg[nl - 3] = g0[a][nf[a] - 3];
+      for (int n2 = 3; n2 < nl - 3; n2++) gg[n2] = g0[a][(n2 - 2) * mgs + 2];
+    }
+  }
+  c->ntot = off;
+  c->bondm.assign(4 * off, 0);
+  c->bondv1.assign(2, 0.f);
+  c->bondn1.assign(1, 0);
+  c->kn = stk_knobs{0.10, 0.0, 0.10, 1.0, 1.0, 1.0, 1e4, 0, 0.0, 0.0, 0.0};
+  if (hipStreamCreate(&c->st) != hipSuccess) { delete c; return STK_EHIP; }
+  hipEventCreate(&c->ev0);
+  hipEventCreate(&c->ev1);
+  // per-level geometry tables
+  for (int l = 0; l <= levels; l++) {
+    Level &L = c->lv[l];
+    for (int a = 0; a < 3; a++) {
+      int nl = (int)L.g[a].size();
+      std::vector<double> iA(nl, 0.0), iB(nl, 0.0);
+      for (int i = 1; i < nl; i++) iA[i] = 1.0 / (L.g[a][i] - L.g[a][i - 1]);
+      for (int i = 1; i + 1 < nl; i++) iB[i] = 1.0 / (L.g[a][i + 1] - L.g[a][i - 1]);
+      std::vector<float> fA(iA.begin(), iA.end()), fB(iB.begin(), iB.end());
+      if (upload(&L.iA64[a], iA) || upload(&L.iB64[a], iB) || upload(&L.iA32[a], fA) || upload(&L.iB32[a], fB) ||
+          upload(&L.g64[a], L.g[a])) {
+        stk_destroy(c);
+        return STK_ENOMEM;
+      }
+    }
+    for (int k = 0; k < 5; k++)
+      if (dalloc(&L.e64[k], L.n) || dalloc(&L.e32[k], L.n)) { stk_destroy(c); return STK_ENOMEM; }
+    if (dalloc(&L.flags, L.n)) { stk_destroy(c); return STK_ENOMEM; }
+  }
+  if (dalloc(&c->nu, off) || dalloc(&c->ro, off) || dalloc(&c->nd, off) || dalloc(&c->nxx, c->lv[0].n) ||
+      dalloc(&c->nxy, c->lv[0].n) || dalloc(&c->nxz, c->lv[0].n) || dalloc(&c->nyz, c->lv[0].n)) {
+    stk_destroy(c);
+    return STK_ENOMEM;
+  }
+  hipMemset(c->nu, 0, off * 4); hipMemset(c->ro, 0, off * 4); hipMemset(c->nd, 0, off * 4);
+  // transfer tables
+  for (int l = 1; l <= levels; l++) {
+    Level &F = c->lv[l - 1], &C = c->lv[l];
+    int nfa[3] = {F.X, F.Y, F.Z}, nca[3] = {C.X, C.Y, C.Z};
+    for (int kind = 0; kind < 4; kind++)
+      for (int a = 0; a < 3; a++) {
+        AxisHost h = build_axis(kind, nfa[a], nca[a], F.g[a], C.g[a]);
+        if (upload_axis(C.ax[kind][a], h)) { stk_destroy(c); return STK_ENOMEM; }
+      }
+  }
+  *out = c;
+  return STK_OK;
+}
+
+void stk_destroy(stk_ctx *c) {
+  if (!c) return;
+  if (g_bound == c) g_bound = nullptr;
+  hipDeviceSynchronize();
+  for (int l = 0; l <= c->L; l++) {
+    Level &L = c->lv[l];
+    for (int a = 0; a < 3; a++) { dfree(L.iA64[a]); dfree(L.iB64[a]); dfree(L.iA32[a]); dfree(L.iB32[a]); dfree(L.g64[a]); }
+    for (int k = 0; k < 5; k++) { dfree(L.e64[k]); dfree(L.e32[k]); }
+    dfree(L.flags);
+    dfree(L.bc_r); dfree(L.bc_j); dfree(L.bc_negk); dfree(L.bc_c);
+    for (int k = 0; k < 2; k++) { dfree(L.x32[k]); dfree(L.x64[k]); }
+    dfree(L.b32); dfree(L.r32); dfree(L.b64); dfree(L.r64);
+    for (int k = 0; k < 3; k++) dfree(L.ref[k]);
+    for (int k = 0; k < 4; k++)
+      for (int a = 0; a < 3; a++) L.ax[k][a].release();
+  }
+  dfree(c->nu); dfree(c->ro); dfree(c->nd); dfree(c->nxx); dfree(c->nxy); dfree(c->nxz); dfree(c->nyz);
+  dfree(c->ainv64); dfree(c->ainvP64); dfree(c->ainv32);
+  dfree(c->X); dfree(c->B); dfree(c->R); dfree(c->W2); dfree(c->T1); dfree(c->red); dfree(c->scal);
+  for (auto p : c->Zs) dfree(p);
+  for (auto p : c->Qs) dfree(p);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+}
+
+int stk_level_info(const stk_ctx *c, int l, int out6[6]) {
+  if (!c || l < 0 || l > c->L) return STK_EINVAL;
+  const Level &L = c->lv[l];
+  out6[0] = L.X; out6[1] = L.Y; out6[2] = L.Z; out6[3] = L.XY; out6[4] = (int)L.n; out6[5] = (int)L.off;
+  return STK_OK;
+}
+
+int stk_set_knobs(stk_ctx *c, const stk_knobs *k) {
+  if (!c || !k || k->viscmod < 0 || k->viscmod > 2 || !(k->nubeg > 0) || !(k->nuend >= k->nubeg)) return STK_EINVAL;
+  c->kn = *k;
+  c->prepared = false;
+  return STK_OK;
+}
+
+void stk_default_opts(stk_solver_opts *o, int levels) {
+  memset(o, 0, sizeof(*o));
+  o->precision = 0;
+  o->restart = 32;
+  int pre[12] = {2, 4, 8, 16, 32, 64, 64, 64, 64, 64, 64, 64};
+  for (int l = 0; l < 12; l++) { o->pre[l] = pre[l]; o->post[l] = pre[l]; }
+  (void)levels;
+  o->tile[0] = kTileDefault[0]; o->tile[1] = kTileDefault[1]; o->tile[2] = kTileDefault[2];
+  o->omega = 1.0;
+  o->tol = 1e-8;
+  o->maxit = 200;
+}
+
+// boundary grammar line projected on every level (parallel/in3mg.c:299-375)
+int stk_bc_line(stk_ctx *c, int var, int m10, int m11, int m20, int m21, int m30, int m31, float cnst,
+                float koef, int dm1, int dm2, int dm3) {
+  if (!c || var < 0 || var > 3) return STK_EINVAL;
+  const Level &F = c->lv[0];
+  long mgs = 1;
+  for (int l = 0; l <= c->L; l++, mgs *= 2) {
+    const Level &L = c->lv[l];
+    int a1 = lvl_index(m10, F.X, L.X, mgs), b1 = lvl_index(m11, F.X, L.X, mgs);
+    int a2 = lvl_index(m20, F.Y, L.Y, mgs), b2 = lvl_index(m21, F.Y, L.Y, mgs);
+    int a3 = lvl_index(m30, F.Z, L.Z, mgs), b3 = lvl_index(m31, F.Z, L.Z, mgs);
+    for (int m3 = a3; m3 <= b3; m3++)
+      for (int m1 = a1; m1 <= b1; m1++)
+        for (int m2 = a2; m2 <= b2; m2++) {
+          if (m1 < 0 || m1 >= L.X || m2 < 0 || m2 >= L.Y || m3 < 0 || m3 >= L.Z) return STK_EINVAL;
+          long m4 = L.off + (long)m3 * L.XY + (long)m1 * L.Y + m2;
+          int sx = dm1, sy = dm2, sz = dm3;
+          if (std::abs(dm1) > 1) sx = -(L.X - 1);
+          if (std::abs(dm2) > 1) sy = -(L.Y - 1);
+          if (std::abs(dm3) > 1) sz = -(L.Z - 1);
+          long m6 = m4 + (long)sz * L.XY + (long)sx * L.Y + sy;
+          int nb = (int)c->bondn1.size();
+          c->bondm[m4 * 4 + var] = nb;
+          c->bondv1.push_back(cnst);
+          if (dm1 || dm2 || dm3) {
+            c->bondv1.push_back(koef);
+            c->bondn1.push_back((int)(m6 * 4 + var + 1));
+          } else {
+            c->bondv1.push_back(0.f);
+            c->bondn1.push_back(0);
+          }
+        }
+  }
+  c->prepared = false;
+  return STK_OK;
+}
+
+int stk_set_bc_tables(stk_ctx *c, const int *bondm, int nbond, const float *bondv1, const int *bondn1) {
+  if (!c || !bondm || nbond < 0 || (nbond && (!bondv1 || !bondn1))) return STK_EINVAL;
+  c->bondm.assign(bondm, bondm + 4 * c->ntot);
+  c->bondv1.assign(bondv1, bondv1 + 2 * (nbond + 1));
+  c->bondn1.assign(bondn1, bondn1 + nbond + 1);
+  for (long r = 0; r < 4 * c->ntot; r++)
+    if (c->bondm[r] < 0 || c->bondm[r] > nbond) return STK_EINVAL;
+  c->prepared = false;
+  return STK_OK;
+}
+
+int stk_get_bc_tables(const stk_ctx *c, int *bondm, int *nbond, float *bondv1, int *bondn1) {
+  if (!c || !nbond) return STK_EINVAL;
+  *nbond = (int)c->bondn1.size() - 1;
+  if (bondm) memcpy(bondm, c->bondm.data(), c->bondm.size() * sizeof(int));
+  if (bondv1) memcpy(bondv1, c->bondv1.data(), c->bondv1.size() * sizeof(float));
+  if (bondn1) memcpy(bondn1, c->bondn1.data(), c->bondn1.size() * sizeof(int));
+  return STK_OK;
+}
+
+int stk_set_fields(stk_ctx *c, const float *nu, const float *ro, const float *nxx, const float *nxy,
+                   const float *nxz, const float *nyz) {
+  if (!c || !nu || !ro || !nxx || !nxy || !nxz || !nyz) return STK_EINVAL;
+  long n0 = c->lv[0].n;
+  for (long i = 0; i < n0; i++)
+    if (!(nu[i] > 0) || !std::isfinite(nu[i]) || !std::isfinite(ro[i]) || !(nxx[i] > 0) || !(nxy[i] > 0) ||
+        !(nxz[i] > 0) || !(nyz[i] > 0))
+      return STK_EINVAL;
+  HIPCHK(hipMemcpy(c->nu, nu, n0 * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->ro, ro, n0 * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->nxx, nxx, n0 * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->nxy, nxy, n0 * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->nxz, nxz, n0 * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->nyz, nyz, n0 * 4, hipMemcpyHostToDevice));
+  c->fields_set = true;
+  c->prepared = false;
+  return STK_OK;
+}
+
+int stk_get_fields(stk_ctx *c, float *nu, float *ro, float *nd, float *nxx, float *nxy, float *nxz, float *nyz) {
+  if (!c) return STK_EINVAL;
+  long n0 = c->lv[0].n;
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (nu) HIPCHK(hipMemcpy(nu, c->nu, c->ntot * 4, hipMemcpyDeviceToHost));
+  if (ro) HIPCHK(hipMemcpy(ro, c->ro, c->ntot * 4, hipMemcpyDeviceToHost));
+  if (nd) HIPCHK(hipMemcpy(nd, c->nd, c->ntot * 4, hipMemcpyDeviceToHost));
+  if (nxx) HIPCHK(hipMemcpy(nxx, c->nxx, n0 * 4, hipMemcpyDeviceToHost));
+  if (nxy) HIPCHK(hipMemcpy(nxy, c->nxy, n0 * 4, hipMemcpyDeviceToHost));
+  if (nxz) HIPCHK(hipMemcpy(nxz, c->nxz, n0 * 4, hipMemcpyDeviceToHost));
+  if (nyz) HIPCHK(hipMemcpy(nyz, c->nyz, n0 * 4, hipMemcpyDeviceToHost));
+  return STK_OK;
+}
+
+int stk_markers_to_nodes(stk_ctx *c, long nmark, const float *mx, const float *my, const float *mz,
+                         const float *mnu, const float *mro, double nukoef) {
+  if (!c || nmark <= 0 || nmark > 0x7ffffff0L || !mx || !my || !mz || !mnu || !mro) return STK_EINVAL;
+  Level &F = c->lv[0];
+  const int ncell = (F.X - 1) * (F.Y - 1) * (F.Z - 1);
+  float *in[5] = {}, *so[5] = {};
+  int *key = nullptr, *val = nullptr, *skey = nullptr, *sval = nullptr, *start = nullptr;
+  void *tmp = nullptr;
+  size_t tmpb = 0;
+  int rc = STK_OK;
+  const float *src[5] = {mx, my, mz, mnu, mro};
+  for (int k = 0; k < 5 && !rc; k++) rc = dalloc(&in[k], nmark) || dalloc(&so[k], nmark) ? STK_ENOMEM : 0;
+  if (!rc) rc = (dalloc(&key, nmark) || dalloc(&val, nmark) || dalloc(&skey, nmark) || dalloc(&sval, nmark) ||
+                 dalloc(&start, ncell + 2)) ? STK_ENOMEM : 0;
+  if (!rc) {
+    for (int k = 0; k < 5; k++) hipMemcpyAsync(in[k], src[k], nmark * 4, hipMemcpyHostToDevice, c->st);
+    hipLaunchKernelGGL(k_marker_cells, dim3(blocks(nmark, 256)), dim3(256), 0, c->st, nmark, in[0], in[1], in[2],
+                       F.g64[0], F.g64[1], F.g64[2], F.X, F.Y, F.Z, key, val);
+    hipcub::DeviceRadixSort::SortPairs(nullptr, tmpb, key, skey, val, sval, (int)nmark, 0, 32, c->st);
+    if (dalloc((unsigned char **)&tmp, tmpb)) rc = STK_ENOMEM;
+  }
+  if (!rc) {
+    hipcub::DeviceRadixSort::SortPairs(tmp, tmpb, key, skey, val, sval, (int)nmark, 0, 32, c->st);
+    hipLaunchKernelGGL(k_marker_permute, dim3(blocks(nmark, 256)), dim3(256), 0, c->st, nmark, sval, in[0], in[1],
+                       in[2], in[3], in[4], so[0], so[1], so[2], so[3], so[4]);
+    hipLaunchKernelGGL(k_cell_start, dim3(blocks(nmark + 1, 256)), dim3(256), 0, c->st, nmark, skey, ncell + 1, start);
+    hipLaunchKernelGGL(k_markers_gather, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, F.g64[0],
+                       F.g64[1], F.g64[2], start, so[0], so[1], so[2], so[3], so[4], c->kn.nubeg, c->kn.nuend,
+                       nukoef, c->kn.viscmod, c->nu, c->ro, c->nxx, c->nxy, c->nxz, c->nyz);
+    if (hipStreamSynchronize(c->st) != hipSuccess) rc = STK_EHIP;
+  }
+  for (int k = 0; k < 5; k++) { dfree(in[k]); dfree(so[k]); }
+  dfree(key); dfree(val); dfree(skey); dfree(sval); dfree(start);
+  if (tmp) hipFree(tmp);
+  if (!rc) { c->fields_set = true; c->prepared = false; }
+  return rc;
+}
+
+// flags + boundary lists of every level from the reference tables
+static int build_rows(stk_ctx *c) {
+  for (int l = 0; l <= c->L; l++) {
+    Level &L = c->lv[l];
+    L.hflags.assign(L.n, 0);
+    std::vector<long> bcs;
+    for (int m3 = 0; m3 < L.Z; m3++)
+      for (int m1 = 0; m1 < L.X; m1++)
+        for (int m2 = 0; m2 < L.Y; m2++) {
+          long n = (long)m3 * L.XY + (long)m1 * L.Y + m2;
+          unsigned char f = 0;
+          for (int cc = 0; cc < 4; cc++) {
+            if (!comp_exists(L, m1, m2, m3, cc)) continue;
+            long r = 4 * (L.off + n) + cc;
+            if (c->bondm[r]) {
+              f |= KIND_BC << (2 * cc);
+              bcs.push_back(r);
+            } else {
+              if (!eq_inside(L, m1, m2, m3, cc)) {
+                fprintf(stderr, "pardiso2_amd: level %d unknown (%d,%d,%d,%d) needs a boundary equation\n", l, m1,
+                        m2, m3, cc);
+                return STK_EINVAL;
+              }
+              f |= KIND_EQ << (2 * cc);
+            }
+          }
+          L.hflags[n] = f;
+        }
+    // dependency depth of each boundary row
+    long base = 4 * L.off;
+    std::vector<int> depth(bcs.size(), 0);
+    int maxd = 0;
+    for (size_t i = 0; i < bcs.size(); i++) {
+      long r = bcs[i];
+      int d = 0;
+      while (true) {
+        int b = c->bondm[r];
+        if (!b || !c->bondn1[b]) break;
+        long j = c->bondn1[b] - 1;
+        if (j < base || j >= base + 4 * L.n) return STK_EINVAL;  // partner outside the level
+        long nj = (j - base) / 4;
+        int cj = (int)((j - base) & 3);
+        if (kind_of(L.hflags[nj], cj) != KIND_BC) break;
+        r = j;
+        if (++d > 64) return STK_EINVAL;  // cyclic boundary equations
+      }
+      depth[i] = d;
+      maxd = std::max(maxd, d);
+    }
+    L.hbc_r.clear(); L.hbc_j.clear(); L.hbc_negk.clear(); L.hbc_c.clear();
+    L.bcoff.assign(1, 0);
+    for (int d = 0; d <= maxd; d++) {
+      for (size_t i = 0; i < bcs.size(); i++) {
+        if (depth[i] != d) continue;
+        long r = bcs[i];
+        int b = c->bondm[r];
+        L.hbc_r.push_back((int)(r - base));
+        L.hbc_j.push_back(c->bondn1[b] ? (int)(c->bondn1[b] - 1 - base) : -1);
+        L.hbc_negk.push_back((double)(-c->bondv1[2 * b + 1]));
+        L.hbc_c.push_back((double)c->bondv1[2 * b + 0]);
+      }
+      L.bcoff.push_back((int)L.hbc_r.size());
+    }
+    L.nbc = (int)L.hbc_r.size();
+    dfree(L.bc_r); dfree(L.bc_j); dfree(L.bc_negk); dfree(L.bc_c);
+    if (upload(&L.bc_r, L.hbc_r) || upload(&L.bc_j, L.hbc_j) || upload(&L.bc_negk, L.hbc_negk) ||
+        upload(&L.bc_c, L.hbc_c))
+      return STK_ENOMEM;
+    HIPCHK(hipMemcpy(L.flags, L.hflags.data(), L.n, hipMemcpyHostToDevice));
+  }
+  return STK_OK;
+}
+
+int stk_prepare(stk_ctx *c) {
+  if (!c) return STK_EINVAL;
+  if (!c->fields_set) return STK_ESTATE;
+  int rc = build_rows(c);
+  if (rc) return rc;
+  // nu/ro restriction to every level (parallel/move3mg.c:2247)
+  for (int l = 1; l <= c->L; l++) {
+    Level &F = c->lv[l - 1], &C = c->lv[l];
+    CompTab t;
+    for (int a = 0; a < 3; a++) t.ax[a] = C.ax[3][a].tab();
+    hipLaunchKernelGGL(k_ronurestrict, dim3(blocks(C.n, 256)), dim3(256), 0, c->st, F.X, F.Y, C.X, C.Y, C.Z, t,
+                       c->nu + F.off, c->ro + F.off, c->nu + C.off, c->ro + C.off, c->kn.viscmod);
+  }
+  for (int l = 0; l <= c->L; l++) {
+    Level &L = c->lv[l];
+    hipLaunchKernelGGL(k_visc_prep, dim3(blocks(L.n, 256)), dim3(256), 0, c->st, L.X, L.Y, L.Z, l == 0 ? 1 : 0,
+                       c->nu + L.off, c->nxx, c->nxy, c->nxz, c->nyz, c->kn.nubeg, c->kn.nuend, c->kn.viscmod,
+                       L.e64[0], L.e64[1], L.e64[2], L.e64[3], L.e64[4], L.e32[0], L.e32[1], L.e32[2], L.e32[3],
+                       L.e32[4]);
+    // nd = cell viscosity of the continuity stencil (continerr, move3mg.c:1549)
+    hipLaunchKernelGGL((k_convert<float, float>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, L.n, L.e32[0],
+                       c->nd + L.off);
+  }
+  rc = ensure_outer(c, 1);
+  if (rc) return rc;
+  Level &F = c->lv[0];
+  hipLaunchKernelGGL(k_rhs, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, F.flags, c->ro,
+                     c->kn.gx, c->kn.gy, c->kn.gz, c->B);
+  if (F.nbc) hipLaunchKernelGGL(k_bc_rhs, dim3(blocks(F.nbc, 256)), dim3(256), 0, c->st, F.nbc, F.bc_r, F.bc_c, c->B);
+  hipLaunchKernelGGL(k_weights, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, F.flags, F.e64[0],
+                     F.g64[0], F.g64[1], F.g64[2], c->W2);
+  rc = coarse_inverse(c, 0, &c->ainv64);
+  if (rc) return rc;
+  Level &C = c->lv[c->L];
+  long nn = 16 * C.n * C.n;
+  dfree(c->ainv32);
+  if (dalloc(&c->ainv32, nn)) return STK_ENOMEM;
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks(nn, 256)), dim3(256), 0, c->st, nn, c->ainv64,
+                     c->ainv32);
+  dfree(c->ainvP64);
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->prepared = true;
+  return STK_OK;
+}
+
+int stk_set_solution(stk_ctx *c, const double *x) {
+  if (!c || !x) return STK_EINVAL;
+  int rc = ensure_outer(c, 1);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(c->X, x, 4 * c->lv[0].n * 8, hipMemcpyHostToDevice));
+  return STK_OK;
+}
+
+int stk_get_solution(stk_ctx *c, double *x) {
+  if (!c || !x || !c->X) return STK_EINVAL;
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(x, c->X, 4 * c->lv[0].n * 8, hipMemcpyDeviceToHost));
+  return STK_OK;
+}
+
+int stk_relres(stk_ctx *c, double *rel, double blk[3]) {
+  if (!c || !c->prepared) return STK_ESTATE;
+  true_residual(c);
+  double s[4];
+  int rc = blocks_norms(c, c->R, c->B, s);
+  if (rc) return rc;
+  double nb = std::sqrt(s[3]);
+  if (rel) *rel = std::sqrt(s[0] + s[1] + s[2]) / (nb > 0 ? nb : 1.0);
+  if (blk) { blk[0] = std::sqrt(s[0]); blk[1] = std::sqrt(s[1]); blk[2] = std::sqrt(s[2]); }
+  return STK_OK;
+}
+
+int stk_pressure_gauge(stk_ctx *c, int px, int py, int pz, double pinit) {
+  if (!c || !c->X) return STK_EINVAL;
+  Level &F = c->lv[0];
+  if (px < 1 || py < 1 || pz < 1 || px >= F.X || py >= F.Y || pz >= F.Z) return STK_EINVAL;
+  long n = (long)pz * F.XY + (long)px * F.Y + py;
+  hipLaunchKernelGGL((k_convert<double, double>), dim3(1), dim3(1), 0, c->st, 1L, c->X + 4 * n, c->scal + 16);
+  hipLaunchKernelGGL(k_pshift, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.n, F.flags, c->X, c->scal + 16, pinit);
+  HIPCHK(hipStreamSynchronize(c->st));
+  return STK_OK;
+}
+
+int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
+  if (!c || !o) return STK_EINVAL;
+  if (!c->prepared) return STK_ESTATE;
+  if (o->restart < 1 || o->restart > 64 || o->maxit < 1 || !(o->tol > 0)) return STK_EINVAL;
+  int tile[3] = {o->tile[0] ? o->tile[0] : kTileDefault[0], o->tile[1] ? o->tile[1] : kTileDefault[1],
+                 o->tile[2] ? o->tile[2] : kTileDefault[2]};
+  if (tile[0] < 2 || tile[1] < 1 || tile[2] < 1 || (size_t)(tile[0] + 2) * (tile[1] + 2) * (tile[2] + 2) * 32 > 65536)
+    return STK_EINVAL;
+  const bool fp64 = o->precision == 1;
+  int rc = ensure_vectors(c, !fp64, fp64);
+  if (rc) return rc;
+  rc = ensure_outer(c, o->restart);
+  if (rc) return rc;
+  stk_stats local;
+  stk_stats *S = st ? st : &local;
+  memset(S, 0, sizeof(*S));
+  long n4 = 4 * c->lv[0].n;
+  auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(hipEventRecord(c->ev0, c->st));
+  true_residual(c);
+  double s4[4];
+  rc = blocks_norms(c, c->R, c->B, s4);
+  if (rc) return rc;
+  const double nb = std::sqrt(s4[3]) > 0 ? std::sqrt(s4[3]) : 1.0;
+  double rel = std::sqrt(s4[0] + s4[1] + s4[2]) / nb;
+  S->relres0 = rel;
+  S->hist[0] = rel;
+  int it = 0, k = 0;
+  double hr[2];
+  while (rel > o->tol && it < o->maxit) {
+    double *Z = c->Zs[k], *Q = c->Qs[k];
+    if (fp64) precond<double>(c, o, c->R, Z, tile);
+    else precond<float>(c, o, c->R, Z, tile);
+    rowop<double, false>(c, 0, Z, nullptr, Q, 1);
+    if (k > 0) {
+      std::vector<const double *> bs(c->Qs.begin(), c->Qs.begin() + k);
+      multidot(c, Q, bs, c->scal + 32);
+      multiaxpy(c, k, c->scal + 32, Q, Z);
+    }
+    multidot(c, Q, {Q, c->R}, c->scal);
+    hipLaunchKernelGGL(k_gcr_scalars, dim3(1), dim3(1), 0, c->st, c->scal, c->scal + 2);
+    hipLaunchKernelGGL(k_scale2, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, c->scal + 2, Q, Z);
+    hipLaunchKernelGGL(k_update, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, c->scal + 3, Z, Q, c->X, c->R);
+    multidot(c, c->R, {c->R}, c->scal + 4);
+    HIPCHK(hipMemcpyAsync(hr, c->scal + 4, 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double nrel = std::sqrt(std::max(hr[0], 0.0)) / nb;
+    it++;
+    k++;
+    if (!std::isfinite(nrel)) { S->status = STK_ENAN; break; }
+    if (k == o->restart) {
+      k = 0;
+      true_residual(c);
+      multidot(c, c->R, {c->R}, c->scal + 4);
+      HIPCHK(hipMemcpyAsync(hr, c->scal + 4, 8, hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+      nrel = std::sqrt(hr[0]) / nb;
+    }
+    rel = nrel;
+    if (it < 256) S->hist[it] = rel;
+    if (o->verbose) fprintf(stderr, "pardiso2_amd: it %d relres %.3e\n", it, rel);
+  }
+  // final residual recomputed from b - A x
+  true_residual(c);
+  rc = blocks_norms(c, c->R, c->B, s4);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(c->ev1, c->st));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  (void)t0;
+  S->seconds = ms * 1e-3;
+  S->iterations = it;
+  S->relres = std::sqrt(s4[0] + s4[1] + s4[2]) / nb;
+  S->blocks[0] = std::sqrt(s4[0]); S->blocks[1] = std::sqrt(s4[1]); S->blocks[2] = std::sqrt(s4[2]);
+  if (!S->status) S->status = (S->relres <= o->tol * 1.0000001) ? STK_OK : STK_ENOCONV;
+  return S->status;
+}
+
+// ---- reference arrays and kernels on the context -------------------------
+static int ensure_ref(stk_ctx *c, int l) {
+  Level &L = c->lv[l];
+  for (int k = 0; k < 3; k++)
+    if (!L.ref[k]) {
+      if (dalloc(&L.ref[k], 4 * L.n)) return STK_ENOMEM;
+      hipMemset(L.ref[k], 0, 4 * L.n * 8);
+    }
+  return STK_OK;
+}
+
+int stk_vec_set(stk_ctx *c, int which, int l, const double *h) {
+  if (!c || which < 0 || which > 2 || l < 0 || l > c->L || !h) return STK_EINVAL;
+  int rc = ensure_ref(c, l);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(c->lv[l].ref[which], h, 4 * c->lv[l].n * 8, hipMemcpyHostToDevice));
+  return STK_OK;
+}
+
+int stk_vec_get(stk_ctx *c, int which, int l, double *h) {
+  if (!c || which < 0 || which > 2 || l < 0 || l > c->L || !h) return STK_EINVAL;
+  int rc = ensure_ref(c, l);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(h, c->lv[l].ref[which], 4 * c->lv[l].n * 8, hipMemcpyDeviceToHost));
+  return STK_OK;
+}
+
+int stk_apply(stk_ctx *c, int l, const double *x, double *y) {
+  if (!c || !c->prepared || l < 0 || l > c->L || !x || !y) return STK_EINVAL;
+  Level &L = c->lv[l];
+  double *dx, *dy;
+  if (dalloc(&dx, 4 * L.n) || dalloc(&dy, 4 * L.n)) return STK_ENOMEM;
+  HIPCHK(hipMemcpy(dx, x, 4 * L.n * 8, hipMemcpyHostToDevice));
+  rowop<double, false>(c, l, dx, nullptr, dy, 1);
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(y, dy, 4 * L.n * 8, hipMemcpyDeviceToHost));
+  dfree(dx); dfree(dy);
+  return STK_OK;
+}
+
+// FP64 coupled sweeps on the reference arrays of level l, then fre0
+int stk_smooth(stk_ctx *c, int l, int sweeps, int ty, int tx, int tz) {
+  if (!c || !c->prepared || l < 0 || l > c->L || sweeps < 0 || ty < 1 || tx < 1 || tz < 1) return STK_EINVAL;
+  if ((size_t)(ty + 2) * (tx + 2) * (tz + 2) * 32 > 65536) return STK_EINVAL;
+  int rc = ensure_ref(c, l) || ensure_vectors(c, false, true);
+  if (rc) return STK_ENOMEM;
+  Level &L = c->lv[l];
+  size_t bytes = 4 * L.n * 8;
+  HIPCHK(hipMemcpyAsync(L.x64[L.cur64], L.ref[0], bytes, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(L.b64, L.ref[1], bytes, hipMemcpyDeviceToDevice, c->st));
+  int tile[3] = {ty, tx, tz};
+  smooth<double>(c, l, sweeps, tile, 1);
+  rowop<double, true>(c, l, L.x64[L.cur64], L.b64, L.ref[2], 1);
+  HIPCHK(hipMemcpyAsync(L.ref[0], L.x64[L.cur64], bytes, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return STK_OK;
+}
+
+int stk_coarse_solve(stk_ctx *c) {
+  if (!c || !c->prepared) return STK_ESTATE;
+  int rc = ensure_ref(c, c->L);
+  if (rc) return rc;
+  if (!c->ainvP64) {
+    rc = coarse_inverse(c, 1, &c->ainvP64);
+    if (rc) return rc;
+  }
+  Level &C = c->lv[c->L];
+  int n4 = (int)(4 * C.n);
+  hipLaunchKernelGGL((k_gemv<double, double>), dim3(blocks(n4, 4)), dim3(256), 0, c->st, c->ainvP64, C.ref[1],
+                     C.ref[0], n4);
+  HIPCHK(hipStreamSynchronize(c->st));
+  return STK_OK;
+}
+
+int stk_bind(stk_ctx *c) {
+  g_bound = c;
+  return STK_OK;
+}
+
+// ---- reference-named entry points ----------------------------------------
+
+// parallel/gaus3mg_MG.c:2-17
+void matclear(int rm, double *adr) {
+  for (int m1 = 0; m1 < rm; m1++)
+    for (int m2 = 0; m2 <= rm; m2++) *adr++ = 0;
+}
+
+// parallel/gaus3mg_MG.c:21-61: row-combination elimination without pivoting
+// on the rm x (rm+1) augmented matrix, back substitution skipping zero pivots
+void gausmat(int rm, double *adr, double *adr1) {
+  const int w = rm + 1;
+  for (int m1 = 0; m1 < rm - 1; m1++) {
+    double v1 = adr[m1 * w + m1];
+    if (v1 == 0) continue;
+    for (int m2 = m1 + 1; m2 < rm; m2++) {
+      double v2 = adr[m2 * w + m1];
+      if (v2 == 0) continue;
+      for (int m3 = m1 + 1; m3 <= rm; m3++) adr[m2 * w + m3] = adr[m2 * w + m3] / v2 - adr[m1 * w + m3] / v1;
+    }
+  }
+  for (int m1 = rm - 1; m1 >= 0; m1--) {
+    adr1[m1] = 0;
+    double v1 = adr[m1 * w + m1];
+    if (v1 != 0) {
+      double v2 = adr[m1 * w + rm] / v1;
+      adr1[m1] = v2;
+      for (int m2 = 0; m2 <= m1 - 1; m2++) {
+        double a = adr[m2 * w + m1];
+        if (a != 0) adr[m2 * w + rm] -= a * v2;
+      }
+    }
+  }
+}
+
+// parallel/gaus3mg_MG.c:311-365: store the row in the bound context's
+// ELL row store (57 slots per node) and its right-hand side in fre1
+int gausmat2(int am, long mcmax, long pos0cur1, long *un1, double *ui1) {
+  (void)am;
+  stk_ctx *c = g_bound;
+  if (!c) return STK_ENOCTX;
+  if (mcmax < 0 || mcmax >= 4 * c->ntot || pos0cur1 < 0 || pos0cur1 + un1[0] > 57 * c->ntot) return STK_EINVAL;
+  if (c->hval1.empty()) {
+    c->hval1.assign(57 * c->ntot, 0.0);
+    c->hlin0.assign(57 * c->ntot, 0);
+    c->hnum0.assign(4 * c->ntot, 0);
+    c->hpos0.assign(4 * c->ntot, 0);
+    c->hfre1.assign(4 * c->ntot, 0.0);
+  }
+  c->hfre1[mcmax] = ui1[0];
+  c->hpos0[mcmax] = (int)pos0cur1;
+  c->hnum0[mcmax] = 0;
+  for (long m2 = 1; m2 <= un1[0]; m2++) {
+    c->hlin0[pos0cur1] = (int)un1[m2];
+    c->hval1[pos0cur1] = ui1[m2];
+    pos0cur1++;
+    c->hnum0[mcmax]++;
+  }
+  return 0;
+}
+
+// parallel/gaus3mg_MG.c:64-307.  am=1 accumulates one coarsest-level row
+// (level-relative indices, duplicate columns summed, zeros dropped);
+// am=0 solves the accumulated system with the device dense inverse and
+// writes sol0 of the coarsest level at 4*(mgp[L]+node)+c (the reference
+// writes mgp[L]+4*node+c, see parallel/gaus3mg_MG.c:234-238).
+int pardisomat(int am, long mcmax, long mcmax2, long pos0cur1, int tid, long *un1, double *ui1) {
+  (void)mcmax2; (void)pos0cur1; (void)tid;
+  stk_ctx *c = g_bound;
+  if (!c) return STK_ENOCTX;
+  Level &C = c->lv[c->L];
+  int n = (int)(4 * C.n);
+  if (am == 0) {
+    if (c->ca.size() != (size_t)n * n) return STK_ESTATE;
+    double *A, *Bm, *fac, *db, *dx;
+    int *sing;
+    if (dalloc(&A, (size_t)n * n) || dalloc(&Bm, (size_t)n * n) || dalloc(&fac, n) || dalloc(&sing, 1) ||
+        dalloc(&db, n) || dalloc(&dx, n))
+      return STK_ENOMEM;
+    std::vector<double> I((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) I[(size_t)i * n + i] = 1.0;
+    HIPCHK(hipMemcpy(A, c->ca.data(), (size_t)n * n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(Bm, I.data(), (size_t)n * n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(db, c->cb.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(sing, 0, 4));
+    for (int k = 0; k < n; k++) {
+      hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, c->st, A, Bm, n, k, fac, sing);
+      hipLaunchKernelGGL(k_gj_elim, dim3(blocks((long)n * n, 256)), dim3(256), 0, c->st, A, Bm, n, k, fac);
+    }
+    hipLaunchKernelGGL((k_gemv<double, double>), dim3(blocks(n, 4)), dim3(256), 0, c->st, Bm, db, dx, n);
+    int hs = 0;
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(&hs, sing, 4, hipMemcpyDeviceToHost));
+    int rc = ensure_ref(c, c->L);
+    if (!rc && !hs) HIPCHK(hipMemcpy(C.ref[0], dx, (size_t)n * 8, hipMemcpyDeviceToDevice));
+    dfree(A); dfree(Bm); dfree(fac); dfree(sing); dfree(db); dfree(dx);
+    return hs ? STK_ESINGULAR : rc;
+  }
+  if (mcmax < 0 || mcmax >= n) return STK_EINVAL;
+  if (c->ca.size() != (size_t)n * n) {
+    c->ca.assign((size_t)n * n, 0.0);
+    c->cb.assign(n, 0.0);
+  }
+  c->cb[mcmax] = ui1[0];
+  for (long m2 = 1; m2 <= un1[0]; m2++) {
+    if (un1[m2] < 0 || un1[m2] >= n) return STK_EINVAL;
+    if (ui1[m2]) c->ca[(size_t)mcmax * n + un1[m2]] += ui1[m2];
+  }
+  return 0;
+}
+
+// parallel/move3mg.c:3855-3960 on the bound context: sweeps on level mgi,
+// then fre0 = fre1 - A sol0 (rows [mcmin, mcmax] are the level's rows)
+int gausmat4(int am, long mcmax, long mcmin, int mgi) {
+  (void)mcmax; (void)mcmin;
+  stk_ctx *c = g_bound;
+  if (!c) return STK_ENOCTX;
+  return stk_smooth(c, mgi, am, kTileDefault[0], kTileDefault[1], kTileDefault[2]);
+}
+
+// parallel/move3mg.c:3966-4354
+void vprestrict(int mgi) {
+  stk_ctx *c = g_bound;
+  if (!c || !c->prepared || mgi < 1 || mgi > c->L) return;
+  if (ensure_ref(c, mgi) || ensure_ref(c, mgi - 1)) return;
+  restrict_to<double>(c, mgi, c->lv[mgi - 1].ref[2], c->lv[mgi].ref[1], c->lv[mgi].ref[0], 1);
+  hipStreamSynchronize(c->st);
+}
+
+// parallel/move3mg.c:3535-3850
+void vpprolong(int mgi, int ynreset) {
+  stk_ctx *c = g_bound;
+  if (!c || !c->prepared || mgi < 1 || mgi > c->L) return;
+  if (ensure_ref(c, mgi) || ensure_ref(c, mgi - 1)) return;
+  prolong_from<double>(c, mgi, c->lv[mgi].ref[0], c->lv[mgi - 1].ref[0], 1, ynreset);
+  hipStreamSynchronize(c->st);
+}
+
+// parallel/move3mg.c:3387-3528
+void ronurestrict(int mgi) {
+  stk_ctx *c = g_bound;
+  if (!c || mgi < 1 || mgi > c->L) return;
+  Level &F = c->lv[mgi - 1], &C = c->lv[mgi];
+  CompTab t;
+  for (int a = 0; a < 3; a++) t.ax[a] = C.ax[3][a].tab();
+  hipLaunchKernelGGL(k_ronurestrict, dim3(blocks(C.n, 256)), dim3(256), 0, c->st, F.X, F.Y, C.X, C.Y, C.Z, t,
+                     c->nu + F.off, c->ro + F.off, c->nu + C.off, c->ro + C.off, c->kn.viscmod);
+  hipStreamSynchronize(c->st);
+}
+
+// parallel/move3mg.c:1791-3382: prepare and solve the bound context
+void vpiterate(int m0, int f0) {
+  (void)m0; (void)f0;
+  stk_ctx *c = g_bound;
+  if (!c) return;
+  if (!c->prepared && stk_prepare(c)) return;
+  stk_solver_opts o;
+  stk_default_opts(&o, c->L);
+  stk_solve(c, &o, nullptr);
+}
+
+}  // extern "C"

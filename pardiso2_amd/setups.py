@@ -110,8 +110,9 @@ def free_slip_lines(nx: int, ny: int, nz: int, gauge=(2, 2, 2), pgauge=0.0) -> L
     L.append((0, 1, X, Y, Y, 1, Z, 0.0, 1.0, 0, -1, 0))
     L.append((0, 1, X, 1, Y, 1, 1, 0.0, 1.0, 0, 0, +1))
     L.append((0, 1, X, 1, Y, Z, Z, 0.0, 1.0, 0, 0, -1))
-    g = gauge
-    L.append((0, g[0], g[0], g[1], g[1], g[2], g[2], float(pgauge), 0.0, 0, 0, 0))
+    if gauge is not None:
+        g = gauge
+        L.append((0, g[0], g[0], g[1], g[1], g[2], g[2], float(pgauge), 0.0, 0, 0, 0))
     return L
 
 
