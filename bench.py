@@ -1,0 +1,191 @@
+#!/usr/bin/env python
+"""Stokes solves/s and p99 latency to a 1e-8 FP64 relative residual on one MI355X.
+
+Workload (default): single 129^3-node solve, 6 multigrid levels
+(129/67/36/20/12/8), log10(viscosity) a seeded Gaussian random field
+(correlation length 0.1 box, clipped to [18, 24]), density 3300 +/- 50,
+free-slip box, gravity -z.  Node fields are built on the device from 2x2x2
+markers per cell once; a step is one solve from those resident node fields:
+prepare (viscosity restriction to all levels, stencil viscosities,
+right-hand side, coarsest inverse) + hydrostatic guess + FP64 outer
+iteration with FP32 V-cycle preconditioning until ||W r||/||W b|| <= 1e-8.
+
+The line also carries the roofline of the finest-level FP32 smoother sweep
+(64 algorithmic bytes per node and sweep) and the CPU oracle's time for the
+same algorithm on the same problem (bounded sample, single thread).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+PEAK_HBM_GBS = 8000.0
+SMOOTHER_BYTES_PER_NODE = 64.0   # FP32 sweep: x(16) + b(16) + 4 viscosities(16) read, x(16) written
+RESIDUAL_BYTES_PER_NODE = 112.0  # FP64 residual: x(32) + b(32) + 4 viscosities(16) read, r(32) written
+
+
+def make_problem(workload: str, n: int, seed: int):
+    from pardiso2_amd import setups
+    L = setups.coarsest_levels(n)
+    if workload == "randfield":
+        pb = setups.random_field(n, L, seed=seed)
+    elif workload == "sinker":
+        pb = setups.sinker(n, L, seed=seed)
+    elif workload == "spheres":
+        pb = setups.random_spheres(n, L, seed=seed)
+    else:
+        raise SystemExit(f"unknown workload {workload}")
+    pb.bc = setups.free_slip_lines(n, n, n, gauge=None)
+    return pb
+
+
+def sweeps_for(L: int):
+    pre = [2, 4, 8, 16, 32, 64, 64, 64, 64, 64, 64][:L] + [0]
+    return pre, pre
+
+
+def cpu_baseline(pb, pre, post, its_needed: int, budget_s: float):
+    """The oracle's FP64 GCR (same algorithm, tile-coloured smoother) on the
+    same problem, single thread, for a bounded number of outer iterations."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as O
+    O.build()
+    t0 = time.perf_counter()
+    o = O.from_problem(pb)
+    setup_s = time.perf_counter() - t0
+    # one iteration to estimate, then as many as fit the budget
+    it, _, _, spi = o.gcr(pre, post, mode=1, tol=1e-30, maxit=1, restart=32)
+    k = int(max(1, min(20, (budget_s - spi) / max(spi, 1e-6))))
+    it, _, hist, spi = o.gcr(pre, post, mode=1, tol=1e-30, maxit=k, restart=32)
+    per_solve = spi * its_needed
+    return {"value": 1.0 / per_solve if per_solve > 0 else None, "unit": "solves/s", "cores": 1,
+            "kind": "port",
+            "sample": (f"{it} outer iterations ({spi:.3f} s each) of the FP64 GCR + velocity V-cycle on the "
+                       f"same {pb.n}^3 problem, single thread, times {its_needed} iterations of the FP64 "
+                       f"device solve; oracle setup {setup_s:.1f} s not counted")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="randfield")
+    ap.add_argument("--n", type=int, default=129)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--precision", type=int, default=0, help="0 mixed FP32/FP64, 1 FP64")
+    ap.add_argument("--restart", type=int, default=128)
+    ap.add_argument("--maxit", type=int, default=800)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64-ref", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: multi-GPU runs are not part of this build "
+              "(instances shard by request; see the design notes).", file=sys.stderr)
+        sys.exit(2)
+
+    from pardiso2_amd import _lib as S
+    pb = make_problem(args.workload, args.n, args.seed)
+    L = pb.levels
+    pre, post = sweeps_for(L)
+    ctx = S.from_problem(pb)
+    opts = S.default_opts(L, precision=args.precision, pre=pre, post=post, tol=1e-8, maxit=args.maxit,
+                          restart=args.restart)
+
+    def step():
+        ctx.prepare()
+        ctx.hydrostatic_guess()
+        return ctx.solve(opts)
+
+    for _ in range(args.warmup):
+        step()
+    lat, its, rels, solve_s = [], [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        st = step()
+        lat.append(time.perf_counter() - ts)
+        its.append(st.iterations)
+        rels.append(st.relres)
+        solve_s.append(st.seconds)
+        if st.status != 0:
+            print(f"warning: solve status {st.status} relres {st.relres:.3e}", file=sys.stderr)
+    total = time.perf_counter() - t0
+    lat_ms = np.array(lat) * 1e3
+
+    # roofline of the finest-level FP32 smoother and FP64 residual (HIP events)
+    n0 = ctx.n0
+    ms_sm = ctx.time_kernel(0, 0, reps=50)
+    ms_res = ctx.time_kernel(4, 0, reps=50)
+    gbs_sm = SMOOTHER_BYTES_PER_NODE * n0 / (ms_sm * 1e-3) / 1e9
+    gbs_res = RESIDUAL_BYTES_PER_NODE * n0 / (ms_res * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(HERE, "profiles", "k_smooth_float_traffic.json")
+    if os.path.exists(tf):
+        try:
+            with open(tf) as f:
+                tj = json.load(f)
+            if tj.get("nodes") == n0:
+                traffic = tj.get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    fp64_its = None
+    if not args.no_fp64_ref:
+        o64 = S.default_opts(L, precision=1, pre=pre, post=post, tol=1e-8, maxit=args.maxit, restart=args.restart)
+        ctx.prepare()
+        ctx.hydrostatic_guess()
+        s64 = ctx.solve(o64)
+        fp64_its = s64.iterations
+        fp64_s = s64.seconds
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(pb, pre, post, fp64_its or its[-1], args.cpu_budget)
+        except Exception as e:  # the baseline must not hide the device result
+            cpu = {"value": None, "unit": "solves/s", "cores": 1, "kind": "port", "sample": f"failed: {e}"}
+
+    out = {
+        "metric": "Stokes solves/s to 1e-8 FP64 relative residual",
+        "value": args.steps / total,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 smoother / f64 outer" if args.precision == 0 else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload} {args.n}^3 single solve, {L + 1} levels, free slip, gravity -z",
+                   "nodes": n0, "levels": L + 1, "seed": args.seed, "tol": 1e-8,
+                   "precision": "mixed" if args.precision == 0 else "fp64"},
+        "p50_ms": float(np.percentile(lat_ms, 50)),
+        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "iterations": its,
+        "relres": max(rels),
+        "solve_ms_device": float(np.mean(solve_s) * 1e3),
+        "fp64_iterations": fp64_its,
+        "roofline": {"bound": "hbm", "kernel": "k_smooth<float> finest level", "achieved": gbs_sm,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs_sm / PEAK_HBM_GBS, "traffic": traffic,
+                     "ms_per_launch": ms_sm, "bytes_per_launch": SMOOTHER_BYTES_PER_NODE * n0},
+        "residual_fp64": {"achieved": gbs_res, "unit": "GB/s", "frac": gbs_res / PEAK_HBM_GBS,
+                          "ms_per_launch": ms_res},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -57,6 +57,8 @@ typedef struct {
   int post[12];      /* post-smoothing sweeps per level */
   int tile[3];       /* smoother tile (y, x, z) in nodes; 0 = default */
   double omega;      /* pressure block scale of the preconditioner */
+  int cycle;         /* multigrid cycle index: 1 V-cycle, 2 W-cycle */
+  int coarse_viscmod; /* averaging of coarse-level viscosities: -1 = knobs.viscmod, 0/1/2 */
   double tol;        /* target relative residual ||W r|| / ||W b|| */
   int maxit;         /* outer iteration cap */
   int verbose;
@@ -118,6 +120,13 @@ int stk_get_solution(stk_ctx *ctx, double *sol0_level0);
 int stk_solve(stk_ctx *ctx, const stk_solver_opts *opts, stk_stats *stats);
 /* Relative residual of the current finest solution (FP64, device). */
 int stk_relres(stk_ctx *ctx, double *relres, double blocks[3]);
+/* Hydrostatic initial guess: v = 0, P integrated up each column through the
+ * z-momentum rows (the reference seeds P this way, parallel/in3mg.c:927-938). */
+int stk_hydrostatic_guess(stk_ctx *ctx);
+/* Average device time (ms) of one launch of a finest-level kernel, HIP events
+ * on the library stream.  which: 0 FP32 smoother sweep, 1 FP64 smoother
+ * sweep, 2 FP64 operator apply, 3 FP32 residual, 4 FP64 residual. */
+int stk_time_kernel(stk_ctx *ctx, int which, int level, int reps, double *ms_avg);
 /* Shift P so that P(px,py,pz) = pinit (parallel/move3mg.c:2835-2856). */
 int stk_pressure_gauge(stk_ctx *ctx, int px, int py, int pz, double pinit);
 

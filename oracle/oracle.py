@@ -64,6 +64,10 @@ def lib():
         L.orc_relres.argtypes = [C.c_void_p, dp]
         L.orc_solve.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_double, C.c_int, dp]
         L.orc_band_solve.argtypes = [C.c_void_p, dp, C.POINTER(C.c_long)]
+        L.orc_gcr.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                              dp, dp, dp]
+        L.orc_hydrostatic.argtypes = [C.c_void_p, dp]
+        L.orc_weights.argtypes = [C.c_void_p, dp]
         _lib = L
     return _lib
 
@@ -203,6 +207,16 @@ class Oracle:
         hist = np.zeros(maxcyc + 1)
         it = lib().orc_solve(self.h, _ip(p), _ip(q), mode, tol, maxcyc, _dp(hist))
         return it, hist[: abs(it) + 1]
+
+    def gcr(self, pre, post, mode=1, omega=1.0, tol=1e-8, maxit=300, restart=32, hydro=1):
+        p = np.ascontiguousarray(pre, dtype=np.int32)
+        q = np.ascontiguousarray(post, dtype=np.int32)
+        x = np.zeros(4 * self.info[0][4])
+        hist = np.zeros(maxit + 1)
+        spi = C.c_double()
+        it = lib().orc_gcr(self.h, _ip(p), _ip(q), mode, omega, tol, maxit, restart, hydro, _dp(x), _dp(hist),
+                           C.byref(spi))
+        return it, x, hist[: max(it, 0) + 1], spi.value
 
     def band_solve(self):
         x = np.zeros(4 * self.info[0][4])

@@ -14,6 +14,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #define ABSV(a) ((a) >= 0 ? (a) : -(a))
 #define NID(o, l, m1, m2, m3) ((o)->mgp[l] + (long)(m3) * (o)->mgxy[l] + (long)(m1) * (o)->mgy[l] + (m2))
@@ -1410,4 +1411,141 @@ int orc_band_solve(orc_t *o, double *x, long *band_out) {
   }
   free(U); free(buf); free(f);
   return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* FP64 outer iteration on the assembled finest operator: GCR with the right  */
+/* preconditioner z_p = omega*eta_cell*r_p (continuity rows), pressure        */
+/* boundary rows by substitution, z_v = one velocity-block V-cycle of the     */
+/* reference smoother/transfers for r_v - G z_p.  The same algorithm the      */
+/* device solver runs; used as the CPU baseline and as a solution check.      */
+/* ------------------------------------------------------------------------- */
+static void ell_apply(const orc_t *o, long n4, const double *x, double *y) {
+  for (long r = 0; r < n4; r++) {
+    double s = 0;
+    for (int k = 0; k < o->num0[r]; k++) s += o->val1[o->pos0[r] + k] * x[o->lin0[o->pos0[r] + k]];
+    y[r] = o->num0[r] ? s : 0.0;
+  }
+}
+
+static double wdot(long n4, const double *w2, const double *a, const double *b) {
+  double s = 0;
+  for (long i = 0; i < n4; i++) s += w2[i] * a[i] * b[i];
+  return s;
+}
+
+void orc_weights(orc_t *o, double *w2) {
+  const double *gx = o->mggx[0], *gy = o->mggy[0], *gz = o->mggz[0];
+  for (int m3 = 0; m3 < o->mgz[0]; m3++)
+    for (int m1 = 0; m1 < o->mgx[0]; m1++)
+      for (int m2 = 0; m2 < o->mgy[0]; m2++) {
+        long n = NID(o, 0, m1, m2, m3);
+        for (int c = 0; c < 4; c++) {
+          long r = 4 * n + c;
+          double w = o->num0[r] ? 1.0 : 0.0;
+          if (c == 0 && o->num0[r] && !o->bondm[r]) {
+            double h = ((gx[m1] - gx[m1 - 1]) + (gy[m2] - gy[m2 - 1]) + (gz[m3] - gz[m3 - 1])) / 3.0;
+            w = o->nd[n] / h;
+          }
+          w2[r] = w * w;
+        }
+      }
+}
+
+/* v = 0 and P integrated up each column through the z rows (in3mg.c:927-938) */
+void orc_hydrostatic(orc_t *o, double *x) {
+  long n4 = 4 * o->mgn[0];
+  memset(x, 0, sizeof(double) * n4);
+  for (int m1 = 0; m1 < o->mgx[0]; m1++)
+    for (int m2 = 0; m2 < o->mgy[0]; m2++)
+      for (int m3 = 0; m3 + 1 < o->mgz[0]; m3++) {
+        long r = 4 * NID(o, 0, m1, m2, m3) + 3;
+        if (!o->num0[r] || o->bondm[r] || m1 + 1 >= o->mgx[0] || m2 + 1 >= o->mgy[0]) continue;
+        long plo = 4 * NID(o, 0, m1 + 1, m2 + 1, m3), phi = 4 * NID(o, 0, m1 + 1, m2 + 1, m3 + 1);
+        double zkf = 2.0 / (o->mggz[0][m3 + 1] - o->mggz[0][m3 - 1]);
+        x[phi] = x[plo] - o->fre1[r] / zkf;
+      }
+  int vonly = o->vonly;
+  o->vonly = 0;
+  orc_bc_pass(o, 0, x);
+  o->vonly = vonly;
+}
+
+int orc_gcr(orc_t *o, const int *pre, const int *post, int mode, double omega, double tol, int maxit,
+            int restart, int hydro, double *x_out, double *hist, double *seconds_per_it) {
+  long n4 = 4 * o->mgn[0];
+  double *b = (double *)malloc(sizeof(double) * n4), *x = (double *)calloc(n4, sizeof(double));
+  double *r = (double *)malloc(sizeof(double) * n4), *w2 = (double *)malloc(sizeof(double) * n4);
+  double *Z = (double *)malloc(sizeof(double) * n4 * restart), *Q = (double *)malloc(sizeof(double) * n4 * restart);
+  double *fsave = (double *)malloc(sizeof(double) * 4 * o->ntot);
+  memcpy(b, o->fre1, sizeof(double) * n4);
+  memcpy(fsave, o->fre1, sizeof(double) * 4 * o->ntot);
+  orc_weights(o, w2);
+  o->vonly = 1;
+  if (orc_coarse_factor(o)) return -1000;
+  if (hydro) orc_hydrostatic(o, x);
+  ell_apply(o, n4, x, r);
+  for (long i = 0; i < n4; i++) r[i] = b[i] - r[i];
+  double nb = sqrt(wdot(n4, w2, b, b));
+  if (nb == 0) nb = 1;
+  double rel = sqrt(wdot(n4, w2, r, r)) / nb;
+  if (hist) hist[0] = rel;
+  int it = 0, k = 0;
+  clock_t t0 = clock();
+  while (rel > tol && it < maxit) {
+    double *z = Z + (long)k * n4, *q = Q + (long)k * n4;
+    /* pressure part */
+    for (long i = 0; i < n4; i++) z[i] = 0;
+    for (long i = 0; i < n4; i += 4)
+      if (o->num0[i] && !o->bondm[i]) z[i] = omega * o->nd[i / 4] * r[i];
+    {
+      /* pressure boundary rows: z_r = r_r + k z_j in dependency order */
+      double *ftmp = o->fre1;
+      memcpy(o->tmp, o->fre1, sizeof(double) * n4);
+      memcpy(o->fre1, r, sizeof(double) * n4);
+      int vo = o->vonly;
+      o->vonly = 0;
+      /* run the pass on P rows only: velocity BC rows of z are overwritten below */
+      orc_bc_pass(o, 0, z);
+      o->vonly = vo;
+      memcpy(ftmp, o->tmp, sizeof(double) * n4);
+      for (long i = 0; i < n4; i++)
+        if (i & 3) z[i] = 0;
+    }
+    /* velocity right-hand side r_v - G z_p */
+    ell_apply(o, n4, z, q);
+    for (long i = 0; i < n4; i++) o->fre1[i] = (i & 3) ? ((o->num0[i] && !o->bondm[i]) ? r[i] - q[i] : r[i]) : 0.0;
+    memset(o->sol0, 0, sizeof(double) * 4 * o->ntot);
+    orc_vcycle(o, pre, post, mode, 32, 4, 4);
+    for (long i = 0; i < n4; i++)
+      if (i & 3) z[i] = o->sol0[i];
+    memcpy(o->fre1, fsave, sizeof(double) * 4 * o->ntot);
+    ell_apply(o, n4, z, q);
+    for (int j = 0; j < k; j++) {
+      double *zj = Z + (long)j * n4, *qj = Q + (long)j * n4;
+      double be = wdot(n4, w2, q, qj);
+      for (long i = 0; i < n4; i++) { q[i] -= be * qj[i]; z[i] -= be * zj[i]; }
+    }
+    double nq = sqrt(wdot(n4, w2, q, q));
+    for (long i = 0; i < n4; i++) { q[i] /= nq; z[i] /= nq; }
+    double al = wdot(n4, w2, r, q);
+    for (long i = 0; i < n4; i++) { x[i] += al * z[i]; r[i] -= al * q[i]; }
+    it++;
+    k++;
+    if (k == restart) {
+      k = 0;
+      ell_apply(o, n4, x, r);
+      for (long i = 0; i < n4; i++) r[i] = b[i] - r[i];
+    }
+    rel = sqrt(wdot(n4, w2, r, r)) / nb;
+    if (hist) hist[it] = rel;
+    if (!(rel == rel)) break;
+  }
+  if (seconds_per_it) *seconds_per_it = it ? (double)(clock() - t0) / CLOCKS_PER_SEC / it : 0.0;
+  o->vonly = 0;
+  memcpy(o->fre1, fsave, sizeof(double) * 4 * o->ntot);
+  memcpy(o->sol0, x, sizeof(double) * n4);
+  if (x_out) memcpy(x_out, x, sizeof(double) * n4);
+  free(b); free(x); free(r); free(w2); free(Z); free(Q); free(fsave);
+  return it;
 }

@@ -112,6 +112,12 @@ double orc_relres(orc_t *o, double *blocks3);
 int orc_solve(orc_t *o, const int *pre, const int *post, int mode, double tol, int maxcyc,
               double *relres_hist);
 
+/* FP64 GCR with block preconditioner (velocity V-cycle + eta-scaled P) */
+void orc_weights(orc_t *o, double *w2);
+void orc_hydrostatic(orc_t *o, double *x);
+int orc_gcr(orc_t *o, const int *pre, const int *post, int mode, double omega, double tol, int maxit,
+            int restart, int hydro, double *x_out, double *hist, double *seconds_per_it);
+
 /* banded direct solver restatement (parallel/gausjomp.c:70-254) */
 int orc_band_solve(orc_t *o, double *x, long *band_out);
 

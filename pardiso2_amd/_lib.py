@@ -21,7 +21,7 @@ EXPORTS = [
     "stk_create", "stk_destroy", "stk_level_info", "stk_set_knobs", "stk_default_opts", "stk_bc_line",
     "stk_set_bc_tables", "stk_get_bc_tables", "stk_set_fields", "stk_markers_to_nodes", "stk_get_fields",
     "stk_prepare", "stk_set_solution", "stk_get_solution", "stk_solve", "stk_relres",
-    "stk_pressure_gauge", "stk_vec_set", "stk_vec_get", "stk_apply", "stk_smooth", "stk_coarse_solve",
+    "stk_pressure_gauge", "stk_hydrostatic_guess", "stk_time_kernel", "stk_vec_set", "stk_vec_get", "stk_apply", "stk_smooth", "stk_coarse_solve",
     "stk_bind", "matclear", "gausmat", "gausmat2", "pardisomat", "gausmat4", "vprestrict", "vpprolong",
     "ronurestrict", "vpiterate",
 ]
@@ -37,6 +37,7 @@ class Knobs(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("restart", C.c_int), ("pre", C.c_int * 12),
                 ("post", C.c_int * 12), ("tile", C.c_int * 3), ("omega", C.c_double),
+                ("cycle", C.c_int), ("coarse_viscmod", C.c_int),
                 ("tol", C.c_double), ("maxit", C.c_int), ("verbose", C.c_int)]
 
 
@@ -80,6 +81,8 @@ def lib():
         L.stk_solve.argtypes = [vp, C.POINTER(Opts), C.POINTER(Stats)]
         L.stk_relres.argtypes = [vp, dp, dp]
         L.stk_pressure_gauge.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.stk_hydrostatic_guess.argtypes = [vp]
+        L.stk_time_kernel.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
         L.stk_vec_set.argtypes = [vp, C.c_int, C.c_int, dp]
         L.stk_vec_get.argtypes = [vp, C.c_int, C.c_int, dp]
         L.stk_apply.argtypes = [vp, C.c_int, dp, dp]
@@ -224,6 +227,14 @@ class Stokes:
 
     def pressure_gauge(self, px, py, pz, pinit=0.0):
         check(lib().stk_pressure_gauge(self.h, px, py, pz, pinit), "stk_pressure_gauge")
+
+    def hydrostatic_guess(self):
+        check(lib().stk_hydrostatic_guess(self.h), "stk_hydrostatic_guess")
+
+    def time_kernel(self, which, level=0, reps=20):
+        ms = C.c_double()
+        check(lib().stk_time_kernel(self.h, which, level, reps, C.byref(ms)), "stk_time_kernel")
+        return ms.value
 
     def vec_set(self, which, level, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -116,31 +116,32 @@ __global__ void k_bc_rhs(int nbc, const int *__restrict__ r, const double *__res
 // order, move3mg.c:3937-3958), otherwise y = A x.  withp=0 leaves pressure
 // rows at zero (velocity block).  Boundary rows are done by k_bc_rowop.
 // ---------------------------------------------------------------------------
+template <typename T, int C, bool RES>
+__device__ __forceinline__ void rowop_one(const Geo<T> &g, unsigned char f, long n, int m1, int m2, int m3,
+                                          const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ y,
+                                          int withp) {
+  int k = kind_of(f, C);
+  if (k != KIND_EQ || (!withp && C == 0)) {
+    if (k == KIND_NONE || (!withp && C == 0)) y[4 * n + C] = T(0);
+    return;
+  }
+  T s[18];
+  row_coef_c<T, C>(g, m1, m2, m3, s);
+  y[4 * n + C] = row_dot<T, C, RES>(g, s, x, n, RES ? b[4 * n + C] : T(0));
+}
+
 template <typename T, bool RES>
 __global__ void k_rowop(Geo<T> g, const unsigned char *__restrict__ flags, const T *__restrict__ x,
                         const T *__restrict__ b, T *__restrict__ y, int withp) {
-  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  long n = u >> 2;
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (n >= (long)g.XY * g.Z) return;
-  int c = (int)(u & 3);
-  int k = kind_of(flags[n], c);
-  if (k != KIND_EQ || (!withp && c == 0)) {
-    if (k == KIND_NONE || (!withp && c == 0)) y[u] = T(0);
-    return;
-  }
   int m1, m2, m3;
   decode(n, g.X, g.Y, g.XY, m1, m2, m3);
-  T s[18];
-  int ns = row_coef<T>(g, c, m1, m2, m3, s);
-  T acc = RES ? b[u] : T(0);
-#pragma unroll
-  for (int q = 1; q <= 17; q++) {
-    if (q > ns) break;
-    long uq = 4 * (n + (long)kSlot[c][q][2] * g.XY + (long)kSlot[c][q][0] * g.Y + kSlot[c][q][1]) + kSlot[c][q][3];
-    if (RES) acc -= s[q] * x[uq];
-    else acc += s[q] * x[uq];
-  }
-  y[u] = acc;
+  unsigned char f = flags[n];
+  rowop_one<T, 0, RES>(g, f, n, m1, m2, m3, x, b, y, withp);
+  rowop_one<T, 1, RES>(g, f, n, m1, m2, m3, x, b, y, withp);
+  rowop_one<T, 2, RES>(g, f, n, m1, m2, m3, x, b, y, withp);
+  rowop_one<T, 3, RES>(g, f, n, m1, m2, m3, x, b, y, withp);
 }
 
 // boundary rows  x_r - k x_j = b_r  (bonderr, move3mg.c:1682-1729)
@@ -174,6 +175,42 @@ __global__ void k_bc_rowop(int nbc, const int *__restrict__ br, const int *__res
 // so the sweep is deterministic.  Boundary rows are skipped here and done
 // by k_bc_update afterwards.  The tile is written to xnew.
 // ---------------------------------------------------------------------------
+// one phase of the tile smoother for component C (colour for velocities)
+template <typename T, int C>
+__device__ __forceinline__ void smooth_phase(const Geo<T> &g, const unsigned char *__restrict__ flags,
+                                             const T *__restrict__ b, T *sx, int ty, int tx, int tz, int x0,
+                                             int y0, int z0, int colour, T v0koef, T p0koef) {
+  const int hy = ty + 2, hx = tx + 2;
+  const int hty = (ty + 1) / 2;
+  const int ncand = (C == 0) ? ty * tx * tz : hty * tx * tz;
+  for (int i = threadIdx.x; i < ncand; i += blockDim.x) {
+    int ly, lx, lz;
+    if (C == 0) {
+      ly = i % ty; lx = (i / ty) % tx; lz = i / (ty * tx);
+    } else {
+      int kk = i % hty; lx = (i / hty) % tx; lz = i / (hty * tx);
+      int par = (colour - (x0 + lx) - (z0 + lz) - y0) & 1;
+      ly = 2 * kk + par;
+      if (ly >= ty) continue;
+    }
+    int m1 = x0 + lx, m2 = y0 + ly, m3 = z0 + lz;
+    if (m1 >= g.X || m2 >= g.Y || m3 >= g.Z) continue;
+    long n = g.nid(m1, m2, m3);
+    if (kind_of(flags[n], C) != KIND_EQ) continue;
+    T s[18];
+    row_coef_c<T, C>(g, m1, m2, m3, s);
+    T ival = -b[4 * n + C];
+    const int base = ((lz + 1) * hx + (lx + 1)) * hy + (ly + 1);
+#pragma unroll
+    for (int q = 1; q <= Slot<C>::ns; q++) {
+      const int li = base + (Slot<C>::d[q][2] * hx + Slot<C>::d[q][0]) * hy + Slot<C>::d[q][1];
+      ival += s[q] * sx[4 * li + Slot<C>::d[q][3]];
+    }
+    if (C == 0) sx[4 * base] -= ival * g.ec[n] * p0koef;
+    else sx[4 * base + C] -= ival / s[1] * v0koef;
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *__restrict__ flags,
                                                 const T *__restrict__ xold, T *__restrict__ xnew,
@@ -199,38 +236,12 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
   }
   __syncthreads();
   const int nphase = withp ? 7 : 6;
-  const int hty = (ty + 1) / 2;
   for (int ph = 0; ph < nphase; ph++) {
-    const int c = (ph < 6) ? 1 + ph / 2 : 0;
     const int colour = ph & 1;
-    const int ncand = (c == 0) ? ty * tx * tz : hty * tx * tz;
-    for (int i = threadIdx.x; i < ncand; i += blockDim.x) {
-      int ly, lx, lz;
-      if (c == 0) {
-        ly = i % ty; lx = (i / ty) % tx; lz = i / (ty * tx);
-      } else {
-        int kk = i % hty; lx = (i / hty) % tx; lz = i / (hty * tx);
-        int par = (colour - (x0 + lx) - (z0 + lz) - y0) & 1;
-        ly = 2 * kk + par;
-        if (ly >= ty) continue;
-      }
-      int m1 = x0 + lx, m2 = y0 + ly, m3 = z0 + lz;
-      if (m1 >= g.X || m2 >= g.Y || m3 >= g.Z) continue;
-      long n = g.nid(m1, m2, m3);
-      if (kind_of(flags[n], c) != KIND_EQ) continue;
-      T s[18];
-      int ns = row_coef<T>(g, c, m1, m2, m3, s);
-      T ival = -b[4 * n + c];
-      const int base = ((lz + 1) * hx + (lx + 1)) * hy + (ly + 1);
-#pragma unroll
-      for (int q = 1; q <= 17; q++) {
-        if (q > ns) break;
-        int li = base + (kSlot[c][q][2] * hx + kSlot[c][q][0]) * hy + kSlot[c][q][1];
-        ival += s[q] * sx[4 * li + kSlot[c][q][3]];
-      }
-      if (c == 0) sx[4 * base] -= ival * g.ec[n] * p0koef;
-      else sx[4 * base + c] -= ival / s[1] * v0koef;
-    }
+    if (ph < 2) smooth_phase<T, 1>(g, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
+    else if (ph < 4) smooth_phase<T, 2>(g, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
+    else if (ph < 6) smooth_phase<T, 3>(g, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
+    else smooth_phase<T, 0>(g, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
     __syncthreads();
   }
   // write the tile interior
@@ -693,6 +704,29 @@ __global__ void k_pshift(long nn, const unsigned char *__restrict__ flags, doubl
   long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (n >= nn) return;
   if (kind_of(flags[n], 0) != KIND_NONE) x[4 * n] += pinit - *ref;
+}
+
+// Hydrostatic initial guess: velocities zero, P integrated column by column
+// through the z-momentum rows (with v = 0 a z row reads
+// zkf*(P(k) - P(k+1)) = b), the way the reference seeds its first pressure
+// field (parallel/in3mg.c:927-938).  One thread per (m1, m2) column.
+__global__ void k_hydro(Geo<double> g, const unsigned char *__restrict__ flags, const double *__restrict__ b,
+                        double *x) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.X * g.Y) return;
+  int m1 = t / g.Y, m2 = t - m1 * g.Y;
+  for (int m3 = 0; m3 < g.Z; m3++) {
+    long n = g.nid(m1, m2, m3);
+    for (int c = 1; c < 4; c++) x[4 * n + c] = 0.0;
+  }
+  if (m1 + 1 >= g.X || m2 + 1 >= g.Y) return;
+  for (int m3 = 0; m3 + 1 < g.Z; m3++) {
+    long n = g.nid(m1, m2, m3);
+    if (kind_of(flags[n], 3) != KIND_EQ) continue;
+    double zkf = 2.0 * g.iBz[m3];
+    long plo = 4 * g.nid(m1 + 1, m2 + 1, m3), phi = 4 * g.nid(m1 + 1, m2 + 1, m3 + 1);
+    x[phi] = x[plo] - b[4 * n + 3] / zkf;
+  }
 }
 
 }  // namespace stk

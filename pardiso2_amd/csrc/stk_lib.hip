@@ -131,7 +131,7 @@ struct stk_ctx {
   float *nu = nullptr, *ro = nullptr;                  // all levels
   float *nxx = nullptr, *nxy = nullptr, *nxz = nullptr, *nyz = nullptr;
   float *nd = nullptr;                                 // all levels (cell viscosity)
-  bool fields_set = false, prepared = false;
+  bool fields_set = false, prepared = false, rows_dirty = true;
   // coarsest inverse
   int nc = 0;
   double *ainv64 = nullptr, *ainvP64 = nullptr;  // velocity block / all rows
@@ -141,6 +141,7 @@ struct stk_ctx {
   std::vector<double *> Zs, Qs;
   double *red = nullptr, *scal = nullptr;
   int nred = 1024;
+  int cvm = -1;  // coarse-level averaging actually used by the prepared operators
   hipStream_t st = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // compat row store (gausmat2 / pardisomat)
@@ -341,7 +342,7 @@ template <typename T, bool RES>
 void rowop(stk_ctx *c, int l, const T *x, const T *b, T *y, int withp) {
   Level &L = c->lv[l];
   Geo<T> g = L.geo<T>(c->kn.p2koef);
-  hipLaunchKernelGGL((k_rowop<T, RES>), dim3(blocks(4 * L.n, 256)), dim3(256), 0, c->st, g, L.flags, x, b, y, withp);
+  hipLaunchKernelGGL((k_rowop<T, RES>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, g, L.flags, x, b, y, withp);
   if (L.nbc)
     hipLaunchKernelGGL((k_bc_rowop<T, RES>), dim3(blocks(L.nbc, 256)), dim3(256), 0, c->st, L.nbc, L.bc_r,
                        L.bc_j, L.bc_negk, x, b, y, withp);
@@ -374,24 +375,33 @@ template <typename T> const T *ainv(stk_ctx *c);
 template <> const float *ainv<float>(stk_ctx *c) { return c->ainv32; }
 template <> const double *ainv<double>(stk_ctx *c) { return c->ainv64; }
 
-// velocity-block V-cycle; level-0 right-hand side already in lvb<T>(lv[0])
+// velocity-block multigrid cycle on level l from the current x_l (cycle
+// index gamma: 1 V-cycle, 2 W-cycle); shape of the reference's V-cycle
+// (move3mg.c:2710-2792) with the coarsest level solved exactly every visit
+template <typename T>
+void mg_cycle(stk_ctx *c, const stk_solver_opts *o, const int *tile, int l, int gamma) {
+  const int L = c->L;
+  if (l == L) {
+    Level &C = c->lv[L];
+    int n4 = (int)(4 * C.n);
+    hipLaunchKernelGGL((k_gemv<T, T>), dim3(blocks(n4, 4)), dim3(256), 0, c->st, ainv<T>(c), lvb<T>(C), lvx<T>(C), n4);
+    return;
+  }
+  Level &V = c->lv[l];
+  smooth<T>(c, l, o->pre[l], tile, 0);
+  rowop<T, true>(c, l, lvx<T>(V), lvb<T>(V), lvr<T>(V), 0);
+  restrict_to<T>(c, l + 1, lvr<T>(V), lvb<T>(c->lv[l + 1]), lvx<T>(c->lv[l + 1]), 0);
+  const int g = (l + 1 == L) ? 1 : gamma;
+  for (int k = 0; k < g; k++) mg_cycle<T>(c, o, tile, l + 1, gamma);
+  prolong_from<T>(c, l + 1, lvx<T>(c->lv[l + 1]), lvx<T>(V), 0, 0);
+  smooth<T>(c, l, o->post[l], tile, 0);
+}
+
 template <typename T>
 void vcycle_v(stk_ctx *c, const stk_solver_opts *o, const int *tile) {
-  const int L = c->L;
-  for (int l = 0; l < L; l++) {
-    Level &V = c->lv[l];
-    if (l == 0) hipMemsetAsync(lvx<T>(V), 0, 4 * V.n * sizeof(T), c->st);
-    smooth<T>(c, l, o->pre[l], tile, 0);
-    rowop<T, true>(c, l, lvx<T>(V), lvb<T>(V), lvr<T>(V), 0);
-    restrict_to<T>(c, l + 1, lvr<T>(V), lvb<T>(c->lv[l + 1]), lvx<T>(c->lv[l + 1]), 0);
-  }
-  Level &C = c->lv[L];
-  int n4 = (int)(4 * C.n);
-  hipLaunchKernelGGL((k_gemv<T, T>), dim3(blocks(n4, 4)), dim3(256), 0, c->st, ainv<T>(c), lvb<T>(C), lvx<T>(C), n4);
-  for (int l = L - 1; l >= 0; l--) {
-    prolong_from<T>(c, l + 1, lvx<T>(c->lv[l + 1]), lvx<T>(c->lv[l]), 0, 0);
-    smooth<T>(c, l, o->post[l], tile, 0);
-  }
+  Level &V = c->lv[0];
+  hipMemsetAsync(lvx<T>(V), 0, 4 * V.n * sizeof(T), c->st);
+  mg_cycle<T>(c, o, tile, 0, o->cycle == 2 ? 2 : 1);
 }
 
 // z = M^{-1} r : pressure by eta scaling, velocity by one V-cycle
@@ -451,7 +461,7 @@ int ensure_outer(stk_ctx *c, int m) {
   long n4 = 4 * c->lv[0].n;
   if (!c->X) {
     if (dalloc(&c->X, n4) || dalloc(&c->B, n4) || dalloc(&c->R, n4) || dalloc(&c->W2, n4) || dalloc(&c->T1, n4) ||
-        dalloc(&c->red, (size_t)c->nred * 16) || dalloc(&c->scal, 128))
+        dalloc(&c->red, (size_t)c->nred * 16) || dalloc(&c->scal, 256))
       return STK_ENOMEM;
     hipMemset(c->X, 0, n4 * 8);
   }
@@ -626,14 +636,16 @@ int stk_set_knobs(stk_ctx *c, const stk_knobs *k) {
 void stk_default_opts(stk_solver_opts *o, int levels) {
   memset(o, 0, sizeof(*o));
   o->precision = 0;
-  o->restart = 32;
+  o->restart = 128;
   int pre[12] = {2, 4, 8, 16, 32, 64, 64, 64, 64, 64, 64, 64};
   for (int l = 0; l < 12; l++) { o->pre[l] = pre[l]; o->post[l] = pre[l]; }
   (void)levels;
   o->tile[0] = kTileDefault[0]; o->tile[1] = kTileDefault[1]; o->tile[2] = kTileDefault[2];
-  o->omega = 1.0;
+  o->omega = 4.0 / 3.0;  // isoviscous Schur constant of the 2eta(e - div/3) stress form
+  o->cycle = 1;
+  o->coarse_viscmod = -1;
   o->tol = 1e-8;
-  o->maxit = 200;
+  o->maxit = 800;
 }
 
 // boundary grammar line projected on every level (parallel/in3mg.c:299-375)
@@ -670,6 +682,7 @@ int stk_bc_line(stk_ctx *c, int var, int m10, int m11, int m20, int m21, int m30
         }
   }
   c->prepared = false;
+  c->rows_dirty = true;
   return STK_OK;
 }
 
@@ -681,6 +694,7 @@ int stk_set_bc_tables(stk_ctx *c, const int *bondm, int nbond, const float *bond
   for (long r = 0; r < 4 * c->ntot; r++)
     if (c->bondm[r] < 0 || c->bondm[r] > nbond) return STK_EINVAL;
   c->prepared = false;
+  c->rows_dirty = true;
   return STK_OK;
 }
 
@@ -837,23 +851,32 @@ static int build_rows(stk_ctx *c) {
   return STK_OK;
 }
 
-int stk_prepare(stk_ctx *c) {
-  if (!c) return STK_EINVAL;
+int stk_prepare_cvm(stk_ctx *c, int cvm);
+int stk_prepare(stk_ctx *c) { return stk_prepare_cvm(c, -1); }
+
+int stk_prepare_cvm(stk_ctx *c, int cvm) {
+  if (!c || cvm < -1 || cvm > 2) return STK_EINVAL;
+  const int vmc = (cvm < 0) ? c->kn.viscmod : cvm;
   if (!c->fields_set) return STK_ESTATE;
-  int rc = build_rows(c);
-  if (rc) return rc;
+  int rc = STK_OK;
+  if (c->rows_dirty) {
+    rc = build_rows(c);
+    if (rc) return rc;
+    c->rows_dirty = false;
+  }
   // nu/ro restriction to every level (parallel/move3mg.c:2247)
   for (int l = 1; l <= c->L; l++) {
     Level &F = c->lv[l - 1], &C = c->lv[l];
     CompTab t;
     for (int a = 0; a < 3; a++) t.ax[a] = C.ax[3][a].tab();
     hipLaunchKernelGGL(k_ronurestrict, dim3(blocks(C.n, 256)), dim3(256), 0, c->st, F.X, F.Y, C.X, C.Y, C.Z, t,
-                       c->nu + F.off, c->ro + F.off, c->nu + C.off, c->ro + C.off, c->kn.viscmod);
+                       c->nu + F.off, c->ro + F.off, c->nu + C.off, c->ro + C.off, vmc);
   }
   for (int l = 0; l <= c->L; l++) {
     Level &L = c->lv[l];
     hipLaunchKernelGGL(k_visc_prep, dim3(blocks(L.n, 256)), dim3(256), 0, c->st, L.X, L.Y, L.Z, l == 0 ? 1 : 0,
-                       c->nu + L.off, c->nxx, c->nxy, c->nxz, c->nyz, c->kn.nubeg, c->kn.nuend, c->kn.viscmod,
+                       c->nu + L.off, c->nxx, c->nxy, c->nxz, c->nyz, c->kn.nubeg, c->kn.nuend,
+                       l == 0 ? c->kn.viscmod : vmc,
                        L.e64[0], L.e64[1], L.e64[2], L.e64[3], L.e64[4], L.e32[0], L.e32[1], L.e32[2], L.e32[3],
                        L.e32[4]);
     // nd = cell viscosity of the continuity stencil (continerr, move3mg.c:1549)
@@ -879,6 +902,7 @@ int stk_prepare(stk_ctx *c) {
   dfree(c->ainvP64);
   HIPCHK(hipStreamSynchronize(c->st));
   c->prepared = true;
+  c->cvm = vmc;
   return STK_OK;
 }
 
@@ -920,10 +944,79 @@ int stk_pressure_gauge(stk_ctx *c, int px, int py, int pz, double pinit) {
   return STK_OK;
 }
 
+int stk_hydrostatic_guess(stk_ctx *c) {
+  if (!c || !c->prepared) return STK_ESTATE;
+  Level &F = c->lv[0];
+  long n4 = 4 * F.n;
+  HIPCHK(hipMemsetAsync(c->X, 0, n4 * 8, c->st));
+  Geo<double> g = F.geo<double>(c->kn.p2koef);
+  hipLaunchKernelGGL(k_hydro, dim3(blocks((long)F.X * F.Y, 64)), dim3(64), 0, c->st, g, F.flags, c->B, c->X);
+  bc_passes<double>(c, F, c->B, c->X, 0);
+  HIPCHK(hipStreamSynchronize(c->st));
+  return STK_OK;
+}
+
+// average device time of one level-0 kernel launch, measured with HIP events
+// on the library stream: which = 0 FP32 smoother sweep (velocity block),
+// 1 FP64 smoother sweep, 2 FP64 operator apply (A x, all rows),
+// 3 FP32 residual, 4 FP64 residual (b - A x)
+int stk_time_kernel(stk_ctx *c, int which, int level, int reps, double *ms_avg) {
+  if (!c || !c->prepared || level < 0 || level > c->L || reps < 1 || !ms_avg) return STK_EINVAL;
+  int rc = ensure_vectors(c, true, true);
+  if (rc) return rc;
+  Level &L = c->lv[level];
+  int tile[3] = {kTileDefault[0], kTileDefault[1], kTileDefault[2]};
+  int ty = tile[0], tx = tile[1], tz = tile[2];
+  int nty = (L.Y + ty - 1) / ty, ntx = (L.X + tx - 1) / tx, ntz = (L.Z + tz - 1) / tz;
+  auto launch = [&]() {
+    if (which == 0) {
+      Geo<float> g = L.geo<float>(c->kn.p2koef);
+      hipLaunchKernelGGL(k_smooth<float>, dim3(nty * ntx * ntz), dim3(256), (size_t)(ty + 2) * (tx + 2) * (tz + 2) * 16,
+                         c->st, g, L.flags, L.x32[0], L.x32[1], L.b32, ty, tx, tz, ntx, nty, 0, (float)c->kn.v0koef,
+                         (float)c->kn.p0koef);
+    } else if (which == 1) {
+      Geo<double> g = L.geo<double>(c->kn.p2koef);
+      hipLaunchKernelGGL(k_smooth<double>, dim3(nty * ntx * ntz), dim3(256), (size_t)(ty + 2) * (tx + 2) * (tz + 2) * 32,
+                         c->st, g, L.flags, L.x64[0], L.x64[1], L.b64, ty, tx, tz, ntx, nty, 0, c->kn.v0koef,
+                         c->kn.p0koef);
+    } else if (which == 2) {
+      Geo<double> g = L.geo<double>(c->kn.p2koef);
+      hipLaunchKernelGGL((k_rowop<double, false>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, g, L.flags,
+                         L.x64[0], nullptr, L.r64, 1);
+    } else if (which == 3) {
+      Geo<float> g = L.geo<float>(c->kn.p2koef);
+      hipLaunchKernelGGL((k_rowop<float, true>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, g, L.flags,
+                         L.x32[0], L.b32, L.r32, 0);
+    } else {
+      Geo<double> g = L.geo<double>(c->kn.p2koef);
+      hipLaunchKernelGGL((k_rowop<double, true>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, g, L.flags,
+                         L.x64[0], L.b64, L.r64, 1);
+    }
+  };
+  launch();
+  HIPCHK(hipEventRecord(c->ev0, c->st));
+  for (int i = 0; i < reps; i++) launch();
+  HIPCHK(hipEventRecord(c->ev1, c->st));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *ms_avg = ms / reps;
+  return STK_OK;
+}
+
 int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   if (!c || !o) return STK_EINVAL;
   if (!c->prepared) return STK_ESTATE;
-  if (o->restart < 1 || o->restart > 64 || o->maxit < 1 || !(o->tol > 0)) return STK_EINVAL;
+  if (o->restart < 1 || o->restart > 128 || o->maxit < 1 || !(o->tol > 0) || o->cycle < 0 || o->cycle > 2 ||
+      o->coarse_viscmod < -1 || o->coarse_viscmod > 2)
+    return STK_EINVAL;
+  {
+    const int want = (o->coarse_viscmod < 0) ? c->kn.viscmod : o->coarse_viscmod;
+    if (want != c->cvm) {
+      int rc0 = stk_prepare_cvm(c, want);
+      if (rc0) return rc0;
+    }
+  }
   int tile[3] = {o->tile[0] ? o->tile[0] : kTileDefault[0], o->tile[1] ? o->tile[1] : kTileDefault[1],
                  o->tile[2] ? o->tile[2] : kTileDefault[2]};
   if (tile[0] < 2 || tile[1] < 1 || tile[2] < 1 || (size_t)(tile[0] + 2) * (tile[1] + 2) * (tile[2] + 2) * 32 > 65536)

@@ -193,6 +193,60 @@ __device__ __forceinline__ int row_coef(const Geo<T> &g, int c, int m1, int m2, 
   return 6;
 }
 
+// compile-time slot tables: Slot<C>::d[q] = {dm1, dm2, dm3, comp}
+template <int C> struct Slot;
+template <> struct Slot<0> {
+  static constexpr int ns = 6;
+  static constexpr signed char d[18][4] = {
+      {0, 0, 0, 0}, {-1, -1, -1, 1}, {0, -1, -1, 1}, {-1, -1, -1, 2}, {-1, 0, -1, 2}, {-1, -1, -1, 3},
+      {-1, -1, 0, 3}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0},
+      {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+};
+template <> struct Slot<1> {
+  static constexpr int ns = 17;
+  static constexpr signed char d[18][4] = {
+      {0, 0, 0, 0}, {0, 0, 0, 1}, {-1, 0, 0, 1}, {1, 0, 0, 1}, {0, -1, 0, 1}, {0, 1, 0, 1}, {0, 0, -1, 1},
+      {0, 0, 1, 1}, {-1, 0, 0, 2}, {-1, 1, 0, 2}, {0, 0, 0, 2}, {0, 1, 0, 2}, {-1, 0, 0, 3}, {-1, 0, 1, 3},
+      {0, 0, 0, 3}, {0, 0, 1, 3}, {0, 1, 1, 0}, {1, 1, 1, 0}};
+};
+template <> struct Slot<2> {
+  static constexpr int ns = 17;
+  static constexpr signed char d[18][4] = {
+      {0, 0, 0, 0}, {0, 0, 0, 2}, {-1, 0, 0, 2}, {1, 0, 0, 2}, {0, -1, 0, 2}, {0, 1, 0, 2}, {0, 0, -1, 2},
+      {0, 0, 1, 2}, {0, -1, 0, 1}, {0, 0, 0, 1}, {1, -1, 0, 1}, {1, 0, 0, 1}, {0, -1, 0, 3}, {0, 0, 0, 3},
+      {0, -1, 1, 3}, {0, 0, 1, 3}, {1, 0, 1, 0}, {1, 1, 1, 0}};
+};
+template <> struct Slot<3> {
+  static constexpr int ns = 17;
+  static constexpr signed char d[18][4] = {
+      {0, 0, 0, 0}, {0, 0, 0, 3}, {-1, 0, 0, 3}, {1, 0, 0, 3}, {0, -1, 0, 3}, {0, 1, 0, 3}, {0, 0, -1, 3},
+      {0, 0, 1, 3}, {0, 0, -1, 2}, {0, 1, -1, 2}, {0, 0, 0, 2}, {0, 1, 0, 2}, {0, 0, -1, 1}, {0, 0, 0, 1},
+      {1, 0, -1, 1}, {1, 0, 0, 1}, {1, 1, 0, 0}, {1, 1, 1, 0}};
+};
+
+template <typename T, int C>
+__device__ __forceinline__ void row_coef_c(const Geo<T> &g, int m1, int m2, int m3, T s[18]) {
+  if (C == 1) row_x<T>(g, m1, m2, m3, s);
+  else if (C == 2) row_y<T>(g, m1, m2, m3, s);
+  else if (C == 3) row_z<T>(g, m1, m2, m3, s);
+  else row_p<T>(g, m1, m2, m3, s);
+}
+
+// sum over the slots of a row against an AoS vector x (4 per node):
+// RES: b - sum s*x in slot order, otherwise sum s*x
+template <typename T, int C, bool RES>
+__device__ __forceinline__ T row_dot(const Geo<T> &g, const T s[18], const T *__restrict__ x, long n, T b) {
+  T acc = RES ? b : T(0);
+#pragma unroll
+  for (int q = 1; q <= Slot<C>::ns; q++) {
+    const long uq = 4 * (n + (long)Slot<C>::d[q][2] * g.XY + (long)Slot<C>::d[q][0] * g.Y + Slot<C>::d[q][1]) +
+                    Slot<C>::d[q][3];
+    if (RES) acc -= s[q] * x[uq];
+    else acc += s[q] * x[uq];
+  }
+  return acc;
+}
+
 // unknown kind flags: two bits per component in one byte per node
 enum { KIND_NONE = 0, KIND_EQ = 1, KIND_BC = 2 };
 __host__ __device__ __forceinline__ int kind_of(unsigned char f, int c) { return (f >> (2 * c)) & 3; }

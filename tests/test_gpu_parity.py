@@ -1,0 +1,215 @@
+"""Device kernels against the CPU oracle on the same inputs, through the C ABI.
+
+Bit-exact where the reference's arithmetic order is sequential and the
+device repeats it (marker averaging, viscosity restriction, FP64 residual,
+coloured smoother sweep, restriction, prolongation); within 1e-12 relative
+otherwise.
+"""
+import numpy as np
+import pytest
+
+from pardiso2_amd import _lib as S
+from pardiso2_amd import setups
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(oracle_mod, n=17, L=2, p2=0.1, eta_in=1e21, viscmod=0, seed=1):
+    pb = setups.sinker(n, L, eta_in=eta_in, eta_out=1e20, seed=seed)
+    pb.bc = setups.free_slip_lines(n, n, n, gauge=None)
+    pb.knobs.update(dict(p2koef=p2, viscmod=viscmod))
+    o = oracle_mod.from_problem(pb)
+    g = S.from_problem(pb)
+    return pb, o, g
+
+
+@pytest.fixture(scope="module")
+def pair(oracle_mod):
+    pb, o, g = _pair(oracle_mod)
+    yield pb, o, g
+    g.close()
+
+
+def test_marker_averaging_bit_exact(pair):
+    pb, o, g = pair
+    f = g.fields()
+    for k in ("nu", "ro", "nxx", "nxy", "nxz", "nyz"):
+        a = f[k]
+        np.testing.assert_array_equal(a, o.arr(k)[: a.size], err_msg=k)
+
+
+def test_viscosity_restriction_bit_exact_and_nd(pair):
+    pb, o, g = pair
+    f = g.fields()
+    n0 = o.info[0][4]
+    np.testing.assert_array_equal(f["nu"][n0:], o.arr("nu")[n0:])
+    np.testing.assert_array_equal(f["ro"][n0:], o.arr("ro")[n0:])
+    # nd (cell viscosity of the continuity rows) on every continuity row
+    num0, bondm = o.arr("num0"), o.arr("bondm")
+    cont = np.where((num0[0::4] > 0) & (bondm[0::4] == 0))[0]
+    np.testing.assert_array_equal(f["nd"][cont], o.arr("nd")[cont])
+
+
+def test_fp64_residual_matches_oracle(pair):
+    pb, o, g = pair
+    n4 = 4 * o.info[0][4]
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(n4) * 1e-15
+    o.arr("sol0")[:n4] = x
+    o.residual(0)
+    r_or = o.arr("fre0")[:n4].copy()
+    g.vec_set(S.SOL0, 0, x)
+    g.vec_set(S.FRE1, 0, o.arr("fre1")[:n4].copy())
+    g.smooth(0, 0)
+    r_g = g.vec_get(S.FRE0, 0)
+    rows = o.arr("num0")[:n4] > 0
+    np.testing.assert_allclose(r_g[rows], r_or[rows], rtol=1e-12, atol=1e-12 * np.abs(r_or).max())
+    assert np.mean(r_g[rows] == r_or[rows]) > 0.999  # same summation order: bit-identical
+
+
+def test_operator_apply_matches_assembled_rows(pair):
+    pb, o, g = pair
+    n4 = 4 * o.info[0][4]
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(n4)
+    y = g.apply(0, x)
+    num0, pos0, lin0, val1 = o.arr("num0"), o.arr("pos0"), o.arr("lin0"), o.arr("val1")
+    yo = np.zeros(n4)
+    for r in range(n4):
+        k = num0[r]
+        if k:
+            p = pos0[r]
+            yo[r] = val1[p:p + k] @ x[lin0[p:p + k]]
+    np.testing.assert_allclose(y, yo, rtol=1e-12, atol=1e-12 * np.abs(yo).max())
+
+
+@pytest.mark.parametrize("tile", [(32, 4, 4), (8, 2, 3)])
+def test_coloured_sweep_matches_oracle(oracle_mod, tile):
+    pb, o, g = _pair(oracle_mod, seed=3)
+    try:
+        n4 = 4 * o.info[0][4]
+        rng = np.random.default_rng(4)
+        x = rng.standard_normal(n4) * 1e-16
+        o.arr("sol0")[:n4] = x
+        g.vec_set(S.SOL0, 0, x)
+        g.vec_set(S.FRE1, 0, o.arr("fre1")[:n4].copy())
+        o.sweep_tiles(3, 0, *tile)
+        g.smooth(0, 3, tile=tile)
+        xo = o.arr("sol0")[:n4]
+        xg = g.vec_get(S.SOL0, 0)
+        np.testing.assert_allclose(xg, xo, rtol=1e-12, atol=1e-12 * np.abs(xo).max())
+        fo = o.arr("fre0")[:n4]
+        fg = g.vec_get(S.FRE0, 0)
+        rows = o.arr("num0")[:n4] > 0
+        np.testing.assert_allclose(fg[rows], fo[rows], rtol=1e-10, atol=1e-10 * np.abs(fo).max())
+    finally:
+        g.close()
+
+
+def test_restriction_and_prolongation_bit_exact(pair):
+    pb, o, g = pair
+    g.bind()
+    L = S.lib()
+    n4 = 4 * o.info[0][4]
+    rng = np.random.default_rng(5)
+    r = rng.standard_normal(n4)
+    o.arr("fre0")[:n4] = r
+    g.vec_set(S.FRE0, 0, r)
+    o.vprestrict(1)
+    L.vprestrict(1)
+    b1, n41 = 4 * o.info[1][5], 4 * o.info[1][4]
+    np.testing.assert_array_equal(g.vec_get(S.FRE1, 1), o.arr("fre1")[b1:b1 + n41])
+    xc = rng.standard_normal(n41)
+    xf = rng.standard_normal(n4)
+    o.arr("sol0")[b1:b1 + n41] = xc
+    o.arr("sol0")[:n4] = xf
+    g.vec_set(S.SOL0, 1, xc)
+    g.vec_set(S.SOL0, 0, xf)
+    o.vpprolong(1, 0)
+    L.vpprolong(1, 0)
+    np.testing.assert_array_equal(g.vec_get(S.SOL0, 0), o.arr("sol0")[:n4])
+
+
+def test_ronurestrict_entry_point(pair):
+    pb, o, g = pair
+    g.bind()
+    before = g.fields()["nu"].copy()
+    S.lib().ronurestrict(1)
+    after = g.fields()["nu"]
+    np.testing.assert_array_equal(after, before)  # recomputing level 1 from level 0 is idempotent
+
+
+def test_coarse_solve_matches_dense_lu(oracle_mod):
+    n = 13
+    pb = setups.sinker(n, 1, eta_in=1e21, eta_out=1e20)
+    pb.bc = setups.free_slip_lines(n, n, n, gauge=(2, 2, 2))
+    o = oracle_mod.from_problem(pb)
+    g = S.from_problem(pb)
+    try:
+        L = pb.levels
+        bL, nL = 4 * o.info[L][5], 4 * o.info[L][4]
+        rng = np.random.default_rng(7)
+        rhs = rng.standard_normal(nL) * (o.arr("num0")[bL:bL + nL] > 0)
+        o.arr("fre1")[bL:bL + nL] = rhs
+        o.coarse_solve()
+        g.vec_set(S.FRE1, L, rhs)
+        g.coarse_solve()
+        xo = o.arr("sol0")[bL:bL + nL]
+        xg = g.vec_get(S.SOL0, L)
+        np.testing.assert_allclose(xg, xo, rtol=1e-9, atol=1e-9 * np.abs(xo).max())
+    finally:
+        g.close()
+
+
+def test_pardisomat_and_gausmat2_entry_points(oracle_mod):
+    """Rows handed over in the reference's un1/ui1 format (pardisomat(1,..)),
+    then pardisomat(0,..) solves the coarsest system on the device and stores
+    the solution at sol0[4*(mgp[L]+node)+c]."""
+    import ctypes as C
+    n = 13
+    pb = setups.sinker(n, 1, eta_in=1e21, eta_out=1e20)
+    pb.bc = setups.free_slip_lines(n, n, n, gauge=(2, 2, 2))
+    o = oracle_mod.from_problem(pb)
+    g = S.from_problem(pb)
+    try:
+        g.bind()
+        L = pb.levels
+        bL, nL = 4 * o.info[L][5], 4 * o.info[L][4]
+        num0 = o.arr("num0")
+        un1 = (C.c_long * 64)()
+        ui1 = (C.c_double * 64)()
+        rng = np.random.default_rng(8)
+        rhs = rng.standard_normal(nL)
+        for i in range(nL):
+            r = bL + i
+            if num0[r] == 0:
+                un1[0], un1[1], ui1[0], ui1[1] = 1, i, 0.0, 1.0
+                rhs[i] = 0.0
+            else:
+                idx, val, _ = o.row(r)
+                un1[0] = len(idx)
+                for k, (j, v) in enumerate(zip(idx - bL, val)):
+                    un1[k + 1] = int(j)
+                    ui1[k + 1] = float(v)
+                ui1[0] = rhs[i]
+            assert S.lib().pardisomat(1, i, r, 0, 0, un1, ui1) == 0
+            assert S.lib().gausmat2(1, r, 57 * (r // 4) + (0, 6, 23, 40)[r % 4], un1, ui1) == 0
+        assert S.lib().pardisomat(0, nL, 0, 0, 0, un1, ui1) == 0
+        o.arr("fre1")[bL:bL + nL] = rhs
+        o.coarse_solve()
+        np.testing.assert_allclose(g.vec_get(S.SOL0, L), o.arr("sol0")[bL:bL + nL], rtol=1e-9,
+                                   atol=1e-9 * np.abs(o.arr("sol0")[bL:bL + nL]).max())
+    finally:
+        g.close()
+
+
+def test_small_dense_elimination_entry_points():
+    import ctypes as C
+    L = S.lib()
+    a = np.array([[4.0, 1.0, 0.0, 1.0], [2.0, 5.0, 1.0, 2.0], [0.0, 1.0, 3.0, 3.0]])  # 3 x (3+1)
+    m = np.ascontiguousarray(a.copy())
+    x = np.zeros(3)
+    L.gausmat(3, m.ctypes.data_as(C.POINTER(C.c_double)), x.ctypes.data_as(C.POINTER(C.c_double)))
+    np.testing.assert_allclose(a[:, :3] @ x, a[:, 3], rtol=1e-13)
+    L.matclear(3, m.ctypes.data_as(C.POINTER(C.c_double)))
+    assert not m.any()
