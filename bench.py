@@ -32,6 +32,13 @@ SMOOTHER_BYTES_PER_NODE = 64.0   # FP32 sweep: x(16) + b(16) + 4 viscosities(16)
 RESIDUAL_BYTES_PER_NODE = 112.0  # FP64 residual: x(32) + b(32) + 4 viscosities(16) read, r(32) written
 
 
+KERNELS = {"smooth32": 0, "smooth64": 1, "apply64": 2, "res32": 3, "res64": 4, "copy": 5}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def make_problem(workload: str, n: int, seed: int):
     from pardiso2_amd import setups
     L = setups.coarsest_levels(n)
@@ -87,6 +94,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-ref", action="store_true")
+    ap.add_argument("--kernel", choices=sorted(KERNELS), default=None,
+                    help="only launch one finest-level kernel --reps times (for counter runs) and exit")
+    ap.add_argument("--reps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
@@ -98,6 +108,11 @@ def main():
     L = pb.levels
     pre, post = sweeps_for(L)
     ctx = S.from_problem(pb)
+    log(f"{args.workload} {args.n}^3 ready on the device ({ctx.n0} nodes, {L + 1} levels)")
+    if args.kernel:
+        ms = ctx.time_kernel(KERNELS[args.kernel], 0, reps=args.reps)
+        log(f"kernel {args.kernel}: {args.reps} launches, {ms:.4f} ms each")
+        return
     opts = S.default_opts(L, precision=args.precision, pre=pre, post=post, tol=1e-8, maxit=args.maxit,
                           restart=args.restart)
 
@@ -106,8 +121,9 @@ def main():
         ctx.hydrostatic_guess()
         return ctx.solve(opts)
 
-    for _ in range(args.warmup):
-        step()
+    for i in range(args.warmup):
+        st = step()
+        log(f"warmup {i}: {st.iterations} iterations, relres {st.relres:.3e}, status {st.status}")
     lat, its, rels, solve_s = [], [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -117,6 +133,7 @@ def main():
         its.append(st.iterations)
         rels.append(st.relres)
         solve_s.append(st.seconds)
+        log(f"step: {st.iterations} iterations, relres {st.relres:.3e}, {lat[-1] * 1e3:.1f} ms")
         if st.status != 0:
             print(f"warning: solve status {st.status} relres {st.relres:.3e}", file=sys.stderr)
     total = time.perf_counter() - t0
@@ -129,15 +146,12 @@ def main():
     gbs_sm = SMOOTHER_BYTES_PER_NODE * n0 / (ms_sm * 1e-3) / 1e9
     gbs_res = RESIDUAL_BYTES_PER_NODE * n0 / (ms_res * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(HERE, "profiles", "k_smooth_float_traffic.json")
+    tf = os.path.join(HERE, "profiles", f"{args.workload}{args.n}_finest_kernels_pmc.json")
     if os.path.exists(tf):
-        try:
-            with open(tf) as f:
-                tj = json.load(f)
-            if tj.get("nodes") == n0:
-                traffic = tj.get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+        with open(tf) as f:
+            tj = json.load(f)
+        if tj.get("nodes") == n0 and "smooth32" in tj.get("kernels", {}):
+            traffic = tj["kernels"]["smooth32"]["hbm_bytes_per_launch"]
 
     fp64_its = None
     if not args.no_fp64_ref:
@@ -146,9 +160,10 @@ def main():
         ctx.hydrostatic_guess()
         s64 = ctx.solve(o64)
         fp64_its = s64.iterations
-        fp64_s = s64.seconds
+        log(f"fp64 reference solve: {s64.iterations} iterations, relres {s64.relres:.3e}, {s64.seconds:.2f} s")
 
     cpu = None
+    log(f"smoother {ms_sm:.4f} ms ({gbs_sm:.0f} GB/s), fp64 residual {ms_res:.4f} ms ({gbs_res:.0f} GB/s)")
     if not args.no_cpu:
         try:
             cpu = cpu_baseline(pb, pre, post, fp64_its or its[-1], args.cpu_budget)

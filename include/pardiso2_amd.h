@@ -56,12 +56,16 @@ typedef struct {
   int pre[12];       /* pre-smoothing sweeps per level */
   int post[12];      /* post-smoothing sweeps per level */
   int tile[3];       /* smoother tile (y, x, z) in nodes; 0 = default */
-  double omega;      /* pressure block scale of the preconditioner */
+  double omega;      /* pressure block scale of the preconditioner (schur = 0) */
   int cycle;         /* multigrid cycle index: 1 V-cycle, 2 W-cycle */
   int coarse_viscmod; /* averaging of coarse-level viscosities: -1 = knobs.viscmod, 0/1/2 */
   double tol;        /* target relative residual ||W r|| / ||W b|| */
   int maxit;         /* outer iteration cap */
   int verbose;
+  int schur;         /* pressure Schur approximation: 0 omega*eta_cell scaling,
+                        1 weighted BFBt with one aggregation-multigrid V-cycle per K solve */
+  int ksweeps;       /* red/black Gauss-Seidel sweeps before and after each K-cycle level */
+  double kscale;     /* coarse-correction scale of the aggregation K-cycle */
 } stk_solver_opts;
 
 typedef struct {
@@ -125,7 +129,8 @@ int stk_relres(stk_ctx *ctx, double *relres, double blocks[3]);
 int stk_hydrostatic_guess(stk_ctx *ctx);
 /* Average device time (ms) of one launch of a finest-level kernel, HIP events
  * on the library stream.  which: 0 FP32 smoother sweep, 1 FP64 smoother
- * sweep, 2 FP64 operator apply, 3 FP32 residual, 4 FP64 residual. */
+ * sweep, 2 FP64 operator apply, 3 FP32 residual, 4 FP64 residual, 5 plain copy
+ * of 4*mgn[l] doubles (32 B read + 32 B written per node). */
 int stk_time_kernel(stk_ctx *ctx, int which, int level, int reps, double *ms_avg);
 /* Shift P so that P(px,py,pz) = pinit (parallel/move3mg.c:2835-2856). */
 int stk_pressure_gauge(stk_ctx *ctx, int px, int py, int pz, double pinit);
@@ -142,6 +147,17 @@ int stk_apply(stk_ctx *ctx, int level, const double *x, double *y);
  * level, FP64, all rows (pressure Uzawa step included), then fre0. */
 int stk_smooth(stk_ctx *ctx, int level, int sweeps, int ty, int tx, int tz);
 int stk_coarse_solve(stk_ctx *ctx);
+
+/* ---- Schur approximation, for inspection ------------------------------ */
+/* y = K_l x with K_l the pressure-cell operator of level l (level 0:
+ * B D^-1 G, coarser: aggregated Galerkin), cell vectors of mgn[l] doubles */
+int stk_kop_apply(stk_ctx *ctx, int level, const double *x, double *y);
+/* x = one K-cycle applied to b (level-0 cell vectors), FP32 (precision 0) or FP64 */
+int stk_kcycle_apply(stk_ctx *ctx, int precision, int ksweeps, double kscale, const double *b, double *x);
+/* pressure slots of z = the preconditioner's Schur block applied to the
+ * pressure slots of r (opts->schur 1 or 2), before the boundary-row
+ * substitution; velocity slots of z are 0 */
+int stk_schur_apply(stk_ctx *ctx, const stk_solver_opts *opts, const double *r, double *z);
 
 /* ---- reference entry points (act on the bound context) ---------------- */
 int stk_bind(stk_ctx *ctx);

@@ -23,7 +23,7 @@ EXPORTS = [
     "stk_prepare", "stk_set_solution", "stk_get_solution", "stk_solve", "stk_relres",
     "stk_pressure_gauge", "stk_hydrostatic_guess", "stk_time_kernel", "stk_vec_set", "stk_vec_get", "stk_apply", "stk_smooth", "stk_coarse_solve",
     "stk_bind", "matclear", "gausmat", "gausmat2", "pardisomat", "gausmat4", "vprestrict", "vpprolong",
-    "ronurestrict", "vpiterate",
+    "ronurestrict", "vpiterate", "stk_kop_apply", "stk_kcycle_apply", "stk_schur_apply",
 ]
 
 
@@ -38,7 +38,8 @@ class Opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("restart", C.c_int), ("pre", C.c_int * 12),
                 ("post", C.c_int * 12), ("tile", C.c_int * 3), ("omega", C.c_double),
                 ("cycle", C.c_int), ("coarse_viscmod", C.c_int),
-                ("tol", C.c_double), ("maxit", C.c_int), ("verbose", C.c_int)]
+                ("tol", C.c_double), ("maxit", C.c_int), ("verbose", C.c_int),
+                ("schur", C.c_int), ("ksweeps", C.c_int), ("kscale", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -89,6 +90,9 @@ def lib():
         L.stk_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.stk_coarse_solve.argtypes = [vp]
         L.stk_bind.argtypes = [vp]
+        L.stk_kop_apply.argtypes = [vp, C.c_int, dp, dp]
+        L.stk_kcycle_apply.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, dp]
+        L.stk_schur_apply.argtypes = [vp, C.POINTER(Opts), dp, dp]
         L.matclear.argtypes = [C.c_int, dp]
         L.matclear.restype = None
         L.gausmat.argtypes = [C.c_int, dp, dp]
@@ -260,6 +264,27 @@ class Stokes:
 
     def bind(self):
         check(lib().stk_bind(self.h), "stk_bind")
+
+    def kop_apply(self, level, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.size == self.info[level][4]
+        y = np.zeros_like(x)
+        check(lib().stk_kop_apply(self.h, level, _dp(x), _dp(y)), "stk_kop_apply")
+        return y
+
+    def kcycle_apply(self, b, precision=1, ksweeps=2, kscale=2.0):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        assert b.size == self.n0
+        x = np.zeros_like(b)
+        check(lib().stk_kcycle_apply(self.h, precision, ksweeps, kscale, _dp(b), _dp(x)), "stk_kcycle_apply")
+        return x
+
+    def schur_apply(self, r, opts=None, **kw):
+        o = opts if opts is not None else default_opts(self.levels, **kw)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros_like(r)
+        check(lib().stk_schur_apply(self.h, C.byref(o), _dp(r), _dp(z)), "stk_schur_apply")
+        return z
 
 
 SOL0, FRE1, FRE0 = 0, 1, 2

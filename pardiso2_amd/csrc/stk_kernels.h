@@ -699,6 +699,12 @@ __global__ void k_convert(long n, const TS *__restrict__ a, TD *b) {
   if (u < n) b[u] = (TD)a[u];
 }
 
+// plain device copy: the known-size kernel that HBM byte counters are scaled by
+__global__ void __launch_bounds__(256) k_stream_copy(long n, const double *__restrict__ a, double *__restrict__ b) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u < n) b[u] = a[u];
+}
+
 __global__ void k_pshift(long nn, const unsigned char *__restrict__ flags, double *x, const double *ref,
                          double pinit) {
   long n = blockIdx.x * (long)blockDim.x + threadIdx.x;

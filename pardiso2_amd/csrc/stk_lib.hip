@@ -14,6 +14,7 @@
 #include "../../include/pardiso2_amd.h"
 #include "stk_kernels.h"
 #include "stk_markers.h"
+#include "stk_schur.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -92,6 +93,13 @@ struct Level {
   double *ref[3] = {};
   // transfer from level-1 (kinds N, C, P, R per axis)
   AxisDev ax[4][3];
+  // pressure-cell operator K of the Schur approximation and its K-cycle vectors
+  double *k64[7] = {};
+  float *k32[7] = {};
+  float *kx32 = nullptr, *kb32 = nullptr, *kr32 = nullptr;
+  double *kx64 = nullptr, *kb64 = nullptr, *kr64 = nullptr;
+  int *klo[3] = {}, *kcnt[3] = {};  // cells of level-1 owned by each cell of this level, per axis
+  int *kpar[3] = {};                // cell of level+1 owning each cell of this level, per axis
   template <typename T> Geo<T> geo(double p2) const;
 };
 
@@ -144,6 +152,14 @@ struct stk_ctx {
   int cvm = -1;  // coarse-level averaging actually used by the prepared operators
   hipStream_t st = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Schur approximation: level-0 velocity D^-1, work vectors, coarsest K inverse
+  double *dinv64 = nullptr, *bfy64 = nullptr, *bfw64 = nullptr;
+  float *dinv32 = nullptr, *bfy32 = nullptr, *bfw32 = nullptr;
+  double *kinv64 = nullptr;
+  float *kinv32 = nullptr;
+  int *kcell = nullptr;
+  int nkc = 0;
+  bool ktabs = false;
   // compat row store (gausmat2 / pardisomat)
   std::vector<double> hval1, hfre1;
   std::vector<int> hlin0, hnum0, hpos0;
@@ -404,12 +420,111 @@ void vcycle_v(stk_ctx *c, const stk_solver_opts *o, const int *tile) {
   mg_cycle<T>(c, o, tile, 0, o->cycle == 2 ? 2 : 1);
 }
 
-// z = M^{-1} r : pressure by eta scaling, velocity by one V-cycle
+// ---------------------------------------------------------------------------
+// Schur approximation (weighted BFBt) and its aggregation K-cycle
+// ---------------------------------------------------------------------------
+template <typename T> KPlanesC<T> kplanes(const Level &L);
+template <> KPlanesC<float> kplanes<float>(const Level &L) {
+  KPlanesC<float> k;
+  for (int q = 0; q < 7; q++) k.p[q] = L.k32[q];
+  return k;
+}
+template <> KPlanesC<double> kplanes<double>(const Level &L) {
+  KPlanesC<double> k;
+  for (int q = 0; q < 7; q++) k.p[q] = L.k64[q];
+  return k;
+}
+template <typename T> T *kx(Level &L);
+template <> float *kx<float>(Level &L) { return L.kx32; }
+template <> double *kx<double>(Level &L) { return L.kx64; }
+template <typename T> T *kb(Level &L);
+template <> float *kb<float>(Level &L) { return L.kb32; }
+template <> double *kb<double>(Level &L) { return L.kb64; }
+template <typename T> T *kr(Level &L);
+template <> float *kr<float>(Level &L) { return L.kr32; }
+template <> double *kr<double>(Level &L) { return L.kr64; }
+template <typename T> T *bfy(stk_ctx *c);
+template <> float *bfy<float>(stk_ctx *c) { return c->bfy32; }
+template <> double *bfy<double>(stk_ctx *c) { return c->bfy64; }
+template <typename T> T *bfw(stk_ctx *c);
+template <> float *bfw<float>(stk_ctx *c) { return c->bfw32; }
+template <> double *bfw<double>(stk_ctx *c) { return c->bfw64; }
+template <typename T> const T *dinvt(stk_ctx *c);
+template <> const float *dinvt<float>(stk_ctx *c) { return c->dinv32; }
+template <> const double *dinvt<double>(stk_ctx *c) { return c->dinv64; }
+template <typename T> const T *kinv(stk_ctx *c);
+template <> const float *kinv<float>(stk_ctx *c) { return c->kinv32; }
+template <> const double *kinv<double>(stk_ctx *c) { return c->kinv64; }
+template <typename T> const T *iAt(const Level &L, int a);
+template <> const float *iAt<float>(const Level &L, int a) { return L.iA32[a]; }
+template <> const double *iAt<double>(const Level &L, int a) { return L.iA64[a]; }
+template <typename T> const T *iBt(const Level &L, int a);
+template <> const float *iBt<float>(const Level &L, int a) { return L.iB32[a]; }
+template <> const double *iBt<double>(const Level &L, int a) { return L.iB64[a]; }
+
+KAgg kagg(const Level &C) {
+  KAgg t;
+  for (int a = 0; a < 3; a++) { t.lo[a] = C.klo[a]; t.cnt[a] = C.kcnt[a]; }
+  return t;
+}
+
+// one aggregation V-cycle for K x = b on level l (x_l = 0 on entry)
+template <typename T>
+void kcycle(stk_ctx *c, int l, int nu, T scale) {
+  Level &V = c->lv[l];
+  if (l == c->L) {
+    hipLaunchKernelGGL(k_kcoarse<T>, dim3(blocks(c->nkc, 64)), dim3(64), 0, c->st, c->nkc, c->kcell, kinv<T>(c),
+                       kb<T>(V), kx<T>(V));
+    return;
+  }
+  const KPlanesC<T> k = kplanes<T>(V);
+  const unsigned nb = blocks(V.n, 256);
+  for (int s = 0; s < nu; s++)
+    for (int col = 0; col < 2; col++)
+      hipLaunchKernelGGL(k_kgs<T>, dim3(nb), dim3(256), 0, c->st, V.X, V.Y, V.Z, k, kb<T>(V), kx<T>(V), col);
+  hipLaunchKernelGGL(k_kres<T>, dim3(nb), dim3(256), 0, c->st, V.X, V.Y, V.Z, k, kb<T>(V), kx<T>(V), kr<T>(V));
+  Level &C = c->lv[l + 1];
+  hipLaunchKernelGGL(k_krestrict<T>, dim3(blocks(C.n, 256)), dim3(256), 0, c->st, V.X, V.Y, C.X, C.Y, C.Z, kagg(C),
+                     kr<T>(V), kb<T>(C), kx<T>(C));
+  kcycle<T>(c, l + 1, nu, scale);
+  hipLaunchKernelGGL(k_kprolong<T>, dim3(nb), dim3(256), 0, c->st, V.X, V.Y, V.Z, C.X, C.Y, V.kpar[0], V.kpar[1],
+                     V.kpar[2], k.p[0], kx<T>(C), kx<T>(V), scale);
+  for (int s = 0; s < nu; s++)
+    for (int col = 1; col >= 0; col--)
+      hipLaunchKernelGGL(k_kgs<T>, dim3(nb), dim3(256), 0, c->st, V.X, V.Y, V.Z, k, kb<T>(V), kx<T>(V), col);
+}
+
+// z_p = sign K^-1 (B D^-1 A D^-1 G) K^-1 r_p on the continuity cells,
+// velocity slots of z cleared.  The BFBt product approximates (B A^-1 G)^-1;
+// the preconditioner's pressure block is S = -B A^-1 G, hence sign = -1.
+template <typename T>
+void schur_bfbt(stk_ctx *c, const stk_solver_opts *o, const double *r, double *z, double sign) {
+  Level &F = c->lv[0];
+  const unsigned nb = blocks(F.n, 256);
+  const T sc = (T)o->kscale;
+  const int nu = o->ksweeps;
+  hipLaunchKernelGGL(k_bf_gather<T>, dim3(nb), dim3(256), 0, c->st, F.n, F.flags, r, kb<T>(F));
+  hipMemsetAsync(kx<T>(F), 0, F.n * sizeof(T), c->st);
+  kcycle<T>(c, 0, nu, sc);
+  hipLaunchKernelGGL(k_bf_grad<T>, dim3(nb), dim3(256), 0, c->st, F.X, F.Y, F.Z, iBt<T>(F, 0), iBt<T>(F, 1),
+                     iBt<T>(F, 2), dinvt<T>(c), kx<T>(F), bfy<T>(c));
+  rowop<T, false>(c, 0, bfy<T>(c), nullptr, bfw<T>(c), 0);
+  hipLaunchKernelGGL(k_bf_div<T>, dim3(nb), dim3(256), 0, c->st, F.X, F.Y, F.Z, F.flags, iAt<T>(F, 0),
+                     iAt<T>(F, 1), iAt<T>(F, 2), dinvt<T>(c), bfw<T>(c), kb<T>(F));
+  hipMemsetAsync(kx<T>(F), 0, F.n * sizeof(T), c->st);
+  kcycle<T>(c, 0, nu, sc);
+  hipLaunchKernelGGL(k_bf_scatter<T>, dim3(nb), dim3(256), 0, c->st, F.n, F.flags, kx<T>(F), sign, z);
+}
+
+// z = M^{-1} r : pressure by the Schur approximation, velocity by one V-cycle
 template <typename T>
 void precond(stk_ctx *c, const stk_solver_opts *o, const double *r, double *z, const int *tile) {
   Level &F = c->lv[0];
   long n4 = 4 * F.n;
-  hipLaunchKernelGGL(k_pscale, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, F.flags, F.e64[0], r, z, o->omega);
+  if (o->schur >= 1)
+    schur_bfbt<T>(c, o, r, z, o->schur == 1 ? -1.0 : 1.0);
+  else
+    hipLaunchKernelGGL(k_pscale, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, F.flags, F.e64[0], r, z, o->omega);
   bc_passes<double>(c, F, r, z, 2);
   Geo<double> g = F.geo<double>(c->kn.p2koef);
   hipLaunchKernelGGL(k_vrhs<T>, dim3(blocks(n4, 256)), dim3(256), 0, c->st, g, F.flags, r, z, lvb<T>(F));
@@ -511,6 +626,19 @@ int coarse_inverse(stk_ctx *c, int withp, double **out) {
 // ===========================================================================
 // C ABI
 // ===========================================================================
+// one K-cycle on host cell vectors (inspection entry point)
+template <typename T>
+static void kcycle_host(stk_ctx *c, int nu, double scale, const double *b, double *x) {
+  Level &F = c->lv[0];
+  std::vector<T> hb(b, b + F.n), hx(F.n);
+  hipMemcpy(kb<T>(F), hb.data(), F.n * sizeof(T), hipMemcpyHostToDevice);
+  hipMemsetAsync(kx<T>(F), 0, F.n * sizeof(T), c->st);
+  kcycle<T>(c, 0, nu, (T)scale);
+  hipStreamSynchronize(c->st);
+  hipMemcpy(hx.data(), kx<T>(F), F.n * sizeof(T), hipMemcpyDeviceToHost);
+  for (long i = 0; i < F.n; i++) x[i] = (double)hx[i];
+}
+
 extern "C" {
 
 int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *gx, const double *gy,
@@ -607,7 +735,12 @@ void stk_destroy(stk_ctx *c) {
     for (int k = 0; k < 3; k++) dfree(L.ref[k]);
     for (int k = 0; k < 4; k++)
       for (int a = 0; a < 3; a++) L.ax[k][a].release();
+    for (int q = 0; q < 7; q++) { dfree(L.k64[q]); dfree(L.k32[q]); }
+    dfree(L.kx32); dfree(L.kb32); dfree(L.kr32); dfree(L.kx64); dfree(L.kb64); dfree(L.kr64);
+    for (int a = 0; a < 3; a++) { dfree(L.klo[a]); dfree(L.kcnt[a]); dfree(L.kpar[a]); }
   }
+  dfree(c->dinv64); dfree(c->bfy64); dfree(c->bfw64); dfree(c->dinv32); dfree(c->bfy32); dfree(c->bfw32);
+  dfree(c->kinv64); dfree(c->kinv32); dfree(c->kcell);
   dfree(c->nu); dfree(c->ro); dfree(c->nd); dfree(c->nxx); dfree(c->nxy); dfree(c->nxz); dfree(c->nyz);
   dfree(c->ainv64); dfree(c->ainvP64); dfree(c->ainv32);
   dfree(c->X); dfree(c->B); dfree(c->R); dfree(c->W2); dfree(c->T1); dfree(c->red); dfree(c->scal);
@@ -646,6 +779,9 @@ void stk_default_opts(stk_solver_opts *o, int levels) {
   o->coarse_viscmod = -1;
   o->tol = 1e-8;
   o->maxit = 800;
+  o->schur = 1;
+  o->ksweeps = 2;
+  o->kscale = 2.0;  // piecewise-constant aggregation undershoots the coarse correction by about 2
 }
 
 // boundary grammar line projected on every level (parallel/in3mg.c:299-375)
@@ -778,6 +914,140 @@ int stk_markers_to_nodes(stk_ctx *c, long nmark, const float *mx, const float *m
   return rc;
 }
 
+// nesting of level-l cells in level-(l-1) cells along one axis: coarse line
+// i lies on the fine line with the same coordinate, so coarse cell m (lines
+// m-1, m) owns fine cells map[m-1]+1 .. map[m]
+static int axis_nesting(const std::vector<double> &gf, const std::vector<double> &gc, std::vector<int> &lo,
+                        std::vector<int> &cnt, std::vector<int> &par) {
+  const int nf = (int)gf.size(), nc = (int)gc.size();
+  std::vector<int> map(nc);
+  int k = 0;
+  for (int i = 0; i < nc; i++) {
+    while (k < nf && gf[k] != gc[i]) k++;
+    if (k == nf) return STK_EINVAL;
+    map[i] = k;
+  }
+  if (map[0] != 0 || map[nc - 1] != nf - 1) return STK_EINVAL;
+  lo.assign(nc, 0);
+  cnt.assign(nc, 0);
+  par.assign(nf, 0);
+  for (int m = 1; m < nc; m++) {
+    lo[m] = map[m - 1] + 1;
+    cnt[m] = map[m] - map[m - 1];
+    for (int f = lo[m]; f < lo[m] + cnt[m]; f++) par[f] = m;
+  }
+  return STK_OK;
+}
+
+// Schur approximation of the prepared operator: D^-1, K on level 0, its
+// aggregated Galerkin hierarchy and the dense inverse of the coarsest K
+// (regularised by s/N * 1 1^T, s the mean diagonal, so the constant pressure
+// mode of the Neumann operator gets a bounded response)
+static int build_schur(stk_ctx *c) {
+  const int L = c->L;
+  Level &F = c->lv[0];
+  const long n4 = 4 * F.n;
+  if (!c->dinv64) {
+    if (dalloc(&c->dinv64, n4) || dalloc(&c->dinv32, n4) || dalloc(&c->bfy64, n4) || dalloc(&c->bfw64, n4) ||
+        dalloc(&c->bfy32, n4) || dalloc(&c->bfw32, n4))
+      return STK_ENOMEM;
+  }
+  for (int l = 0; l <= L; l++) {
+    Level &V = c->lv[l];
+    if (V.kx32) continue;
+    for (int q = 0; q < 7; q++)
+      if (dalloc(&V.k64[q], V.n) || dalloc(&V.k32[q], V.n)) return STK_ENOMEM;
+    if (dalloc(&V.kx32, V.n) || dalloc(&V.kb32, V.n) || dalloc(&V.kr32, V.n) || dalloc(&V.kx64, V.n) ||
+        dalloc(&V.kb64, V.n) || dalloc(&V.kr64, V.n))
+      return STK_ENOMEM;
+    HIPCHK(hipMemset(V.kx32, 0, V.n * 4));
+    HIPCHK(hipMemset(V.kx64, 0, V.n * 8));
+  }
+  if (!c->ktabs) {
+    for (int l = 1; l <= L; l++) {
+      Level &P = c->lv[l - 1], &C = c->lv[l];
+      for (int a = 0; a < 3; a++) {
+        std::vector<int> lo, cnt, par;
+        int rc = axis_nesting(P.g[a], C.g[a], lo, cnt, par);
+        if (rc) return rc;
+        if (upload(&C.klo[a], lo) || upload(&C.kcnt[a], cnt) || upload(&P.kpar[a], par)) return STK_ENOMEM;
+      }
+    }
+    c->ktabs = true;
+  }
+  Geo<double> g = F.geo<double>(c->kn.p2koef);
+  const unsigned nb = blocks(F.n, 256);
+  hipLaunchKernelGGL(k_bf_dinv, dim3(nb), dim3(256), 0, c->st, g, F.flags, c->dinv64, c->dinv32);
+  KPlanes<double> k64;
+  KPlanes<float> k32;
+  for (int q = 0; q < 7; q++) { k64.p[q] = F.k64[q]; k32.p[q] = F.k32[q]; }
+  hipLaunchKernelGGL(k_bf_kop, dim3(nb), dim3(256), 0, c->st, g, F.flags, c->dinv64, k64, k32);
+  for (int l = 1; l <= L; l++) {
+    Level &P = c->lv[l - 1], &C = c->lv[l];
+    KPlanes<double> kc;
+    for (int q = 0; q < 7; q++) kc.p[q] = C.k64[q];
+    hipLaunchKernelGGL(k_kagg<double>, dim3(blocks(C.n, 256)), dim3(256), 0, c->st, P.X, P.Y, C.X, C.Y, C.Z,
+                       kagg(C), kplanes<double>(P), kc);
+    for (int q = 0; q < 7; q++)
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks(C.n, 256)), dim3(256), 0, c->st, C.n, C.k64[q],
+                         C.k32[q]);
+  }
+  // coarsest K: dense inverse on the host (at most a few hundred cells)
+  Level &C = c->lv[L];
+  std::vector<double> h[7];
+  HIPCHK(hipStreamSynchronize(c->st));
+  for (int q = 0; q < 7; q++) {
+    h[q].resize(C.n);
+    HIPCHK(hipMemcpy(h[q].data(), C.k64[q], C.n * 8, hipMemcpyDeviceToHost));
+  }
+  std::vector<int> idx(C.n, -1), cell;
+  for (long n = 0; n < C.n; n++)
+    if (h[0][n] != 0.0) { idx[n] = (int)cell.size(); cell.push_back((int)n); }
+  const int N = (int)cell.size();
+  if (N == 0 || N > 4096) return STK_EINVAL;
+  std::vector<double> A((size_t)N * N, 0.0), I((size_t)N * N, 0.0);
+  double dmean = 0.0;
+  for (int i = 0; i < N; i++) {
+    const long n = cell[i];
+    int m1, m2, m3;
+    m3 = (int)(n / C.XY); m1 = (int)((n % C.XY) / C.Y); m2 = (int)(n % C.Y);
+    A[(size_t)i * N + i] += h[0][n];
+    dmean += h[0][n];
+    const long nbh[6] = {m1 > 0 ? n - C.Y : -1, m1 < C.X - 1 ? n + C.Y : -1, m2 > 0 ? n - 1 : -1,
+                         m2 < C.Y - 1 ? n + 1 : -1, m3 > 0 ? n - C.XY : -1, m3 < C.Z - 1 ? n + C.XY : -1};
+    for (int q = 0; q < 6; q++)
+      if (nbh[q] >= 0 && idx[nbh[q]] >= 0) A[(size_t)i * N + idx[nbh[q]]] += h[q + 1][n];
+    I[(size_t)i * N + i] = 1.0;
+  }
+  dmean /= N;
+  for (auto &v : A) v += dmean / N;
+  // Gauss-Jordan with partial pivoting
+  for (int k = 0; k < N; k++) {
+    int p = k;
+    for (int i = k + 1; i < N; i++)
+      if (std::fabs(A[(size_t)i * N + k]) > std::fabs(A[(size_t)p * N + k])) p = i;
+    if (A[(size_t)p * N + k] == 0.0) return STK_ESINGULAR;
+    if (p != k)
+      for (int j = 0; j < N; j++) {
+        std::swap(A[(size_t)p * N + j], A[(size_t)k * N + j]);
+        std::swap(I[(size_t)p * N + j], I[(size_t)k * N + j]);
+      }
+    const double inv = 1.0 / A[(size_t)k * N + k];
+    for (int j = 0; j < N; j++) { A[(size_t)k * N + j] *= inv; I[(size_t)k * N + j] *= inv; }
+    for (int i = 0; i < N; i++) {
+      if (i == k) continue;
+      const double f = A[(size_t)i * N + k];
+      if (f == 0.0) continue;
+      for (int j = 0; j < N; j++) { A[(size_t)i * N + j] -= f * A[(size_t)k * N + j]; I[(size_t)i * N + j] -= f * I[(size_t)k * N + j]; }
+    }
+  }
+  std::vector<float> I32(I.begin(), I.end());
+  dfree(c->kinv64); dfree(c->kinv32); dfree(c->kcell);
+  if (upload(&c->kinv64, I) || upload(&c->kinv32, I32) || upload(&c->kcell, cell)) return STK_ENOMEM;
+  c->nkc = N;
+  return STK_OK;
+}
+
 // flags + boundary lists of every level from the reference tables
 static int build_rows(stk_ctx *c) {
   for (int l = 0; l <= c->L; l++) {
@@ -900,6 +1170,8 @@ int stk_prepare_cvm(stk_ctx *c, int cvm) {
   hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks(nn, 256)), dim3(256), 0, c->st, nn, c->ainv64,
                      c->ainv32);
   dfree(c->ainvP64);
+  rc = build_schur(c);
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(c->st));
   c->prepared = true;
   c->cvm = vmc;
@@ -961,7 +1233,8 @@ int stk_hydrostatic_guess(stk_ctx *c) {
 // 1 FP64 smoother sweep, 2 FP64 operator apply (A x, all rows),
 // 3 FP32 residual, 4 FP64 residual (b - A x)
 int stk_time_kernel(stk_ctx *c, int which, int level, int reps, double *ms_avg) {
-  if (!c || !c->prepared || level < 0 || level > c->L || reps < 1 || !ms_avg) return STK_EINVAL;
+  if (!c || !c->prepared || level < 0 || level > c->L || reps < 1 || !ms_avg || which < 0 || which > 5)
+    return STK_EINVAL;
   int rc = ensure_vectors(c, true, true);
   if (rc) return rc;
   Level &L = c->lv[level];
@@ -987,10 +1260,13 @@ int stk_time_kernel(stk_ctx *c, int which, int level, int reps, double *ms_avg) 
       Geo<float> g = L.geo<float>(c->kn.p2koef);
       hipLaunchKernelGGL((k_rowop<float, true>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, g, L.flags,
                          L.x32[0], L.b32, L.r32, 0);
-    } else {
+    } else if (which == 4) {
       Geo<double> g = L.geo<double>(c->kn.p2koef);
       hipLaunchKernelGGL((k_rowop<double, true>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, g, L.flags,
                          L.x64[0], L.b64, L.r64, 1);
+    } else {
+      hipLaunchKernelGGL(k_stream_copy, dim3(blocks(4 * L.n, 256)), dim3(256), 0, c->st, 4 * L.n, L.x64[0],
+                         L.x64[1]);
     }
   };
   launch();
@@ -1008,7 +1284,8 @@ int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   if (!c || !o) return STK_EINVAL;
   if (!c->prepared) return STK_ESTATE;
   if (o->restart < 1 || o->restart > 128 || o->maxit < 1 || !(o->tol > 0) || o->cycle < 0 || o->cycle > 2 ||
-      o->coarse_viscmod < -1 || o->coarse_viscmod > 2)
+      o->coarse_viscmod < -1 || o->coarse_viscmod > 2 || o->schur < 0 || o->schur > 2 || o->ksweeps < 0 ||
+      o->ksweeps > 16 || !(o->kscale > 0))
     return STK_EINVAL;
   {
     const int want = (o->coarse_viscmod < 0) ? c->kn.viscmod : o->coarse_viscmod;
@@ -1164,6 +1441,45 @@ int stk_coarse_solve(stk_ctx *c) {
   hipLaunchKernelGGL((k_gemv<double, double>), dim3(blocks(n4, 4)), dim3(256), 0, c->st, c->ainvP64, C.ref[1],
                      C.ref[0], n4);
   HIPCHK(hipStreamSynchronize(c->st));
+  return STK_OK;
+}
+
+// ---- inspection of the Schur approximation (tests) ----------------------
+int stk_kop_apply(stk_ctx *c, int l, const double *x, double *y) {
+  if (!c || !c->prepared || l < 0 || l > c->L || !x || !y) return STK_EINVAL;
+  Level &V = c->lv[l];
+  double *dx, *dy, *db;
+  if (dalloc(&dx, V.n) || dalloc(&dy, V.n) || dalloc(&db, V.n)) return STK_ENOMEM;
+  HIPCHK(hipMemcpy(dx, x, V.n * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(db, 0, V.n * 8));
+  hipLaunchKernelGGL(k_kres<double>, dim3(blocks(V.n, 256)), dim3(256), 0, c->st, V.X, V.Y, V.Z, kplanes<double>(V),
+                     db, dx, dy);
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(y, dy, V.n * 8, hipMemcpyDeviceToHost));
+  for (long i = 0; i < V.n; i++) y[i] = -y[i];
+  dfree(dx); dfree(dy); dfree(db);
+  return STK_OK;
+}
+
+int stk_kcycle_apply(stk_ctx *c, int precision, int ksweeps, double kscale, const double *b, double *x) {
+  if (!c || !c->prepared || !b || !x || ksweeps < 0 || ksweeps > 16 || !(kscale > 0)) return STK_EINVAL;
+  if (precision) kcycle_host<double>(c, ksweeps, kscale, b, x);
+  else kcycle_host<float>(c, ksweeps, kscale, b, x);
+  HIPCHK(hipGetLastError());
+  return STK_OK;
+}
+
+int stk_schur_apply(stk_ctx *c, const stk_solver_opts *o, const double *r, double *z) {
+  if (!c || !c->prepared || !o || !r || !z || o->schur < 1 || o->schur > 2) return STK_EINVAL;
+  const long n4 = 4 * c->lv[0].n;
+  double *dr, *dz;
+  if (dalloc(&dr, n4) || dalloc(&dz, n4)) return STK_ENOMEM;
+  HIPCHK(hipMemcpy(dr, r, n4 * 8, hipMemcpyHostToDevice));
+  if (o->precision) schur_bfbt<double>(c, o, dr, dz, o->schur == 1 ? -1.0 : 1.0);
+  else schur_bfbt<float>(c, o, dr, dz, o->schur == 1 ? -1.0 : 1.0);
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(z, dz, n4 * 8, hipMemcpyDeviceToHost));
+  dfree(dr); dfree(dz);
   return STK_OK;
 }
 

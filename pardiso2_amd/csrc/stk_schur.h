@@ -1,0 +1,257 @@
+// Pressure Schur-complement approximation of the block preconditioner:
+// weighted BFBt with the velocity diagonal D as weight,
+//     S^-1 ~ K^-1 (B D^-1 A D^-1 G) K^-1,   K = B D^-1 G,
+// B the continuity rows (parallel/move3mg.c:1455-1573), G the pressure
+// columns of the momentum rows (slots 16/17, move3mg.c:923-925), A the
+// velocity block, D = |diagonal| of the momentum equations (boundary rows
+// excluded, so walls act as Neumann faces of K).  K is a 7-point operator on
+// the pressure cells of a level; its inverse is approximated by one
+// aggregation multigrid V-cycle (piecewise-constant transfers along the
+// level hierarchy's own cell nesting, Galerkin coarse operators, red/black
+// Gauss-Seidel, dense inverse on the coarsest level).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "stk_rows.h"
+
+namespace stk {
+
+template <typename T> struct KPlanes { T *p[7]; };  // diag, -x, +x, -y, +y, -z, +z
+template <typename T> struct KPlanesC { const T *p[7]; };
+
+// 1 / |diagonal| of the level's momentum equations; 0 on other unknowns
+__global__ void k_bf_dinv(Geo<double> g, const unsigned char *__restrict__ flags, double *d64, float *d32) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= (long)g.XY * g.Z) return;
+  int m1, m2, m3;
+  decode(n, g.X, g.Y, g.XY, m1, m2, m3);
+  const unsigned char f = flags[n];
+  double s[18], v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (kind_of(f, 1) == KIND_EQ) { row_coef_c<double, 1>(g, m1, m2, m3, s); v[1] = 1.0 / fabs(s[1]); }
+  if (kind_of(f, 2) == KIND_EQ) { row_coef_c<double, 2>(g, m1, m2, m3, s); v[2] = 1.0 / fabs(s[1]); }
+  if (kind_of(f, 3) == KIND_EQ) { row_coef_c<double, 3>(g, m1, m2, m3, s); v[3] = 1.0 / fabs(s[1]); }
+  for (int c = 0; c < 4; c++) {
+    d64[4 * n + c] = v[c];
+    d32[4 * n + c] = (float)v[c];
+  }
+}
+
+// K = B D^-1 G on the finest level's continuity cells.  For axis a the +face
+// of cell (m1,m2,m3) is the velocity at node (m - 1 + e_a), coefficient
+// +iA_a[m_a] in B; the -face is at node (m - 1), coefficient -iA_a[m_a].  A
+// face velocity u at axis index k has G = +2 iB_a[k] on its lower cell and
+// -2 iB_a[k] on its upper cell.
+__global__ void k_bf_kop(Geo<double> g, const unsigned char *__restrict__ flags, const double *__restrict__ dinv,
+                         KPlanes<double> k64, KPlanes<float> k32) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= (long)g.XY * g.Z) return;
+  int m1, m2, m3;
+  decode(n, g.X, g.Y, g.XY, m1, m2, m3);
+  double kc[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (kind_of(flags[n], 0) == KIND_EQ) {
+    const long base = g.nid(m1 - 1, m2 - 1, m3 - 1);  // node of the three -faces
+    const long step[3] = {(long)g.Y, 1L, (long)g.XY};
+    const int mm[3] = {m1, m2, m3};
+    const double *iA[3] = {g.iAx, g.iAy, g.iAz};
+    const double *iB[3] = {g.iBx, g.iBy, g.iBz};
+    for (int a = 0; a < 3; a++) {
+      const int m = mm[a];
+      const double ap = iA[a][m] * dinv[4 * (base + step[a]) + 1 + a] * 2.0 * iB[a][m];
+      const double am = iA[a][m] * dinv[4 * base + 1 + a] * 2.0 * iB[a][m - 1];
+      kc[0] += ap + am;
+      kc[1 + 2 * a] = -am;
+      kc[2 + 2 * a] = -ap;
+    }
+  }
+  for (int q = 0; q < 7; q++) {
+    k64.p[q][n] = kc[q];
+    k32.p[q][n] = (float)kc[q];
+  }
+}
+
+// y = D^-1 G p on the velocity unknowns of level 0 (P slots and
+// non-equation unknowns 0); p is a cell vector (one value per node)
+template <typename T>
+__global__ void k_bf_grad(int X, int Y, int Z, const T *__restrict__ iBx, const T *__restrict__ iBy,
+                          const T *__restrict__ iBz, const T *__restrict__ dinv, const T *__restrict__ p, T *y) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XY = X * Y;
+  if (n >= (long)XY * Z) return;
+  int m1, m2, m3;
+  decode(n, X, Y, XY, m1, m2, m3);
+  y[4 * n] = T(0);
+  const long up = n + XY + Y + 1;  // cell (m1+1, m2+1, m3+1)
+  const long stp[3] = {(long)Y, 1L, (long)XY};
+  const int mm[3] = {m1, m2, m3};
+  const T *iB[3] = {iBx, iBy, iBz};
+  for (int a = 0; a < 3; a++) {
+    const T d = dinv[4 * n + 1 + a];
+    T v = T(0);
+    if (d != T(0)) v = d * (T(2) * iB[a][mm[a]]) * (p[up - stp[a]] - p[up]);
+    y[4 * n + 1 + a] = v;
+  }
+}
+
+// t = B D^-1 w on the continuity cells of level 0 (0 elsewhere)
+template <typename T>
+__global__ void k_bf_div(int X, int Y, int Z, const unsigned char *__restrict__ flags, const T *__restrict__ iAx,
+                         const T *__restrict__ iAy, const T *__restrict__ iAz, const T *__restrict__ dinv,
+                         const T *__restrict__ w, T *t) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XY = X * Y;
+  if (n >= (long)XY * Z) return;
+  if (kind_of(flags[n], 0) != KIND_EQ) { t[n] = T(0); return; }
+  int m1, m2, m3;
+  decode(n, X, Y, XY, m1, m2, m3);
+  const long base = n - XY - Y - 1;
+  const long stp[3] = {(long)Y, 1L, (long)XY};
+  const int mm[3] = {m1, m2, m3};
+  const T *iA[3] = {iAx, iAy, iAz};
+  T acc = T(0);
+  for (int a = 0; a < 3; a++) {
+    const long up = 4 * (base + stp[a]) + 1 + a, lo = 4 * base + 1 + a;
+    acc += iA[a][mm[a]] * (dinv[up] * w[up] - dinv[lo] * w[lo]);
+  }
+  t[n] = acc;
+}
+
+// cell vector from / to the pressure slots of an AoS vector (continuity cells only)
+template <typename T>
+__global__ void k_bf_gather(long nn, const unsigned char *__restrict__ flags, const double *__restrict__ r, T *b) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  b[n] = (kind_of(flags[n], 0) == KIND_EQ) ? (T)r[4 * n] : T(0);
+}
+
+template <typename T>
+__global__ void k_bf_scatter(long nn, const unsigned char *__restrict__ flags, const T *__restrict__ x, double sign,
+                             double *z) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  z[4 * n + 0] = (kind_of(flags[n], 0) == KIND_EQ) ? sign * (double)x[n] : 0.0;
+  z[4 * n + 1] = 0.0;
+  z[4 * n + 2] = 0.0;
+  z[4 * n + 3] = 0.0;
+}
+
+// Galerkin coarse K by aggregation: coarse cell (cx,cy,cz) owns fine cells
+// lo_a[c] .. lo_a[c]+cnt_a[c]-1 per axis; couplings inside the aggregate
+// fold into its diagonal, couplings across a face go to that face's plane.
+struct KAgg {
+  const int *lo[3];
+  const int *cnt[3];
+};
+
+template <typename T>
+__global__ void k_kagg(int Xf, int Yf, int Xc, int Yc, int Zc, KAgg t, KPlanesC<T> kf, KPlanes<T> kc) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XYc = Xc * Yc, XYf = Xf * Yf;
+  if (n >= (long)XYc * Zc) return;
+  int cx, cy, cz;
+  decode(n, Xc, Yc, XYc, cx, cy, cz);
+  T acc[7] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+  const int c3[3] = {cx, cy, cz};
+  int lo[3], cnt[3];
+  for (int a = 0; a < 3; a++) {
+    lo[a] = t.lo[a][c3[a]];
+    cnt[a] = t.cnt[a][c3[a]];
+  }
+  for (int iz = 0; iz < cnt[2]; iz++)
+    for (int ix = 0; ix < cnt[0]; ix++)
+      for (int iy = 0; iy < cnt[1]; iy++) {
+        const long f = (long)(lo[2] + iz) * XYf + (long)(lo[0] + ix) * Yf + (lo[1] + iy);
+        acc[0] += kf.p[0][f];
+        const int ii[3] = {ix, iy, iz};
+        for (int a = 0; a < 3; a++) {
+          const T km = kf.p[1 + 2 * a][f], kp = kf.p[2 + 2 * a][f];
+          if (ii[a] > 0) acc[0] += km; else acc[1 + 2 * a] += km;
+          if (ii[a] < cnt[a] - 1) acc[0] += kp; else acc[2 + 2 * a] += kp;
+        }
+      }
+  for (int q = 0; q < 7; q++) kc.p[q][n] = acc[q];
+}
+
+// one red/black Gauss-Seidel half-sweep of K x = b on cells of one colour
+template <typename T>
+__global__ void k_kgs(int X, int Y, int Z, KPlanesC<T> k, const T *__restrict__ b, T *x, int colour) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XY = X * Y;
+  if (n >= (long)XY * Z) return;
+  const T d = k.p[0][n];
+  if (d == T(0)) return;
+  int m1, m2, m3;
+  decode(n, X, Y, XY, m1, m2, m3);
+  if (((m1 + m2 + m3) & 1) != colour) return;
+  T acc = b[n];
+  if (m1 > 0) acc -= k.p[1][n] * x[n - Y];
+  if (m1 < X - 1) acc -= k.p[2][n] * x[n + Y];
+  if (m2 > 0) acc -= k.p[3][n] * x[n - 1];
+  if (m2 < Y - 1) acc -= k.p[4][n] * x[n + 1];
+  if (m3 > 0) acc -= k.p[5][n] * x[n - XY];
+  if (m3 < Z - 1) acc -= k.p[6][n] * x[n + XY];
+  x[n] = acc / d;
+}
+
+// r = b - K x
+template <typename T>
+__global__ void k_kres(int X, int Y, int Z, KPlanesC<T> k, const T *__restrict__ b, const T *__restrict__ x, T *r) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XY = X * Y;
+  if (n >= (long)XY * Z) return;
+  const T d = k.p[0][n];
+  if (d == T(0)) { r[n] = T(0); return; }
+  int m1, m2, m3;
+  decode(n, X, Y, XY, m1, m2, m3);
+  T acc = b[n] - d * x[n];
+  if (m1 > 0) acc -= k.p[1][n] * x[n - Y];
+  if (m1 < X - 1) acc -= k.p[2][n] * x[n + Y];
+  if (m2 > 0) acc -= k.p[3][n] * x[n - 1];
+  if (m2 < Y - 1) acc -= k.p[4][n] * x[n + 1];
+  if (m3 > 0) acc -= k.p[5][n] * x[n - XY];
+  if (m3 < Z - 1) acc -= k.p[6][n] * x[n + XY];
+  r[n] = acc;
+}
+
+// b_c = sum of the aggregate's fine residuals, x_c = 0
+template <typename T>
+__global__ void k_krestrict(int Xf, int Yf, int Xc, int Yc, int Zc, KAgg t, const T *__restrict__ rf, T *bc, T *xc) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XYc = Xc * Yc, XYf = Xf * Yf;
+  if (n >= (long)XYc * Zc) return;
+  int cx, cy, cz;
+  decode(n, Xc, Yc, XYc, cx, cy, cz);
+  const int lx = t.lo[0][cx], ly = t.lo[1][cy], lz = t.lo[2][cz];
+  const int nx = t.cnt[0][cx], ny = t.cnt[1][cy], nz = t.cnt[2][cz];
+  T acc = T(0);
+  for (int iz = 0; iz < nz; iz++)
+    for (int ix = 0; ix < nx; ix++)
+      for (int iy = 0; iy < ny; iy++) acc += rf[(long)(lz + iz) * XYf + (long)(lx + ix) * Yf + (ly + iy)];
+  bc[n] = acc;
+  xc[n] = T(0);
+}
+
+// x_f += scale * x_c(parent) on fine cells with an equation
+template <typename T>
+__global__ void k_kprolong(int Xf, int Yf, int Zf, int Xc, int Yc, const int *__restrict__ px,
+                           const int *__restrict__ py, const int *__restrict__ pz, const T *__restrict__ kdiag,
+                           const T *__restrict__ xc, T *xf, T scale) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XYf = Xf * Yf;
+  if (n >= (long)XYf * Zf) return;
+  if (kdiag[n] == T(0)) return;
+  int m1, m2, m3;
+  decode(n, Xf, Yf, XYf, m1, m2, m3);
+  xf[n] += scale * xc[(long)pz[m3] * Xc * Yc + (long)px[m1] * Yc + py[m2]];
+}
+
+// coarsest level: x[cell[i]] = sum_j inv[i][j] b[cell[j]]
+template <typename T>
+__global__ void k_kcoarse(int nc, const int *__restrict__ cell, const T *__restrict__ inv, const T *__restrict__ b,
+                          T *x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  T acc = T(0);
+  for (int j = 0; j < nc; j++) acc += inv[(long)i * nc + j] * b[cell[j]];
+  x[cell[i]] = acc;
+}
+
+}  // namespace stk

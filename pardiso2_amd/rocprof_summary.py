@@ -1,0 +1,86 @@
+"""Summarise rocprofv3 CSV output (kernel stats and PMC counter collection).
+
+    python -m pardiso2_amd.rocprof_summary stats <dir> [--top N]
+    python -m pardiso2_amd.rocprof_summary pmc <dir> [--kernel SUBSTR]
+
+`stats` reads the *_kernel_stats.csv that `rocprofv3 --kernel-trace --stats
+--output-format csv` writes; `pmc` reads *_counter_collection.csv of a
+`rocprofv3 --pmc ... --output-format csv` run and prints, per kernel name and
+counter, the number of dispatches and the mean value per dispatch.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import os
+import sys
+from collections import defaultdict
+
+
+def _find(d, suffix):
+    hits = sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True))
+    if not hits:
+        raise SystemExit(f"no *{suffix} under {d}")
+    return hits
+
+
+def _col(row, *names):
+    for n in names:
+        if n in row and row[n] != "":
+            return row[n]
+    raise KeyError(f"none of {names} in {list(row)}")
+
+
+def stats(d, top):
+    rows = []
+    for f in _find(d, "kernel_stats.csv"):
+        with open(f) as fh:
+            rows += list(csv.DictReader(fh))
+    tot = sum(float(_col(r, "TotalDurationNs")) for r in rows)
+    rows.sort(key=lambda r: -float(_col(r, "TotalDurationNs")))
+    print(f"{'calls':>8} {'total ms':>10} {'avg us':>10} {'min us':>9} {'max us':>9} {'%':>6}  kernel")
+    for r in rows[:top]:
+        t = float(_col(r, "TotalDurationNs"))
+        print(f"{int(_col(r, 'Calls')):8d} {t / 1e6:10.3f} {float(_col(r, 'AverageNs')) / 1e3:10.2f} "
+              f"{float(_col(r, 'MinNs')) / 1e3:9.2f} {float(_col(r, 'MaxNs')) / 1e3:9.2f} {100 * t / tot:6.2f}  "
+              f"{_col(r, 'Name', 'KernelName')[:110]}")
+    print(f"total kernel time {tot / 1e6:.3f} ms over {sum(int(_col(r, 'Calls')) for r in rows)} dispatches")
+
+
+def pmc(d, kernel):
+    per = defaultdict(float)  # (kernel, counter, dispatch) -> value summed over the row's dimensions
+    for f in _find(d, "counter_collection.csv"):
+        with open(f) as fh:
+            for i, r in enumerate(csv.DictReader(fh)):
+                name = _col(r, "Kernel_Name", "KernelName", "Kernel-Name")
+                if kernel and kernel not in name:
+                    continue
+                try:
+                    disp = _col(r, "Dispatch_Id", "DispatchId", "Correlation_Id")
+                except KeyError:
+                    disp = i
+                cn = _col(r, "Counter_Name", "CounterName")
+                per[(name, cn, disp)] += float(_col(r, "Counter_Value", "CounterValue"))
+    acc = defaultdict(list)
+    for (name, cn, _), v in per.items():
+        acc[(name, cn)].append(v)
+    for (name, cn), v in sorted(acc.items()):
+        print(f"{cn:>12} dispatches {len(v):5d} mean {sum(v) / len(v):14.1f}  {name[:100]}")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("mode", choices=["stats", "pmc"])
+    ap.add_argument("dir")
+    ap.add_argument("--top", type=int, default=25)
+    ap.add_argument("--kernel", default=None)
+    a = ap.parse_args(argv)
+    if a.mode == "stats":
+        stats(a.dir, a.top)
+    else:
+        pmc(a.dir, a.kernel)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
