@@ -52,7 +52,7 @@ typedef struct {
 
 typedef struct {
   int precision;     /* 0: FP32 V-cycle inside FP64 outer iteration; 1: FP64 throughout */
-  int restart;       /* GCR direction history length (1..64) */
+  int restart;       /* Krylov directions kept before a restart (1..128) */
   int pre[12];       /* pre-smoothing sweeps per level */
   int post[12];      /* post-smoothing sweeps per level */
   int tile[3];       /* smoother tile (y, x, z) in nodes; 0 = default */
@@ -66,6 +66,7 @@ typedef struct {
                         1 weighted BFBt with one aggregation-multigrid V-cycle per K solve */
   int ksweeps;       /* red/black Gauss-Seidel sweeps before and after each K-cycle level */
   double kscale;     /* coarse-correction scale of the aggregation K-cycle */
+  int krylov;        /* outer Krylov method: 0 GCR, 1 flexible GMRES */
 } stk_solver_opts;
 
 typedef struct {
@@ -144,7 +145,9 @@ int stk_vec_get(stk_ctx *ctx, int which, int level, double *host);
 /* y = A x on a level (FP64, matrix-free), x/y host arrays of 4*mgn[l] */
 int stk_apply(stk_ctx *ctx, int level, const double *x, double *y);
 /* Smoother sweeps in tile-coloured order on the context's sol0/fre1 of one
- * level, FP64, all rows (pressure Uzawa step included), then fre0. */
+ * level, FP64, all rows (pressure Uzawa step included), then fre0.
+ * ty = tx = tz = 0 selects one tile covering the whole level (the LDS-resident
+ * kernel the V-cycle uses on levels of at most 2048 nodes in FP64). */
 int stk_smooth(stk_ctx *ctx, int level, int sweeps, int ty, int tx, int tz);
 int stk_coarse_solve(stk_ctx *ctx);
 

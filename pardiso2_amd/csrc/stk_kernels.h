@@ -256,6 +256,91 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
   }
 }
 
+// Boundary rows along their dependency chains in one launch.  Row i's chain
+// is r_0 (the row itself), r_1, ..., r_d, each x_{r_t} = b_{r_t} + k_t x_{r_{t+1}},
+// ending at x_{j_i} with j_i not a boundary row (or no partner, j_i = -1).
+// Every thread evaluates its whole chain from the final partner, so all rows
+// of a chain see the same values without ordering the launch by depth; the
+// partners read are never written here.
+struct BcChain {
+  int nbc, dm;
+  const int *r, *len, *j;
+  const double *k;
+};
+
+template <typename T>
+__device__ __forceinline__ void bc_chain_one(const BcChain &ch, int i, const T *__restrict__ b, T *x, int mode) {
+  const long o = (long)i * ch.dm;
+  const int r0 = ch.r[o];
+  if ((mode == 1 && (r0 & 3) == 0) || (mode == 2 && (r0 & 3) != 0)) return;
+  const int jf = ch.j[i];
+  T v = (jf >= 0) ? x[jf] : T(0);
+  for (int t = ch.len[i] - 1; t >= 0; t--) v = b[ch.r[o + t]] + (T)ch.k[o + t] * v;
+  x[r0] = v;
+}
+
+template <typename T>
+__global__ void k_bc_chain(BcChain ch, const T *__restrict__ b, T *x, int mode) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ch.nbc) bc_chain_one<T>(ch, i, b, x, mode);
+}
+
+// Whole-level smoother for small levels: one workgroup holds the level's
+// (P,vx,vy,vz) in LDS in the global layout and runs all sweeps there, each
+// sweep the phases vx(red,black), vy(red,black), vz(red,black)[, P] over the
+// whole level, then the boundary chains.  It is the tile-coloured sweep with
+// one tile covering the level.
+template <typename T, int C>
+__device__ __forceinline__ void level_phase(const Geo<T> &g, const unsigned char *__restrict__ flags,
+                                            const T *__restrict__ b, T *sx, int colour, T v0koef, T p0koef) {
+  const int hty = (g.Y + 1) / 2;
+  const int ncand = (C == 0) ? g.XY * g.Z : hty * g.X * g.Z;
+  for (int i = threadIdx.x; i < ncand; i += blockDim.x) {
+    int m1, m2, m3;
+    if (C == 0) {
+      m2 = i % g.Y; m1 = (i / g.Y) % g.X; m3 = i / g.XY;
+    } else {
+      const int kk = i % hty;
+      m1 = (i / hty) % g.X; m3 = i / (hty * g.X);
+      m2 = 2 * kk + ((colour - m1 - m3) & 1);
+      if (m2 >= g.Y) continue;
+    }
+    const long n = g.nid(m1, m2, m3);
+    if (kind_of(flags[n], C) != KIND_EQ) continue;
+    T s[18];
+    row_coef_c<T, C>(g, m1, m2, m3, s);
+    T ival = -b[4 * n + C];
+#pragma unroll
+    for (int q = 1; q <= Slot<C>::ns; q++) {
+      const long nb = n + (long)Slot<C>::d[q][2] * g.XY + (long)Slot<C>::d[q][0] * g.Y + Slot<C>::d[q][1];
+      ival += s[q] * sx[4 * nb + Slot<C>::d[q][3]];
+    }
+    if (C == 0) sx[4 * n] -= ival * g.ec[n] * p0koef;
+    else sx[4 * n + C] -= ival / s[1] * v0koef;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_smooth_level(Geo<T> g, const unsigned char *__restrict__ flags, T *x,
+                                                       const T *__restrict__ b, int sweeps, int withp, T v0koef,
+                                                       T p0koef, BcChain ch) {
+  extern __shared__ unsigned char smem_raw[];
+  T *sx = reinterpret_cast<T *>(smem_raw);
+  const int n4 = 4 * g.XY * g.Z;
+  for (int i = threadIdx.x; i < n4; i += blockDim.x) sx[i] = x[i];
+  __syncthreads();
+  const int mode = withp ? 0 : 1;
+  for (int s = 0; s < sweeps; s++) {
+    for (int col = 0; col < 2; col++) { level_phase<T, 1>(g, flags, b, sx, col, v0koef, p0koef); __syncthreads(); }
+    for (int col = 0; col < 2; col++) { level_phase<T, 2>(g, flags, b, sx, col, v0koef, p0koef); __syncthreads(); }
+    for (int col = 0; col < 2; col++) { level_phase<T, 3>(g, flags, b, sx, col, v0koef, p0koef); __syncthreads(); }
+    if (withp) { level_phase<T, 0>(g, flags, b, sx, 0, v0koef, p0koef); __syncthreads(); }
+    for (int i = threadIdx.x; i < ch.nbc; i += blockDim.x) bc_chain_one<T>(ch, i, b, sx, mode);
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < n4; i += blockDim.x) x[i] = sx[i];
+}
+
 // boundary-row update x_r -= (x_r - k x_j - b_r), one dependency depth
 // mode 0: all boundary rows, 1: velocity rows only, 2: pressure rows only
 template <typename T>
@@ -598,6 +683,73 @@ __global__ void k_reduce_rows(const double *__restrict__ partial, int nb, int k,
     __syncthreads();
   }
   if (threadIdx.x == 0) out[i] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// Krylov basis kernels: one pass over the vectors for all k directions
+// ---------------------------------------------------------------------------
+constexpr int kMaxDirs = 129;
+struct PtrsK {
+  const double *p[kMaxDirs];
+};
+
+// partial[i*gridDim + block] = sum over the block's slices of w2 * a * V_i
+__global__ void __launch_bounds__(256) k_mdot(long n4, const double *__restrict__ w2, const double *__restrict__ a,
+                                              PtrsK V, int k, double *partial) {
+  __shared__ double acc[kMaxDirs][4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = threadIdx.x; i < kMaxDirs * 4; i += blockDim.x) acc[i >> 2][i & 3] = 0.0;
+  __syncthreads();
+  constexpr int E = 8;
+  for (long base = (long)blockIdx.x * 256 * E; base < n4; base += (long)gridDim.x * 256 * E) {
+    double wa[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const long u = base + threadIdx.x + 256L * e;
+      wa[e] = (u < n4) ? w2[u] * a[u] : 0.0;
+    }
+    for (int i = 0; i < k; i++) {
+      const double *__restrict__ v = V.p[i];
+      double sum = 0.0;
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        const long u = base + threadIdx.x + 256L * e;
+        if (u < n4) sum += wa[e] * v[u];
+      }
+      for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+      if (lane == 0) acc[i][wave] += sum;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < k; i += blockDim.x)
+    partial[(long)i * gridDim.x + blockIdx.x] = (acc[i][0] + acc[i][1]) + (acc[i][2] + acc[i][3]);
+}
+
+// out = a - sum_i coef_i V_i (out may alias a); partial[block] = sum of w2 out^2
+__global__ void __launch_bounds__(256) k_maxpy_norm(long n4, int k, const double *__restrict__ coef, PtrsK V,
+                                                    const double *a, double *out, const double *__restrict__ w2,
+                                                    double *partial) {
+  __shared__ double sc[kMaxDirs];
+  __shared__ double red[4];
+  for (int i = threadIdx.x; i < k; i += blockDim.x) sc[i] = coef[i];
+  __syncthreads();
+  double ss = 0.0;
+  for (long u = (long)blockIdx.x * blockDim.x + threadIdx.x; u < n4; u += (long)gridDim.x * blockDim.x) {
+    double v = a[u];
+    for (int i = 0; i < k; i++) v -= sc[i] * V.p[i][u];
+    out[u] = v;
+    ss += w2[u] * v * v;
+  }
+  for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// y = s * x
+__global__ void k_scale_to(long n4, double s, const double *__restrict__ x, double *y) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u < n4) y[u] = s * x[u];
 }
 
 // a -= sum_i c_i A_i ; b -= sum_i c_i B_i

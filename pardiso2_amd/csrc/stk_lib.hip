@@ -85,6 +85,11 @@ struct Level {
   double *bc_negk = nullptr, *bc_c = nullptr;
   std::vector<int> hbc_r, hbc_j;
   std::vector<double> hbc_negk, hbc_c;
+  // the same rows as dependency chains (one launch per pass)
+  int ch_dm = 1;
+  int *ch_r = nullptr, *ch_len = nullptr, *ch_j = nullptr;
+  double *ch_k = nullptr;
+  BcChain chain() const { return BcChain{nbc, ch_dm, ch_r, ch_len, ch_j, ch_k}; }
   // vectors
   float *x32[2] = {}, *b32 = nullptr, *r32 = nullptr;
   double *x64[2] = {}, *b64 = nullptr, *r64 = nullptr;
@@ -149,6 +154,7 @@ struct stk_ctx {
   std::vector<double *> Zs, Qs;
   double *red = nullptr, *scal = nullptr;
   int nred = 1024;
+  double *kred = nullptr, *hcol = nullptr;  // flexible GMRES: per-block dots, Hessenberg column / update coefficients
   int cvm = -1;  // coarse-level averaging actually used by the prepared operators
   hipStream_t st = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -329,19 +335,24 @@ int ensure_vectors(stk_ctx *c, bool want32, bool want64) {
 // ---------------------------------------------------------------------------
 const int kTileDefault[3] = {32, 4, 4};
 
+// boundary rows after a sweep: one launch over the dependency chains
 template <typename T>
 void bc_passes(stk_ctx *c, Level &L, const T *b, T *x, int mode) {
-  for (size_t d = 0; d + 1 < L.bcoff.size(); d++) {
-    int beg = L.bcoff[d], end = L.bcoff[d + 1];
-    if (end > beg)
-      hipLaunchKernelGGL(k_bc_update<T>, dim3(blocks(end - beg, 256)), dim3(256), 0, c->st, beg, end, L.bc_r,
-                         L.bc_j, L.bc_negk, b, x, mode);
-  }
+  if (L.nbc) hipLaunchKernelGGL(k_bc_chain<T>, dim3(blocks(L.nbc, 256)), dim3(256), 0, c->st, L.chain(), b, x, mode);
 }
 
+// whole-level LDS smoother for levels whose four unknowns fit in 64 KB
+const size_t kLevelLds = 65536;
+
 template <typename T>
-void smooth(stk_ctx *c, int l, int sweeps, const int *tile, int withp) {
+void smooth(stk_ctx *c, int l, int sweeps, const int *tile, int withp, bool whole_level = false) {
   Level &L = c->lv[l];
+  if (whole_level && sweeps > 0 && 4 * (size_t)L.n * sizeof(T) <= kLevelLds) {
+    Geo<T> g = L.geo<T>(c->kn.p2koef);
+    hipLaunchKernelGGL(k_smooth_level<T>, dim3(1), dim3(1024), 4 * (size_t)L.n * sizeof(T), c->st, g, L.flags,
+                       lvx<T>(L), lvb<T>(L), sweeps, withp, (T)c->kn.v0koef, (T)c->kn.p0koef, L.chain());
+    return;
+  }
   int ty = tile[0], tx = tile[1], tz = tile[2];
   int nty = (L.Y + ty - 1) / ty, ntx = (L.X + tx - 1) / tx, ntz = (L.Z + tz - 1) / tz;
   size_t lds = (size_t)(ty + 2) * (tx + 2) * (tz + 2) * 4 * sizeof(T);
@@ -404,13 +415,13 @@ void mg_cycle(stk_ctx *c, const stk_solver_opts *o, const int *tile, int l, int 
     return;
   }
   Level &V = c->lv[l];
-  smooth<T>(c, l, o->pre[l], tile, 0);
+  smooth<T>(c, l, o->pre[l], tile, 0, true);
   rowop<T, true>(c, l, lvx<T>(V), lvb<T>(V), lvr<T>(V), 0);
   restrict_to<T>(c, l + 1, lvr<T>(V), lvb<T>(c->lv[l + 1]), lvx<T>(c->lv[l + 1]), 0);
   const int g = (l + 1 == L) ? 1 : gamma;
   for (int k = 0; k < g; k++) mg_cycle<T>(c, o, tile, l + 1, gamma);
   prolong_from<T>(c, l + 1, lvx<T>(c->lv[l + 1]), lvx<T>(V), 0, 0);
-  smooth<T>(c, l, o->post[l], tile, 0);
+  smooth<T>(c, l, o->post[l], tile, 0, true);
 }
 
 template <typename T>
@@ -576,7 +587,8 @@ int ensure_outer(stk_ctx *c, int m) {
   long n4 = 4 * c->lv[0].n;
   if (!c->X) {
     if (dalloc(&c->X, n4) || dalloc(&c->B, n4) || dalloc(&c->R, n4) || dalloc(&c->W2, n4) || dalloc(&c->T1, n4) ||
-        dalloc(&c->red, (size_t)c->nred * 16) || dalloc(&c->scal, 256))
+        dalloc(&c->red, (size_t)c->nred * 16) || dalloc(&c->scal, 256) ||
+        dalloc(&c->kred, (size_t)c->nred * (kMaxDirs + 1)) || dalloc(&c->hcol, 2 * kMaxDirs + 2))
       return STK_ENOMEM;
     hipMemset(c->X, 0, n4 * 8);
   }
@@ -730,6 +742,7 @@ void stk_destroy(stk_ctx *c) {
     for (int k = 0; k < 5; k++) { dfree(L.e64[k]); dfree(L.e32[k]); }
     dfree(L.flags);
     dfree(L.bc_r); dfree(L.bc_j); dfree(L.bc_negk); dfree(L.bc_c);
+    dfree(L.ch_r); dfree(L.ch_len); dfree(L.ch_j); dfree(L.ch_k);
     for (int k = 0; k < 2; k++) { dfree(L.x32[k]); dfree(L.x64[k]); }
     dfree(L.b32); dfree(L.r32); dfree(L.b64); dfree(L.r64);
     for (int k = 0; k < 3; k++) dfree(L.ref[k]);
@@ -744,6 +757,7 @@ void stk_destroy(stk_ctx *c) {
   dfree(c->nu); dfree(c->ro); dfree(c->nd); dfree(c->nxx); dfree(c->nxy); dfree(c->nxz); dfree(c->nyz);
   dfree(c->ainv64); dfree(c->ainvP64); dfree(c->ainv32);
   dfree(c->X); dfree(c->B); dfree(c->R); dfree(c->W2); dfree(c->T1); dfree(c->red); dfree(c->scal);
+  dfree(c->kred); dfree(c->hcol);
   for (auto p : c->Zs) dfree(p);
   for (auto p : c->Qs) dfree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -770,7 +784,7 @@ void stk_default_opts(stk_solver_opts *o, int levels) {
   memset(o, 0, sizeof(*o));
   o->precision = 0;
   o->restart = 128;
-  int pre[12] = {2, 4, 8, 16, 32, 64, 64, 64, 64, 64, 64, 64};
+  int pre[12] = {2, 4, 8, 16, 16, 16, 16, 16, 16, 16, 16, 16};
   for (int l = 0; l < 12; l++) { o->pre[l] = pre[l]; o->post[l] = pre[l]; }
   (void)levels;
   o->tile[0] = kTileDefault[0]; o->tile[1] = kTileDefault[1]; o->tile[2] = kTileDefault[2];
@@ -780,6 +794,7 @@ void stk_default_opts(stk_solver_opts *o, int levels) {
   o->tol = 1e-8;
   o->maxit = 800;
   o->schur = 1;
+  o->krylov = 0;  // flexible GMRES with one classical Gram-Schmidt pass loses orthogonality: GCR needs fewer iterations
   o->ksweeps = 2;
   o->kscale = 2.0;  // piecewise-constant aggregation undershoots the coarse correction by about 2
 }
@@ -1112,6 +1127,29 @@ static int build_rows(stk_ctx *c) {
       L.bcoff.push_back((int)L.hbc_r.size());
     }
     L.nbc = (int)L.hbc_r.size();
+    {
+      std::vector<int> pos(4 * L.n, -1);
+      for (int i = 0; i < L.nbc; i++) pos[L.hbc_r[i]] = i;
+      L.ch_dm = maxd + 1;
+      std::vector<int> cr((size_t)std::max(L.nbc, 1) * L.ch_dm, 0), cl(std::max(L.nbc, 1), 0), cj(std::max(L.nbc, 1), -1);
+      std::vector<double> ck((size_t)std::max(L.nbc, 1) * L.ch_dm, 0.0);
+      for (int i = 0; i < L.nbc; i++) {
+        int p = i, t = 0, j;
+        while (true) {
+          cr[(size_t)i * L.ch_dm + t] = L.hbc_r[p];
+          ck[(size_t)i * L.ch_dm + t] = -L.hbc_negk[p];
+          t++;
+          j = L.hbc_j[p];
+          if (j < 0 || pos[j] < 0 || t >= L.ch_dm) break;
+          p = pos[j];
+        }
+        cl[i] = t;
+        cj[i] = j;
+      }
+      dfree(L.ch_r); dfree(L.ch_len); dfree(L.ch_j); dfree(L.ch_k);
+      if (upload(&L.ch_r, cr) || upload(&L.ch_len, cl) || upload(&L.ch_j, cj) || upload(&L.ch_k, ck))
+        return STK_ENOMEM;
+    }
     dfree(L.bc_r); dfree(L.bc_j); dfree(L.bc_negk); dfree(L.bc_c);
     if (upload(&L.bc_r, L.hbc_r) || upload(&L.bc_j, L.hbc_j) || upload(&L.bc_negk, L.hbc_negk) ||
         upload(&L.bc_c, L.hbc_c))
@@ -1280,12 +1318,97 @@ int stk_time_kernel(stk_ctx *c, int which, int level, int reps, double *ms_avg) 
   return STK_OK;
 }
 
+// Flexible GMRES(m), right-preconditioned, in the W-weighted inner product:
+// V_j orthonormal in <.,.>_W (classical Gram-Schmidt, one pass over the
+// basis for the dots and one for the update), Z_j = M^-1 V_j kept for the
+// solution update at the end of each cycle, Givens rotations on the host.
+// Returns the number of iterations done; *rel is the true relative residual
+// after the last cycle.
+static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double nb, double *rel, stk_stats *S,
+                  int &status) {
+  const long n4 = 4 * c->lv[0].n;
+  const int m = o->restart;
+  const bool fp64 = o->precision == 1;
+  const unsigned nbk = (unsigned)std::min<long>(c->nred, std::max<long>(1, (n4 + 2047) / 2048));
+  const unsigned nbu = blocks(n4, 256) < (unsigned)c->nred ? blocks(n4, 256) : (unsigned)c->nred;
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), hh(m + 2), y(m);
+  int it = 0;
+  double r = *rel;
+  while (r > o->tol && it < o->maxit) {
+    // v_0 = R / ||R||_W
+    const double beta = r * nb;
+    hipLaunchKernelGGL(k_scale_to, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, 1.0 / beta, c->R, c->Qs[0]);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int j = 0;
+    for (; j < m && it < o->maxit; j++) {
+      double *Z = c->Zs[j], *W = c->Qs[j + 1];
+      if (fp64) precond<double>(c, o, c->Qs[j], Z, tile);
+      else precond<float>(c, o, c->Qs[j], Z, tile);
+      rowop<double, false>(c, 0, Z, nullptr, W, 1);
+      PtrsK P{};
+      for (int i = 0; i <= j; i++) P.p[i] = c->Qs[i];
+      hipLaunchKernelGGL(k_mdot, dim3(nbk), dim3(256), 0, c->st, n4, c->W2, W, P, j + 1, c->kred);
+      hipLaunchKernelGGL(k_reduce_rows, dim3(j + 1), dim3(256), 0, c->st, c->kred, (int)nbk, j + 1, c->hcol);
+      hipLaunchKernelGGL(k_maxpy_norm, dim3(nbu), dim3(256), 0, c->st, n4, j + 1, c->hcol, P, W, W, c->W2, c->kred);
+      hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, c->st, c->kred, (int)nbu, 1, c->hcol + j + 1);
+      HIPCHK(hipMemcpyAsync(hh.data(), c->hcol, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+      // classical Gram-Schmidt: ||w - sum h_i v_i||^2 was accumulated directly
+      const double hn = std::sqrt(std::max(hh[j + 1], 0.0));
+      for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = hh[i];
+      H[(size_t)(j + 1) * m + j] = hn;
+      it++;
+      if (!std::isfinite(hn)) { status = STK_ENAN; break; }
+      if (hn > 0) hipLaunchKernelGGL(k_scale_to, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, 1.0 / hn, W, W);
+      for (int i = 0; i < j; i++) {
+        const double a = H[(size_t)i * m + j], b = H[(size_t)(i + 1) * m + j];
+        H[(size_t)i * m + j] = cs[i] * a + sn[i] * b;
+        H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+      }
+      const double a = H[(size_t)j * m + j], b = H[(size_t)(j + 1) * m + j];
+      const double d = std::hypot(a, b);
+      cs[j] = d > 0 ? a / d : 1.0;
+      sn[j] = d > 0 ? b / d : 0.0;
+      H[(size_t)j * m + j] = d;
+      H[(size_t)(j + 1) * m + j] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      const double est = std::fabs(g[j + 1]) / nb;
+      if (it < 256) S->hist[it] = est;
+      if (o->verbose) fprintf(stderr, "pardiso2_amd: it %d relres~ %.3e\n", it, est);
+      if (est <= o->tol || hn == 0.0) { j++; break; }
+    }
+    if (status) break;
+    // x += Z y with H y = g (upper triangular, j columns)
+    const int k = j;
+    for (int i = k - 1; i >= 0; i--) {
+      double acc = g[i];
+      for (int q = i + 1; q < k; q++) acc -= H[(size_t)i * m + q] * y[q];
+      y[i] = acc / H[(size_t)i * m + i];
+    }
+    for (int i = 0; i < k; i++) y[i] = -y[i];
+    HIPCHK(hipMemcpyAsync(c->hcol, y.data(), k * sizeof(double), hipMemcpyHostToDevice, c->st));
+    PtrsK P{};
+    for (int i = 0; i < k; i++) P.p[i] = c->Zs[i];
+    hipLaunchKernelGGL(k_maxpy_norm, dim3(nbu), dim3(256), 0, c->st, n4, k, c->hcol, P, c->X, c->X, c->W2, c->kred);
+    true_residual(c);
+    double s4[4];
+    int rc = blocks_norms(c, c->R, c->B, s4);
+    if (rc) { status = rc; break; }
+    r = std::sqrt(s4[0] + s4[1] + s4[2]) / nb;
+    if (!std::isfinite(r)) { status = STK_ENAN; break; }
+  }
+  *rel = r;
+  return it;
+}
+
 int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   if (!c || !o) return STK_EINVAL;
   if (!c->prepared) return STK_ESTATE;
   if (o->restart < 1 || o->restart > 128 || o->maxit < 1 || !(o->tol > 0) || o->cycle < 0 || o->cycle > 2 ||
       o->coarse_viscmod < -1 || o->coarse_viscmod > 2 || o->schur < 0 || o->schur > 2 || o->ksweeps < 0 ||
-      o->ksweeps > 16 || !(o->kscale > 0))
+      o->ksweeps > 16 || !(o->kscale > 0) || o->krylov < 0 || o->krylov > 1)
     return STK_EINVAL;
   {
     const int want = (o->coarse_viscmod < 0) ? c->kn.viscmod : o->coarse_viscmod;
@@ -1301,7 +1424,7 @@ int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   const bool fp64 = o->precision == 1;
   int rc = ensure_vectors(c, !fp64, fp64);
   if (rc) return rc;
-  rc = ensure_outer(c, o->restart);
+  rc = ensure_outer(c, o->restart + 1);
   if (rc) return rc;
   stk_stats local;
   stk_stats *S = st ? st : &local;
@@ -1319,7 +1442,12 @@ int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   S->hist[0] = rel;
   int it = 0, k = 0;
   double hr[2];
-  while (rel > o->tol && it < o->maxit) {
+  if (o->krylov == 1) {
+    int status = 0;
+    it = fgmres(c, o, tile, nb, &rel, S, status);
+    S->status = status;
+  }
+  while (o->krylov == 0 && rel > o->tol && it < o->maxit) {
     double *Z = c->Zs[k], *Q = c->Qs[k];
     if (fp64) precond<double>(c, o, c->R, Z, tile);
     else precond<float>(c, o, c->R, Z, tile);
@@ -1412,8 +1540,10 @@ int stk_apply(stk_ctx *c, int l, const double *x, double *y) {
 
 // FP64 coupled sweeps on the reference arrays of level l, then fre0
 int stk_smooth(stk_ctx *c, int l, int sweeps, int ty, int tx, int tz) {
-  if (!c || !c->prepared || l < 0 || l > c->L || sweeps < 0 || ty < 1 || tx < 1 || tz < 1) return STK_EINVAL;
-  if ((size_t)(ty + 2) * (tx + 2) * (tz + 2) * 32 > 65536) return STK_EINVAL;
+  if (!c || !c->prepared || l < 0 || l > c->L || sweeps < 0) return STK_EINVAL;
+  const bool whole = (ty == 0 && tx == 0 && tz == 0);
+  if (whole && 4 * (size_t)c->lv[l].n * sizeof(double) > kLevelLds) return STK_EINVAL;
+  if (!whole && (ty < 1 || tx < 1 || tz < 1 || (size_t)(ty + 2) * (tx + 2) * (tz + 2) * 32 > 65536)) return STK_EINVAL;
   int rc = ensure_ref(c, l) || ensure_vectors(c, false, true);
   if (rc) return STK_ENOMEM;
   Level &L = c->lv[l];
@@ -1421,7 +1551,7 @@ int stk_smooth(stk_ctx *c, int l, int sweeps, int ty, int tx, int tz) {
   HIPCHK(hipMemcpyAsync(L.x64[L.cur64], L.ref[0], bytes, hipMemcpyDeviceToDevice, c->st));
   HIPCHK(hipMemcpyAsync(L.b64, L.ref[1], bytes, hipMemcpyDeviceToDevice, c->st));
   int tile[3] = {ty, tx, tz};
-  smooth<double>(c, l, sweeps, tile, 1);
+  smooth<double>(c, l, sweeps, tile, 1, whole);
   rowop<double, true>(c, l, L.x64[L.cur64], L.b64, L.ref[2], 1);
   HIPCHK(hipMemcpyAsync(L.ref[0], L.x64[L.cur64], bytes, hipMemcpyDeviceToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));

@@ -106,6 +106,29 @@ def test_coloured_sweep_matches_oracle(oracle_mod, tile):
         g.close()
 
 
+def test_whole_level_sweep_matches_oracle(oracle_mod):
+    """The LDS-resident whole-level smoother equals the tile sweep with one
+    tile covering the level (level 1 of a 17^3 hierarchy: 11^3 nodes)."""
+    pb, o, g = _pair(oracle_mod, seed=5)
+    try:
+        l = 1
+        X, Y, Z = o.info[l][:3]
+        b4, n4 = 4 * o.info[l][5], 4 * o.info[l][4]
+        rng = np.random.default_rng(6)
+        x = rng.standard_normal(n4) * 1e-16
+        f = rng.standard_normal(n4) * (o.arr("num0")[b4:b4 + n4] > 0)
+        o.arr("sol0")[b4:b4 + n4] = x
+        o.arr("fre1")[b4:b4 + n4] = f
+        g.vec_set(S.SOL0, l, x)
+        g.vec_set(S.FRE1, l, f)
+        o.sweep_tiles(3, l, Y, X, Z)
+        g.smooth(l, 3, tile=(0, 0, 0))
+        xo = o.arr("sol0")[b4:b4 + n4]
+        np.testing.assert_allclose(g.vec_get(S.SOL0, l), xo, rtol=1e-12, atol=1e-12 * np.abs(xo).max())
+    finally:
+        g.close()
+
+
 def test_restriction_and_prolongation_bit_exact(pair):
     pb, o, g = pair
     g.bind()
