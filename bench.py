@@ -80,6 +80,130 @@ def cpu_baseline(pb, pre, post, its_needed: int, budget_s: float):
                        f"device solve; oracle setup {setup_s:.1f} s not counted")}
 
 
+def smoother_roofline(ctx, nodes):
+    ms = ctx.time_kernel(0, 0, reps=50)
+    gbs = SMOOTHER_BYTES_PER_NODE * nodes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": f"k_smooth<float> finest level of a {round(nodes ** (1 / 3))}^3 instance",
+            "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS, "traffic": None,
+            "ms_per_launch": ms, "bytes_per_launch": SMOOTHER_BYTES_PER_NODE * nodes}
+
+
+def cpu_sample(pb, its_needed, budget_s):
+    pre, post = sweeps_for(pb.levels)
+    try:
+        return cpu_baseline(pb, pre, post, its_needed, budget_s)
+    except Exception as e:  # the baseline must not hide the device result
+        return {"value": None, "unit": "solves/s", "cores": 1, "kind": "port", "sample": f"failed: {e}"}
+
+
+def bench_batched(args):
+    """64 independent instances, sizes drawn from {33,49,65,97}^3, 1-4 seeded
+    inclusions each (contrast 1e0..1e6); a step prepares and solves all of
+    them concurrently (stk_solve_batch, one stream per instance)."""
+    from pardiso2_amd import _lib as S
+    from pardiso2_amd import setups
+    rng = np.random.default_rng(args.seed)
+    sizes = [int(v) for v in rng.choice([33, 49, 65, 97], size=args.instances)]
+    problems = [setups.inclusions(n, seed=args.seed * 1000 + i) for i, n in enumerate(sizes)]
+    ctxs = [S.from_problem(pb) for pb in problems]
+    log(f"{len(ctxs)} instances ready: " + ", ".join(f"{n}^3 x{sizes.count(n)}" for n in sorted(set(sizes))))
+    mk = lambda pb, prec: S.default_opts(pb.levels, precision=prec, tol=1e-8, maxit=args.maxit)
+    opts = [mk(pb, args.precision) for pb in problems]
+    for i in range(args.warmup):
+        rc, st = S.solve_batch(ctxs, opts, threads=args.threads, prepare=True)
+        log(f"warmup {i}: rc {rc}, iterations {min(s.iterations for s in st)}..{max(s.iterations for s in st)}")
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        rc, st = S.solve_batch(ctxs, opts, threads=args.threads, prepare=True)
+        lat.append(time.perf_counter() - ts)
+        log(f"batch: rc {rc}, {lat[-1]:.2f} s, worst relres {max(s.relres for s in st):.2e}")
+    total = time.perf_counter() - t0
+    conv = sum(1 for s in st if s.status == 0 and s.relres <= 1e-8)
+    # the plain FP64 single-instance path over the same instances, one at a time
+    ref = None
+    if not args.no_fp64_ref:
+        o64 = [mk(pb, 1) for pb in problems]
+        ts = time.perf_counter()
+        its64 = []
+        for c, o in zip(ctxs, o64):
+            c.prepare()
+            c.hydrostatic_guess()
+            its64.append(c.solve(o).iterations)
+        t64 = time.perf_counter() - ts
+        ref = {"fp64_serial_solves_per_s": len(ctxs) / t64, "fp64_serial_s": t64,
+               "fp64_iterations_mean": float(np.mean(its64))}
+        log(f"plain FP64, one instance at a time: {t64:.2f} s")
+    big = max(range(len(ctxs)), key=lambda i: sizes[i])
+    out = {
+        "metric": "Stokes solves/s to 1e-8 FP64 relative residual (batched instances)",
+        "value": args.instances * args.steps / total, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": total / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 smoother / f64 outer" if args.precision == 0 else "f64", "data": "synthetic",
+        "config": {"workload": f"batched: {args.instances} instances, sizes from {{33,49,65,97}}^3, 1-4 inclusions, "
+                               f"contrast 1e0..1e6, {args.threads} concurrent streams",
+                   "sizes": {str(n): sizes.count(n) for n in sorted(set(sizes))}, "seed": args.seed, "tol": 1e-8},
+        "converged": conv, "p50_ms": float(np.percentile(np.array(lat) * 1e3, 50)),
+        "p99_ms": float(np.percentile(np.array(lat) * 1e3, 99)),
+        "iterations_mean": float(np.mean([s.iterations for s in st])),
+        "fp64_single_instance": ref,
+        "speedup_vs_fp64_single_instance": (args.instances * args.steps / total) / ref["fp64_serial_solves_per_s"]
+        if ref else None,
+        "roofline": smoother_roofline(ctxs[big], ctxs[big].n0),
+        "cpu_baseline": None if args.no_cpu else cpu_sample(problems[0], st[0].iterations, args.cpu_budget),
+    }
+    print(json.dumps(out), flush=True)
+    for c in ctxs:
+        c.close()
+
+
+def bench_served(args):
+    """Poisson arrivals of Stokes requests, sizes {33,49,65,97,129}^3 with
+    weights .3/.25/.2/.15/.1, random-field viscosity (contrast 1e2..1e6),
+    two-phase density; continuous batching over resident contexts."""
+    from pardiso2_amd import serve
+    from pardiso2_amd import setups
+    sizes, weights = [33, 49, 65, 97, 129], [0.3, 0.25, 0.2, 0.15, 0.1]
+    rng = np.random.default_rng(args.seed)
+    payloads = {}
+    for n in sizes:
+        payloads[n] = [serve.payload_from_problem(setups.served_request(n, args.seed * 100 + k,
+                                                                        float(rng.uniform(2.0, 6.0))))
+                       for k in range(3)]
+    log("payloads ready (node fields of 3 seeded requests per size)")
+    trace = serve.poisson_trace(args.rate, args.duration, sizes, weights, args.seed)
+    srv = serve.Server(workers=args.threads, opts_kw=dict(precision=args.precision, tol=1e-8, maxit=args.maxit))
+    srv.start()
+    # warm the per-size context pools (one request of each size, not timed)
+    warm = [(0.0, n) for n in sizes]
+    serve.run_stream(srv, warm, payloads, seed=args.seed)
+    srv.done.clear()
+    log(f"replaying {len(trace)} arrivals over {args.duration:.0f} s at {args.rate:.0f}/s")
+    done, t0 = serve.run_stream(srv, trace, payloads, seed=args.seed + 1)
+    srv.stop()
+    lat = np.array([(r.done - r.arrival) * 1e3 for r in done])
+    makespan = max(r.done for r in done) - t0
+    ok = [r for r in done if not r.error and r.stats is not None and r.stats.status == 0]
+    out = {
+        "metric": "Stokes solves/s and p50/p99 time-to-converged at 1e-8 (served stream)",
+        "value": len(ok) / makespan, "unit": "solves/s", "n_gpus": 1, "steps": len(trace), "warmup": len(warm),
+        "ms_per_step": makespan / max(len(trace), 1) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 smoother / f64 outer" if args.precision == 0 else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"served: Poisson {args.rate:.0f} req/s for {args.duration:.0f} s, sizes "
+                               "{33,49,65,97,129}^3 (.3/.25/.2/.15/.1), random-field contrast 1e2..1e6",
+                   "workers": args.threads, "seed": args.seed, "tol": 1e-8},
+        "requests": len(trace), "converged": len(ok), "errors": sum(1 for r in done if r.error),
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "service_p50_ms": float(np.percentile([(r.done - r.start) * 1e3 for r in done], 50)),
+        "makespan_s": makespan, "contexts_created": srv.pool.created,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,11 +221,19 @@ def main():
     ap.add_argument("--kernel", choices=sorted(KERNELS), default=None,
                     help="only launch one finest-level kernel --reps times (for counter runs) and exit")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--instances", type=int, default=64, help="batched workload: number of instances")
+    ap.add_argument("--threads", type=int, default=16, help="batched/served: host workers (one stream each)")
+    ap.add_argument("--rate", type=float, default=50.0, help="served workload: arrivals per second")
+    ap.add_argument("--duration", type=float, default=60.0, help="served workload: arrival window in seconds")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
               "(instances shard by request; see the design notes).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        return bench_batched(args)
+    if args.workload == "served":
+        return bench_served(args)
 
     from pardiso2_amd import _lib as S
     pb = make_problem(args.workload, args.n, args.seed)

@@ -24,6 +24,7 @@
  */
 #ifndef PARDISO2_AMD_H
 #define PARDISO2_AMD_H
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -150,6 +151,19 @@ int stk_apply(stk_ctx *ctx, int level, const double *x, double *y);
  * kernel the V-cycle uses on levels of at most 2048 nodes in FP64). */
 int stk_smooth(stk_ctx *ctx, int level, int sweeps, int ty, int tx, int tz);
 int stk_coarse_solve(stk_ctx *ctx);
+
+/* ---- many independent instances --------------------------------------- */
+/* Solve n contexts concurrently: `threads` host workers (<= 0: 16) take the
+ * contexts in order, each context on its own stream, so the device overlaps
+ * the instances.  mode 0: solve from the current solution; mode 1: prepare,
+ * hydrostatic guess, then solve.  opts holds 1 (shared) or n entries, stats n
+ * entries (or NULL).  Returns STK_OK, else the first hard error, else
+ * STK_ENOCONV if some instance did not converge.  Results equal those of
+ * handling each context alone. */
+int stk_solve_batch(int n, stk_ctx *const *ctxs, const stk_solver_opts *opts, int nopts, stk_stats *stats,
+                    int threads, int mode);
+/* free and total device memory (admission control of a request loop) */
+int stk_mem_info(size_t *free_bytes, size_t *total_bytes);
 
 /* ---- Schur approximation, for inspection ------------------------------ */
 /* y = K_l x with K_l the pressure-cell operator of level l (level 0:

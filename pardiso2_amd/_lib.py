@@ -24,6 +24,7 @@ EXPORTS = [
     "stk_pressure_gauge", "stk_hydrostatic_guess", "stk_time_kernel", "stk_vec_set", "stk_vec_get", "stk_apply", "stk_smooth", "stk_coarse_solve",
     "stk_bind", "matclear", "gausmat", "gausmat2", "pardisomat", "gausmat4", "vprestrict", "vpprolong",
     "ronurestrict", "vpiterate", "stk_kop_apply", "stk_kcycle_apply", "stk_schur_apply",
+    "stk_solve_batch", "stk_mem_info",
 ]
 
 
@@ -91,6 +92,9 @@ def lib():
         L.stk_coarse_solve.argtypes = [vp]
         L.stk_bind.argtypes = [vp]
         L.stk_kop_apply.argtypes = [vp, C.c_int, dp, dp]
+        L.stk_solve_batch.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(Opts), C.c_int, C.POINTER(Stats), C.c_int,
+                                      C.c_int]
+        L.stk_mem_info.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
         L.stk_kcycle_apply.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, dp]
         L.stk_schur_apply.argtypes = [vp, C.POINTER(Opts), dp, dp]
         L.matclear.argtypes = [C.c_int, dp]
@@ -288,6 +292,31 @@ class Stokes:
 
 
 SOL0, FRE1, FRE0 = 0, 1, 2
+
+
+def solve_batch(contexts, opts=None, threads=16, prepare=False):
+    """Solve contexts concurrently (stk_solve_batch); opts is one Opts for all
+    or a list; prepare=True also prepares each context and seeds the
+    hydrostatic guess inside the workers; returns (rc, [Stats])."""
+    n = len(contexts)
+    hs = (C.c_void_p * n)(*[c.h.value if isinstance(c.h, C.c_void_p) else c.h for c in contexts])
+    if opts is None:
+        opts = default_opts(contexts[0].levels)
+    if isinstance(opts, Opts):
+        oa, no = (Opts * 1)(opts), 1
+    else:
+        oa, no = (Opts * n)(*opts), n
+    st = (Stats * n)()
+    rc = lib().stk_solve_batch(n, hs, oa, no, st, threads, 1 if prepare else 0)
+    if rc not in (0, -5, -6):
+        check(rc, "stk_solve_batch")
+    return rc, list(st)
+
+
+def mem_info():
+    f, t = C.c_size_t(), C.c_size_t()
+    check(lib().stk_mem_info(C.byref(f), C.byref(t)), "stk_mem_info")
+    return f.value, t.value
 
 
 def from_problem(pb, markers=True, knobs=None):

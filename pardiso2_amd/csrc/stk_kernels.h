@@ -746,6 +746,89 @@ __global__ void __launch_bounds__(256) k_maxpy_norm(long n4, int k, const double
   if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// Delayed classical Gram-Schmidt with reorthogonalisation (one pass for the
+// dots, one for the updates per Arnoldi step).  Pass 1: partial sums of
+// a_i = <V_i,u>, b_i = <V_i,w> (i < k), <u,w>, <u,u> in the W2 inner product.
+__global__ void __launch_bounds__(256) k_dots2(long n4, const double *__restrict__ w2, const double *__restrict__ u,
+                                               const double *__restrict__ w, PtrsK V, int k, double *partial) {
+  __shared__ double acc[2 * kMaxDirs + 2][4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nd = 2 * k + 2;
+  for (int i = threadIdx.x; i < (2 * kMaxDirs + 2) * 4; i += blockDim.x) acc[i >> 2][i & 3] = 0.0;
+  __syncthreads();
+  constexpr int E = 4;
+  for (long base = (long)blockIdx.x * 256 * E; base < n4; base += (long)gridDim.x * 256 * E) {
+    double wu[E], ww[E];
+    double suw = 0.0, suu = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const long p = base + threadIdx.x + 256L * e;
+      const double q = (p < n4) ? w2[p] : 0.0;
+      const double uu = (p < n4) ? u[p] : 0.0, wv = (p < n4) ? w[p] : 0.0;
+      wu[e] = q * uu;
+      ww[e] = q * wv;
+      suw += wu[e] * wv;
+      suu += wu[e] * uu;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      suw += __shfl_xor(suw, off);
+      suu += __shfl_xor(suu, off);
+    }
+    if (lane == 0) { acc[2 * k][wave] += suw; acc[2 * k + 1][wave] += suu; }
+    for (int i = 0; i < k; i++) {
+      const double *__restrict__ v = V.p[i];
+      double sa = 0.0, sb = 0.0;
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        const long p = base + threadIdx.x + 256L * e;
+        if (p < n4) {
+          const double vv = v[p];
+          sa += wu[e] * vv;
+          sb += ww[e] * vv;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        sa += __shfl_xor(sa, off);
+        sb += __shfl_xor(sb, off);
+      }
+      if (lane == 0) { acc[i][wave] += sa; acc[k + i][wave] += sb; }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nd; i += blockDim.x)
+    partial[(long)i * gridDim.x + blockIdx.x] = (acc[i][0] + acc[i][1]) + (acc[i][2] + acc[i][3]);
+}
+
+// Pass 2: u <- (u - sum_i a_i V_i) * su (the reorthogonalised, normalised
+// basis vector v_k), w <- w - sum_i h_i V_i - h_k v_k, partial sums of w2 w^2.
+// coef = [a_0..a_{k-1}, h_0..h_{k-1}, h_k, su]
+__global__ void __launch_bounds__(256) k_update2(long n4, int k, const double *__restrict__ coef, PtrsK V, double *u,
+                                                 double *w, const double *__restrict__ w2, double *partial) {
+  __shared__ double sc[2 * kMaxDirs + 2];
+  __shared__ double red[4];
+  for (int i = threadIdx.x; i < 2 * k + 2; i += blockDim.x) sc[i] = coef[i];
+  __syncthreads();
+  const double hk = sc[2 * k], su = sc[2 * k + 1];
+  double ss = 0.0;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n4; p += (long)gridDim.x * blockDim.x) {
+    double uu = u[p], wv = w[p];
+    for (int i = 0; i < k; i++) {
+      const double vv = V.p[i][p];
+      uu -= sc[i] * vv;
+      wv -= sc[k + i] * vv;
+    }
+    uu *= su;
+    wv -= hk * uu;
+    u[p] = uu;
+    w[p] = wv;
+    ss += w2[p] * wv * wv;
+  }
+  for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 // y = s * x
 __global__ void k_scale_to(long n4, double s, const double *__restrict__ x, double *y) {
   long u = blockIdx.x * (long)blockDim.x + threadIdx.x;

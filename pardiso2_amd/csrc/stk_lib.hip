@@ -8,7 +8,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "../../include/pardiso2_amd.h"
@@ -254,6 +256,19 @@ int upload(T **d, const std::vector<T> &h) {
   return STK_OK;
 }
 
+// stream-ordered copies that complete before returning (the library stream
+// does not synchronise with the null stream)
+#define H2D(c, dst, src, bytes)                                                         \
+  do {                                                                                  \
+    HIPCHK(hipMemcpyAsync((dst), (src), (bytes), hipMemcpyHostToDevice, (c)->st));      \
+    HIPCHK(hipStreamSynchronize((c)->st));                                              \
+  } while (0)
+#define D2H(c, dst, src, bytes)                                                         \
+  do {                                                                                  \
+    HIPCHK(hipMemcpyAsync((dst), (src), (bytes), hipMemcpyDeviceToHost, (c)->st));      \
+    HIPCHK(hipStreamSynchronize((c)->st));                                              \
+  } while (0)
+
 int upload_axis(AxisDev &d, const AxisHost &h) {
   d.release();
   std::vector<float> w32(h.w.begin(), h.w.end()), cw32(h.cw.begin(), h.cw.end());
@@ -317,14 +332,14 @@ int ensure_vectors(stk_ctx *c, bool want32, bool want64) {
     if (want32 && !L.b32) {
       if (dalloc(&L.x32[0], n4) || dalloc(&L.x32[1], n4) || dalloc(&L.b32, n4) || dalloc(&L.r32, n4))
         return STK_ENOMEM;
-      hipMemset(L.x32[0], 0, n4 * 4); hipMemset(L.x32[1], 0, n4 * 4);
-      hipMemset(L.b32, 0, n4 * 4); hipMemset(L.r32, 0, n4 * 4);
+      hipMemsetAsync(L.x32[0], 0, n4 * 4, c->st); hipMemsetAsync(L.x32[1], 0, n4 * 4, c->st);
+      hipMemsetAsync(L.b32, 0, n4 * 4, c->st); hipMemsetAsync(L.r32, 0, n4 * 4, c->st);
     }
     if (want64 && !L.b64) {
       if (dalloc(&L.x64[0], n4) || dalloc(&L.x64[1], n4) || dalloc(&L.b64, n4) || dalloc(&L.r64, n4))
         return STK_ENOMEM;
-      hipMemset(L.x64[0], 0, n4 * 8); hipMemset(L.x64[1], 0, n4 * 8);
-      hipMemset(L.b64, 0, n4 * 8); hipMemset(L.r64, 0, n4 * 8);
+      hipMemsetAsync(L.x64[0], 0, n4 * 8, c->st); hipMemsetAsync(L.x64[1], 0, n4 * 8, c->st);
+      hipMemsetAsync(L.b64, 0, n4 * 8, c->st); hipMemsetAsync(L.r64, 0, n4 * 8, c->st);
     }
   }
   return STK_OK;
@@ -588,9 +603,9 @@ int ensure_outer(stk_ctx *c, int m) {
   if (!c->X) {
     if (dalloc(&c->X, n4) || dalloc(&c->B, n4) || dalloc(&c->R, n4) || dalloc(&c->W2, n4) || dalloc(&c->T1, n4) ||
         dalloc(&c->red, (size_t)c->nred * 16) || dalloc(&c->scal, 256) ||
-        dalloc(&c->kred, (size_t)c->nred * (kMaxDirs + 1)) || dalloc(&c->hcol, 2 * kMaxDirs + 2))
+        dalloc(&c->kred, (size_t)c->nred * (2 * kMaxDirs + 2)) || dalloc(&c->hcol, 2 * kMaxDirs + 2))
       return STK_ENOMEM;
-    hipMemset(c->X, 0, n4 * 8);
+    hipMemsetAsync(c->X, 0, n4 * 8, c->st);
   }
   while ((int)c->Zs.size() < m) {
     double *z, *q;
@@ -643,11 +658,12 @@ template <typename T>
 static void kcycle_host(stk_ctx *c, int nu, double scale, const double *b, double *x) {
   Level &F = c->lv[0];
   std::vector<T> hb(b, b + F.n), hx(F.n);
-  hipMemcpy(kb<T>(F), hb.data(), F.n * sizeof(T), hipMemcpyHostToDevice);
+  hipMemcpyAsync(kb<T>(F), hb.data(), F.n * sizeof(T), hipMemcpyHostToDevice, c->st);
   hipMemsetAsync(kx<T>(F), 0, F.n * sizeof(T), c->st);
   kcycle<T>(c, 0, nu, (T)scale);
   hipStreamSynchronize(c->st);
-  hipMemcpy(hx.data(), kx<T>(F), F.n * sizeof(T), hipMemcpyDeviceToHost);
+  hipMemcpyAsync(hx.data(), kx<T>(F), F.n * sizeof(T), hipMemcpyDeviceToHost, c->st);
+  hipStreamSynchronize(c->st);
   for (long i = 0; i < F.n; i++) x[i] = (double)hx[i];
 }
 
@@ -690,7 +706,7 @@ int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *
   c->bondv1.assign(2, 0.f);
   c->bondn1.assign(1, 0);
   c->kn = stk_knobs{0.10, 0.0, 0.10, 1.0, 1.0, 1.0, 1e4, 0, 0.0, 0.0, 0.0};
-  if (hipStreamCreate(&c->st) != hipSuccess) { delete c; return STK_EHIP; }
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; return STK_EHIP; }
   hipEventCreate(&c->ev0);
   hipEventCreate(&c->ev1);
   // per-level geometry tables
@@ -717,7 +733,11 @@ int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *
     stk_destroy(c);
     return STK_ENOMEM;
   }
-  hipMemset(c->nu, 0, off * 4); hipMemset(c->ro, 0, off * 4); hipMemset(c->nd, 0, off * 4);
+  hipMemsetAsync(c->nu, 0, off * 4, c->st); hipMemsetAsync(c->ro, 0, off * 4, c->st);
+  hipMemsetAsync(c->nd, 0, off * 4, c->st);
+  // stress-point viscosities no marker reaches read as 0 before the clamp
+  for (float *p : {c->nxx, c->nxy, c->nxz, c->nyz}) hipMemsetAsync(p, 0, c->lv[0].n * 4, c->st);
+  hipStreamSynchronize(c->st);
   // transfer tables
   for (int l = 1; l <= levels; l++) {
     Level &F = c->lv[l - 1], &C = c->lv[l];
@@ -794,7 +814,7 @@ void stk_default_opts(stk_solver_opts *o, int levels) {
   o->tol = 1e-8;
   o->maxit = 800;
   o->schur = 1;
-  o->krylov = 0;  // flexible GMRES with one classical Gram-Schmidt pass loses orthogonality: GCR needs fewer iterations
+  o->krylov = 1;  // GCR can stagnate when the preconditioned operator is indefinite (sharp strong inclusions)
   o->ksweeps = 2;
   o->kscale = 2.0;  // piecewise-constant aggregation undershoots the coarse correction by about 2
 }
@@ -866,12 +886,12 @@ int stk_set_fields(stk_ctx *c, const float *nu, const float *ro, const float *nx
     if (!(nu[i] > 0) || !std::isfinite(nu[i]) || !std::isfinite(ro[i]) || !(nxx[i] > 0) || !(nxy[i] > 0) ||
         !(nxz[i] > 0) || !(nyz[i] > 0))
       return STK_EINVAL;
-  HIPCHK(hipMemcpy(c->nu, nu, n0 * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->ro, ro, n0 * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->nxx, nxx, n0 * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->nxy, nxy, n0 * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->nxz, nxz, n0 * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->nyz, nyz, n0 * 4, hipMemcpyHostToDevice));
+  H2D(c, c->nu, nu, n0 * 4);
+  H2D(c, c->ro, ro, n0 * 4);
+  H2D(c, c->nxx, nxx, n0 * 4);
+  H2D(c, c->nxy, nxy, n0 * 4);
+  H2D(c, c->nxz, nxz, n0 * 4);
+  H2D(c, c->nyz, nyz, n0 * 4);
   c->fields_set = true;
   c->prepared = false;
   return STK_OK;
@@ -881,13 +901,13 @@ int stk_get_fields(stk_ctx *c, float *nu, float *ro, float *nd, float *nxx, floa
   if (!c) return STK_EINVAL;
   long n0 = c->lv[0].n;
   HIPCHK(hipStreamSynchronize(c->st));
-  if (nu) HIPCHK(hipMemcpy(nu, c->nu, c->ntot * 4, hipMemcpyDeviceToHost));
-  if (ro) HIPCHK(hipMemcpy(ro, c->ro, c->ntot * 4, hipMemcpyDeviceToHost));
-  if (nd) HIPCHK(hipMemcpy(nd, c->nd, c->ntot * 4, hipMemcpyDeviceToHost));
-  if (nxx) HIPCHK(hipMemcpy(nxx, c->nxx, n0 * 4, hipMemcpyDeviceToHost));
-  if (nxy) HIPCHK(hipMemcpy(nxy, c->nxy, n0 * 4, hipMemcpyDeviceToHost));
-  if (nxz) HIPCHK(hipMemcpy(nxz, c->nxz, n0 * 4, hipMemcpyDeviceToHost));
-  if (nyz) HIPCHK(hipMemcpy(nyz, c->nyz, n0 * 4, hipMemcpyDeviceToHost));
+  if (nu) D2H(c, nu, c->nu, c->ntot * 4);
+  if (ro) D2H(c, ro, c->ro, c->ntot * 4);
+  if (nd) D2H(c, nd, c->nd, c->ntot * 4);
+  if (nxx) D2H(c, nxx, c->nxx, n0 * 4);
+  if (nxy) D2H(c, nxy, c->nxy, n0 * 4);
+  if (nxz) D2H(c, nxz, c->nxz, n0 * 4);
+  if (nyz) D2H(c, nyz, c->nyz, n0 * 4);
   return STK_OK;
 }
 
@@ -975,8 +995,8 @@ static int build_schur(stk_ctx *c) {
     if (dalloc(&V.kx32, V.n) || dalloc(&V.kb32, V.n) || dalloc(&V.kr32, V.n) || dalloc(&V.kx64, V.n) ||
         dalloc(&V.kb64, V.n) || dalloc(&V.kr64, V.n))
       return STK_ENOMEM;
-    HIPCHK(hipMemset(V.kx32, 0, V.n * 4));
-    HIPCHK(hipMemset(V.kx64, 0, V.n * 8));
+    HIPCHK(hipMemsetAsync(V.kx32, 0, V.n * 4, c->st));
+    HIPCHK(hipMemsetAsync(V.kx64, 0, V.n * 8, c->st));
   }
   if (!c->ktabs) {
     for (int l = 1; l <= L; l++) {
@@ -1013,7 +1033,7 @@ static int build_schur(stk_ctx *c) {
   HIPCHK(hipStreamSynchronize(c->st));
   for (int q = 0; q < 7; q++) {
     h[q].resize(C.n);
-    HIPCHK(hipMemcpy(h[q].data(), C.k64[q], C.n * 8, hipMemcpyDeviceToHost));
+    D2H(c, h[q].data(), C.k64[q], C.n * 8);
   }
   std::vector<int> idx(C.n, -1), cell;
   for (long n = 0; n < C.n; n++)
@@ -1154,7 +1174,7 @@ static int build_rows(stk_ctx *c) {
     if (upload(&L.bc_r, L.hbc_r) || upload(&L.bc_j, L.hbc_j) || upload(&L.bc_negk, L.hbc_negk) ||
         upload(&L.bc_c, L.hbc_c))
       return STK_ENOMEM;
-    HIPCHK(hipMemcpy(L.flags, L.hflags.data(), L.n, hipMemcpyHostToDevice));
+    H2D(c, L.flags, L.hflags.data(), L.n);
   }
   return STK_OK;
 }
@@ -1220,14 +1240,14 @@ int stk_set_solution(stk_ctx *c, const double *x) {
   if (!c || !x) return STK_EINVAL;
   int rc = ensure_outer(c, 1);
   if (rc) return rc;
-  HIPCHK(hipMemcpy(c->X, x, 4 * c->lv[0].n * 8, hipMemcpyHostToDevice));
+  H2D(c, c->X, x, 4 * c->lv[0].n * 8);
   return STK_OK;
 }
 
 int stk_get_solution(stk_ctx *c, double *x) {
   if (!c || !x || !c->X) return STK_EINVAL;
   HIPCHK(hipStreamSynchronize(c->st));
-  HIPCHK(hipMemcpy(x, c->X, 4 * c->lv[0].n * 8, hipMemcpyDeviceToHost));
+  D2H(c, x, c->X, 4 * c->lv[0].n * 8);
   return STK_OK;
 }
 
@@ -1318,74 +1338,111 @@ int stk_time_kernel(stk_ctx *c, int which, int level, int reps, double *ms_avg) 
   return STK_OK;
 }
 
-// Flexible GMRES(m), right-preconditioned, in the W-weighted inner product:
-// V_j orthonormal in <.,.>_W (classical Gram-Schmidt, one pass over the
-// basis for the dots and one for the update), Z_j = M^-1 V_j kept for the
-// solution update at the end of each cycle, Givens rotations on the host.
-// Returns the number of iterations done; *rel is the true relative residual
-// after the last cycle.
+// Flexible GMRES(m), right-preconditioned, in the W-weighted inner product,
+// with delayed classical Gram-Schmidt reorthogonalisation: step j
+// preconditions the once-orthogonalised candidate u (Z_j = M^-1 u), forms
+// w = A Z_j, then one pass computes the dots of u and w with the basis and a
+// second pass reorthogonalises u into v_j and orthogonalises w against
+// v_0..v_j.  The Hessenberg column of step j-1 is corrected by u's
+// reorthogonalisation before its Givens rotation, so the residual estimate
+// lags one step (the last step's preconditioning is discarded on exit).
+// Returns the number of Arnoldi steps used; *rel is the true relative
+// residual after the last cycle.
 static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double nb, double *rel, stk_stats *S,
                   int &status) {
   const long n4 = 4 * c->lv[0].n;
   const int m = o->restart;
   const bool fp64 = o->precision == 1;
-  const unsigned nbk = (unsigned)std::min<long>(c->nred, std::max<long>(1, (n4 + 2047) / 2048));
+  const unsigned nbk = (unsigned)std::min<long>(c->nred, std::max<long>(1, (n4 + 1023) / 1024));
   const unsigned nbu = blocks(n4, 256) < (unsigned)c->nred ? blocks(n4, 256) : (unsigned)c->nred;
-  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), hh(m + 2), y(m);
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), hh(2 * m + 2), y(m), coef(2 * m + 2);
+  auto h = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
+  auto givens = [&](int col) {
+    for (int i = 0; i < col; i++) {
+      const double a = h(i, col), b = h(i + 1, col);
+      h(i, col) = cs[i] * a + sn[i] * b;
+      h(i + 1, col) = -sn[i] * a + cs[i] * b;
+    }
+    const double a = h(col, col), b = h(col + 1, col);
+    const double d = std::hypot(a, b);
+    cs[col] = d > 0 ? a / d : 1.0;
+    sn[col] = d > 0 ? b / d : 0.0;
+    h(col, col) = d;
+    h(col + 1, col) = 0.0;
+    g[col + 1] = -sn[col] * g[col];
+    g[col] = cs[col] * g[col];
+  };
   int it = 0;
   double r = *rel;
   while (r > o->tol && it < o->maxit) {
-    // v_0 = R / ||R||_W
     const double beta = r * nb;
     hipLaunchKernelGGL(k_scale_to, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, 1.0 / beta, c->R, c->Qs[0]);
     std::fill(g.begin(), g.end(), 0.0);
+    std::fill(H.begin(), H.end(), 0.0);
     g[0] = beta;
-    int j = 0;
-    for (; j < m && it < o->maxit; j++) {
-      double *Z = c->Zs[j], *W = c->Qs[j + 1];
-      if (fp64) precond<double>(c, o, c->Qs[j], Z, tile);
-      else precond<float>(c, o, c->Qs[j], Z, tile);
+    int j = 0, ncols = 0;
+    bool conv = false;
+    while (j < m && it < o->maxit) {
+      double *u = c->Qs[j], *Z = c->Zs[j], *W = c->Qs[j + 1];
+      if (fp64) precond<double>(c, o, u, Z, tile);
+      else precond<float>(c, o, u, Z, tile);
       rowop<double, false>(c, 0, Z, nullptr, W, 1);
       PtrsK P{};
-      for (int i = 0; i <= j; i++) P.p[i] = c->Qs[i];
-      hipLaunchKernelGGL(k_mdot, dim3(nbk), dim3(256), 0, c->st, n4, c->W2, W, P, j + 1, c->kred);
-      hipLaunchKernelGGL(k_reduce_rows, dim3(j + 1), dim3(256), 0, c->st, c->kred, (int)nbk, j + 1, c->hcol);
-      hipLaunchKernelGGL(k_maxpy_norm, dim3(nbu), dim3(256), 0, c->st, n4, j + 1, c->hcol, P, W, W, c->W2, c->kred);
-      hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, c->st, c->kred, (int)nbu, 1, c->hcol + j + 1);
-      HIPCHK(hipMemcpyAsync(hh.data(), c->hcol, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, c->st));
+      for (int i = 0; i < j; i++) P.p[i] = c->Qs[i];
+      hipLaunchKernelGGL(k_dots2, dim3(nbk), dim3(256), 0, c->st, n4, c->W2, u, W, P, j, c->kred);
+      hipLaunchKernelGGL(k_reduce_rows, dim3(2 * j + 2), dim3(256), 0, c->st, c->kred, (int)nbk, 2 * j + 2, c->hcol);
+      HIPCHK(hipMemcpyAsync(hh.data(), c->hcol, (2 * j + 2) * sizeof(double), hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
-      // classical Gram-Schmidt: ||w - sum h_i v_i||^2 was accumulated directly
-      const double hn = std::sqrt(std::max(hh[j + 1], 0.0));
-      for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = hh[i];
-      H[(size_t)(j + 1) * m + j] = hn;
-      it++;
-      if (!std::isfinite(hn)) { status = STK_ENAN; break; }
-      if (hn > 0) hipLaunchKernelGGL(k_scale_to, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, 1.0 / hn, W, W);
-      for (int i = 0; i < j; i++) {
-        const double a = H[(size_t)i * m + j], b = H[(size_t)(i + 1) * m + j];
-        H[(size_t)i * m + j] = cs[i] * a + sn[i] * b;
-        H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+      const double *a = hh.data(), *b = hh.data() + j;
+      const double cuw = hh[2 * j], cuu = hh[2 * j + 1];
+      double asq = 0.0, ab = 0.0;
+      for (int i = 0; i < j; i++) { asq += a[i] * a[i]; ab += a[i] * b[i]; }
+      const double nu = std::sqrt(cuu - asq > 0.5 * cuu ? cuu - asq : cuu);
+      if (!std::isfinite(nu) || nu <= 0) { status = STK_ENAN; break; }
+      if (j >= 1) {
+        // u = V a + nu v_j: fold the reorthogonalisation into column j-1
+        const double hj = h(j, j - 1);
+        for (int i = 0; i < j; i++) h(i, j - 1) += hj * a[i];
+        h(j, j - 1) = hj * nu;
+        givens(j - 1);
+        const double est = std::fabs(g[j]) / nb;
+        if (it < 256) S->hist[it] = est;
+        if (o->verbose) fprintf(stderr, "pardiso2_amd: it %d relres~ %.3e\n", it, est);
+        ncols = j;
+        if (est <= o->tol) { conv = true; break; }
       }
-      const double a = H[(size_t)j * m + j], b = H[(size_t)(j + 1) * m + j];
-      const double d = std::hypot(a, b);
-      cs[j] = d > 0 ? a / d : 1.0;
-      sn[j] = d > 0 ? b / d : 0.0;
-      H[(size_t)j * m + j] = d;
-      H[(size_t)(j + 1) * m + j] = 0.0;
-      g[j + 1] = -sn[j] * g[j];
-      g[j] = cs[j] * g[j];
-      const double est = std::fabs(g[j + 1]) / nb;
-      if (it < 256) S->hist[it] = est;
-      if (o->verbose) fprintf(stderr, "pardiso2_amd: it %d relres~ %.3e\n", it, est);
-      if (est <= o->tol || hn == 0.0) { j++; break; }
+      const double hjj = (cuw - ab) / nu;
+      for (int i = 0; i < j; i++) { coef[i] = a[i]; coef[j + i] = b[i]; }
+      coef[2 * j] = hjj;
+      coef[2 * j + 1] = 1.0 / nu;
+      HIPCHK(hipMemcpyAsync(c->hcol, coef.data(), (2 * j + 2) * sizeof(double), hipMemcpyHostToDevice, c->st));
+      hipLaunchKernelGGL(k_update2, dim3(nbu), dim3(256), 0, c->st, n4, j, c->hcol, P, u, W, c->W2, c->kred);
+      hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, c->st, c->kred, (int)nbu, 1, c->hcol + 2 * m + 1);
+      double hn2 = 0.0;
+      HIPCHK(hipMemcpyAsync(&hn2, c->hcol + 2 * m + 1, sizeof(double), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+      const double hn = std::sqrt(std::max(hn2, 0.0));
+      if (!std::isfinite(hn)) { status = STK_ENAN; break; }
+      for (int i = 0; i < j; i++) h(i, j) = b[i];
+      h(j, j) = hjj;
+      h(j + 1, j) = hn;
+      it++;
+      j++;
+      if (hn == 0.0) break;
+      hipLaunchKernelGGL(k_scale_to, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, 1.0 / hn, W, W);
     }
     if (status) break;
-    // x += Z y with H y = g (upper triangular, j columns)
-    const int k = j;
+    if (!conv && j > ncols) {
+      givens(j - 1);  // the last column keeps its single orthogonalisation
+      ncols = j;
+      if (it < 256) S->hist[it] = std::fabs(g[j]) / nb;
+    }
+    // x += Z y with H y = g over the ncols finished columns
+    const int k = ncols;
     for (int i = k - 1; i >= 0; i--) {
       double acc = g[i];
-      for (int q = i + 1; q < k; q++) acc -= H[(size_t)i * m + q] * y[q];
-      y[i] = acc / H[(size_t)i * m + i];
+      for (int q = i + 1; q < k; q++) acc -= h(i, q) * y[q];
+      y[i] = acc / h(i, i);
     }
     for (int i = 0; i < k; i++) y[i] = -y[i];
     HIPCHK(hipMemcpyAsync(c->hcol, y.data(), k * sizeof(double), hipMemcpyHostToDevice, c->st));
@@ -1503,7 +1560,7 @@ static int ensure_ref(stk_ctx *c, int l) {
   for (int k = 0; k < 3; k++)
     if (!L.ref[k]) {
       if (dalloc(&L.ref[k], 4 * L.n)) return STK_ENOMEM;
-      hipMemset(L.ref[k], 0, 4 * L.n * 8);
+      hipMemsetAsync(L.ref[k], 0, 4 * L.n * 8, c->st);
     }
   return STK_OK;
 }
@@ -1512,7 +1569,7 @@ int stk_vec_set(stk_ctx *c, int which, int l, const double *h) {
   if (!c || which < 0 || which > 2 || l < 0 || l > c->L || !h) return STK_EINVAL;
   int rc = ensure_ref(c, l);
   if (rc) return rc;
-  HIPCHK(hipMemcpy(c->lv[l].ref[which], h, 4 * c->lv[l].n * 8, hipMemcpyHostToDevice));
+  H2D(c, c->lv[l].ref[which], h, 4 * c->lv[l].n * 8);
   return STK_OK;
 }
 
@@ -1521,7 +1578,7 @@ int stk_vec_get(stk_ctx *c, int which, int l, double *h) {
   int rc = ensure_ref(c, l);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(c->st));
-  HIPCHK(hipMemcpy(h, c->lv[l].ref[which], 4 * c->lv[l].n * 8, hipMemcpyDeviceToHost));
+  D2H(c, h, c->lv[l].ref[which], 4 * c->lv[l].n * 8);
   return STK_OK;
 }
 
@@ -1530,10 +1587,10 @@ int stk_apply(stk_ctx *c, int l, const double *x, double *y) {
   Level &L = c->lv[l];
   double *dx, *dy;
   if (dalloc(&dx, 4 * L.n) || dalloc(&dy, 4 * L.n)) return STK_ENOMEM;
-  HIPCHK(hipMemcpy(dx, x, 4 * L.n * 8, hipMemcpyHostToDevice));
+  H2D(c, dx, x, 4 * L.n * 8);
   rowop<double, false>(c, l, dx, nullptr, dy, 1);
   HIPCHK(hipStreamSynchronize(c->st));
-  HIPCHK(hipMemcpy(y, dy, 4 * L.n * 8, hipMemcpyDeviceToHost));
+  D2H(c, y, dy, 4 * L.n * 8);
   dfree(dx); dfree(dy);
   return STK_OK;
 }
@@ -1574,18 +1631,61 @@ int stk_coarse_solve(stk_ctx *c) {
   return STK_OK;
 }
 
+// ---- many independent instances ---------------------------------------------
+// Each context owns its stream and buffers, so instances solve concurrently:
+// `threads` host workers take contexts in order and run stk_solve on each;
+// the device overlaps the work of the different streams.
+int stk_solve_batch(int n, stk_ctx *const *ctxs, const stk_solver_opts *opts, int nopts, stk_stats *stats,
+                    int threads, int mode) {
+  if (mode < 0 || mode > 1) return STK_EINVAL;
+  if (n < 0 || (n > 0 && (!ctxs || !opts || (nopts != 1 && nopts != n)))) return STK_EINVAL;
+  for (int i = 0; i < n; i++)
+    if (!ctxs[i]) return STK_EINVAL;
+  if (n == 0) return STK_OK;
+  if (threads <= 0) threads = 16;
+  threads = std::min(threads, n);
+  std::vector<int> rcs(n, STK_OK);
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    for (int i; (i = next.fetch_add(1)) < n;) {
+      int rc = STK_OK;
+      if (mode == 1) {
+        rc = stk_prepare(ctxs[i]);
+        if (!rc) rc = stk_hydrostatic_guess(ctxs[i]);
+      }
+      rcs[i] = rc ? rc : stk_solve(ctxs[i], &opts[nopts == 1 ? 0 : i], stats ? &stats[i] : nullptr);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < threads; t++) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+  int rc = STK_OK;
+  for (int i = 0; i < n; i++)
+    if (rcs[i] != STK_OK && (rc == STK_OK || rc == STK_ENOCONV)) rc = rcs[i];
+  return rc;
+}
+
+int stk_mem_info(size_t *free_bytes, size_t *total_bytes) {
+  if (!free_bytes || !total_bytes) return STK_EINVAL;
+  HIPCHK(hipMemGetInfo(free_bytes, total_bytes));
+  return STK_OK;
+}
+
 // ---- inspection of the Schur approximation (tests) ----------------------
 int stk_kop_apply(stk_ctx *c, int l, const double *x, double *y) {
   if (!c || !c->prepared || l < 0 || l > c->L || !x || !y) return STK_EINVAL;
   Level &V = c->lv[l];
   double *dx, *dy, *db;
   if (dalloc(&dx, V.n) || dalloc(&dy, V.n) || dalloc(&db, V.n)) return STK_ENOMEM;
-  HIPCHK(hipMemcpy(dx, x, V.n * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemset(db, 0, V.n * 8));
+  H2D(c, dx, x, V.n * 8);
+  HIPCHK(hipMemsetAsync(db, 0, V.n * 8, c->st));
   hipLaunchKernelGGL(k_kres<double>, dim3(blocks(V.n, 256)), dim3(256), 0, c->st, V.X, V.Y, V.Z, kplanes<double>(V),
                      db, dx, dy);
   HIPCHK(hipStreamSynchronize(c->st));
-  HIPCHK(hipMemcpy(y, dy, V.n * 8, hipMemcpyDeviceToHost));
+  D2H(c, y, dy, V.n * 8);
   for (long i = 0; i < V.n; i++) y[i] = -y[i];
   dfree(dx); dfree(dy); dfree(db);
   return STK_OK;
@@ -1604,11 +1704,11 @@ int stk_schur_apply(stk_ctx *c, const stk_solver_opts *o, const double *r, doubl
   const long n4 = 4 * c->lv[0].n;
   double *dr, *dz;
   if (dalloc(&dr, n4) || dalloc(&dz, n4)) return STK_ENOMEM;
-  HIPCHK(hipMemcpy(dr, r, n4 * 8, hipMemcpyHostToDevice));
+  H2D(c, dr, r, n4 * 8);
   if (o->precision) schur_bfbt<double>(c, o, dr, dz, o->schur == 1 ? -1.0 : 1.0);
   else schur_bfbt<float>(c, o, dr, dz, o->schur == 1 ? -1.0 : 1.0);
   HIPCHK(hipStreamSynchronize(c->st));
-  HIPCHK(hipMemcpy(z, dz, n4 * 8, hipMemcpyDeviceToHost));
+  D2H(c, z, dz, n4 * 8);
   dfree(dr); dfree(dz);
   return STK_OK;
 }
@@ -1699,10 +1799,10 @@ int pardisomat(int am, long mcmax, long mcmax2, long pos0cur1, int tid, long *un
       return STK_ENOMEM;
     std::vector<double> I((size_t)n * n, 0.0);
     for (int i = 0; i < n; i++) I[(size_t)i * n + i] = 1.0;
-    HIPCHK(hipMemcpy(A, c->ca.data(), (size_t)n * n * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(Bm, I.data(), (size_t)n * n * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(db, c->cb.data(), (size_t)n * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(sing, 0, 4));
+    H2D(c, A, c->ca.data(), (size_t)n * n * 8);
+    H2D(c, Bm, I.data(), (size_t)n * n * 8);
+    H2D(c, db, c->cb.data(), (size_t)n * 8);
+    HIPCHK(hipMemsetAsync(sing, 0, 4, c->st));
     for (int k = 0; k < n; k++) {
       hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, c->st, A, Bm, n, k, fac, sing);
       hipLaunchKernelGGL(k_gj_elim, dim3(blocks((long)n * n, 256)), dim3(256), 0, c->st, A, Bm, n, k, fac);
@@ -1710,9 +1810,12 @@ int pardisomat(int am, long mcmax, long mcmax2, long pos0cur1, int tid, long *un
     hipLaunchKernelGGL((k_gemv<double, double>), dim3(blocks(n, 4)), dim3(256), 0, c->st, Bm, db, dx, n);
     int hs = 0;
     HIPCHK(hipStreamSynchronize(c->st));
-    HIPCHK(hipMemcpy(&hs, sing, 4, hipMemcpyDeviceToHost));
+    D2H(c, &hs, sing, 4);
     int rc = ensure_ref(c, c->L);
-    if (!rc && !hs) HIPCHK(hipMemcpy(C.ref[0], dx, (size_t)n * 8, hipMemcpyDeviceToDevice));
+    if (!rc && !hs) {
+      HIPCHK(hipMemcpyAsync(C.ref[0], dx, (size_t)n * 8, hipMemcpyDeviceToDevice, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+    }
     dfree(A); dfree(Bm); dfree(fac); dfree(sing); dfree(db); dfree(dx);
     return hs ? STK_ESINGULAR : rc;
   }

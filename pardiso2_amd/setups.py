@@ -245,7 +245,7 @@ def _box(n: int, L: int, mnu, mro, mx, my, mz, name, knobs=None) -> Problem:
     k = dict(DEFAULT_KNOBS)
     if knobs:
         k.update(knobs)
-    return Problem(n=n, levels=L, gx=g, gy=g.copy(), gz=g.copy(), bc=free_slip_lines(n, n, n),
+    return Problem(n=n, levels=L, gx=g, gy=g.copy(), gz=g.copy(), bc=free_slip_lines(n, n, n, gauge=None),
                    mx=mx, my=my, mz=mz, mnu=mnu.astype(np.float32), mro=mro.astype(np.float32),
                    knobs=k, name=name)
 
@@ -304,3 +304,39 @@ def coarsest_levels(n: int, target: int = 8) -> int:
         m = (m - 5) // 2 + 5
         L += 1
     return L
+
+
+def inclusions(n: int, L: int = None, seed: int = 0, count: int = None, rmin=0.05, rmax=0.15, eta_m=1e21,
+               decades=(0.0, 6.0), rho_m=3300.0, rho_s=3200.0, knobs=None) -> Problem:
+    """1-4 seeded spherical inclusions, each with viscosity eta_m * 10^(+-u),
+    u log-uniform contrast in ``decades`` (stronger or weaker at random)."""
+    L = coarsest_levels(n) if L is None else L
+    g = box_grid(n)
+    mx, my, mz = markers_regular(g, g, g, jitter=0.25, seed=seed)
+    rng = np.random.default_rng(1000 + seed)
+    count = int(rng.integers(1, 5)) if count is None else count
+    mnu = np.full(mx.shape, eta_m)
+    mro = np.full(mx.shape, rho_m)
+    o = box_origin(n)
+    for _ in range(count):
+        c = rng.uniform(0.2, 0.8, 3) + o
+        r = rng.uniform(rmin, rmax)
+        eta = eta_m * 10.0 ** (rng.choice([-1.0, 1.0]) * rng.uniform(*decades))
+        ins = (mx - c[0]) ** 2 + (my - c[1]) ** 2 + (mz - c[2]) ** 2 < r * r
+        mnu[ins] = eta
+        mro[ins] = rho_s
+    return _box(n, L, mnu, mro, mx, my, mz, f"inclusions{n}", knobs)
+
+
+def served_request(n: int, seed: int, decades: float, knobs=None) -> Problem:
+    """Random-field viscosity spanning ``decades`` orders of magnitude around
+    1e21 (seeded), two-phase density 3300 / 3200 split by the field's sign."""
+    L = coarsest_levels(n)
+    g = box_grid(n)
+    mx, my, mz = markers_regular(g, g, g, jitter=0.25, seed=seed)
+    f = gaussian_random_field(33, 0.1, seed)
+    o = box_origin(n)
+    s = sample_trilinear(f, mx - o, my - o, mz - o)
+    le = np.clip(21.0 + 0.25 * decades * s, 21.0 - 0.5 * decades, 21.0 + 0.5 * decades)
+    mro = np.where(s > 0, 3200.0, 3300.0)
+    return _box(n, L, 10.0 ** le, mro, mx, my, mz, f"served{n}", knobs)
