@@ -240,7 +240,8 @@ def main():
     L = pb.levels
     pre, post = sweeps_for(L)
     ctx = S.from_problem(pb)
-    log(f"{args.workload} {args.n}^3 ready on the device ({ctx.n0} nodes, {L + 1} levels)")
+    tiles = ctx.autotune(args.precision)
+    log(f"{args.workload} {args.n}^3 ready on the device ({ctx.n0} nodes, {L + 1} levels), smoother tiles {tiles}")
     if args.kernel:
         ms = ctx.time_kernel(KERNELS[args.kernel], 0, reps=args.reps)
         log(f"kernel {args.kernel}: {args.reps} launches, {ms:.4f} ms each")
@@ -324,6 +325,7 @@ def main():
         "relres": max(rels),
         "solve_ms_device": float(np.mean(solve_s) * 1e3),
         "fp64_iterations": fp64_its,
+        "smoother_tiles": tiles,
         "roofline": {"bound": "hbm", "kernel": "k_smooth<float> finest level", "achieved": gbs_sm,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs_sm / PEAK_HBM_GBS, "traffic": traffic,
                      "ms_per_launch": ms_sm, "bytes_per_launch": SMOOTHER_BYTES_PER_NODE * n0},

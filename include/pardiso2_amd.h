@@ -56,7 +56,7 @@ typedef struct {
   int restart;       /* Krylov directions kept before a restart (1..128) */
   int pre[12];       /* pre-smoothing sweeps per level */
   int post[12];      /* post-smoothing sweeps per level */
-  int tile[3];       /* smoother tile (y, x, z) in nodes; 0 = default */
+  int tile[3];       /* smoother tile (y, x, z) in nodes; 0 0 0 = per level: autotuned, else 32 4 4 */
   double omega;      /* pressure block scale of the preconditioner (schur = 0) */
   int cycle;         /* multigrid cycle index: 1 V-cycle, 2 W-cycle */
   int coarse_viscmod; /* averaging of coarse-level viscosities: -1 = knobs.viscmod, 0/1/2 */
@@ -148,9 +148,19 @@ int stk_apply(stk_ctx *ctx, int level, const double *x, double *y);
 /* Smoother sweeps in tile-coloured order on the context's sol0/fre1 of one
  * level, FP64, all rows (pressure Uzawa step included), then fre0.
  * ty = tx = tz = 0 selects one tile covering the whole level (the LDS-resident
- * kernel the V-cycle uses on levels of at most 2048 nodes in FP64). */
+ * kernel the V-cycle uses on levels of at most 2048 nodes in FP64); tz < 0
+ * selects the tile (ty, tx, -tz) with its viscosities staged in LDS. */
 int stk_smooth(stk_ctx *ctx, int level, int sweeps, int ty, int tx, int tz);
 int stk_coarse_solve(stk_ctx *ctx);
+
+/* ---- smoother tile autotuning ------------------------------------------ */
+/* Time the tile-coloured smoother of every level that is not LDS-resident on
+ * a list of tile shapes, each with and without the stencil viscosities staged
+ * in LDS (HIP events, reps launches each), and keep the fastest per level for
+ * solves whose opts->tile is 0 0 0.  precision 0 FP32, 1 FP64. */
+int stk_autotune(stk_ctx *ctx, int precision, int reps);
+/* tile (y, x, z, viscosities staged) a solve uses on a level */
+int stk_level_tile(const stk_ctx *ctx, int level, int precision, int out4[4]);
 
 /* ---- many independent instances --------------------------------------- */
 /* Solve n contexts concurrently: `threads` host workers (<= 0: 16) take the

@@ -24,7 +24,7 @@ EXPORTS = [
     "stk_pressure_gauge", "stk_hydrostatic_guess", "stk_time_kernel", "stk_vec_set", "stk_vec_get", "stk_apply", "stk_smooth", "stk_coarse_solve",
     "stk_bind", "matclear", "gausmat", "gausmat2", "pardisomat", "gausmat4", "vprestrict", "vpprolong",
     "ronurestrict", "vpiterate", "stk_kop_apply", "stk_kcycle_apply", "stk_schur_apply",
-    "stk_solve_batch", "stk_mem_info",
+    "stk_solve_batch", "stk_mem_info", "stk_autotune", "stk_level_tile",
 ]
 
 
@@ -95,6 +95,8 @@ def lib():
         L.stk_solve_batch.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(Opts), C.c_int, C.POINTER(Stats), C.c_int,
                                       C.c_int]
         L.stk_mem_info.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+        L.stk_autotune.argtypes = [vp, C.c_int, C.c_int]
+        L.stk_level_tile.argtypes = [vp, C.c_int, C.c_int, ip]
         L.stk_kcycle_apply.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, dp]
         L.stk_schur_apply.argtypes = [vp, C.POINTER(Opts), dp, dp]
         L.matclear.argtypes = [C.c_int, dp]
@@ -268,6 +270,15 @@ class Stokes:
 
     def bind(self):
         check(lib().stk_bind(self.h), "stk_bind")
+
+    def autotune(self, precision=0, reps=5):
+        check(lib().stk_autotune(self.h, precision, reps), "stk_autotune")
+        out = []
+        for l in range(self.levels + 1):
+            t = np.zeros(4, np.int32)
+            check(lib().stk_level_tile(self.h, l, precision, _ip(t)), "stk_level_tile")
+            out.append(tuple(int(v) for v in t))
+        return out
 
     def kop_apply(self, level, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

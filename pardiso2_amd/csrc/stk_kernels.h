@@ -177,7 +177,7 @@ __global__ void k_bc_rowop(int nbc, const int *__restrict__ br, const int *__res
 // ---------------------------------------------------------------------------
 // one phase of the tile smoother for component C (colour for velocities)
 template <typename T, int C>
-__device__ __forceinline__ void smooth_phase(const Geo<T> &g, const unsigned char *__restrict__ flags,
+__device__ __forceinline__ void smooth_phase(const Geo<T> &g, const Geo<T> &gv, const unsigned char *__restrict__ flags,
                                              const T *__restrict__ b, T *sx, int ty, int tx, int tz, int x0,
                                              int y0, int z0, int colour, T v0koef, T p0koef) {
   const int hy = ty + 2, hx = tx + 2;
@@ -198,7 +198,7 @@ __device__ __forceinline__ void smooth_phase(const Geo<T> &g, const unsigned cha
     long n = g.nid(m1, m2, m3);
     if (kind_of(flags[n], C) != KIND_EQ) continue;
     T s[18];
-    row_coef_c<T, C>(g, m1, m2, m3, s);
+    row_coef_c<T, C>(gv, m1, m2, m3, s);
     T ival = -b[4 * n + C];
     const int base = ((lz + 1) * hx + (lx + 1)) * hy + (ly + 1);
 #pragma unroll
@@ -211,7 +211,11 @@ __device__ __forceinline__ void smooth_phase(const Geo<T> &g, const unsigned cha
   }
 }
 
-template <typename T>
+// SV: the tile's stencil viscosities (nodes [x0, x0+tx] x [y0, y0+ty] x
+// [z0, z0+tz], every offset a row uses is 0 or +1) are staged in LDS after the
+// unknowns, and the rows read them through a Geo whose node index maps into
+// that block, so each viscosity comes from HBM once per tile.
+template <typename T, bool SV>
 __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *__restrict__ flags,
                                                 const T *__restrict__ xold, T *__restrict__ xnew,
                                                 const T *__restrict__ b, int ty, int tx, int tz,
@@ -223,6 +227,28 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
   const int by = bt % nty, bx = (bt / nty) % ntx, bz = bt / (nty * ntx);
   const int y0 = by * ty, x0 = bx * tx, z0 = bz * tz;
   const int nh = hy * hx * hz;
+  Geo<T> gv = g;
+  if (SV) {
+    const int vy = ty + 1, vxy = (ty + 1) * (tx + 1), nv = vxy * (tz + 1);
+    T *sv = sx + 4 * nh;
+    const int np = (g.p2 != T(0)) ? 5 : 4;
+    const T *src[5] = {g.ec, g.exy, g.exz, g.eyz, g.ed};
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+      const int ly = i % vy, lx = (i / vy) % (tx + 1), lz = i / vxy;
+      const int m2 = y0 + ly, m1 = x0 + lx, m3 = z0 + lz;
+      const bool in = m1 < g.X && m2 < g.Y && m3 < g.Z;
+      const long n = g.nid(in ? m1 : 0, in ? m2 : 0, in ? m3 : 0);
+      for (int q = 0; q < np; q++) sv[q * nv + i] = in ? src[q][n] : T(0);
+    }
+    const long off = (long)z0 * vxy + (long)x0 * vy + y0;
+    gv.XY = vxy;
+    gv.Y = vy;
+    gv.ec = sv - off;
+    gv.exy = sv + nv - off;
+    gv.exz = sv + 2 * nv - off;
+    gv.eyz = sv + 3 * nv - off;
+    gv.ed = sv + 4 * nv - off;
+  }
   // stage tile + halo
   for (int i = threadIdx.x; i < nh; i += blockDim.x) {
     int ly = i % hy, lx = (i / hy) % hx, lz = i / (hy * hx);
@@ -238,10 +264,10 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
   const int nphase = withp ? 7 : 6;
   for (int ph = 0; ph < nphase; ph++) {
     const int colour = ph & 1;
-    if (ph < 2) smooth_phase<T, 1>(g, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
-    else if (ph < 4) smooth_phase<T, 2>(g, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
-    else if (ph < 6) smooth_phase<T, 3>(g, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
-    else smooth_phase<T, 0>(g, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
+    if (ph < 2) smooth_phase<T, 1>(g, gv, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
+    else if (ph < 4) smooth_phase<T, 2>(g, gv, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
+    else if (ph < 6) smooth_phase<T, 3>(g, gv, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
+    else smooth_phase<T, 0>(g, gv, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
     __syncthreads();
   }
   // write the tile interior
@@ -590,6 +616,52 @@ __global__ void k_gj_elim(double *A, double *B, int n, int k, const double *__re
   if (j == k) A[(long)i * n + k] = 0.0;
 }
 
+// Compact dense rows of the coarsest velocity block: row i is unknown act[i]
+// (velocity unknowns with an equation or a boundary row; pressure and absent
+// unknowns are left out), columns mapped through idx (-1: left out)
+__global__ void k_dense_rows_c(Geo<double> g, const unsigned char *__restrict__ flags, int nbc,
+                               const int *__restrict__ br, const int *__restrict__ bj,
+                               const double *__restrict__ negk, const int *__restrict__ act,
+                               const int *__restrict__ idx, double *A, int N) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int u = act[i];
+  const long n = u >> 2;
+  const int c = u & 3;
+  double *row = A + (long)i * N;
+  if (kind_of(flags[n], c) == KIND_BC) {
+    for (int q = 0; q < nbc; q++)
+      if (br[q] == u) {
+        row[i] += 1.0;
+        if (bj[q] >= 0 && idx[bj[q]] >= 0) row[idx[bj[q]]] += negk[q];
+        break;
+      }
+    return;
+  }
+  int m1, m2, m3;
+  decode(n, g.X, g.Y, g.XY, m1, m2, m3);
+  double s[18];
+  const int ns = row_coef<double>(g, c, m1, m2, m3, s);
+  for (int q = 1; q <= ns; q++) {
+    const long uq = 4 * (n + (long)kSlot[c][q][2] * g.XY + (long)kSlot[c][q][0] * g.Y + kSlot[c][q][1]) + kSlot[c][q][3];
+    const int j = idx[uq];
+    if (j >= 0) row[j] += s[q];
+  }
+}
+
+// x[act[i]] = sum_j M[i][j] b[act[j]], one wavefront per row
+template <typename TM, typename T>
+__global__ void k_gemv_c(const TM *__restrict__ M, const int *__restrict__ act, const T *__restrict__ b, T *x, int N) {
+  const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  if (row >= N) return;
+  const TM *mr = M + (long)row * N;
+  double acc = 0.0;
+  for (int j = lane; j < N; j += 64) acc += (double)mr[j] * (double)b[act[j]];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) x[act[row]] = (T)acc;
+}
+
 // y = M x for a dense row-major n x n matrix, one wavefront per row
 template <typename TM, typename T>
 __global__ void k_gemv(const TM *__restrict__ M, const T *__restrict__ x, T *y, int n) {
@@ -756,7 +828,7 @@ __global__ void __launch_bounds__(256) k_dots2(long n4, const double *__restrict
   const int nd = 2 * k + 2;
   for (int i = threadIdx.x; i < (2 * kMaxDirs + 2) * 4; i += blockDim.x) acc[i >> 2][i & 3] = 0.0;
   __syncthreads();
-  constexpr int E = 4;
+  constexpr int E = 8;
   for (long base = (long)blockIdx.x * 256 * E; base < n4; base += (long)gridDim.x * 256 * E) {
     double wu[E], ww[E];
     double suw = 0.0, suu = 0.0;

@@ -8,9 +8,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <array>
 #include <atomic>
 #include <memory>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pardiso2_amd.h"
@@ -96,6 +98,8 @@ struct Level {
   float *x32[2] = {}, *b32 = nullptr, *r32 = nullptr;
   double *x64[2] = {}, *b64 = nullptr, *r64 = nullptr;
   int cur32 = 0, cur64 = 0;
+  // smoother tile (y, x, z, stage viscosities in LDS) picked by stk_autotune (0: default)
+  int tune32[4] = {0, 0, 0, 0}, tune64[4] = {0, 0, 0, 0};
   // reference arrays (compat): sol0, fre1, fre0
   double *ref[3] = {};
   // transfer from level-1 (kinds N, C, P, R per axis)
@@ -149,8 +153,10 @@ struct stk_ctx {
   bool fields_set = false, prepared = false, rows_dirty = true;
   // coarsest inverse
   int nc = 0;
-  double *ainv64 = nullptr, *ainvP64 = nullptr;  // velocity block / all rows
+  double *ainv64 = nullptr, *ainvP64 = nullptr;  // velocity block (compact, nca x nca) / all rows
   float *ainv32 = nullptr;
+  int *cact = nullptr;  // coarsest velocity unknowns of the compact block
+  int nca = 0;
   // outer iteration vectors (finest level)
   double *X = nullptr, *B = nullptr, *R = nullptr, *W2 = nullptr, *T1 = nullptr;
   std::vector<double *> Zs, Qs;
@@ -359,9 +365,26 @@ void bc_passes(stk_ctx *c, Level &L, const T *b, T *x, int mode) {
 // whole-level LDS smoother for levels whose four unknowns fit in 64 KB
 const size_t kLevelLds = 65536;
 
+// LDS of one smoother tile: unknowns with a one-node halo, plus (sv) five
+// viscosity planes of (ty+1)(tx+1)(tz+1) nodes
+template <typename T>
+size_t smooth_lds(const int *tile, bool sv) {
+  const size_t nh = (size_t)(tile[0] + 2) * (tile[1] + 2) * (tile[2] + 2);
+  const size_t nv = (size_t)(tile[0] + 1) * (tile[1] + 1) * (tile[2] + 1);
+  return (4 * nh + (sv ? 5 * nv : 0)) * sizeof(T);
+}
+
+const int kTileDefault4[4] = {32, 4, 4, 0};
+template <typename T> const int *tuned_tile(const Level &L);
+template <> const int *tuned_tile<float>(const Level &L) { return L.tune32[0] ? L.tune32 : kTileDefault4; }
+template <> const int *tuned_tile<double>(const Level &L) { return L.tune64[0] ? L.tune64 : kTileDefault4; }
+
+// tile == nullptr: the level's autotuned tile (or the default one)
 template <typename T>
 void smooth(stk_ctx *c, int l, int sweeps, const int *tile, int withp, bool whole_level = false) {
   Level &L = c->lv[l];
+  const bool tuned = !tile;
+  if (!tile) tile = tuned_tile<T>(L);
   if (whole_level && sweeps > 0 && 4 * (size_t)L.n * sizeof(T) <= kLevelLds) {
     Geo<T> g = L.geo<T>(c->kn.p2koef);
     hipLaunchKernelGGL(k_smooth_level<T>, dim3(1), dim3(1024), 4 * (size_t)L.n * sizeof(T), c->st, g, L.flags,
@@ -370,11 +393,16 @@ void smooth(stk_ctx *c, int l, int sweeps, const int *tile, int withp, bool whol
   }
   int ty = tile[0], tx = tile[1], tz = tile[2];
   int nty = (L.Y + ty - 1) / ty, ntx = (L.X + tx - 1) / tx, ntz = (L.Z + tz - 1) / tz;
-  size_t lds = (size_t)(ty + 2) * (tx + 2) * (tz + 2) * 4 * sizeof(T);
+  const size_t lds = smooth_lds<T>(tile, false), ldsv = smooth_lds<T>(tile, true);
+  const bool sv = tuned && tile[3] && ldsv <= kLevelLds;
   Geo<T> g = L.geo<T>(c->kn.p2koef);
   for (int s = 0; s < sweeps; s++) {
-    hipLaunchKernelGGL(k_smooth<T>, dim3(nty * ntx * ntz), dim3(256), lds, c->st, g, L.flags, lvx<T>(L),
-                       lvxo<T>(L), lvb<T>(L), ty, tx, tz, ntx, nty, withp, (T)c->kn.v0koef, (T)c->kn.p0koef);
+    if (sv)
+      hipLaunchKernelGGL((k_smooth<T, true>), dim3(nty * ntx * ntz), dim3(256), ldsv, c->st, g, L.flags, lvx<T>(L),
+                         lvxo<T>(L), lvb<T>(L), ty, tx, tz, ntx, nty, withp, (T)c->kn.v0koef, (T)c->kn.p0koef);
+    else
+      hipLaunchKernelGGL((k_smooth<T, false>), dim3(nty * ntx * ntz), dim3(256), lds, c->st, g, L.flags, lvx<T>(L),
+                         lvxo<T>(L), lvb<T>(L), ty, tx, tz, ntx, nty, withp, (T)c->kn.v0koef, (T)c->kn.p0koef);
     lvflip<T>(L);
     bc_passes<T>(c, L, lvb<T>(L), lvx<T>(L), withp ? 0 : 1);
   }
@@ -426,7 +454,9 @@ void mg_cycle(stk_ctx *c, const stk_solver_opts *o, const int *tile, int l, int 
   if (l == L) {
     Level &C = c->lv[L];
     int n4 = (int)(4 * C.n);
-    hipLaunchKernelGGL((k_gemv<T, T>), dim3(blocks(n4, 4)), dim3(256), 0, c->st, ainv<T>(c), lvb<T>(C), lvx<T>(C), n4);
+    (void)n4;  // unknowns outside the compact block keep the 0 the restriction wrote
+    hipLaunchKernelGGL((k_gemv_c<T, T>), dim3(blocks(c->nca, 4)), dim3(256), 0, c->st, ainv<T>(c), c->cact, lvb<T>(C),
+                       lvx<T>(C), c->nca);
     return;
   }
   Level &V = c->lv[l];
@@ -619,6 +649,54 @@ int ensure_outer(stk_ctx *c, int m) {
 // ---------------------------------------------------------------------------
 // coarsest-level inverse (Gauss-Jordan with partial pivoting, FP64)
 // ---------------------------------------------------------------------------
+// velocity block of the coarsest level: Gauss-Jordan on the compact system
+// of velocity unknowns with an equation or boundary row (N x N, N < 4 mgn)
+int coarse_inverse_v(stk_ctx *c) {
+  Level &C = c->lv[c->L];
+  const int n4 = (int)(4 * C.n);
+  std::vector<int> act, idx(n4, -1);
+  for (int u = 0; u < n4; u++) {
+    const int cc = u & 3;
+    if (cc != 0 && kind_of(C.hflags[u >> 2], cc) != KIND_NONE) {
+      idx[u] = (int)act.size();
+      act.push_back(u);
+    }
+  }
+  const int N = (int)act.size();
+  if (N == 0) return STK_EINVAL;
+  int *dact = nullptr, *didx = nullptr;
+  if (upload(&dact, act) || upload(&didx, idx)) return STK_ENOMEM;
+  double *A, *Bm, *fac;
+  int *sing;
+  if (dalloc(&A, (size_t)N * N) || dalloc(&Bm, (size_t)N * N) || dalloc(&fac, N) || dalloc(&sing, 1))
+    return STK_ENOMEM;
+  HIPCHK(hipMemsetAsync(A, 0, (size_t)N * N * 8, c->st));
+  HIPCHK(hipMemsetAsync(sing, 0, 4, c->st));
+  std::vector<double> I((size_t)N * N, 0.0);
+  for (int i = 0; i < N; i++) I[(size_t)i * N + i] = 1.0;
+  HIPCHK(hipMemcpyAsync(Bm, I.data(), (size_t)N * N * 8, hipMemcpyHostToDevice, c->st));
+  Geo<double> g = C.geo<double>(c->kn.p2koef);
+  hipLaunchKernelGGL(k_dense_rows_c, dim3(blocks(N, 128)), dim3(128), 0, c->st, g, C.flags, C.nbc, C.bc_r, C.bc_j,
+                     C.bc_negk, dact, didx, A, N);
+  for (int k = 0; k < N; k++) {
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, c->st, A, Bm, N, k, fac, sing);
+    hipLaunchKernelGGL(k_gj_elim, dim3(blocks((long)N * N, 256)), dim3(256), 0, c->st, A, Bm, N, k, fac);
+  }
+  int hs = 0;
+  HIPCHK(hipMemcpyAsync(&hs, sing, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  dfree(A); dfree(fac); dfree(sing); dfree(didx);
+  if (hs) { dfree(Bm); dfree(dact); return STK_ESINGULAR; }
+  dfree(c->ainv64); dfree(c->ainv32); dfree(c->cact);
+  c->ainv64 = Bm;
+  c->cact = dact;
+  c->nca = N;
+  if (dalloc(&c->ainv32, (size_t)N * N)) return STK_ENOMEM;
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks((long)N * N, 256)), dim3(256), 0, c->st, (long)N * N,
+                     c->ainv64, c->ainv32);
+  return STK_OK;
+}
+
 int coarse_inverse(stk_ctx *c, int withp, double **out) {
   Level &C = c->lv[c->L];
   int n = (int)(4 * C.n);
@@ -653,6 +731,60 @@ int coarse_inverse(stk_ctx *c, int withp, double **out) {
 // ===========================================================================
 // C ABI
 // ===========================================================================
+// smoother tile autotuning (timed on the library stream)
+template <typename T>
+static double time_smooth(stk_ctx *c, int l, const int *tile, int reps) {
+  Level &L = c->lv[l];
+  const int ty = tile[0], tx = tile[1], tz = tile[2];
+  const int nty = (L.Y + ty - 1) / ty, ntx = (L.X + tx - 1) / tx, ntz = (L.Z + tz - 1) / tz;
+  const bool sv = tile[3] && smooth_lds<T>(tile, true) <= kLevelLds;
+  const size_t lds = smooth_lds<T>(tile, sv);
+  Geo<T> g = L.geo<T>(c->kn.p2koef);
+  T *x0 = lvx<T>(L), *x1 = lvxo<T>(L);
+  auto launch = [&]() {
+    if (sv)
+      hipLaunchKernelGGL((k_smooth<T, true>), dim3(nty * ntx * ntz), dim3(256), lds, c->st, g, L.flags, x0, x1,
+                         lvb<T>(L), ty, tx, tz, ntx, nty, 0, (T)c->kn.v0koef, (T)c->kn.p0koef);
+    else
+      hipLaunchKernelGGL((k_smooth<T, false>), dim3(nty * ntx * ntz), dim3(256), lds, c->st, g, L.flags, x0, x1,
+                         lvb<T>(L), ty, tx, tz, ntx, nty, 0, (T)c->kn.v0koef, (T)c->kn.p0koef);
+  };
+  launch();
+  hipEventRecord(c->ev0, c->st);
+  for (int i = 0; i < reps; i++) launch();
+  hipEventRecord(c->ev1, c->st);
+  hipEventSynchronize(c->ev1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  return ms / reps;
+}
+
+template <typename T>
+static int autotune_levels(stk_ctx *c, int reps) {
+  static const int shape[][3] = {{32, 4, 4}, {32, 4, 8}, {32, 8, 4}, {16, 8, 8}, {64, 4, 2}, {64, 2, 4},
+                                 {16, 4, 4}, {32, 2, 4}, {16, 4, 8}, {8, 8, 8}, {64, 2, 2}, {32, 2, 8}};
+  const int ns = (int)(sizeof(shape) / sizeof(shape[0]));
+  std::vector<std::array<int, 4>> cand;
+  for (int k = 0; k < ns; k++)
+    for (int sv = 0; sv < 2; sv++) {
+      const std::array<int, 4> t = {shape[k][0], shape[k][1], shape[k][2], sv};
+      if (smooth_lds<T>(t.data(), sv) <= kLevelLds) cand.push_back(t);
+    }
+  for (int l = 0; l <= c->L; l++) {
+    Level &L = c->lv[l];
+    if (4 * (size_t)L.n * sizeof(T) <= kLevelLds) continue;  // smoothed by the whole-level kernel
+    double best = 1e30;
+    int bi = 0;
+    for (int k = 0; k < (int)cand.size(); k++) {
+      const double ms = time_smooth<T>(c, l, cand[k].data(), reps);
+      if (ms < best) { best = ms; bi = k; }
+    }
+    int *t = std::is_same<T, float>::value ? L.tune32 : L.tune64;
+    for (int a = 0; a < 4; a++) t[a] = cand[bi][a];
+  }
+  return hipGetLastError() == hipSuccess ? STK_OK : STK_EHIP;
+}
+
 // one K-cycle on host cell vectors (inspection entry point)
 template <typename T>
 static void kcycle_host(stk_ctx *c, int nu, double scale, const double *b, double *x) {
@@ -775,7 +907,7 @@ void stk_destroy(stk_ctx *c) {
   dfree(c->dinv64); dfree(c->bfy64); dfree(c->bfw64); dfree(c->dinv32); dfree(c->bfy32); dfree(c->bfw32);
   dfree(c->kinv64); dfree(c->kinv32); dfree(c->kcell);
   dfree(c->nu); dfree(c->ro); dfree(c->nd); dfree(c->nxx); dfree(c->nxy); dfree(c->nxz); dfree(c->nyz);
-  dfree(c->ainv64); dfree(c->ainvP64); dfree(c->ainv32);
+  dfree(c->ainv64); dfree(c->ainvP64); dfree(c->ainv32); dfree(c->cact);
   dfree(c->X); dfree(c->B); dfree(c->R); dfree(c->W2); dfree(c->T1); dfree(c->red); dfree(c->scal);
   dfree(c->kred); dfree(c->hcol);
   for (auto p : c->Zs) dfree(p);
@@ -807,7 +939,7 @@ void stk_default_opts(stk_solver_opts *o, int levels) {
   int pre[12] = {2, 4, 8, 16, 16, 16, 16, 16, 16, 16, 16, 16};
   for (int l = 0; l < 12; l++) { o->pre[l] = pre[l]; o->post[l] = pre[l]; }
   (void)levels;
-  o->tile[0] = kTileDefault[0]; o->tile[1] = kTileDefault[1]; o->tile[2] = kTileDefault[2];
+  o->tile[0] = o->tile[1] = o->tile[2] = 0;  // per-level autotuned tile, else 32x4x4
   o->omega = 4.0 / 3.0;  // isoviscous Schur constant of the 2eta(e - div/3) stress form
   o->cycle = 1;
   o->coarse_viscmod = -1;
@@ -1219,14 +1351,8 @@ int stk_prepare_cvm(stk_ctx *c, int cvm) {
   if (F.nbc) hipLaunchKernelGGL(k_bc_rhs, dim3(blocks(F.nbc, 256)), dim3(256), 0, c->st, F.nbc, F.bc_r, F.bc_c, c->B);
   hipLaunchKernelGGL(k_weights, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, F.flags, F.e64[0],
                      F.g64[0], F.g64[1], F.g64[2], c->W2);
-  rc = coarse_inverse(c, 0, &c->ainv64);
+  rc = coarse_inverse_v(c);
   if (rc) return rc;
-  Level &C = c->lv[c->L];
-  long nn = 16 * C.n * C.n;
-  dfree(c->ainv32);
-  if (dalloc(&c->ainv32, nn)) return STK_ENOMEM;
-  hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks(nn, 256)), dim3(256), 0, c->st, nn, c->ainv64,
-                     c->ainv32);
   dfree(c->ainvP64);
   rc = build_schur(c);
   if (rc) return rc;
@@ -1296,20 +1422,32 @@ int stk_time_kernel(stk_ctx *c, int which, int level, int reps, double *ms_avg) 
   int rc = ensure_vectors(c, true, true);
   if (rc) return rc;
   Level &L = c->lv[level];
-  int tile[3] = {kTileDefault[0], kTileDefault[1], kTileDefault[2]};
+  const int *tt = (which == 1) ? tuned_tile<double>(L) : tuned_tile<float>(L);
+  int tile[3] = {tt[0], tt[1], tt[2]};
   int ty = tile[0], tx = tile[1], tz = tile[2];
   int nty = (L.Y + ty - 1) / ty, ntx = (L.X + tx - 1) / tx, ntz = (L.Z + tz - 1) / tz;
   auto launch = [&]() {
     if (which == 0) {
       Geo<float> g = L.geo<float>(c->kn.p2koef);
-      hipLaunchKernelGGL(k_smooth<float>, dim3(nty * ntx * ntz), dim3(256), (size_t)(ty + 2) * (tx + 2) * (tz + 2) * 16,
-                         c->st, g, L.flags, L.x32[0], L.x32[1], L.b32, ty, tx, tz, ntx, nty, 0, (float)c->kn.v0koef,
-                         (float)c->kn.p0koef);
+      if (tt[3])
+        hipLaunchKernelGGL((k_smooth<float, true>), dim3(nty * ntx * ntz), dim3(256), smooth_lds<float>(tile, true),
+                           c->st, g, L.flags, L.x32[0], L.x32[1], L.b32, ty, tx, tz, ntx, nty, 0, (float)c->kn.v0koef,
+                           (float)c->kn.p0koef);
+      else
+        hipLaunchKernelGGL((k_smooth<float, false>), dim3(nty * ntx * ntz), dim3(256), smooth_lds<float>(tile, false),
+                           c->st, g, L.flags, L.x32[0], L.x32[1], L.b32, ty, tx, tz, ntx, nty, 0, (float)c->kn.v0koef,
+                           (float)c->kn.p0koef);
     } else if (which == 1) {
       Geo<double> g = L.geo<double>(c->kn.p2koef);
-      hipLaunchKernelGGL(k_smooth<double>, dim3(nty * ntx * ntz), dim3(256), (size_t)(ty + 2) * (tx + 2) * (tz + 2) * 32,
-                         c->st, g, L.flags, L.x64[0], L.x64[1], L.b64, ty, tx, tz, ntx, nty, 0, c->kn.v0koef,
-                         c->kn.p0koef);
+      const bool sv = tt[3] && smooth_lds<double>(tile, true) <= kLevelLds;
+      if (sv)
+        hipLaunchKernelGGL((k_smooth<double, true>), dim3(nty * ntx * ntz), dim3(256), smooth_lds<double>(tile, true),
+                           c->st, g, L.flags, L.x64[0], L.x64[1], L.b64, ty, tx, tz, ntx, nty, 0, c->kn.v0koef,
+                           c->kn.p0koef);
+      else
+        hipLaunchKernelGGL((k_smooth<double, false>), dim3(nty * ntx * ntz), dim3(256), smooth_lds<double>(tile, false),
+                           c->st, g, L.flags, L.x64[0], L.x64[1], L.b64, ty, tx, tz, ntx, nty, 0, c->kn.v0koef,
+                           c->kn.p0koef);
     } else if (which == 2) {
       Geo<double> g = L.geo<double>(c->kn.p2koef);
       hipLaunchKernelGGL((k_rowop<double, false>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, g, L.flags,
@@ -1353,8 +1491,10 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
   const long n4 = 4 * c->lv[0].n;
   const int m = o->restart;
   const bool fp64 = o->precision == 1;
-  const unsigned nbk = (unsigned)std::min<long>(c->nred, std::max<long>(1, (n4 + 1023) / 1024));
-  const unsigned nbu = blocks(n4, 256) < (unsigned)c->nred ? blocks(n4, 256) : (unsigned)c->nred;
+  // dots: one block per 2048-element slice up to nred blocks; updates: up to
+  // 4 nred blocks (the norm partials need one double per block)
+  const unsigned nbk = (unsigned)std::min<long>(c->nred, std::max<long>(1, (n4 + 2047) / 2048));
+  const unsigned nbu = (unsigned)std::min<long>(4L * c->nred, (long)blocks(n4, 256));
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), hh(2 * m + 2), y(m), coef(2 * m + 2);
   auto h = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
   auto givens = [&](int col) {
@@ -1474,9 +1614,10 @@ int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
       if (rc0) return rc0;
     }
   }
-  int tile[3] = {o->tile[0] ? o->tile[0] : kTileDefault[0], o->tile[1] ? o->tile[1] : kTileDefault[1],
-                 o->tile[2] ? o->tile[2] : kTileDefault[2]};
-  if (tile[0] < 2 || tile[1] < 1 || tile[2] < 1 || (size_t)(tile[0] + 2) * (tile[1] + 2) * (tile[2] + 2) * 32 > 65536)
+  const bool explicit_tile = o->tile[0] || o->tile[1] || o->tile[2];
+  const int *tile = explicit_tile ? o->tile : nullptr;
+  if (explicit_tile && (o->tile[0] < 2 || o->tile[1] < 1 || o->tile[2] < 1 ||
+                        (size_t)(o->tile[0] + 2) * (o->tile[1] + 2) * (o->tile[2] + 2) * 32 > 65536))
     return STK_EINVAL;
   const bool fp64 = o->precision == 1;
   int rc = ensure_vectors(c, !fp64, fp64);
@@ -1599,6 +1740,8 @@ int stk_apply(stk_ctx *c, int l, const double *x, double *y) {
 int stk_smooth(stk_ctx *c, int l, int sweeps, int ty, int tx, int tz) {
   if (!c || !c->prepared || l < 0 || l > c->L || sweeps < 0) return STK_EINVAL;
   const bool whole = (ty == 0 && tx == 0 && tz == 0);
+  const bool staged = tz < 0;
+  if (staged) tz = -tz;
   if (whole && 4 * (size_t)c->lv[l].n * sizeof(double) > kLevelLds) return STK_EINVAL;
   if (!whole && (ty < 1 || tx < 1 || tz < 1 || (size_t)(ty + 2) * (tx + 2) * (tz + 2) * 32 > 65536)) return STK_EINVAL;
   int rc = ensure_ref(c, l) || ensure_vectors(c, false, true);
@@ -1607,8 +1750,16 @@ int stk_smooth(stk_ctx *c, int l, int sweeps, int ty, int tx, int tz) {
   size_t bytes = 4 * L.n * 8;
   HIPCHK(hipMemcpyAsync(L.x64[L.cur64], L.ref[0], bytes, hipMemcpyDeviceToDevice, c->st));
   HIPCHK(hipMemcpyAsync(L.b64, L.ref[1], bytes, hipMemcpyDeviceToDevice, c->st));
-  int tile[3] = {ty, tx, tz};
-  smooth<double>(c, l, sweeps, tile, 1, whole);
+  int tile[4] = {ty, tx, tz, 1};
+  if (staged) {
+    if (smooth_lds<double>(tile, true) > kLevelLds) return STK_EINVAL;
+    const int keep[4] = {L.tune64[0], L.tune64[1], L.tune64[2], L.tune64[3]};
+    for (int a = 0; a < 4; a++) L.tune64[a] = tile[a];
+    smooth<double>(c, l, sweeps, nullptr, 1, false);
+    for (int a = 0; a < 4; a++) L.tune64[a] = keep[a];
+  } else {
+    smooth<double>(c, l, sweeps, tile, 1, whole);
+  }
   rowop<double, true>(c, l, L.x64[L.cur64], L.b64, L.ref[2], 1);
   HIPCHK(hipMemcpyAsync(L.ref[0], L.x64[L.cur64], bytes, hipMemcpyDeviceToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -1628,6 +1779,22 @@ int stk_coarse_solve(stk_ctx *c) {
   hipLaunchKernelGGL((k_gemv<double, double>), dim3(blocks(n4, 4)), dim3(256), 0, c->st, c->ainvP64, C.ref[1],
                      C.ref[0], n4);
   HIPCHK(hipStreamSynchronize(c->st));
+  return STK_OK;
+}
+
+// ---- smoother tile autotuning -----------------------------------------------
+int stk_autotune(stk_ctx *c, int precision, int reps) {
+  if (!c || !c->prepared || reps < 1 || precision < 0 || precision > 1) return STK_EINVAL;
+  int rc = ensure_vectors(c, precision == 0, precision == 1);
+  if (rc) return rc;
+  return precision ? autotune_levels<double>(c, reps) : autotune_levels<float>(c, reps);
+}
+
+int stk_level_tile(const stk_ctx *c, int level, int precision, int out4[4]) {
+  if (!c || level < 0 || level > c->L || !out4) return STK_EINVAL;
+  const Level &L = c->lv[level];
+  const int *t = precision ? tuned_tile<double>(L) : tuned_tile<float>(L);
+  for (int a = 0; a < 4; a++) out4[a] = t[a];
   return STK_OK;
 }
 

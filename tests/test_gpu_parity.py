@@ -83,7 +83,7 @@ def test_operator_apply_matches_assembled_rows(pair):
     np.testing.assert_allclose(y, yo, rtol=1e-12, atol=1e-12 * np.abs(yo).max())
 
 
-@pytest.mark.parametrize("tile", [(32, 4, 4), (8, 2, 3)])
+@pytest.mark.parametrize("tile", [(32, 4, 4), (8, 2, 3), (8, 2, -3), (16, 4, -2)])
 def test_coloured_sweep_matches_oracle(oracle_mod, tile):
     pb, o, g = _pair(oracle_mod, seed=3)
     try:
@@ -93,7 +93,7 @@ def test_coloured_sweep_matches_oracle(oracle_mod, tile):
         o.arr("sol0")[:n4] = x
         g.vec_set(S.SOL0, 0, x)
         g.vec_set(S.FRE1, 0, o.arr("fre1")[:n4].copy())
-        o.sweep_tiles(3, 0, *tile)
+        o.sweep_tiles(3, 0, tile[0], tile[1], abs(tile[2]))
         g.smooth(0, 3, tile=tile)
         xo = o.arr("sol0")[:n4]
         xg = g.vec_get(S.SOL0, 0)
