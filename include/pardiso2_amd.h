@@ -68,6 +68,8 @@ typedef struct {
   int ksweeps;       /* red/black Gauss-Seidel sweeps before and after each K-cycle level */
   double kscale;     /* coarse-correction scale of the aggregation K-cycle */
   int krylov;        /* outer Krylov method: 0 GCR, 1 flexible GMRES */
+  int graph;         /* 1: replay the preconditioner from a captured HIP graph (one host thread
+                        per process capturing at a time; stk_solve_batch launches eagerly), 0: eager */
 } stk_solver_opts;
 
 typedef struct {

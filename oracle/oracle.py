@@ -38,6 +38,8 @@ def lib():
         L.orc_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_int]
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_set_vonly.argtypes = [C.c_void_p, C.c_int]
+        L.orc_threads.restype = C.c_int
+        L.orc_threads.argtypes = []
         L.orc_set_knobs.argtypes = [C.c_void_p] + [C.c_double] * 7 + [C.c_int] + [C.c_double] * 3
         L.orc_level_info.argtypes = [C.c_void_p, C.c_int, ip]
         L.orc_grid.restype = dp
@@ -255,3 +257,8 @@ def from_problem(pb, L=None):
     if rc:
         raise RuntimeError(f"coarse LU failed rc={rc}")
     return o
+
+
+def threads() -> int:
+    """OpenMP threads of the oracle's parallel loops."""
+    return int(lib().orc_threads())

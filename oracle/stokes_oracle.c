@@ -15,6 +15,10 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
 
 #define ABSV(a) ((a) >= 0 ? (a) : -(a))
 #define NID(o, l, m1, m2, m3) ((o)->mgp[l] + (long)(m3) * (o)->mgxy[l] + (long)(m1) * (o)->mgy[l] + (m2))
@@ -926,6 +930,8 @@ static inline void row_update(orc_t *o, long m1, double *x) {
 
 void orc_residual(orc_t *o, int l) {
   long mcmin = o->mgp[l] * 4, mcmax = (o->mgp[l] + o->mgn[l]) * 4 - 1;
+  /* rows are independent: the thread count does not change any value */
+#pragma omp parallel for schedule(static)
   for (long m1 = mcmin; m1 <= mcmax; m1++)
     if (o->num0[m1]) {
       double ival = o->fre1[m1];
@@ -1022,12 +1028,17 @@ void orc_sweep_tiles(orc_t *o, int am, int l, int ty, int tx, int tz) {
   static const int comp[4] = {1, 2, 3, 0};
   for (int s = 0; s < am; s++) {
     memcpy(old + a, cur + a, sizeof(double) * (e - a));
-    tilectx_t t;
-    t.mgp = o->mgp[l]; t.Y = Y; t.XY = o->mgxy[l]; t.ty = ty; t.tx = tx; t.tz = tz;
-    for (t.z0 = 0; t.z0 < Z; t.z0 += tz)
-      for (t.x0 = 0; t.x0 < X; t.x0 += tx)
-        for (t.y0 = 0; t.y0 < Y; t.y0 += ty)
+    /* tiles only read their own nodes in cur and the rest in old, so they are
+       independent within a sweep and may run in any order or in parallel */
+    const int ntz = (Z + tz - 1) / tz, ntx = (X + tx - 1) / tx, nty = (Y + ty - 1) / ty;
+#pragma omp parallel for collapse(3) schedule(static)
+    for (int kz = 0; kz < ntz; kz++)
+      for (int kx = 0; kx < ntx; kx++)
+        for (int ky = 0; ky < nty; ky++)
           for (int ph = 0; ph < (o->vonly ? 6 : 7); ph++) {
+            tilectx_t t;
+            t.mgp = o->mgp[l]; t.Y = Y; t.XY = o->mgxy[l]; t.ty = ty; t.tx = tx; t.tz = tz;
+            t.z0 = kz * tz; t.x0 = kx * tx; t.y0 = ky * ty;
             int c = comp[ph / 2];
             int colour = ph & 1;
             for (int m3 = t.z0; m3 < t.z0 + tz && m3 < Z; m3++)
@@ -1421,6 +1432,7 @@ int orc_band_solve(orc_t *o, double *x, long *band_out) {
 /* device solver runs; used as the CPU baseline and as a solution check.      */
 /* ------------------------------------------------------------------------- */
 static void ell_apply(const orc_t *o, long n4, const double *x, double *y) {
+#pragma omp parallel for schedule(static)
   for (long r = 0; r < n4; r++) {
     double s = 0;
     for (int k = 0; k < o->num0[r]; k++) s += o->val1[o->pos0[r] + k] * x[o->lin0[o->pos0[r] + k]];
@@ -1430,6 +1442,7 @@ static void ell_apply(const orc_t *o, long n4, const double *x, double *y) {
 
 static double wdot(long n4, const double *w2, const double *a, const double *b) {
   double s = 0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
   for (long i = 0; i < n4; i++) s += w2[i] * a[i] * b[i];
   return s;
 }
@@ -1485,17 +1498,21 @@ int orc_gcr(orc_t *o, const int *pre, const int *post, int mode, double omega, d
   if (orc_coarse_factor(o)) return -1000;
   if (hydro) orc_hydrostatic(o, x);
   ell_apply(o, n4, x, r);
+  OMP_FOR
   for (long i = 0; i < n4; i++) r[i] = b[i] - r[i];
   double nb = sqrt(wdot(n4, w2, b, b));
   if (nb == 0) nb = 1;
   double rel = sqrt(wdot(n4, w2, r, r)) / nb;
   if (hist) hist[0] = rel;
   int it = 0, k = 0;
-  clock_t t0 = clock();
+  struct timespec ts0, ts1;
+  clock_gettime(CLOCK_MONOTONIC, &ts0);
   while (rel > tol && it < maxit) {
     double *z = Z + (long)k * n4, *q = Q + (long)k * n4;
     /* pressure part */
+    OMP_FOR
     for (long i = 0; i < n4; i++) z[i] = 0;
+    OMP_FOR
     for (long i = 0; i < n4; i += 4)
       if (o->num0[i] && !o->bondm[i]) z[i] = omega * o->nd[i / 4] * r[i];
     {
@@ -1509,14 +1526,17 @@ int orc_gcr(orc_t *o, const int *pre, const int *post, int mode, double omega, d
       orc_bc_pass(o, 0, z);
       o->vonly = vo;
       memcpy(ftmp, o->tmp, sizeof(double) * n4);
+      OMP_FOR
       for (long i = 0; i < n4; i++)
         if (i & 3) z[i] = 0;
     }
     /* velocity right-hand side r_v - G z_p */
     ell_apply(o, n4, z, q);
+    OMP_FOR
     for (long i = 0; i < n4; i++) o->fre1[i] = (i & 3) ? ((o->num0[i] && !o->bondm[i]) ? r[i] - q[i] : r[i]) : 0.0;
     memset(o->sol0, 0, sizeof(double) * 4 * o->ntot);
     orc_vcycle(o, pre, post, mode, 32, 4, 4);
+    OMP_FOR
     for (long i = 0; i < n4; i++)
       if (i & 3) z[i] = o->sol0[i];
     memcpy(o->fre1, fsave, sizeof(double) * 4 * o->ntot);
@@ -1524,28 +1544,43 @@ int orc_gcr(orc_t *o, const int *pre, const int *post, int mode, double omega, d
     for (int j = 0; j < k; j++) {
       double *zj = Z + (long)j * n4, *qj = Q + (long)j * n4;
       double be = wdot(n4, w2, q, qj);
+      OMP_FOR
       for (long i = 0; i < n4; i++) { q[i] -= be * qj[i]; z[i] -= be * zj[i]; }
     }
     double nq = sqrt(wdot(n4, w2, q, q));
+    OMP_FOR
     for (long i = 0; i < n4; i++) { q[i] /= nq; z[i] /= nq; }
     double al = wdot(n4, w2, r, q);
+    OMP_FOR
     for (long i = 0; i < n4; i++) { x[i] += al * z[i]; r[i] -= al * q[i]; }
     it++;
     k++;
     if (k == restart) {
       k = 0;
       ell_apply(o, n4, x, r);
+      OMP_FOR
       for (long i = 0; i < n4; i++) r[i] = b[i] - r[i];
     }
     rel = sqrt(wdot(n4, w2, r, r)) / nb;
     if (hist) hist[it] = rel;
     if (!(rel == rel)) break;
   }
-  if (seconds_per_it) *seconds_per_it = it ? (double)(clock() - t0) / CLOCKS_PER_SEC / it : 0.0;
+  clock_gettime(CLOCK_MONOTONIC, &ts1);
+  if (seconds_per_it)
+    *seconds_per_it = it ? ((double)(ts1.tv_sec - ts0.tv_sec) + 1e-9 * (double)(ts1.tv_nsec - ts0.tv_nsec)) / it : 0.0;
   o->vonly = 0;
   memcpy(o->fre1, fsave, sizeof(double) * 4 * o->ntot);
   memcpy(o->sol0, x, sizeof(double) * n4);
   if (x_out) memcpy(x_out, x, sizeof(double) * n4);
   free(b); free(x); free(r); free(w2); free(Z); free(Q); free(fsave);
   return it;
+}
+
+/* threads the OpenMP loops of this file use (1 without OpenMP) */
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }

@@ -121,6 +121,8 @@ int orc_gcr(orc_t *o, const int *pre, const int *post, int mode, double omega, d
 /* banded direct solver restatement (parallel/gausjomp.c:70-254) */
 int orc_band_solve(orc_t *o, double *x, long *band_out);
 
+int orc_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

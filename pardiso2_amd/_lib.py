@@ -40,7 +40,8 @@ class Opts(C.Structure):
                 ("post", C.c_int * 12), ("tile", C.c_int * 3), ("omega", C.c_double),
                 ("cycle", C.c_int), ("coarse_viscmod", C.c_int),
                 ("tol", C.c_double), ("maxit", C.c_int), ("verbose", C.c_int),
-                ("schur", C.c_int), ("ksweeps", C.c_int), ("kscale", C.c_double), ("krylov", C.c_int)]
+                ("schur", C.c_int), ("ksweeps", C.c_int), ("kscale", C.c_double), ("krylov", C.c_int),
+                ("graph", C.c_int)]
 
 
 class Stats(C.Structure):

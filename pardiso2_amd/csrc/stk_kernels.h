@@ -178,8 +178,8 @@ __global__ void k_bc_rowop(int nbc, const int *__restrict__ br, const int *__res
 // one phase of the tile smoother for component C (colour for velocities)
 template <typename T, int C>
 __device__ __forceinline__ void smooth_phase(const Geo<T> &g, const Geo<T> &gv, const unsigned char *__restrict__ flags,
-                                             const T *__restrict__ b, T *sx, int ty, int tx, int tz, int x0,
-                                             int y0, int z0, int colour, T v0koef, T p0koef) {
+                                             const T *__restrict__ b, T *sx, int nh, int ty, int tx, int tz,
+                                             int x0, int y0, int z0, int colour, T v0koef, T p0koef) {
   const int hy = ty + 2, hx = tx + 2;
   const int hty = (ty + 1) / 2;
   const int ncand = (C == 0) ? ty * tx * tz : hty * tx * tz;
@@ -204,10 +204,10 @@ __device__ __forceinline__ void smooth_phase(const Geo<T> &g, const Geo<T> &gv, 
 #pragma unroll
     for (int q = 1; q <= Slot<C>::ns; q++) {
       const int li = base + (Slot<C>::d[q][2] * hx + Slot<C>::d[q][0]) * hy + Slot<C>::d[q][1];
-      ival += s[q] * sx[4 * li + Slot<C>::d[q][3]];
+      ival += s[q] * sx[Slot<C>::d[q][3] * nh + li];
     }
-    if (C == 0) sx[4 * base] -= ival * g.ec[n] * p0koef;
-    else sx[4 * base + C] -= ival / s[1] * v0koef;
+    if (C == 0) sx[base] -= ival * g.ec[n] * p0koef;
+    else sx[C * nh + base] -= ival / s[1] * v0koef;
   }
 }
 
@@ -258,16 +258,16 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
       const T *p = xold + 4 * g.nid(m1, m2, m3);
       v0 = p[0]; v1 = p[1]; v2 = p[2]; v3 = p[3];
     }
-    sx[4 * i + 0] = v0; sx[4 * i + 1] = v1; sx[4 * i + 2] = v2; sx[4 * i + 3] = v3;
+    sx[i] = v0; sx[nh + i] = v1; sx[2 * nh + i] = v2; sx[3 * nh + i] = v3;
   }
   __syncthreads();
   const int nphase = withp ? 7 : 6;
   for (int ph = 0; ph < nphase; ph++) {
     const int colour = ph & 1;
-    if (ph < 2) smooth_phase<T, 1>(g, gv, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
-    else if (ph < 4) smooth_phase<T, 2>(g, gv, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
-    else if (ph < 6) smooth_phase<T, 3>(g, gv, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
-    else smooth_phase<T, 0>(g, gv, flags, b, sx, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
+    if (ph < 2) smooth_phase<T, 1>(g, gv, flags, b, sx, nh, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
+    else if (ph < 4) smooth_phase<T, 2>(g, gv, flags, b, sx, nh, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
+    else if (ph < 6) smooth_phase<T, 3>(g, gv, flags, b, sx, nh, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
+    else smooth_phase<T, 0>(g, gv, flags, b, sx, nh, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
     __syncthreads();
   }
   // write the tile interior
@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
     if (m1 >= g.X || m2 >= g.Y || m3 >= g.Z) continue;
     int li = ((lz + 1) * hx + (lx + 1)) * hy + (ly + 1);
     T *p = xnew + 4 * g.nid(m1, m2, m3);
-    p[0] = sx[4 * li + 0]; p[1] = sx[4 * li + 1]; p[2] = sx[4 * li + 2]; p[3] = sx[4 * li + 3];
+    p[0] = sx[li]; p[1] = sx[nh + li]; p[2] = sx[2 * nh + li]; p[3] = sx[3 * nh + li];
   }
 }
 
@@ -319,6 +319,7 @@ __global__ void k_bc_chain(BcChain ch, const T *__restrict__ b, T *x, int mode) 
 template <typename T, int C>
 __device__ __forceinline__ void level_phase(const Geo<T> &g, const unsigned char *__restrict__ flags,
                                             const T *__restrict__ b, T *sx, int colour, T v0koef, T p0koef) {
+  const int N = g.XY * g.Z;
   const int hty = (g.Y + 1) / 2;
   const int ncand = (C == 0) ? g.XY * g.Z : hty * g.X * g.Z;
   for (int i = threadIdx.x; i < ncand; i += blockDim.x) {
@@ -339,10 +340,10 @@ __device__ __forceinline__ void level_phase(const Geo<T> &g, const unsigned char
 #pragma unroll
     for (int q = 1; q <= Slot<C>::ns; q++) {
       const long nb = n + (long)Slot<C>::d[q][2] * g.XY + (long)Slot<C>::d[q][0] * g.Y + Slot<C>::d[q][1];
-      ival += s[q] * sx[4 * nb + Slot<C>::d[q][3]];
+      ival += s[q] * sx[Slot<C>::d[q][3] * N + nb];
     }
-    if (C == 0) sx[4 * n] -= ival * g.ec[n] * p0koef;
-    else sx[4 * n + C] -= ival / s[1] * v0koef;
+    if (C == 0) sx[n] -= ival * g.ec[n] * p0koef;
+    else sx[C * N + n] -= ival / s[1] * v0koef;
   }
 }
 
@@ -351,9 +352,9 @@ __global__ void __launch_bounds__(1024) k_smooth_level(Geo<T> g, const unsigned 
                                                        const T *__restrict__ b, int sweeps, int withp, T v0koef,
                                                        T p0koef, BcChain ch) {
   extern __shared__ unsigned char smem_raw[];
-  T *sx = reinterpret_cast<T *>(smem_raw);
-  const int n4 = 4 * g.XY * g.Z;
-  for (int i = threadIdx.x; i < n4; i += blockDim.x) sx[i] = x[i];
+  T *sx = reinterpret_cast<T *>(smem_raw);  // component-major: sx[c * N + node]
+  const int N = g.XY * g.Z, n4 = 4 * N;
+  for (int i = threadIdx.x; i < n4; i += blockDim.x) sx[(i & 3) * N + (i >> 2)] = x[i];
   __syncthreads();
   const int mode = withp ? 0 : 1;
   for (int s = 0; s < sweeps; s++) {
@@ -361,10 +362,18 @@ __global__ void __launch_bounds__(1024) k_smooth_level(Geo<T> g, const unsigned 
     for (int col = 0; col < 2; col++) { level_phase<T, 2>(g, flags, b, sx, col, v0koef, p0koef); __syncthreads(); }
     for (int col = 0; col < 2; col++) { level_phase<T, 3>(g, flags, b, sx, col, v0koef, p0koef); __syncthreads(); }
     if (withp) { level_phase<T, 0>(g, flags, b, sx, 0, v0koef, p0koef); __syncthreads(); }
-    for (int i = threadIdx.x; i < ch.nbc; i += blockDim.x) bc_chain_one<T>(ch, i, b, sx, mode);
+    for (int i = threadIdx.x; i < ch.nbc; i += blockDim.x) {
+      const long o = (long)i * ch.dm;
+      const int r0 = ch.r[o];
+      if ((mode == 1 && (r0 & 3) == 0) || (mode == 2 && (r0 & 3) != 0)) continue;
+      const int jf = ch.j[i];
+      T v = (jf >= 0) ? sx[(jf & 3) * N + (jf >> 2)] : T(0);
+      for (int t = ch.len[i] - 1; t >= 0; t--) v = b[ch.r[o + t]] + (T)ch.k[o + t] * v;
+      sx[(r0 & 3) * N + (r0 >> 2)] = v;
+    }
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < n4; i += blockDim.x) x[i] = sx[i];
+  for (int i = threadIdx.x; i < n4; i += blockDim.x) x[i] = sx[(i & 3) * N + (i >> 2)];
 }
 
 // boundary-row update x_r -= (x_r - k x_j - b_r), one dependency depth
@@ -899,6 +908,14 @@ __global__ void __launch_bounds__(256) k_update2(long n4, int k, const double *_
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
   __syncthreads();
   if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// x = v (a fill kernel instead of a memset node, so the preconditioner can be
+// captured in a HIP graph)
+template <typename T>
+__global__ void k_fill(long n, T v, T *x) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u < n) x[u] = v;
 }
 
 // y = s * x

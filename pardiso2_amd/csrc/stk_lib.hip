@@ -163,6 +163,12 @@ struct stk_ctx {
   double *red = nullptr, *scal = nullptr;
   int nred = 1024;
   double *kred = nullptr, *hcol = nullptr;  // flexible GMRES: per-block dots, Hessenberg column / update coefficients
+  // preconditioner captured as a HIP graph: z = M^-1 gin written to gout
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_ok = false;
+  stk_solver_opts gkey{};
+  double *gin = nullptr, *gout = nullptr;
   int cvm = -1;  // coarse-level averaging actually used by the prepared operators
   hipStream_t st = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -472,7 +478,7 @@ void mg_cycle(stk_ctx *c, const stk_solver_opts *o, const int *tile, int l, int 
 template <typename T>
 void vcycle_v(stk_ctx *c, const stk_solver_opts *o, const int *tile) {
   Level &V = c->lv[0];
-  hipMemsetAsync(lvx<T>(V), 0, 4 * V.n * sizeof(T), c->st);
+  hipLaunchKernelGGL(k_fill<T>, dim3(blocks(4 * V.n, 256)), dim3(256), 0, c->st, 4 * V.n, T(0), lvx<T>(V));
   mg_cycle<T>(c, o, tile, 0, o->cycle == 2 ? 2 : 1);
 }
 
@@ -560,14 +566,14 @@ void schur_bfbt(stk_ctx *c, const stk_solver_opts *o, const double *r, double *z
   const T sc = (T)o->kscale;
   const int nu = o->ksweeps;
   hipLaunchKernelGGL(k_bf_gather<T>, dim3(nb), dim3(256), 0, c->st, F.n, F.flags, r, kb<T>(F));
-  hipMemsetAsync(kx<T>(F), 0, F.n * sizeof(T), c->st);
+  hipLaunchKernelGGL(k_fill<T>, dim3(nb), dim3(256), 0, c->st, F.n, T(0), kx<T>(F));
   kcycle<T>(c, 0, nu, sc);
   hipLaunchKernelGGL(k_bf_grad<T>, dim3(nb), dim3(256), 0, c->st, F.X, F.Y, F.Z, iBt<T>(F, 0), iBt<T>(F, 1),
                      iBt<T>(F, 2), dinvt<T>(c), kx<T>(F), bfy<T>(c));
   rowop<T, false>(c, 0, bfy<T>(c), nullptr, bfw<T>(c), 0);
   hipLaunchKernelGGL(k_bf_div<T>, dim3(nb), dim3(256), 0, c->st, F.X, F.Y, F.Z, F.flags, iAt<T>(F, 0),
                      iAt<T>(F, 1), iAt<T>(F, 2), dinvt<T>(c), bfw<T>(c), kb<T>(F));
-  hipMemsetAsync(kx<T>(F), 0, F.n * sizeof(T), c->st);
+  hipLaunchKernelGGL(k_fill<T>, dim3(nb), dim3(256), 0, c->st, F.n, T(0), kx<T>(F));
   kcycle<T>(c, 0, nu, sc);
   hipLaunchKernelGGL(k_bf_scatter<T>, dim3(nb), dim3(256), 0, c->st, F.n, F.flags, kx<T>(F), sign, z);
 }
@@ -687,13 +693,19 @@ int coarse_inverse_v(stk_ctx *c) {
   HIPCHK(hipStreamSynchronize(c->st));
   dfree(A); dfree(fac); dfree(sing); dfree(didx);
   if (hs) { dfree(Bm); dfree(dact); return STK_ESINGULAR; }
-  dfree(c->ainv64); dfree(c->ainv32); dfree(c->cact);
-  c->ainv64 = Bm;
-  c->cact = dact;
+  if (N != c->nca || !c->ainv64) {  // same size: refill in place (captured graphs keep their pointers)
+    dfree(c->ainv64); dfree(c->ainv32); dfree(c->cact);
+    if (dalloc(&c->ainv64, (size_t)N * N) || dalloc(&c->ainv32, (size_t)N * N) || dalloc(&c->cact, N))
+      return STK_ENOMEM;
+    c->graph_ok = false;
+  }
+  HIPCHK(hipMemcpyAsync(c->ainv64, Bm, (size_t)N * N * 8, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->cact, dact, (size_t)N * 4, hipMemcpyDeviceToDevice, c->st));
   c->nca = N;
-  if (dalloc(&c->ainv32, (size_t)N * N)) return STK_ENOMEM;
   hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks((long)N * N, 256)), dim3(256), 0, c->st, (long)N * N,
                      c->ainv64, c->ainv32);
+  HIPCHK(hipStreamSynchronize(c->st));
+  dfree(Bm); dfree(dact);
   return STK_OK;
 }
 
@@ -763,16 +775,22 @@ template <typename T>
 static int autotune_levels(stk_ctx *c, int reps) {
   static const int shape[][3] = {{32, 4, 4}, {32, 4, 8}, {32, 8, 4}, {16, 8, 8}, {64, 4, 2}, {64, 2, 4},
                                  {16, 4, 4}, {32, 2, 4}, {16, 4, 8}, {8, 8, 8}, {64, 2, 2}, {32, 2, 8}};
+  static const int xz[][2] = {{4, 4}, {2, 4}, {4, 2}, {4, 8}, {8, 4}, {2, 8}, {2, 2}};
   const int ns = (int)(sizeof(shape) / sizeof(shape[0]));
-  std::vector<std::array<int, 4>> cand;
-  for (int k = 0; k < ns; k++)
-    for (int sv = 0; sv < 2; sv++) {
-      const std::array<int, 4> t = {shape[k][0], shape[k][1], shape[k][2], sv};
-      if (smooth_lds<T>(t.data(), sv) <= kLevelLds) cand.push_back(t);
-    }
   for (int l = 0; l <= c->L; l++) {
     Level &L = c->lv[l];
     if (4 * (size_t)L.n * sizeof(T) <= kLevelLds) continue;  // smoothed by the whole-level kernel
+    // fixed shapes, plus tile heights that split the level's y extent evenly
+    std::vector<std::array<int, 4>> cand;
+    auto add = [&](int ty, int tx, int tz) {
+      for (int sv = 0; sv < 2; sv++) {
+        const std::array<int, 4> t = {ty, tx, tz, sv};
+        if (ty >= 2 && smooth_lds<T>(t.data(), sv) <= kLevelLds) cand.push_back(t);
+      }
+    };
+    for (int k = 0; k < ns; k++) add(shape[k][0], shape[k][1], shape[k][2]);
+    for (int parts = 2; parts <= 8; parts++)
+      for (const auto &q : xz) add((L.Y + parts - 1) / parts, q[0], q[1]);
     double best = 1e30;
     int bi = 0;
     for (int k = 0; k < (int)cand.size(); k++) {
@@ -909,7 +927,9 @@ void stk_destroy(stk_ctx *c) {
   dfree(c->nu); dfree(c->ro); dfree(c->nd); dfree(c->nxx); dfree(c->nxy); dfree(c->nxz); dfree(c->nyz);
   dfree(c->ainv64); dfree(c->ainvP64); dfree(c->ainv32); dfree(c->cact);
   dfree(c->X); dfree(c->B); dfree(c->R); dfree(c->W2); dfree(c->T1); dfree(c->red); dfree(c->scal);
-  dfree(c->kred); dfree(c->hcol);
+  dfree(c->kred); dfree(c->hcol); dfree(c->gin); dfree(c->gout);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
   for (auto p : c->Zs) dfree(p);
   for (auto p : c->Qs) dfree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -927,6 +947,7 @@ int stk_level_info(const stk_ctx *c, int l, int out6[6]) {
 
 int stk_set_knobs(stk_ctx *c, const stk_knobs *k) {
   if (!c || !k || k->viscmod < 0 || k->viscmod > 2 || !(k->nubeg > 0) || !(k->nuend >= k->nubeg)) return STK_EINVAL;
+  c->graph_ok = c->graph_ok && !memcmp(&c->kn, k, sizeof *k);  // relaxation weights are captured by value
   c->kn = *k;
   c->prepared = false;
   return STK_OK;
@@ -948,6 +969,7 @@ void stk_default_opts(stk_solver_opts *o, int levels) {
   o->schur = 1;
   o->krylov = 1;  // GCR can stagnate when the preconditioned operator is indefinite (sharp strong inclusions)
   o->ksweeps = 2;
+  o->graph = 0;  // replay gains ~5% on small levels; captures from several host threads at once are not supported
   o->kscale = 2.0;  // piecewise-constant aggregation undershoots the coarse correction by about 2
 }
 
@@ -1209,14 +1231,22 @@ static int build_schur(stk_ctx *c) {
     }
   }
   std::vector<float> I32(I.begin(), I.end());
-  dfree(c->kinv64); dfree(c->kinv32); dfree(c->kcell);
-  if (upload(&c->kinv64, I) || upload(&c->kinv32, I32) || upload(&c->kcell, cell)) return STK_ENOMEM;
+  if (N != c->nkc || !c->kinv64) {  // same size: refill in place (captured graphs keep their pointers)
+    dfree(c->kinv64); dfree(c->kinv32); dfree(c->kcell);
+    if (dalloc(&c->kinv64, (size_t)N * N) || dalloc(&c->kinv32, (size_t)N * N) || dalloc(&c->kcell, N))
+      return STK_ENOMEM;
+    c->graph_ok = false;
+  }
+  H2D(c, c->kinv64, I.data(), (size_t)N * N * 8);
+  H2D(c, c->kinv32, I32.data(), (size_t)N * N * 4);
+  H2D(c, c->kcell, cell.data(), (size_t)N * 4);
   c->nkc = N;
   return STK_OK;
 }
 
 // flags + boundary lists of every level from the reference tables
 static int build_rows(stk_ctx *c) {
+  c->graph_ok = false;  // boundary lists are reallocated
   for (int l = 0; l <= c->L; l++) {
     Level &L = c->lv[l];
     L.hflags.assign(L.n, 0);
@@ -1476,6 +1506,50 @@ int stk_time_kernel(stk_ctx *c, int which, int level, int reps, double *ms_avg) 
   return STK_OK;
 }
 
+// ---- preconditioner as a HIP graph ---------------------------------------
+// The preconditioner is a fixed sequence of launches on the library stream
+// with no host synchronisation, memsets or allocations, so it is captured
+// once (input gin, output gout) and replayed every Krylov step.  The capture
+// stays valid while the options that shape the sequence and the buffers it
+// names are unchanged (a prepare that keeps the sizes refills in place).
+static bool same_shape(const stk_solver_opts &a, const stk_solver_opts &b) {
+  return a.precision == b.precision && !memcmp(a.pre, b.pre, sizeof a.pre) && !memcmp(a.post, b.post, sizeof a.post) &&
+         !memcmp(a.tile, b.tile, sizeof a.tile) && a.omega == b.omega && a.cycle == b.cycle && a.schur == b.schur &&
+         a.ksweeps == b.ksweeps && a.kscale == b.kscale;
+}
+
+static int ensure_graph(stk_ctx *c, const stk_solver_opts *o, const int *tile) {
+  if (c->graph_ok && same_shape(c->gkey, *o)) return STK_OK;
+  const long n4 = 4 * c->lv[0].n;
+  if (!c->gin && (dalloc(&c->gin, n4) || dalloc(&c->gout, n4))) return STK_ENOMEM;
+  if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  if (o->precision == 1) precond<double>(c, o, c->gin, c->gout, tile);
+  else precond<float>(c, o, c->gin, c->gout, tile);
+  HIPCHK(hipStreamEndCapture(c->st, &c->graph));
+  HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  c->gkey = *o;
+  c->graph_ok = true;
+  return STK_OK;
+}
+
+// z = M^-1 r, replayed from the captured graph or launched eagerly
+static int apply_precond(stk_ctx *c, const stk_solver_opts *o, const int *tile, const double *r, double *z) {
+  const long n4 = 4 * c->lv[0].n;
+  if (o->graph) {
+    HIPCHK(hipMemcpyAsync(c->gin, r, n4 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipGraphLaunch(c->gexec, c->st));
+    HIPCHK(hipMemcpyAsync(z, c->gout, n4 * 8, hipMemcpyDeviceToDevice, c->st));
+  } else if (o->precision == 1) {
+    precond<double>(c, o, r, z, tile);
+  } else {
+    precond<float>(c, o, r, z, tile);
+  }
+  return STK_OK;
+}
+
 // Flexible GMRES(m), right-preconditioned, in the W-weighted inner product,
 // with delayed classical Gram-Schmidt reorthogonalisation: step j
 // preconditions the once-orthogonalised candidate u (Z_j = M^-1 u), forms
@@ -1524,8 +1598,8 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
     bool conv = false;
     while (j < m && it < o->maxit) {
       double *u = c->Qs[j], *Z = c->Zs[j], *W = c->Qs[j + 1];
-      if (fp64) precond<double>(c, o, u, Z, tile);
-      else precond<float>(c, o, u, Z, tile);
+      int prc = apply_precond(c, o, tile, u, Z);
+      if (prc) { status = prc; break; }
       rowop<double, false>(c, 0, Z, nullptr, W, 1);
       PtrsK P{};
       for (int i = 0; i < j; i++) P.p[i] = c->Qs[i];
@@ -1605,7 +1679,7 @@ int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   if (!c->prepared) return STK_ESTATE;
   if (o->restart < 1 || o->restart > 128 || o->maxit < 1 || !(o->tol > 0) || o->cycle < 0 || o->cycle > 2 ||
       o->coarse_viscmod < -1 || o->coarse_viscmod > 2 || o->schur < 0 || o->schur > 2 || o->ksweeps < 0 ||
-      o->ksweeps > 16 || !(o->kscale > 0) || o->krylov < 0 || o->krylov > 1)
+      o->ksweeps > 16 || !(o->kscale > 0) || o->krylov < 0 || o->krylov > 1 || o->graph < 0 || o->graph > 1)
     return STK_EINVAL;
   {
     const int want = (o->coarse_viscmod < 0) ? c->kn.viscmod : o->coarse_viscmod;
@@ -1624,6 +1698,10 @@ int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   if (rc) return rc;
   rc = ensure_outer(c, o->restart + 1);
   if (rc) return rc;
+  if (o->graph) {
+    rc = ensure_graph(c, o, tile);
+    if (rc) return rc;
+  }
   stk_stats local;
   stk_stats *S = st ? st : &local;
   memset(S, 0, sizeof(*S));
@@ -1647,8 +1725,8 @@ int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   }
   while (o->krylov == 0 && rel > o->tol && it < o->maxit) {
     double *Z = c->Zs[k], *Q = c->Qs[k];
-    if (fp64) precond<double>(c, o, c->R, Z, tile);
-    else precond<float>(c, o, c->R, Z, tile);
+    rc = apply_precond(c, o, tile, c->R, Z);
+    if (rc) return rc;
     rowop<double, false>(c, 0, Z, nullptr, Q, 1);
     if (k > 0) {
       std::vector<const double *> bs(c->Qs.begin(), c->Qs.begin() + k);
@@ -1785,6 +1863,7 @@ int stk_coarse_solve(stk_ctx *c) {
 // ---- smoother tile autotuning -----------------------------------------------
 int stk_autotune(stk_ctx *c, int precision, int reps) {
   if (!c || !c->prepared || reps < 1 || precision < 0 || precision > 1) return STK_EINVAL;
+  c->graph_ok = false;  // captured launches use the tiles of the time of capture
   int rc = ensure_vectors(c, precision == 0, precision == 1);
   if (rc) return rc;
   return precision ? autotune_levels<double>(c, reps) : autotune_levels<float>(c, reps);
@@ -1822,7 +1901,9 @@ int stk_solve_batch(int n, stk_ctx *const *ctxs, const stk_solver_opts *opts, in
         rc = stk_prepare(ctxs[i]);
         if (!rc) rc = stk_hydrostatic_guess(ctxs[i]);
       }
-      rcs[i] = rc ? rc : stk_solve(ctxs[i], &opts[nopts == 1 ? 0 : i], stats ? &stats[i] : nullptr);
+      stk_solver_opts oi = opts[nopts == 1 ? 0 : i];
+      oi.graph = 0;  // stream capture from concurrent host threads is not used
+      rcs[i] = rc ? rc : stk_solve(ctxs[i], &oi, stats ? &stats[i] : nullptr);
     }
   };
   std::vector<std::thread> pool;

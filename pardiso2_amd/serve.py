@@ -103,6 +103,7 @@ class Server:
                  keep_solutions: bool = False):
         self.workers = workers
         self.opts_kw = dict(opts_kw or {})
+        self.opts_kw["graph"] = 0  # workers launch eagerly (no concurrent stream captures)
         self.restart = self.opts_kw.get("restart", 128)
         self.pool = ContextPool(self.restart)
         self.reserve = reserve_bytes
