@@ -132,7 +132,7 @@ int stk_relres(stk_ctx *ctx, double *relres, double blocks[3]);
  * z-momentum rows (the reference seeds P this way, parallel/in3mg.c:927-938). */
 int stk_hydrostatic_guess(stk_ctx *ctx);
 /* Average device time (ms) of one launch of a finest-level kernel, HIP events
- * on the library stream.  which: 0 FP32 smoother sweep, 1 FP64 smoother
+ * on the library stream (smoothers on the default 32x4x4 tile).  which: 0 FP32 smoother sweep, 1 FP64 smoother
  * sweep, 2 FP64 operator apply, 3 FP32 residual, 4 FP64 residual, 5 plain copy
  * of 4*mgn[l] doubles (32 B read + 32 B written per node). */
 int stk_time_kernel(stk_ctx *ctx, int which, int level, int reps, double *ms_avg);

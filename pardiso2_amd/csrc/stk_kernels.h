@@ -828,8 +828,9 @@ __global__ void __launch_bounds__(256) k_maxpy_norm(long n4, int k, const double
 }
 
 // Delayed classical Gram-Schmidt with reorthogonalisation (one pass for the
-// dots, one for the updates per Arnoldi step).  Pass 1: partial sums of
-// a_i = <V_i,u>, b_i = <V_i,w> (i < k), <u,w>, <u,u> in the W2 inner product.
+// dots, one for the updates per Arnoldi step).  Vectors are read as double2
+// (n4 is a multiple of 4).  Pass 1: partial sums of a_i = <V_i,u>,
+// b_i = <V_i,w> (i < k), <u,w>, <u,u> in the W2 inner product.
 __global__ void __launch_bounds__(256) k_dots2(long n4, const double *__restrict__ w2, const double *__restrict__ u,
                                                const double *__restrict__ w, PtrsK V, int k, double *partial) {
   __shared__ double acc[2 * kMaxDirs + 2][4];
@@ -837,19 +838,23 @@ __global__ void __launch_bounds__(256) k_dots2(long n4, const double *__restrict
   const int nd = 2 * k + 2;
   for (int i = threadIdx.x; i < (2 * kMaxDirs + 2) * 4; i += blockDim.x) acc[i >> 2][i & 3] = 0.0;
   __syncthreads();
-  constexpr int E = 8;
-  for (long base = (long)blockIdx.x * 256 * E; base < n4; base += (long)gridDim.x * 256 * E) {
-    double wu[E], ww[E];
+  constexpr int E = 8;  // double2 per thread per slice
+  const long n2 = n4 >> 1;
+  const double2 *__restrict__ W2 = reinterpret_cast<const double2 *>(w2);
+  const double2 *__restrict__ U = reinterpret_cast<const double2 *>(u);
+  const double2 *__restrict__ Wv = reinterpret_cast<const double2 *>(w);
+  for (long base = (long)blockIdx.x * 256 * E; base < n2; base += (long)gridDim.x * 256 * E) {
+    double2 wu[E], ww[E];
     double suw = 0.0, suu = 0.0;
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const long p = base + threadIdx.x + 256L * e;
-      const double q = (p < n4) ? w2[p] : 0.0;
-      const double uu = (p < n4) ? u[p] : 0.0, wv = (p < n4) ? w[p] : 0.0;
-      wu[e] = q * uu;
-      ww[e] = q * wv;
-      suw += wu[e] * wv;
-      suu += wu[e] * uu;
+      double2 q = make_double2(0.0, 0.0), uu = q, wv = q;
+      if (p < n2) { q = W2[p]; uu = U[p]; wv = Wv[p]; }
+      wu[e] = make_double2(q.x * uu.x, q.y * uu.y);
+      ww[e] = make_double2(q.x * wv.x, q.y * wv.y);
+      suw += wu[e].x * wv.x + wu[e].y * wv.y;
+      suu += wu[e].x * uu.x + wu[e].y * uu.y;
     }
     for (int off = 32; off > 0; off >>= 1) {
       suw += __shfl_xor(suw, off);
@@ -857,15 +862,15 @@ __global__ void __launch_bounds__(256) k_dots2(long n4, const double *__restrict
     }
     if (lane == 0) { acc[2 * k][wave] += suw; acc[2 * k + 1][wave] += suu; }
     for (int i = 0; i < k; i++) {
-      const double *__restrict__ v = V.p[i];
+      const double2 *__restrict__ v = reinterpret_cast<const double2 *>(V.p[i]);
       double sa = 0.0, sb = 0.0;
 #pragma unroll
       for (int e = 0; e < E; e++) {
         const long p = base + threadIdx.x + 256L * e;
-        if (p < n4) {
-          const double vv = v[p];
-          sa += wu[e] * vv;
-          sb += ww[e] * vv;
+        if (p < n2) {
+          const double2 vv = v[p];
+          sa += wu[e].x * vv.x + wu[e].y * vv.y;
+          sb += ww[e].x * vv.x + ww[e].y * vv.y;
         }
       }
       for (int off = 32; off > 0; off >>= 1) {
@@ -890,19 +895,28 @@ __global__ void __launch_bounds__(256) k_update2(long n4, int k, const double *_
   for (int i = threadIdx.x; i < 2 * k + 2; i += blockDim.x) sc[i] = coef[i];
   __syncthreads();
   const double hk = sc[2 * k], su = sc[2 * k + 1];
+  const long n2 = n4 >> 1;
+  double2 *U = reinterpret_cast<double2 *>(u);
+  double2 *Wv = reinterpret_cast<double2 *>(w);
+  const double2 *__restrict__ W2 = reinterpret_cast<const double2 *>(w2);
   double ss = 0.0;
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n4; p += (long)gridDim.x * blockDim.x) {
-    double uu = u[p], wv = w[p];
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n2; p += (long)gridDim.x * blockDim.x) {
+    double2 uu = U[p], wv = Wv[p];
     for (int i = 0; i < k; i++) {
-      const double vv = V.p[i][p];
-      uu -= sc[i] * vv;
-      wv -= sc[k + i] * vv;
+      const double2 vv = reinterpret_cast<const double2 *>(V.p[i])[p];
+      uu.x -= sc[i] * vv.x;
+      uu.y -= sc[i] * vv.y;
+      wv.x -= sc[k + i] * vv.x;
+      wv.y -= sc[k + i] * vv.y;
     }
-    uu *= su;
-    wv -= hk * uu;
-    u[p] = uu;
-    w[p] = wv;
-    ss += w2[p] * wv * wv;
+    uu.x *= su;
+    uu.y *= su;
+    wv.x -= hk * uu.x;
+    wv.y -= hk * uu.y;
+    U[p] = uu;
+    Wv[p] = wv;
+    const double2 q = W2[p];
+    ss += q.x * wv.x * wv.x + q.y * wv.y * wv.y;
   }
   for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;

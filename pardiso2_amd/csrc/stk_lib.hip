@@ -791,11 +791,14 @@ static int autotune_levels(stk_ctx *c, int reps) {
     for (int k = 0; k < ns; k++) add(shape[k][0], shape[k][1], shape[k][2]);
     for (int parts = 2; parts <= 8; parts++)
       for (const auto &q : xz) add((L.Y + parts - 1) / parts, q[0], q[1]);
-    double best = 1e30;
+    // candidate 0 is the default 32x4x4 tile; another shape must beat it by 3 %
+    // so that timing noise does not move the choice
+    double best = 1e30, base = 0.0;
     int bi = 0;
     for (int k = 0; k < (int)cand.size(); k++) {
       const double ms = time_smooth<T>(c, l, cand[k].data(), reps);
-      if (ms < best) { best = ms; bi = k; }
+      if (k == 0) base = ms;
+      if (ms < best && (k == 0 || ms < 0.97 * base)) { best = ms; bi = k; }
     }
     int *t = std::is_same<T, float>::value ? L.tune32 : L.tune64;
     for (int a = 0; a < 4; a++) t[a] = cand[bi][a];
@@ -1452,7 +1455,7 @@ int stk_time_kernel(stk_ctx *c, int which, int level, int reps, double *ms_avg) 
   int rc = ensure_vectors(c, true, true);
   if (rc) return rc;
   Level &L = c->lv[level];
-  const int *tt = (which == 1) ? tuned_tile<double>(L) : tuned_tile<float>(L);
+  const int *tt = kTileDefault4;  // the default tile: timings comparable across runs and with counter runs
   int tile[3] = {tt[0], tt[1], tt[2]};
   int ty = tile[0], tx = tile[1], tz = tile[2];
   int nty = (L.Y + ty - 1) / ty, ntx = (L.X + tx - 1) / tx, ntz = (L.Z + tz - 1) / tz;
@@ -1567,8 +1570,8 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
   const bool fp64 = o->precision == 1;
   // dots: one block per 2048-element slice up to nred blocks; updates: up to
   // 4 nred blocks (the norm partials need one double per block)
-  const unsigned nbk = (unsigned)std::min<long>(c->nred, std::max<long>(1, (n4 + 2047) / 2048));
-  const unsigned nbu = (unsigned)std::min<long>(4L * c->nred, (long)blocks(n4, 256));
+  const unsigned nbk = (unsigned)std::min<long>(c->nred, std::max<long>(1, (n4 / 2 + 2047) / 2048));
+  const unsigned nbu = (unsigned)std::min<long>(4L * c->nred, (long)blocks(n4 / 2, 256));
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), hh(2 * m + 2), y(m), coef(2 * m + 2);
   auto h = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
   auto givens = [&](int col) {

@@ -1,12 +1,15 @@
 """Summarise rocprofv3 CSV output (kernel stats and PMC counter collection).
 
     python -m pardiso2_amd.rocprof_summary stats <dir> [--top N]
-    python -m pardiso2_amd.rocprof_summary pmc <dir> [--kernel SUBSTR]
+    python -m pardiso2_amd.rocprof_summary trace <dir> --kernel SUBSTR [--last N]
+    python -m pardiso2_amd.rocprof_summary pmc <dir> [--kernel SUBSTR] [--last N]
 
 `stats` reads the *_kernel_stats.csv that `rocprofv3 --kernel-trace --stats
---output-format csv` writes; `pmc` reads *_counter_collection.csv of a
-`rocprofv3 --pmc ... --output-format csv` run and prints, per kernel name and
-counter, the number of dispatches and the mean value per dispatch.
+--output-format csv` writes; `trace` reads the same run's *_kernel_trace.csv
+and prints the duration statistics of the matching dispatches (the last N of
+them with --last); `pmc` reads *_counter_collection.csv of a `rocprofv3 --pmc
+... --output-format csv` run and prints, per kernel name and counter, the
+number of dispatches and the mean value per dispatch (of the last N).
 """
 from __future__ import annotations
 
@@ -48,7 +51,27 @@ def stats(d, top):
     print(f"total kernel time {tot / 1e6:.3f} ms over {sum(int(_col(r, 'Calls')) for r in rows)} dispatches")
 
 
-def pmc(d, kernel):
+def trace(d, kernel, last):
+    rows = []
+    for f in _find(d, "kernel_trace.csv"):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                name = _col(r, "Kernel_Name", "KernelName")
+                if kernel in name:
+                    t0, t1 = float(_col(r, "Start_Timestamp")), float(_col(r, "End_Timestamp"))
+                    rows.append((int(_col(r, "Dispatch_Id", "Correlation_Id")), name, t1 - t0))
+    rows.sort()
+    if last:
+        rows = rows[-last:]
+    if not rows:
+        raise SystemExit(f"no dispatch of {kernel}")
+    ds = sorted(r[2] for r in rows)
+    print(f"kernel {rows[-1][1][:110]}")
+    print(f"dispatches {len(ds)}  avg us {sum(ds) / len(ds) / 1e3:.3f}  min us {ds[0] / 1e3:.3f}  "
+          f"median us {ds[len(ds) // 2] / 1e3:.3f}  max us {ds[-1] / 1e3:.3f}")
+
+
+def pmc(d, kernel, last=0):
     per = defaultdict(float)  # (kernel, counter, dispatch) -> value summed over the row's dimensions
     for f in _find(d, "counter_collection.csv"):
         with open(f) as fh:
@@ -63,23 +86,28 @@ def pmc(d, kernel):
                 cn = _col(r, "Counter_Name", "CounterName")
                 per[(name, cn, disp)] += float(_col(r, "Counter_Value", "CounterValue"))
     acc = defaultdict(list)
-    for (name, cn, _), v in per.items():
+    for (name, cn, disp), v in sorted(per.items(), key=lambda kv: (kv[0][0], kv[0][1], int(kv[0][2]))):
         acc[(name, cn)].append(v)
+    if last:
+        acc = {key: v[-last:] for key, v in acc.items()}
     for (name, cn), v in sorted(acc.items()):
         print(f"{cn:>12} dispatches {len(v):5d} mean {sum(v) / len(v):14.1f}  {name[:100]}")
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["stats", "pmc"])
+    ap.add_argument("mode", choices=["stats", "trace", "pmc"])
     ap.add_argument("dir")
     ap.add_argument("--top", type=int, default=25)
     ap.add_argument("--kernel", default=None)
+    ap.add_argument("--last", type=int, default=0)
     a = ap.parse_args(argv)
     if a.mode == "stats":
         stats(a.dir, a.top)
+    elif a.mode == "trace":
+        trace(a.dir, a.kernel or "", a.last)
     else:
-        pmc(a.dir, a.kernel)
+        pmc(a.dir, a.kernel, a.last)
 
 
 if __name__ == "__main__":
