@@ -59,16 +59,23 @@ def sweeps_for(L: int):
     return pre, pre
 
 
-def cpu_baseline(pb, pre, post, its_needed: int, budget_s: float):
+def cpu_baseline(pb, pre, post, its_needed: int, budget_s: float, check=None):
     """The oracle's FP64 GCR (tile-coloured smoother, the device's V-cycle) on
     the same problem with the host's OpenMP threads, for a bounded number of
-    outer iterations."""
+    outer iterations.  ``check`` (a device solution) is first scored by the
+    oracle's own assembled FP64 operator; the relative residual is returned
+    under "oracle_relres"."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import oracle as O
     O.build()
     t0 = time.perf_counter()
     o = O.from_problem(pb)
     setup_s = time.perf_counter() - t0
+    orel = None
+    if check is not None:
+        o.arr("sol0")[: check.size] = check
+        orel = float(o.relres()[0])
+        o.arr("sol0")[:] = 0.0
     # one iteration to estimate, then as many as fit the budget
     it, _, _, spi = o.gcr(pre, post, mode=1, tol=1e-30, maxit=1, restart=32)
     k = int(max(1, min(20, (budget_s - spi) / max(spi, 1e-6))))
@@ -76,11 +83,33 @@ def cpu_baseline(pb, pre, post, its_needed: int, budget_s: float):
     per_solve = spi * its_needed
     nt = O.threads()
     return {"value": 1.0 / per_solve if per_solve > 0 else None, "unit": "solves/s", "cores": nt,
-            "kind": "port",
+            "kind": "port", "oracle_relres": orel,
             "sample": (f"{it} outer iterations ({spi:.3f} s each, wall clock) of the oracle's FP64 GCR + "
                        f"velocity V-cycle (tile-coloured smoother, mass-matrix Schur scaling) on the same "
                        f"{pb.n}^3 problem with {nt} OpenMP threads, times the {its_needed} iterations the "
                        f"FP64 device solve needs; oracle setup {setup_s:.1f} s not counted")}
+
+
+def solution_error(ctx, x):
+    """Max and RMS difference of solution x from the context's current (FP64)
+    solution, velocities and pressures separately, relative to the FP64
+    solution's max and RMS; pressures compared at a common gauge (cell 2,2,2)."""
+    def gauged(v):
+        v = v.copy()
+        p = v[0::4]
+        X, Y = ctx.info[0][0], ctx.info[0][1]
+        ref = p[2 * X * Y + 2 * Y + 2]
+        v[0::4] = p - ref
+        return v
+    ctx.pressure_gauge(2, 2, 2, 0.0)
+    ref = ctx.solution()
+    a, b = gauged(x), ref
+    out = {}
+    for name, sl in (("velocity", np.arange(a.size) % 4 != 0), ("pressure", np.arange(a.size) % 4 == 0)):
+        d, r = a[sl] - b[sl], b[sl]
+        out[name] = {"max_rel": float(np.abs(d).max() / max(np.abs(r).max(), 1e-300)),
+                     "rms_rel": float(np.sqrt(np.mean(d * d)) / max(np.sqrt(np.mean(r * r)), 1e-300))}
+    return out
 
 
 def smoother_roofline(ctx, nodes):
@@ -110,7 +139,8 @@ def bench_batched(args):
     problems = [setups.inclusions(n, seed=args.seed * 1000 + i) for i, n in enumerate(sizes)]
     ctxs = [S.from_problem(pb) for pb in problems]
     log(f"{len(ctxs)} instances ready: " + ", ".join(f"{n}^3 x{sizes.count(n)}" for n in sorted(set(sizes))))
-    mk = lambda pb, prec: S.default_opts(pb.levels, precision=prec, tol=1e-8, maxit=args.maxit)
+    gkw = {} if args.graph is None else {"graph": args.graph}
+    mk = lambda pb, prec: S.default_opts(pb.levels, precision=prec, tol=1e-8, maxit=args.maxit, **gkw)
     opts = [mk(pb, args.precision) for pb in problems]
     for i in range(args.warmup):
         rc, st = S.solve_batch(ctxs, opts, threads=args.threads, prepare=True)
@@ -147,6 +177,7 @@ def bench_batched(args):
         "dtype": "f32 smoother / f64 outer" if args.precision == 0 else "f64", "data": "synthetic",
         "config": {"workload": f"batched: {args.instances} instances, sizes from {{33,49,65,97}}^3, 1-4 inclusions, "
                                f"contrast 1e0..1e6, {args.threads} concurrent streams",
+                   "graph": int(opts[0].graph),
                    "sizes": {str(n): sizes.count(n) for n in sorted(set(sizes))}, "seed": args.seed, "tol": 1e-8},
         "converged": conv, "p50_ms": float(np.percentile(np.array(lat) * 1e3, 50)),
         "p99_ms": float(np.percentile(np.array(lat) * 1e3, 99)),
@@ -228,6 +259,7 @@ def main():
     ap.add_argument("--threads", type=int, default=16, help="batched/served: host workers (one stream each)")
     ap.add_argument("--rate", type=float, default=50.0, help="served workload: arrivals per second")
     ap.add_argument("--duration", type=float, default=60.0, help="served workload: arrival window in seconds")
+    ap.add_argument("--graph", type=int, default=None, help="1: replay the preconditioner from HIP graphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
@@ -291,7 +323,8 @@ def main():
         if tj.get("nodes") == n0:
             traffic = k.get("hbm_bytes_per_launch")
 
-    fp64_its = None
+    x_dev = ctx.solution()
+    fp64_its, err = None, None
     if not args.no_fp64_ref:
         o64 = S.default_opts(L, precision=1, pre=pre, post=post, tol=1e-8, maxit=args.maxit, restart=args.restart)
         ctx.prepare()
@@ -299,12 +332,13 @@ def main():
         s64 = ctx.solve(o64)
         fp64_its = s64.iterations
         log(f"fp64 reference solve: {s64.iterations} iterations, relres {s64.relres:.3e}, {s64.seconds:.2f} s")
+        err = solution_error(ctx, x_dev)
 
     cpu = None
     log(f"smoother {ms_sm:.4f} ms ({gbs_sm:.0f} GB/s), fp64 residual {ms_res:.4f} ms ({gbs_res:.0f} GB/s)")
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(pb, pre, post, fp64_its or its[-1], args.cpu_budget)
+            cpu = cpu_baseline(pb, pre, post, fp64_its or its[-1], args.cpu_budget, check=x_dev)
         except Exception as e:  # the baseline must not hide the device result
             cpu = {"value": None, "unit": "solves/s", "cores": 1, "kind": "port", "sample": f"failed: {e}"}
 
@@ -330,6 +364,8 @@ def main():
         "relres": max(rels),
         "solve_ms_device": float(np.mean(solve_s) * 1e3),
         "fp64_iterations": fp64_its,
+        "error_vs_fp64_solve": err,
+        "oracle_relres": cpu.get("oracle_relres") if cpu else None,
         "smoother_tiles": tiles,
         "roofline": {"bound": "hbm", "kernel": "k_smooth<float> finest level, 32x4x4 tile", "achieved": gbs_sm,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs_sm / PEAK_HBM_GBS, "traffic": traffic,

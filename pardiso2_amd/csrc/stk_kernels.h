@@ -421,40 +421,33 @@ template <typename T> __device__ __forceinline__ T tw(const double *a, const flo
 template <> __device__ __forceinline__ double tw<double>(const double *a, const float *, long i) { return a[i]; }
 template <> __device__ __forceinline__ float tw<float>(const double *, const float *b, long i) { return b[i]; }
 
-// vprestrict: coarse b from fine residual r (parallel/move3mg.c:3966-4354)
-template <typename T>
-__global__ void k_restrict(int Xf, int Yf, int Zf, int Xc, int Yc, int Zc, CompTab t0, CompTab t1,
-                           CompTab t2, CompTab t3, const unsigned char *__restrict__ ff,
-                           const unsigned char *__restrict__ fc, const T *__restrict__ rf,
-                           const float *__restrict__ ndf, const float *__restrict__ ndc, T *bc, T *xc,
-                           int withp) {
-  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  long n = u >> 2;
-  const int XYc = Xc * Yc, XYf = Xf * Yf;
-  if (n >= (long)XYc * Zc) return;
-  int c = (int)(u & 3);
+// vprestrict: coarse b from fine residual r (parallel/move3mg.c:3966-4354).
+// One thread per coarse node does its four unknowns, each summed over its
+// fine contributors in the reference's order.
+template <typename T, int C>
+__device__ __forceinline__ void restrict_one(const CompTab &t, int Xf, int Yf, int XYf, int cx, int cy, int cz, long n,
+                                             const unsigned char *__restrict__ ff, const unsigned char *__restrict__ fc,
+                                             const T *__restrict__ rf, const float *__restrict__ ndf,
+                                             const float *__restrict__ ndc, T *bc, T *xc) {
+  const long u = 4 * n + C;
   xc[u] = T(0);
-  int kc = kind_of(fc[n], c);
-  if (kc != KIND_EQ || (!withp && c == 0)) { bc[u] = T(0); return; }
-  int cx, cy, cz;
-  decode(n, Xc, Yc, XYc, cx, cy, cz);
-  const CompTab &t = (c == 0) ? t0 : (c == 1) ? t1 : (c == 2) ? t2 : t3;
+  if (kind_of(fc[n], C) != KIND_EQ) { bc[u] = T(0); return; }
   T acc = T(0), ws = T(0);
   for (int a = t.ax[2].cbeg[cz]; a < t.ax[2].cbeg[cz + 1]; a++) {
-    int k = t.ax[2].cfi[a];
-    T wz = tw<T>(t.ax[2].cw64, t.ax[2].cw32, a);
+    const int k = t.ax[2].cfi[a];
+    const T wz = tw<T>(t.ax[2].cw64, t.ax[2].cw32, a);
     for (int bq = t.ax[0].cbeg[cx]; bq < t.ax[0].cbeg[cx + 1]; bq++) {
-      int i = t.ax[0].cfi[bq];
-      T wx = tw<T>(t.ax[0].cw64, t.ax[0].cw32, bq);
+      const int i = t.ax[0].cfi[bq];
+      const T wx = tw<T>(t.ax[0].cw64, t.ax[0].cw32, bq);
       for (int e = t.ax[1].cbeg[cy]; e < t.ax[1].cbeg[cy + 1]; e++) {
-        int j = t.ax[1].cfi[e];
-        long nf = (long)k * XYf + (long)i * Yf + j;
-        if (kind_of(ff[nf], c) != KIND_EQ) continue;
-        T wy = tw<T>(t.ax[1].cw64, t.ax[1].cw32, e);
-        T swt = wx * wy * wz;
+        const int j = t.ax[1].cfi[e];
+        const long nf = (long)k * XYf + (long)i * Yf + j;
+        if (kind_of(ff[nf], C) != KIND_EQ) continue;
+        const T wy = tw<T>(t.ax[1].cw64, t.ax[1].cw32, e);
+        const T swt = wx * wy * wz;
         T iv;
-        if (c == 0) iv = rf[4 * nf] * (T)ndf[nf] / (T)ndc[n] * swt;
-        else iv = rf[4 * nf + c] * swt;
+        if (C == 0) iv = rf[4 * nf] * (T)ndf[nf] / (T)ndc[n] * swt;
+        else iv = rf[4 * nf + C] * swt;
         acc += iv;
         ws += swt;
       }
@@ -463,36 +456,62 @@ __global__ void k_restrict(int Xf, int Yf, int Zf, int Xc, int Yc, int Zc, CompT
   bc[u] = (ws != T(0)) ? acc / ws : acc;
 }
 
-// vpprolong: fine x += sum coarse x * w * koef (parallel/move3mg.c:3535-3850)
 template <typename T>
-__global__ void k_prolong(int Xf, int Yf, int Zf, int Xc, int Yc, int Zc, CompTab t0, CompTab t1,
-                          CompTab t2, CompTab t3, const unsigned char *__restrict__ ff,
-                          const T *__restrict__ xc, T *xf, T pk, T vk, int withp, int ynreset) {
-  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  long n = u >> 2;
+__global__ void k_restrict(int Xf, int Yf, int Zf, int Xc, int Yc, int Zc, CompTab t0, CompTab t1,
+                           CompTab t2, CompTab t3, const unsigned char *__restrict__ ff,
+                           const unsigned char *__restrict__ fc, const T *__restrict__ rf,
+                           const float *__restrict__ ndf, const float *__restrict__ ndc, T *bc, T *xc,
+                           int withp) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
   const int XYc = Xc * Yc, XYf = Xf * Yf;
-  if (n >= (long)XYf * Zf) return;
-  int c = (int)(u & 3);
-  if (kind_of(ff[n], c) == KIND_NONE || (!withp && c == 0)) return;
-  int m1, m2, m3;
-  decode(n, Xf, Yf, XYf, m1, m2, m3);
-  const CompTab &t = (c == 0) ? t0 : (c == 1) ? t1 : (c == 2) ? t2 : t3;
-  const T k = (c == 0) ? pk : vk;
-  int lx = t.ax[0].lo[m1], ly = t.ax[1].lo[m2], lz = t.ax[2].lo[m3];
+  if (n >= (long)XYc * Zc) return;
+  int cx, cy, cz;
+  decode(n, Xc, Yc, XYc, cx, cy, cz);
+  if (withp) restrict_one<T, 0>(t0, Xf, Yf, XYf, cx, cy, cz, n, ff, fc, rf, ndf, ndc, bc, xc);
+  else { xc[4 * n] = T(0); bc[4 * n] = T(0); }
+  restrict_one<T, 1>(t1, Xf, Yf, XYf, cx, cy, cz, n, ff, fc, rf, ndf, ndc, bc, xc);
+  restrict_one<T, 2>(t2, Xf, Yf, XYf, cx, cy, cz, n, ff, fc, rf, ndf, ndc, bc, xc);
+  restrict_one<T, 3>(t3, Xf, Yf, XYf, cx, cy, cz, n, ff, fc, rf, ndf, ndc, bc, xc);
+}
+
+// vpprolong: fine x += sum coarse x * w * koef (parallel/move3mg.c:3535-3850),
+// one thread per fine node and its four unknowns
+template <typename T, int C>
+__device__ __forceinline__ void prolong_one(const CompTab &t, int m1, int m2, int m3, long n, int XYc, int Yc,
+                                            const unsigned char *__restrict__ ff, const T *__restrict__ xc, T *xf,
+                                            T k, int ynreset) {
+  if (kind_of(ff[n], C) == KIND_NONE) return;
+  const long u = 4 * n + C;
+  const int lx = t.ax[0].lo[m1], ly = t.ax[1].lo[m2], lz = t.ax[2].lo[m3];
   T v = ynreset ? T(0) : xf[u];
   for (int dz = 0; dz < 2; dz++) {
-    T wz = tw<T>(t.ax[2].w64, t.ax[2].w32, 2 * m3 + dz);
+    const T wz = tw<T>(t.ax[2].w64, t.ax[2].w32, 2 * m3 + dz);
     for (int dx = 0; dx < 2; dx++) {
-      T wx = tw<T>(t.ax[0].w64, t.ax[0].w32, 2 * m1 + dx);
+      const T wx = tw<T>(t.ax[0].w64, t.ax[0].w32, 2 * m1 + dx);
       for (int dy = 0; dy < 2; dy++) {
-        T wy = tw<T>(t.ax[1].w64, t.ax[1].w32, 2 * m2 + dy);
-        long nc = (long)(lz + dz) * XYc + (long)(lx + dx) * Yc + (ly + dy);
-        T swt = wx * wy * wz;
-        v += xc[4 * nc + c] * swt * k;
+        const T wy = tw<T>(t.ax[1].w64, t.ax[1].w32, 2 * m2 + dy);
+        const long nc = (long)(lz + dz) * XYc + (long)(lx + dx) * Yc + (ly + dy);
+        const T swt = wx * wy * wz;
+        v += xc[4 * nc + C] * swt * k;
       }
     }
   }
   xf[u] = v;
+}
+
+template <typename T>
+__global__ void k_prolong(int Xf, int Yf, int Zf, int Xc, int Yc, int Zc, CompTab t0, CompTab t1,
+                          CompTab t2, CompTab t3, const unsigned char *__restrict__ ff,
+                          const T *__restrict__ xc, T *xf, T pk, T vk, int withp, int ynreset) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XYc = Xc * Yc, XYf = Xf * Yf;
+  if (n >= (long)XYf * Zf) return;
+  int m1, m2, m3;
+  decode(n, Xf, Yf, XYf, m1, m2, m3);
+  if (withp) prolong_one<T, 0>(t0, m1, m2, m3, n, XYc, Yc, ff, xc, xf, pk, ynreset);
+  prolong_one<T, 1>(t1, m1, m2, m3, n, XYc, Yc, ff, xc, xf, vk, ynreset);
+  prolong_one<T, 2>(t2, m1, m2, m3, n, XYc, Yc, ff, xc, xf, vk, ynreset);
+  prolong_one<T, 3>(t3, m1, m2, m3, n, XYc, Yc, ff, xc, xf, vk, ynreset);
 }
 
 // ronurestrict: coarse nu/ro from fine nodes (parallel/move3mg.c:3387-3528),

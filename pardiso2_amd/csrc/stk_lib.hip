@@ -434,7 +434,7 @@ CompTab comptab(const Level &L, int comp) {
 template <typename T>
 void restrict_to(stk_ctx *c, int l, const T *rf, T *bcoarse, T *xcoarse, int withp) {
   Level &F = c->lv[l - 1], &C = c->lv[l];
-  hipLaunchKernelGGL(k_restrict<T>, dim3(blocks(4 * C.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, C.X, C.Y, C.Z,
+  hipLaunchKernelGGL(k_restrict<T>, dim3(blocks(C.n, 128)), dim3(128), 0, c->st, F.X, F.Y, F.Z, C.X, C.Y, C.Z,
                      comptab(C, 0), comptab(C, 1), comptab(C, 2), comptab(C, 3), F.flags, C.flags, rf,
                      c->nd + F.off, c->nd + C.off, bcoarse, xcoarse, withp);
 }
@@ -442,7 +442,7 @@ void restrict_to(stk_ctx *c, int l, const T *rf, T *bcoarse, T *xcoarse, int wit
 template <typename T>
 void prolong_from(stk_ctx *c, int l, const T *xcoarse, T *xf, int withp, int ynreset) {
   Level &F = c->lv[l - 1], &C = c->lv[l];
-  hipLaunchKernelGGL(k_prolong<T>, dim3(blocks(4 * F.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, C.X, C.Y, C.Z,
+  hipLaunchKernelGGL(k_prolong<T>, dim3(blocks(F.n, 128)), dim3(128), 0, c->st, F.X, F.Y, F.Z, C.X, C.Y, C.Z,
                      comptab(C, 0), comptab(C, 1), comptab(C, 2), comptab(C, 3), F.flags, xcoarse, xf,
                      (T)c->kn.p1koef, (T)c->kn.v1koef, withp, ynreset);
 }
@@ -1677,6 +1677,20 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
   return it;
 }
 
+// allocations a solve needs, and the preconditioner capture when replay is on
+static int solve_setup(stk_ctx *c, const stk_solver_opts *o) {
+  const bool fp64 = o->precision == 1;
+  int rc = ensure_vectors(c, !fp64, fp64);
+  if (rc) return rc;
+  rc = ensure_outer(c, o->restart + 1);
+  if (rc) return rc;
+  if (o->graph) {
+    const bool explicit_tile = o->tile[0] || o->tile[1] || o->tile[2];
+    rc = ensure_graph(c, o, explicit_tile ? o->tile : nullptr);
+  }
+  return rc;
+}
+
 int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   if (!c || !o) return STK_EINVAL;
   if (!c->prepared) return STK_ESTATE;
@@ -1696,15 +1710,8 @@ int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   if (explicit_tile && (o->tile[0] < 2 || o->tile[1] < 1 || o->tile[2] < 1 ||
                         (size_t)(o->tile[0] + 2) * (o->tile[1] + 2) * (o->tile[2] + 2) * 32 > 65536))
     return STK_EINVAL;
-  const bool fp64 = o->precision == 1;
-  int rc = ensure_vectors(c, !fp64, fp64);
+  int rc = solve_setup(c, o);
   if (rc) return rc;
-  rc = ensure_outer(c, o->restart + 1);
-  if (rc) return rc;
-  if (o->graph) {
-    rc = ensure_graph(c, o, tile);
-    if (rc) return rc;
-  }
   stk_stats local;
   stk_stats *S = st ? st : &local;
   memset(S, 0, sizeof(*S));
@@ -1894,6 +1901,17 @@ int stk_solve_batch(int n, stk_ctx *const *ctxs, const stk_solver_opts *opts, in
   if (threads <= 0) threads = 16;
   threads = std::min(threads, n);
   std::vector<int> rcs(n, STK_OK);
+  // graph replay: capture every context's preconditioner here, one at a time
+  // on this thread (prepared and with its buffers allocated), so the workers
+  // only replay
+  for (int i = 0; i < n; i++) {
+    const stk_solver_opts &oi = opts[nopts == 1 ? 0 : i];
+    if (!oi.graph) continue;
+    stk_ctx *c = ctxs[i];
+    int rc = c->prepared ? STK_OK : stk_prepare(c);
+    if (!rc) rc = solve_setup(c, &oi);
+    if (rc) return rc;
+  }
   std::atomic<int> next{0};
   auto work = [&]() {
     int dev = 0;
@@ -1904,8 +1922,7 @@ int stk_solve_batch(int n, stk_ctx *const *ctxs, const stk_solver_opts *opts, in
         rc = stk_prepare(ctxs[i]);
         if (!rc) rc = stk_hydrostatic_guess(ctxs[i]);
       }
-      stk_solver_opts oi = opts[nopts == 1 ? 0 : i];
-      oi.graph = 0;  // stream capture from concurrent host threads is not used
+      const stk_solver_opts &oi = opts[nopts == 1 ? 0 : i];
       rcs[i] = rc ? rc : stk_solve(ctxs[i], &oi, stats ? &stats[i] : nullptr);
     }
   };
