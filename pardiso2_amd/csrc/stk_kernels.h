@@ -9,6 +9,19 @@ template <typename T> struct V4;
 template <> struct V4<float> { using t = float4; };
 template <> struct V4<double> { using t = double4; };
 
+// Workgroups go to the 8 XCDs round-robin (block b on XCD b % 8), and each
+// XCD has its own L2.  xcd_block() renumbers blocks so that every XCD owns a
+// contiguous run of the index space (a slab of tiles or nodes): neighbouring
+// blocks then share halos and neighbour viscosities in the same L2.  The map
+// is a bijection of [0, gridDim.x).
+constexpr int kXcds = 8;
+__device__ __forceinline__ long xcd_block() {
+  const long nb = gridDim.x, b = blockIdx.x;
+  const long x = b % kXcds, j = b / kXcds;
+  const long q = nb / kXcds, r = nb % kXcds;
+  return x * q + (x < r ? x : r) + j;
+}
+
 __device__ __forceinline__ void decode(long n, int X, int Y, int XY, int &m1, int &m2, int &m3) {
   m3 = (int)(n / XY);
   int rem = (int)(n - (long)m3 * XY);
@@ -133,7 +146,7 @@ __device__ __forceinline__ void rowop_one(const Geo<T> &g, unsigned char f, long
 template <typename T, bool RES>
 __global__ void k_rowop(Geo<T> g, const unsigned char *__restrict__ flags, const T *__restrict__ x,
                         const T *__restrict__ b, T *__restrict__ y, int withp) {
-  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long n = xcd_block() * blockDim.x + threadIdx.x;
   if (n >= (long)g.XY * g.Z) return;
   int m1, m2, m3;
   decode(n, g.X, g.Y, g.XY, m1, m2, m3);
@@ -223,7 +236,7 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
   extern __shared__ unsigned char smem_raw[];
   T *sx = reinterpret_cast<T *>(smem_raw);
   const int hy = ty + 2, hx = tx + 2, hz = tz + 2;
-  const int bt = blockIdx.x;
+  const int bt = (int)xcd_block();
   const int by = bt % nty, bx = (bt / nty) % ntx, bz = bt / (nty * ntx);
   const int y0 = by * ty, x0 = bx * tx, z0 = bz * tz;
   const int nh = hy * hx * hz;
@@ -421,17 +434,26 @@ template <typename T> __device__ __forceinline__ T tw(const double *a, const flo
 template <> __device__ __forceinline__ double tw<double>(const double *a, const float *, long i) { return a[i]; }
 template <> __device__ __forceinline__ float tw<float>(const double *, const float *b, long i) { return b[i]; }
 
-// vprestrict: coarse b from fine residual r (parallel/move3mg.c:3966-4354).
-// One thread per coarse node does its four unknowns, each summed over its
-// fine contributors in the reference's order.
-template <typename T, int C>
-__device__ __forceinline__ void restrict_one(const CompTab &t, int Xf, int Yf, int XYf, int cx, int cy, int cz, long n,
-                                             const unsigned char *__restrict__ ff, const unsigned char *__restrict__ fc,
-                                             const T *__restrict__ rf, const float *__restrict__ ndf,
-                                             const float *__restrict__ ndc, T *bc, T *xc) {
-  const long u = 4 * n + C;
+// vprestrict: coarse b from fine residual r (parallel/move3mg.c:3966-4354);
+// one thread per coarse unknown, summed over its fine contributors in the
+// reference's order.  The four components of a node run on neighbouring
+// lanes through one code path (the component picks the axis tables).
+template <typename T>
+__global__ void k_restrict(int Xf, int Yf, int Zf, int Xc, int Yc, int Zc, CompTab t0, CompTab t1,
+                           CompTab t2, CompTab t3, const unsigned char *__restrict__ ff,
+                           const unsigned char *__restrict__ fc, const T *__restrict__ rf,
+                           const float *__restrict__ ndf, const float *__restrict__ ndc, T *bc, T *xc,
+                           int withp) {
+  const long u = xcd_block() * blockDim.x + threadIdx.x;
+  const long n = u >> 2;
+  const int XYc = Xc * Yc, XYf = Xf * Yf;
+  if (n >= (long)XYc * Zc) return;
+  const int c = (int)(u & 3);
   xc[u] = T(0);
-  if (kind_of(fc[n], C) != KIND_EQ) { bc[u] = T(0); return; }
+  if (kind_of(fc[n], c) != KIND_EQ || (!withp && c == 0)) { bc[u] = T(0); return; }
+  int cx, cy, cz;
+  decode(n, Xc, Yc, XYc, cx, cy, cz);
+  const CompTab &t = (c == 0) ? t0 : (c == 1) ? t1 : (c == 2) ? t2 : t3;
   T acc = T(0), ws = T(0);
   for (int a = t.ax[2].cbeg[cz]; a < t.ax[2].cbeg[cz + 1]; a++) {
     const int k = t.ax[2].cfi[a];
@@ -442,36 +464,18 @@ __device__ __forceinline__ void restrict_one(const CompTab &t, int Xf, int Yf, i
       for (int e = t.ax[1].cbeg[cy]; e < t.ax[1].cbeg[cy + 1]; e++) {
         const int j = t.ax[1].cfi[e];
         const long nf = (long)k * XYf + (long)i * Yf + j;
-        if (kind_of(ff[nf], C) != KIND_EQ) continue;
+        if (kind_of(ff[nf], c) != KIND_EQ) continue;
         const T wy = tw<T>(t.ax[1].cw64, t.ax[1].cw32, e);
         const T swt = wx * wy * wz;
         T iv;
-        if (C == 0) iv = rf[4 * nf] * (T)ndf[nf] / (T)ndc[n] * swt;
-        else iv = rf[4 * nf + C] * swt;
+        if (c == 0) iv = rf[4 * nf] * (T)ndf[nf] / (T)ndc[n] * swt;
+        else iv = rf[4 * nf + c] * swt;
         acc += iv;
         ws += swt;
       }
     }
   }
   bc[u] = (ws != T(0)) ? acc / ws : acc;
-}
-
-template <typename T>
-__global__ void k_restrict(int Xf, int Yf, int Zf, int Xc, int Yc, int Zc, CompTab t0, CompTab t1,
-                           CompTab t2, CompTab t3, const unsigned char *__restrict__ ff,
-                           const unsigned char *__restrict__ fc, const T *__restrict__ rf,
-                           const float *__restrict__ ndf, const float *__restrict__ ndc, T *bc, T *xc,
-                           int withp) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const int XYc = Xc * Yc, XYf = Xf * Yf;
-  if (n >= (long)XYc * Zc) return;
-  int cx, cy, cz;
-  decode(n, Xc, Yc, XYc, cx, cy, cz);
-  if (withp) restrict_one<T, 0>(t0, Xf, Yf, XYf, cx, cy, cz, n, ff, fc, rf, ndf, ndc, bc, xc);
-  else { xc[4 * n] = T(0); bc[4 * n] = T(0); }
-  restrict_one<T, 1>(t1, Xf, Yf, XYf, cx, cy, cz, n, ff, fc, rf, ndf, ndc, bc, xc);
-  restrict_one<T, 2>(t2, Xf, Yf, XYf, cx, cy, cz, n, ff, fc, rf, ndf, ndc, bc, xc);
-  restrict_one<T, 3>(t3, Xf, Yf, XYf, cx, cy, cz, n, ff, fc, rf, ndf, ndc, bc, xc);
 }
 
 // vpprolong: fine x += sum coarse x * w * koef (parallel/move3mg.c:3535-3850),
@@ -503,7 +507,7 @@ template <typename T>
 __global__ void k_prolong(int Xf, int Yf, int Zf, int Xc, int Yc, int Zc, CompTab t0, CompTab t1,
                           CompTab t2, CompTab t3, const unsigned char *__restrict__ ff,
                           const T *__restrict__ xc, T *xf, T pk, T vk, int withp, int ynreset) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long n = xcd_block() * blockDim.x + threadIdx.x;
   const int XYc = Xc * Yc, XYf = Xf * Yf;
   if (n >= (long)XYf * Zf) return;
   int m1, m2, m3;
@@ -721,7 +725,7 @@ __global__ void k_pscale(long n4, const unsigned char *__restrict__ flags, const
 template <typename T>
 __global__ void k_vrhs(Geo<double> g, const unsigned char *__restrict__ flags, const double *__restrict__ r,
                        const double *__restrict__ zp, T *bv) {
-  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long u = xcd_block() * blockDim.x + threadIdx.x;
   long n = u >> 2;
   if (n >= (long)g.XY * g.Z) return;
   int c = (int)(u & 3);

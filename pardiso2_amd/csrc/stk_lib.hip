@@ -163,6 +163,7 @@ struct stk_ctx {
   double *red = nullptr, *scal = nullptr;
   int nred = 1024;
   double *kred = nullptr, *hcol = nullptr;  // flexible GMRES: per-block dots, Hessenberg column / update coefficients
+  double *hpin = nullptr;  // pinned host staging of the per-step scalars (2 kMaxDirs + 8 doubles)
   // preconditioner captured as a HIP graph: z = M^-1 gin written to gout
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -434,7 +435,7 @@ CompTab comptab(const Level &L, int comp) {
 template <typename T>
 void restrict_to(stk_ctx *c, int l, const T *rf, T *bcoarse, T *xcoarse, int withp) {
   Level &F = c->lv[l - 1], &C = c->lv[l];
-  hipLaunchKernelGGL(k_restrict<T>, dim3(blocks(C.n, 128)), dim3(128), 0, c->st, F.X, F.Y, F.Z, C.X, C.Y, C.Z,
+  hipLaunchKernelGGL(k_restrict<T>, dim3(blocks(4 * C.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, C.X, C.Y, C.Z,
                      comptab(C, 0), comptab(C, 1), comptab(C, 2), comptab(C, 3), F.flags, C.flags, rf,
                      c->nd + F.off, c->nd + C.off, bcoarse, xcoarse, withp);
 }
@@ -624,8 +625,9 @@ int blocks_norms(stk_ctx *c, const double *r, const double *b, double out4[4]) {
   long n4 = 4 * c->lv[0].n;
   hipLaunchKernelGGL(k_blocks, dim3(c->nred), dim3(256), 0, c->st, n4, c->lv[0].flags, c->W2, r, b, c->red);
   hipLaunchKernelGGL(k_reduce_rows, dim3(4), dim3(256), 0, c->st, c->red, c->nred, 4, c->scal + 8);
-  HIPCHK(hipMemcpyAsync(out4, c->scal + 8, 4 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(c->hpin, c->scal + 8, 4 * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
+  memcpy(out4, c->hpin, 4 * sizeof(double));
   return STK_OK;
 }
 
@@ -639,7 +641,9 @@ int ensure_outer(stk_ctx *c, int m) {
   if (!c->X) {
     if (dalloc(&c->X, n4) || dalloc(&c->B, n4) || dalloc(&c->R, n4) || dalloc(&c->W2, n4) || dalloc(&c->T1, n4) ||
         dalloc(&c->red, (size_t)c->nred * 16) || dalloc(&c->scal, 256) ||
-        dalloc(&c->kred, (size_t)c->nred * (2 * kMaxDirs + 2)) || dalloc(&c->hcol, 2 * kMaxDirs + 2))
+        dalloc(&c->kred, (size_t)4 * c->nred * (2 * kMaxDirs + 2)) || dalloc(&c->hcol, 2 * kMaxDirs + 2))
+      return STK_ENOMEM;
+    if (hipHostMalloc((void **)&c->hpin, (2 * kMaxDirs + 8) * sizeof(double), hipHostMallocDefault) != hipSuccess)
       return STK_ENOMEM;
     hipMemsetAsync(c->X, 0, n4 * 8, c->st);
   }
@@ -931,6 +935,7 @@ void stk_destroy(stk_ctx *c) {
   dfree(c->ainv64); dfree(c->ainvP64); dfree(c->ainv32); dfree(c->cact);
   dfree(c->X); dfree(c->B); dfree(c->R); dfree(c->W2); dfree(c->T1); dfree(c->red); dfree(c->scal);
   dfree(c->kred); dfree(c->hcol); dfree(c->gin); dfree(c->gout);
+  if (c->hpin) hipHostFree(c->hpin);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
   if (c->graph) hipGraphDestroy(c->graph);
   for (auto p : c->Zs) dfree(p);
@@ -1568,9 +1573,9 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
   const long n4 = 4 * c->lv[0].n;
   const int m = o->restart;
   const bool fp64 = o->precision == 1;
-  // dots: one block per 2048-element slice up to nred blocks; updates: up to
-  // 4 nred blocks (the norm partials need one double per block)
-  const unsigned nbk = (unsigned)std::min<long>(c->nred, std::max<long>(1, (n4 / 2 + 2047) / 2048));
+  // dots: one block per 2048-element slice up to 4 nred blocks (partials of
+  // 2j+2 sums per block); updates: up to 4 nred blocks
+  const unsigned nbk = (unsigned)std::min<long>(4L * c->nred, std::max<long>(1, (n4 / 2 + 2047) / 2048));
   const unsigned nbu = (unsigned)std::min<long>(4L * c->nred, (long)blocks(n4 / 2, 256));
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), hh(2 * m + 2), y(m), coef(2 * m + 2);
   auto h = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
@@ -1608,8 +1613,9 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
       for (int i = 0; i < j; i++) P.p[i] = c->Qs[i];
       hipLaunchKernelGGL(k_dots2, dim3(nbk), dim3(256), 0, c->st, n4, c->W2, u, W, P, j, c->kred);
       hipLaunchKernelGGL(k_reduce_rows, dim3(2 * j + 2), dim3(256), 0, c->st, c->kred, (int)nbk, 2 * j + 2, c->hcol);
-      HIPCHK(hipMemcpyAsync(hh.data(), c->hcol, (2 * j + 2) * sizeof(double), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipMemcpyAsync(c->hpin, c->hcol, (2 * j + 2) * sizeof(double), hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
+      memcpy(hh.data(), c->hpin, (2 * j + 2) * sizeof(double));
       const double *a = hh.data(), *b = hh.data() + j;
       const double cuw = hh[2 * j], cuu = hh[2 * j + 1];
       double asq = 0.0, ab = 0.0;
@@ -1632,12 +1638,14 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
       for (int i = 0; i < j; i++) { coef[i] = a[i]; coef[j + i] = b[i]; }
       coef[2 * j] = hjj;
       coef[2 * j + 1] = 1.0 / nu;
-      HIPCHK(hipMemcpyAsync(c->hcol, coef.data(), (2 * j + 2) * sizeof(double), hipMemcpyHostToDevice, c->st));
+      memcpy(c->hpin, coef.data(), (2 * j + 2) * sizeof(double));  // copied before the stream reuses hpin
+      HIPCHK(hipMemcpyAsync(c->hcol, c->hpin, (2 * j + 2) * sizeof(double), hipMemcpyHostToDevice, c->st));
       hipLaunchKernelGGL(k_update2, dim3(nbu), dim3(256), 0, c->st, n4, j, c->hcol, P, u, W, c->W2, c->kred);
       hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, c->st, c->kred, (int)nbu, 1, c->hcol + 2 * m + 1);
-      double hn2 = 0.0;
-      HIPCHK(hipMemcpyAsync(&hn2, c->hcol + 2 * m + 1, sizeof(double), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipMemcpyAsync(c->hpin + 2 * kMaxDirs + 4, c->hcol + 2 * m + 1, sizeof(double), hipMemcpyDeviceToHost,
+                            c->st));
       HIPCHK(hipStreamSynchronize(c->st));
+      const double hn2 = c->hpin[2 * kMaxDirs + 4];
       const double hn = std::sqrt(std::max(hn2, 0.0));
       if (!std::isfinite(hn)) { status = STK_ENAN; break; }
       for (int i = 0; i < j; i++) h(i, j) = b[i];
@@ -1662,7 +1670,9 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
       y[i] = acc / h(i, i);
     }
     for (int i = 0; i < k; i++) y[i] = -y[i];
-    HIPCHK(hipMemcpyAsync(c->hcol, y.data(), k * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));  // the staging buffer may still feed the last coefficient copy
+    memcpy(c->hpin, y.data(), k * sizeof(double));
+    HIPCHK(hipMemcpyAsync(c->hcol, c->hpin, k * sizeof(double), hipMemcpyHostToDevice, c->st));
     PtrsK P{};
     for (int i = 0; i < k; i++) P.p[i] = c->Zs[i];
     hipLaunchKernelGGL(k_maxpy_norm, dim3(nbu), dim3(256), 0, c->st, n4, k, c->hcol, P, c->X, c->X, c->W2, c->kred);
@@ -1748,8 +1758,9 @@ int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
     hipLaunchKernelGGL(k_scale2, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, c->scal + 2, Q, Z);
     hipLaunchKernelGGL(k_update, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, c->scal + 3, Z, Q, c->X, c->R);
     multidot(c, c->R, {c->R}, c->scal + 4);
-    HIPCHK(hipMemcpyAsync(hr, c->scal + 4, 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(c->hpin, c->scal + 4, 8, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    hr[0] = c->hpin[0];
     double nrel = std::sqrt(std::max(hr[0], 0.0)) / nb;
     it++;
     k++;
@@ -1758,8 +1769,9 @@ int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
       k = 0;
       true_residual(c);
       multidot(c, c->R, {c->R}, c->scal + 4);
-      HIPCHK(hipMemcpyAsync(hr, c->scal + 4, 8, hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipMemcpyAsync(c->hpin, c->scal + 4, 8, hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
+      hr[0] = c->hpin[0];
       nrel = std::sqrt(hr[0]) / nb;
     }
     rel = nrel;

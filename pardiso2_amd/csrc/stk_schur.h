@@ -73,7 +73,7 @@ __global__ void k_bf_kop(Geo<double> g, const unsigned char *__restrict__ flags,
 template <typename T>
 __global__ void k_bf_grad(int X, int Y, int Z, const T *__restrict__ iBx, const T *__restrict__ iBy,
                           const T *__restrict__ iBz, const T *__restrict__ dinv, const T *__restrict__ p, T *y) {
-  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long n = xcd_block() * blockDim.x + threadIdx.x;
   const int XY = X * Y;
   if (n >= (long)XY * Z) return;
   int m1, m2, m3;
@@ -96,7 +96,7 @@ template <typename T>
 __global__ void k_bf_div(int X, int Y, int Z, const unsigned char *__restrict__ flags, const T *__restrict__ iAx,
                          const T *__restrict__ iAy, const T *__restrict__ iAz, const T *__restrict__ dinv,
                          const T *__restrict__ w, T *t) {
-  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long n = xcd_block() * blockDim.x + threadIdx.x;
   const int XY = X * Y;
   if (n >= (long)XY * Z) return;
   if (kind_of(flags[n], 0) != KIND_EQ) { t[n] = T(0); return; }
@@ -173,7 +173,7 @@ __global__ void k_kagg(int Xf, int Yf, int Xc, int Yc, int Zc, KAgg t, KPlanesC<
 // one red/black Gauss-Seidel half-sweep of K x = b on cells of one colour
 template <typename T>
 __global__ void k_kgs(int X, int Y, int Z, KPlanesC<T> k, const T *__restrict__ b, T *x, int colour) {
-  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long n = xcd_block() * blockDim.x + threadIdx.x;
   const int XY = X * Y;
   if (n >= (long)XY * Z) return;
   const T d = k.p[0][n];
@@ -194,7 +194,7 @@ __global__ void k_kgs(int X, int Y, int Z, KPlanesC<T> k, const T *__restrict__ 
 // r = b - K x
 template <typename T>
 __global__ void k_kres(int X, int Y, int Z, KPlanesC<T> k, const T *__restrict__ b, const T *__restrict__ x, T *r) {
-  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long n = xcd_block() * blockDim.x + threadIdx.x;
   const int XY = X * Y;
   if (n >= (long)XY * Z) return;
   const T d = k.p[0][n];
@@ -234,7 +234,7 @@ template <typename T>
 __global__ void k_kprolong(int Xf, int Yf, int Zf, int Xc, int Yc, const int *__restrict__ px,
                            const int *__restrict__ py, const int *__restrict__ pz, const T *__restrict__ kdiag,
                            const T *__restrict__ xc, T *xf, T scale) {
-  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long n = xcd_block() * blockDim.x + threadIdx.x;
   const int XYf = Xf * Yf;
   if (n >= (long)XYf * Zf) return;
   if (kdiag[n] == T(0)) return;
