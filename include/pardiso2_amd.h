@@ -153,6 +153,11 @@ int stk_apply(stk_ctx *ctx, int level, const double *x, double *y);
  * kernel the V-cycle uses on levels of at most 2048 nodes in FP64); tz < 0
  * selects the tile (ty, tx, -tz) with its viscosities staged in LDS. */
 int stk_smooth(stk_ctx *ctx, int level, int sweeps, int ty, int tx, int tz);
+/* The same in FP32 arithmetic (precision 0: the mixed-precision V-cycle's
+ * smoother, on FP32 roundings of sol0/fre1; sol0 is widened back and fre0 is
+ * the FP64 residual) or FP64 (precision 1, = stk_smooth).  Same reference
+ * routine: gausmat4, parallel/move3mg.c:3855-3960. */
+int stk_smooth_ex(stk_ctx *ctx, int level, int sweeps, int ty, int tx, int tz, int precision);
 int stk_coarse_solve(stk_ctx *ctx);
 
 /* ---- smoother tile autotuning ------------------------------------------ */

@@ -54,6 +54,7 @@ def lib():
         L.orc_gausmat4.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_residual.argtypes = [C.c_void_p, C.c_int]
         L.orc_sweep_tiles.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_eq_box.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_bc_pass.argtypes = [C.c_void_p, C.c_int, dp]
         L.orc_vprestrict.argtypes = [C.c_void_p, C.c_int]
         L.orc_vpprolong.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -180,6 +181,12 @@ class Oracle:
 
     def sweep_tiles(self, am, l, ty=64, tx=4, tz=4):
         lib().orc_sweep_tiles(self.h, am, l, ty, tx, tz)
+
+    def eq_box(self, l):
+        """(y0, x0, z0, y1, x1, z1) of the nodes with equation rows on level l."""
+        b = np.zeros(6, np.int32)
+        lib().orc_eq_box(self.h, l, b.ctypes.data)
+        return tuple(int(v) for v in b)
 
     def vprestrict(self, l):
         lib().orc_vprestrict(self.h, l)

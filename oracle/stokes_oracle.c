@@ -954,7 +954,8 @@ void orc_gausmat4(orc_t *o, int am, int l) {
 
 /* ------------------------------------------------------------------------- */
 /* Coloured tile-hybrid sweep: the same gausmat4 row updates in the order the */
-/* GPU smoother uses.  Nodes are grouped in ty x tx x tz tiles (y fastest).   */
+/* GPU smoother uses.  Nodes are grouped in ty x tx x tz tiles (y fastest)   */
+/* laid from the lower corner of the equation-row box (orc_eq_box).          */
 /* Inside a tile the phases are vx(red, black), vy(red, black), vz(red,       */
 /* black), P, colour = (m1+m2+m3)&1; unknowns of other tiles are read from    */
 /* the state at the start of the sweep (block-Jacobi between tiles).          */
@@ -1020,17 +1021,43 @@ void orc_bc_pass(orc_t *o, int l, double *x) {
       }
 }
 
+/* Bounding box {y0, x0, z0, y1, x1, z1} (upper bounds exclusive) of the     */
+/* nodes that hold an equation row (a row that is not a boundary equation);  */
+/* the tiles start at its lower corner.  The whole level if there is none.   */
+void orc_eq_box(const orc_t *o, int l, int box[6]) {
+  const int X = o->mgx[l], Y = o->mgy[l], Z = o->mgz[l];
+  int b[6] = {Y, X, Z, 0, 0, 0};
+  for (int m3 = 0; m3 < Z; m3++)
+    for (int m1 = 0; m1 < X; m1++)
+      for (int m2 = 0; m2 < Y; m2++) {
+        long r0 = NID(o, l, m1, m2, m3) * 4;
+        int eq = 0;
+        for (int c = 0; c < 4; c++) eq |= o->num0[r0 + c] && !o->bondm[r0 + c];
+        if (!eq) continue;
+        const int m[3] = {m2, m1, m3};
+        for (int k = 0; k < 3; k++) {
+          if (m[k] < b[k]) b[k] = m[k];
+          if (m[k] + 1 > b[k + 3]) b[k + 3] = m[k] + 1;
+        }
+      }
+  if (b[3] == 0) { b[0] = b[1] = b[2] = 0; b[3] = Y; b[4] = X; b[5] = Z; }
+  for (int k = 0; k < 6; k++) box[k] = b[k];
+}
+
 void orc_sweep_tiles(orc_t *o, int am, int l, int ty, int tx, int tz) {
   const int X = o->mgx[l], Y = o->mgy[l], Z = o->mgz[l];
   long a = o->mgp[l] * 4, e = (o->mgp[l] + o->mgn[l]) * 4;
   double *old = o->tmp;
   double *cur = o->sol0;
   static const int comp[4] = {1, 2, 3, 0};
+  int box[6];
+  orc_eq_box(o, l, box);
   for (int s = 0; s < am; s++) {
     memcpy(old + a, cur + a, sizeof(double) * (e - a));
     /* tiles only read their own nodes in cur and the rest in old, so they are
        independent within a sweep and may run in any order or in parallel */
-    const int ntz = (Z + tz - 1) / tz, ntx = (X + tx - 1) / tx, nty = (Y + ty - 1) / ty;
+    const int ntz = (box[5] - box[2] + tz - 1) / tz, ntx = (box[4] - box[1] + tx - 1) / tx,
+              nty = (box[3] - box[0] + ty - 1) / ty;
 #pragma omp parallel for collapse(3) schedule(static)
     for (int kz = 0; kz < ntz; kz++)
       for (int kx = 0; kx < ntx; kx++)
@@ -1038,7 +1065,7 @@ void orc_sweep_tiles(orc_t *o, int am, int l, int ty, int tx, int tz) {
           for (int ph = 0; ph < (o->vonly ? 6 : 7); ph++) {
             tilectx_t t;
             t.mgp = o->mgp[l]; t.Y = Y; t.XY = o->mgxy[l]; t.ty = ty; t.tx = tx; t.tz = tz;
-            t.z0 = kz * tz; t.x0 = kx * tx; t.y0 = ky * ty;
+            t.z0 = box[2] + kz * tz; t.x0 = box[1] + kx * tx; t.y0 = box[0] + ky * ty;
             int c = comp[ph / 2];
             int colour = ph & 1;
             for (int m3 = t.z0; m3 < t.z0 + tz && m3 < Z; m3++)

@@ -93,6 +93,7 @@ int orc_get_row(const orc_t *o, long r, int *idx, double *val, double *rhs);
 void orc_gausmat4(orc_t *o, int am, int l);
 void orc_residual(orc_t *o, int l);
 /* coloured tile-hybrid sweep of the same row updates (the GPU ordering) */
+void orc_eq_box(const orc_t *o, int l, int box[6]);
 void orc_sweep_tiles(orc_t *o, int am, int l, int ty, int tx, int tz);
 void orc_bc_pass(orc_t *o, int l, double *x);
 

@@ -187,21 +187,64 @@ __global__ void k_bc_rowop(int nbc, const int *__restrict__ br, const int *__res
 // (m1+m2+m3)&1; halo values stay those of xold (block-Jacobi across tiles),
 // so the sweep is deterministic.  Boundary rows are skipped here and done
 // by k_bc_update afterwards.  The tile is written to xnew.
+// Division by a small runtime divisor d (2 <= d < 2^16) for numerators
+// i < 2^16: i / d = umulhi(i, ceil(2^32 / d)).  With m = (2^32 + e) / d,
+// 0 <= e < d, the product exceeds i/d by i e / (d 2^32) < 1/d, less than the
+// distance from i/d to the next integer, so the floor is exact.  d == 1 is
+// passed through.
+struct FDiv {
+  unsigned d, m;
+};
+__host__ __device__ inline FDiv make_fdiv(unsigned d) {
+  return FDiv{d, d > 1 ? (unsigned)((0x100000000ull + d - 1) / d) : 0u};
+}
+__device__ __forceinline__ unsigned fdiv(unsigned i, FDiv f) { return f.d == 1 ? i : __umulhi(i, f.m); }
+
+// Velocity update residual / diagonal: FP64 divides as the reference does;
+// the FP32 smoother multiplies by the hardware reciprocal (1 ulp).
+__device__ __forceinline__ double diag_step(double r, double d) { return r / d; }
+__device__ __forceinline__ float diag_step(float r, float d) { return r * __builtin_amdgcn_rcpf(d); }
+
+// Smoother tile of ty*tx*tz nodes (y fastest) with its one-node halo
+// (hy = ty+2, hx = tx+2).  The ntx*nty*ntz tiles cover the box of nodes that
+// hold equation rows, from its lower corner (oy, ox, oz): nodes outside it
+// carry only boundary rows (set by the boundary chains after each sweep) or
+// no unknowns.
+struct TileGeo {
+  int ty, tx, tz, ntx, nty, ntz, oy, ox, oz;
+  FDiv dty, dtx, dhy, dhx, dhty;  // ty, tx, ty+2, tx+2, ceil(ty/2)
+};
+// box = {y0, x0, z0, y1, x1, z1}, upper bounds exclusive
+inline TileGeo make_tile(int ty, int tx, int tz, const int box[6]) {
+  TileGeo t;
+  t.ty = ty; t.tx = tx; t.tz = tz;
+  t.oy = box[0]; t.ox = box[1]; t.oz = box[2];
+  t.nty = (box[3] - box[0] + ty - 1) / ty; t.ntx = (box[4] - box[1] + tx - 1) / tx;
+  t.ntz = (box[5] - box[2] + tz - 1) / tz;
+  t.dty = make_fdiv(ty); t.dtx = make_fdiv(tx); t.dhy = make_fdiv(ty + 2); t.dhx = make_fdiv(tx + 2);
+  t.dhty = make_fdiv((ty + 1) / 2);
+  return t;
+}
+
 // ---------------------------------------------------------------------------
 // one phase of the tile smoother for component C (colour for velocities)
 template <typename T, int C>
 __device__ __forceinline__ void smooth_phase(const Geo<T> &g, const Geo<T> &gv, const unsigned char *__restrict__ flags,
-                                             const T *__restrict__ b, T *sx, int nh, int ty, int tx, int tz,
+                                             const T *__restrict__ b, T *sx, int nh, const TileGeo &tg,
                                              int x0, int y0, int z0, int colour, T v0koef, T p0koef) {
+  const int ty = tg.ty, tx = tg.tx, tz = tg.tz;
   const int hy = ty + 2, hx = tx + 2;
   const int hty = (ty + 1) / 2;
   const int ncand = (C == 0) ? ty * tx * tz : hty * tx * tz;
   for (int i = threadIdx.x; i < ncand; i += blockDim.x) {
     int ly, lx, lz;
     if (C == 0) {
-      ly = i % ty; lx = (i / ty) % tx; lz = i / (ty * tx);
+      const int q = (int)fdiv(i, tg.dty);
+      ly = i - q * ty; lz = (int)fdiv(q, tg.dtx); lx = q - lz * tx;
     } else {
-      int kk = i % hty; lx = (i / hty) % tx; lz = i / (hty * tx);
+      const int q = (int)fdiv(i, tg.dhty);
+      const int kk = i - q * hty;
+      lz = (int)fdiv(q, tg.dtx); lx = q - lz * tx;
       int par = (colour - (x0 + lx) - (z0 + lz) - y0) & 1;
       ly = 2 * kk + par;
       if (ly >= ty) continue;
@@ -217,10 +260,10 @@ __device__ __forceinline__ void smooth_phase(const Geo<T> &g, const Geo<T> &gv, 
 #pragma unroll
     for (int q = 1; q <= Slot<C>::ns; q++) {
       const int li = base + (Slot<C>::d[q][2] * hx + Slot<C>::d[q][0]) * hy + Slot<C>::d[q][1];
-      ival += s[q] * sx[Slot<C>::d[q][3] * nh + li];
+      ival = mac(ival, s[q], sx[Slot<C>::d[q][3] * nh + li]);
     }
     if (C == 0) sx[base] -= ival * g.ec[n] * p0koef;
-    else sx[C * nh + base] -= ival / s[1] * v0koef;
+    else sx[C * nh + base] -= diag_step(ival, s[1]) * v0koef;
   }
 }
 
@@ -231,14 +274,15 @@ __device__ __forceinline__ void smooth_phase(const Geo<T> &g, const Geo<T> &gv, 
 template <typename T, bool SV>
 __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *__restrict__ flags,
                                                 const T *__restrict__ xold, T *__restrict__ xnew,
-                                                const T *__restrict__ b, int ty, int tx, int tz,
-                                                int ntx, int nty, int withp, T v0koef, T p0koef) {
+                                                const T *__restrict__ b, TileGeo tg, int withp, T v0koef,
+                                                T p0koef) {
   extern __shared__ unsigned char smem_raw[];
   T *sx = reinterpret_cast<T *>(smem_raw);
+  const int ty = tg.ty, tx = tg.tx, tz = tg.tz, ntx = tg.ntx, nty = tg.nty;
   const int hy = ty + 2, hx = tx + 2, hz = tz + 2;
   const int bt = (int)xcd_block();
   const int by = bt % nty, bx = (bt / nty) % ntx, bz = bt / (nty * ntx);
-  const int y0 = by * ty, x0 = bx * tx, z0 = bz * tz;
+  const int y0 = tg.oy + by * ty, x0 = tg.ox + bx * tx, z0 = tg.oz + bz * tz;
   const int nh = hy * hx * hz;
   Geo<T> gv = g;
   if (SV) {
@@ -264,7 +308,8 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
   }
   // stage tile + halo
   for (int i = threadIdx.x; i < nh; i += blockDim.x) {
-    int ly = i % hy, lx = (i / hy) % hx, lz = i / (hy * hx);
+    const int q = (int)fdiv(i, tg.dhy);
+    const int ly = i - q * hy, lz = (int)fdiv(q, tg.dhx), lx = q - lz * hx;
     int m2 = y0 + ly - 1, m1 = x0 + lx - 1, m3 = z0 + lz - 1;
     T v0 = T(0), v1 = T(0), v2 = T(0), v3 = T(0);
     if (m1 >= 0 && m1 < g.X && m2 >= 0 && m2 < g.Y && m3 >= 0 && m3 < g.Z) {
@@ -277,16 +322,17 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
   const int nphase = withp ? 7 : 6;
   for (int ph = 0; ph < nphase; ph++) {
     const int colour = ph & 1;
-    if (ph < 2) smooth_phase<T, 1>(g, gv, flags, b, sx, nh, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
-    else if (ph < 4) smooth_phase<T, 2>(g, gv, flags, b, sx, nh, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
-    else if (ph < 6) smooth_phase<T, 3>(g, gv, flags, b, sx, nh, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
-    else smooth_phase<T, 0>(g, gv, flags, b, sx, nh, ty, tx, tz, x0, y0, z0, colour, v0koef, p0koef);
+    if (ph < 2) smooth_phase<T, 1>(g, gv, flags, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
+    else if (ph < 4) smooth_phase<T, 2>(g, gv, flags, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
+    else if (ph < 6) smooth_phase<T, 3>(g, gv, flags, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
+    else smooth_phase<T, 0>(g, gv, flags, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
     __syncthreads();
   }
   // write the tile interior
   const int nt = ty * tx * tz;
   for (int i = threadIdx.x; i < nt; i += blockDim.x) {
-    int ly = i % ty, lx = (i / ty) % tx, lz = i / (ty * tx);
+    const int q = (int)fdiv(i, tg.dty);
+    const int ly = i - q * ty, lz = (int)fdiv(q, tg.dtx), lx = q - lz * tx;
     int m1 = x0 + lx, m2 = y0 + ly, m3 = z0 + lz;
     if (m1 >= g.X || m2 >= g.Y || m3 >= g.Z) continue;
     int li = ((lz + 1) * hx + (lx + 1)) * hy + (ly + 1);
@@ -353,10 +399,10 @@ __device__ __forceinline__ void level_phase(const Geo<T> &g, const unsigned char
 #pragma unroll
     for (int q = 1; q <= Slot<C>::ns; q++) {
       const long nb = n + (long)Slot<C>::d[q][2] * g.XY + (long)Slot<C>::d[q][0] * g.Y + Slot<C>::d[q][1];
-      ival += s[q] * sx[Slot<C>::d[q][3] * N + nb];
+      ival = mac(ival, s[q], sx[Slot<C>::d[q][3] * N + nb]);
     }
     if (C == 0) sx[n] -= ival * g.ec[n] * p0koef;
-    else sx[C * N + n] -= ival / s[1] * v0koef;
+    else sx[C * N + n] -= diag_step(ival, s[1]) * v0koef;
   }
 }
 
@@ -956,6 +1002,12 @@ __global__ void k_fill(long n, T v, T *x) {
 }
 
 // y = s * x
+template <typename A, typename B>
+__global__ void k_cast(long n, const A *__restrict__ x, B *__restrict__ y) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u < n) y[u] = (B)x[u];
+}
+
 __global__ void k_scale_to(long n4, double s, const double *__restrict__ x, double *y) {
   long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (u < n4) y[u] = s * x[u];

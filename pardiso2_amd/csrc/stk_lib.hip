@@ -82,6 +82,8 @@ struct Level {
   float *e32[5] = {};
   unsigned char *flags = nullptr;
   std::vector<unsigned char> hflags;
+  // bounding box of the nodes with equation rows {y0, x0, z0, y1, x1, z1}: the smoother's tiles cover it
+  int box[6] = {0, 0, 0, 0, 0, 0};
   // boundary rows, sorted by dependency depth
   int nbc = 0;
   std::vector<int> bcoff;
@@ -399,17 +401,17 @@ void smooth(stk_ctx *c, int l, int sweeps, const int *tile, int withp, bool whol
     return;
   }
   int ty = tile[0], tx = tile[1], tz = tile[2];
-  int nty = (L.Y + ty - 1) / ty, ntx = (L.X + tx - 1) / tx, ntz = (L.Z + tz - 1) / tz;
+  const TileGeo tg = make_tile(ty, tx, tz, L.box);
   const size_t lds = smooth_lds<T>(tile, false), ldsv = smooth_lds<T>(tile, true);
   const bool sv = tuned && tile[3] && ldsv <= kLevelLds;
   Geo<T> g = L.geo<T>(c->kn.p2koef);
   for (int s = 0; s < sweeps; s++) {
     if (sv)
-      hipLaunchKernelGGL((k_smooth<T, true>), dim3(nty * ntx * ntz), dim3(256), ldsv, c->st, g, L.flags, lvx<T>(L),
-                         lvxo<T>(L), lvb<T>(L), ty, tx, tz, ntx, nty, withp, (T)c->kn.v0koef, (T)c->kn.p0koef);
+      hipLaunchKernelGGL((k_smooth<T, true>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), ldsv, c->st, g, L.flags, lvx<T>(L),
+                         lvxo<T>(L), lvb<T>(L), tg, withp, (T)c->kn.v0koef, (T)c->kn.p0koef);
     else
-      hipLaunchKernelGGL((k_smooth<T, false>), dim3(nty * ntx * ntz), dim3(256), lds, c->st, g, L.flags, lvx<T>(L),
-                         lvxo<T>(L), lvb<T>(L), ty, tx, tz, ntx, nty, withp, (T)c->kn.v0koef, (T)c->kn.p0koef);
+      hipLaunchKernelGGL((k_smooth<T, false>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), lds, c->st, g, L.flags, lvx<T>(L),
+                         lvxo<T>(L), lvb<T>(L), tg, withp, (T)c->kn.v0koef, (T)c->kn.p0koef);
     lvflip<T>(L);
     bc_passes<T>(c, L, lvb<T>(L), lvx<T>(L), withp ? 0 : 1);
   }
@@ -752,18 +754,18 @@ template <typename T>
 static double time_smooth(stk_ctx *c, int l, const int *tile, int reps) {
   Level &L = c->lv[l];
   const int ty = tile[0], tx = tile[1], tz = tile[2];
-  const int nty = (L.Y + ty - 1) / ty, ntx = (L.X + tx - 1) / tx, ntz = (L.Z + tz - 1) / tz;
+  const TileGeo tg = make_tile(ty, tx, tz, L.box);
   const bool sv = tile[3] && smooth_lds<T>(tile, true) <= kLevelLds;
   const size_t lds = smooth_lds<T>(tile, sv);
   Geo<T> g = L.geo<T>(c->kn.p2koef);
   T *x0 = lvx<T>(L), *x1 = lvxo<T>(L);
   auto launch = [&]() {
     if (sv)
-      hipLaunchKernelGGL((k_smooth<T, true>), dim3(nty * ntx * ntz), dim3(256), lds, c->st, g, L.flags, x0, x1,
-                         lvb<T>(L), ty, tx, tz, ntx, nty, 0, (T)c->kn.v0koef, (T)c->kn.p0koef);
+      hipLaunchKernelGGL((k_smooth<T, true>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), lds, c->st, g, L.flags, x0, x1,
+                         lvb<T>(L), tg, 0, (T)c->kn.v0koef, (T)c->kn.p0koef);
     else
-      hipLaunchKernelGGL((k_smooth<T, false>), dim3(nty * ntx * ntz), dim3(256), lds, c->st, g, L.flags, x0, x1,
-                         lvb<T>(L), ty, tx, tz, ntx, nty, 0, (T)c->kn.v0koef, (T)c->kn.p0koef);
+      hipLaunchKernelGGL((k_smooth<T, false>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), lds, c->st, g, L.flags, x0, x1,
+                         lvb<T>(L), tg, 0, (T)c->kn.v0koef, (T)c->kn.p0koef);
   };
   launch();
   hipEventRecord(c->ev0, c->st);
@@ -784,7 +786,7 @@ static int autotune_levels(stk_ctx *c, int reps) {
   for (int l = 0; l <= c->L; l++) {
     Level &L = c->lv[l];
     if (4 * (size_t)L.n * sizeof(T) <= kLevelLds) continue;  // smoothed by the whole-level kernel
-    // fixed shapes, plus tile heights that split the level's y extent evenly
+    // fixed shapes, plus tile heights that split the y extent of the equation-row box evenly
     std::vector<std::array<int, 4>> cand;
     auto add = [&](int ty, int tx, int tz) {
       for (int sv = 0; sv < 2; sv++) {
@@ -794,7 +796,7 @@ static int autotune_levels(stk_ctx *c, int reps) {
     };
     for (int k = 0; k < ns; k++) add(shape[k][0], shape[k][1], shape[k][2]);
     for (int parts = 2; parts <= 8; parts++)
-      for (const auto &q : xz) add((L.Y + parts - 1) / parts, q[0], q[1]);
+      for (const auto &q : xz) add((L.box[3] - L.box[0] + parts - 1) / parts, q[0], q[1]);
     // candidate 0 is the default 32x4x4 tile; another shape must beat it by 3 %
     // so that timing noise does not move the choice
     double best = 1e30, base = 0.0;
@@ -1281,6 +1283,23 @@ static int build_rows(stk_ctx *c) {
           }
           L.hflags[n] = f;
         }
+    // equation-row box (the whole level if there are none)
+    int bx[6] = {L.Y, L.X, L.Z, 0, 0, 0};
+    for (int m3 = 0; m3 < L.Z; m3++)
+      for (int m1 = 0; m1 < L.X; m1++)
+        for (int m2 = 0; m2 < L.Y; m2++) {
+          const unsigned char f = L.hflags[(long)m3 * L.XY + (long)m1 * L.Y + m2];
+          bool eq = false;
+          for (int cc = 0; cc < 4; cc++) eq = eq || kind_of(f, cc) == KIND_EQ;
+          if (!eq) continue;
+          const int m[3] = {m2, m1, m3};
+          for (int a = 0; a < 3; a++) {
+            bx[a] = std::min(bx[a], m[a]);
+            bx[a + 3] = std::max(bx[a + 3], m[a] + 1);
+          }
+        }
+    if (bx[3] == 0) { bx[0] = bx[1] = bx[2] = 0; bx[3] = L.Y; bx[4] = L.X; bx[5] = L.Z; }
+    for (int a = 0; a < 6; a++) L.box[a] = bx[a];
     // dependency depth of each boundary row
     long base = 4 * L.off;
     std::vector<int> depth(bcs.size(), 0);
@@ -1463,28 +1482,28 @@ int stk_time_kernel(stk_ctx *c, int which, int level, int reps, double *ms_avg) 
   const int *tt = kTileDefault4;  // the default tile: timings comparable across runs and with counter runs
   int tile[3] = {tt[0], tt[1], tt[2]};
   int ty = tile[0], tx = tile[1], tz = tile[2];
-  int nty = (L.Y + ty - 1) / ty, ntx = (L.X + tx - 1) / tx, ntz = (L.Z + tz - 1) / tz;
+  const TileGeo tg = make_tile(ty, tx, tz, L.box);
   auto launch = [&]() {
     if (which == 0) {
       Geo<float> g = L.geo<float>(c->kn.p2koef);
       if (tt[3])
-        hipLaunchKernelGGL((k_smooth<float, true>), dim3(nty * ntx * ntz), dim3(256), smooth_lds<float>(tile, true),
-                           c->st, g, L.flags, L.x32[0], L.x32[1], L.b32, ty, tx, tz, ntx, nty, 0, (float)c->kn.v0koef,
+        hipLaunchKernelGGL((k_smooth<float, true>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), smooth_lds<float>(tile, true),
+                           c->st, g, L.flags, L.x32[0], L.x32[1], L.b32, tg, 0, (float)c->kn.v0koef,
                            (float)c->kn.p0koef);
       else
-        hipLaunchKernelGGL((k_smooth<float, false>), dim3(nty * ntx * ntz), dim3(256), smooth_lds<float>(tile, false),
-                           c->st, g, L.flags, L.x32[0], L.x32[1], L.b32, ty, tx, tz, ntx, nty, 0, (float)c->kn.v0koef,
+        hipLaunchKernelGGL((k_smooth<float, false>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), smooth_lds<float>(tile, false),
+                           c->st, g, L.flags, L.x32[0], L.x32[1], L.b32, tg, 0, (float)c->kn.v0koef,
                            (float)c->kn.p0koef);
     } else if (which == 1) {
       Geo<double> g = L.geo<double>(c->kn.p2koef);
       const bool sv = tt[3] && smooth_lds<double>(tile, true) <= kLevelLds;
       if (sv)
-        hipLaunchKernelGGL((k_smooth<double, true>), dim3(nty * ntx * ntz), dim3(256), smooth_lds<double>(tile, true),
-                           c->st, g, L.flags, L.x64[0], L.x64[1], L.b64, ty, tx, tz, ntx, nty, 0, c->kn.v0koef,
+        hipLaunchKernelGGL((k_smooth<double, true>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), smooth_lds<double>(tile, true),
+                           c->st, g, L.flags, L.x64[0], L.x64[1], L.b64, tg, 0, c->kn.v0koef,
                            c->kn.p0koef);
       else
-        hipLaunchKernelGGL((k_smooth<double, false>), dim3(nty * ntx * ntz), dim3(256), smooth_lds<double>(tile, false),
-                           c->st, g, L.flags, L.x64[0], L.x64[1], L.b64, ty, tx, tz, ntx, nty, 0, c->kn.v0koef,
+        hipLaunchKernelGGL((k_smooth<double, false>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), smooth_lds<double>(tile, false),
+                           c->st, g, L.flags, L.x64[0], L.x64[1], L.b64, tg, 0, c->kn.v0koef,
                            c->kn.p0koef);
     } else if (which == 2) {
       Geo<double> g = L.geo<double>(c->kn.p2koef);
@@ -1838,32 +1857,57 @@ int stk_apply(stk_ctx *c, int l, const double *x, double *y) {
 
 // FP64 coupled sweeps on the reference arrays of level l, then fre0
 int stk_smooth(stk_ctx *c, int l, int sweeps, int ty, int tx, int tz) {
-  if (!c || !c->prepared || l < 0 || l > c->L || sweeps < 0) return STK_EINVAL;
+  return stk_smooth_ex(c, l, sweeps, ty, tx, tz, 1);
+}
+
+int stk_smooth_ex(stk_ctx *c, int l, int sweeps, int ty, int tx, int tz, int precision) {
+  if (!c || !c->prepared || l < 0 || l > c->L || sweeps < 0 || precision < 0 || precision > 1) return STK_EINVAL;
   const bool whole = (ty == 0 && tx == 0 && tz == 0);
   const bool staged = tz < 0;
   if (staged) tz = -tz;
-  if (whole && 4 * (size_t)c->lv[l].n * sizeof(double) > kLevelLds) return STK_EINVAL;
+  const size_t tsize = precision ? sizeof(double) : sizeof(float);
+  if (whole && 4 * (size_t)c->lv[l].n * tsize > kLevelLds) return STK_EINVAL;
   if (!whole && (ty < 1 || tx < 1 || tz < 1 || (size_t)(ty + 2) * (tx + 2) * (tz + 2) * 32 > 65536)) return STK_EINVAL;
-  int rc = ensure_ref(c, l) || ensure_vectors(c, false, true);
+  int rc = ensure_ref(c, l) || ensure_vectors(c, !precision, true);
   if (rc) return STK_ENOMEM;
   Level &L = c->lv[l];
-  size_t bytes = 4 * L.n * 8;
-  HIPCHK(hipMemcpyAsync(L.x64[L.cur64], L.ref[0], bytes, hipMemcpyDeviceToDevice, c->st));
-  HIPCHK(hipMemcpyAsync(L.b64, L.ref[1], bytes, hipMemcpyDeviceToDevice, c->st));
+  const long n4 = 4 * L.n;
+  const size_t bytes = n4 * 8;
   int tile[4] = {ty, tx, tz, 1};
-  if (staged) {
-    if (smooth_lds<double>(tile, true) > kLevelLds) return STK_EINVAL;
-    const int keep[4] = {L.tune64[0], L.tune64[1], L.tune64[2], L.tune64[3]};
-    for (int a = 0; a < 4; a++) L.tune64[a] = tile[a];
-    smooth<double>(c, l, sweeps, nullptr, 1, false);
-    for (int a = 0; a < 4; a++) L.tune64[a] = keep[a];
+  if (precision) {
+    HIPCHK(hipMemcpyAsync(L.x64[L.cur64], L.ref[0], bytes, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(L.b64, L.ref[1], bytes, hipMemcpyDeviceToDevice, c->st));
+    if (staged) {
+      if (smooth_lds<double>(tile, true) > kLevelLds) return STK_EINVAL;
+      const int keep[4] = {L.tune64[0], L.tune64[1], L.tune64[2], L.tune64[3]};
+      for (int a = 0; a < 4; a++) L.tune64[a] = tile[a];
+      smooth<double>(c, l, sweeps, nullptr, 1, false);
+      for (int a = 0; a < 4; a++) L.tune64[a] = keep[a];
+    } else {
+      smooth<double>(c, l, sweeps, tile, 1, whole);
+    }
   } else {
-    smooth<double>(c, l, sweeps, tile, 1, whole);
+    // FP32 arithmetic (the mixed-precision V-cycle's smoother) on rounded copies
+    hipLaunchKernelGGL((k_cast<double, float>), dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, L.ref[0],
+                       L.x32[L.cur32]);
+    hipLaunchKernelGGL((k_cast<double, float>), dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, L.ref[1], L.b32);
+    if (staged) {
+      if (smooth_lds<float>(tile, true) > kLevelLds) return STK_EINVAL;
+      const int keep[4] = {L.tune32[0], L.tune32[1], L.tune32[2], L.tune32[3]};
+      for (int a = 0; a < 4; a++) L.tune32[a] = tile[a];
+      smooth<float>(c, l, sweeps, nullptr, 1, false);
+      for (int a = 0; a < 4; a++) L.tune32[a] = keep[a];
+    } else {
+      smooth<float>(c, l, sweeps, tile, 1, whole);
+    }
+    hipLaunchKernelGGL((k_cast<double, double>), dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, L.ref[1], L.b64);
+    hipLaunchKernelGGL((k_cast<float, double>), dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, L.x32[L.cur32],
+                       L.x64[L.cur64]);
   }
   rowop<double, true>(c, l, L.x64[L.cur64], L.b64, L.ref[2], 1);
   HIPCHK(hipMemcpyAsync(L.ref[0], L.x64[L.cur64], bytes, hipMemcpyDeviceToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
-  return STK_OK;
+  return hipGetLastError() == hipSuccess ? STK_OK : STK_EHIP;
 }
 
 int stk_coarse_solve(stk_ctx *c) {

@@ -15,6 +15,7 @@
 // families, grad-div cell) computed once per viscosity field.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 namespace stk {
 
@@ -42,6 +43,13 @@ __constant__ signed char kSlot[4][18][4] = {
      {1, 1, 0, 0}, {1, 1, 1, 0}},
 };
 
+// Multiply-accumulate: FP32 rows fuse (one rounding), FP64 rows round the
+// product and the sum separately, as the reference's C does.
+__device__ __forceinline__ float mac(float acc, float a, float b) { return __builtin_fmaf(a, b, acc); }
+__device__ __forceinline__ double mac(double acc, double a, double b) { return acc + a * b; }
+__device__ __forceinline__ float msub(float acc, float a, float b) { return __builtin_fmaf(-a, b, acc); }
+__device__ __forceinline__ double msub(double acc, double a, double b) { return acc - a * b; }
+
 // Per-level geometry and stencil viscosities in arithmetic type T.
 template <typename T>
 struct Geo {
@@ -58,6 +66,16 @@ struct Geo {
 // mode): a[0..5] = vx v0, vx v2, vy v0, vy v1, vz v0, vz v4 with weight w.
 template <typename T, int AX>
 __device__ __forceinline__ void snn(T w, T ne, T xk, T yk, T zk, T a[6]) {
+  if constexpr (std::is_same<T, float>::value) {
+    // FP32 rows (the mixed-precision smoother): 4/3 and 2/3 as constants, no divisions
+    const float wn = w * ne, f4 = wn * (4.0f / 3.0f), f2 = wn * (2.0f / 3.0f);
+    const float p = (AX == 0) ? f4 : f2, q = (AX == 1) ? f4 : f2, r = (AX == 2) ? f4 : f2;
+    const float sp = (AX == 0) ? -1.0f : 1.0f, sq = (AX == 1) ? -1.0f : 1.0f, sr = (AX == 2) ? -1.0f : 1.0f;
+    a[0] = sp * p * xk; a[1] = -a[0];
+    a[2] = sq * q * yk; a[3] = -a[2];
+    a[4] = sr * r * zk; a[5] = -a[4];
+    return;
+  }
   if (AX == 0) {
     a[0] = -w * ne * T(4) / T(3) * xk; a[1] = +w * ne * T(4) / T(3) * xk;
     a[2] = +w * ne * T(2) / T(3) * yk; a[3] = -w * ne * T(2) / T(3) * yk;
@@ -241,8 +259,7 @@ __device__ __forceinline__ T row_dot(const Geo<T> &g, const T s[18], const T *__
   for (int q = 1; q <= Slot<C>::ns; q++) {
     const long uq = 4 * (n + (long)Slot<C>::d[q][2] * g.XY + (long)Slot<C>::d[q][0] * g.Y + Slot<C>::d[q][1]) +
                     Slot<C>::d[q][3];
-    if (RES) acc -= s[q] * x[uq];
-    else acc += s[q] * x[uq];
+    acc = RES ? msub(acc, s[q], x[uq]) : mac(acc, s[q], x[uq]);
   }
   return acc;
 }

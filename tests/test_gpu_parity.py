@@ -106,6 +106,39 @@ def test_coloured_sweep_matches_oracle(oracle_mod, tile):
         g.close()
 
 
+@pytest.mark.parametrize("tile", [(32, 4, 4), (8, 2, 3), (16, 4, -2), (0, 0, 0)])
+def test_fp32_sweep_matches_fp64_oracle_to_fp32_accuracy(oracle_mod, tile):
+    """The FP32 smoother of the mixed-precision V-cycle (fused multiply-adds,
+    4/3 and 2/3 as constants, hardware reciprocal of the diagonal) makes the
+    oracle's FP64 row updates to FP32 accuracy: max error <= 2e-5 of max |x|
+    after 3 sweeps.  A wrong coefficient or slot is an O(1) error.  Tile
+    (0, 0, 0) is the LDS-resident whole-level kernel (level 1, 11^3 nodes)."""
+    pb, o, g = _pair(oracle_mod, seed=3)
+    try:
+        l = 1 if tile == (0, 0, 0) else 0
+        X, Y, Z = o.info[l][:3]
+        b4, n4 = 4 * o.info[l][5], 4 * o.info[l][4]
+        rng = np.random.default_rng(8)
+        x = rng.standard_normal(n4) * 1e-16
+        f = rng.standard_normal(n4) * (o.arr("num0")[b4:b4 + n4] > 0)
+        o.arr("sol0")[b4:b4 + n4] = x
+        o.arr("fre1")[b4:b4 + n4] = f
+        g.vec_set(S.SOL0, l, x)
+        g.vec_set(S.FRE1, l, f)
+        if tile == (0, 0, 0):
+            o.sweep_tiles(3, l, Y, X, Z)
+        else:
+            o.sweep_tiles(3, l, tile[0], tile[1], abs(tile[2]))
+        g.smooth(l, 3, tile=tile, precision=0)
+        xo = o.arr("sol0")[b4:b4 + n4]
+        xg = g.vec_get(S.SOL0, l)
+        err = np.abs(xg - xo).max() / np.abs(xo).max()
+        assert err <= 2e-5, err
+        assert err > 0  # the FP32 path really ran in FP32
+    finally:
+        g.close()
+
+
 def test_whole_level_sweep_matches_oracle(oracle_mod):
     """The LDS-resident whole-level smoother equals the tile sweep with one
     tile covering the level (level 1 of a 17^3 hierarchy: 11^3 nodes)."""

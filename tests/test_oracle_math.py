@@ -218,3 +218,34 @@ def test_gcr_solution_matches_direct_solve(oracle_mod):
     pr = (np.arange(x.size) % 4 == 0) & (num0 > 0)
     dp = x[pr] - xd[pr]
     assert np.abs(dp - dp.mean()).max() <= 1e-6 * np.abs(xd[pr]).max()
+
+
+def test_equation_box_of_ghost_layer_boxes(oracle_mod):
+    """With walls on lines 1 and n-2 every level's equation rows sit on nodes
+    1..n-2 in each direction; a sweep whose single tile spans that box equals
+    one whose single tile spans the whole level (no equation row is left out)."""
+    for n in (17, 33):
+        pb = setups.sinker(n, setups.coarsest_levels(n))
+        o = oracle_mod.from_problem(pb)
+        for l in range(pb.levels + 1):
+            X, Y, Z = o.info[l][:3]
+            assert o.eq_box(l) == (1, 1, 1, Y - 1, X - 1, Z - 1)
+    pb = setups.sinker(17, 2, seed=5)
+    o = oracle_mod.from_problem(pb)
+    n4 = 4 * o.info[0][4]
+    x = np.random.default_rng(3).standard_normal(n4) * 1e-15
+    o.arr("sol0")[:n4] = x
+    o.sweep_tiles(2, 0, 15, 15, 15)  # one tile over the box (1..15)
+    y_box = o.arr("sol0")[:n4].copy()
+    o.arr("sol0")[:n4] = x
+    o.sweep_tiles(2, 0, 64, 64, 64)  # one tile over the whole level
+    np.testing.assert_array_equal(o.arr("sol0")[:n4], y_box)
+
+
+def test_tile_index_division_by_reciprocal():
+    """The smoother decodes tile indices as umulhi(i, ceil(2^32/d)), exact for
+    numerators below 2^16 and divisors 2 <= d < 2^16."""
+    i = np.arange(1 << 16, dtype=np.uint64)
+    for d in list(range(2, 140)) + [255, 256, 1000, 4097, 65535]:
+        m = np.uint64(((1 << 32) + d - 1) // d)
+        np.testing.assert_array_equal((i * m) >> np.uint64(32), i // np.uint64(d))
