@@ -155,9 +155,11 @@ int stk_apply(stk_ctx *ctx, int level, const double *x, double *y);
 int stk_smooth(stk_ctx *ctx, int level, int sweeps, int ty, int tx, int tz);
 /* The same in FP32 arithmetic (precision 0: the mixed-precision V-cycle's
  * smoother, on FP32 roundings of sol0/fre1; sol0 is widened back and fre0 is
- * the FP64 residual) or FP64 (precision 1, = stk_smooth).  Same reference
- * routine: gausmat4, parallel/move3mg.c:3855-3960. */
-int stk_smooth_ex(stk_ctx *ctx, int level, int sweeps, int ty, int tx, int tz, int precision);
+ * the FP64 residual) or FP64 (precision 1), on all rows or (velocity_only)
+ * on the velocity rows as the V-cycle of the velocity block does.
+ * stk_smooth = stk_smooth_ex(..., 1, 0).  Same reference routine: gausmat4,
+ * parallel/move3mg.c:3855-3960. */
+int stk_smooth_ex(stk_ctx *ctx, int level, int sweeps, int ty, int tx, int tz, int precision, int velocity_only);
 int stk_coarse_solve(stk_ctx *ctx);
 
 /* ---- smoother tile autotuning ------------------------------------------ */

@@ -55,6 +55,7 @@ def lib():
         L.orc_residual.argtypes = [C.c_void_p, C.c_int]
         L.orc_sweep_tiles.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_eq_box.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_set_vonly.argtypes = [C.c_void_p, C.c_int]
         L.orc_bc_pass.argtypes = [C.c_void_p, C.c_int, dp]
         L.orc_vprestrict.argtypes = [C.c_void_p, C.c_int]
         L.orc_vpprolong.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -181,6 +182,10 @@ class Oracle:
 
     def sweep_tiles(self, am, l, ty=64, tx=4, tz=4):
         lib().orc_sweep_tiles(self.h, am, l, ty, tx, tz)
+
+    def set_vonly(self, flag):
+        """Velocity rows only (pressure rows left out of sweeps and boundary passes)."""
+        lib().orc_set_vonly(self.h, int(flag))
 
     def eq_box(self, l):
         """(y0, x0, z0, y1, x1, z1) of the nodes with equation rows on level l."""

@@ -90,7 +90,7 @@ def lib():
         L.stk_vec_get.argtypes = [vp, C.c_int, C.c_int, dp]
         L.stk_apply.argtypes = [vp, C.c_int, dp, dp]
         L.stk_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
-        L.stk_smooth_ex.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.stk_smooth_ex.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.stk_coarse_solve.argtypes = [vp]
         L.stk_bind.argtypes = [vp]
         L.stk_kop_apply.argtypes = [vp, C.c_int, dp, dp]
@@ -264,11 +264,11 @@ class Stokes:
         check(lib().stk_apply(self.h, level, _dp(x), _dp(y)), "stk_apply")
         return y
 
-    def smooth(self, level, sweeps, tile=(32, 4, 4), precision=1):
-        if precision == 1:
+    def smooth(self, level, sweeps, tile=(32, 4, 4), precision=1, velocity_only=False):
+        if precision == 1 and not velocity_only:
             check(lib().stk_smooth(self.h, level, sweeps, *tile), "stk_smooth")
         else:
-            check(lib().stk_smooth_ex(self.h, level, sweeps, *tile, precision), "stk_smooth_ex")
+            check(lib().stk_smooth_ex(self.h, level, sweeps, *tile, precision, int(velocity_only)), "stk_smooth_ex")
 
     def coarse_solve(self):
         check(lib().stk_coarse_solve(self.h), "stk_coarse_solve")

@@ -375,25 +375,70 @@ __global__ void k_bc_chain(BcChain ch, const T *__restrict__ b, T *x, int mode) 
 // sweep the phases vx(red,black), vy(red,black), vz(red,black)[, P] over the
 // whole level, then the boundary chains.  It is the tile-coloured sweep with
 // one tile covering the level.
+// divisors of the whole-level index decode: Y, X, ceil(Y/2)
+struct LevelDiv {
+  FDiv dy, dx, dhty;
+};
+
+// Velocity rows of a small level in FP32, computed once per viscosity field:
+// coef[((C-1)*18 + q)*N + n] = slot q of the C row at node n (q = 1..17),
+// slot 0 = the reciprocal of the diagonal the FP32 update multiplies by.
+__global__ void k_level_coef(Geo<float> g, const unsigned char *__restrict__ flags, float *coef) {
+  const int N = g.XY * g.Z;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  int m1, m2, m3;
+  decode(n, g.X, g.Y, g.XY, m1, m2, m3);
+  float s[18];
+  for (int C = 1; C <= 3; C++) {
+    for (int q = 0; q < 18; q++) s[q] = 0.0f;
+    if (kind_of(flags[n], C) == KIND_EQ) {
+      if (C == 1) row_coef_c<float, 1>(g, m1, m2, m3, s);
+      else if (C == 2) row_coef_c<float, 2>(g, m1, m2, m3, s);
+      else row_coef_c<float, 3>(g, m1, m2, m3, s);
+      s[0] = __builtin_amdgcn_rcpf(s[1]);
+    }
+    for (int q = 0; q < 18; q++) coef[(long)((C - 1) * 18 + q) * N + n] = s[q];
+  }
+}
+
 template <typename T, int C>
 __device__ __forceinline__ void level_phase(const Geo<T> &g, const unsigned char *__restrict__ flags,
-                                            const T *__restrict__ b, T *sx, int colour, T v0koef, T p0koef) {
+                                            const T *__restrict__ b, T *sx, int colour, T v0koef, T p0koef,
+                                            const LevelDiv &ld, const float *__restrict__ coef) {
   const int N = g.XY * g.Z;
   const int hty = (g.Y + 1) / 2;
   const int ncand = (C == 0) ? g.XY * g.Z : hty * g.X * g.Z;
   for (int i = threadIdx.x; i < ncand; i += blockDim.x) {
     int m1, m2, m3;
     if (C == 0) {
-      m2 = i % g.Y; m1 = (i / g.Y) % g.X; m3 = i / g.XY;
+      const int q = (int)fdiv(i, ld.dy);
+      m2 = i - q * g.Y; m3 = (int)fdiv(q, ld.dx); m1 = q - m3 * g.X;
     } else {
-      const int kk = i % hty;
-      m1 = (i / hty) % g.X; m3 = i / (hty * g.X);
+      const int q = (int)fdiv(i, ld.dhty);
+      const int kk = i - q * hty;
+      m3 = (int)fdiv(q, ld.dx); m1 = q - m3 * g.X;
       m2 = 2 * kk + ((colour - m1 - m3) & 1);
       if (m2 >= g.Y) continue;
     }
     const long n = g.nid(m1, m2, m3);
     if (kind_of(flags[n], C) != KIND_EQ) continue;
     T s[18];
+    if constexpr (std::is_same<T, float>::value && C != 0) {
+      if (coef) {
+        // precomputed rows: slot 0 holds 1/diagonal
+#pragma unroll
+        for (int q = 0; q <= Slot<C>::ns; q++) s[q] = coef[(long)((C - 1) * 18 + q) * N + n];
+        T ival = -b[4 * n + C];
+#pragma unroll
+        for (int q = 1; q <= Slot<C>::ns; q++) {
+          const long nb = n + (long)Slot<C>::d[q][2] * g.XY + (long)Slot<C>::d[q][0] * g.Y + Slot<C>::d[q][1];
+          ival = mac(ival, s[q], sx[Slot<C>::d[q][3] * N + nb]);
+        }
+        sx[C * N + n] -= ival * s[0] * v0koef;
+        continue;
+      }
+    }
     row_coef_c<T, C>(g, m1, m2, m3, s);
     T ival = -b[4 * n + C];
 #pragma unroll
@@ -409,7 +454,8 @@ __device__ __forceinline__ void level_phase(const Geo<T> &g, const unsigned char
 template <typename T>
 __global__ void __launch_bounds__(1024) k_smooth_level(Geo<T> g, const unsigned char *__restrict__ flags, T *x,
                                                        const T *__restrict__ b, int sweeps, int withp, T v0koef,
-                                                       T p0koef, BcChain ch) {
+                                                       T p0koef, BcChain ch, LevelDiv ld,
+                                                       const float *__restrict__ coef) {
   extern __shared__ unsigned char smem_raw[];
   T *sx = reinterpret_cast<T *>(smem_raw);  // component-major: sx[c * N + node]
   const int N = g.XY * g.Z, n4 = 4 * N;
@@ -417,10 +463,10 @@ __global__ void __launch_bounds__(1024) k_smooth_level(Geo<T> g, const unsigned 
   __syncthreads();
   const int mode = withp ? 0 : 1;
   for (int s = 0; s < sweeps; s++) {
-    for (int col = 0; col < 2; col++) { level_phase<T, 1>(g, flags, b, sx, col, v0koef, p0koef); __syncthreads(); }
-    for (int col = 0; col < 2; col++) { level_phase<T, 2>(g, flags, b, sx, col, v0koef, p0koef); __syncthreads(); }
-    for (int col = 0; col < 2; col++) { level_phase<T, 3>(g, flags, b, sx, col, v0koef, p0koef); __syncthreads(); }
-    if (withp) { level_phase<T, 0>(g, flags, b, sx, 0, v0koef, p0koef); __syncthreads(); }
+    for (int col = 0; col < 2; col++) { level_phase<T, 1>(g, flags, b, sx, col, v0koef, p0koef, ld, coef); __syncthreads(); }
+    for (int col = 0; col < 2; col++) { level_phase<T, 2>(g, flags, b, sx, col, v0koef, p0koef, ld, coef); __syncthreads(); }
+    for (int col = 0; col < 2; col++) { level_phase<T, 3>(g, flags, b, sx, col, v0koef, p0koef, ld, coef); __syncthreads(); }
+    if (withp) { level_phase<T, 0>(g, flags, b, sx, 0, v0koef, p0koef, ld, coef); __syncthreads(); }
     for (int i = threadIdx.x; i < ch.nbc; i += blockDim.x) {
       const long o = (long)i * ch.dm;
       const int r0 = ch.r[o];
@@ -430,6 +476,94 @@ __global__ void __launch_bounds__(1024) k_smooth_level(Geo<T> g, const unsigned 
       for (int t = ch.len[i] - 1; t >= 0; t--) v = b[ch.r[o + t]] + (T)ch.k[o + t] * v;
       sx[(r0 & 3) * N + (r0 >> 2)] = v;
     }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < n4; i += blockDim.x) x[i] = sx[(i & 3) * N + (i >> 2)];
+}
+
+// Whole-level FP32 velocity smoother (the V-cycle's small levels) when each
+// thread owns at most one candidate per colour: rows come from k_level_coef,
+// and the rows and right-hand side a phase needs are loaded during the phase
+// before it, so a phase costs its LDS reads and one barrier.  The boundary
+// chains are composed once to x_r = c + K x_j and kept in LDS after the
+// unknowns (4 words per chain).
+template <int C>
+__device__ __forceinline__ void lrow_load(const float *__restrict__ coef, const float *__restrict__ b, int N, int n,
+                                          int f, float s[18], float &bv) {
+  if (n < 0 || kind_of((unsigned char)f, C) != KIND_EQ) return;
+#pragma unroll
+  for (int q = 0; q <= Slot<C>::ns; q++) s[q] = coef[(long)((C - 1) * 18 + q) * N + n];
+  bv = b[4 * n + C];
+}
+
+// n: this thread's candidate node for the phase, or -1; f: its flags
+template <int C>
+__device__ __forceinline__ void lrow_update(const Geo<float> &g, float *sx, int N, int n, int f, const float s[18],
+                                            float bv, float v0koef) {
+  if (n < 0 || kind_of((unsigned char)f, C) != KIND_EQ) return;
+  float ival = -bv;
+#pragma unroll
+  for (int q = 1; q <= Slot<C>::ns; q++) {
+    const int nb = n + Slot<C>::d[q][2] * g.XY + Slot<C>::d[q][0] * g.Y + Slot<C>::d[q][1];
+    ival = mac(ival, s[q], sx[Slot<C>::d[q][3] * N + nb]);
+  }
+  sx[C * N + n] -= ival * s[0] * v0koef;
+}
+
+__global__ void __launch_bounds__(1024) k_smooth_level_pc(Geo<float> g, const unsigned char *__restrict__ flags,
+                                                          float *x, const float *__restrict__ b, int sweeps,
+                                                          float v0koef, BcChain ch, LevelDiv ld,
+                                                          const float *__restrict__ coef) {
+  extern __shared__ unsigned char smem_raw[];
+  float *sx = reinterpret_cast<float *>(smem_raw);  // component-major: sx[c * N + node]
+  const int N = g.XY * g.Z, n4 = 4 * N;
+  int *cr = reinterpret_cast<int *>(sx + n4);       // chain target in sx (-1: pressure row)
+  int *cj = cr + ch.nbc;                            // partner in sx
+  float *cc = reinterpret_cast<float *>(cj + ch.nbc), *ck = cc + ch.nbc;
+  for (int i = threadIdx.x; i < n4; i += blockDim.x) sx[(i & 3) * N + (i >> 2)] = x[i];
+  for (int i = threadIdx.x; i < ch.nbc; i += blockDim.x) {
+    const long o = (long)i * ch.dm;
+    const int r0 = ch.r[o], jf = ch.j[i];
+    float c = 0.0f, K = jf >= 0 ? 1.0f : 0.0f;
+    for (int u = ch.len[i] - 1; u >= 0; u--) {
+      const float k = (float)ch.k[o + u];
+      c = b[ch.r[o + u]] + k * c;
+      K = k * K;
+    }
+    cr[i] = (r0 & 3) == 0 ? -1 : (r0 & 3) * N + (r0 >> 2);
+    cj[i] = jf >= 0 ? (jf & 3) * N + (jf >> 2) : 0;
+    cc[i] = c; ck[i] = K;
+  }
+  // this thread's candidate node of each colour
+  const int hty = (g.Y + 1) / 2, ncand = hty * g.X * g.Z;
+  int nc0 = -1, nc1 = -1;
+  if ((int)threadIdx.x < ncand) {
+    const int i = threadIdx.x;
+    const int q = (int)fdiv(i, ld.dhty), kk = i - q * hty;
+    const int m3 = (int)fdiv(q, ld.dx), m1 = q - m3 * g.X;
+    const int p0 = (0 - m1 - m3) & 1;
+    const int m2a = 2 * kk + p0, m2b = 2 * kk + (1 - p0);
+    if (m2a < g.Y) nc0 = (int)g.nid(m1, m2a, m3);
+    if (m2b < g.Y) nc1 = (int)g.nid(m1, m2b, m3);
+  }
+  const int f0 = nc0 >= 0 ? flags[nc0] : 0, f1 = nc1 >= 0 ? flags[nc1] : 0;
+  float sa[18], sb[18], ba = 0.0f, bb = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 18; q++) { sa[q] = 0.0f; sb[q] = 0.0f; }
+  lrow_load<1>(coef, b, N, nc0, f0, sa, ba);
+  __syncthreads();
+  for (int s = 0; s < sweeps; s++) {
+    // keep the 6 x 17 neighbour addresses from being hoisted out of the
+    // sweep loop into registers (an empty asm that may change the nodes)
+    asm volatile("" : "+v"(nc0), "+v"(nc1));
+    lrow_load<1>(coef, b, N, nc1, f1, sb, bb); lrow_update<1>(g, sx, N, nc0, f0, sa, ba, v0koef); __syncthreads();
+    lrow_load<2>(coef, b, N, nc0, f0, sa, ba); lrow_update<1>(g, sx, N, nc1, f1, sb, bb, v0koef); __syncthreads();
+    lrow_load<2>(coef, b, N, nc1, f1, sb, bb); lrow_update<2>(g, sx, N, nc0, f0, sa, ba, v0koef); __syncthreads();
+    lrow_load<3>(coef, b, N, nc0, f0, sa, ba); lrow_update<2>(g, sx, N, nc1, f1, sb, bb, v0koef); __syncthreads();
+    lrow_load<3>(coef, b, N, nc1, f1, sb, bb); lrow_update<3>(g, sx, N, nc0, f0, sa, ba, v0koef); __syncthreads();
+    lrow_load<1>(coef, b, N, nc0, f0, sa, ba); lrow_update<3>(g, sx, N, nc1, f1, sb, bb, v0koef); __syncthreads();
+    for (int i = threadIdx.x; i < ch.nbc; i += blockDim.x)
+      if (cr[i] >= 0) sx[cr[i]] = cc[i] + ck[i] * sx[cj[i]];
     __syncthreads();
   }
   for (int i = threadIdx.x; i < n4; i += blockDim.x) x[i] = sx[(i & 3) * N + (i >> 2)];
@@ -462,6 +596,7 @@ struct Ptrs16 {
   const double *p[16];
 };
 
+constexpr int kPad = 6;  // padded gather-list width (2:1 hierarchies need at most 5)
 struct AxisTab {
   const int *lo;        // [nf]
   const double *w64;    // [2*nf] weights for lo, lo+1
@@ -470,6 +605,9 @@ struct AxisTab {
   const int *cfi;       // fine index of each entry
   const double *cw64;   // weight of each entry
   const float *cw32;
+  const int *pfi;       // [kPad*nc] the same lists padded to kPad entries (fine index -1: none)
+  const double *pw64;
+  const float *pw32;
 };
 
 struct CompTab {
@@ -521,6 +659,62 @@ __global__ void k_restrict(int Xf, int Yf, int Zf, int Xc, int Yc, int Zc, CompT
       }
     }
   }
+  bc[u] = (ws != T(0)) ? acc / ws : acc;
+}
+
+// The same sums from the padded lists when no coarse index has more than K
+// fine entries per axis (K = 4 on the fine levels of a 2:1 hierarchy, 5 near
+// the coarsest): the KxKxK candidate loop has fixed trip counts and its loads
+// are independent of each other.  Padding and non-equation candidates add an
+// exact zero, so the sums and their order are those of k_restrict.
+template <typename T, int K>
+__global__ void __launch_bounds__(256) k_restrictK(int Xf, int Yf, int Zf, int Xc, int Yc, int Zc, CompTab t0,
+                                                   CompTab t1, CompTab t2, CompTab t3,
+                                                   const unsigned char *__restrict__ ff,
+                                                   const unsigned char *__restrict__ fc, const T *__restrict__ rf,
+                                                   const float *__restrict__ ndf, const float *__restrict__ ndc,
+                                                   T *bc, T *xc, int withp) {
+  const long u = xcd_block() * blockDim.x + threadIdx.x;
+  const long n = u >> 2;
+  const int XYc = Xc * Yc, XYf = Xf * Yf;
+  if (n >= (long)XYc * Zc) return;
+  const int c = (int)(u & 3);
+  xc[u] = T(0);
+  if (kind_of(fc[n], c) != KIND_EQ || (!withp && c == 0)) { bc[u] = T(0); return; }
+  int cx, cy, cz;
+  decode(n, Xc, Yc, XYc, cx, cy, cz);
+  const CompTab &t = (c == 0) ? t0 : (c == 1) ? t1 : (c == 2) ? t2 : t3;
+  long oz[K], ox[K];
+  int jy[K];
+  T wz[K], wx[K], wy[K];
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    const int k = t.ax[2].pfi[kPad * cz + q], i = t.ax[0].pfi[kPad * cx + q];
+    jy[q] = t.ax[1].pfi[kPad * cy + q];
+    oz[q] = k < 0 ? -1 : (long)k * XYf;
+    ox[q] = i < 0 ? -1 : (long)i * Yf;
+    wz[q] = tw<T>(t.ax[2].pw64, t.ax[2].pw32, kPad * cz + q);
+    wx[q] = tw<T>(t.ax[0].pw64, t.ax[0].pw32, kPad * cx + q);
+    wy[q] = tw<T>(t.ax[1].pw64, t.ax[1].pw32, kPad * cy + q);
+  }
+  const T ndn = (c == 0) ? (T)ndc[n] : T(1);
+  T acc = T(0), ws = T(0);
+#pragma unroll
+  for (int a = 0; a < K; a++)
+#pragma unroll
+    for (int b = 0; b < K; b++)
+#pragma unroll
+      for (int e = 0; e < K; e++) {
+        const bool ok = oz[a] >= 0 && ox[b] >= 0 && jy[e] >= 0;
+        const long nf = ok ? oz[a] + ox[b] + jy[e] : 0;
+        const bool eq = ok && kind_of(ff[nf], c) == KIND_EQ;
+        const T swt = wx[b] * wy[e] * wz[a];
+        T iv;
+        if (c == 0) iv = rf[4 * nf] * (T)ndf[nf] / ndn * swt;
+        else iv = rf[4 * nf + c] * swt;
+        acc += eq ? iv : T(0);
+        ws += eq ? swt : T(0);
+      }
   bc[u] = (ws != T(0)) ? acc / ws : acc;
 }
 

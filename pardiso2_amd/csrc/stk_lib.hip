@@ -59,15 +59,21 @@ struct AxisHost {
   std::vector<double> w;  // 2 per fine index
   std::vector<int> cbeg, cfi;
   std::vector<double> cw;
+  int kmax = 0;           // longest gather list
+  std::vector<int> pfi;   // lists padded to kPad entries (if kmax <= kPad)
+  std::vector<double> pw;
 };
 
 struct AxisDev {
-  int *lo = nullptr, *cbeg = nullptr, *cfi = nullptr;
-  double *w64 = nullptr, *cw64 = nullptr;
-  float *w32 = nullptr, *cw32 = nullptr;
-  AxisTab tab() const { return AxisTab{lo, w64, w32, cbeg, cfi, cw64, cw32}; }
+  int *lo = nullptr, *cbeg = nullptr, *cfi = nullptr, *pfi = nullptr;
+  double *w64 = nullptr, *cw64 = nullptr, *pw64 = nullptr;
+  float *w32 = nullptr, *cw32 = nullptr, *pw32 = nullptr;
+  int kmax = 0;
+  AxisTab tab() const { return AxisTab{lo, w64, w32, cbeg, cfi, cw64, cw32, pfi, pw64, pw32}; }
   void release() {
     dfree(lo); dfree(cbeg); dfree(cfi); dfree(w64); dfree(cw64); dfree(w32); dfree(cw32);
+    dfree(pfi); dfree(pw64); dfree(pw32);
+    kmax = 0;
   }
 };
 
@@ -80,6 +86,7 @@ struct Level {
   float *iA32[3] = {}, *iB32[3] = {};
   double *e64[5] = {};
   float *e32[5] = {};
+  float *lc32 = nullptr;  // FP32 velocity rows of an LDS-resident level (54 planes), from its viscosities
   unsigned char *flags = nullptr;
   std::vector<unsigned char> hflags;
   // bounding box of the nodes with equation rows {y0, x0, z0, y1, x1, z1}: the smoother's tiles cover it
@@ -139,6 +146,7 @@ Geo<float> Level::geo<float>(double p2) const {
 
 struct stk_ctx {
   int L = 0;
+  size_t lds_max = 65536;  // dynamic LDS one workgroup may use on this device
   int per[3] = {0, 0, 0};
   Level lv[MAXLV];
   long ntot = 0;
@@ -259,6 +267,16 @@ AxisHost build_axis(int kind, int nf, int nc, const std::vector<double> &gf, con
     }
   }
   a.cbeg[nc] = (int)a.cfi.size();
+  for (int C = 0; C < nc; C++) a.kmax = std::max(a.kmax, a.cbeg[C + 1] - a.cbeg[C]);
+  if (a.kmax <= kPad) {
+    a.pfi.assign(kPad * (size_t)nc, -1);
+    a.pw.assign(kPad * (size_t)nc, 0.0);
+    for (int C = 0; C < nc; C++)
+      for (int e = a.cbeg[C]; e < a.cbeg[C + 1]; e++) {
+        a.pfi[kPad * C + (e - a.cbeg[C])] = a.cfi[e];
+        a.pw[kPad * C + (e - a.cbeg[C])] = a.cw[e];
+      }
+  }
   return a;
 }
 
@@ -295,6 +313,13 @@ int upload_axis(AxisDev &d, const AxisHost &h) {
   rc |= upload(&d.cfi, h.cfi);
   rc |= upload(&d.cw64, h.cw);
   rc |= upload(&d.cw32, cw32);
+  d.kmax = h.kmax;
+  if (h.kmax <= kPad) {
+    std::vector<float> pw32(h.pw.begin(), h.pw.end());
+    rc |= upload(&d.pfi, h.pfi);
+    rc |= upload(&d.pw64, h.pw);
+    rc |= upload(&d.pw32, pw32);
+  }
   return rc ? STK_ENOMEM : STK_OK;
 }
 
@@ -396,8 +421,19 @@ void smooth(stk_ctx *c, int l, int sweeps, const int *tile, int withp, bool whol
   if (!tile) tile = tuned_tile<T>(L);
   if (whole_level && sweeps > 0 && 4 * (size_t)L.n * sizeof(T) <= kLevelLds) {
     Geo<T> g = L.geo<T>(c->kn.p2koef);
+    const LevelDiv ld{make_fdiv(L.Y), make_fdiv(L.X), make_fdiv((L.Y + 1) / 2)};
+    const float *coef = std::is_same<T, float>::value ? L.lc32 : nullptr;
+    const long ncand = (long)((L.Y + 1) / 2) * L.X * L.Z;
+    if constexpr (std::is_same<T, float>::value) {
+      const size_t lds_pc = 4 * (size_t)L.n * sizeof(float) + 16 * (size_t)L.nbc;
+      if (coef && !withp && ncand <= 1024 && lds_pc <= c->lds_max) {
+        hipLaunchKernelGGL(k_smooth_level_pc, dim3(1), dim3(1024), lds_pc, c->st, g, L.flags, lvx<float>(L),
+                           lvb<float>(L), sweeps, (float)c->kn.v0koef, L.chain(), ld, coef);
+        return;
+      }
+    }
     hipLaunchKernelGGL(k_smooth_level<T>, dim3(1), dim3(1024), 4 * (size_t)L.n * sizeof(T), c->st, g, L.flags,
-                       lvx<T>(L), lvb<T>(L), sweeps, withp, (T)c->kn.v0koef, (T)c->kn.p0koef, L.chain());
+                       lvx<T>(L), lvb<T>(L), sweeps, withp, (T)c->kn.v0koef, (T)c->kn.p0koef, L.chain(), ld, coef);
     return;
   }
   int ty = tile[0], tx = tile[1], tz = tile[2];
@@ -434,12 +470,31 @@ CompTab comptab(const Level &L, int comp) {
   return t;
 }
 
+// longest gather list of the axis tables the restriction uses (kinds N, C,
+// P), or kPad + 1 if a table is not padded
+int gather_width(const Level &C) {
+  int k = 0;
+  for (int kind = 0; kind < 3; kind++)
+    for (int a = 0; a < 3; a++) k = std::max(k, C.ax[kind][a].pfi ? C.ax[kind][a].kmax : kPad + 1);
+  return k;
+}
+
 template <typename T>
 void restrict_to(stk_ctx *c, int l, const T *rf, T *bcoarse, T *xcoarse, int withp) {
   Level &F = c->lv[l - 1], &C = c->lv[l];
-  hipLaunchKernelGGL(k_restrict<T>, dim3(blocks(4 * C.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, C.X, C.Y, C.Z,
-                     comptab(C, 0), comptab(C, 1), comptab(C, 2), comptab(C, 3), F.flags, C.flags, rf,
-                     c->nd + F.off, c->nd + C.off, bcoarse, xcoarse, withp);
+  const int kw = gather_width(C);
+  if (kw <= 4)
+    hipLaunchKernelGGL((k_restrictK<T, 4>), dim3(blocks(4 * C.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, C.X,
+                       C.Y, C.Z, comptab(C, 0), comptab(C, 1), comptab(C, 2), comptab(C, 3), F.flags, C.flags, rf,
+                       c->nd + F.off, c->nd + C.off, bcoarse, xcoarse, withp);
+  else if (kw <= kPad)
+    hipLaunchKernelGGL((k_restrictK<T, kPad>), dim3(blocks(4 * C.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, C.X,
+                       C.Y, C.Z, comptab(C, 0), comptab(C, 1), comptab(C, 2), comptab(C, 3), F.flags, C.flags, rf,
+                       c->nd + F.off, c->nd + C.off, bcoarse, xcoarse, withp);
+  else
+    hipLaunchKernelGGL(k_restrict<T>, dim3(blocks(4 * C.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, C.X, C.Y, C.Z,
+                       comptab(C, 0), comptab(C, 1), comptab(C, 2), comptab(C, 3), F.flags, C.flags, rf,
+                       c->nd + F.off, c->nd + C.off, bcoarse, xcoarse, withp);
 }
 
 template <typename T>
@@ -835,6 +890,12 @@ int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *
   if (xper || yper || zper) return STK_EINVAL;  // periodic boxes: not supported by the device path
   stk_ctx *c = new stk_ctx();
   c->L = levels;
+  {
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+      c->lds_max = (size_t)lds;
+  }
   const double *g0[3] = {gx, gy, gz};
   int nf[3] = {nx, ny, nz};
   long off = 0;
@@ -919,6 +980,7 @@ void stk_destroy(stk_ctx *c) {
     Level &L = c->lv[l];
     for (int a = 0; a < 3; a++) { dfree(L.iA64[a]); dfree(L.iB64[a]); dfree(L.iA32[a]); dfree(L.iB32[a]); dfree(L.g64[a]); }
     for (int k = 0; k < 5; k++) { dfree(L.e64[k]); dfree(L.e32[k]); }
+    dfree(L.lc32);
     dfree(L.flags);
     dfree(L.bc_r); dfree(L.bc_j); dfree(L.bc_negk); dfree(L.bc_c);
     dfree(L.ch_r); dfree(L.ch_len); dfree(L.ch_j); dfree(L.ch_k);
@@ -967,7 +1029,8 @@ void stk_default_opts(stk_solver_opts *o, int levels) {
   memset(o, 0, sizeof(*o));
   o->precision = 0;
   o->restart = 128;
-  int pre[12] = {2, 4, 8, 16, 16, 16, 16, 16, 16, 16, 16, 16};
+  // per level from the finest; measured over random-field, inclusion and sinker boxes (33^3..129^3)
+  int pre[12] = {2, 4, 8, 8, 12, 12, 12, 12, 12, 12, 12, 12};
   for (int l = 0; l < 12; l++) { o->pre[l] = pre[l]; o->post[l] = pre[l]; }
   (void)levels;
   o->tile[0] = o->tile[1] = o->tile[2] = 0;  // per-level autotuned tile, else 32x4x4
@@ -1399,6 +1462,12 @@ int stk_prepare_cvm(stk_ctx *c, int cvm) {
     // nd = cell viscosity of the continuity stencil (continerr, move3mg.c:1549)
     hipLaunchKernelGGL((k_convert<float, float>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, L.n, L.e32[0],
                        c->nd + L.off);
+    // rows of the FP32 whole-level smoother
+    if (4 * (size_t)L.n * sizeof(float) <= kLevelLds) {
+      if (!L.lc32 && dalloc(&L.lc32, (size_t)54 * L.n)) return STK_ENOMEM;
+      hipLaunchKernelGGL(k_level_coef, dim3(blocks(L.n, 256)), dim3(256), 0, c->st, L.geo<float>(c->kn.p2koef),
+                         L.flags, L.lc32);
+    }
   }
   rc = ensure_outer(c, 1);
   if (rc) return rc;
@@ -1857,11 +1926,12 @@ int stk_apply(stk_ctx *c, int l, const double *x, double *y) {
 
 // FP64 coupled sweeps on the reference arrays of level l, then fre0
 int stk_smooth(stk_ctx *c, int l, int sweeps, int ty, int tx, int tz) {
-  return stk_smooth_ex(c, l, sweeps, ty, tx, tz, 1);
+  return stk_smooth_ex(c, l, sweeps, ty, tx, tz, 1, 0);
 }
 
-int stk_smooth_ex(stk_ctx *c, int l, int sweeps, int ty, int tx, int tz, int precision) {
+int stk_smooth_ex(stk_ctx *c, int l, int sweeps, int ty, int tx, int tz, int precision, int velocity_only) {
   if (!c || !c->prepared || l < 0 || l > c->L || sweeps < 0 || precision < 0 || precision > 1) return STK_EINVAL;
+  const int withp = velocity_only ? 0 : 1;
   const bool whole = (ty == 0 && tx == 0 && tz == 0);
   const bool staged = tz < 0;
   if (staged) tz = -tz;
@@ -1881,10 +1951,10 @@ int stk_smooth_ex(stk_ctx *c, int l, int sweeps, int ty, int tx, int tz, int pre
       if (smooth_lds<double>(tile, true) > kLevelLds) return STK_EINVAL;
       const int keep[4] = {L.tune64[0], L.tune64[1], L.tune64[2], L.tune64[3]};
       for (int a = 0; a < 4; a++) L.tune64[a] = tile[a];
-      smooth<double>(c, l, sweeps, nullptr, 1, false);
+      smooth<double>(c, l, sweeps, nullptr, withp, false);
       for (int a = 0; a < 4; a++) L.tune64[a] = keep[a];
     } else {
-      smooth<double>(c, l, sweeps, tile, 1, whole);
+      smooth<double>(c, l, sweeps, tile, withp, whole);
     }
   } else {
     // FP32 arithmetic (the mixed-precision V-cycle's smoother) on rounded copies
@@ -1895,10 +1965,10 @@ int stk_smooth_ex(stk_ctx *c, int l, int sweeps, int ty, int tx, int tz, int pre
       if (smooth_lds<float>(tile, true) > kLevelLds) return STK_EINVAL;
       const int keep[4] = {L.tune32[0], L.tune32[1], L.tune32[2], L.tune32[3]};
       for (int a = 0; a < 4; a++) L.tune32[a] = tile[a];
-      smooth<float>(c, l, sweeps, nullptr, 1, false);
+      smooth<float>(c, l, sweeps, nullptr, withp, false);
       for (int a = 0; a < 4; a++) L.tune32[a] = keep[a];
     } else {
-      smooth<float>(c, l, sweeps, tile, 1, whole);
+      smooth<float>(c, l, sweeps, tile, withp, whole);
     }
     hipLaunchKernelGGL((k_cast<double, double>), dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, L.ref[1], L.b64);
     hipLaunchKernelGGL((k_cast<float, double>), dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, L.x32[L.cur32],

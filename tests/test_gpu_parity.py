@@ -106,14 +106,18 @@ def test_coloured_sweep_matches_oracle(oracle_mod, tile):
         g.close()
 
 
-@pytest.mark.parametrize("tile", [(32, 4, 4), (8, 2, 3), (16, 4, -2), (0, 0, 0)])
-def test_fp32_sweep_matches_fp64_oracle_to_fp32_accuracy(oracle_mod, tile):
+@pytest.mark.parametrize("tile,vonly", [((32, 4, 4), False), ((8, 2, 3), False), ((16, 4, -2), False),
+                                        ((0, 0, 0), False), ((0, 0, 0), True), ((32, 4, 4), True)])
+def test_fp32_sweep_matches_fp64_oracle_to_fp32_accuracy(oracle_mod, tile, vonly):
     """The FP32 smoother of the mixed-precision V-cycle (fused multiply-adds,
     4/3 and 2/3 as constants, hardware reciprocal of the diagonal) makes the
     oracle's FP64 row updates to FP32 accuracy: max error <= 2e-5 of max |x|
     after 3 sweeps.  A wrong coefficient or slot is an O(1) error.  Tile
-    (0, 0, 0) is the LDS-resident whole-level kernel (level 1, 11^3 nodes)."""
+    (0, 0, 0) is the LDS-resident whole-level kernel (level 1, 11^3 nodes);
+    velocity-only on it is the V-cycle's kernel with precomputed rows and
+    composed boundary chains."""
     pb, o, g = _pair(oracle_mod, seed=3)
+    o.set_vonly(vonly)
     try:
         l = 1 if tile == (0, 0, 0) else 0
         X, Y, Z = o.info[l][:3]
@@ -129,7 +133,7 @@ def test_fp32_sweep_matches_fp64_oracle_to_fp32_accuracy(oracle_mod, tile):
             o.sweep_tiles(3, l, Y, X, Z)
         else:
             o.sweep_tiles(3, l, tile[0], tile[1], abs(tile[2]))
-        g.smooth(l, 3, tile=tile, precision=0)
+        g.smooth(l, 3, tile=tile, precision=0, velocity_only=vonly)
         xo = o.arr("sol0")[b4:b4 + n4]
         xg = g.vec_get(S.SOL0, l)
         err = np.abs(xg - xo).max() / np.abs(xo).max()
