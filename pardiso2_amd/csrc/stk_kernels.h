@@ -127,34 +127,36 @@ __global__ void k_bc_rhs(int nbc, const int *__restrict__ r, const double *__res
 // ---------------------------------------------------------------------------
 // Row operator on equation rows: RES -> y = b - A x (gausmat4's residual
 // order, move3mg.c:3937-3958), otherwise y = A x.  withp=0 leaves pressure
-// rows at zero (velocity block).  Boundary rows are done by k_bc_rowop.
+// rows at zero (velocity block).  The four unknowns of a node are written
+// with one vector store; boundary rows get a placeholder 0 there and their
+// values from k_bc_rowop, which always follows.
 // ---------------------------------------------------------------------------
 template <typename T, int C, bool RES>
-__device__ __forceinline__ void rowop_one(const Geo<T> &g, unsigned char f, long n, int m1, int m2, int m3,
-                                          const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ y,
-                                          int withp) {
-  int k = kind_of(f, C);
-  if (k != KIND_EQ || (!withp && C == 0)) {
-    if (k == KIND_NONE || (!withp && C == 0)) y[4 * n + C] = T(0);
-    return;
-  }
+__device__ __forceinline__ T rowop_one(const Geo<T> &g, unsigned char f, long n, int m1, int m2, int m3,
+                                       const T *__restrict__ x, T bv, int withp) {
+  if (kind_of(f, C) != KIND_EQ || (!withp && C == 0)) return T(0);
   T s[18];
   row_coef_c<T, C>(g, m1, m2, m3, s);
-  y[4 * n + C] = row_dot<T, C, RES>(g, s, x, n, RES ? b[4 * n + C] : T(0));
+  return row_dot<T, C, RES>(g, s, x, n, bv);
 }
 
 template <typename T, bool RES>
 __global__ void k_rowop(Geo<T> g, const unsigned char *__restrict__ flags, const T *__restrict__ x,
                         const T *__restrict__ b, T *__restrict__ y, int withp) {
+  using V = typename V4<T>::t;
   long n = xcd_block() * blockDim.x + threadIdx.x;
   if (n >= (long)g.XY * g.Z) return;
   int m1, m2, m3;
   decode(n, g.X, g.Y, g.XY, m1, m2, m3);
   unsigned char f = flags[n];
-  rowop_one<T, 0, RES>(g, f, n, m1, m2, m3, x, b, y, withp);
-  rowop_one<T, 1, RES>(g, f, n, m1, m2, m3, x, b, y, withp);
-  rowop_one<T, 2, RES>(g, f, n, m1, m2, m3, x, b, y, withp);
-  rowop_one<T, 3, RES>(g, f, n, m1, m2, m3, x, b, y, withp);
+  V bn = {T(0), T(0), T(0), T(0)};
+  if (RES) bn = reinterpret_cast<const V *>(b)[n];
+  V out;
+  out.x = rowop_one<T, 0, RES>(g, f, n, m1, m2, m3, x, bn.x, withp);
+  out.y = rowop_one<T, 1, RES>(g, f, n, m1, m2, m3, x, bn.y, withp);
+  out.z = rowop_one<T, 2, RES>(g, f, n, m1, m2, m3, x, bn.z, withp);
+  out.w = rowop_one<T, 3, RES>(g, f, n, m1, m2, m3, x, bn.w, withp);
+  reinterpret_cast<V *>(y)[n] = out;
 }
 
 // boundary rows  x_r - k x_j = b_r  (bonderr, move3mg.c:1682-1729)

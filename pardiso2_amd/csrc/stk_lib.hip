@@ -186,8 +186,8 @@ struct stk_ctx {
   // Schur approximation: level-0 velocity D^-1, work vectors, coarsest K inverse
   double *dinv64 = nullptr, *bfy64 = nullptr, *bfw64 = nullptr;
   float *dinv32 = nullptr, *bfy32 = nullptr, *bfw32 = nullptr;
-  double *kinv64 = nullptr;
-  float *kinv32 = nullptr;
+  double *kinvT64 = nullptr;  // coarsest K inverse, transposed: [j * nkc + i] = inverse(i, j)
+  float *kinvT32 = nullptr;
   int *kcell = nullptr;
   int nkc = 0;
   bool ktabs = false;
@@ -572,15 +572,18 @@ template <> double *bfw<double>(stk_ctx *c) { return c->bfw64; }
 template <typename T> const T *dinvt(stk_ctx *c);
 template <> const float *dinvt<float>(stk_ctx *c) { return c->dinv32; }
 template <> const double *dinvt<double>(stk_ctx *c) { return c->dinv64; }
-template <typename T> const T *kinv(stk_ctx *c);
-template <> const float *kinv<float>(stk_ctx *c) { return c->kinv32; }
-template <> const double *kinv<double>(stk_ctx *c) { return c->kinv64; }
+template <typename T> const T *kinvT(stk_ctx *c);
+template <> const float *kinvT<float>(stk_ctx *c) { return c->kinvT32; }
+template <> const double *kinvT<double>(stk_ctx *c) { return c->kinvT64; }
 template <typename T> const T *iAt(const Level &L, int a);
 template <> const float *iAt<float>(const Level &L, int a) { return L.iA32[a]; }
 template <> const double *iAt<double>(const Level &L, int a) { return L.iA64[a]; }
 template <typename T> const T *iBt(const Level &L, int a);
 template <> const float *iBt<float>(const Level &L, int a) { return L.iB32[a]; }
 template <> const double *iBt<double>(const Level &L, int a) { return L.iB64[a]; }
+
+// levels of at most this many cells finish the K-cycle in one workgroup
+const long kKTailCells = (long)kTailPT * 1024;
 
 KAgg kagg(const Level &C) {
   KAgg t;
@@ -592,9 +595,33 @@ KAgg kagg(const Level &C) {
 template <typename T>
 void kcycle(stk_ctx *c, int l, int nu, T scale) {
   Level &V = c->lv[l];
+  // the remaining levels are small: one workgroup runs them without launches
+  int small = 0;
+  while (l + small < c->L && c->lv[l + small].n <= kKTailCells) small++;
+  if (l + small == c->L && small <= kKTail && c->nkc <= kTailCoarse) {
+    KTail<T> t;
+    t.nlev = small; t.nu = nu; t.nkc = c->nkc; t.scale = scale; t.kcell = c->kcell; t.kinvT = kinvT<T>(c);
+    for (int i = 0; i <= small; i++) {
+      Level &W = c->lv[l + i];
+      KTailLevel<T> &d = t.lv[i];
+      d.X = W.X; d.Y = W.Y; d.Z = W.Z;
+      d.k = kplanes<T>(W);
+      d.b = kb<T>(W); d.x = kx<T>(W); d.r = kr<T>(W);
+      d.agg = i > 0 ? kagg(W) : KAgg{};
+      d.px = W.kpar[0]; d.py = W.kpar[1]; d.pz = W.kpar[2];
+    }
+    switch (small) {
+      case 0: hipLaunchKernelGGL((k_ktail<T, 0>), dim3(1), dim3(1024), 0, c->st, t); break;
+      case 1: hipLaunchKernelGGL((k_ktail<T, 1>), dim3(1), dim3(1024), 0, c->st, t); break;
+      case 2: hipLaunchKernelGGL((k_ktail<T, 2>), dim3(1), dim3(1024), 0, c->st, t); break;
+      case 3: hipLaunchKernelGGL((k_ktail<T, 3>), dim3(1), dim3(1024), 0, c->st, t); break;
+      default: hipLaunchKernelGGL((k_ktail<T, 4>), dim3(1), dim3(1024), 0, c->st, t); break;
+    }
+    return;
+  }
   if (l == c->L) {
-    hipLaunchKernelGGL(k_kcoarse<T>, dim3(blocks(c->nkc, 64)), dim3(64), 0, c->st, c->nkc, c->kcell, kinv<T>(c),
-                       kb<T>(V), kx<T>(V));
+    hipLaunchKernelGGL(k_kcoarse<T>, dim3(blocks(c->nkc, 64)), dim3(64), c->nkc * sizeof(T), c->st, c->nkc,
+                       c->kcell, kinvT<T>(c), kb<T>(V), kx<T>(V));
     return;
   }
   const KPlanesC<T> k = kplanes<T>(V);
@@ -994,7 +1021,7 @@ void stk_destroy(stk_ctx *c) {
     for (int a = 0; a < 3; a++) { dfree(L.klo[a]); dfree(L.kcnt[a]); dfree(L.kpar[a]); }
   }
   dfree(c->dinv64); dfree(c->bfy64); dfree(c->bfw64); dfree(c->dinv32); dfree(c->bfy32); dfree(c->bfw32);
-  dfree(c->kinv64); dfree(c->kinv32); dfree(c->kcell);
+  dfree(c->kcell); dfree(c->kinvT64); dfree(c->kinvT32);
   dfree(c->nu); dfree(c->ro); dfree(c->nd); dfree(c->nxx); dfree(c->nxy); dfree(c->nxz); dfree(c->nyz);
   dfree(c->ainv64); dfree(c->ainvP64); dfree(c->ainv32); dfree(c->cact);
   dfree(c->X); dfree(c->B); dfree(c->R); dfree(c->W2); dfree(c->T1); dfree(c->red); dfree(c->scal);
@@ -1304,14 +1331,23 @@ static int build_schur(stk_ctx *c) {
     }
   }
   std::vector<float> I32(I.begin(), I.end());
-  if (N != c->nkc || !c->kinv64) {  // same size: refill in place (captured graphs keep their pointers)
-    dfree(c->kinv64); dfree(c->kinv32); dfree(c->kcell);
-    if (dalloc(&c->kinv64, (size_t)N * N) || dalloc(&c->kinv32, (size_t)N * N) || dalloc(&c->kcell, N))
+  if (N != c->nkc || !c->kinvT64) {  // same size: refill in place (captured graphs keep their pointers)
+    dfree(c->kcell); dfree(c->kinvT64); dfree(c->kinvT32);
+    if (dalloc(&c->kcell, N) || dalloc(&c->kinvT64, (size_t)N * N) || dalloc(&c->kinvT32, (size_t)N * N))
       return STK_ENOMEM;
     c->graph_ok = false;
   }
-  H2D(c, c->kinv64, I.data(), (size_t)N * N * 8);
-  H2D(c, c->kinv32, I32.data(), (size_t)N * N * 4);
+  {
+    std::vector<double> IT((size_t)N * N);
+    std::vector<float> IT32((size_t)N * N);
+    for (long i = 0; i < N; i++)
+      for (long j = 0; j < N; j++) {
+        IT[j * N + i] = I[i * N + j];
+        IT32[j * N + i] = I32[i * N + j];
+      }
+    H2D(c, c->kinvT64, IT.data(), (size_t)N * N * 8);
+    H2D(c, c->kinvT32, IT32.data(), (size_t)N * N * 4);
+  }
   H2D(c, c->kcell, cell.data(), (size_t)N * 4);
   c->nkc = N;
   return STK_OK;

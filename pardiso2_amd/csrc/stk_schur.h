@@ -171,16 +171,16 @@ __global__ void k_kagg(int Xf, int Yf, int Xc, int Yc, int Zc, KAgg t, KPlanesC<
 }
 
 // one red/black Gauss-Seidel half-sweep of K x = b on cells of one colour
+// new value of cell n (false: the cell keeps its value)
 template <typename T>
-__global__ void k_kgs(int X, int Y, int Z, KPlanesC<T> k, const T *__restrict__ b, T *x, int colour) {
-  long n = xcd_block() * blockDim.x + threadIdx.x;
+__device__ __forceinline__ bool kgs_val(int X, int Y, int Z, const KPlanesC<T> &k, const T *__restrict__ b,
+                                        const T *x, int colour, long n, T &out) {
   const int XY = X * Y;
-  if (n >= (long)XY * Z) return;
   const T d = k.p[0][n];
-  if (d == T(0)) return;
+  if (d == T(0)) return false;
   int m1, m2, m3;
   decode(n, X, Y, XY, m1, m2, m3);
-  if (((m1 + m2 + m3) & 1) != colour) return;
+  if (((m1 + m2 + m3) & 1) != colour) return false;
   T acc = b[n];
   if (m1 > 0) acc -= k.p[1][n] * x[n - Y];
   if (m1 < X - 1) acc -= k.p[2][n] * x[n + Y];
@@ -188,17 +188,31 @@ __global__ void k_kgs(int X, int Y, int Z, KPlanesC<T> k, const T *__restrict__ 
   if (m2 < Y - 1) acc -= k.p[4][n] * x[n + 1];
   if (m3 > 0) acc -= k.p[5][n] * x[n - XY];
   if (m3 < Z - 1) acc -= k.p[6][n] * x[n + XY];
-  x[n] = acc / d;
+  out = acc / d;
+  return true;
+}
+
+template <typename T>
+__device__ __forceinline__ void kgs_cell(int X, int Y, int Z, const KPlanesC<T> &k, const T *__restrict__ b, T *x,
+                                         int colour, long n) {
+  T v;
+  if (kgs_val<T>(X, Y, Z, k, b, x, colour, n, v)) x[n] = v;
+}
+
+template <typename T>
+__global__ void k_kgs(int X, int Y, int Z, KPlanesC<T> k, const T *__restrict__ b, T *x, int colour) {
+  long n = xcd_block() * blockDim.x + threadIdx.x;
+  if (n >= (long)X * Y * Z) return;
+  kgs_cell<T>(X, Y, Z, k, b, x, colour, n);
 }
 
 // r = b - K x
 template <typename T>
-__global__ void k_kres(int X, int Y, int Z, KPlanesC<T> k, const T *__restrict__ b, const T *__restrict__ x, T *r) {
-  long n = xcd_block() * blockDim.x + threadIdx.x;
+__device__ __forceinline__ T kres_val(int X, int Y, int Z, const KPlanesC<T> &k, const T *__restrict__ b,
+                                      const T *__restrict__ x, long n) {
   const int XY = X * Y;
-  if (n >= (long)XY * Z) return;
   const T d = k.p[0][n];
-  if (d == T(0)) { r[n] = T(0); return; }
+  if (d == T(0)) return T(0);
   int m1, m2, m3;
   decode(n, X, Y, XY, m1, m2, m3);
   T acc = b[n] - d * x[n];
@@ -208,15 +222,27 @@ __global__ void k_kres(int X, int Y, int Z, KPlanesC<T> k, const T *__restrict__
   if (m2 < Y - 1) acc -= k.p[4][n] * x[n + 1];
   if (m3 > 0) acc -= k.p[5][n] * x[n - XY];
   if (m3 < Z - 1) acc -= k.p[6][n] * x[n + XY];
-  r[n] = acc;
+  return acc;
+}
+
+template <typename T>
+__device__ __forceinline__ void kres_cell(int X, int Y, int Z, const KPlanesC<T> &k, const T *__restrict__ b,
+                                          const T *__restrict__ x, T *r, long n) {
+  r[n] = kres_val<T>(X, Y, Z, k, b, x, n);
+}
+
+template <typename T>
+__global__ void k_kres(int X, int Y, int Z, KPlanesC<T> k, const T *__restrict__ b, const T *__restrict__ x, T *r) {
+  long n = xcd_block() * blockDim.x + threadIdx.x;
+  if (n >= (long)X * Y * Z) return;
+  kres_cell<T>(X, Y, Z, k, b, x, r, n);
 }
 
 // b_c = sum of the aggregate's fine residuals, x_c = 0
 template <typename T>
-__global__ void k_krestrict(int Xf, int Yf, int Xc, int Yc, int Zc, KAgg t, const T *__restrict__ rf, T *bc, T *xc) {
-  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void krestrict_cell(int Xf, int Yf, int Xc, int Yc, const KAgg &t,
+                                               const T *__restrict__ rf, T *bc, T *xc, long n) {
   const int XYc = Xc * Yc, XYf = Xf * Yf;
-  if (n >= (long)XYc * Zc) return;
   int cx, cy, cz;
   decode(n, Xc, Yc, XYc, cx, cy, cz);
   const int lx = t.lo[0][cx], ly = t.lo[1][cy], lz = t.lo[2][cz];
@@ -229,29 +255,207 @@ __global__ void k_krestrict(int Xf, int Yf, int Xc, int Yc, int Zc, KAgg t, cons
   xc[n] = T(0);
 }
 
+template <typename T>
+__global__ void k_krestrict(int Xf, int Yf, int Xc, int Yc, int Zc, KAgg t, const T *__restrict__ rf, T *bc, T *xc) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= (long)Xc * Yc * Zc) return;
+  krestrict_cell<T>(Xf, Yf, Xc, Yc, t, rf, bc, xc, n);
+}
+
 // x_f += scale * x_c(parent) on fine cells with an equation
+template <typename T>
+__device__ __forceinline__ bool kprolong_val(int Xf, int Yf, int Xc, int Yc, const int *__restrict__ px,
+                                             const int *__restrict__ py, const int *__restrict__ pz,
+                                             const T *__restrict__ kdiag, const T *__restrict__ xc, const T *xf,
+                                             T scale, long n, T &out) {
+  if (kdiag[n] == T(0)) return false;
+  const int XYf = Xf * Yf;
+  int m1, m2, m3;
+  decode(n, Xf, Yf, XYf, m1, m2, m3);
+  out = xf[n] + scale * xc[(long)pz[m3] * Xc * Yc + (long)px[m1] * Yc + py[m2]];
+  return true;
+}
+
+template <typename T>
+__device__ __forceinline__ void kprolong_cell(int Xf, int Yf, int Xc, int Yc, const int *__restrict__ px,
+                                              const int *__restrict__ py, const int *__restrict__ pz,
+                                              const T *__restrict__ kdiag, const T *__restrict__ xc, T *xf, T scale,
+                                              long n) {
+  T v;
+  if (kprolong_val<T>(Xf, Yf, Xc, Yc, px, py, pz, kdiag, xc, xf, scale, n, v)) xf[n] = v;
+}
+
 template <typename T>
 __global__ void k_kprolong(int Xf, int Yf, int Zf, int Xc, int Yc, const int *__restrict__ px,
                            const int *__restrict__ py, const int *__restrict__ pz, const T *__restrict__ kdiag,
                            const T *__restrict__ xc, T *xf, T scale) {
   long n = xcd_block() * blockDim.x + threadIdx.x;
-  const int XYf = Xf * Yf;
-  if (n >= (long)XYf * Zf) return;
-  if (kdiag[n] == T(0)) return;
-  int m1, m2, m3;
-  decode(n, Xf, Yf, XYf, m1, m2, m3);
-  xf[n] += scale * xc[(long)pz[m3] * Xc * Yc + (long)px[m1] * Yc + py[m2]];
+  if (n >= (long)Xf * Yf * Zf) return;
+  kprolong_cell<T>(Xf, Yf, Xc, Yc, px, py, pz, kdiag, xc, xf, scale, n);
 }
 
-// coarsest level: x[cell[i]] = sum_j inv[i][j] b[cell[j]]
+// coarsest level: x[cell[i]] = sum_j inv[i][j] b[cell[j]], from the
+// transposed inverse invT[j][i] (lanes read neighbouring rows) and b of the
+// active cells staged in LDS (nc values)
 template <typename T>
-__global__ void k_kcoarse(int nc, const int *__restrict__ cell, const T *__restrict__ inv, const T *__restrict__ b,
+__global__ void k_kcoarse(int nc, const int *__restrict__ cell, const T *__restrict__ invT, const T *__restrict__ b,
                           T *x) {
+  extern __shared__ unsigned char smem_raw[];
+  T *sb = reinterpret_cast<T *>(smem_raw);
+  for (int j = threadIdx.x; j < nc; j += blockDim.x) sb[j] = b[cell[j]];
+  __syncthreads();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nc) return;
   T acc = T(0);
-  for (int j = 0; j < nc; j++) acc += inv[(long)i * nc + j] * b[cell[j]];
+  for (int j = 0; j < nc; j++) acc += invT[(long)j * nc + i] * sb[j];
   x[cell[i]] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// The small levels of the K-cycle in one workgroup: the same cell updates,
+// transfers and coarsest solve as the kernels above, level after level, with
+// workgroup barriers where the separate kernels had launch boundaries.
+// lv[0 .. nlev-1] are the smoothed levels, lv[nlev] the coarsest; agg of
+// lv[i+1] groups the cells of lv[i], px/py/pz of lv[i] name the parents.
+// A thread owns cells tid, tid + 1024, ... (at most kTailPT); it computes
+// their new values kTailB at a time before storing them, so their loads
+// overlap (a phase reads only cells it does not write).  The coarsest product reads the
+// transposed inverse (one thread per row, coalesced) in k_kcoarse's order.
+// ---------------------------------------------------------------------------
+constexpr int kKTail = 4;        // smoothed levels in the tail
+constexpr int kTailPT = 2;       // cells per thread (levels of at most 2048 cells: one workgroup is
+                                 // faster than launches only while a phase is about one memory round trip)
+constexpr int kTailB = 2;        // cells computed together
+constexpr int kTailCoarse = 2048;
+template <typename T> struct KTailLevel {
+  int X, Y, Z;
+  KPlanesC<T> k;
+  T *b, *x, *r;
+  KAgg agg;
+  const int *px, *py, *pz;
+};
+template <typename T> struct KTail {
+  int nlev, nu, nkc;
+  T scale;
+  const int *kcell;
+  const T *kinvT;  // [j * nkc + i] = inverse(i, j)
+  KTailLevel<T> lv[kKTail + 1];
+};
+
+template <typename T>
+__device__ __forceinline__ void tail_kgs(const KTailLevel<T> &V, int col) {
+  const long n = (long)V.X * V.Y * V.Z;
+  // thread index the compiler may not assume constant across phases: keeps
+  // it from holding every cell's plane addresses in registers for the kernel
+  long t0 = threadIdx.x;
+  asm volatile("" : "+v"(t0));
+#pragma unroll 1
+  for (int q0 = 0; q0 < kTailPT; q0 += kTailB) {
+    T v[kTailB];
+    unsigned ok = 0;
+#pragma unroll
+    for (int q = 0; q < kTailB; q++) {
+      const long u = t0 + (long)(q0 + q) * blockDim.x;
+      if (u < n && kgs_val<T>(V.X, V.Y, V.Z, V.k, V.b, V.x, col, u, v[q])) ok |= 1u << q;
+    }
+#pragma unroll
+    for (int q = 0; q < kTailB; q++)
+      if (ok & (1u << q)) V.x[t0 + (long)(q0 + q) * blockDim.x] = v[q];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void tail_kres(const KTailLevel<T> &V) {
+  const long n = (long)V.X * V.Y * V.Z;
+  // thread index the compiler may not assume constant across phases: keeps
+  // it from holding every cell's plane addresses in registers for the kernel
+  long t0 = threadIdx.x;
+  asm volatile("" : "+v"(t0));
+#pragma unroll 1
+  for (int q0 = 0; q0 < kTailPT; q0 += kTailB) {
+    T v[kTailB];
+#pragma unroll
+    for (int q = 0; q < kTailB; q++) {
+      const long u = t0 + (long)(q0 + q) * blockDim.x;
+      if (u < n) v[q] = kres_val<T>(V.X, V.Y, V.Z, V.k, V.b, V.x, u);
+    }
+#pragma unroll
+    for (int q = 0; q < kTailB; q++) {
+      const long u = t0 + (long)(q0 + q) * blockDim.x;
+      if (u < n) V.r[u] = v[q];
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void tail_kprolong(const KTailLevel<T> &V, const KTailLevel<T> &C, T scale) {
+  const long n = (long)V.X * V.Y * V.Z;
+  // thread index the compiler may not assume constant across phases: keeps
+  // it from holding every cell's plane addresses in registers for the kernel
+  long t0 = threadIdx.x;
+  asm volatile("" : "+v"(t0));
+#pragma unroll 1
+  for (int q0 = 0; q0 < kTailPT; q0 += kTailB) {
+    T v[kTailB];
+    unsigned ok = 0;
+#pragma unroll
+    for (int q = 0; q < kTailB; q++) {
+      const long u = t0 + (long)(q0 + q) * blockDim.x;
+      if (u < n && kprolong_val<T>(V.X, V.Y, C.X, C.Y, V.px, V.py, V.pz, V.k.p[0], C.x, V.x, scale, u, v[q]))
+        ok |= 1u << q;
+    }
+#pragma unroll
+    for (int q = 0; q < kTailB; q++)
+      if (ok & (1u << q)) V.x[t0 + (long)(q0 + q) * blockDim.x] = v[q];
+  }
+}
+
+// NL = t.nlev as a constant, so that every lv[] index is a constant and the
+// argument block stays in scalar registers
+template <typename T, int NL>
+__global__ void __launch_bounds__(1024) k_ktail(KTail<T> t) {
+  __shared__ T sb[kTailCoarse];
+  const int tid = threadIdx.x, nt = blockDim.x;
+#pragma unroll
+  for (int i = 0; i < NL; i++) {
+    const KTailLevel<T> &V = t.lv[i];
+    const long n = (long)V.X * V.Y * V.Z;
+#pragma unroll 1
+    for (int s = 0; s < t.nu; s++)
+      for (int col = 0; col < 2; col++) {
+        tail_kgs<T>(V, col);
+        __syncthreads();
+      }
+    tail_kres<T>(V);
+    __syncthreads();
+    const KTailLevel<T> &C = t.lv[i + 1];
+    const long nc = (long)C.X * C.Y * C.Z;
+    for (long u = tid; u < nc; u += nt) krestrict_cell<T>(V.X, V.Y, C.X, C.Y, C.agg, V.r, C.b, C.x, u);
+    __syncthreads();
+  }
+  {
+    const KTailLevel<T> &C = t.lv[NL];
+    for (int j = tid; j < t.nkc; j += nt) sb[j] = C.b[t.kcell[j]];
+    __syncthreads();
+    for (int row = tid; row < t.nkc; row += nt) {
+      T acc = T(0);
+      for (int j = 0; j < t.nkc; j++) acc += t.kinvT[(long)j * t.nkc + row] * sb[j];
+      C.x[t.kcell[row]] = acc;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = NL - 1; i >= 0; i--) {
+    const KTailLevel<T> &V = t.lv[i], &C = t.lv[i + 1];
+    tail_kprolong<T>(V, C, t.scale);
+    __syncthreads();
+#pragma unroll 1
+    for (int s = 0; s < t.nu; s++)
+      for (int col = 1; col >= 0; col--) {
+        tail_kgs<T>(V, col);
+        __syncthreads();
+      }
+  }
 }
 
 }  // namespace stk
