@@ -1103,7 +1103,7 @@ __global__ void __launch_bounds__(256) k_dots2(long n4, const double *__restrict
   const int nd = 2 * k + 2;
   for (int i = threadIdx.x; i < (2 * kMaxDirs + 2) * 4; i += blockDim.x) acc[i >> 2][i & 3] = 0.0;
   __syncthreads();
-  constexpr int E = 8;  // double2 per thread per slice
+  constexpr int E = 4;  // double2 per thread per slice; two basis vectors per step
   const long n2 = n4 >> 1;
   const double2 *__restrict__ W2 = reinterpret_cast<const double2 *>(w2);
   const double2 *__restrict__ U = reinterpret_cast<const double2 *>(u);
@@ -1126,23 +1126,36 @@ __global__ void __launch_bounds__(256) k_dots2(long n4, const double *__restrict
       suu += __shfl_xor(suu, off);
     }
     if (lane == 0) { acc[2 * k][wave] += suw; acc[2 * k + 1][wave] += suu; }
-    for (int i = 0; i < k; i++) {
-      const double2 *__restrict__ v = reinterpret_cast<const double2 *>(V.p[i]);
-      double sa = 0.0, sb = 0.0;
+    for (int i = 0; i < k; i += 2) {
+      const bool two = i + 1 < k;
+      const double2 *__restrict__ v0 = reinterpret_cast<const double2 *>(V.p[i]);
+      const double2 *__restrict__ v1 = reinterpret_cast<const double2 *>(V.p[two ? i + 1 : i]);
+      double2 a0[E], a1[E];
 #pragma unroll
       for (int e = 0; e < E; e++) {
         const long p = base + threadIdx.x + 256L * e;
-        if (p < n2) {
-          const double2 vv = v[p];
-          sa += wu[e].x * vv.x + wu[e].y * vv.y;
-          sb += ww[e].x * vv.x + ww[e].y * vv.y;
-        }
+        a0[e] = make_double2(0.0, 0.0);
+        a1[e] = a0[e];
+        if (p < n2) { a0[e] = v0[p]; a1[e] = v1[p]; }
+      }
+      double sa0 = 0.0, sb0 = 0.0, sa1 = 0.0, sb1 = 0.0;
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        sa0 += wu[e].x * a0[e].x + wu[e].y * a0[e].y;
+        sb0 += ww[e].x * a0[e].x + ww[e].y * a0[e].y;
+        sa1 += wu[e].x * a1[e].x + wu[e].y * a1[e].y;
+        sb1 += ww[e].x * a1[e].x + ww[e].y * a1[e].y;
       }
       for (int off = 32; off > 0; off >>= 1) {
-        sa += __shfl_xor(sa, off);
-        sb += __shfl_xor(sb, off);
+        sa0 += __shfl_xor(sa0, off);
+        sb0 += __shfl_xor(sb0, off);
+        sa1 += __shfl_xor(sa1, off);
+        sb1 += __shfl_xor(sb1, off);
       }
-      if (lane == 0) { acc[i][wave] += sa; acc[k + i][wave] += sb; }
+      if (lane == 0) {
+        acc[i][wave] += sa0; acc[k + i][wave] += sb0;
+        if (two) { acc[i + 1][wave] += sa1; acc[k + i + 1][wave] += sb1; }
+      }
     }
   }
   __syncthreads();
@@ -1167,6 +1180,7 @@ __global__ void __launch_bounds__(256) k_update2(long n4, int k, const double *_
   double ss = 0.0;
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n2; p += (long)gridDim.x * blockDim.x) {
     double2 uu = U[p], wv = Wv[p];
+#pragma unroll 8
     for (int i = 0; i < k; i++) {
       const double2 vv = reinterpret_cast<const double2 *>(V.p[i])[p];
       uu.x -= sc[i] * vv.x;
