@@ -215,7 +215,7 @@ def bench_served(args):
     serve.run_stream(srv, warm, payloads, seed=args.seed)
     srv.done.clear()
     log(f"replaying {len(trace)} arrivals over {args.duration:.0f} s at {args.rate:.0f}/s")
-    done, t0 = serve.run_stream(srv, trace, payloads, seed=args.seed + 1)
+    done, t0 = serve.run_stream(srv, trace, payloads, seed=args.seed + 1, progress=log)
     srv.stop()
     lat = np.array([(r.done - r.arrival) * 1e3 for r in done])
     makespan = max(r.done for r in done) - t0

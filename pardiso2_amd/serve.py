@@ -194,20 +194,34 @@ def poisson_trace(rate: float, duration: float, sizes, weights, seed: int):
         out.append((t, int(rng.choice(sizes, p=np.asarray(weights) / np.sum(weights)))))
 
 
-def run_stream(server: Server, trace, payloads: dict, seed: int = 0):
+def run_stream(server: Server, trace, payloads: dict, seed: int = 0, progress=None, every: float = 10.0):
     """Replay an arrival trace against a started server; returns the finished
-    requests.  payloads[n] is a list of payload dicts for size n."""
+    requests.  payloads[n] is a list of payload dicts for size n.  progress,
+    if given, is called with a status line every `every` seconds."""
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
+    last = t0
+
+    def report():
+        nonlocal last
+        now = time.perf_counter()
+        if progress is not None and now - last >= every:
+            last = now
+            with server.lock:
+                nd = len(server.done)
+            progress(f"{now - t0:.0f} s: {nd} of {len(trace)} requests done")
+
     for rid, (ta, n) in enumerate(trace):
         now = time.perf_counter() - t0
         if ta > now:
             time.sleep(ta - now)
         p = payloads[n][int(rng.integers(len(payloads[n])))]
         server.submit(Request(arrival=t0 + ta, rid=rid, n=n, payload=p))
+        report()
     while True:
         with server.lock:
             if len(server.done) >= len(trace):
                 break
+        report()
         time.sleep(0.01)
     return sorted(server.done, key=lambda r: r.rid), t0
