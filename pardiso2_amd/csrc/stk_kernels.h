@@ -833,61 +833,92 @@ __global__ void k_dense_rows(Geo<double> g, const unsigned char *__restrict__ fl
   }
 }
 
-// pivot search, swap, normalise of step k on [A | B]; saves column k factors
-__global__ void __launch_bounds__(1024) k_gj_pivot(double *A, double *B, int n, int k, double *fac,
-                                                   int *sing) {
-  __shared__ double bv[1024];
-  __shared__ int bi[1024];
-  double best = -1.0;
-  int bidx = n;
-  for (int i = k + threadIdx.x; i < n; i += blockDim.x) {
-    double v = fabs(A[(long)i * n + k]);
-    if (v > best || (v == best && i < bidx)) { best = v; bidx = i; }
-  }
-  bv[threadIdx.x] = best;
-  bi[threadIdx.x] = bidx;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      double v = bv[threadIdx.x + s];
-      int j = bi[threadIdx.x + s];
-      if (v > bv[threadIdx.x] || (v == bv[threadIdx.x] && j < bi[threadIdx.x])) {
-        bv[threadIdx.x] = v;
-        bi[threadIdx.x] = j;
-      }
-    }
-    __syncthreads();
-  }
-  int p = bi[0];
-  if (bv[0] == 0.0) {
-    if (threadIdx.x == 0) *sing = k + 1;
-    return;
-  }
-  if (p != k)
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-      double t = A[(long)k * n + j]; A[(long)k * n + j] = A[(long)p * n + j]; A[(long)p * n + j] = t;
-      t = B[(long)k * n + j]; B[(long)k * n + j] = B[(long)p * n + j]; B[(long)p * n + j] = t;
-    }
-  __syncthreads();
-  double piv = A[(long)k * n + k];
-  __syncthreads();
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    A[(long)k * n + j] /= piv;
-    B[(long)k * n + j] /= piv;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < n; i += blockDim.x) fac[i] = (i == k) ? 0.0 : A[(long)i * n + k];
+// Gauss-Jordan inverse with partial pivoting and no row swaps, two light
+// launches per column k.  k_gj_select: the pivot p is the unused row with
+// the largest |A[i][k]| (lowest index on ties), from the per-row candidates
+// cand[i] the previous elimination left (step 0 reads column 0); it records
+// p_k and d_k = A[p][k].  k_gj_elim: one thread per element of rows i != p
+// applies A[i][j>k] -= f A[p][j] and B[i][:] -= f B[p][:], f = A[i][k] / d_k,
+// and the thread of column k+1 leaves cand[i] = |A[i][k+1]| (-1 for used
+// rows).  Row p is only read in its step.  Afterwards row k of the inverse is B[p_k][:] / d_k (k_gj_final).
+__device__ __forceinline__ void gj_better(double v, int i, double &bv, int &bi) {
+  if (v > bv || (v == bv && i < bi)) { bv = v; bi = i; }
 }
 
-__global__ void k_gj_elim(double *A, double *B, int n, int k, const double *__restrict__ fac) {
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  int i = (int)(t / n), j = (int)(t % n);
-  if (i >= n) return;
-  double f = fac[i];
-  if (f == 0.0) return;
-  if (j > k) A[(long)i * n + j] -= f * A[(long)k * n + j];
-  B[(long)i * n + j] -= f * B[(long)k * n + j];
-  if (j == k) A[(long)i * n + k] = 0.0;
+__global__ void __launch_bounds__(1024) k_gj_select(const double *__restrict__ A, int n, int k, int *used, int *prow,
+                                                    double *pval, const double *__restrict__ cand, int *sing) {
+  __shared__ double rv[1024];
+  __shared__ int ri[1024];
+  double best = -1.0;
+  int bidx = n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = (k == 0) ? fabs(A[(long)i * n]) : cand[i];
+    gj_better(v, i, best, bidx);
+  }
+  rv[threadIdx.x] = best;
+  ri[threadIdx.x] = bidx;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) gj_better(rv[threadIdx.x + s], ri[threadIdx.x + s], rv[threadIdx.x], ri[threadIdx.x]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int p = ri[0];
+    if (!(rv[0] > 0.0) || p >= n) {
+      if (!*sing) *sing = k + 1;
+      prow[k] = -1;
+      return;
+    }
+    prow[k] = p;
+    pval[k] = A[(long)p * n + k];
+    used[p] = 1;
+  }
+}
+
+__global__ void k_gj_elim(double *A, double *B, int n, int k, const int *__restrict__ used,
+                          const int *__restrict__ prow, const double *__restrict__ pval, double *cand) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long nn = (long)n * n;
+  if (t >= 2 * nn) return;
+  const int p = prow[k];
+  if (p < 0) return;  // singular: no further updates
+  const bool inA = t < nn;
+  const long e = inA ? t : t - nn;
+  const int i = (int)(e / n), j = (int)(e - (long)i * n);
+  if (inA && j <= k) return;  // columns up to k of A are no longer needed
+  if (i == p) {
+    if (inA && j == k + 1) cand[i] = -1.0;
+    return;
+  }
+  const double aik = A[(long)i * n + k];  // column k is not written in this step
+  if (aik == 0.0) {  // nothing to eliminate in this row (most rows: the matrix is sparse)
+    if (inA && j == k + 1) cand[i] = used[i] ? -1.0 : fabs(A[e]);
+    return;
+  }
+  const double f = aik / pval[k];
+  if (inA) {
+    const double v = A[e] - f * A[(long)p * n + j];
+    A[e] = v;
+    if (j == k + 1) cand[i] = used[i] ? -1.0 : fabs(v);
+  } else {
+    B[e] -= f * B[(long)p * n + j];
+  }
+}
+
+// inv[k][:] = B[p_k][:] / A[p_k][k]
+__global__ void k_gj_final(const double *__restrict__ B, int n, const int *__restrict__ prow,
+                           const double *__restrict__ pval, double *inv) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)n * n) return;
+  const int k = (int)(t / n), j = (int)(t % n);
+  inv[t] = B[(long)prow[k] * n + j] / pval[k];
+}
+
+// B = identity
+__global__ void k_eye(double *B, int n) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)n * n) return;
+  B[t] = (t / n == t % n) ? 1.0 : 0.0;
 }
 
 // Compact dense rows of the coarsest velocity block: row i is unknown act[i]

@@ -743,6 +743,33 @@ int ensure_outer(stk_ctx *c, int m) {
 // ---------------------------------------------------------------------------
 // coarsest-level inverse (Gauss-Jordan with partial pivoting, FP64)
 // ---------------------------------------------------------------------------
+// inv = A^-1 for a device n x n row-major A (overwritten); STK_ESINGULAR if a
+// pivot column has no nonzero left.  Two launches per column.
+int gj_invert(stk_ctx *c, double *A, int n, double *inv) {
+  double *B = nullptr, *pval = nullptr, *cand = nullptr;
+  int *used = nullptr, *prow = nullptr, *sing = nullptr;
+  if (dalloc(&B, (size_t)n * n) || dalloc(&pval, n) || dalloc(&cand, n) || dalloc(&used, n) || dalloc(&prow, n) ||
+      dalloc(&sing, 1)) {
+    dfree(B); dfree(pval); dfree(cand); dfree(used); dfree(prow); dfree(sing);
+    return STK_ENOMEM;
+  }
+  hipLaunchKernelGGL(k_eye, dim3(blocks((long)n * n, 256)), dim3(256), 0, c->st, B, n);
+  HIPCHK(hipMemsetAsync(used, 0, (size_t)n * 4, c->st));
+  HIPCHK(hipMemsetAsync(sing, 0, 4, c->st));
+  for (int k = 0; k < n; k++) {
+    hipLaunchKernelGGL(k_gj_select, dim3(1), dim3(1024), 0, c->st, A, n, k, used, prow, pval, cand, sing);
+    hipLaunchKernelGGL(k_gj_elim, dim3(blocks(2L * n * n, 256)), dim3(256), 0, c->st, A, B, n, k, used, prow, pval,
+                       cand);
+  }
+  int hs = 0;
+  D2H(c, &hs, sing, 4);
+  if (!hs)
+    hipLaunchKernelGGL(k_gj_final, dim3(blocks((long)n * n, 256)), dim3(256), 0, c->st, B, n, prow, pval, inv);
+  HIPCHK(hipStreamSynchronize(c->st));
+  dfree(B); dfree(pval); dfree(cand); dfree(used); dfree(prow); dfree(sing);
+  return hs ? STK_ESINGULAR : (hipGetLastError() == hipSuccess ? STK_OK : STK_EHIP);
+}
+
 // velocity block of the coarsest level: Gauss-Jordan on the compact system
 // of velocity unknowns with an equation or boundary row (N x N, N < 4 mgn)
 int coarse_inverse_v(stk_ctx *c) {
@@ -760,69 +787,45 @@ int coarse_inverse_v(stk_ctx *c) {
   if (N == 0) return STK_EINVAL;
   int *dact = nullptr, *didx = nullptr;
   if (upload(&dact, act) || upload(&didx, idx)) return STK_ENOMEM;
-  double *A, *Bm, *fac;
-  int *sing;
-  if (dalloc(&A, (size_t)N * N) || dalloc(&Bm, (size_t)N * N) || dalloc(&fac, N) || dalloc(&sing, 1))
-    return STK_ENOMEM;
+  double *A = nullptr, *inv = nullptr;
+  if (dalloc(&A, (size_t)N * N) || dalloc(&inv, (size_t)N * N)) { dfree(A); dfree(dact); dfree(didx); return STK_ENOMEM; }
   HIPCHK(hipMemsetAsync(A, 0, (size_t)N * N * 8, c->st));
-  HIPCHK(hipMemsetAsync(sing, 0, 4, c->st));
-  std::vector<double> I((size_t)N * N, 0.0);
-  for (int i = 0; i < N; i++) I[(size_t)i * N + i] = 1.0;
-  HIPCHK(hipMemcpyAsync(Bm, I.data(), (size_t)N * N * 8, hipMemcpyHostToDevice, c->st));
   Geo<double> g = C.geo<double>(c->kn.p2koef);
   hipLaunchKernelGGL(k_dense_rows_c, dim3(blocks(N, 128)), dim3(128), 0, c->st, g, C.flags, C.nbc, C.bc_r, C.bc_j,
                      C.bc_negk, dact, didx, A, N);
-  for (int k = 0; k < N; k++) {
-    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, c->st, A, Bm, N, k, fac, sing);
-    hipLaunchKernelGGL(k_gj_elim, dim3(blocks((long)N * N, 256)), dim3(256), 0, c->st, A, Bm, N, k, fac);
-  }
-  int hs = 0;
-  HIPCHK(hipMemcpyAsync(&hs, sing, 4, hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
-  dfree(A); dfree(fac); dfree(sing); dfree(didx);
-  if (hs) { dfree(Bm); dfree(dact); return STK_ESINGULAR; }
+  const int rc = gj_invert(c, A, N, inv);
+  dfree(A); dfree(didx);
+  if (rc) { dfree(inv); dfree(dact); return rc; }
   if (N != c->nca || !c->ainv64) {  // same size: refill in place (captured graphs keep their pointers)
     dfree(c->ainv64); dfree(c->ainv32); dfree(c->cact);
     if (dalloc(&c->ainv64, (size_t)N * N) || dalloc(&c->ainv32, (size_t)N * N) || dalloc(&c->cact, N))
       return STK_ENOMEM;
     c->graph_ok = false;
   }
-  HIPCHK(hipMemcpyAsync(c->ainv64, Bm, (size_t)N * N * 8, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->ainv64, inv, (size_t)N * N * 8, hipMemcpyDeviceToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->cact, dact, (size_t)N * 4, hipMemcpyDeviceToDevice, c->st));
   c->nca = N;
   hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks((long)N * N, 256)), dim3(256), 0, c->st, (long)N * N,
                      c->ainv64, c->ainv32);
   HIPCHK(hipStreamSynchronize(c->st));
-  dfree(Bm); dfree(dact);
+  dfree(inv); dfree(dact);
   return STK_OK;
 }
 
 int coarse_inverse(stk_ctx *c, int withp, double **out) {
   Level &C = c->lv[c->L];
   int n = (int)(4 * C.n);
-  double *A, *Bm, *fac;
-  int *sing;
-  if (dalloc(&A, (size_t)n * n) || dalloc(&Bm, (size_t)n * n) || dalloc(&fac, n) || dalloc(&sing, 1))
-    return STK_ENOMEM;
+  double *A = nullptr, *inv = nullptr;
+  if (dalloc(&A, (size_t)n * n) || dalloc(&inv, (size_t)n * n)) { dfree(A); return STK_ENOMEM; }
   HIPCHK(hipMemsetAsync(A, 0, (size_t)n * n * 8, c->st));
-  HIPCHK(hipMemsetAsync(sing, 0, 4, c->st));
-  std::vector<double> I((size_t)n * n, 0.0);
-  for (int i = 0; i < n; i++) I[(size_t)i * n + i] = 1.0;
-  HIPCHK(hipMemcpyAsync(Bm, I.data(), (size_t)n * n * 8, hipMemcpyHostToDevice, c->st));
   Geo<double> g = C.geo<double>(c->kn.p2koef);
   hipLaunchKernelGGL(k_dense_rows, dim3(blocks(n, 128)), dim3(128), 0, c->st, g, C.flags, C.nbc, C.bc_r, C.bc_j,
                      C.bc_negk, A, n, withp);
-  for (int k = 0; k < n; k++) {
-    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, c->st, A, Bm, n, k, fac, sing);
-    hipLaunchKernelGGL(k_gj_elim, dim3(blocks((long)n * n, 256)), dim3(256), 0, c->st, A, Bm, n, k, fac);
-  }
-  int hs = 0;
-  HIPCHK(hipMemcpyAsync(&hs, sing, 4, hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
-  dfree(A); dfree(fac); dfree(sing);
-  if (hs) { dfree(Bm); return STK_ESINGULAR; }
+  const int rc = gj_invert(c, A, n, inv);
+  dfree(A);
+  if (rc) { dfree(inv); return rc; }
   dfree(*out);
-  *out = Bm;
+  *out = inv;
   return STK_OK;
 }
 
@@ -2222,32 +2225,22 @@ int pardisomat(int am, long mcmax, long mcmax2, long pos0cur1, int tid, long *un
   int n = (int)(4 * C.n);
   if (am == 0) {
     if (c->ca.size() != (size_t)n * n) return STK_ESTATE;
-    double *A, *Bm, *fac, *db, *dx;
-    int *sing;
-    if (dalloc(&A, (size_t)n * n) || dalloc(&Bm, (size_t)n * n) || dalloc(&fac, n) || dalloc(&sing, 1) ||
-        dalloc(&db, n) || dalloc(&dx, n))
+    double *A = nullptr, *Bm = nullptr, *db = nullptr, *dx = nullptr;
+    if (dalloc(&A, (size_t)n * n) || dalloc(&Bm, (size_t)n * n) || dalloc(&db, n) || dalloc(&dx, n)) {
+      dfree(A); dfree(Bm); dfree(db); dfree(dx);
       return STK_ENOMEM;
-    std::vector<double> I((size_t)n * n, 0.0);
-    for (int i = 0; i < n; i++) I[(size_t)i * n + i] = 1.0;
-    H2D(c, A, c->ca.data(), (size_t)n * n * 8);
-    H2D(c, Bm, I.data(), (size_t)n * n * 8);
-    H2D(c, db, c->cb.data(), (size_t)n * 8);
-    HIPCHK(hipMemsetAsync(sing, 0, 4, c->st));
-    for (int k = 0; k < n; k++) {
-      hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, c->st, A, Bm, n, k, fac, sing);
-      hipLaunchKernelGGL(k_gj_elim, dim3(blocks((long)n * n, 256)), dim3(256), 0, c->st, A, Bm, n, k, fac);
     }
-    hipLaunchKernelGGL((k_gemv<double, double>), dim3(blocks(n, 4)), dim3(256), 0, c->st, Bm, db, dx, n);
-    int hs = 0;
-    HIPCHK(hipStreamSynchronize(c->st));
-    D2H(c, &hs, sing, 4);
+    H2D(c, A, c->ca.data(), (size_t)n * n * 8);
+    H2D(c, db, c->cb.data(), (size_t)n * 8);
+    const int hs = gj_invert(c, A, n, Bm);
+    if (!hs) hipLaunchKernelGGL((k_gemv<double, double>), dim3(blocks(n, 4)), dim3(256), 0, c->st, Bm, db, dx, n);
     int rc = ensure_ref(c, c->L);
     if (!rc && !hs) {
       HIPCHK(hipMemcpyAsync(C.ref[0], dx, (size_t)n * 8, hipMemcpyDeviceToDevice, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
     }
-    dfree(A); dfree(Bm); dfree(fac); dfree(sing); dfree(db); dfree(dx);
-    return hs ? STK_ESINGULAR : rc;
+    dfree(A); dfree(Bm); dfree(db); dfree(dx);
+    return hs ? hs : rc;
   }
   if (mcmax < 0 || mcmax >= n) return STK_EINVAL;
   if (c->ca.size() != (size_t)n * n) {
