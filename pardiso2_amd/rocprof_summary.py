@@ -3,13 +3,17 @@
     python -m pardiso2_amd.rocprof_summary stats <dir> [--top N]
     python -m pardiso2_amd.rocprof_summary trace <dir> --kernel SUBSTR [--last N]
     python -m pardiso2_amd.rocprof_summary pmc <dir> [--kernel SUBSTR] [--last N]
+    python -m pardiso2_amd.rocprof_summary window <dir> --since SUBSTR [--nth K] [--top N]
 
 `stats` reads the *_kernel_stats.csv that `rocprofv3 --kernel-trace --stats
 --output-format csv` writes; `trace` reads the same run's *_kernel_trace.csv
 and prints the duration statistics of the matching dispatches (the last N of
 them with --last); `pmc` reads *_counter_collection.csv of a `rocprofv3 --pmc
 ... --output-format csv` run and prints, per kernel name and counter, the
-number of dispatches and the mean value per dispatch (of the last N).
+number of dispatches and the mean value per dispatch (of the last N);
+`window` takes the kernel trace from the K-th last dispatch whose name holds
+SUBSTR to the end (e.g. one timed solve step) and prints per-kernel totals,
+the window's span and the share of it the device spent in kernels.
 """
 from __future__ import annotations
 
@@ -71,6 +75,31 @@ def trace(d, kernel, last):
           f"median us {ds[len(ds) // 2] / 1e3:.3f}  max us {ds[-1] / 1e3:.3f}")
 
 
+def window(d, since, nth, top):
+    rows = []
+    for f in _find(d, "kernel_trace.csv"):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                rows.append((float(_col(r, "Start_Timestamp")), float(_col(r, "End_Timestamp")),
+                             _col(r, "Kernel_Name", "KernelName")))
+    rows.sort()
+    marks = [i for i, r in enumerate(rows) if since in r[2]]
+    if len(marks) < nth:
+        raise SystemExit(f"fewer than {nth} dispatches of {since}")
+    rows = rows[marks[-nth]:]
+    span = rows[-1][1] - rows[0][0]
+    per = defaultdict(lambda: [0, 0.0])
+    for t0, t1, name in rows:
+        per[name][0] += 1
+        per[name][1] += t1 - t0
+    busy = sum(v[1] for v in per.values())
+    print(f"window: {len(rows)} dispatches, span {span / 1e6:.3f} ms, in kernels {busy / 1e6:.3f} ms "
+          f"({100 * busy / span:.1f} %)")
+    print(f"{'calls':>8} {'total ms':>10} {'avg us':>10} {'% span':>7}  kernel")
+    for name, (cnt, t) in sorted(per.items(), key=lambda kv: -kv[1][1])[:top]:
+        print(f"{cnt:8d} {t / 1e6:10.3f} {t / cnt / 1e3:10.2f} {100 * t / span:7.2f}  {name[:110]}")
+
+
 def pmc(d, kernel, last=0):
     per = defaultdict(float)  # (kernel, counter, dispatch) -> value summed over the row's dimensions
     for f in _find(d, "counter_collection.csv"):
@@ -96,16 +125,20 @@ def pmc(d, kernel, last=0):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["stats", "trace", "pmc"])
+    ap.add_argument("mode", choices=["stats", "trace", "pmc", "window"])
     ap.add_argument("dir")
     ap.add_argument("--top", type=int, default=25)
     ap.add_argument("--kernel", default=None)
     ap.add_argument("--last", type=int, default=0)
+    ap.add_argument("--since", default=None)
+    ap.add_argument("--nth", type=int, default=1)
     a = ap.parse_args(argv)
     if a.mode == "stats":
         stats(a.dir, a.top)
     elif a.mode == "trace":
         trace(a.dir, a.kernel or "", a.last)
+    elif a.mode == "window":
+        window(a.dir, a.since or "", a.nth, a.top)
     else:
         pmc(a.dir, a.kernel, a.last)
 

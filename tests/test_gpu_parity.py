@@ -190,6 +190,27 @@ def test_restriction_and_prolongation_bit_exact(pair):
     np.testing.assert_array_equal(g.vec_get(S.SOL0, 0), o.arr("sol0")[:n4])
 
 
+def test_restriction_bit_exact_with_five_entry_gathers(oracle_mod):
+    """27^3 -> 16^3 -> 10^3 -> 7^3: on the last restriction a coarse index
+    gathers 5 fine indices on some axes (the 6-wide padded kernel); the
+    others use the 4-wide one."""
+    pb, o, g = _pair(oracle_mod, n=27, L=3, seed=6)
+    try:
+        g.bind()
+        rng = np.random.default_rng(9)
+        for l in (1, 2, 3):
+            bf, nf4 = 4 * o.info[l - 1][5], 4 * o.info[l - 1][4]
+            r = rng.standard_normal(nf4)
+            o.arr("fre0")[bf:bf + nf4] = r
+            g.vec_set(S.FRE0, l - 1, r)
+            o.vprestrict(l)
+            S.lib().vprestrict(l)
+            bc, nc4 = 4 * o.info[l][5], 4 * o.info[l][4]
+            np.testing.assert_array_equal(g.vec_get(S.FRE1, l), o.arr("fre1")[bc:bc + nc4], err_msg=f"level {l}")
+    finally:
+        g.close()
+
+
 def test_ronurestrict_entry_point(pair):
     pb, o, g = pair
     g.bind()

@@ -1,0 +1,31 @@
+"""The rocprofv3 CSV summariser on a small synthetic trace."""
+import csv
+
+from pardiso2_amd import rocprof_summary as R
+
+
+def _trace(tmp_path, rows):
+    with open(tmp_path / "x_kernel_trace.csv", "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Start_Timestamp", "End_Timestamp", "Dispatch_Id"])
+        w.writeheader()
+        for i, (name, t0, t1) in enumerate(rows):
+            w.writerow({"Kernel_Name": name, "Start_Timestamp": t0, "End_Timestamp": t1, "Dispatch_Id": i})
+
+
+def test_window_from_the_kth_last_marker(tmp_path, capsys):
+    _trace(tmp_path, [("setup", 0, 10), ("mark", 100, 110), ("a", 110, 150), ("mark", 1000, 1010),
+                      ("a", 1010, 1050), ("b", 1100, 1200)])
+    R.window(str(tmp_path), "mark", 1, 10)
+    out = capsys.readouterr().out
+    assert "window: 3 dispatches, span 0.000 ms" in out
+    assert "(75.0 %)" in out  # (10 + 40 + 100) ns of a 200 ns span
+    R.window(str(tmp_path), "mark", 2, 10)
+    out = capsys.readouterr().out
+    assert "window: 5 dispatches" in out
+
+
+def test_trace_last_dispatches(tmp_path, capsys):
+    _trace(tmp_path, [("k1", 0, 1000), ("k1", 2000, 5000), ("k2", 0, 10)])
+    R.trace(str(tmp_path), "k1", 1)
+    out = capsys.readouterr().out
+    assert "dispatches 1  avg us 3.000" in out
