@@ -140,7 +140,11 @@ def bench_batched(args):
     ctxs = [S.from_problem(pb) for pb in problems]
     log(f"{len(ctxs)} instances ready: " + ", ".join(f"{n}^3 x{sizes.count(n)}" for n in sorted(set(sizes))))
     gkw = {} if args.graph is None else {"graph": args.graph}
-    mk = lambda pb, prec: S.default_opts(pb.levels, precision=prec, tol=1e-8, maxit=args.maxit, **gkw)
+    # Krylov vectors borrowed per solve (history = 1): memory follows the
+    # solves in flight, not the 64 resident instances, which leaves room for
+    # restart 256 (hard inclusion instances stall under restart 128)
+    mk = lambda pb, prec: S.default_opts(pb.levels, precision=prec, tol=1e-8, maxit=args.maxit,
+                                         restart=args.batch_restart, history=1, **gkw)
     opts = [mk(pb, args.precision) for pb in problems]
     for i in range(args.warmup):
         rc, st = S.solve_batch(ctxs, opts, threads=args.threads, prepare=True)
@@ -177,7 +181,7 @@ def bench_batched(args):
         "dtype": "f32 smoother / f64 outer" if args.precision == 0 else "f64", "data": "synthetic",
         "config": {"workload": f"batched: {args.instances} instances, sizes from {{33,49,65,97}}^3, 1-4 inclusions, "
                                f"contrast 1e0..1e6, {args.threads} concurrent streams",
-                   "graph": int(opts[0].graph),
+                   "graph": int(opts[0].graph), "restart": args.batch_restart, "history": "borrowed per solve",
                    "sizes": {str(n): sizes.count(n) for n in sorted(set(sizes))}, "seed": args.seed, "tol": 1e-8},
         "converged": conv, "p50_ms": float(np.percentile(np.array(lat) * 1e3, 50)),
         "p99_ms": float(np.percentile(np.array(lat) * 1e3, 99)),
@@ -257,6 +261,7 @@ def main():
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--instances", type=int, default=64, help="batched workload: number of instances")
     ap.add_argument("--threads", type=int, default=16, help="batched/served: host workers (one stream each)")
+    ap.add_argument("--batch-restart", type=int, default=256, help="batched workload: FGMRES restart")
     ap.add_argument("--rate", type=float, default=50.0, help="served workload: arrivals per second")
     ap.add_argument("--duration", type=float, default=60.0, help="served workload: arrival window in seconds")
     ap.add_argument("--graph", type=int, default=None, help="1: replay the preconditioner from HIP graphs")

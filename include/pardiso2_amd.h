@@ -53,7 +53,7 @@ typedef struct {
 
 typedef struct {
   int precision;     /* 0: FP32 V-cycle inside FP64 outer iteration; 1: FP64 throughout */
-  int restart;       /* Krylov directions kept before a restart (1..128) */
+  int restart;       /* Krylov directions kept before a restart (1..256) */
   int pre[12];       /* pre-smoothing sweeps per level */
   int post[12];      /* post-smoothing sweeps per level */
   int tile[3];       /* smoother tile (y, x, z) in nodes; 0 0 0 = per level: autotuned, else 32 4 4 */
@@ -70,6 +70,10 @@ typedef struct {
   int krylov;        /* outer Krylov method: 0 GCR, 1 flexible GMRES */
   int graph;         /* 1: replay the preconditioner from a captured HIP graph (one host thread
                         per process capturing at a time; stk_solve_batch launches eagerly), 0: eager */
+  int history;       /* 0: the context keeps its Krylov vectors between solves; 1: it borrows them
+                        from a process-wide pool for the solve (many contexts, few solving at once:
+                        memory follows the solves in flight; a solve waits while the device has no
+                        room for another set) */
 } stk_solver_opts;
 
 typedef struct {

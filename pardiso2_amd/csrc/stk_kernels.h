@@ -1065,7 +1065,7 @@ __global__ void k_reduce_rows(const double *__restrict__ partial, int nb, int k,
 // ---------------------------------------------------------------------------
 // Krylov basis kernels: one pass over the vectors for all k directions
 // ---------------------------------------------------------------------------
-constexpr int kMaxDirs = 129;
+constexpr int kMaxDirs = 257;  // restart + 1
 struct PtrsK {
   const double *p[kMaxDirs];
 };

@@ -5,12 +5,14 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <array>
 #include <atomic>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -626,9 +628,13 @@ void kcycle(stk_ctx *c, int l, int nu, T scale) {
   }
   const KPlanesC<T> k = kplanes<T>(V);
   const unsigned nb = blocks(V.n, 256);
-  for (int s = 0; s < nu; s++)
-    for (int col = 0; col < 2; col++)
-      hipLaunchKernelGGL(k_kgs<T>, dim3(nb), dim3(256), 0, c->st, V.X, V.Y, V.Z, k, kb<T>(V), kx<T>(V), col);
+  auto sweeps = [&](int c1) {
+    for (int s = 0; s < nu; s++)
+      for (int q = 0; q < 2; q++)
+        hipLaunchKernelGGL(k_kgs<T>, dim3(nb), dim3(256), 0, c->st, V.X, V.Y, V.Z, k, kb<T>(V), kx<T>(V),
+                           q == 0 ? c1 : 1 - c1);
+  };
+  sweeps(0);
   hipLaunchKernelGGL(k_kres<T>, dim3(nb), dim3(256), 0, c->st, V.X, V.Y, V.Z, k, kb<T>(V), kx<T>(V), kr<T>(V));
   Level &C = c->lv[l + 1];
   hipLaunchKernelGGL(k_krestrict<T>, dim3(blocks(C.n, 256)), dim3(256), 0, c->st, V.X, V.Y, C.X, C.Y, C.Z, kagg(C),
@@ -636,9 +642,7 @@ void kcycle(stk_ctx *c, int l, int nu, T scale) {
   kcycle<T>(c, l + 1, nu, scale);
   hipLaunchKernelGGL(k_kprolong<T>, dim3(nb), dim3(256), 0, c->st, V.X, V.Y, V.Z, C.X, C.Y, V.kpar[0], V.kpar[1],
                      V.kpar[2], k.p[0], kx<T>(C), kx<T>(V), scale);
-  for (int s = 0; s < nu; s++)
-    for (int col = 1; col >= 0; col--)
-      hipLaunchKernelGGL(k_kgs<T>, dim3(nb), dim3(256), 0, c->st, V.X, V.Y, V.Z, k, kb<T>(V), kx<T>(V), col);
+  sweeps(1);
 }
 
 // z_p = sign K^-1 (B D^-1 A D^-1 G) K^-1 r_p on the continuity cells,
@@ -720,6 +724,93 @@ int true_residual(stk_ctx *c) {
   return STK_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Krylov vectors borrowed for one solve (history = 1).  Sets are keyed by
+// vector length and kept for reuse; a set needs m + 1 basis and m
+// preconditioned vectors.  When the device has no room for a new set, the
+// request waits for another solve to return one, and frees idle sets of
+// other lengths first.
+// ---------------------------------------------------------------------------
+struct HistSet {
+  int dev = 0;
+  long n4 = 0;
+  int m = 0;
+  std::vector<double *> Z, Q;
+};
+struct HistPool {
+  std::mutex mu;
+  std::condition_variable cv;
+  std::vector<HistSet *> idle;
+  int lent = 0;
+};
+static HistPool g_hist;
+
+static void hist_free(HistSet *h) {
+  for (auto p : h->Z) dfree(p);
+  for (auto p : h->Q) dfree(p);
+  delete h;
+}
+
+static HistSet *hist_alloc(int dev, long n4, int m) {
+  HistSet *h = new HistSet();
+  h->dev = dev; h->n4 = n4; h->m = m;
+  for (int i = 0; i <= m; i++) {
+    double *q = nullptr, *z = nullptr;
+    if (dalloc(&q, n4) || (i < m && dalloc(&z, n4))) {
+      dfree(q);
+      hist_free(h);
+      return nullptr;
+    }
+    h->Q.push_back(q);
+    if (i < m) h->Z.push_back(z);
+  }
+  return h;
+}
+
+static HistSet *hist_acquire(long n4, int m) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::unique_lock<std::mutex> lk(g_hist.mu);
+  for (;;) {
+    for (size_t i = 0; i < g_hist.idle.size(); i++) {
+      HistSet *h = g_hist.idle[i];
+      if (h->dev == dev && h->n4 == n4 && h->m >= m) {
+        g_hist.idle.erase(g_hist.idle.begin() + i);
+        g_hist.lent++;
+        return h;
+      }
+    }
+    HistSet *h = hist_alloc(dev, n4, m);
+    if (!h) {  // no room: free idle sets of this device, then wait for a lent one
+      bool freed = false;
+      for (size_t i = 0; i < g_hist.idle.size();)
+        if (g_hist.idle[i]->dev == dev) { hist_free(g_hist.idle[i]); g_hist.idle.erase(g_hist.idle.begin() + i); freed = true; }
+        else i++;
+      if (freed) h = hist_alloc(dev, n4, m);
+    }
+    if (h) { g_hist.lent++; return h; }
+    if (g_hist.lent == 0) return nullptr;  // nothing will come back: out of memory
+    g_hist.cv.wait(lk);
+  }
+}
+
+static void hist_release(HistSet *h) {
+  {
+    std::lock_guard<std::mutex> lk(g_hist.mu);
+    g_hist.idle.push_back(h);
+    g_hist.lent--;
+  }
+  g_hist.cv.notify_all();
+}
+
+// the context's solve-time view of a borrowed set, returned on scope exit
+struct HistLease {
+  stk_ctx *c = nullptr;
+  HistSet *h = nullptr;
+  std::vector<double *> keepZ, keepQ;
+  ~HistLease();
+};
+
 int ensure_outer(stk_ctx *c, int m) {
   long n4 = 4 * c->lv[0].n;
   if (!c->X) {
@@ -738,6 +829,14 @@ int ensure_outer(stk_ctx *c, int m) {
     c->Qs.push_back(q);
   }
   return STK_OK;
+}
+
+HistLease::~HistLease() {
+  if (!h) return;
+  (void)hipStreamSynchronize(c->st);  // the stream is done with the vectors
+  c->Zs = std::move(keepZ);
+  c->Qs = std::move(keepQ);
+  hist_release(h);
 }
 
 // ---------------------------------------------------------------------------
@@ -1508,7 +1607,7 @@ int stk_prepare_cvm(stk_ctx *c, int cvm) {
                          L.flags, L.lc32);
     }
   }
-  rc = ensure_outer(c, 1);
+  rc = ensure_outer(c, 0);
   if (rc) return rc;
   Level &F = c->lv[0];
   hipLaunchKernelGGL(k_rhs, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, F.flags, c->ro,
@@ -1529,7 +1628,7 @@ int stk_prepare_cvm(stk_ctx *c, int cvm) {
 
 int stk_set_solution(stk_ctx *c, const double *x) {
   if (!c || !x) return STK_EINVAL;
-  int rc = ensure_outer(c, 1);
+  int rc = ensure_outer(c, 0);
   if (rc) return rc;
   H2D(c, c->X, x, 4 * c->lv[0].n * 8);
   return STK_OK;
@@ -1815,12 +1914,25 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
 }
 
 // allocations a solve needs, and the preconditioner capture when replay is on
-static int solve_setup(stk_ctx *c, const stk_solver_opts *o) {
+static int solve_setup(stk_ctx *c, const stk_solver_opts *o, HistLease &lease) {
   const bool fp64 = o->precision == 1;
   int rc = ensure_vectors(c, !fp64, fp64);
   if (rc) return rc;
-  rc = ensure_outer(c, o->restart + 1);
-  if (rc) return rc;
+  if (o->history == 1) {
+    rc = ensure_outer(c, 0);
+    if (rc) return rc;
+    HistSet *h = hist_acquire(4 * c->lv[0].n, o->restart);
+    if (!h) return STK_ENOMEM;
+    lease.c = c;
+    lease.h = h;
+    lease.keepZ = std::move(c->Zs);
+    lease.keepQ = std::move(c->Qs);
+    c->Zs.assign(h->Z.begin(), h->Z.begin() + o->restart);
+    c->Qs.assign(h->Q.begin(), h->Q.begin() + o->restart + 1);
+  } else {
+    rc = ensure_outer(c, o->restart + 1);
+    if (rc) return rc;
+  }
   if (o->graph) {
     const bool explicit_tile = o->tile[0] || o->tile[1] || o->tile[2];
     rc = ensure_graph(c, o, explicit_tile ? o->tile : nullptr);
@@ -1831,9 +1943,10 @@ static int solve_setup(stk_ctx *c, const stk_solver_opts *o) {
 int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   if (!c || !o) return STK_EINVAL;
   if (!c->prepared) return STK_ESTATE;
-  if (o->restart < 1 || o->restart > 128 || o->maxit < 1 || !(o->tol > 0) || o->cycle < 0 || o->cycle > 2 ||
+  if (o->restart < 1 || o->restart > kMaxDirs - 1 || o->maxit < 1 || !(o->tol > 0) || o->cycle < 0 || o->cycle > 2 ||
       o->coarse_viscmod < -1 || o->coarse_viscmod > 2 || o->schur < 0 || o->schur > 2 || o->ksweeps < 0 ||
-      o->ksweeps > 16 || !(o->kscale > 0) || o->krylov < 0 || o->krylov > 1 || o->graph < 0 || o->graph > 1)
+      o->ksweeps > 16 || !(o->kscale > 0) || o->krylov < 0 || o->krylov > 1 || o->graph < 0 || o->graph > 1 ||
+      o->history < 0 || o->history > 1)
     return STK_EINVAL;
   {
     const int want = (o->coarse_viscmod < 0) ? c->kn.viscmod : o->coarse_viscmod;
@@ -1847,7 +1960,8 @@ int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   if (explicit_tile && (o->tile[0] < 2 || o->tile[1] < 1 || o->tile[2] < 1 ||
                         (size_t)(o->tile[0] + 2) * (o->tile[1] + 2) * (o->tile[2] + 2) * 32 > 65536))
     return STK_EINVAL;
-  int rc = solve_setup(c, o);
+  HistLease lease;  // borrowed Krylov vectors (history = 1) go back when the solve returns
+  int rc = solve_setup(c, o, lease);
   if (rc) return rc;
   stk_stats local;
   stk_stats *S = st ? st : &local;
@@ -2074,7 +2188,10 @@ int stk_solve_batch(int n, stk_ctx *const *ctxs, const stk_solver_opts *opts, in
     if (!oi.graph) continue;
     stk_ctx *c = ctxs[i];
     int rc = c->prepared ? STK_OK : stk_prepare(c);
-    if (!rc) rc = solve_setup(c, &oi);
+    if (!rc) {
+      HistLease lease;  // a borrowed set goes straight back: the capture does not use it
+      rc = solve_setup(c, &oi, lease);
+    }
     if (rc) return rc;
   }
   std::atomic<int> next{0};

@@ -176,11 +176,11 @@ template <typename T>
 __device__ __forceinline__ bool kgs_val(int X, int Y, int Z, const KPlanesC<T> &k, const T *__restrict__ b,
                                         const T *x, int colour, long n, T &out) {
   const int XY = X * Y;
-  const T d = k.p[0][n];
-  if (d == T(0)) return false;
   int m1, m2, m3;
   decode(n, X, Y, XY, m1, m2, m3);
-  if (((m1 + m2 + m3) & 1) != colour) return false;
+  if (((m1 + m2 + m3) & 1) != colour) return false;  // before any load: half the cells stop here
+  const T d = k.p[0][n];
+  if (d == T(0)) return false;
   T acc = b[n];
   if (m1 > 0) acc -= k.p[1][n] * x[n - Y];
   if (m1 < X - 1) acc -= k.p[2][n] * x[n + Y];

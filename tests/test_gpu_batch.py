@@ -44,6 +44,38 @@ def test_batch_equals_alone():
             c.close()
 
 
+def test_borrowed_krylov_vectors_give_the_same_solves():
+    """history = 1 (Krylov vectors from the shared pool, restart 256) gives
+    bit-identical solves to the context's own vectors, alone and in a batch
+    whose instances outnumber the workers (sets are reused across sizes)."""
+    probs = _problems()
+    ref = []
+    for pb in probs:
+        g = S.from_problem(pb)
+        try:
+            g.hydrostatic_guess()
+            st = g.solve(S.default_opts(pb.levels, restart=256))
+            ref.append((g.solution(), st.iterations))
+        finally:
+            g.close()
+    ctxs = [S.from_problem(pb) for pb in probs]
+    try:
+        for (xr, itr), c, pb in zip(ref, ctxs, probs):
+            c.hydrostatic_guess()
+            st = c.solve(S.default_opts(pb.levels, restart=256, history=1))
+            assert st.status == 0 and st.iterations == itr
+            np.testing.assert_array_equal(c.solution(), xr)
+        rc, st = S.solve_batch(ctxs, [S.default_opts(pb.levels, restart=256, history=1) for pb in probs], threads=2,
+                               prepare=True)
+        assert rc == 0
+        for (xr, itr), c, s in zip(ref, ctxs, st):
+            assert s.iterations == itr
+            np.testing.assert_array_equal(c.solution(), xr)
+    finally:
+        for c in ctxs:
+            c.close()
+
+
 def test_served_stream_equals_alone():
     pbs = {17: setups.served_request(17, 1, 4.0), 21: setups.served_request(21, 2, 5.0)}
     payloads = {n: [serve.payload_from_problem(pb)] for n, pb in pbs.items()}
