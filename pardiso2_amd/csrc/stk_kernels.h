@@ -1209,24 +1209,58 @@ __global__ void __launch_bounds__(256) k_update2(long n4, int k, const double *_
   double2 *Wv = reinterpret_cast<double2 *>(w);
   const double2 *__restrict__ W2 = reinterpret_cast<const double2 *>(w2);
   double ss = 0.0;
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n2; p += (long)gridDim.x * blockDim.x) {
-    double2 uu = U[p], wv = Wv[p];
-#pragma unroll 8
-    for (int i = 0; i < k; i++) {
-      const double2 vv = reinterpret_cast<const double2 *>(V.p[i])[p];
-      uu.x -= sc[i] * vv.x;
-      uu.y -= sc[i] * vv.y;
-      wv.x -= sc[k + i] * vv.x;
-      wv.y -= sc[k + i] * vv.y;
+  // E double2 per thread per slice, basis vectors two at a time (their loads
+  // in flight together); each element subtracts the terms in basis order
+  constexpr int E = 4;
+  for (long base = (long)blockIdx.x * 256 * E; base < n2; base += (long)gridDim.x * 256 * E) {
+    double2 uu[E], wv[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const long p = base + threadIdx.x + 256L * e;
+      uu[e] = make_double2(0.0, 0.0);
+      wv[e] = uu[e];
+      if (p < n2) { uu[e] = U[p]; wv[e] = Wv[p]; }
     }
-    uu.x *= su;
-    uu.y *= su;
-    wv.x -= hk * uu.x;
-    wv.y -= hk * uu.y;
-    U[p] = uu;
-    Wv[p] = wv;
-    const double2 q = W2[p];
-    ss += q.x * wv.x * wv.x + q.y * wv.y * wv.y;
+    for (int i = 0; i < k; i += 2) {
+      const bool two = i + 1 < k;
+      const double2 *__restrict__ v0 = reinterpret_cast<const double2 *>(V.p[i]);
+      const double2 *__restrict__ v1 = reinterpret_cast<const double2 *>(V.p[two ? i + 1 : i]);
+      double2 a0[E], a1[E];
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        const long p = base + threadIdx.x + 256L * e;
+        a0[e] = make_double2(0.0, 0.0);
+        a1[e] = a0[e];
+        if (p < n2) { a0[e] = v0[p]; a1[e] = v1[p]; }
+      }
+      const double c0 = sc[i], d0 = sc[k + i];
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        uu[e].x -= c0 * a0[e].x; uu[e].y -= c0 * a0[e].y;
+        wv[e].x -= d0 * a0[e].x; wv[e].y -= d0 * a0[e].y;
+      }
+      if (two) {
+        const double c1 = sc[i + 1], d1 = sc[k + i + 1];
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          uu[e].x -= c1 * a1[e].x; uu[e].y -= c1 * a1[e].y;
+          wv[e].x -= d1 * a1[e].x; wv[e].y -= d1 * a1[e].y;
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const long p = base + threadIdx.x + 256L * e;
+      if (p >= n2) continue;
+      uu[e].x *= su;
+      uu[e].y *= su;
+      wv[e].x -= hk * uu[e].x;
+      wv[e].y -= hk * uu[e].y;
+      U[p] = uu[e];
+      Wv[p] = wv[e];
+      const double2 q = W2[p];
+      ss += q.x * wv[e].x * wv[e].x + q.y * wv[e].y * wv[e].y;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;

@@ -1799,10 +1799,10 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
   const long n4 = 4 * c->lv[0].n;
   const int m = o->restart;
   const bool fp64 = o->precision == 1;
-  // dots: one block per 1024-element slice up to 4 nred blocks (partials of
-  // 2j+2 sums per block); updates: up to 4 nred blocks
+  // dots and updates: one block per 1024-element slice up to 4 nred blocks
+  // (partials of 2j+2 sums per block for the dots)
   const unsigned nbk = (unsigned)std::min<long>(4L * c->nred, std::max<long>(1, (n4 / 2 + 1023) / 1024));
-  const unsigned nbu = (unsigned)std::min<long>(4L * c->nred, (long)blocks(n4 / 2, 256));
+  const unsigned nbu = (unsigned)std::min<long>(4L * c->nred, std::max<long>(1, (n4 / 2 + 1023) / 1024));
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), hh(2 * m + 2), y(m), coef(2 * m + 2);
   auto h = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
   auto givens = [&](int col) {
