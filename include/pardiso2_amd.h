@@ -217,6 +217,11 @@ int pardisomat(int am, long mcmax, long mcmax2, long pos0cur1, int tid, long *un
 int gausmat4(int am, long mcmax, long mcmin, int mgi);
 /* parallel/move3mg.c:3966 -- residual restriction to level mgi */
 void vprestrict(int mgi);
+/* The mixed-precision V-cycle's restriction of the velocity rows (FP32,
+ * pressure rows 0) on the context's copies: fre0 of level l-1 (rounded to
+ * FP32) -> fre1 of level l (widened).  Same reference routine as vprestrict
+ * (parallel/move3mg.c:3966-4354), summed axis by axis. */
+int stk_vrestrict32(stk_ctx *ctx, int level);
 /* parallel/move3mg.c:3535 -- correction prolongation from level mgi */
 void vpprolong(int mgi, int ynreset);
 /* parallel/move3mg.c:3387 -- nu/ro restriction to level mgi */

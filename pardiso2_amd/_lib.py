@@ -21,7 +21,7 @@ EXPORTS = [
     "stk_create", "stk_destroy", "stk_level_info", "stk_set_knobs", "stk_default_opts", "stk_bc_line",
     "stk_set_bc_tables", "stk_get_bc_tables", "stk_set_fields", "stk_markers_to_nodes", "stk_get_fields",
     "stk_prepare", "stk_set_solution", "stk_get_solution", "stk_solve", "stk_relres",
-    "stk_pressure_gauge", "stk_hydrostatic_guess", "stk_time_kernel", "stk_vec_set", "stk_vec_get", "stk_apply", "stk_smooth", "stk_smooth_ex", "stk_coarse_solve",
+    "stk_pressure_gauge", "stk_hydrostatic_guess", "stk_time_kernel", "stk_vec_set", "stk_vec_get", "stk_apply", "stk_smooth", "stk_smooth_ex", "stk_vrestrict32", "stk_coarse_solve",
     "stk_bind", "matclear", "gausmat", "gausmat2", "pardisomat", "gausmat4", "vprestrict", "vpprolong",
     "ronurestrict", "vpiterate", "stk_kop_apply", "stk_kcycle_apply", "stk_schur_apply",
     "stk_solve_batch", "stk_mem_info", "stk_autotune", "stk_level_tile",
@@ -91,6 +91,7 @@ def lib():
         L.stk_apply.argtypes = [vp, C.c_int, dp, dp]
         L.stk_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.stk_smooth_ex.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.stk_vrestrict32.argtypes = [vp, C.c_int]
         L.stk_coarse_solve.argtypes = [vp]
         L.stk_bind.argtypes = [vp]
         L.stk_kop_apply.argtypes = [vp, C.c_int, dp, dp]
@@ -269,6 +270,9 @@ class Stokes:
             check(lib().stk_smooth(self.h, level, sweeps, *tile), "stk_smooth")
         else:
             check(lib().stk_smooth_ex(self.h, level, sweeps, *tile, precision, int(velocity_only)), "stk_smooth_ex")
+
+    def vrestrict32(self, level):
+        check(lib().stk_vrestrict32(self.h, level), "stk_vrestrict32")
 
     def coarse_solve(self):
         check(lib().stk_coarse_solve(self.h), "stk_coarse_solve")

@@ -720,6 +720,109 @@ __global__ void __launch_bounds__(256) k_restrictK(int Xf, int Yf, int Zf, int X
   bc[u] = (ws != T(0)) ? acc / ws : acc;
 }
 
+// Separable FP32 restriction of the V-cycle's velocity rows.  A coarse
+// unknown's candidates are a tensor product of per-axis lists, so its sum
+// over the equation-row fine residuals, sum_z wz sum_x wx sum_y wy r, runs one
+// axis at a time: y (k_rsep_y, which also drops the fine rows that are not
+// equations), x (k_rsep_x), then z with the division by the weight sum
+// (k_rsep_z).  The weight sums come from the same passes with an indicator
+// in place of r (mode 1), once per prepare.  Sums are rounded in this order,
+// not the reference's, which the FP32 path does not promise.
+struct SepTab {
+  const int *fi;    // [kPad * nc] padded fine indices (-1: none)
+  const float *w;
+};
+struct SepComp {
+  SepTab ax[3];     // x, y, z
+};
+struct SepComps {
+  SepComp c[3];     // vx, vy, vz
+};
+
+// out[(k * Xf + i) * Yc + cy] = (0, sum_y for vx, vy, vz)
+__global__ void k_rsep_y(int Xf, int Yf, int Zf, int Yc, SepComps t, const unsigned char *__restrict__ ff,
+                         const float *__restrict__ rf, float4 *__restrict__ out, int mode) {
+  const long n = xcd_block() * blockDim.x + threadIdx.x;
+  if (n >= (long)Zf * Xf * Yc) return;
+  const int cy = (int)(n % Yc);
+  const long q = n / Yc;
+  const long row = q * Yf;  // (k * Xf + i) * Yf
+  float o[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const SepTab &ty = t.c[c].ax[1];
+#pragma unroll
+    for (int e = 0; e < kPad; e++) {
+      const int j = ty.fi[kPad * cy + e];
+      if (j < 0) break;
+      const long nf = row + j;
+      const bool eq = kind_of(ff[nf], c + 1) == KIND_EQ;
+      const float v = mode ? 1.0f : rf[4 * nf + c + 1];
+      o[c] += eq ? ty.w[kPad * cy + e] * v : 0.0f;
+    }
+  }
+  out[n] = make_float4(0.0f, o[0], o[1], o[2]);
+}
+
+// out[(k * Xc + cx) * Yc + cy] = sum over x of in[(k * Xf + i) * Yc + cy]
+__global__ void k_rsep_x(int Xf, int Xc, int Yc, int Zf, SepComps t, const float4 *__restrict__ in,
+                         float4 *__restrict__ out) {
+  const long n = xcd_block() * blockDim.x + threadIdx.x;
+  if (n >= (long)Zf * Xc * Yc) return;
+  const int cy = (int)(n % Yc);
+  const long q = n / Yc;
+  const int cx = (int)(q % Xc);
+  const long k = q / Xc;
+  float o[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const SepTab &tx = t.c[c].ax[0];
+#pragma unroll
+    for (int e = 0; e < kPad; e++) {
+      const int i = tx.fi[kPad * cx + e];
+      if (i < 0) break;
+      const float4 v = in[(k * Xf + i) * Yc + cy];
+      o[c] += tx.w[kPad * cx + e] * (c == 0 ? v.y : c == 1 ? v.z : v.w);
+    }
+  }
+  out[n] = make_float4(0.0f, o[0], o[1], o[2]);
+}
+
+// mode 1: ws[n] = the sums (weight sums); mode 0: bc = sum / ws on the
+// coarse equation rows (0 elsewhere and on pressure rows), xc = 0
+__global__ void k_rsep_z(int Xc, int Yc, int Zc, SepComps t, const float4 *__restrict__ in,
+                         const unsigned char *__restrict__ fc, float4 *ws, float *bc, float *xc, int mode) {
+  const long n = xcd_block() * blockDim.x + threadIdx.x;
+  const long XYc = (long)Xc * Yc;
+  if (n >= XYc * Zc) return;
+  const long cz = n / XYc, rem = n - cz * XYc;  // rem = cx * Yc + cy
+  float o[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const SepTab &tz = t.c[c].ax[2];
+#pragma unroll
+    for (int e = 0; e < kPad; e++) {
+      const int k = tz.fi[kPad * cz + e];
+      if (k < 0) break;
+      const float4 v = in[k * XYc + rem];
+      o[c] += tz.w[kPad * cz + e] * (c == 0 ? v.y : c == 1 ? v.z : v.w);
+    }
+  }
+  if (mode) {
+    ws[n] = make_float4(0.0f, o[0], o[1], o[2]);
+    return;
+  }
+  const float4 s = ws[n];
+  const unsigned char f = fc[n];
+  const float sw[3] = {s.y, s.z, s.w};
+  float r[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+    r[c] = kind_of(f, c + 1) == KIND_EQ ? (sw[c] != 0.0f ? o[c] / sw[c] : o[c]) : 0.0f;
+  reinterpret_cast<float4 *>(bc)[n] = make_float4(0.0f, r[0], r[1], r[2]);
+  reinterpret_cast<float4 *>(xc)[n] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 // vpprolong: fine x += sum coarse x * w * koef (parallel/move3mg.c:3535-3850),
 // one thread per fine node and its four unknowns
 template <typename T, int C>

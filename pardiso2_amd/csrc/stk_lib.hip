@@ -89,6 +89,8 @@ struct Level {
   double *e64[5] = {};
   float *e32[5] = {};
   float *lc32 = nullptr;  // FP32 velocity rows of an LDS-resident level (54 planes), from its viscosities
+  // separable FP32 restriction into this level: the y and x partial sums and the weight sums
+  float4 *rs_t1 = nullptr, *rs_t2 = nullptr, *rs_ws = nullptr;
   unsigned char *flags = nullptr;
   std::vector<unsigned char> hflags;
   // bounding box of the nodes with equation rows {y0, x0, z0, y1, x1, z1}: the smoother's tiles cover it
@@ -481,9 +483,40 @@ int gather_width(const Level &C) {
   return k;
 }
 
+SepComps sepcomps(const Level &C) {
+  static const int K[4][3] = {{2, 2, 2}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}};
+  SepComps t;
+  for (int c = 0; c < 3; c++)
+    for (int a = 0; a < 3; a++) {
+      const AxisDev &d = C.ax[K[c + 1][a]][a];
+      t.c[c].ax[a] = SepTab{d.pfi, d.pw32};
+    }
+  return t;
+}
+
+// the three passes of the separable restriction F = level l-1 -> C = level l
+// (mode 1: weight sums into C.rs_ws)
+void rsep(stk_ctx *c, int l, const float *rf, float *bcoarse, float *xcoarse, int mode) {
+  Level &F = c->lv[l - 1], &C = c->lv[l];
+  const SepComps t = sepcomps(C);
+  const long n1 = (long)F.Z * F.X * C.Y, n2 = (long)F.Z * C.X * C.Y;
+  hipLaunchKernelGGL(k_rsep_y, dim3(blocks(n1, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, C.Y, t, F.flags, rf,
+                     C.rs_t1, mode);
+  hipLaunchKernelGGL(k_rsep_x, dim3(blocks(n2, 256)), dim3(256), 0, c->st, F.X, C.X, C.Y, F.Z, t, C.rs_t1, C.rs_t2);
+  hipLaunchKernelGGL(k_rsep_z, dim3(blocks(C.n, 256)), dim3(256), 0, c->st, C.X, C.Y, C.Z, t, C.rs_t2, C.flags,
+                     C.rs_ws, bcoarse, xcoarse, mode);
+}
+
 template <typename T>
 void restrict_to(stk_ctx *c, int l, const T *rf, T *bcoarse, T *xcoarse, int withp) {
   Level &F = c->lv[l - 1], &C = c->lv[l];
+  if constexpr (std::is_same<T, float>::value) {
+    if (!withp && C.rs_ws) {  // the V-cycle's velocity rows
+      rsep(c, l, rf, bcoarse, xcoarse, 0);
+      return;
+    }
+  }
+  (void)F;
   const int kw = gather_width(C);
   if (kw <= 4)
     hipLaunchKernelGGL((k_restrictK<T, 4>), dim3(blocks(4 * C.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, C.X,
@@ -1109,7 +1142,7 @@ void stk_destroy(stk_ctx *c) {
     Level &L = c->lv[l];
     for (int a = 0; a < 3; a++) { dfree(L.iA64[a]); dfree(L.iB64[a]); dfree(L.iA32[a]); dfree(L.iB32[a]); dfree(L.g64[a]); }
     for (int k = 0; k < 5; k++) { dfree(L.e64[k]); dfree(L.e32[k]); }
-    dfree(L.lc32);
+    dfree(L.lc32); dfree(L.rs_t1); dfree(L.rs_t2); dfree(L.rs_ws);
     dfree(L.flags);
     dfree(L.bc_r); dfree(L.bc_j); dfree(L.bc_negk); dfree(L.bc_c);
     dfree(L.ch_r); dfree(L.ch_len); dfree(L.ch_j); dfree(L.ch_k);
@@ -1605,6 +1638,14 @@ int stk_prepare_cvm(stk_ctx *c, int cvm) {
       if (!L.lc32 && dalloc(&L.lc32, (size_t)54 * L.n)) return STK_ENOMEM;
       hipLaunchKernelGGL(k_level_coef, dim3(blocks(L.n, 256)), dim3(256), 0, c->st, L.geo<float>(c->kn.p2koef),
                          L.flags, L.lc32);
+    }
+    // separable FP32 restriction into this level: buffers and weight sums (flags only)
+    if (l >= 1 && gather_width(L) <= kPad) {
+      const Level &F = c->lv[l - 1];
+      if (!L.rs_ws && (dalloc(&L.rs_t1, (size_t)F.Z * F.X * L.Y) || dalloc(&L.rs_t2, (size_t)F.Z * L.X * L.Y) ||
+                       dalloc(&L.rs_ws, (size_t)L.n)))
+        return STK_ENOMEM;
+      rsep(c, l, nullptr, nullptr, nullptr, 1);
     }
   }
   rc = ensure_outer(c, 0);
@@ -2388,6 +2429,19 @@ void vprestrict(int mgi) {
   if (ensure_ref(c, mgi) || ensure_ref(c, mgi - 1)) return;
   restrict_to<double>(c, mgi, c->lv[mgi - 1].ref[2], c->lv[mgi].ref[1], c->lv[mgi].ref[0], 1);
   hipStreamSynchronize(c->st);
+}
+
+int stk_vrestrict32(stk_ctx *c, int l) {
+  if (!c || !c->prepared || l < 1 || l > c->L) return STK_EINVAL;
+  int rc = ensure_ref(c, l) || ensure_ref(c, l - 1) || ensure_vectors(c, true, false);
+  if (rc) return STK_ENOMEM;
+  Level &F = c->lv[l - 1], &C = c->lv[l];
+  const long nf4 = 4 * F.n, nc4 = 4 * C.n;
+  hipLaunchKernelGGL((k_cast<double, float>), dim3(blocks(nf4, 256)), dim3(256), 0, c->st, nf4, F.ref[2], F.r32);
+  restrict_to<float>(c, l, F.r32, C.b32, lvx<float>(C), 0);
+  hipLaunchKernelGGL((k_cast<float, double>), dim3(blocks(nc4, 256)), dim3(256), 0, c->st, nc4, C.b32, C.ref[1]);
+  HIPCHK(hipStreamSynchronize(c->st));
+  return hipGetLastError() == hipSuccess ? STK_OK : STK_EHIP;
 }
 
 // parallel/move3mg.c:3535-3850
