@@ -230,35 +230,42 @@ inline TileGeo make_tile(int ty, int tx, int tz, const int box[6]) {
 
 // ---------------------------------------------------------------------------
 // one phase of the tile smoother for component C (colour for velocities)
-template <typename T, int C>
-__device__ __forceinline__ void smooth_phase(const Geo<T> &g, const Geo<T> &gv, const unsigned char *__restrict__ flags,
+template <typename T, int C, bool SV>
+__device__ __forceinline__ void smooth_phase(const Geo<T> &g, const Geo<T> &gv, const unsigned char *__restrict__ sf,
                                              const T *__restrict__ b, T *sx, int nh, const TileGeo &tg,
                                              int x0, int y0, int z0, int colour, T v0koef, T p0koef) {
-  const int ty = tg.ty, tx = tg.tx, tz = tg.tz;
+  int ty = tg.ty, tx = tg.tx, tz = tg.tz;
+  // values derived from the tile shape are rebuilt per phase rather than
+  // hoisted out of the caller's phase loop and held in scalar registers
+  asm volatile("" : "+s"(ty), "+s"(tx), "+s"(tz));
   const int hy = ty + 2, hx = tx + 2;
   const int hty = (ty + 1) / 2;
   const int ncand = (C == 0) ? ty * tx * tz : hty * tx * tz;
   for (int i = threadIdx.x; i < ncand; i += blockDim.x) {
     int ly, lx, lz;
+    // index products in 24-bit multiplies (full rate): tile and level
+    // extents stay below 2^24 and node numbers below 2^32
     if (C == 0) {
       const int q = (int)fdiv(i, tg.dty);
-      ly = i - q * ty; lz = (int)fdiv(q, tg.dtx); lx = q - lz * tx;
+      ly = i - (int)__umul24(q, ty); lz = (int)fdiv(q, tg.dtx); lx = q - (int)__umul24(lz, tx);
     } else {
       const int q = (int)fdiv(i, tg.dhty);
-      const int kk = i - q * hty;
-      lz = (int)fdiv(q, tg.dtx); lx = q - lz * tx;
+      const int kk = i - (int)__umul24(q, hty);
+      lz = (int)fdiv(q, tg.dtx); lx = q - (int)__umul24(lz, tx);
       int par = (colour - (x0 + lx) - (z0 + lz) - y0) & 1;
       ly = 2 * kk + par;
       if (ly >= ty) continue;
     }
     int m1 = x0 + lx, m2 = y0 + ly, m3 = z0 + lz;
     if (m1 >= g.X || m2 >= g.Y || m3 >= g.Z) continue;
-    long n = g.nid(m1, m2, m3);
-    if (kind_of(flags[n], C) != KIND_EQ) continue;
+    const unsigned n = __umul24(m3, g.XY) + __umul24(m1, g.Y) + m2;
+    if (kind_of(sf[__umul24(__umul24(lz, tx) + lx, ty) + ly], C) != KIND_EQ) continue;
     T s[18];
-    row_coef_c<T, C>(gv, m1, m2, m3, s);
-    T ival = -b[4 * n + C];
-    const int base = ((lz + 1) * hx + (lx + 1)) * hy + (ly + 1);
+    // the row's viscosities: in the tile's LDS block (SV) or in HBM
+    const unsigned v = SV ? __umul24(__umul24(lz, tx + 1) + lx, ty + 1) + ly : n;
+    row_coef_c<T, C>(gv, m1, m2, m3, (long)v, s);
+    T ival = -b[4 * (size_t)n + C];
+    const int base = (int)__umul24(__umul24(lz + 1, hx) + lx + 1, hy) + ly + 1;
 #pragma unroll
     for (int q = 1; q <= Slot<C>::ns; q++) {
       const int li = base + (Slot<C>::d[q][2] * hx + Slot<C>::d[q][0]) * hy + Slot<C>::d[q][1];
@@ -271,8 +278,10 @@ __device__ __forceinline__ void smooth_phase(const Geo<T> &g, const Geo<T> &gv, 
 
 // SV: the tile's stencil viscosities (nodes [x0, x0+tx] x [y0, y0+ty] x
 // [z0, z0+tz], every offset a row uses is 0 or +1) are staged in LDS after the
-// unknowns, and the rows read them through a Geo whose node index maps into
-// that block, so each viscosity comes from HBM once per tile.
+// unknowns, and the rows read them through a Geo whose arrays are that block
+// (strides ty+1, (ty+1)(tx+1)), so each viscosity comes from HBM once per tile.
+// The interior's unknown kinds are staged too (last in LDS, one byte per node),
+// so a phase's row loads do not wait on a flag load from HBM.
 template <typename T, bool SV>
 __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *__restrict__ flags,
                                                 const T *__restrict__ xold, T *__restrict__ xnew,
@@ -286,6 +295,7 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
   const int by = bt % nty, bx = (bt / nty) % ntx, bz = bt / (nty * ntx);
   const int y0 = tg.oy + by * ty, x0 = tg.ox + bx * tx, z0 = tg.oz + bz * tz;
   const int nh = hy * hx * hz;
+  unsigned char *sf = smem_raw + (4 * nh + (SV ? 5 * (ty + 1) * (tx + 1) * (tz + 1) : 0)) * sizeof(T);
   Geo<T> gv = g;
   if (SV) {
     const int vy = ty + 1, vxy = (ty + 1) * (tx + 1), nv = vxy * (tz + 1);
@@ -299,14 +309,13 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
       const long n = g.nid(in ? m1 : 0, in ? m2 : 0, in ? m3 : 0);
       for (int q = 0; q < np; q++) sv[q * nv + i] = in ? src[q][n] : T(0);
     }
-    const long off = (long)z0 * vxy + (long)x0 * vy + y0;
     gv.XY = vxy;
     gv.Y = vy;
-    gv.ec = sv - off;
-    gv.exy = sv + nv - off;
-    gv.exz = sv + 2 * nv - off;
-    gv.eyz = sv + 3 * nv - off;
-    gv.ed = sv + 4 * nv - off;
+    gv.ec = sv;
+    gv.exy = sv + nv;
+    gv.exz = sv + 2 * nv;
+    gv.eyz = sv + 3 * nv;
+    gv.ed = sv + 4 * nv;
   }
   // stage tile + halo
   for (int i = threadIdx.x; i < nh; i += blockDim.x) {
@@ -314,20 +323,25 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
     const int ly = i - q * hy, lz = (int)fdiv(q, tg.dhx), lx = q - lz * hx;
     int m2 = y0 + ly - 1, m1 = x0 + lx - 1, m3 = z0 + lz - 1;
     T v0 = T(0), v1 = T(0), v2 = T(0), v3 = T(0);
+    unsigned char f = 0;
     if (m1 >= 0 && m1 < g.X && m2 >= 0 && m2 < g.Y && m3 >= 0 && m3 < g.Z) {
-      const T *p = xold + 4 * g.nid(m1, m2, m3);
+      const long n = g.nid(m1, m2, m3);
+      const T *p = xold + 4 * n;
       v0 = p[0]; v1 = p[1]; v2 = p[2]; v3 = p[3];
+      f = flags[n];
     }
     sx[i] = v0; sx[nh + i] = v1; sx[2 * nh + i] = v2; sx[3 * nh + i] = v3;
+    if (lx >= 1 && lx <= tx && ly >= 1 && ly <= ty && lz >= 1 && lz <= tz)
+      sf[__umul24(__umul24(lz - 1, tx) + lx - 1, ty) + ly - 1] = f;
   }
   __syncthreads();
   const int nphase = withp ? 7 : 6;
   for (int ph = 0; ph < nphase; ph++) {
     const int colour = ph & 1;
-    if (ph < 2) smooth_phase<T, 1>(g, gv, flags, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
-    else if (ph < 4) smooth_phase<T, 2>(g, gv, flags, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
-    else if (ph < 6) smooth_phase<T, 3>(g, gv, flags, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
-    else smooth_phase<T, 0>(g, gv, flags, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
+    if (ph < 2) smooth_phase<T, 1, SV>(g, gv, sf, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
+    else if (ph < 4) smooth_phase<T, 2, SV>(g, gv, sf, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
+    else if (ph < 6) smooth_phase<T, 3, SV>(g, gv, sf, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
+    else smooth_phase<T, 0, SV>(g, gv, sf, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
     __syncthreads();
   }
   // write the tile interior

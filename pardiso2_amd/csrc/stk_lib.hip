@@ -409,7 +409,8 @@ template <typename T>
 size_t smooth_lds(const int *tile, bool sv) {
   const size_t nh = (size_t)(tile[0] + 2) * (tile[1] + 2) * (tile[2] + 2);
   const size_t nv = (size_t)(tile[0] + 1) * (tile[1] + 1) * (tile[2] + 1);
-  return (4 * nh + (sv ? 5 * nv : 0)) * sizeof(T);
+  const size_t nt = (size_t)tile[0] * tile[1] * tile[2];  // interior kinds, one byte per node
+  return (4 * nh + (sv ? 5 * nv : 0)) * sizeof(T) + ((nt + 15) & ~(size_t)15);
 }
 
 const int kTileDefault4[4] = {32, 4, 4, 0};

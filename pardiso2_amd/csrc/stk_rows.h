@@ -107,90 +107,96 @@ __device__ __forceinline__ void cadd(T w, T ne, T xk, T yk, T zk, T a[6]) {
   a[4] = -zk * w * ne; a[5] = +zk * w * ne;
 }
 
+// Row builders: (m1,m2,m3) index the spacing tables; v is the node's index
+// in the viscosity arrays (strides g.Y, g.XY), from which every viscosity the
+// row reads lies at offset 0 or +1 per axis.
 // x-momentum row at vx(m1,m2,m3); s[1..17]
 template <typename T>
-__device__ __forceinline__ void row_x(const Geo<T> &g, int m1, int m2, int m3, T s[18]) {
+__device__ __forceinline__ void row_x(const Geo<T> &g, int m1, int m2, int m3, long v, T s[18]) {
+  const long Y = g.Y, XY = g.XY;
   const T xkf = T(2) * g.iBx[m1];
   const T ykf = g.iAy[m2 + 1];
   const T zkf = g.iAz[m3 + 1];
   T a[6], e[4];
   s[16] = +xkf; s[17] = -xkf;
-  snn<T, 0>(-xkf, g.ec[g.nid(m1, m2 + 1, m3 + 1)], g.iAx[m1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+  snn<T, 0>(-xkf, g.ec[v + (XY + 1)], g.iAx[m1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
   s[2] = a[0]; s[1] = a[1]; s[8] = a[2]; s[9] = a[3]; s[12] = a[4]; s[13] = a[5];
-  snn<T, 0>(xkf, g.ec[g.nid(m1 + 1, m2 + 1, m3 + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+  snn<T, 0>(xkf, g.ec[v + (XY + Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
   s[1] += a[0]; s[3] = a[1]; s[10] = a[2]; s[11] = a[3]; s[14] = a[4]; s[15] = a[5];
-  sshear<T>(-ykf, g.exy[g.nid(m1, m2, m3)], g.iBy[m2], g.iBx[m1], e);
+  sshear<T>(-ykf, g.exy[v], g.iBy[m2], g.iBx[m1], e);
   s[1] += e[1]; s[4] = e[0]; s[8] += e[2]; s[10] += e[3];
-  sshear<T>(ykf, g.exy[g.nid(m1, m2 + 1, m3)], g.iBy[m2 + 1], g.iBx[m1], e);
+  sshear<T>(ykf, g.exy[v + 1], g.iBy[m2 + 1], g.iBx[m1], e);
   s[1] += e[0]; s[5] = e[1]; s[9] += e[2]; s[11] += e[3];
-  sshear<T>(-zkf, g.exz[g.nid(m1, m2, m3)], g.iBz[m3], g.iBx[m1], e);
+  sshear<T>(-zkf, g.exz[v], g.iBz[m3], g.iBx[m1], e);
   s[1] += e[1]; s[6] = e[0]; s[12] += e[2]; s[14] += e[3];
-  sshear<T>(zkf, g.exz[g.nid(m1, m2, m3 + 1)], g.iBz[m3 + 1], g.iBx[m1], e);
+  sshear<T>(zkf, g.exz[v + XY], g.iBz[m3 + 1], g.iBx[m1], e);
   s[1] += e[0]; s[7] = e[1]; s[13] += e[2]; s[15] += e[3];
   if (g.p2 != T(0)) {
-    cadd<T>(-xkf * g.p2, g.ed[g.nid(m1, m2 + 1, m3 + 1)], g.iAx[m1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+    cadd<T>(-xkf * g.p2, g.ed[v + (XY + 1)], g.iAx[m1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
     s[2] += a[0]; s[1] += a[1]; s[8] += a[2]; s[9] += a[3]; s[12] += a[4]; s[13] += a[5];
-    cadd<T>(xkf * g.p2, g.ed[g.nid(m1 + 1, m2 + 1, m3 + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+    cadd<T>(xkf * g.p2, g.ed[v + (XY + Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
     s[1] += a[0]; s[3] += a[1]; s[10] += a[2]; s[11] += a[3]; s[14] += a[4]; s[15] += a[5];
   }
 }
 
 // y-momentum row at vy(m1,m2,m3)
 template <typename T>
-__device__ __forceinline__ void row_y(const Geo<T> &g, int m1, int m2, int m3, T s[18]) {
+__device__ __forceinline__ void row_y(const Geo<T> &g, int m1, int m2, int m3, long v, T s[18]) {
+  const long Y = g.Y, XY = g.XY;
   const T xkf = g.iAx[m1 + 1];
   const T ykf = T(2) * g.iBy[m2];
   const T zkf = g.iAz[m3 + 1];
   T a[6], e[4];
   s[16] = +ykf; s[17] = -ykf;
-  snn<T, 1>(-ykf, g.ec[g.nid(m1 + 1, m2, m3 + 1)], g.iAx[m1 + 1], g.iAy[m2], g.iAz[m3 + 1], a);
+  snn<T, 1>(-ykf, g.ec[v + (XY + Y)], g.iAx[m1 + 1], g.iAy[m2], g.iAz[m3 + 1], a);
   s[8] = a[0]; s[10] = a[1]; s[4] = a[2]; s[1] = a[3]; s[12] = a[4]; s[14] = a[5];
-  snn<T, 1>(ykf, g.ec[g.nid(m1 + 1, m2 + 1, m3 + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+  snn<T, 1>(ykf, g.ec[v + (XY + Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
   s[9] = a[0]; s[11] = a[1]; s[1] += a[2]; s[5] = a[3]; s[13] = a[4]; s[15] = a[5];
   // sxy(m1,m2,m3,-xkf): vx lower/upper along y, vy lower/upper along x
-  sshear<T>(-xkf, g.exy[g.nid(m1, m2, m3)], g.iBy[m2], g.iBx[m1], e);
+  sshear<T>(-xkf, g.exy[v], g.iBy[m2], g.iBx[m1], e);
   s[8] += e[0]; s[9] += e[1]; s[2] = e[2]; s[1] += e[3];
-  sshear<T>(xkf, g.exy[g.nid(m1 + 1, m2, m3)], g.iBy[m2], g.iBx[m1 + 1], e);
+  sshear<T>(xkf, g.exy[v + Y], g.iBy[m2], g.iBx[m1 + 1], e);
   s[10] += e[0]; s[11] += e[1]; s[1] += e[2]; s[3] = e[3];
   // syz(m1,m2,m3,-zkf): vy lower/upper along z, vz lower/upper along y
-  sshear<T>(-zkf, g.eyz[g.nid(m1, m2, m3)], g.iBz[m3], g.iBy[m2], e);
+  sshear<T>(-zkf, g.eyz[v], g.iBz[m3], g.iBy[m2], e);
   s[6] = e[0]; s[1] += e[1]; s[12] += e[2]; s[13] += e[3];
-  sshear<T>(zkf, g.eyz[g.nid(m1, m2, m3 + 1)], g.iBz[m3 + 1], g.iBy[m2], e);
+  sshear<T>(zkf, g.eyz[v + XY], g.iBz[m3 + 1], g.iBy[m2], e);
   s[1] += e[0]; s[7] = e[1]; s[14] += e[2]; s[15] += e[3];
   if (g.p2 != T(0)) {
-    cadd<T>(-ykf * g.p2, g.ed[g.nid(m1 + 1, m2, m3 + 1)], g.iAx[m1 + 1], g.iAy[m2], g.iAz[m3 + 1], a);
+    cadd<T>(-ykf * g.p2, g.ed[v + (XY + Y)], g.iAx[m1 + 1], g.iAy[m2], g.iAz[m3 + 1], a);
     s[8] += a[0]; s[10] += a[1]; s[4] += a[2]; s[1] += a[3]; s[12] += a[4]; s[14] += a[5];
-    cadd<T>(ykf * g.p2, g.ed[g.nid(m1 + 1, m2 + 1, m3 + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+    cadd<T>(ykf * g.p2, g.ed[v + (XY + Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
     s[9] += a[0]; s[11] += a[1]; s[1] += a[2]; s[5] += a[3]; s[13] += a[4]; s[15] += a[5];
   }
 }
 
 // z-momentum row at vz(m1,m2,m3)
 template <typename T>
-__device__ __forceinline__ void row_z(const Geo<T> &g, int m1, int m2, int m3, T s[18]) {
+__device__ __forceinline__ void row_z(const Geo<T> &g, int m1, int m2, int m3, long v, T s[18]) {
+  const long Y = g.Y, XY = g.XY;
   const T xkf = g.iAx[m1 + 1];
   const T ykf = g.iAy[m2 + 1];
   const T zkf = T(2) * g.iBz[m3];
   T a[6], e[4];
   s[16] = +zkf; s[17] = -zkf;
-  snn<T, 2>(-zkf, g.ec[g.nid(m1 + 1, m2 + 1, m3)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3], a);
+  snn<T, 2>(-zkf, g.ec[v + (Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3], a);
   s[12] = a[0]; s[14] = a[1]; s[8] = a[2]; s[9] = a[3]; s[6] = a[4]; s[1] = a[5];
-  snn<T, 2>(zkf, g.ec[g.nid(m1 + 1, m2 + 1, m3 + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+  snn<T, 2>(zkf, g.ec[v + (XY + Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
   s[13] = a[0]; s[15] = a[1]; s[10] = a[2]; s[11] = a[3]; s[1] += a[4]; s[7] = a[5];
   // sxz(m1,m2,m3,-xkf): vx lower/upper along z, vz lower/upper along x
-  sshear<T>(-xkf, g.exz[g.nid(m1, m2, m3)], g.iBz[m3], g.iBx[m1], e);
+  sshear<T>(-xkf, g.exz[v], g.iBz[m3], g.iBx[m1], e);
   s[12] += e[0]; s[13] += e[1]; s[2] = e[2]; s[1] += e[3];
-  sshear<T>(xkf, g.exz[g.nid(m1 + 1, m2, m3)], g.iBz[m3], g.iBx[m1 + 1], e);
+  sshear<T>(xkf, g.exz[v + Y], g.iBz[m3], g.iBx[m1 + 1], e);
   s[14] += e[0]; s[15] += e[1]; s[1] += e[2]; s[3] = e[3];
   // syz(m1,m2,m3,-ykf): vy lower/upper along z, vz lower/upper along y
-  sshear<T>(-ykf, g.eyz[g.nid(m1, m2, m3)], g.iBz[m3], g.iBy[m2], e);
+  sshear<T>(-ykf, g.eyz[v], g.iBz[m3], g.iBy[m2], e);
   s[8] += e[0]; s[10] += e[1]; s[4] = e[2]; s[1] += e[3];
-  sshear<T>(ykf, g.eyz[g.nid(m1, m2 + 1, m3)], g.iBz[m3], g.iBy[m2 + 1], e);
+  sshear<T>(ykf, g.eyz[v + 1], g.iBz[m3], g.iBy[m2 + 1], e);
   s[9] += e[0]; s[11] += e[1]; s[1] += e[2]; s[5] = e[3];
   if (g.p2 != T(0)) {
-    cadd<T>(-zkf * g.p2, g.ed[g.nid(m1 + 1, m2 + 1, m3)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3], a);
+    cadd<T>(-zkf * g.p2, g.ed[v + (Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3], a);
     s[12] += a[0]; s[14] += a[1]; s[8] += a[2]; s[9] += a[3]; s[6] += a[4]; s[1] += a[5];
-    cadd<T>(zkf * g.p2, g.ed[g.nid(m1 + 1, m2 + 1, m3 + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+    cadd<T>(zkf * g.p2, g.ed[v + (XY + Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
     s[13] += a[0]; s[15] += a[1]; s[10] += a[2]; s[11] += a[3]; s[1] += a[4]; s[7] += a[5];
   }
 }
@@ -204,9 +210,10 @@ __device__ __forceinline__ void row_p(const Geo<T> &g, int m1, int m2, int m3, T
 
 template <typename T>
 __device__ __forceinline__ int row_coef(const Geo<T> &g, int c, int m1, int m2, int m3, T s[18]) {
-  if (c == 1) { row_x<T>(g, m1, m2, m3, s); return 17; }
-  if (c == 2) { row_y<T>(g, m1, m2, m3, s); return 17; }
-  if (c == 3) { row_z<T>(g, m1, m2, m3, s); return 17; }
+  const long v = g.nid(m1, m2, m3);
+  if (c == 1) { row_x<T>(g, m1, m2, m3, v, s); return 17; }
+  if (c == 2) { row_y<T>(g, m1, m2, m3, v, s); return 17; }
+  if (c == 3) { row_z<T>(g, m1, m2, m3, v, s); return 17; }
   row_p<T>(g, m1, m2, m3, s);
   return 6;
 }
@@ -243,11 +250,15 @@ template <> struct Slot<3> {
 };
 
 template <typename T, int C>
-__device__ __forceinline__ void row_coef_c(const Geo<T> &g, int m1, int m2, int m3, T s[18]) {
-  if (C == 1) row_x<T>(g, m1, m2, m3, s);
-  else if (C == 2) row_y<T>(g, m1, m2, m3, s);
-  else if (C == 3) row_z<T>(g, m1, m2, m3, s);
+__device__ __forceinline__ void row_coef_c(const Geo<T> &g, int m1, int m2, int m3, long v, T s[18]) {
+  if (C == 1) row_x<T>(g, m1, m2, m3, v, s);
+  else if (C == 2) row_y<T>(g, m1, m2, m3, v, s);
+  else if (C == 3) row_z<T>(g, m1, m2, m3, v, s);
   else row_p<T>(g, m1, m2, m3, s);
+}
+template <typename T, int C>
+__device__ __forceinline__ void row_coef_c(const Geo<T> &g, int m1, int m2, int m3, T s[18]) {
+  row_coef_c<T, C>(g, m1, m2, m3, g.nid(m1, m2, m3), s);
 }
 
 // sum over the slots of a row against an AoS vector x (4 per node):
