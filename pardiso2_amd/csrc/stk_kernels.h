@@ -201,6 +201,9 @@ __host__ __device__ inline FDiv make_fdiv(unsigned d) {
   return FDiv{d, d > 1 ? (unsigned)((0x100000000ull + d - 1) / d) : 0u};
 }
 __device__ __forceinline__ unsigned fdiv(unsigned i, FDiv f) { return f.d == 1 ? i : __umulhi(i, f.m); }
+// i / D for a divisor known at compile time (D > 0), else by f
+template <int D>
+__device__ __forceinline__ unsigned tdiv(unsigned i, FDiv f) { return D ? i / (unsigned)(D ? D : 1) : fdiv(i, f); }
 
 // Velocity update residual / diagonal: FP64 divides as the reference does;
 // the FP32 smoother multiplies by the hardware reciprocal (1 ulp).
@@ -230,14 +233,14 @@ inline TileGeo make_tile(int ty, int tx, int tz, const int box[6]) {
 
 // ---------------------------------------------------------------------------
 // one phase of the tile smoother for component C (colour for velocities)
-template <typename T, int C, bool SV>
+template <typename T, int C, bool SV, int TY, int TX, int TZ>
 __device__ __forceinline__ void smooth_phase(const Geo<T> &g, const Geo<T> &gv, const unsigned char *__restrict__ sf,
                                              const T *__restrict__ b, T *sx, int nh, const TileGeo &tg,
                                              int x0, int y0, int z0, int colour, T v0koef, T p0koef) {
-  int ty = tg.ty, tx = tg.tx, tz = tg.tz;
-  // values derived from the tile shape are rebuilt per phase rather than
-  // hoisted out of the caller's phase loop and held in scalar registers
-  asm volatile("" : "+s"(ty), "+s"(tx), "+s"(tz));
+  int ty = TY ? TY : tg.ty, tx = TX ? TX : tg.tx, tz = TZ ? TZ : tg.tz;
+  // values derived from a run-time tile shape are rebuilt per phase rather
+  // than hoisted out of the caller's phase loop and held in scalar registers
+  if constexpr (!TY) asm volatile("" : "+s"(ty), "+s"(tx), "+s"(tz));
   const int hy = ty + 2, hx = tx + 2;
   const int hty = (ty + 1) / 2;
   const int ncand = (C == 0) ? ty * tx * tz : hty * tx * tz;
@@ -246,12 +249,12 @@ __device__ __forceinline__ void smooth_phase(const Geo<T> &g, const Geo<T> &gv, 
     // index products in 24-bit multiplies (full rate): tile and level
     // extents stay below 2^24 and node numbers below 2^32
     if (C == 0) {
-      const int q = (int)fdiv(i, tg.dty);
-      ly = i - (int)__umul24(q, ty); lz = (int)fdiv(q, tg.dtx); lx = q - (int)__umul24(lz, tx);
+      const int q = (int)tdiv<TY>(i, tg.dty);
+      ly = i - (int)__umul24(q, ty); lz = (int)tdiv<TX>(q, tg.dtx); lx = q - (int)__umul24(lz, tx);
     } else {
-      const int q = (int)fdiv(i, tg.dhty);
+      const int q = (int)tdiv<(TY + 1) / 2>(i, tg.dhty);
       const int kk = i - (int)__umul24(q, hty);
-      lz = (int)fdiv(q, tg.dtx); lx = q - (int)__umul24(lz, tx);
+      lz = (int)tdiv<TX>(q, tg.dtx); lx = q - (int)__umul24(lz, tx);
       int par = (colour - (x0 + lx) - (z0 + lz) - y0) & 1;
       ly = 2 * kk + par;
       if (ly >= ty) continue;
@@ -282,14 +285,14 @@ __device__ __forceinline__ void smooth_phase(const Geo<T> &g, const Geo<T> &gv, 
 // (strides ty+1, (ty+1)(tx+1)), so each viscosity comes from HBM once per tile.
 // The interior's unknown kinds are staged too (last in LDS, one byte per node),
 // so a phase's row loads do not wait on a flag load from HBM.
-template <typename T, bool SV>
+template <typename T, bool SV, int TY = 0, int TX = 0, int TZ = 0>
 __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *__restrict__ flags,
                                                 const T *__restrict__ xold, T *__restrict__ xnew,
                                                 const T *__restrict__ b, TileGeo tg, int withp, T v0koef,
                                                 T p0koef) {
   extern __shared__ unsigned char smem_raw[];
   T *sx = reinterpret_cast<T *>(smem_raw);
-  const int ty = tg.ty, tx = tg.tx, tz = tg.tz, ntx = tg.ntx, nty = tg.nty;
+  const int ty = TY ? TY : tg.ty, tx = TX ? TX : tg.tx, tz = TZ ? TZ : tg.tz, ntx = tg.ntx, nty = tg.nty;
   const int hy = ty + 2, hx = tx + 2, hz = tz + 2;
   const int bt = (int)xcd_block();
   const int by = bt % nty, bx = (bt / nty) % ntx, bz = bt / (nty * ntx);
@@ -319,8 +322,8 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
   }
   // stage tile + halo
   for (int i = threadIdx.x; i < nh; i += blockDim.x) {
-    const int q = (int)fdiv(i, tg.dhy);
-    const int ly = i - q * hy, lz = (int)fdiv(q, tg.dhx), lx = q - lz * hx;
+    const int q = (int)tdiv<TY ? TY + 2 : 0>(i, tg.dhy);
+    const int ly = i - q * hy, lz = (int)tdiv<TX ? TX + 2 : 0>(q, tg.dhx), lx = q - lz * hx;
     int m2 = y0 + ly - 1, m1 = x0 + lx - 1, m3 = z0 + lz - 1;
     T v0 = T(0), v1 = T(0), v2 = T(0), v3 = T(0);
     unsigned char f = 0;
@@ -338,17 +341,17 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
   const int nphase = withp ? 7 : 6;
   for (int ph = 0; ph < nphase; ph++) {
     const int colour = ph & 1;
-    if (ph < 2) smooth_phase<T, 1, SV>(g, gv, sf, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
-    else if (ph < 4) smooth_phase<T, 2, SV>(g, gv, sf, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
-    else if (ph < 6) smooth_phase<T, 3, SV>(g, gv, sf, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
-    else smooth_phase<T, 0, SV>(g, gv, sf, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
+    if (ph < 2) smooth_phase<T, 1, SV, TY, TX, TZ>(g, gv, sf, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
+    else if (ph < 4) smooth_phase<T, 2, SV, TY, TX, TZ>(g, gv, sf, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
+    else if (ph < 6) smooth_phase<T, 3, SV, TY, TX, TZ>(g, gv, sf, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
+    else smooth_phase<T, 0, SV, TY, TX, TZ>(g, gv, sf, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
     __syncthreads();
   }
   // write the tile interior
   const int nt = ty * tx * tz;
   for (int i = threadIdx.x; i < nt; i += blockDim.x) {
-    const int q = (int)fdiv(i, tg.dty);
-    const int ly = i - q * ty, lz = (int)fdiv(q, tg.dtx), lx = q - lz * tx;
+    const int q = (int)tdiv<TY>(i, tg.dty);
+    const int ly = i - q * ty, lz = (int)tdiv<TX>(q, tg.dtx), lx = q - lz * tx;
     int m1 = x0 + lx, m2 = y0 + ly, m3 = z0 + lz;
     if (m1 >= g.X || m2 >= g.Y || m3 >= g.Z) continue;
     int li = ((lz + 1) * hx + (lx + 1)) * hy + (ly + 1);

@@ -418,6 +418,23 @@ template <typename T> const int *tuned_tile(const Level &L);
 template <> const int *tuned_tile<float>(const Level &L) { return L.tune32[0] ? L.tune32 : kTileDefault4; }
 template <> const int *tuned_tile<double>(const Level &L) { return L.tune64[0] ? L.tune64 : kTileDefault4; }
 
+// One launch of the tile smoother.  The 32x4x4 tile (the default, and the
+// usual choice on the finest levels) runs an instance with its shape compiled
+// in: the rows' LDS offsets become immediates and index divisions shifts.
+template <typename T>
+void launch_smooth(stk_ctx *c, const int *tile, bool sv, const TileGeo &tg, const Geo<T> &g, const unsigned char *flags,
+                   const T *xo, T *xn, const T *b, int withp) {
+  const dim3 grid(tg.nty * tg.ntx * tg.ntz), blk(256);
+  const size_t lds = smooth_lds<T>(tile, sv);
+  const T v0 = (T)c->kn.v0koef, p0 = (T)c->kn.p0koef;
+  if (sv)
+    hipLaunchKernelGGL((k_smooth<T, true>), grid, blk, lds, c->st, g, flags, xo, xn, b, tg, withp, v0, p0);
+  else if (tile[0] == 32 && tile[1] == 4 && tile[2] == 4)
+    hipLaunchKernelGGL((k_smooth<T, false, 32, 4, 4>), grid, blk, lds, c->st, g, flags, xo, xn, b, tg, withp, v0, p0);
+  else
+    hipLaunchKernelGGL((k_smooth<T, false>), grid, blk, lds, c->st, g, flags, xo, xn, b, tg, withp, v0, p0);
+}
+
 // tile == nullptr: the level's autotuned tile (or the default one)
 template <typename T>
 void smooth(stk_ctx *c, int l, int sweeps, const int *tile, int withp, bool whole_level = false) {
@@ -443,16 +460,10 @@ void smooth(stk_ctx *c, int l, int sweeps, const int *tile, int withp, bool whol
   }
   int ty = tile[0], tx = tile[1], tz = tile[2];
   const TileGeo tg = make_tile(ty, tx, tz, L.box);
-  const size_t lds = smooth_lds<T>(tile, false), ldsv = smooth_lds<T>(tile, true);
-  const bool sv = tuned && tile[3] && ldsv <= kLevelLds;
+  const bool sv = tuned && tile[3] && smooth_lds<T>(tile, true) <= kLevelLds;
   Geo<T> g = L.geo<T>(c->kn.p2koef);
   for (int s = 0; s < sweeps; s++) {
-    if (sv)
-      hipLaunchKernelGGL((k_smooth<T, true>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), ldsv, c->st, g, L.flags, lvx<T>(L),
-                         lvxo<T>(L), lvb<T>(L), tg, withp, (T)c->kn.v0koef, (T)c->kn.p0koef);
-    else
-      hipLaunchKernelGGL((k_smooth<T, false>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), lds, c->st, g, L.flags, lvx<T>(L),
-                         lvxo<T>(L), lvb<T>(L), tg, withp, (T)c->kn.v0koef, (T)c->kn.p0koef);
+    launch_smooth<T>(c, tile, sv, tg, g, L.flags, lvx<T>(L), lvxo<T>(L), lvb<T>(L), withp);
     lvflip<T>(L);
     bc_passes<T>(c, L, lvb<T>(L), lvx<T>(L), withp ? 0 : 1);
   }
@@ -974,17 +985,9 @@ static double time_smooth(stk_ctx *c, int l, const int *tile, int reps) {
   const int ty = tile[0], tx = tile[1], tz = tile[2];
   const TileGeo tg = make_tile(ty, tx, tz, L.box);
   const bool sv = tile[3] && smooth_lds<T>(tile, true) <= kLevelLds;
-  const size_t lds = smooth_lds<T>(tile, sv);
   Geo<T> g = L.geo<T>(c->kn.p2koef);
   T *x0 = lvx<T>(L), *x1 = lvxo<T>(L);
-  auto launch = [&]() {
-    if (sv)
-      hipLaunchKernelGGL((k_smooth<T, true>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), lds, c->st, g, L.flags, x0, x1,
-                         lvb<T>(L), tg, 0, (T)c->kn.v0koef, (T)c->kn.p0koef);
-    else
-      hipLaunchKernelGGL((k_smooth<T, false>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), lds, c->st, g, L.flags, x0, x1,
-                         lvb<T>(L), tg, 0, (T)c->kn.v0koef, (T)c->kn.p0koef);
-  };
+  auto launch = [&]() { launch_smooth<T>(c, tile, sv, tg, g, L.flags, x0, x1, lvb<T>(L), 0); };
   launch();
   hipEventRecord(c->ev0, c->st);
   for (int i = 0; i < reps; i++) launch();
@@ -1735,25 +1738,11 @@ int stk_time_kernel(stk_ctx *c, int which, int level, int reps, double *ms_avg) 
   auto launch = [&]() {
     if (which == 0) {
       Geo<float> g = L.geo<float>(c->kn.p2koef);
-      if (tt[3])
-        hipLaunchKernelGGL((k_smooth<float, true>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), smooth_lds<float>(tile, true),
-                           c->st, g, L.flags, L.x32[0], L.x32[1], L.b32, tg, 0, (float)c->kn.v0koef,
-                           (float)c->kn.p0koef);
-      else
-        hipLaunchKernelGGL((k_smooth<float, false>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), smooth_lds<float>(tile, false),
-                           c->st, g, L.flags, L.x32[0], L.x32[1], L.b32, tg, 0, (float)c->kn.v0koef,
-                           (float)c->kn.p0koef);
+      launch_smooth<float>(c, tile, tt[3] != 0, tg, g, L.flags, L.x32[0], L.x32[1], L.b32, 0);
     } else if (which == 1) {
       Geo<double> g = L.geo<double>(c->kn.p2koef);
       const bool sv = tt[3] && smooth_lds<double>(tile, true) <= kLevelLds;
-      if (sv)
-        hipLaunchKernelGGL((k_smooth<double, true>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), smooth_lds<double>(tile, true),
-                           c->st, g, L.flags, L.x64[0], L.x64[1], L.b64, tg, 0, c->kn.v0koef,
-                           c->kn.p0koef);
-      else
-        hipLaunchKernelGGL((k_smooth<double, false>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256), smooth_lds<double>(tile, false),
-                           c->st, g, L.flags, L.x64[0], L.x64[1], L.b64, tg, 0, c->kn.v0koef,
-                           c->kn.p0koef);
+      launch_smooth<double>(c, tile, sv, tg, g, L.flags, L.x64[0], L.x64[1], L.b64, 0);
     } else if (which == 2) {
       Geo<double> g = L.geo<double>(c->kn.p2koef);
       hipLaunchKernelGGL((k_rowop<double, false>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, g, L.flags,
