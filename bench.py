@@ -55,7 +55,7 @@ def make_problem(workload: str, n: int, seed: int):
 
 
 def sweeps_for(L: int):
-    pre = [2, 4, 8, 8, 12, 12, 12, 12, 12, 12, 12][:L] + [0]
+    pre = [3, 6, 8, 8, 12, 12, 12, 12, 12, 12, 12][:L] + [0]
     return pre, pre
 
 

@@ -327,15 +327,15 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
     int m2 = y0 + ly - 1, m1 = x0 + lx - 1, m3 = z0 + lz - 1;
     T v0 = T(0), v1 = T(0), v2 = T(0), v3 = T(0);
     unsigned char f = 0;
+    const bool interior = lx >= 1 && lx <= tx && ly >= 1 && ly <= ty && lz >= 1 && lz <= tz;
     if (m1 >= 0 && m1 < g.X && m2 >= 0 && m2 < g.Y && m3 >= 0 && m3 < g.Z) {
       const long n = g.nid(m1, m2, m3);
       const T *p = xold + 4 * n;
       v0 = p[0]; v1 = p[1]; v2 = p[2]; v3 = p[3];
-      f = flags[n];
+      if (interior) f = flags[n];  // kinds of the halo are not needed
     }
     sx[i] = v0; sx[nh + i] = v1; sx[2 * nh + i] = v2; sx[3 * nh + i] = v3;
-    if (lx >= 1 && lx <= tx && ly >= 1 && ly <= ty && lz >= 1 && lz <= tz)
-      sf[__umul24(__umul24(lz - 1, tx) + lx - 1, ty) + ly - 1] = f;
+    if (interior) sf[__umul24(__umul24(lz - 1, tx) + lx - 1, ty) + ly - 1] = f;
   }
   __syncthreads();
   const int nphase = withp ? 7 : 6;

@@ -1195,8 +1195,8 @@ void stk_default_opts(stk_solver_opts *o, int levels) {
   memset(o, 0, sizeof(*o));
   o->precision = 0;
   o->restart = 128;
-  // per level from the finest; measured over random-field, inclusion and sinker boxes (33^3..129^3)
-  int pre[12] = {2, 4, 8, 8, 12, 12, 12, 12, 12, 12, 12, 12};
+  // per level from the finest; measured over random-field, inclusion and sinker boxes (33^3..257^3)
+  int pre[12] = {3, 6, 8, 8, 12, 12, 12, 12, 12, 12, 12, 12};
   for (int l = 0; l < 12; l++) { o->pre[l] = pre[l]; o->post[l] = pre[l]; }
   (void)levels;
   o->tile[0] = o->tile[1] = o->tile[2] = 0;  // per-level autotuned tile, else 32x4x4

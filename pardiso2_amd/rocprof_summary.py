@@ -3,7 +3,7 @@
     python -m pardiso2_amd.rocprof_summary stats <dir> [--top N]
     python -m pardiso2_amd.rocprof_summary trace <dir> --kernel SUBSTR [--last N]
     python -m pardiso2_amd.rocprof_summary pmc <dir> [--kernel SUBSTR] [--last N]
-    python -m pardiso2_amd.rocprof_summary window <dir> --since SUBSTR [--nth K] [--top N]
+    python -m pardiso2_amd.rocprof_summary window <dir> --since SUBSTR [--nth K] [--top N] [--by-grid]
 
 `stats` reads the *_kernel_stats.csv that `rocprofv3 --kernel-trace --stats
 --output-format csv` writes; `trace` reads the same run's *_kernel_trace.csv
@@ -13,7 +13,9 @@ them with --last); `pmc` reads *_counter_collection.csv of a `rocprofv3 --pmc
 number of dispatches and the mean value per dispatch (of the last N);
 `window` takes the kernel trace from the K-th last dispatch whose name holds
 SUBSTR to the end (e.g. one timed solve step) and prints per-kernel totals,
-the window's span and the share of it the device spent in kernels.
+the window's span and the share of it the device spent in kernels (with
+--by-grid, one line per kernel and launch shape, so the levels of a hierarchy
+show apart).
 """
 from __future__ import annotations
 
@@ -75,13 +77,15 @@ def trace(d, kernel, last):
           f"median us {ds[len(ds) // 2] / 1e3:.3f}  max us {ds[-1] / 1e3:.3f}")
 
 
-def window(d, since, nth, top):
+def window(d, since, nth, top, by_grid=False):
     rows = []
     for f in _find(d, "kernel_trace.csv"):
         with open(f) as fh:
             for r in csv.DictReader(fh):
-                rows.append((float(_col(r, "Start_Timestamp")), float(_col(r, "End_Timestamp")),
-                             _col(r, "Kernel_Name", "KernelName")))
+                name = _col(r, "Kernel_Name", "KernelName")
+                if by_grid:  # one line per launch shape: grid x workgroup size
+                    name = f"[grid {_col(r, 'Grid_Size_X', 'Grid_Size')}/{_col(r, 'Workgroup_Size_X', 'Workgroup_Size')}] {name}"
+                rows.append((float(_col(r, "Start_Timestamp")), float(_col(r, "End_Timestamp")), name))
     rows.sort()
     marks = [i for i, r in enumerate(rows) if since in r[2]]
     if len(marks) < nth:
@@ -132,13 +136,14 @@ def main(argv=None):
     ap.add_argument("--last", type=int, default=0)
     ap.add_argument("--since", default=None)
     ap.add_argument("--nth", type=int, default=1)
+    ap.add_argument("--by-grid", action="store_true", help="window: one line per kernel and launch shape")
     a = ap.parse_args(argv)
     if a.mode == "stats":
         stats(a.dir, a.top)
     elif a.mode == "trace":
         trace(a.dir, a.kernel or "", a.last)
     elif a.mode == "window":
-        window(a.dir, a.since or "", a.nth, a.top)
+        window(a.dir, a.since or "", a.nth, a.top, a.by_grid)
     else:
         pmc(a.dir, a.kernel, a.last)
 

@@ -29,3 +29,17 @@ def test_trace_last_dispatches(tmp_path, capsys):
     R.trace(str(tmp_path), "k1", 1)
     out = capsys.readouterr().out
     assert "dispatches 1  avg us 3.000" in out
+
+
+def test_window_by_launch_shape(tmp_path, capsys):
+    cols = ["Kernel_Name", "Start_Timestamp", "End_Timestamp", "Dispatch_Id", "Grid_Size_X", "Workgroup_Size_X"]
+    with open(tmp_path / "x_kernel_trace.csv", "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=cols)
+        w.writeheader()
+        for i, (name, t0, t1, g) in enumerate([("mark", 0, 10, 64), ("sweep", 10, 30, 1024),
+                                                ("sweep", 30, 40, 256), ("sweep", 40, 60, 1024)]):
+            w.writerow(dict(zip(cols, [name, t0, t1, i, g, 64])))
+    R.window(str(tmp_path), "mark", 1, 10, by_grid=True)
+    out = capsys.readouterr().out
+    assert "       2      0.000       0.02" in out and "[grid 1024/64] sweep" in out  # two launches, 20 ns each
+    assert "[grid 256/64] sweep" in out
