@@ -174,6 +174,13 @@ int stk_coarse_solve(stk_ctx *ctx);
 int stk_autotune(stk_ctx *ctx, int precision, int reps);
 /* tile (y, x, z, viscosities staged) a solve uses on a level */
 int stk_level_tile(const stk_ctx *ctx, int level, int precision, int out4[4]);
+/* 1 when the tile sweeps of shape ty x tx x tz on a level set the level's
+ * boundary rows themselves (each tile evaluates the chains it owns in
+ * composed form), 0 when a separate chain pass follows every sweep (no
+ * boundary rows, or a chain whose row and partner fall in different tiles).
+ * Builds the tile lists if they do not exist yet.  No counterpart in the
+ * reference: an inspection hook for the tests, like stk_level_tile. */
+int stk_tile_chains(stk_ctx *ctx, int level, int ty, int tx, int tz);
 
 /* ---- many independent instances --------------------------------------- */
 /* Solve n contexts concurrently: `threads` host workers (<= 0: 16) take the

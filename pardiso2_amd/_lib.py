@@ -24,7 +24,7 @@ EXPORTS = [
     "stk_pressure_gauge", "stk_hydrostatic_guess", "stk_time_kernel", "stk_vec_set", "stk_vec_get", "stk_apply", "stk_smooth", "stk_smooth_ex", "stk_vrestrict32", "stk_coarse_solve",
     "stk_bind", "matclear", "gausmat", "gausmat2", "pardisomat", "gausmat4", "vprestrict", "vpprolong",
     "ronurestrict", "vpiterate", "stk_kop_apply", "stk_kcycle_apply", "stk_schur_apply",
-    "stk_solve_batch", "stk_mem_info", "stk_autotune", "stk_level_tile",
+    "stk_solve_batch", "stk_mem_info", "stk_autotune", "stk_level_tile", "stk_tile_chains",
 ]
 
 
@@ -100,6 +100,7 @@ def lib():
         L.stk_mem_info.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
         L.stk_autotune.argtypes = [vp, C.c_int, C.c_int]
         L.stk_level_tile.argtypes = [vp, C.c_int, C.c_int, ip]
+        L.stk_tile_chains.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.stk_kcycle_apply.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, dp]
         L.stk_schur_apply.argtypes = [vp, C.POINTER(Opts), dp, dp]
         L.matclear.argtypes = [C.c_int, dp]
@@ -288,6 +289,12 @@ class Stokes:
             check(lib().stk_level_tile(self.h, l, precision, _ip(t)), "stk_level_tile")
             out.append(tuple(int(v) for v in t))
         return out
+
+    def tile_chains(self, level, tile):
+        """True when sweeps of this tile shape set the level's boundary rows in the tiles."""
+        rc = lib().stk_tile_chains(self.h, level, tile[0], tile[1], abs(tile[2]))
+        check(min(rc, 0), "stk_tile_chains")
+        return rc == 1
 
     def kop_apply(self, level, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -279,17 +279,69 @@ __device__ __forceinline__ void smooth_phase(const Geo<T> &g, const Geo<T> &gv, 
   }
 }
 
+// Boundary rows along their dependency chains in one launch.  Row i's chain
+// is r_0 (the row itself), r_1, ..., r_d, each x_{r_t} = b_{r_t} + k_t x_{r_{t+1}},
+// ending at x_{j_i} with j_i not a boundary row (or no partner, j_i = -1).
+// Every thread evaluates its whole chain from the final partner, so all rows
+// of a chain see the same values without ordering the launch by depth; the
+// partners read are never written here.
+struct BcChain {
+  int nbc, dm;
+  const int *r, *len, *j;
+  const double *k;
+};
+
+// A smoothing call's boundary chains in composed form, evaluated by the tile
+// sweeps themselves: with b fixed for the call, chain i is
+// x_{r_0} = c_i + K_i x_{j_i}, where c_i and K_i fold the chain's b and k
+// terms from the partner end (K_i = 0 without a partner).  A chain whose row
+// lies in a tile's interior belongs to that tile, and its partner too (the
+// tile sets the row in LDS before its write-back); any other chain belongs
+// to the tile whose interior holds its partner (or, without one, the tile
+// nearest its row) and is stored to HBM directly.  Everything but off is in
+// list order (the chains of tile t at positions off[t] .. off[t+1]-1); in
+// device memory, passed by one pointer.
+template <typename T>
+struct TileChains {
+  const int *off;   // [ntiles + 1]
+  const int *idx;   // chain number at each position
+  const int *head;  // its r_0
+  const int *rl;    // LDS index of r_0 in the owning tile (-1: outside every tile)
+  const int *pl;    // LDS index of the partner (-1: none)
+  T *cc, *ck;       // its composed terms
+};
+
+// c and K at every list position from the current b (mode 1: velocity rows
+// only; pressure chains are marked by K = NaN and left alone)
+template <typename T>
+__global__ void k_bc_compose(BcChain ch, const int *__restrict__ idx, const T *__restrict__ b, T *cc, T *ck,
+                             int mode) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ch.nbc) return;
+  const int i = idx[q];
+  const long o = (long)i * ch.dm;
+  T c = T(0), K = ch.j[i] >= 0 ? T(1) : T(0);
+  for (int t = ch.len[i] - 1; t >= 0; t--) {
+    const T k = (T)ch.k[o + t];
+    c = b[ch.r[o + t]] + k * c;
+    K = k * K;
+  }
+  if (mode == 1 && (ch.r[o] & 3) == 0) K = __builtin_nan("");
+  cc[q] = c;
+  ck[q] = K;
+}
+
 // SV: the tile's stencil viscosities (nodes [x0, x0+tx] x [y0, y0+ty] x
 // [z0, z0+tz], every offset a row uses is 0 or +1) are staged in LDS after the
 // unknowns, and the rows read them through a Geo whose arrays are that block
 // (strides ty+1, (ty+1)(tx+1)), so each viscosity comes from HBM once per tile.
 // The interior's unknown kinds are staged too (last in LDS, one byte per node),
 // so a phase's row loads do not wait on a flag load from HBM.
-template <typename T, bool SV, int TY = 0, int TX = 0, int TZ = 0>
+template <typename T, bool SV, int TY = 0, int TX = 0, int TZ = 0, bool CH = false>
 __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *__restrict__ flags,
                                                 const T *__restrict__ xold, T *__restrict__ xnew,
                                                 const T *__restrict__ b, TileGeo tg, int withp, T v0koef,
-                                                T p0koef) {
+                                                T p0koef, const TileChains<T> *__restrict__ tcs) {
   extern __shared__ unsigned char smem_raw[];
   T *sx = reinterpret_cast<T *>(smem_raw);
   const int ty = TY ? TY : tg.ty, tx = TX ? TX : tg.tx, tz = TZ ? TZ : tg.tz, ntx = tg.ntx, nty = tg.nty;
@@ -298,6 +350,8 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
   const int by = bt % nty, bx = (bt / nty) % ntx, bz = bt / (nty * ntx);
   const int y0 = tg.oy + by * ty, x0 = tg.ox + bx * tx, z0 = tg.oz + bz * tz;
   const int nh = hy * hx * hz;
+  int q0 = 0, q1 = 0;
+  if (CH) { q0 = tcs->off[bt]; q1 = tcs->off[bt + 1]; }  // this tile's chains, read early
   unsigned char *sf = smem_raw + (4 * nh + (SV ? 5 * (ty + 1) * (tx + 1) * (tz + 1) : 0)) * sizeof(T);
   Geo<T> gv = g;
   if (SV) {
@@ -347,6 +401,23 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
     else smooth_phase<T, 0, SV, TY, TX, TZ>(g, gv, sf, b, sx, nh, tg, x0, y0, z0, colour, v0koef, p0koef);
     __syncthreads();
   }
+  if (CH) {
+    // this tile's chains on the new values of their partners; partners are
+    // never chain rows, so the LDS writes here read no value they change
+    const int *__restrict__ rl = tcs->rl;
+    const int *__restrict__ pl = tcs->pl;
+    const T *__restrict__ cc = tcs->cc;
+    const T *__restrict__ ck = tcs->ck;
+    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+      const int r = rl[q], j = pl[q];
+      const T K = ck[q];
+      if (K != K) continue;  // a pressure chain in a velocity-only sweep
+      const T v = j >= 0 ? cc[q] + K * sx[j] : cc[q];
+      if (r >= 0) sx[r] = v;
+      else xnew[tcs->head[q]] = v;
+    }
+    __syncthreads();
+  }
   // write the tile interior
   const int nt = ty * tx * tz;
   for (int i = threadIdx.x; i < nt; i += blockDim.x) {
@@ -360,17 +431,6 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
   }
 }
 
-// Boundary rows along their dependency chains in one launch.  Row i's chain
-// is r_0 (the row itself), r_1, ..., r_d, each x_{r_t} = b_{r_t} + k_t x_{r_{t+1}},
-// ending at x_{j_i} with j_i not a boundary row (or no partner, j_i = -1).
-// Every thread evaluates its whole chain from the final partner, so all rows
-// of a chain see the same values without ordering the launch by depth; the
-// partners read are never written here.
-struct BcChain {
-  int nbc, dm;
-  const int *r, *len, *j;
-  const double *k;
-};
 
 template <typename T>
 __device__ __forceinline__ void bc_chain_one(const BcChain &ch, int i, const T *__restrict__ b, T *x, int mode) {

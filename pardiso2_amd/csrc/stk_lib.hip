@@ -107,6 +107,21 @@ struct Level {
   int *ch_r = nullptr, *ch_len = nullptr, *ch_j = nullptr;
   double *ch_k = nullptr;
   BcChain chain() const { return BcChain{nbc, ch_dm, ch_r, ch_len, ch_j, ch_k}; }
+  std::vector<int> hch_r0, hch_j;  // head row and final partner of each chain (host)
+  // chains per smoother tile of shape bt_key, for sweeps that set them in place
+  int bt_key[3] = {0, 0, 0};
+  bool bt_ok = false;
+  int *bt_off = nullptr, *bt_idx = nullptr, *bt_head = nullptr, *bt_rl = nullptr, *bt_pl = nullptr;
+  float *bt_cc32 = nullptr, *bt_ck32 = nullptr;
+  double *bt_cc64 = nullptr, *bt_ck64 = nullptr;
+  TileChains<float> *bt_tc32 = nullptr;
+  TileChains<double> *bt_tc64 = nullptr;
+  void free_tile_chains() {
+    dfree(bt_off); dfree(bt_idx); dfree(bt_head); dfree(bt_rl); dfree(bt_pl); dfree(bt_cc32); dfree(bt_ck32); dfree(bt_cc64); dfree(bt_ck64);
+    dfree(bt_tc32); dfree(bt_tc64);
+    bt_key[0] = bt_key[1] = bt_key[2] = 0;
+    bt_ok = false;
+  }
   // vectors
   float *x32[2] = {}, *b32 = nullptr, *r32 = nullptr;
   double *x64[2] = {}, *b64 = nullptr, *r64 = nullptr;
@@ -421,18 +436,97 @@ template <> const int *tuned_tile<double>(const Level &L) { return L.tune64[0] ?
 // One launch of the tile smoother.  The 32x4x4 tile (the default, and the
 // usual choice on the finest levels) runs an instance with its shape compiled
 // in: the rows' LDS offsets become immediates and index divisions shifts.
+// tcs: the tile chains of this shape (the sweep sets the boundary rows), or
+// nullptr (a separate chain pass follows).
 template <typename T>
 void launch_smooth(stk_ctx *c, const int *tile, bool sv, const TileGeo &tg, const Geo<T> &g, const unsigned char *flags,
-                   const T *xo, T *xn, const T *b, int withp) {
+                   const T *xo, T *xn, const T *b, int withp, const TileChains<T> *tcs = nullptr) {
   const dim3 grid(tg.nty * tg.ntx * tg.ntz), blk(256);
   const size_t lds = smooth_lds<T>(tile, sv);
   const T v0 = (T)c->kn.v0koef, p0 = (T)c->kn.p0koef;
-  if (sv)
-    hipLaunchKernelGGL((k_smooth<T, true>), grid, blk, lds, c->st, g, flags, xo, xn, b, tg, withp, v0, p0);
-  else if (tile[0] == 32 && tile[1] == 4 && tile[2] == 4)
-    hipLaunchKernelGGL((k_smooth<T, false, 32, 4, 4>), grid, blk, lds, c->st, g, flags, xo, xn, b, tg, withp, v0, p0);
-  else
-    hipLaunchKernelGGL((k_smooth<T, false>), grid, blk, lds, c->st, g, flags, xo, xn, b, tg, withp, v0, p0);
+  const bool t344 = tile[0] == 32 && tile[1] == 4 && tile[2] == 4;
+#define STK_SMOOTH(...) \
+  hipLaunchKernelGGL((k_smooth<__VA_ARGS__>), grid, blk, lds, c->st, g, flags, xo, xn, b, tg, withp, v0, p0, tcs)
+  if (sv && tcs) STK_SMOOTH(T, true, 0, 0, 0, true);
+  else if (sv) STK_SMOOTH(T, true);
+  else if (t344 && tcs) STK_SMOOTH(T, false, 32, 4, 4, true);
+  else if (t344) STK_SMOOTH(T, false, 32, 4, 4);
+  else if (tcs) STK_SMOOTH(T, false, 0, 0, 0, true);
+  else STK_SMOOTH(T, false);
+#undef STK_SMOOTH
+}
+
+template <typename T> TileChains<T> *tile_chains(Level &L);
+template <> TileChains<float> *tile_chains<float>(Level &L) { return L.bt_tc32; }
+template <> TileChains<double> *tile_chains<double>(Level &L) { return L.bt_tc64; }
+template <typename T> T *chain_c(Level &L);
+template <> float *chain_c<float>(Level &L) { return L.bt_cc32; }
+template <> double *chain_c<double>(Level &L) { return L.bt_cc64; }
+template <typename T> T *chain_k(Level &L);
+template <> float *chain_k<float>(Level &L) { return L.bt_ck32; }
+template <> double *chain_k<double>(Level &L) { return L.bt_ck64; }
+
+// The per-tile chain lists of `tile` on level L, built on first use.  Not
+// while the stream is being captured (then the sweeps keep the separate
+// pass; the graph is captured after the lists of its tiles exist).  Owners
+// as TileChains describes; false: the separate pass (no chains, or a chain
+// whose row and partner lie in different tiles, or whose partner lies
+// outside every tile).
+bool tile_bc(stk_ctx *c, Level &L, const int *tile) {
+  if (!L.nbc) return false;
+  if (L.bt_key[0] == tile[0] && L.bt_key[1] == tile[1] && L.bt_key[2] == tile[2]) return L.bt_ok;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(c->st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
+  if (L.bt_ok) {  // a captured sweep may name the old lists
+    c->graph_ok = false;
+    (void)hipStreamSynchronize(c->st);
+  }
+  L.free_tile_chains();
+  for (int a = 0; a < 3; a++) L.bt_key[a] = tile[a];
+  const int ty = tile[0], tx = tile[1], tz = tile[2], hy = ty + 2, hx = tx + 2, nh = hy * hx * (tz + 2);
+  const TileGeo tg = make_tile(ty, tx, tz, L.box);
+  const int nt = tg.nty * tg.ntx * tg.ntz;
+  auto clampi = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+  // tile of unknown u (clamped into the tile grid), whether u is in its
+  // interior, and u's LDS index there
+  auto place = [&](int u, int &t, int &lds) {
+    const long n = u >> 2;
+    const int m3 = (int)(n / L.XY), rem = (int)(n - (long)m3 * L.XY), m1 = rem / L.Y, m2 = rem - m1 * L.Y;
+    const int ky = clampi((m2 - tg.oy) / ty, tg.nty - 1);
+    const int kx = clampi((m1 - tg.ox) / tx, tg.ntx - 1);
+    const int kz = clampi((m3 - tg.oz) / tz, tg.ntz - 1);
+    const int ly = m2 - (tg.oy + ky * ty), lx = m1 - (tg.ox + kx * tx), lz = m3 - (tg.oz + kz * tz);
+    t = (kz * tg.ntx + kx) * tg.nty + ky;
+    lds = (u & 3) * nh + ((lz + 1) * hx + (lx + 1)) * hy + (ly + 1);
+    return ly >= 0 && ly < ty && lx >= 0 && lx < tx && lz >= 0 && lz < tz;
+  };
+  std::vector<int> owner(L.nbc), rl(L.nbc), pl(L.nbc), off(nt + 1, 0), idx(L.nbc);
+  for (int i = 0; i < L.nbc; i++) {
+    int tr, lr, tp = -1, lp = -1;
+    const bool rin = place(L.hch_r0[i], tr, lr);
+    if (L.hch_j[i] >= 0 && !place(L.hch_j[i], tp, lp)) return false;
+    if (rin && L.hch_j[i] >= 0 && tp != tr) return false;
+    owner[i] = rin ? tr : (L.hch_j[i] >= 0 ? tp : tr);
+    rl[i] = rin ? lr : -1;
+    pl[i] = lp;
+    off[owner[i] + 1]++;
+  }
+  for (int t = 0; t < nt; t++) off[t + 1] += off[t];
+  std::vector<int> fill(off.begin(), off.end() - 1);
+  for (int i = 0; i < L.nbc; i++) idx[fill[owner[i]]++] = i;
+  std::vector<int> head(L.nbc), rlq(L.nbc), plq(L.nbc);  // in list order
+  for (int q = 0; q < L.nbc; q++) { head[q] = L.hch_r0[idx[q]]; rlq[q] = rl[idx[q]]; plq[q] = pl[idx[q]]; }
+  if (upload(&L.bt_off, off) || upload(&L.bt_idx, idx) || upload(&L.bt_head, head) || upload(&L.bt_rl, rlq) ||
+      upload(&L.bt_pl, plq) || dalloc(&L.bt_cc32, L.nbc) || dalloc(&L.bt_ck32, L.nbc) || dalloc(&L.bt_cc64, L.nbc) ||
+      dalloc(&L.bt_ck64, L.nbc))
+    return false;
+  const std::vector<TileChains<float>> t32{
+      TileChains<float>{L.bt_off, L.bt_idx, L.bt_head, L.bt_rl, L.bt_pl, L.bt_cc32, L.bt_ck32}};
+  const std::vector<TileChains<double>> t64{
+      TileChains<double>{L.bt_off, L.bt_idx, L.bt_head, L.bt_rl, L.bt_pl, L.bt_cc64, L.bt_ck64}};
+  if (upload(&L.bt_tc32, t32) || upload(&L.bt_tc64, t64)) return false;
+  L.bt_ok = true;
+  return true;
 }
 
 // tile == nullptr: the level's autotuned tile (or the default one)
@@ -462,10 +556,15 @@ void smooth(stk_ctx *c, int l, int sweeps, const int *tile, int withp, bool whol
   const TileGeo tg = make_tile(ty, tx, tz, L.box);
   const bool sv = tuned && tile[3] && smooth_lds<T>(tile, true) <= kLevelLds;
   Geo<T> g = L.geo<T>(c->kn.p2koef);
+  const bool inplace = sweeps > 0 && tile_bc(c, L, tile);
+  if (inplace)  // the chains of this call's b, for all its sweeps
+    hipLaunchKernelGGL(k_bc_compose<T>, dim3(blocks(L.nbc, 256)), dim3(256), 0, c->st, L.chain(), L.bt_idx,
+                       lvb<T>(L), chain_c<T>(L), chain_k<T>(L), withp ? 0 : 1);
   for (int s = 0; s < sweeps; s++) {
-    launch_smooth<T>(c, tile, sv, tg, g, L.flags, lvx<T>(L), lvxo<T>(L), lvb<T>(L), withp);
+    launch_smooth<T>(c, tile, sv, tg, g, L.flags, lvx<T>(L), lvxo<T>(L), lvb<T>(L), withp,
+                     inplace ? tile_chains<T>(L) : nullptr);
     lvflip<T>(L);
-    bc_passes<T>(c, L, lvb<T>(L), lvx<T>(L), withp ? 0 : 1);
+    if (!inplace) bc_passes<T>(c, L, lvb<T>(L), lvx<T>(L), withp ? 0 : 1);
   }
 }
 
@@ -1147,6 +1246,7 @@ void stk_destroy(stk_ctx *c) {
     for (int a = 0; a < 3; a++) { dfree(L.iA64[a]); dfree(L.iB64[a]); dfree(L.iA32[a]); dfree(L.iB32[a]); dfree(L.g64[a]); }
     for (int k = 0; k < 5; k++) { dfree(L.e64[k]); dfree(L.e32[k]); }
     dfree(L.lc32); dfree(L.rs_t1); dfree(L.rs_t2); dfree(L.rs_ws);
+    L.free_tile_chains();
     dfree(L.flags);
     dfree(L.bc_r); dfree(L.bc_j); dfree(L.bc_negk); dfree(L.bc_c);
     dfree(L.ch_r); dfree(L.ch_len); dfree(L.ch_j); dfree(L.ch_k);
@@ -1593,6 +1693,10 @@ static int build_rows(stk_ctx *c) {
         cl[i] = t;
         cj[i] = j;
       }
+      L.hch_r0.assign(L.nbc, 0);
+      L.hch_j.assign(L.nbc, -1);
+      for (int i = 0; i < L.nbc; i++) { L.hch_r0[i] = cr[(size_t)i * L.ch_dm]; L.hch_j[i] = cj[i]; }
+      L.free_tile_chains();  // rebuilt on the next tile sweep
       dfree(L.ch_r); dfree(L.ch_len); dfree(L.ch_j); dfree(L.ch_k);
       if (upload(&L.ch_r, cr) || upload(&L.ch_len, cl) || upload(&L.ch_j, cj) || upload(&L.ch_k, ck))
         return STK_ENOMEM;
@@ -1789,6 +1893,11 @@ static int ensure_graph(stk_ctx *c, const stk_solver_opts *o, const int *tile) {
   if (!c->gin && (dalloc(&c->gin, n4) || dalloc(&c->gout, n4))) return STK_ENOMEM;
   if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+  // the tile sweeps' chain lists exist before capture, so the graph uses them
+  for (int l = 0; l < c->L; l++) {
+    Level &L = c->lv[l];
+    (void)tile_bc(c, L, tile ? tile : (o->precision == 1 ? tuned_tile<double>(L) : tuned_tile<float>(L)));
+  }
   HIPCHK(hipStreamSynchronize(c->st));
   HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
   if (o->precision == 1) precond<double>(c, o, c->gin, c->gout, tile);
@@ -2195,6 +2304,12 @@ int stk_level_tile(const stk_ctx *c, int level, int precision, int out4[4]) {
   const int *t = precision ? tuned_tile<double>(L) : tuned_tile<float>(L);
   for (int a = 0; a < 4; a++) out4[a] = t[a];
   return STK_OK;
+}
+
+int stk_tile_chains(stk_ctx *c, int level, int ty, int tx, int tz) {
+  if (!c || level < 0 || level >= c->L || ty < 1 || tx < 1 || tz < 1) return STK_EINVAL;
+  const int t[3] = {ty, tx, tz};
+  return tile_bc(c, c->lv[level], t) ? 1 : 0;
 }
 
 // ---- many independent instances ---------------------------------------------
