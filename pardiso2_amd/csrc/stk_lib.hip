@@ -772,11 +772,20 @@ void kcycle(stk_ctx *c, int l, int nu, T scale) {
   }
   const KPlanesC<T> k = kplanes<T>(V);
   const unsigned nb = blocks(V.n, 256);
+  // one thread per cell of the colour where its index decode is exact
+  const unsigned long hy = (V.Y + 1) / 2, ncol = hy * V.X * V.Z;
+  const bool compact = hy * ncol < (1ul << 32);
+  const FDiv dhy = make_fdiv((unsigned)hy), dx = make_fdiv((unsigned)V.X);
   auto sweeps = [&](int c1) {
     for (int s = 0; s < nu; s++)
-      for (int q = 0; q < 2; q++)
-        hipLaunchKernelGGL(k_kgs<T>, dim3(nb), dim3(256), 0, c->st, V.X, V.Y, V.Z, k, kb<T>(V), kx<T>(V),
-                           q == 0 ? c1 : 1 - c1);
+      for (int q = 0; q < 2; q++) {
+        const int col = q == 0 ? c1 : 1 - c1;
+        if (compact)
+          hipLaunchKernelGGL(k_kgs_c<T>, dim3(blocks(ncol, 256)), dim3(256), 0, c->st, V.X, V.Y, V.Z, dhy, dx, k,
+                             kb<T>(V), kx<T>(V), col);
+        else
+          hipLaunchKernelGGL(k_kgs<T>, dim3(nb), dim3(256), 0, c->st, V.X, V.Y, V.Z, k, kb<T>(V), kx<T>(V), col);
+      }
   };
   sweeps(0);
   hipLaunchKernelGGL(k_kres<T>, dim3(nb), dim3(256), 0, c->st, V.X, V.Y, V.Z, k, kb<T>(V), kx<T>(V), kr<T>(V));

@@ -173,12 +173,9 @@ __global__ void k_kagg(int Xf, int Yf, int Xc, int Yc, int Zc, KAgg t, KPlanesC<
 // one red/black Gauss-Seidel half-sweep of K x = b on cells of one colour
 // new value of cell n (false: the cell keeps its value)
 template <typename T>
-__device__ __forceinline__ bool kgs_val(int X, int Y, int Z, const KPlanesC<T> &k, const T *__restrict__ b,
-                                        const T *x, int colour, long n, T &out) {
+__device__ __forceinline__ bool kgs_at(int X, int Y, int Z, const KPlanesC<T> &k, const T *__restrict__ b,
+                                       const T *x, long n, int m1, int m2, int m3, T &out) {
   const int XY = X * Y;
-  int m1, m2, m3;
-  decode(n, X, Y, XY, m1, m2, m3);
-  if (((m1 + m2 + m3) & 1) != colour) return false;  // before any load: half the cells stop here
   const T d = k.p[0][n];
   if (d == T(0)) return false;
   T acc = b[n];
@@ -193,6 +190,15 @@ __device__ __forceinline__ bool kgs_val(int X, int Y, int Z, const KPlanesC<T> &
 }
 
 template <typename T>
+__device__ __forceinline__ bool kgs_val(int X, int Y, int Z, const KPlanesC<T> &k, const T *__restrict__ b,
+                                        const T *x, int colour, long n, T &out) {
+  int m1, m2, m3;
+  decode(n, X, Y, X * Y, m1, m2, m3);
+  if (((m1 + m2 + m3) & 1) != colour) return false;  // before any load: half the cells stop here
+  return kgs_at<T>(X, Y, Z, k, b, x, n, m1, m2, m3, out);
+}
+
+template <typename T>
 __device__ __forceinline__ void kgs_cell(int X, int Y, int Z, const KPlanesC<T> &k, const T *__restrict__ b, T *x,
                                          int colour, long n) {
   T v;
@@ -204,6 +210,25 @@ __global__ void k_kgs(int X, int Y, int Z, KPlanesC<T> k, const T *__restrict__ 
   long n = xcd_block() * blockDim.x + threadIdx.x;
   if (n >= (long)X * Y * Z) return;
   kgs_cell<T>(X, Y, Z, k, b, x, colour, n);
+}
+
+// The same half-sweep with one thread per cell of the colour (no lanes idle
+// on the other colour): thread i is cell m2 = 2 kk + parity of the row
+// q = (m3, m1), kk = i mod hy, q = i / hy with hy = ceil(Y/2).  The
+// multiply-high divisions are exact while hy^2 X Z < 2^32 (the caller checks).
+template <typename T>
+__global__ void k_kgs_c(int X, int Y, int Z, FDiv dhy, FDiv dx, KPlanesC<T> k, const T *__restrict__ b, T *x,
+                        int colour) {
+  const unsigned i = (unsigned)(xcd_block() * blockDim.x + threadIdx.x);
+  const int hy = (Y + 1) / 2;
+  if (i >= (unsigned)hy * X * Z) return;
+  const int q = (int)fdiv(i, dhy), kk = (int)i - q * hy;
+  const int m3 = (int)fdiv(q, dx), m1 = q - m3 * X;
+  const int m2 = 2 * kk + ((colour - m1 - m3) & 1);
+  if (m2 >= Y) return;
+  const long n = ((long)m3 * X + m1) * Y + m2;
+  T v;
+  if (kgs_at<T>(X, Y, Z, k, b, x, n, m1, m2, m3, v)) x[n] = v;
 }
 
 // r = b - K x
