@@ -124,41 +124,6 @@ __global__ void k_bc_rhs(int nbc, const int *__restrict__ r, const double *__res
   if (i < nbc) b[r[i]] = c[i];
 }
 
-// ---------------------------------------------------------------------------
-// Row operator on equation rows: RES -> y = b - A x (gausmat4's residual
-// order, move3mg.c:3937-3958), otherwise y = A x.  withp=0 leaves pressure
-// rows at zero (velocity block).  The four unknowns of a node are written
-// with one vector store; boundary rows get a placeholder 0 there and their
-// values from k_bc_rowop, which always follows.
-// ---------------------------------------------------------------------------
-template <typename T, int C, bool RES>
-__device__ __forceinline__ T rowop_one(const Geo<T> &g, unsigned char f, long n, int m1, int m2, int m3,
-                                       const T *__restrict__ x, T bv, int withp) {
-  if (kind_of(f, C) != KIND_EQ || (!withp && C == 0)) return T(0);
-  T s[18];
-  row_coef_c<T, C>(g, m1, m2, m3, s);
-  return row_dot<T, C, RES>(g, s, x, n, bv);
-}
-
-template <typename T, bool RES>
-__global__ void k_rowop(Geo<T> g, const unsigned char *__restrict__ flags, const T *__restrict__ x,
-                        const T *__restrict__ b, T *__restrict__ y, int withp) {
-  using V = typename V4<T>::t;
-  long n = xcd_block() * blockDim.x + threadIdx.x;
-  if (n >= (long)g.XY * g.Z) return;
-  int m1, m2, m3;
-  decode(n, g.X, g.Y, g.XY, m1, m2, m3);
-  unsigned char f = flags[n];
-  V bn = {T(0), T(0), T(0), T(0)};
-  if (RES) bn = reinterpret_cast<const V *>(b)[n];
-  V out;
-  out.x = rowop_one<T, 0, RES>(g, f, n, m1, m2, m3, x, bn.x, withp);
-  out.y = rowop_one<T, 1, RES>(g, f, n, m1, m2, m3, x, bn.y, withp);
-  out.z = rowop_one<T, 2, RES>(g, f, n, m1, m2, m3, x, bn.z, withp);
-  out.w = rowop_one<T, 3, RES>(g, f, n, m1, m2, m3, x, bn.w, withp);
-  reinterpret_cast<V *>(y)[n] = out;
-}
-
 // boundary rows  x_r - k x_j = b_r  (bonderr, move3mg.c:1682-1729)
 template <typename T, bool RES>
 __global__ void k_bc_rowop(int nbc, const int *__restrict__ br, const int *__restrict__ bj,
@@ -331,6 +296,31 @@ __global__ void k_bc_compose(BcChain ch, const int *__restrict__ idx, const T *_
   ck[q] = K;
 }
 
+// Stage a tile and its one-node halo of (P, vx, vy, vz) in LDS, component-
+// major (zeros outside the level), and the unknown kinds of the interior.
+template <typename T, int TY, int TX>
+__device__ __forceinline__ void tile_stage(const Geo<T> &g, const unsigned char *__restrict__ flags,
+                                           const T *__restrict__ xold, T *sx, unsigned char *sf, const TileGeo &tg,
+                                           int x0, int y0, int z0, int ty, int tx, int tz, int nh) {
+  const int hy = ty + 2, hx = tx + 2;
+  for (int i = threadIdx.x; i < nh; i += blockDim.x) {
+    const int q = (int)tdiv<TY ? TY + 2 : 0>(i, tg.dhy);
+    const int ly = i - q * hy, lz = (int)tdiv<TX ? TX + 2 : 0>(q, tg.dhx), lx = q - lz * hx;
+    int m2 = y0 + ly - 1, m1 = x0 + lx - 1, m3 = z0 + lz - 1;
+    T v0 = T(0), v1 = T(0), v2 = T(0), v3 = T(0);
+    unsigned char f = 0;
+    const bool interior = lx >= 1 && lx <= tx && ly >= 1 && ly <= ty && lz >= 1 && lz <= tz;
+    if (m1 >= 0 && m1 < g.X && m2 >= 0 && m2 < g.Y && m3 >= 0 && m3 < g.Z) {
+      const long n = g.nid(m1, m2, m3);
+      const T *p = xold + 4 * n;
+      v0 = p[0]; v1 = p[1]; v2 = p[2]; v3 = p[3];
+      if (interior) f = flags[n];  // kinds of the halo are not needed
+    }
+    sx[i] = v0; sx[nh + i] = v1; sx[2 * nh + i] = v2; sx[3 * nh + i] = v3;
+    if (interior) sf[__umul24(__umul24(lz - 1, tx) + lx - 1, ty) + ly - 1] = f;
+  }
+}
+
 // SV: the tile's stencil viscosities (nodes [x0, x0+tx] x [y0, y0+ty] x
 // [z0, z0+tz], every offset a row uses is 0 or +1) are staged in LDS after the
 // unknowns, and the rows read them through a Geo whose arrays are that block
@@ -374,23 +364,7 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
     gv.eyz = sv + 3 * nv;
     gv.ed = sv + 4 * nv;
   }
-  // stage tile + halo
-  for (int i = threadIdx.x; i < nh; i += blockDim.x) {
-    const int q = (int)tdiv<TY ? TY + 2 : 0>(i, tg.dhy);
-    const int ly = i - q * hy, lz = (int)tdiv<TX ? TX + 2 : 0>(q, tg.dhx), lx = q - lz * hx;
-    int m2 = y0 + ly - 1, m1 = x0 + lx - 1, m3 = z0 + lz - 1;
-    T v0 = T(0), v1 = T(0), v2 = T(0), v3 = T(0);
-    unsigned char f = 0;
-    const bool interior = lx >= 1 && lx <= tx && ly >= 1 && ly <= ty && lz >= 1 && lz <= tz;
-    if (m1 >= 0 && m1 < g.X && m2 >= 0 && m2 < g.Y && m3 >= 0 && m3 < g.Z) {
-      const long n = g.nid(m1, m2, m3);
-      const T *p = xold + 4 * n;
-      v0 = p[0]; v1 = p[1]; v2 = p[2]; v3 = p[3];
-      if (interior) f = flags[n];  // kinds of the halo are not needed
-    }
-    sx[i] = v0; sx[nh + i] = v1; sx[2 * nh + i] = v2; sx[3 * nh + i] = v3;
-    if (interior) sf[__umul24(__umul24(lz - 1, tx) + lx - 1, ty) + ly - 1] = f;
-  }
+  tile_stage<T, TY, TX>(g, flags, xold, sx, sf, tg, x0, y0, z0, ty, tx, tz, nh);
   __syncthreads();
   const int nphase = withp ? 7 : 6;
   for (int ph = 0; ph < nphase; ph++) {
@@ -431,6 +405,65 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Row operator on equation rows: RES -> y = b - A x (gausmat4's residual
+// order, move3mg.c:3937-3958), otherwise y = A x.  withp=0 leaves pressure
+// rows at zero (velocity block).  Tiles of the whole level: x and the
+// unknown kinds staged in LDS as in the smoother, then every equation row of
+// the tile's nodes from LDS in slot order.  The four unknowns of a node are
+// written with one vector store; boundary rows get a placeholder 0 there and
+// their values from k_bc_rowop, which always follows.
+// ---------------------------------------------------------------------------
+template <typename T, bool RES, int TY = 0, int TX = 0, int TZ = 0>
+__global__ void __launch_bounds__(256) k_rowop_tile(Geo<T> g, const unsigned char *__restrict__ flags,
+                                                    const T *__restrict__ x, const T *__restrict__ b,
+                                                    T *__restrict__ y, TileGeo tg, int withp) {
+  using V = typename V4<T>::t;
+  extern __shared__ unsigned char smem_raw[];
+  T *sx = reinterpret_cast<T *>(smem_raw);
+  const int ty = TY ? TY : tg.ty, tx = TX ? TX : tg.tx, tz = TZ ? TZ : tg.tz;
+  const int hy = ty + 2, hx = tx + 2, nh = hy * hx * (tz + 2);
+  const int bt = (int)xcd_block();
+  const int by = bt % tg.nty, bx = (bt / tg.nty) % tg.ntx, bz = bt / (tg.nty * tg.ntx);
+  const int y0 = tg.oy + by * ty, x0 = tg.ox + bx * tx, z0 = tg.oz + bz * tz;
+  unsigned char *sf = smem_raw + 4 * nh * sizeof(T);
+  tile_stage<T, TY, TX>(g, flags, x, sx, sf, tg, x0, y0, z0, ty, tx, tz, nh);
+  __syncthreads();
+  const int nt = ty * tx * tz;
+  for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+    const int q = (int)tdiv<TY>(i, tg.dty);
+    const int ly = i - q * ty, lz = (int)tdiv<TX>(q, tg.dtx), lx = q - lz * tx;
+    const int m1 = x0 + lx, m2 = y0 + ly, m3 = z0 + lz;
+    if (m1 >= g.X || m2 >= g.Y || m3 >= g.Z) continue;
+    const unsigned n = __umul24(m3, g.XY) + __umul24(m1, g.Y) + m2;
+    const unsigned char f = sf[i];
+    const int base = (int)__umul24(__umul24(lz + 1, hx) + lx + 1, hy) + ly + 1;
+    V bn = {T(0), T(0), T(0), T(0)};
+    if (RES) bn = reinterpret_cast<const V *>(b)[n];
+    T o[4] = {T(0), T(0), T(0), T(0)};
+    const T bb[4] = {bn.x, bn.y, bn.z, bn.w};
+    auto row = [&](auto cc) {
+      constexpr int C = decltype(cc)::value;
+      if (kind_of(f, C) != KIND_EQ || (!withp && C == 0)) return;
+      T s[18];
+      row_coef_c<T, C>(g, m1, m2, m3, (long)n, s);
+      T acc = RES ? bb[C] : T(0);
+#pragma unroll
+      for (int q2 = 1; q2 <= Slot<C>::ns; q2++) {
+        const int li = base + (Slot<C>::d[q2][2] * hx + Slot<C>::d[q2][0]) * hy + Slot<C>::d[q2][1];
+        const T xv = sx[Slot<C>::d[q2][3] * nh + li];
+        acc = RES ? msub(acc, s[q2], xv) : mac(acc, s[q2], xv);
+      }
+      o[C] = acc;
+    };
+    row(std::integral_constant<int, 0>{});
+    row(std::integral_constant<int, 1>{});
+    row(std::integral_constant<int, 2>{});
+    row(std::integral_constant<int, 3>{});
+    reinterpret_cast<V *>(y)[n] = V{o[0], o[1], o[2], o[3]};
+  }
+}
 
 template <typename T>
 __device__ __forceinline__ void bc_chain_one(const BcChain &ch, int i, const T *__restrict__ b, T *x, int mode) {

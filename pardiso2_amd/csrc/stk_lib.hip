@@ -568,11 +568,20 @@ void smooth(stk_ctx *c, int l, int sweeps, const int *tile, int withp, bool whol
   }
 }
 
+// equation rows over 32x4x4 tiles of the whole level (x staged in LDS), then
+// the boundary rows; y must not alias x
+template <typename T, bool RES>
+void rowop_rows(stk_ctx *c, const Level &L, const T *x, const T *b, T *y, int withp) {
+  const int full[6] = {0, 0, 0, L.Y, L.X, L.Z};
+  const TileGeo tg = make_tile(32, 4, 4, full);
+  hipLaunchKernelGGL((k_rowop_tile<T, RES, 32, 4, 4>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256),
+                     smooth_lds<T>(kTileDefault4, false), c->st, L.geo<T>(c->kn.p2koef), L.flags, x, b, y, tg, withp);
+}
+
 template <typename T, bool RES>
 void rowop(stk_ctx *c, int l, const T *x, const T *b, T *y, int withp) {
   Level &L = c->lv[l];
-  Geo<T> g = L.geo<T>(c->kn.p2koef);
-  hipLaunchKernelGGL((k_rowop<T, RES>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, g, L.flags, x, b, y, withp);
+  rowop_rows<T, RES>(c, L, x, b, y, withp);
   if (L.nbc)
     hipLaunchKernelGGL((k_bc_rowop<T, RES>), dim3(blocks(L.nbc, 256)), dim3(256), 0, c->st, L.nbc, L.bc_r,
                        L.bc_j, L.bc_negk, x, b, y, withp);
@@ -1857,17 +1866,11 @@ int stk_time_kernel(stk_ctx *c, int which, int level, int reps, double *ms_avg) 
       const bool sv = tt[3] && smooth_lds<double>(tile, true) <= kLevelLds;
       launch_smooth<double>(c, tile, sv, tg, g, L.flags, L.x64[0], L.x64[1], L.b64, 0);
     } else if (which == 2) {
-      Geo<double> g = L.geo<double>(c->kn.p2koef);
-      hipLaunchKernelGGL((k_rowop<double, false>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, g, L.flags,
-                         L.x64[0], nullptr, L.r64, 1);
+      rowop_rows<double, false>(c, L, L.x64[0], nullptr, L.r64, 1);
     } else if (which == 3) {
-      Geo<float> g = L.geo<float>(c->kn.p2koef);
-      hipLaunchKernelGGL((k_rowop<float, true>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, g, L.flags,
-                         L.x32[0], L.b32, L.r32, 0);
+      rowop_rows<float, true>(c, L, L.x32[0], L.b32, L.r32, 0);
     } else if (which == 4) {
-      Geo<double> g = L.geo<double>(c->kn.p2koef);
-      hipLaunchKernelGGL((k_rowop<double, true>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, g, L.flags,
-                         L.x64[0], L.b64, L.r64, 1);
+      rowop_rows<double, true>(c, L, L.x64[0], L.b64, L.r64, 1);
     } else {
       hipLaunchKernelGGL(k_stream_copy, dim3(blocks(4 * L.n, 256)), dim3(256), 0, c->st, 4 * L.n, L.x64[0],
                          L.x64[1]);
