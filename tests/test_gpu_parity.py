@@ -108,6 +108,38 @@ def test_coloured_sweep_matches_oracle(oracle_mod, tile):
         g.close()
 
 
+def test_sweep_boundary_rows_both_ways(oracle_mod):
+    """FP64 tile sweeps equal the oracle's both when the tiles set the boundary
+    rows themselves (chains composed once per call, rows written in LDS or
+    stored directly) and when a separate chain pass follows every sweep (some
+    chain's row and partner fall in different tiles).  27^3, 3 levels: both
+    kinds occur among these tiles and levels."""
+    pb, o, g = _pair(oracle_mod, n=27, L=3, seed=6)
+    try:
+        cands = [(32, 4, 4), (8, 2, 3), (16, 4, 2), (4, 4, 4), (3, 3, 3), (5, 2, 7), (2, 2, 2), (7, 3, 5)]
+        found = {}
+        for l in range(3):
+            for t in cands:
+                found.setdefault(g.tile_chains(l, t), (l, t))
+        assert set(found) == {True, False}, found
+        rng = np.random.default_rng(9)
+        for inplace, (l, t) in sorted(found.items()):
+            b4, n4 = 4 * o.info[l][5], 4 * o.info[l][4]
+            x = rng.standard_normal(n4) * 1e-16
+            f = rng.standard_normal(n4) * (o.arr("num0")[b4:b4 + n4] > 0)
+            o.arr("sol0")[b4:b4 + n4] = x
+            o.arr("fre1")[b4:b4 + n4] = f
+            g.vec_set(S.SOL0, l, x)
+            g.vec_set(S.FRE1, l, f)
+            o.sweep_tiles(3, l, t[0], t[1], t[2])
+            g.smooth(l, 3, tile=t, precision=1)
+            xo = o.arr("sol0")[b4:b4 + n4]
+            np.testing.assert_allclose(g.vec_get(S.SOL0, l), xo, rtol=1e-12, atol=1e-12 * np.abs(xo).max(),
+                                       err_msg=f"level {l} tile {t} in-tile chains {inplace}")
+    finally:
+        g.close()
+
+
 @pytest.mark.parametrize("tile,vonly", [((32, 4, 4), False), ((8, 2, 3), False), ((16, 4, -2), False),
                                         ((0, 0, 0), False), ((0, 0, 0), True), ((32, 4, 4), True)])
 def test_fp32_sweep_matches_fp64_oracle_to_fp32_accuracy(oracle_mod, tile, vonly):
