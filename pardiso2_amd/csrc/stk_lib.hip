@@ -135,6 +135,8 @@ struct Level {
   // pressure-cell operator K of the Schur approximation and its K-cycle vectors
   double *k64[7] = {};
   float *k32[7] = {};
+  double *kc64 = nullptr;  // the seven planes colour-compact (Gauss-Seidel half-sweeps), or null
+  float *kc32 = nullptr;
   float *kx32 = nullptr, *kb32 = nullptr, *kr32 = nullptr;
   double *kx64 = nullptr, *kb64 = nullptr, *kr64 = nullptr;
   int *klo[3] = {}, *kcnt[3] = {};  // cells of level-1 owned by each cell of this level, per axis
@@ -698,6 +700,14 @@ void vcycle_v(stk_ctx *c, const stk_solver_opts *o, const int *tile) {
 // ---------------------------------------------------------------------------
 // Schur approximation (weighted BFBt) and its aggregation K-cycle
 // ---------------------------------------------------------------------------
+// cells of one colour of a K level, and whether the colour-compact sweep's
+// index decode is exact there (hy^2 X Z < 2^32)
+inline unsigned long kcolour_n(const Level &V) { return (unsigned long)((V.Y + 1) / 2) * V.X * V.Z; }
+inline bool kcolour_ok(const Level &V) { return (unsigned long)((V.Y + 1) / 2) * kcolour_n(V) < (1ul << 32); }
+template <typename T> T *kcol(Level &L);
+template <> float *kcol<float>(Level &L) { return L.kc32; }
+template <> double *kcol<double>(Level &L) { return L.kc64; }
+
 template <typename T> KPlanesC<T> kplanes(const Level &L);
 template <> KPlanesC<float> kplanes<float>(const Level &L) {
   KPlanesC<float> k;
@@ -781,17 +791,16 @@ void kcycle(stk_ctx *c, int l, int nu, T scale) {
   }
   const KPlanesC<T> k = kplanes<T>(V);
   const unsigned nb = blocks(V.n, 256);
-  // one thread per cell of the colour where its index decode is exact
-  const unsigned long hy = (V.Y + 1) / 2, ncol = hy * V.X * V.Z;
-  const bool compact = hy * ncol < (1ul << 32);
-  const FDiv dhy = make_fdiv((unsigned)hy), dx = make_fdiv((unsigned)V.X);
+  // one thread per cell of the colour on colour-compact planes where they exist
+  const T *kc = kcol<T>(V);
+  const FDiv dhy = make_fdiv((V.Y + 1) / 2), dx = make_fdiv(V.X);
   auto sweeps = [&](int c1) {
     for (int s = 0; s < nu; s++)
       for (int q = 0; q < 2; q++) {
         const int col = q == 0 ? c1 : 1 - c1;
-        if (compact)
-          hipLaunchKernelGGL(k_kgs_c<T>, dim3(blocks(ncol, 256)), dim3(256), 0, c->st, V.X, V.Y, V.Z, dhy, dx, k,
-                             kb<T>(V), kx<T>(V), col);
+        if (kc)
+          hipLaunchKernelGGL(k_kgs_c<T>, dim3(blocks(kcolour_n(V), 256)), dim3(256), 0, c->st, V.X, V.Y, V.Z, dhy,
+                             dx, kc, kb<T>(V), kx<T>(V), col);
         else
           hipLaunchKernelGGL(k_kgs<T>, dim3(nb), dim3(256), 0, c->st, V.X, V.Y, V.Z, k, kb<T>(V), kx<T>(V), col);
       }
@@ -1274,6 +1283,7 @@ void stk_destroy(stk_ctx *c) {
     for (int k = 0; k < 4; k++)
       for (int a = 0; a < 3; a++) L.ax[k][a].release();
     for (int q = 0; q < 7; q++) { dfree(L.k64[q]); dfree(L.k32[q]); }
+    dfree(L.kc64); dfree(L.kc32);
     dfree(L.kx32); dfree(L.kb32); dfree(L.kr32); dfree(L.kx64); dfree(L.kb64); dfree(L.kr64);
     for (int a = 0; a < 3; a++) { dfree(L.klo[a]); dfree(L.kcnt[a]); dfree(L.kpar[a]); }
   }
@@ -1508,6 +1518,8 @@ static int build_schur(stk_ctx *c) {
       return STK_ENOMEM;
     HIPCHK(hipMemsetAsync(V.kx32, 0, V.n * 4, c->st));
     HIPCHK(hipMemsetAsync(V.kx64, 0, V.n * 8, c->st));
+    if (l < L && kcolour_ok(V) && (dalloc(&V.kc64, 14 * kcolour_n(V)) || dalloc(&V.kc32, 14 * kcolour_n(V))))
+      return STK_ENOMEM;
   }
   if (!c->ktabs) {
     for (int l = 1; l <= L; l++) {
@@ -1537,6 +1549,16 @@ static int build_schur(stk_ctx *c) {
     for (int q = 0; q < 7; q++)
       hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks(C.n, 256)), dim3(256), 0, c->st, C.n, C.k64[q],
                          C.k32[q]);
+  }
+  for (int l = 0; l < L; l++) {  // colour-compact copies for the Gauss-Seidel half-sweeps
+    Level &V = c->lv[l];
+    if (!V.kc64) continue;
+    const FDiv dhy = make_fdiv((V.Y + 1) / 2), dx = make_fdiv(V.X);
+    const unsigned nt = blocks(2 * kcolour_n(V), 256);
+    hipLaunchKernelGGL(k_kplanes_colour<double>, dim3(nt), dim3(256), 0, c->st, V.X, V.Y, V.Z, dhy, dx,
+                       kplanes<double>(V), V.kc64);
+    hipLaunchKernelGGL(k_kplanes_colour<float>, dim3(nt), dim3(256), 0, c->st, V.X, V.Y, V.Z, dhy, dx,
+                       kplanes<float>(V), V.kc32);
   }
   // coarsest K: dense inverse on the host (at most a few hundred cells)
   Level &C = c->lv[L];

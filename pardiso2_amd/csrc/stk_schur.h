@@ -216,19 +216,55 @@ __global__ void k_kgs(int X, int Y, int Z, KPlanesC<T> k, const T *__restrict__ 
 // on the other colour): thread i is cell m2 = 2 kk + parity of the row
 // q = (m3, m1), kk = i mod hy, q = i / hy with hy = ceil(Y/2).  The
 // multiply-high divisions are exact while hy^2 X Z < 2^32 (the caller checks).
-template <typename T>
-__global__ void k_kgs_c(int X, int Y, int Z, FDiv dhy, FDiv dx, KPlanesC<T> k, const T *__restrict__ b, T *x,
-                        int colour) {
-  const unsigned i = (unsigned)(xcd_block() * blockDim.x + threadIdx.x);
+// The seven planes come colour-compact (kc[(p * 2 + colour) * ncol + i], from
+// k_kplanes_colour), so a half-sweep reads only its own colour's
+// coefficients, each once and contiguous; the arithmetic is kgs_at's.
+__device__ __forceinline__ void kcell_of(unsigned i, int colour, int X, int Y, FDiv dhy, FDiv dx, int &m1, int &m2,
+                                         int &m3) {
   const int hy = (Y + 1) / 2;
-  if (i >= (unsigned)hy * X * Z) return;
   const int q = (int)fdiv(i, dhy), kk = (int)i - q * hy;
-  const int m3 = (int)fdiv(q, dx), m1 = q - m3 * X;
-  const int m2 = 2 * kk + ((colour - m1 - m3) & 1);
+  m3 = (int)fdiv(q, dx);
+  m1 = q - m3 * X;
+  m2 = 2 * kk + ((colour - m1 - m3) & 1);
+}
+
+template <typename T>
+__global__ void k_kgs_c(int X, int Y, int Z, FDiv dhy, FDiv dx, const T *__restrict__ kc, const T *__restrict__ b,
+                        T *x, int colour) {
+  const unsigned i = (unsigned)(xcd_block() * blockDim.x + threadIdx.x);
+  const unsigned ncol = (unsigned)((Y + 1) / 2) * X * Z;
+  if (i >= ncol) return;
+  int m1, m2, m3;
+  kcell_of(i, colour, X, Y, dhy, dx, m1, m2, m3);
   if (m2 >= Y) return;
   const long n = ((long)m3 * X + m1) * Y + m2;
-  T v;
-  if (kgs_at<T>(X, Y, Z, k, b, x, n, m1, m2, m3, v)) x[n] = v;
+  const int XY = X * Y;
+  const T *__restrict__ p = kc + (size_t)colour * ncol + i;  // plane q at p[2 q ncol]
+  const T d = p[0];
+  if (d == T(0)) return;
+  T acc = b[n];
+  if (m1 > 0) acc -= p[2 * ncol] * x[n - Y];
+  if (m1 < X - 1) acc -= p[4 * (size_t)ncol] * x[n + Y];
+  if (m2 > 0) acc -= p[6 * (size_t)ncol] * x[n - 1];
+  if (m2 < Y - 1) acc -= p[8 * (size_t)ncol] * x[n + 1];
+  if (m3 > 0) acc -= p[10 * (size_t)ncol] * x[n - XY];
+  if (m3 < Z - 1) acc -= p[12 * (size_t)ncol] * x[n + XY];
+  x[n] = acc / d;
+}
+
+// The colour-compact copy of a level's seven K planes (0 past the row ends).
+template <typename T>
+__global__ void k_kplanes_colour(int X, int Y, int Z, FDiv dhy, FDiv dx, KPlanesC<T> k, T *kc) {
+  const unsigned t = (unsigned)(blockIdx.x * blockDim.x + threadIdx.x);
+  const unsigned ncol = (unsigned)((Y + 1) / 2) * X * Z;
+  if (t >= 2 * ncol) return;
+  const int colour = t >= ncol;
+  const unsigned i = t - (colour ? ncol : 0u);
+  int m1, m2, m3;
+  kcell_of(i, colour, X, Y, dhy, dx, m1, m2, m3);
+  const long n = ((long)m3 * X + m1) * Y + m2;
+#pragma unroll
+  for (int q = 0; q < 7; q++) kc[(size_t)(q * 2 + colour) * ncol + i] = m2 < Y ? k.p[q][n] : T(0);
 }
 
 // r = b - K x

@@ -24,12 +24,13 @@ from . import setups
 
 
 def context_bytes(n: int, levels: int, restart: int) -> int:
-    """Device bytes of one context of an n^3 box: per level about 330 B/node
-    (geometry planes, FP32 and FP64 vectors, Schur planes and vectors), plus
-    2 (restart + 1) FP64 Krylov vectors of 4 unknowns per finest node."""
+    """Device bytes of one context of an n^3 box: per level about 415 B/node
+    (geometry planes, FP32 and FP64 vectors, Schur planes in natural and
+    colour-compact order, Schur vectors), plus 2 (restart + 1) FP64 Krylov
+    vectors of 4 unknowns per finest node."""
     total, m = 0, n
     for _ in range(levels + 1):
-        total += 330 * m ** 3
+        total += 415 * m ** 3
         m = (m - 5) // 2 + 5
     return int(total + 2 * (restart + 1) * 32 * n ** 3 + (64 << 20))
 
