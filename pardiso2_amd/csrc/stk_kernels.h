@@ -1118,6 +1118,122 @@ __global__ void k_gj_elim(double *A, double *B, int n, int k, const int *__restr
   }
 }
 
+// Blocked form of the same elimination (same pivot rule): a panel of kb
+// columns is eliminated in one workgroup (k_gjb_panel, the panel in LDS),
+// recording the factors F[i][kk] = A[i][k]/d_k (0 on the pivot row); the
+// panel's kb steps then reach the rest of [A | B] as one rank-kb update,
+// T -= F R'', where R'' are the pivot rows as each step saw them:
+// R''[kk] = T[p_kk] - sum_{q<kk} F[p_kk][q] R''[q] (k_gjb_rows).  Rounding
+// differs from the column-by-column order; the pivots, d_k and the final
+// scaling are those of k_gj_final.
+constexpr int kGjbMax = 16;  // panel width limit
+__global__ void __launch_bounds__(1024) k_gjb_panel(const double *__restrict__ A, int n, int k0, int kb, int *used,
+                                                    int *prow, double *pval, double *F, int *sing) {
+  extern __shared__ unsigned char smem_raw[];
+  double *P = reinterpret_cast<double *>(smem_raw);  // P[kk * n + i]: column-major, no bank conflicts
+  unsigned char *su = smem_raw + (size_t)n * kb * 8;  // used rows, kept in LDS for the panel
+  __shared__ double rv[16];
+  __shared__ int ri[16];
+  __shared__ int piv;
+  if (*sing) return;  // an earlier panel found no pivot
+  for (int e = threadIdx.x; e < n * kb; e += blockDim.x) {
+    const int i = e / kb, kk = e - i * kb;
+    P[kk * n + i] = A[(long)i * n + k0 + kk];
+  }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) su[i] = (unsigned char)used[i];
+  __syncthreads();
+  for (int kk = 0; kk < kb; kk++) {
+    double best = -1.0;
+    int bidx = n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+      if (!su[i]) gj_better(fabs(P[kk * n + i]), i, best, bidx);
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(best, off);
+      const int oi = __shfl_xor(bidx, off);
+      gj_better(ov, oi, best, bidx);
+    }
+    if ((threadIdx.x & 63) == 0) { rv[threadIdx.x >> 6] = best; ri[threadIdx.x >> 6] = bidx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double bv = -1.0;
+      int bi = n;
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) gj_better(rv[w], ri[w], bv, bi);
+      if (!(bv > 0.0) || bi >= n) {
+        if (!*sing) *sing = k0 + kk + 1;
+        prow[k0 + kk] = -1;
+        piv = -1;
+      } else {
+        piv = bi;
+        prow[k0 + kk] = bi;
+        pval[k0 + kk] = P[kk * n + bi];
+        su[bi] = 1;
+        used[bi] = 1;
+      }
+    }
+    __syncthreads();
+    const int p = piv;
+    if (p < 0) return;  // singular: every thread leaves here together
+    const double d = P[kk * n + p];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const double f = (i == p) ? 0.0 : P[kk * n + i] / d;
+      F[(long)i * kGjbMax + kk] = f;
+      if (f != 0.0)
+        for (int q = kk + 1; q < kb; q++) P[q * n + i] -= f * P[q * n + p];
+    }
+    __syncthreads();
+  }
+}
+
+// column c of the trailing part T = [A[:, k0+kb ..) | B]
+__device__ __forceinline__ double *gjb_col(double *A, double *B, int n, int k0, int kb, int c, long i) {
+  const int na = n - k0 - kb;
+  return c < na ? A + i * n + k0 + kb + c : B + i * n + (c - na);
+}
+
+// R''[kk][c] for every trailing column c (one thread per column)
+__global__ void k_gjb_rows(double *A, double *B, int n, int k0, int kb, const int *__restrict__ prow,
+                           const double *__restrict__ F, double *R, const int *__restrict__ sing) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = 2 * n - k0 - kb;
+  if (c >= m || *sing) return;  // after a failed pivot some prow are -1
+  double r[kGjbMax];
+#pragma unroll
+  for (int kk = 0; kk < kGjbMax; kk++) {
+    if (kk >= kb) break;
+    const int p = prow[k0 + kk];
+    double v = *gjb_col(A, B, n, k0, kb, c, p);
+    for (int q = 0; q < kk; q++) v -= F[(long)p * kGjbMax + q] * r[q];
+    r[kk] = v;
+    R[(long)kk * m + c] = v;
+  }
+}
+
+// T[i][c] -= sum_kk F[i][kk] R''[kk][c]: a workgroup takes 256 columns and
+// kGjbRows rows, each thread one column with its R'' values in registers
+constexpr int kGjbRows = 8;
+__global__ void __launch_bounds__(256) k_gjb_update(double *A, double *B, int n, int k0, int kb,
+                                                    const double *__restrict__ F, const double *__restrict__ R,
+                                                    const int *__restrict__ sing) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = 2 * n - k0 - kb;
+  if (c >= m || *sing) return;
+  double r[kGjbMax];
+#pragma unroll
+  for (int kk = 0; kk < kGjbMax; kk++) r[kk] = kk < kb ? R[(long)kk * m + c] : 0.0;
+  const int i0 = blockIdx.y * kGjbRows;
+  for (int i = i0; i < i0 + kGjbRows && i < n; i++) {
+    double acc = 0.0;
+    bool any = false;
+#pragma unroll
+    for (int kk = 0; kk < kGjbMax; kk++) {
+      const double f = kk < kb ? F[(long)i * kGjbMax + kk] : 0.0;
+      any = any || f != 0.0;
+      acc += f * r[kk];
+    }
+    if (any) *gjb_col(A, B, n, k0, kb, c, i) -= acc;
+  }
+}
+
 // inv[k][:] = B[p_k][:] / A[p_k][k]
 __global__ void k_gj_final(const double *__restrict__ B, int n, const int *__restrict__ prow,
                            const double *__restrict__ pval, double *inv) {

@@ -1026,11 +1026,35 @@ int gj_invert(stk_ctx *c, double *A, int n, double *inv) {
   hipLaunchKernelGGL(k_eye, dim3(blocks((long)n * n, 256)), dim3(256), 0, c->st, B, n);
   HIPCHK(hipMemsetAsync(used, 0, (size_t)n * 4, c->st));
   HIPCHK(hipMemsetAsync(sing, 0, 4, c->st));
-  for (int k = 0; k < n; k++) {
-    hipLaunchKernelGGL(k_gj_select, dim3(1), dim3(1024), 0, c->st, A, n, k, used, prow, pval, cand, sing);
-    hipLaunchKernelGGL(k_gj_elim, dim3(blocks(2L * n * n, 256)), dim3(256), 0, c->st, A, B, n, k, used, prow, pval,
-                       cand);
+  // panels of kb columns in one workgroup's LDS (about 3 launches per kb
+  // columns); column by column where a panel of 4 would not fit
+  const int kbl = (int)std::min<long>(kGjbMax, ((long)c->lds_max - 4096 - n) / (8L * n));
+  double *F = nullptr, *R = nullptr;
+  if (kbl >= 4 && (dalloc(&F, (size_t)n * kGjbMax) || dalloc(&R, (size_t)kGjbMax * 2 * n))) {
+    dfree(B); dfree(pval); dfree(cand); dfree(used); dfree(prow); dfree(sing); dfree(F); dfree(R);
+    return STK_ENOMEM;
   }
+  if (kbl >= 4) {
+    (void)hipFuncSetAttribute((const void *)k_gjb_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)((size_t)n * kbl * 8 + n));
+    for (int k0 = 0; k0 < n; k0 += kbl) {
+      const int kb = std::min(kbl, n - k0), m = 2 * n - k0 - kb;
+      hipLaunchKernelGGL(k_gjb_panel, dim3(1), dim3(1024), (size_t)n * kb * 8 + n, c->st, A, n, k0, kb, used, prow,
+                         pval, F, sing);
+      if (m <= 0) continue;
+      hipLaunchKernelGGL(k_gjb_rows, dim3(blocks(m, 256)), dim3(256), 0, c->st, A, B, n, k0, kb, prow, F, R, sing);
+      hipLaunchKernelGGL(k_gjb_update, dim3(blocks(m, 256), (n + kGjbRows - 1) / kGjbRows), dim3(256), 0, c->st, A,
+                         B, n, k0, kb, F, R, sing);
+    }
+  } else {
+    for (int k = 0; k < n; k++) {
+      hipLaunchKernelGGL(k_gj_select, dim3(1), dim3(1024), 0, c->st, A, n, k, used, prow, pval, cand, sing);
+      hipLaunchKernelGGL(k_gj_elim, dim3(blocks(2L * n * n, 256)), dim3(256), 0, c->st, A, B, n, k, used, prow,
+                         pval, cand);
+    }
+  }
+  dfree(F);
+  dfree(R);
   int hs = 0;
   D2H(c, &hs, sing, 4);
   if (!hs)
