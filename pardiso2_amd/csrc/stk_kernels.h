@@ -1324,25 +1324,34 @@ __global__ void k_pscale(long n4, const unsigned char *__restrict__ flags, const
 
 // velocity right-hand side r_v - G z_p of the preconditioner (momentum rows
 // have P only in slots 16/17), boundary rows take r; stored in type T
+// (one thread per node: its four entries in one vector store)
 template <typename T>
 __global__ void k_vrhs(Geo<double> g, const unsigned char *__restrict__ flags, const double *__restrict__ r,
                        const double *__restrict__ zp, T *bv) {
-  long u = xcd_block() * blockDim.x + threadIdx.x;
-  long n = u >> 2;
+  using V = typename V4<T>::t;
+  const long n = xcd_block() * blockDim.x + threadIdx.x;
   if (n >= (long)g.XY * g.Z) return;
-  int c = (int)(u & 3);
-  int k = kind_of(flags[n], c);
-  if (c == 0 || k == KIND_NONE) { bv[u] = T(0); return; }
-  if (k == KIND_BC) { bv[u] = (T)r[u]; return; }
+  const unsigned char f = flags[n];
+  const double4 rn = reinterpret_cast<const double4 *>(r)[n];
+  const double rr[4] = {rn.x, rn.y, rn.z, rn.w};
   int m1, m2, m3;
   decode(n, g.X, g.Y, g.XY, m1, m2, m3);
-  double kf = (c == 1) ? 2.0 * g.iBx[m1] : (c == 2) ? 2.0 * g.iBy[m2] : 2.0 * g.iBz[m3];
-  double acc = r[u];
-  for (int q = 16; q <= 17; q++) {
-    long uq = 4 * (n + (long)kSlot[c][q][2] * g.XY + (long)kSlot[c][q][0] * g.Y + kSlot[c][q][1]);
-    acc -= (q == 16 ? kf : -kf) * zp[uq];
+  T o[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int c = 1; c < 4; c++) {
+    const int k = kind_of(f, c);
+    if (k == KIND_BC) o[c] = (T)rr[c];
+    if (k != KIND_EQ) continue;
+    const double kf = (c == 1) ? 2.0 * g.iBx[m1] : (c == 2) ? 2.0 * g.iBy[m2] : 2.0 * g.iBz[m3];
+    double acc = rr[c];
+#pragma unroll
+    for (int q = 16; q <= 17; q++) {
+      const long uq = 4 * (n + (long)kSlot[c][q][2] * g.XY + (long)kSlot[c][q][0] * g.Y + kSlot[c][q][1]);
+      acc -= (q == 16 ? kf : -kf) * zp[uq];
+    }
+    o[c] = (T)acc;
   }
-  bv[u] = (T)acc;
+  reinterpret_cast<V *>(bv)[n] = V{o[0], o[1], o[2], o[3]};
 }
 
 // z_v = x (velocity unknowns), pressure entries of z kept

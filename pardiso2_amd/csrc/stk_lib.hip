@@ -849,7 +849,7 @@ void precond(stk_ctx *c, const stk_solver_opts *o, const double *r, double *z, c
     hipLaunchKernelGGL(k_pscale, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, F.flags, F.e64[0], r, z, o->omega);
   bc_passes<double>(c, F, r, z, 2);
   Geo<double> g = F.geo<double>(c->kn.p2koef);
-  hipLaunchKernelGGL(k_vrhs<T>, dim3(blocks(n4, 256)), dim3(256), 0, c->st, g, F.flags, r, z, lvb<T>(F));
+  hipLaunchKernelGGL(k_vrhs<T>, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, g, F.flags, r, z, lvb<T>(F));
   vcycle_v<T>(c, o, tile);
   hipLaunchKernelGGL(k_combine<T>, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, lvx<T>(F), z);
 }
