@@ -936,41 +936,42 @@ __global__ void k_rsep_z(int Xc, int Yc, int Zc, SepComps t, const float4 *__res
 // vpprolong: fine x += sum coarse x * w * koef (parallel/move3mg.c:3535-3850),
 // one thread per fine node and its four unknowns
 template <typename T, int C>
-__device__ __forceinline__ void prolong_one(const CompTab &t, int m1, int m2, int m3, long n, int XYc, int Yc,
-                                            const unsigned char *__restrict__ ff, const T *__restrict__ xc, T *xf,
-                                            T k, int ynreset) {
-  if (kind_of(ff[n], C) == KIND_NONE) return;
-  const long u = 4 * n + C;
+__device__ __forceinline__ T prolong_one(const CompTab &t, int m1, int m2, int m3, int XYc, int Yc,
+                                         const T *__restrict__ xc, T v, T k) {
   const int lx = t.ax[0].lo[m1], ly = t.ax[1].lo[m2], lz = t.ax[2].lo[m3];
-  T v = ynreset ? T(0) : xf[u];
   for (int dz = 0; dz < 2; dz++) {
     const T wz = tw<T>(t.ax[2].w64, t.ax[2].w32, 2 * m3 + dz);
     for (int dx = 0; dx < 2; dx++) {
       const T wx = tw<T>(t.ax[0].w64, t.ax[0].w32, 2 * m1 + dx);
       for (int dy = 0; dy < 2; dy++) {
         const T wy = tw<T>(t.ax[1].w64, t.ax[1].w32, 2 * m2 + dy);
-        const long nc = (long)(lz + dz) * XYc + (long)(lx + dx) * Yc + (ly + dy);
+        const int nc = (lz + dz) * XYc + (lx + dx) * Yc + (ly + dy);  // coarse levels stay below 2^29 nodes
         const T swt = wx * wy * wz;
         v += xc[4 * nc + C] * swt * k;
       }
     }
   }
-  xf[u] = v;
+  return v;
 }
 
+// (the node's kinds, its four fine values and the result in one load or store each)
 template <typename T>
 __global__ void k_prolong(int Xf, int Yf, int Zf, int Xc, int Yc, int Zc, CompTab t0, CompTab t1,
                           CompTab t2, CompTab t3, const unsigned char *__restrict__ ff,
                           const T *__restrict__ xc, T *xf, T pk, T vk, int withp, int ynreset) {
+  using V = typename V4<T>::t;
   const long n = xcd_block() * blockDim.x + threadIdx.x;
   const int XYc = Xc * Yc, XYf = Xf * Yf;
   if (n >= (long)XYf * Zf) return;
   int m1, m2, m3;
   decode(n, Xf, Yf, XYf, m1, m2, m3);
-  if (withp) prolong_one<T, 0>(t0, m1, m2, m3, n, XYc, Yc, ff, xc, xf, pk, ynreset);
-  prolong_one<T, 1>(t1, m1, m2, m3, n, XYc, Yc, ff, xc, xf, vk, ynreset);
-  prolong_one<T, 2>(t2, m1, m2, m3, n, XYc, Yc, ff, xc, xf, vk, ynreset);
-  prolong_one<T, 3>(t3, m1, m2, m3, n, XYc, Yc, ff, xc, xf, vk, ynreset);
+  const unsigned char f = ff[n];
+  V x = reinterpret_cast<const V *>(xf)[n];
+  if (withp && kind_of(f, 0) != KIND_NONE) x.x = prolong_one<T, 0>(t0, m1, m2, m3, XYc, Yc, xc, ynreset ? T(0) : x.x, pk);
+  if (kind_of(f, 1) != KIND_NONE) x.y = prolong_one<T, 1>(t1, m1, m2, m3, XYc, Yc, xc, ynreset ? T(0) : x.y, vk);
+  if (kind_of(f, 2) != KIND_NONE) x.z = prolong_one<T, 2>(t2, m1, m2, m3, XYc, Yc, xc, ynreset ? T(0) : x.z, vk);
+  if (kind_of(f, 3) != KIND_NONE) x.w = prolong_one<T, 3>(t3, m1, m2, m3, XYc, Yc, xc, ynreset ? T(0) : x.w, vk);
+  reinterpret_cast<V *>(xf)[n] = x;
 }
 
 // ronurestrict: coarse nu/ro from fine nodes (parallel/move3mg.c:3387-3528),
