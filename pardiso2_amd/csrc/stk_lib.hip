@@ -1584,7 +1584,7 @@ static int build_schur(stk_ctx *c) {
     hipLaunchKernelGGL(k_kplanes_colour<float>, dim3(nt), dim3(256), 0, c->st, V.X, V.Y, V.Z, dhy, dx,
                        kplanes<float>(V), V.kc32);
   }
-  // coarsest K: dense inverse on the host (at most a few hundred cells)
+  // coarsest K: dense rows assembled on the host (at most a few hundred cells), inverted on the device
   Level &C = c->lv[L];
   std::vector<double> h[7];
   HIPCHK(hipStreamSynchronize(c->st));
@@ -1613,25 +1613,20 @@ static int build_schur(stk_ctx *c) {
   }
   dmean /= N;
   for (auto &v : A) v += dmean / N;
-  // Gauss-Jordan with partial pivoting
-  for (int k = 0; k < N; k++) {
-    int p = k;
-    for (int i = k + 1; i < N; i++)
-      if (std::fabs(A[(size_t)i * N + k]) > std::fabs(A[(size_t)p * N + k])) p = i;
-    if (A[(size_t)p * N + k] == 0.0) return STK_ESINGULAR;
-    if (p != k)
-      for (int j = 0; j < N; j++) {
-        std::swap(A[(size_t)p * N + j], A[(size_t)k * N + j]);
-        std::swap(I[(size_t)p * N + j], I[(size_t)k * N + j]);
-      }
-    const double inv = 1.0 / A[(size_t)k * N + k];
-    for (int j = 0; j < N; j++) { A[(size_t)k * N + j] *= inv; I[(size_t)k * N + j] *= inv; }
-    for (int i = 0; i < N; i++) {
-      if (i == k) continue;
-      const double f = A[(size_t)i * N + k];
-      if (f == 0.0) continue;
-      for (int j = 0; j < N; j++) { A[(size_t)i * N + j] -= f * A[(size_t)k * N + j]; I[(size_t)i * N + j] -= f * I[(size_t)k * N + j]; }
-    }
+  // inverse on the device (the same Gauss-Jordan as the velocity block's)
+  {
+    double *dA = nullptr, *dI = nullptr;
+    if (dalloc(&dA, (size_t)N * N) || dalloc(&dI, (size_t)N * N)) { dfree(dA); dfree(dI); return STK_ENOMEM; }
+    const size_t bytes = (size_t)N * N * 8;
+    int rc = (hipMemcpyAsync(dA, A.data(), bytes, hipMemcpyHostToDevice, c->st) == hipSuccess &&
+              hipStreamSynchronize(c->st) == hipSuccess) ? STK_OK : STK_EHIP;
+    if (!rc) rc = gj_invert(c, dA, N, dI);
+    if (!rc && (hipMemcpyAsync(I.data(), dI, bytes, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+                hipStreamSynchronize(c->st) != hipSuccess))
+      rc = STK_EHIP;
+    dfree(dA);
+    dfree(dI);
+    if (rc) return rc;
   }
   std::vector<float> I32(I.begin(), I.end());
   if (N != c->nkc || !c->kinvT64) {  // same size: refill in place (captured graphs keep their pointers)
