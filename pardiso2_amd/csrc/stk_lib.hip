@@ -766,6 +766,10 @@ void kcycle(stk_ctx *c, int l, int nu, T scale) {
   if (l + small == c->L && small <= kKTail && c->nkc <= kTailCoarse) {
     KTail<T> t;
     t.nlev = small; t.nu = nu; t.nkc = c->nkc; t.scale = scale; t.kcell = c->kcell; t.kinvT = kinvT<T>(c);
+    // the first tail level in LDS where it fits beside the coarse vector
+    const size_t lds = small >= 1 ? 10 * (size_t)c->lv[l].n * sizeof(T) : 0;
+    t.stage = small >= 1 && lds + kTailCoarse * sizeof(T) + 1024 <= c->lds_max;
+    const size_t dyn = t.stage ? lds : 0;
     for (int i = 0; i <= small; i++) {
       Level &W = c->lv[l + i];
       KTailLevel<T> &d = t.lv[i];
@@ -776,11 +780,11 @@ void kcycle(stk_ctx *c, int l, int nu, T scale) {
       d.px = W.kpar[0]; d.py = W.kpar[1]; d.pz = W.kpar[2];
     }
     switch (small) {
-      case 0: hipLaunchKernelGGL((k_ktail<T, 0>), dim3(1), dim3(1024), 0, c->st, t); break;
-      case 1: hipLaunchKernelGGL((k_ktail<T, 1>), dim3(1), dim3(1024), 0, c->st, t); break;
-      case 2: hipLaunchKernelGGL((k_ktail<T, 2>), dim3(1), dim3(1024), 0, c->st, t); break;
-      case 3: hipLaunchKernelGGL((k_ktail<T, 3>), dim3(1), dim3(1024), 0, c->st, t); break;
-      default: hipLaunchKernelGGL((k_ktail<T, 4>), dim3(1), dim3(1024), 0, c->st, t); break;
+      case 0: hipLaunchKernelGGL((k_ktail<T, 0>), dim3(1), dim3(1024), dyn, c->st, t); break;
+      case 1: hipLaunchKernelGGL((k_ktail<T, 1>), dim3(1), dim3(1024), dyn, c->st, t); break;
+      case 2: hipLaunchKernelGGL((k_ktail<T, 2>), dim3(1), dim3(1024), dyn, c->st, t); break;
+      case 3: hipLaunchKernelGGL((k_ktail<T, 3>), dim3(1), dim3(1024), dyn, c->st, t); break;
+      default: hipLaunchKernelGGL((k_ktail<T, 4>), dim3(1), dim3(1024), dyn, c->st, t); break;
     }
     return;
   }

@@ -397,6 +397,7 @@ template <typename T> struct KTailLevel {
 };
 template <typename T> struct KTail {
   int nlev, nu, nkc;
+  int stage;  // 1: the first level's planes and vectors are staged in dynamic LDS
   T scale;
   const int *kcell;
   const T *kinvT;  // [j * nkc + i] = inverse(i, j)
@@ -476,10 +477,29 @@ __device__ __forceinline__ void tail_kprolong(const KTailLevel<T> &V, const KTai
 template <typename T, int NL>
 __global__ void __launch_bounds__(1024) k_ktail(KTail<T> t) {
   __shared__ T sb[kTailCoarse];
+  extern __shared__ unsigned char smem_raw[];
   const int tid = threadIdx.x, nt = blockDim.x;
+  // stage: the first level's seven planes, b, x and r in LDS for all its
+  // phases (x goes back to HBM at the end); the same code runs on either copy
+  KTailLevel<T> lv0 = t.lv[0];
+  if (NL >= 1 && t.stage) {
+    const long n0 = (long)lv0.X * lv0.Y * lv0.Z;
+    T *sl = reinterpret_cast<T *>(smem_raw);
+    for (long u = tid; u < n0; u += nt) {
+#pragma unroll
+      for (int q = 0; q < 7; q++) sl[q * n0 + u] = t.lv[0].k.p[q][u];
+      sl[7 * n0 + u] = t.lv[0].b[u];
+      sl[8 * n0 + u] = t.lv[0].x[u];
+    }
+    for (int q = 0; q < 7; q++) lv0.k.p[q] = sl + q * n0;
+    lv0.b = sl + 7 * n0;
+    lv0.x = sl + 8 * n0;
+    lv0.r = sl + 9 * n0;
+    __syncthreads();
+  }
 #pragma unroll
   for (int i = 0; i < NL; i++) {
-    const KTailLevel<T> &V = t.lv[i];
+    const KTailLevel<T> &V = i == 0 ? lv0 : t.lv[i];
     const long n = (long)V.X * V.Y * V.Z;
 #pragma unroll 1
     for (int s = 0; s < t.nu; s++)
@@ -507,7 +527,7 @@ __global__ void __launch_bounds__(1024) k_ktail(KTail<T> t) {
   }
 #pragma unroll
   for (int i = NL - 1; i >= 0; i--) {
-    const KTailLevel<T> &V = t.lv[i], &C = t.lv[i + 1];
+    const KTailLevel<T> &V = i == 0 ? lv0 : t.lv[i], &C = t.lv[i + 1];
     tail_kprolong<T>(V, C, t.scale);
     __syncthreads();
 #pragma unroll 1
@@ -516,6 +536,10 @@ __global__ void __launch_bounds__(1024) k_ktail(KTail<T> t) {
         tail_kgs<T>(V, col);
         __syncthreads();
       }
+  }
+  if (NL >= 1 && t.stage) {
+    const long n0 = (long)lv0.X * lv0.Y * lv0.Z;
+    for (long u = tid; u < n0; u += nt) t.lv[0].x[u] = lv0.x[u];
   }
 }
 
