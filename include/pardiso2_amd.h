@@ -215,7 +215,8 @@ void matclear(int rm, double *adr);
 void gausmat(int rm, double *adr, double *adr1);
 /* parallel/gaus3mg_MG.c:311 -- store one assembled row (un1/ui1 format) */
 int gausmat2(int am, long mcmax, long pos0cur1, long *un1, double *ui1);
-/* parallel/gaus3mg_MG.c:64 -- am=1: add one coarsest-level CSR row;
+/* parallel/gaus3mg_MG.c:64 -- am=1: set one coarsest-level row from its
+ * list (duplicate columns add up; a later call for the row replaces it);
  * am=0: solve the coarsest system (device dense inverse) and store the
  * solution at sol0[4*(mgp[L]+node)+c] of the bound context. */
 int pardisomat(int am, long mcmax, long mcmax2, long pos0cur1, int tid, long *un1, double *ui1);

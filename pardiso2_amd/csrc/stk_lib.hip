@@ -2567,11 +2567,15 @@ int pardisomat(int am, long mcmax, long mcmax2, long pos0cur1, int tid, long *un
     c->ca.assign((size_t)n * n, 0.0);
     c->cb.assign(n, 0.0);
   }
-  c->cb[mcmax] = ui1[0];
-  for (long m2 = 1; m2 <= un1[0]; m2++) {
+  // the call sets row mcmax (duplicates within the list add up), as the
+  // reference rebuilds the row from its list (parallel/gaus3mg_MG.c:256-306)
+  for (long m2 = 1; m2 <= un1[0]; m2++)
     if (un1[m2] < 0 || un1[m2] >= n) return STK_EINVAL;
-    if (ui1[m2]) c->ca[(size_t)mcmax * n + un1[m2]] += ui1[m2];
-  }
+  c->cb[mcmax] = ui1[0];
+  double *row = c->ca.data() + (size_t)mcmax * n;
+  std::fill(row, row + n, 0.0);
+  for (long m2 = 1; m2 <= un1[0]; m2++)
+    if (ui1[m2]) row[un1[m2]] += ui1[m2];
   return 0;
 }
 

@@ -345,6 +345,44 @@ def test_pardisomat_and_gausmat2_entry_points(oracle_mod):
         g.close()
 
 
+def test_pardisomat_identity_and_singular_system():
+    """The device Gauss-Jordan behind pardisomat(0,..): an identity system
+    comes back exactly (pivots 1, no factors), and the same system with one
+    empty row is reported as singular (STK_ESINGULAR, -4) instead of solved.
+    pardisomat(1,..) sets a row as the reference does (a second assembly
+    replaces the first), so a later valid system solves to its own values."""
+    import ctypes as C
+    n = 13
+    pb = setups.sinker(n, 1, eta_in=1e21, eta_out=1e20)
+    pb.bc = setups.free_slip_lines(n, n, n, gauge=(2, 2, 2))
+    g = S.from_problem(pb)
+    try:
+        g.bind()
+        L = pb.levels
+        nL = 4 * g.info[L][4]
+        un1 = (C.c_long * 4)()
+        ui1 = (C.c_double * 4)()
+        rhs = np.random.default_rng(3).standard_normal(nL)
+
+        def hand_over(empty):
+            for i in range(nL):
+                un1[0] = 0 if i == empty else 1
+                un1[1], ui1[0], ui1[1] = i, rhs[i], 1.0
+                assert S.lib().pardisomat(1, i, 0, 0, 0, un1, ui1) == 0
+
+        hand_over(empty=-1)
+        assert S.lib().pardisomat(0, nL, 0, 0, 0, un1, ui1) == 0
+        np.testing.assert_array_equal(g.vec_get(S.SOL0, L), rhs)
+        hand_over(empty=nL // 3)
+        assert S.lib().pardisomat(0, nL, 0, 0, 0, un1, ui1) == -4
+        rhs *= 2.0
+        hand_over(empty=-1)
+        assert S.lib().pardisomat(0, nL, 0, 0, 0, un1, ui1) == 0
+        np.testing.assert_array_equal(g.vec_get(S.SOL0, L), rhs)
+    finally:
+        g.close()
+
+
 def test_small_dense_elimination_entry_points():
     import ctypes as C
     L = S.lib()
