@@ -2532,7 +2532,7 @@ int gausmat2(int am, long mcmax, long pos0cur1, long *un1, double *ui1) {
   return 0;
 }
 
-// parallel/gaus3mg_MG.c:64-307.  am=1 accumulates one coarsest-level row
+// parallel/gaus3mg_MG.c:64-307.  am=1 sets one coarsest-level row
 // (level-relative indices, duplicate columns summed, zeros dropped);
 // am=0 solves the accumulated system with the device dense inverse and
 // writes sol0 of the coarsest level at 4*(mgp[L]+node)+c (the reference
