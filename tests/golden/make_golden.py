@@ -13,6 +13,12 @@ Fixtures (data only, inputs and expected outputs):
                 parsed with the reference grammar, plus the pressure cell of
                 parallel/init_inclusion.t3c:53 (numbers only)
     grid        (nx, ny, nz, levels, xsize, ysize, zsize, xper, yper, zper)
+  mg_vprestrict_pair.npz
+    fre0_l1     level-1 residual fre0 of the same run after the up-cycle
+                smoothing (mg/sol0_prerestrict.txt column 3, written by
+                mg/move3mgold.c:2507-2515 after vprestrict(2))
+    fre1_l2     the restricted level-2 right-hand side fre1 of that cycle
+                (mg/sol0_gaus_precoarsest.txt column 2, move3mgold.c:2521-2527)
 """
 import os
 import sys
@@ -37,6 +43,10 @@ def main(ref):
                         bc=np.array(lines, dtype=np.float64),
                         grid=np.array([n, n, n, 2, 1.0, 1.0, 1.16, 1, 1, 0], dtype=np.float64))
     print("wrote", os.path.join(HERE, "mg_coarsest_rows.npz"))
+    pre = np.loadtxt(os.path.join(ref, "mg", "sol0_prerestrict.txt"))
+    pc = np.loadtxt(os.path.join(ref, "mg", "sol0_gaus_precoarsest.txt"))
+    np.savez_compressed(os.path.join(HERE, "mg_vprestrict_pair.npz"), fre0_l1=pre[:, 2], fre1_l2=pc[:, 1])
+    print("wrote", os.path.join(HERE, "mg_vprestrict_pair.npz"))
 
 
 if __name__ == "__main__":
