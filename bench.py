@@ -112,6 +112,18 @@ def solution_error(ctx, x):
     return out
 
 
+def smoother_instance(ctx, tile):
+    """Template instance name of the FP32 tile smoother the V-cycle runs on
+    the finest level with this tile (the name rocprofv3 reports)."""
+    ty, tx, tz, sv = tile
+    ch = "true" if ctx.tile_chains(0, tile) else "false"
+    if sv:
+        return f"k_smooth<float, true, 0, 0, 0, {ch}>"
+    if (ty, tx, tz) == (32, 4, 4):
+        return f"k_smooth<float, false, 32, 4, 4, {ch}>"
+    return f"k_smooth<float, false, 0, 0, 0, {ch}>"
+
+
 def smoother_roofline(ctx, nodes):
     ms = ctx.time_kernel(0, 0, reps=50)
     gbs = SMOOTHER_BYTES_PER_NODE * nodes / (ms * 1e-3) / 1e9
@@ -247,9 +259,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="randfield")
-    ap.add_argument("--n", type=int, default=129)
-    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--workload", default="spheres")
+    ap.add_argument("--n", type=int, default=257)
+    ap.add_argument("--seed", type=int, default=None, help="problem seed (default: 3 spheres, 2 random field, 1 sinker)")
     ap.add_argument("--precision", type=int, default=0, help="0 mixed FP32/FP64, 1 FP64")
     ap.add_argument("--restart", type=int, default=128)
     ap.add_argument("--maxit", type=int, default=800)
@@ -266,6 +278,8 @@ def main():
     ap.add_argument("--duration", type=float, default=60.0, help="served workload: arrival window in seconds")
     ap.add_argument("--graph", type=int, default=None, help="1: replay the preconditioner from HIP graphs")
     args = ap.parse_args()
+    if args.seed is None:
+        args.seed = {"spheres": 3, "randfield": 2, "sinker": 1}.get(args.workload, 2)
     if args.gpus != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
               "(instances shard by request; see the design notes).", file=sys.stderr)
@@ -280,11 +294,12 @@ def main():
     L = pb.levels
     pre, post = sweeps_for(L)
     ctx = S.from_problem(pb)
-    if args.kernel:  # counter and trace runs: reps+1 launches of this kernel, nothing else timed
-        ms = ctx.time_kernel(KERNELS[args.kernel], 0, reps=args.reps)
-        log(f"kernel {args.kernel}: {args.reps} launches, {ms:.4f} ms each")
-        return
     tiles = ctx.autotune(args.precision)
+    if args.kernel:  # counter and trace runs: reps+1 launches of this kernel after the tile autotuning
+        ms = ctx.time_kernel(KERNELS[args.kernel], 0, reps=args.reps)
+        log(f"kernel {args.kernel}: {args.reps} launches, {ms:.4f} ms each, finest tile {tiles[0]}, "
+            f"{smoother_instance(ctx, tiles[0])}")
+        return
     log(f"{args.workload} {args.n}^3 ready on the device ({ctx.n0} nodes, {L + 1} levels), smoother tiles {tiles}")
     opts = S.default_opts(L, precision=args.precision, pre=pre, post=post, tol=1e-8, maxit=args.maxit,
                           restart=args.restart)
@@ -312,9 +327,11 @@ def main():
     total = time.perf_counter() - t0
     lat_ms = np.array(lat) * 1e3
 
-    # roofline of the finest-level FP32 smoother (default 32x4x4 tile, the one
-    # the counter runs measure) and FP64 residual (HIP events)
+    # roofline of the finest-level FP32 smoother, the instance the timed
+    # solve runs there (tuned tile, boundary rows in the tiles), and of the
+    # FP64 residual (HIP events on the library stream)
     n0 = ctx.n0
+    inst = smoother_instance(ctx, tiles[0])
     ms_sm = ctx.time_kernel(0, 0, reps=50)
     ms_res = ctx.time_kernel(4, 0, reps=50)
     gbs_sm = SMOOTHER_BYTES_PER_NODE * n0 / (ms_sm * 1e-3) / 1e9
@@ -325,7 +342,7 @@ def main():
         with open(tf) as f:
             tj = json.load(f)
         k = tj.get("kernels", {}).get("smooth32", {})
-        if tj.get("nodes") == n0:
+        if tj.get("nodes") == n0 and k.get("instance", inst) == inst:
             traffic = k.get("hbm_bytes_per_launch")
 
     x_dev = ctx.solution()
@@ -372,7 +389,7 @@ def main():
         "error_vs_fp64_solve": err,
         "oracle_relres": cpu.get("oracle_relres") if cpu else None,
         "smoother_tiles": tiles,
-        "roofline": {"bound": "hbm", "kernel": "k_smooth<float> finest level, 32x4x4 tile", "achieved": gbs_sm,
+        "roofline": {"bound": "hbm", "kernel": f"{inst}, finest level, tile {tiles[0][:3]}", "achieved": gbs_sm,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs_sm / PEAK_HBM_GBS, "traffic": traffic,
                      "ms_per_launch": ms_sm, "bytes_per_launch": SMOOTHER_BYTES_PER_NODE * n0},
         "residual_fp64": {"achieved": gbs_res, "unit": "GB/s", "frac": gbs_res / PEAK_HBM_GBS,

@@ -58,7 +58,7 @@ typedef struct {
   int post[12];      /* post-smoothing sweeps per level */
   int tile[3];       /* smoother tile (y, x, z) in nodes; 0 0 0 = per level: autotuned, else 32 4 4 */
   double omega;      /* pressure block scale of the preconditioner (schur = 0) */
-  int cycle;         /* multigrid cycle index: 1 V-cycle, 2 W-cycle */
+  int cycle;         /* multigrid cycle of the velocity block: 0 or 1, a V-cycle (other values: STK_EINVAL) */
   int coarse_viscmod; /* averaging of coarse-level viscosities: -1 = knobs.viscmod, 0/1/2 */
   double tol;        /* target relative residual ||W r|| / ||W b|| */
   int maxit;         /* outer iteration cap */
@@ -135,10 +135,12 @@ int stk_relres(stk_ctx *ctx, double *relres, double blocks[3]);
 /* Hydrostatic initial guess: v = 0, P integrated up each column through the
  * z-momentum rows (the reference seeds P this way, parallel/in3mg.c:927-938). */
 int stk_hydrostatic_guess(stk_ctx *ctx);
-/* Average device time (ms) of one launch of a finest-level kernel, HIP events
- * on the library stream (smoothers on the default 32x4x4 tile).  which: 0 FP32 smoother sweep, 1 FP64 smoother
- * sweep, 2 FP64 operator apply, 3 FP32 residual, 4 FP64 residual, 5 plain copy
- * of 4*mgn[l] doubles (32 B read + 32 B written per node). */
+/* Average device time (ms) of one launch of a kernel of level `level`, HIP
+ * events on the library stream.  which: 0 FP32 smoother sweep as the V-cycle
+ * runs it there (the level's tuned tile, boundary rows set in the tiles where
+ * the level allows), 1 FP64 smoother sweep (32x4x4 tile), 2 FP64 operator
+ * apply, 3 FP32 residual, 4 FP64 residual, 5 plain copy of 4*mgn[l] doubles
+ * (32 B read + 32 B written per node). */
 int stk_time_kernel(stk_ctx *ctx, int which, int level, int reps, double *ms_avg);
 /* Shift P so that P(px,py,pz) = pinit (parallel/move3mg.c:2835-2856). */
 int stk_pressure_gauge(stk_ctx *ctx, int px, int py, int pz, double pinit);

@@ -66,6 +66,8 @@ def lib():
         L.orc_vcycle.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_relres.restype = C.c_double
         L.orc_relres.argtypes = [C.c_void_p, dp]
+        L.orc_relres_rows.restype = C.c_double
+        L.orc_relres_rows.argtypes = [C.c_void_p, dp, dp]
         L.orc_solve.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_double, C.c_int, dp]
         L.orc_band_solve.argtypes = [C.c_void_p, dp, C.POINTER(C.c_long)]
         L.orc_gcr.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
@@ -214,6 +216,21 @@ class Oracle:
         b = np.zeros(3)
         r = lib().orc_relres(self.h, _dp(b))
         return r, b
+
+    def relres_rows(self, x):
+        """Relative residual of finest-level state x with rows built on the
+        fly (levels too large for the row store)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.size == 4 * self.info[0][4]
+        b = np.zeros(3)
+        r = lib().orc_relres_rows(self.h, _dp(x), _dp(b))
+        return r, b
+
+    def set_fields(self, nu, ro, nxx, nxy, nxz, nyz):
+        """Finest-level node properties given directly (instead of ronurecalc)."""
+        n0 = self.info[0][4]
+        for name, v in (("nu", nu), ("ro", ro), ("nxx", nxx), ("nxy", nxy), ("nxz", nxz), ("nyz", nyz)):
+            self.arr(name)[:n0] = np.asarray(v, dtype=np.float32)[:n0]
 
     def solve(self, pre, post, mode=0, tol=1e-8, maxcyc=200):
         p = np.ascontiguousarray(pre, dtype=np.int32)

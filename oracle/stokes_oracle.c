@@ -480,9 +480,6 @@ static void snn_coef(const orc_t *o, int axis, long m1, long m2, long m3, double
   double ne = (l == 0) ? (double)o->nxx[v[7]] : cellvisc(o, v);
   ne = clampnu(o, ne);
   long k = un1[0];
-  double cx = axis == 0 ? 4.0 / 3.0 : -2.0 / 3.0;
-  double cy = axis == 1 ? 4.0 / 3.0 : -2.0 / 3.0;
-  double cz = axis == 2 ? 4.0 / 3.0 : -2.0 / 3.0;
   /* the reference writes the signs literally per stencil; keep its factor order */
   if (axis == 0) {
     un1[k + 1] = v[0] * 4 + 1; ui1[k + 1] = -w * ne * 4.0 / 3.0 * xkf;
@@ -506,7 +503,6 @@ static void snn_coef(const orc_t *o, int axis, long m1, long m2, long m3, double
     un1[k + 5] = v[0] * 4 + 3; ui1[k + 5] = -w * ne * 4.0 / 3.0 * zkf;
     un1[k + 6] = v[4] * 4 + 3; ui1[k + 6] = +w * ne * 4.0 / 3.0 * zkf;
   }
-  (void)cx; (void)cy; (void)cz;
   un1[0] += 6;
 }
 
@@ -1367,6 +1363,53 @@ double orc_relres(orc_t *o, double *blocks3) {
       }
   if (blocks3) { blocks3[0] = sqrt(bm); blocks3[1] = sqrt(bc); blocks3[2] = sqrt(bb); }
   return sb > 0 ? sqrt(sr) / sqrt(sb) : sqrt(sr);
+}
+
+/* The same W-weighted relative residual for a finest-level state x (4 mgn[0]
+   doubles), with every row built on the fly by orc_row instead of read from
+   the row store, so that levels too large to assemble (57 slots per node) can
+   be scored.  Rows are independent; per-thread partial sums over z planes are
+   added in plane order, so the result does not depend on the thread count.
+   Returns the relative residual; blocks3 as orc_relres. */
+double orc_relres_rows(orc_t *o, const double *x, double *blocks3) {
+  const int X = o->mgx[0], Y = o->mgy[0], Z = o->mgz[0];
+  const double *gx = o->mggx[0], *gy = o->mggy[0], *gz = o->mggz[0];
+  double *part = (double *)calloc((size_t)5 * Z, sizeof(double));
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int m3 = 0; m3 < Z; m3++) {
+    long un1[ORC_MAXPOS];
+    double ui1[ORC_MAXPOS];
+    double sr = 0, sb = 0, bm = 0, bc = 0, bb = 0;
+    for (int m1 = 0; m1 < X; m1++)
+      for (int m2 = 0; m2 < Y; m2++) {
+        long n = NID(o, 0, m1, m2, m3);
+        for (int c = 0; c < 4; c++) {
+          int k = orc_row(o, 0, m1, m2, m3, c, un1, ui1);
+          if (!k) continue;
+          long r = n * 4 + c;
+          double res = ui1[0];
+          for (long q = 1; q <= un1[0]; q++) res -= ui1[q] * x[un1[q]];
+          double w = 1.0;
+          if (c == 0 && !o->bondm[r]) {
+            double h = ((gx[m1] - gx[m1 - 1]) + (gy[m2] - gy[m2 - 1]) + (gz[m3] - gz[m3 - 1])) / 3.0;
+            w = o->nd[n] / h;
+          }
+          sr += (w * res) * (w * res);
+          sb += (w * ui1[0]) * (w * ui1[0]);
+          if (o->bondm[r]) bb += res * res;
+          else if (c == 0) bc += (w * res) * (w * res);
+          else bm += res * res;
+        }
+      }
+    double *p = part + 5 * (size_t)m3;
+    p[0] = sr; p[1] = sb; p[2] = bm; p[3] = bc; p[4] = bb;
+  }
+  double t[5] = {0, 0, 0, 0, 0};
+  for (int m3 = 0; m3 < Z; m3++)
+    for (int q = 0; q < 5; q++) t[q] += part[5 * (size_t)m3 + q];
+  free(part);
+  if (blocks3) { blocks3[0] = sqrt(t[2]); blocks3[1] = sqrt(t[3]); blocks3[2] = sqrt(t[4]); }
+  return t[1] > 0 ? sqrt(t[0]) / sqrt(t[1]) : sqrt(t[0]);
 }
 
 int orc_solve(orc_t *o, const int *pre, const int *post, int mode, double tol, int maxcyc,

@@ -110,6 +110,8 @@ void orc_dense_lu_solve(int n, const double *lu, const int *piv, double *b);
 /* V-cycle (canonical shape of mg/move3mgold.c:2486-2600, fixes for K1/K2) */
 void orc_vcycle(orc_t *o, const int *pre, const int *post, int mode, int ty, int tx, int tz);
 double orc_relres(orc_t *o, double *blocks3);
+/* the same for a finest-level state x, rows built on the fly (no row store) */
+double orc_relres_rows(orc_t *o, const double *x, double *blocks3);
 int orc_solve(orc_t *o, const int *pre, const int *post, int mode, double tol, int maxcyc,
               double *relres_hist);
 

@@ -217,6 +217,8 @@ struct stk_ctx {
   std::vector<int> hlin0, hnum0, hpos0;
   std::vector<double> ca;           // coarsest CSR values (dense accumulation)
   std::vector<double> cb;
+  double *cinv = nullptr;           // inverse of ca, kept until a row changes
+  bool cinv_ok = false;
 };
 
 namespace {
@@ -666,11 +668,11 @@ template <typename T> const T *ainv(stk_ctx *c);
 template <> const float *ainv<float>(stk_ctx *c) { return c->ainv32; }
 template <> const double *ainv<double>(stk_ctx *c) { return c->ainv64; }
 
-// velocity-block multigrid cycle on level l from the current x_l (cycle
-// index gamma: 1 V-cycle, 2 W-cycle); shape of the reference's V-cycle
-// (move3mg.c:2710-2792) with the coarsest level solved exactly every visit
+// velocity-block V-cycle on level l from the current x_l; shape of the
+// reference's V-cycle (move3mg.c:2710-2792) with the coarsest level solved
+// exactly every visit
 template <typename T>
-void mg_cycle(stk_ctx *c, const stk_solver_opts *o, const int *tile, int l, int gamma) {
+void mg_cycle(stk_ctx *c, const stk_solver_opts *o, const int *tile, int l) {
   const int L = c->L;
   if (l == L) {
     Level &C = c->lv[L];
@@ -684,8 +686,7 @@ void mg_cycle(stk_ctx *c, const stk_solver_opts *o, const int *tile, int l, int 
   smooth<T>(c, l, o->pre[l], tile, 0, true);
   rowop<T, true>(c, l, lvx<T>(V), lvb<T>(V), lvr<T>(V), 0);
   restrict_to<T>(c, l + 1, lvr<T>(V), lvb<T>(c->lv[l + 1]), lvx<T>(c->lv[l + 1]), 0);
-  const int g = (l + 1 == L) ? 1 : gamma;
-  for (int k = 0; k < g; k++) mg_cycle<T>(c, o, tile, l + 1, gamma);
+  mg_cycle<T>(c, o, tile, l + 1);
   prolong_from<T>(c, l + 1, lvx<T>(c->lv[l + 1]), lvx<T>(V), 0, 0);
   smooth<T>(c, l, o->post[l], tile, 0, true);
 }
@@ -694,7 +695,7 @@ template <typename T>
 void vcycle_v(stk_ctx *c, const stk_solver_opts *o, const int *tile) {
   Level &V = c->lv[0];
   hipLaunchKernelGGL(k_fill<T>, dim3(blocks(4 * V.n, 256)), dim3(256), 0, c->st, 4 * V.n, T(0), lvx<T>(V));
-  mg_cycle<T>(c, o, tile, 0, o->cycle == 2 ? 2 : 1);
+  mg_cycle<T>(c, o, tile, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1295,7 +1296,7 @@ int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *
 void stk_destroy(stk_ctx *c) {
   if (!c) return;
   if (g_bound == c) g_bound = nullptr;
-  hipDeviceSynchronize();
+  if (c->st) (void)hipStreamSynchronize(c->st);  // all of the context's work runs on its stream
   for (int l = 0; l <= c->L; l++) {
     Level &L = c->lv[l];
     for (int a = 0; a < 3; a++) { dfree(L.iA64[a]); dfree(L.iB64[a]); dfree(L.iA32[a]); dfree(L.iB32[a]); dfree(L.g64[a]); }
@@ -1318,7 +1319,7 @@ void stk_destroy(stk_ctx *c) {
   dfree(c->dinv64); dfree(c->bfy64); dfree(c->bfw64); dfree(c->dinv32); dfree(c->bfy32); dfree(c->bfw32);
   dfree(c->kcell); dfree(c->kinvT64); dfree(c->kinvT32);
   dfree(c->nu); dfree(c->ro); dfree(c->nd); dfree(c->nxx); dfree(c->nxy); dfree(c->nxz); dfree(c->nyz);
-  dfree(c->ainv64); dfree(c->ainvP64); dfree(c->ainv32); dfree(c->cact);
+  dfree(c->ainv64); dfree(c->ainvP64); dfree(c->ainv32); dfree(c->cact); dfree(c->cinv);
   dfree(c->X); dfree(c->B); dfree(c->R); dfree(c->W2); dfree(c->T1); dfree(c->red); dfree(c->scal);
   dfree(c->kred); dfree(c->hcol); dfree(c->gin); dfree(c->gout);
   if (c->hpin) hipHostFree(c->hpin);
@@ -1898,14 +1899,25 @@ int stk_time_kernel(stk_ctx *c, int which, int level, int reps, double *ms_avg) 
   int rc = ensure_vectors(c, true, true);
   if (rc) return rc;
   Level &L = c->lv[level];
-  const int *tt = kTileDefault4;  // the default tile: timings comparable across runs and with counter runs
+  const int *tt = kTileDefault4;  // FP64 sweeps: the default tile
   int tile[3] = {tt[0], tt[1], tt[2]};
   int ty = tile[0], tx = tile[1], tz = tile[2];
   const TileGeo tg = make_tile(ty, tx, tz, L.box);
+  // FP32 sweep: the instance the V-cycle runs on this level (the level's
+  // tuned tile, boundary rows set inside the tiles where the level allows),
+  // its chains composed once before the timed launches
+  const int *t32 = tuned_tile<float>(L);
+  const TileGeo tg32 = make_tile(t32[0], t32[1], t32[2], L.box);
+  const bool sv32 = t32[3] && smooth_lds<float>(t32, true) <= kLevelLds;
+  const bool in32 = which == 0 && tile_bc(c, L, t32);
+  if (in32)
+    hipLaunchKernelGGL(k_bc_compose<float>, dim3(blocks(L.nbc, 256)), dim3(256), 0, c->st, L.chain(), L.bt_idx,
+                       L.b32, chain_c<float>(L), chain_k<float>(L), 1);
   auto launch = [&]() {
     if (which == 0) {
       Geo<float> g = L.geo<float>(c->kn.p2koef);
-      launch_smooth<float>(c, tile, tt[3] != 0, tg, g, L.flags, L.x32[0], L.x32[1], L.b32, 0);
+      launch_smooth<float>(c, t32, sv32, tg32, g, L.flags, L.x32[0], L.x32[1], L.b32, 0,
+                           in32 ? tile_chains<float>(L) : nullptr);
     } else if (which == 1) {
       Geo<double> g = L.geo<double>(c->kn.p2koef);
       const bool sv = tt[3] && smooth_lds<double>(tile, true) <= kLevelLds;
@@ -2140,7 +2152,7 @@ static int solve_setup(stk_ctx *c, const stk_solver_opts *o, HistLease &lease) {
 int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   if (!c || !o) return STK_EINVAL;
   if (!c->prepared) return STK_ESTATE;
-  if (o->restart < 1 || o->restart > kMaxDirs - 1 || o->maxit < 1 || !(o->tol > 0) || o->cycle < 0 || o->cycle > 2 ||
+  if (o->restart < 1 || o->restart > kMaxDirs - 1 || o->maxit < 1 || !(o->tol > 0) || o->cycle < 0 || o->cycle > 1 ||
       o->coarse_viscmod < -1 || o->coarse_viscmod > 2 || o->schur < 0 || o->schur > 2 || o->ksweeps < 0 ||
       o->ksweeps > 16 || !(o->kscale > 0) || o->krylov < 0 || o->krylov > 1 || o->graph < 0 || o->graph > 1 ||
       o->history < 0 || o->history > 1)
@@ -2545,22 +2557,28 @@ int pardisomat(int am, long mcmax, long mcmax2, long pos0cur1, int tid, long *un
   int n = (int)(4 * C.n);
   if (am == 0) {
     if (c->ca.size() != (size_t)n * n) return STK_ESTATE;
-    double *A = nullptr, *Bm = nullptr, *db = nullptr, *dx = nullptr;
-    if (dalloc(&A, (size_t)n * n) || dalloc(&Bm, (size_t)n * n) || dalloc(&db, n) || dalloc(&dx, n)) {
-      dfree(A); dfree(Bm); dfree(db); dfree(dx);
-      return STK_ENOMEM;
+    // the inverse is built once and reused until pardisomat(1, ...) changes a row
+    if (!c->cinv_ok) {
+      double *A = nullptr;
+      if (!c->cinv && dalloc(&c->cinv, (size_t)n * n)) return STK_ENOMEM;
+      if (dalloc(&A, (size_t)n * n)) return STK_ENOMEM;
+      H2D(c, A, c->ca.data(), (size_t)n * n * 8);
+      const int hs = gj_invert(c, A, n, c->cinv);
+      dfree(A);
+      if (hs) return hs;
+      c->cinv_ok = true;
     }
-    H2D(c, A, c->ca.data(), (size_t)n * n * 8);
+    double *db = nullptr, *dx = nullptr;
+    if (dalloc(&db, n) || dalloc(&dx, n)) { dfree(db); dfree(dx); return STK_ENOMEM; }
     H2D(c, db, c->cb.data(), (size_t)n * 8);
-    const int hs = gj_invert(c, A, n, Bm);
-    if (!hs) hipLaunchKernelGGL((k_gemv<double, double>), dim3(blocks(n, 4)), dim3(256), 0, c->st, Bm, db, dx, n);
+    hipLaunchKernelGGL((k_gemv<double, double>), dim3(blocks(n, 4)), dim3(256), 0, c->st, c->cinv, db, dx, n);
     int rc = ensure_ref(c, c->L);
-    if (!rc && !hs) {
+    if (!rc) {
       HIPCHK(hipMemcpyAsync(C.ref[0], dx, (size_t)n * 8, hipMemcpyDeviceToDevice, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
     }
-    dfree(A); dfree(Bm); dfree(db); dfree(dx);
-    return hs ? hs : rc;
+    dfree(db); dfree(dx);
+    return rc;
   }
   if (mcmax < 0 || mcmax >= n) return STK_EINVAL;
   if (c->ca.size() != (size_t)n * n) {
@@ -2568,14 +2586,19 @@ int pardisomat(int am, long mcmax, long mcmax2, long pos0cur1, int tid, long *un
     c->cb.assign(n, 0.0);
   }
   // the call sets row mcmax (duplicates within the list add up), as the
-  // reference rebuilds the row from its list (parallel/gaus3mg_MG.c:256-306)
+  // reference rebuilds the row from its list (parallel/gaus3mg_MG.c:256-306);
+  // the kept inverse stays valid while the matrix part of every row is unchanged
   for (long m2 = 1; m2 <= un1[0]; m2++)
     if (un1[m2] < 0 || un1[m2] >= n) return STK_EINVAL;
   c->cb[mcmax] = ui1[0];
-  double *row = c->ca.data() + (size_t)mcmax * n;
-  std::fill(row, row + n, 0.0);
+  std::vector<double> nrow(n, 0.0);
   for (long m2 = 1; m2 <= un1[0]; m2++)
-    if (ui1[m2]) row[un1[m2]] += ui1[m2];
+    if (ui1[m2]) nrow[un1[m2]] += ui1[m2];
+  double *row = c->ca.data() + (size_t)mcmax * n;
+  if (!std::equal(nrow.begin(), nrow.end(), row)) {
+    std::copy(nrow.begin(), nrow.end(), row);
+    c->cinv_ok = false;
+  }
   return 0;
 }
 

@@ -65,29 +65,53 @@ def payload_from_problem(pb) -> dict:
 
 
 class ContextPool:
-    """Resident contexts per grid size, handed to one request at a time."""
+    """Resident contexts per grid size, each handed to one request at a time.
 
-    def __init__(self, restart: int):
+    ``factory(n, payload)`` builds a new context (a library context with the
+    free-slip boundary lines of an n^3 box unless given)."""
+
+    def __init__(self, restart: int, factory=None):
         self.restart = restart
         self.free: dict[int, list] = {}
         self.lock = threading.Lock()
         self.created = 0
+        self.evicted = 0
+        self.factory = factory or self._make
 
-    def acquire(self, n: int, payload: dict):
-        with self.lock:
-            lst = self.free.setdefault(n, [])
-            if lst:
-                return lst.pop()
+    @staticmethod
+    def _make(n: int, payload: dict):
         gx, gy, gz = payload["grid"]
         ctx = S.Stokes(gx, gy, gz, payload["levels"])
         ctx.bc_lines(setups.free_slip_lines(n, n, n, gauge=None))
+        return ctx
+
+    def take_idle(self, n: int):
+        """An idle resident context of size n, removed from the pool, or None."""
+        with self.lock:
+            lst = self.free.get(n)
+            return lst.pop() if lst else None
+
+    def create(self, n: int, payload: dict):
+        ctx = self.factory(n, payload)
         with self.lock:
             self.created += 1
         return ctx
 
     def release(self, n: int, ctx):
         with self.lock:
-            self.free[n].append(ctx)
+            self.free.setdefault(n, []).append(ctx)
+
+    def evict_other(self, n: int) -> bool:
+        """Free the idle contexts of every size but n; True if any was freed."""
+        with self.lock:
+            victims = [c for m, lst in self.free.items() if m != n for c in lst]
+            for m in list(self.free):
+                if m != n:
+                    self.free[m] = []
+            self.evicted += len(victims)
+        for c in victims:
+            c.close()
+        return bool(victims)
 
     def close(self):
         with self.lock:
@@ -98,36 +122,66 @@ class ContextPool:
 
 
 class Server:
-    """Worker threads over a shared admission queue."""
+    """Worker threads over a shared admission queue.
+
+    Admission by device memory: a request reuses an idle resident context of
+    its size when one exists; otherwise a new context is admitted only when
+    the free device memory, less the bytes admitted for contexts that are not
+    fully allocated yet (a context reaches its full size during its first
+    solve, when the Krylov vectors are allocated), covers the context's bytes
+    plus a reserve.  A request that does not fit first frees the idle
+    contexts of other sizes, then waits for running solves to end.  When no
+    solve is running and nothing is pending, it is admitted anyway, so that a
+    request larger than the device fails with an error instead of waiting
+    forever."""
 
     def __init__(self, workers: int = 16, opts_kw: dict | None = None, reserve_bytes: int = 8 << 30,
-                 keep_solutions: bool = False):
+                 keep_solutions: bool = False, mem_info=None, factory=None):
         self.workers = workers
         self.opts_kw = dict(opts_kw or {})
         self.opts_kw["graph"] = 0  # workers launch eagerly (no concurrent stream captures)
         self.restart = self.opts_kw.get("restart", 128)
-        self.pool = ContextPool(self.restart)
+        self.pool = ContextPool(self.restart, factory)
         self.reserve = reserve_bytes
         self.keep = keep_solutions
+        self.mem_info = mem_info or S.mem_info
         self.q: queue.Queue = queue.Queue()
         self.done: list = []
         self.lock = threading.Lock()
         self.mem_cv = threading.Condition()
-        self.committed = 0
+        self.pending = 0   # bytes admitted for contexts not yet fully allocated
+        self.busy = 0      # requests holding a context
         self.threads: list = []
 
-    def _admit(self, need: int):
+    def _context_for(self, n: int, payload: dict):
+        """(context, bytes still pending for it): an idle resident one, or a
+        new one once admitted."""
+        need = context_bytes(n, payload["levels"], self.restart)
         with self.mem_cv:
             while True:
-                free, _ = S.mem_info()
-                if self.committed == 0 or free - need >= self.reserve:
-                    self.committed += need
-                    return
+                ctx = self.pool.take_idle(n)
+                if ctx is not None:
+                    self.busy += 1
+                    return ctx, 0
+                free, _ = self.mem_info()
+                fits = free - self.pending - need >= self.reserve
+                if not fits and self.pool.evict_other(n):
+                    continue
+                if fits or (self.busy == 0 and self.pending == 0):
+                    self.pending += need
+                    self.busy += 1
+                    break
                 self.mem_cv.wait(0.01)
+        try:
+            return self.pool.create(n, payload), need
+        except Exception:
+            self._finish(need)
+            raise
 
-    def _retire(self, need: int):
+    def _finish(self, pending: int):
         with self.mem_cv:
-            self.committed -= need
+            self.pending -= pending
+            self.busy -= 1
             self.mem_cv.notify_all()
 
     def _work(self):
@@ -136,18 +190,14 @@ class Server:
             if req is None:
                 return
             p = req.payload
-            with self.pool.lock:
-                has_ctx = bool(self.pool.free.get(req.n))
-            need = 0 if has_ctx else context_bytes(req.n, p["levels"], self.restart)
-            self._admit(need)
-            req.start = time.perf_counter()
             try:
-                ctx = self.pool.acquire(req.n, p)
+                ctx, pend = self._context_for(req.n, p)
             except Exception as e:  # a failed request must still complete
                 req.error, req.done = repr(e), time.perf_counter()
                 with self.lock:
                     self.done.append(req)
                 continue
+            req.start = time.perf_counter()
             try:
                 k = p["knobs"]
                 ctx.knobs(k["p0koef"], k["p1koef"], k["p2koef"], k["v0koef"], k["v1koef"], k["nubeg"], k["nuend"],
@@ -162,7 +212,7 @@ class Server:
                 req.error = repr(e)
             finally:
                 self.pool.release(req.n, ctx)
-                self._retire(0)
+                self._finish(pend)
             req.done = time.perf_counter()
             with self.lock:
                 self.done.append(req)

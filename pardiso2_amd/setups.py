@@ -188,7 +188,7 @@ def gaussian_random_field(n: int, corr: float, seed: int) -> np.ndarray:
     kx, ky, kz = np.meshgrid(k, k, np.fft.rfftfreq(m, d=1.0 / (m - 1)) * 2 * math.pi, indexing="ij")
     spec = np.exp(-0.25 * (kx ** 2 + ky ** 2 + kz ** 2) * corr ** 2)
     noise = rng.standard_normal((m, m, m))
-    f = np.fft.irfftn(np.fft.rfftn(noise) * np.sqrt(spec), s=(m, m, m))[:n, :n, :n]
+    f = np.fft.irfftn(np.fft.rfftn(noise) * np.sqrt(spec), s=(m, m, m), axes=(0, 1, 2))[:n, :n, :n]
     f = (f - f.mean()) / (f.std() + 1e-300)
     return f
 
