@@ -117,6 +117,16 @@ int stk_set_fields(stk_ctx *ctx, const float *nu, const float *ro, const float *
  * parallel/mark3mg.c:630-1132, for per-marker viscosity and density). */
 int stk_markers_to_nodes(stk_ctx *ctx, long nmark, const float *mx, const float *my,
                          const float *mz, const float *mnu, const float *mro, double nukoef);
+/* The same averaging, with the same results, without allocating: every device
+ * buffer of the call (staged and sorted marker fields, cell keys, cell starts,
+ * sort storage) lives in one static device array per GPU, about 73 MB, held
+ * until the process ends.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Limits: nmark <= 2^20 markers and
+ * (nx-1)(ny-1)(nz-1) <= 2^18 cells, else STK_EINVAL.  Calls on one GPU share
+ * its array: calls that overlap in time on the same GPU race. */
+int stk_markers_to_nodes_noalloc(stk_ctx *ctx, long nmark, const float *mx, const float *my,
+                                 const float *mz, const float *mnu, const float *mro, double nukoef);
 /* node arrays over all levels: nu, ro (ntot), nd (ntot), level-0 nxx..nyz */
 int stk_get_fields(stk_ctx *ctx, float *nu, float *ro, float *nd, float *nxx, float *nxy,
                    float *nxz, float *nyz);

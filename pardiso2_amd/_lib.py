@@ -19,7 +19,8 @@ ERRORS = {0: "ok", -1: "invalid argument", -2: "device out of memory", -3: "HIP 
 
 EXPORTS = [
     "stk_create", "stk_destroy", "stk_level_info", "stk_set_knobs", "stk_default_opts", "stk_bc_line",
-    "stk_set_bc_tables", "stk_get_bc_tables", "stk_set_fields", "stk_markers_to_nodes", "stk_get_fields",
+    "stk_set_bc_tables", "stk_get_bc_tables", "stk_set_fields", "stk_markers_to_nodes",
+    "stk_markers_to_nodes_noalloc", "stk_get_fields",
     "stk_prepare", "stk_set_solution", "stk_get_solution", "stk_solve", "stk_relres",
     "stk_pressure_gauge", "stk_hydrostatic_guess", "stk_time_kernel", "stk_vec_set", "stk_vec_get", "stk_apply", "stk_smooth", "stk_smooth_ex", "stk_vrestrict32", "stk_coarse_solve",
     "stk_bind", "matclear", "gausmat", "gausmat2", "pardisomat", "gausmat4", "vprestrict", "vpprolong",
@@ -77,6 +78,7 @@ def lib():
         L.stk_get_bc_tables.argtypes = [vp, ip, ip, fp, ip]
         L.stk_set_fields.argtypes = [vp] + [fp] * 6
         L.stk_markers_to_nodes.argtypes = [vp, C.c_long] + [fp] * 5 + [C.c_double]
+        L.stk_markers_to_nodes_noalloc.argtypes = [vp, C.c_long] + [fp] * 5 + [C.c_double]
         L.stk_get_fields.argtypes = [vp] + [fp] * 7
         L.stk_prepare.argtypes = [vp]
         L.stk_set_solution.argtypes = [vp, dp]
@@ -202,6 +204,12 @@ class Stokes:
         a = [np.ascontiguousarray(v, dtype=np.float32) for v in (mx, my, mz, mnu, mro)]
         check(lib().stk_markers_to_nodes(self.h, len(a[0]), *[_fp(v) for v in a], nukoef),
               "stk_markers_to_nodes")
+
+    def markers_to_nodes_noalloc(self, mx, my, mz, mnu, mro, nukoef=0.0):
+        """The same averaging without device allocation (static per-GPU scratch);
+        returns the library's status code."""
+        a = [np.ascontiguousarray(v, dtype=np.float32) for v in (mx, my, mz, mnu, mro)]
+        return lib().stk_markers_to_nodes_noalloc(self.h, len(a[0]), *[_fp(v) for v in a], nukoef)
 
     def fields(self):
         nu = np.zeros(self.ntot, np.float32)

@@ -1202,6 +1202,77 @@ static void kcycle_host(stk_ctx *c, int nu, double scale, const double *b, doubl
   for (long i = 0; i < F.n; i++) x[i] = (double)hx[i];
 }
 
+// ---------------------------------------------------------------------------
+// marker -> node averaging: device buffers of one call, and the kernels
+// ---------------------------------------------------------------------------
+struct MarkerBufs {
+  float *in[5] = {}, *so[5] = {};
+  int *key = nullptr, *val = nullptr, *skey = nullptr, *sval = nullptr, *start = nullptr;
+  void *tmp = nullptr;
+  size_t tmpb = 0;
+};
+
+// host marker arrays staged into b.in, binned by cell (stable radix sort),
+// permuted, cell starts, then the per-node gather into the context's fields
+static int markers_run(stk_ctx *c, long nmark, const float *mx, const float *my, const float *mz, const float *mnu,
+                       const float *mro, double nukoef, MarkerBufs &b) {
+  Level &F = c->lv[0];
+  const int ncell = (F.X - 1) * (F.Y - 1) * (F.Z - 1);
+  const float *src[5] = {mx, my, mz, mnu, mro};
+  for (int k = 0; k < 5; k++) HIPCHK(hipMemcpyAsync(b.in[k], src[k], nmark * 4, hipMemcpyHostToDevice, c->st));
+  hipLaunchKernelGGL(k_marker_cells, dim3(blocks(nmark, 256)), dim3(256), 0, c->st, nmark, b.in[0], b.in[1], b.in[2],
+                     F.g64[0], F.g64[1], F.g64[2], F.X, F.Y, F.Z, b.key, b.val);
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(b.tmp, b.tmpb, b.key, b.skey, b.val, b.sval, (int)nmark, 0, 32, c->st));
+  hipLaunchKernelGGL(k_marker_permute, dim3(blocks(nmark, 256)), dim3(256), 0, c->st, nmark, b.sval, b.in[0],
+                     b.in[1], b.in[2], b.in[3], b.in[4], b.so[0], b.so[1], b.so[2], b.so[3], b.so[4]);
+  // every start[0..ncell] is written (no entry needs clearing first)
+  hipLaunchKernelGGL(k_cell_start, dim3(blocks(nmark + 1, 256)), dim3(256), 0, c->st, nmark, b.skey, ncell + 1,
+                     b.start);
+  hipLaunchKernelGGL(k_markers_gather, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, F.g64[0],
+                     F.g64[1], F.g64[2], b.start, b.so[0], b.so[1], b.so[2], b.so[3], b.so[4], c->kn.nubeg,
+                     c->kn.nuend, nukoef, c->kn.viscmod, c->nu, c->ro, c->nxx, c->nxy, c->nxz, c->nyz);
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipGetLastError());
+  c->fields_set = true;
+  c->prepared = false;
+  return STK_OK;
+}
+
+// Scratch of stk_markers_to_nodes_noalloc: 14 arrays of kScratchMarkers
+// 4-byte words (5 staged and 5 sorted marker fields, keys, indices and their
+// sorted copies), the cell starts, and the radix sort's temporary storage,
+// each 256-byte aligned.  One copy per GPU, held until the process ends.
+constexpr long kScratchMarkers = 1L << 20;
+constexpr long kScratchCells = 1L << 18;
+constexpr size_t kScratchArr = ((size_t)kScratchMarkers * 4 + 255) & ~(size_t)255;
+constexpr size_t kScratchStart = ((size_t)(kScratchCells + 2) * 4 + 255) & ~(size_t)255;
+constexpr size_t kScratchSort = (size_t)16 << 20;
+constexpr size_t kScratchBytes = 14 * kScratchArr + kScratchStart + kScratchSort;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of this GPU's scratch array, looked up once per device
+// (host code never uses the array's name as an address).
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[64] = {};
+
+static int scratch_base(unsigned char **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return STK_EINVAL;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "pardiso2_amd: scratch address on device %d: %s\n", dev, hipGetErrorString(e));
+      return STK_EHIP;
+    }
+    g_scratch_ptr[dev] = (unsigned char *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return STK_OK;
+}
+
 extern "C" {
 
 int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *gx, const double *gy,
@@ -1466,37 +1537,59 @@ int stk_markers_to_nodes(stk_ctx *c, long nmark, const float *mx, const float *m
   if (!c || nmark <= 0 || nmark > 0x7ffffff0L || !mx || !my || !mz || !mnu || !mro) return STK_EINVAL;
   Level &F = c->lv[0];
   const int ncell = (F.X - 1) * (F.Y - 1) * (F.Z - 1);
-  float *in[5] = {}, *so[5] = {};
-  int *key = nullptr, *val = nullptr, *skey = nullptr, *sval = nullptr, *start = nullptr;
-  void *tmp = nullptr;
+  MarkerBufs b;
   size_t tmpb = 0;
   int rc = STK_OK;
-  const float *src[5] = {mx, my, mz, mnu, mro};
-  for (int k = 0; k < 5 && !rc; k++) rc = dalloc(&in[k], nmark) || dalloc(&so[k], nmark) ? STK_ENOMEM : 0;
-  if (!rc) rc = (dalloc(&key, nmark) || dalloc(&val, nmark) || dalloc(&skey, nmark) || dalloc(&sval, nmark) ||
-                 dalloc(&start, ncell + 2)) ? STK_ENOMEM : 0;
-  if (!rc) {
-    for (int k = 0; k < 5; k++) hipMemcpyAsync(in[k], src[k], nmark * 4, hipMemcpyHostToDevice, c->st);
-    hipLaunchKernelGGL(k_marker_cells, dim3(blocks(nmark, 256)), dim3(256), 0, c->st, nmark, in[0], in[1], in[2],
-                       F.g64[0], F.g64[1], F.g64[2], F.X, F.Y, F.Z, key, val);
-    hipcub::DeviceRadixSort::SortPairs(nullptr, tmpb, key, skey, val, sval, (int)nmark, 0, 32, c->st);
-    if (dalloc((unsigned char **)&tmp, tmpb)) rc = STK_ENOMEM;
-  }
-  if (!rc) {
-    hipcub::DeviceRadixSort::SortPairs(tmp, tmpb, key, skey, val, sval, (int)nmark, 0, 32, c->st);
-    hipLaunchKernelGGL(k_marker_permute, dim3(blocks(nmark, 256)), dim3(256), 0, c->st, nmark, sval, in[0], in[1],
-                       in[2], in[3], in[4], so[0], so[1], so[2], so[3], so[4]);
-    hipLaunchKernelGGL(k_cell_start, dim3(blocks(nmark + 1, 256)), dim3(256), 0, c->st, nmark, skey, ncell + 1, start);
-    hipLaunchKernelGGL(k_markers_gather, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, F.g64[0],
-                       F.g64[1], F.g64[2], start, so[0], so[1], so[2], so[3], so[4], c->kn.nubeg, c->kn.nuend,
-                       nukoef, c->kn.viscmod, c->nu, c->ro, c->nxx, c->nxy, c->nxz, c->nyz);
-    if (hipStreamSynchronize(c->st) != hipSuccess) rc = STK_EHIP;
-  }
-  for (int k = 0; k < 5; k++) { dfree(in[k]); dfree(so[k]); }
-  dfree(key); dfree(val); dfree(skey); dfree(sval); dfree(start);
-  if (tmp) hipFree(tmp);
-  if (!rc) { c->fields_set = true; c->prepared = false; }
+  for (int k = 0; k < 5 && !rc; k++) rc = dalloc(&b.in[k], nmark) || dalloc(&b.so[k], nmark) ? STK_ENOMEM : 0;
+  if (!rc) rc = (dalloc(&b.key, nmark) || dalloc(&b.val, nmark) || dalloc(&b.skey, nmark) || dalloc(&b.sval, nmark) ||
+                 dalloc(&b.start, ncell + 2)) ? STK_ENOMEM : 0;
+  if (!rc && hipcub::DeviceRadixSort::SortPairs(nullptr, tmpb, b.key, b.skey, b.val, b.sval, (int)nmark, 0, 32,
+                                                c->st) != hipSuccess)
+    rc = STK_EHIP;
+  if (!rc && dalloc((unsigned char **)&b.tmp, tmpb)) rc = STK_ENOMEM;
+  b.tmpb = tmpb;
+  if (!rc) rc = markers_run(c, nmark, mx, my, mz, mnu, mro, nukoef, b);
+  for (int k = 0; k < 5; k++) { dfree(b.in[k]); dfree(b.so[k]); }
+  dfree(b.key); dfree(b.val); dfree(b.skey); dfree(b.sval); dfree(b.start);
+  if (b.tmp) hipFree(b.tmp);
   return rc;
+}
+
+// The same averaging with every device buffer carved from this GPU's static
+// scratch array (no hipMalloc, hipFree or allocating library call).
+int stk_markers_to_nodes_noalloc(stk_ctx *c, long nmark, const float *mx, const float *my, const float *mz,
+                                 const float *mnu, const float *mro, double nukoef) {
+  if (!c || nmark <= 0 || !mx || !my || !mz || !mnu || !mro) return STK_EINVAL;
+  Level &F = c->lv[0];
+  const long ncell = (long)(F.X - 1) * (F.Y - 1) * (F.Z - 1);
+  if (nmark > kScratchMarkers || ncell > kScratchCells) {
+    fprintf(stderr,
+            "pardiso2_amd: stk_markers_to_nodes_noalloc: %ld markers and %ld cells exceed the scratch limit "
+            "(%ld markers, %ld cells)\n",
+            nmark, ncell, kScratchMarkers, kScratchCells);
+    return STK_EINVAL;
+  }
+  unsigned char *base = nullptr;
+  int rc = scratch_base(&base);
+  if (rc) return rc;
+  MarkerBufs b;
+  size_t off = 0;
+  for (int k = 0; k < 5; k++) { b.in[k] = (float *)(base + off); off += kScratchArr; }
+  for (int k = 0; k < 5; k++) { b.so[k] = (float *)(base + off); off += kScratchArr; }
+  int **ints[4] = {&b.key, &b.val, &b.skey, &b.sval};
+  for (int k = 0; k < 4; k++) { *ints[k] = (int *)(base + off); off += kScratchArr; }
+  b.start = (int *)(base + off);
+  off += kScratchStart;
+  b.tmp = base + off;
+  size_t tmpb = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpb, b.key, b.skey, b.val, b.sval, (int)nmark, 0, 32, c->st));
+  if (tmpb > kScratchSort) {
+    fprintf(stderr, "pardiso2_amd: stk_markers_to_nodes_noalloc: the sort needs %zu bytes, the scratch holds %zu\n",
+            tmpb, kScratchSort);
+    return STK_ENOMEM;
+  }
+  b.tmpb = tmpb;
+  return markers_run(c, nmark, mx, my, mz, mnu, mro, nukoef, b);
 }
 
 // nesting of level-l cells in level-(l-1) cells along one axis: coarse line
