@@ -117,6 +117,13 @@ def smoother_instance(ctx, tile):
     the finest level with this tile (the name rocprofv3 reports)."""
     ty, tx, tz, sv = tile
     ch = "true" if ctx.tile_chains(0, tile) else "false"
+    pairs = (ty + 1) // 2 * tx * tz
+    if not sv and pairs <= 512:  # one node pair per thread: the prefetching kernel
+        shape = {(32, 4, 4): "32, 4, 4", (32, 4, 8): "32, 4, 8", (32, 8, 4): "32, 8, 4"}.get((ty, tx, tz), "0, 0, 0")
+        # uniform spacing as the library tests it: one FP32 inverse spacing per axis
+        inv = [(1.0 / np.diff(a)).astype(np.float32) for a in ctx._g]
+        uni = "true" if all(np.all(v == v[0]) for v in inv) else "false"
+        return f"k_vsmooth32<{shape}, {ch}, {uni}, {256 if pairs <= 256 else 512}>"
     if sv:
         return f"k_smooth<float, true, 0, 0, 0, {ch}>"
     if (ty, tx, tz) == (32, 4, 4):

@@ -186,6 +186,9 @@ int stk_coarse_solve(stk_ctx *ctx);
 int stk_autotune(stk_ctx *ctx, int precision, int reps);
 /* tile (y, x, z, viscosities staged) a solve uses on a level */
 int stk_level_tile(const stk_ctx *ctx, int level, int precision, int out4[4]);
+/* set that tile by hand (instead of or after stk_autotune); STK_EINVAL if
+ * its LDS would exceed 64 KB.  No counterpart in the reference: a tuning hook. */
+int stk_set_level_tile(stk_ctx *ctx, int level, int precision, const int tile4[4]);
 /* 1 when the tile sweeps of shape ty x tx x tz on a level set the level's
  * boundary rows themselves (each tile evaluates the chains it owns in
  * composed form), 0 when a separate chain pass follows every sweep (no

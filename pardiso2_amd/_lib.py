@@ -25,7 +25,7 @@ EXPORTS = [
     "stk_pressure_gauge", "stk_hydrostatic_guess", "stk_time_kernel", "stk_vec_set", "stk_vec_get", "stk_apply", "stk_smooth", "stk_smooth_ex", "stk_vrestrict32", "stk_coarse_solve",
     "stk_bind", "matclear", "gausmat", "gausmat2", "pardisomat", "gausmat4", "vprestrict", "vpprolong",
     "ronurestrict", "vpiterate", "stk_kop_apply", "stk_kcycle_apply", "stk_schur_apply",
-    "stk_solve_batch", "stk_mem_info", "stk_autotune", "stk_level_tile", "stk_tile_chains",
+    "stk_solve_batch", "stk_mem_info", "stk_autotune", "stk_level_tile", "stk_set_level_tile", "stk_tile_chains",
 ]
 
 
@@ -102,6 +102,7 @@ def lib():
         L.stk_mem_info.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
         L.stk_autotune.argtypes = [vp, C.c_int, C.c_int]
         L.stk_level_tile.argtypes = [vp, C.c_int, C.c_int, ip]
+        L.stk_set_level_tile.argtypes = [vp, C.c_int, C.c_int, ip]
         L.stk_tile_chains.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.stk_kcycle_apply.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, dp]
         L.stk_schur_apply.argtypes = [vp, C.POINTER(Opts), dp, dp]
@@ -297,6 +298,11 @@ class Stokes:
             check(lib().stk_level_tile(self.h, l, precision, _ip(t)), "stk_level_tile")
             out.append(tuple(int(v) for v in t))
         return out
+
+    def set_tile(self, level, tile, precision=0):
+        """Smoother tile (y, x, z, viscosities staged) of a level, by hand."""
+        t = np.ascontiguousarray(list(tile) + [0] * (4 - len(tile)), dtype=np.int32)
+        check(lib().stk_set_level_tile(self.h, level, precision, _ip(t)), "stk_set_level_tile")
 
     def tile_chains(self, level, tile):
         """True when sweeps of this tile shape set the level's boundary rows in the tiles."""

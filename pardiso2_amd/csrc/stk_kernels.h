@@ -22,6 +22,26 @@ __device__ __forceinline__ long xcd_block() {
   return x * q + (x < r ? x : r) + j;
 }
 
+// Tile of a workgroup for the tile sweeps: the blocks (renumbered by
+// xcd_block, so each XCD owns a contiguous run) walk the tiles in pencils
+// of kPencil tiles along x: within a pencil, y fastest, then x, then z.  An
+// XCD then works on a pencil's z-layers in turn, so the halo planes a tile
+// shares with the tiles below it and beside it were read by tiles that ran
+// shortly before on the same XCD (its L2), not a whole z-layer of the level
+// earlier.  The map is a bijection of the tile indices; results do not
+// depend on it (tiles are independent within a sweep).
+constexpr int kPencil = 8;
+__device__ __forceinline__ void tile_of_block(int p, int nty, int ntx, int ntz, int &by, int &bx, int &bz) {
+  const int full = nty * ntz * kPencil;
+  const int pp = p / full, r = p - pp * full;
+  const int w = min(kPencil, ntx - pp * kPencil);
+  by = r % nty;
+  const int q = r / nty;
+  const int kxi = q % w;
+  bz = q / w;
+  bx = pp * kPencil + kxi;
+}
+
 __device__ __forceinline__ void decode(long n, int X, int Y, int XY, int &m1, int &m2, int &m3) {
   m3 = (int)(n / XY);
   int rem = (int)(n - (long)m3 * XY);
@@ -298,25 +318,62 @@ __global__ void k_bc_compose(BcChain ch, const int *__restrict__ idx, const T *_
 
 // Stage a tile and its one-node halo of (P, vx, vy, vz) in LDS, component-
 // major (zeros outside the level), and the unknown kinds of the interior.
-template <typename T, int TY, int TX>
+template <typename T, int TY, int TX, int TZ = 0, int BS = 256>
 __device__ __forceinline__ void tile_stage(const Geo<T> &g, const unsigned char *__restrict__ flags,
                                            const T *__restrict__ xold, T *sx, unsigned char *sf, const TileGeo &tg,
                                            int x0, int y0, int z0, int ty, int tx, int tz, int nh) {
+  using V = typename V4<T>::t;
   const int hy = ty + 2, hx = tx + 2;
+  constexpr int kNh = (TY && TX && TZ) ? (TY + 2) * (TX + 2) * (TZ + 2) : 0;
+  if constexpr (kNh > 0) {
+    // shape compiled in: every load of the thread's share is issued before
+    // the first LDS store, from a clamped (always valid) address, and the
+    // values outside the level are replaced by zeros afterwards (no branch
+    // around a load, so no wait between them)
+    constexpr int kIt = (kNh + BS - 1) / BS;
+    V v[kIt];
+    unsigned char f[kIt];
+    bool inl[kIt], intr[kIt];
+#pragma unroll
+    for (int k = 0; k < kIt; k++) {
+      const int i = k * BS + (int)threadIdx.x;
+      const int q = i / (TY + 2);
+      const int ly = i - q * (TY + 2), lz = q / (TX + 2), lx = q - lz * (TX + 2);
+      const int m2 = y0 + ly - 1, m1 = x0 + lx - 1, m3 = z0 + lz - 1;
+      inl[k] = i < kNh && m1 >= 0 && m1 < g.X && m2 >= 0 && m2 < g.Y && m3 >= 0 && m3 < g.Z;
+      intr[k] = i < kNh && lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && lz >= 1 && lz <= TZ;
+      const unsigned n = inl[k] ? __umul24(m3, g.XY) + __umul24(m1, g.Y) + m2 : 0u;
+      v[k] = reinterpret_cast<const V *>(xold)[n];
+      f[k] = flags[n];
+    }
+#pragma unroll
+    for (int k = 0; k < kIt; k++) {
+      const int i = k * BS + (int)threadIdx.x;
+      if (i >= kNh) break;
+      const V z = {T(0), T(0), T(0), T(0)};
+      const V w = inl[k] ? v[k] : z;
+      sx[i] = w.x; sx[kNh + i] = w.y; sx[2 * kNh + i] = w.z; sx[3 * kNh + i] = w.w;
+      if (intr[k]) {
+        const int q = i / (TY + 2);
+        const int ly = i - q * (TY + 2), lz = q / (TX + 2), lx = q - lz * (TX + 2);
+        sf[((lz - 1) * TX + lx - 1) * TY + ly - 1] = inl[k] ? f[k] : (unsigned char)0;
+      }
+    }
+    return;
+  }
   for (int i = threadIdx.x; i < nh; i += blockDim.x) {
     const int q = (int)tdiv<TY ? TY + 2 : 0>(i, tg.dhy);
     const int ly = i - q * hy, lz = (int)tdiv<TX ? TX + 2 : 0>(q, tg.dhx), lx = q - lz * hx;
     int m2 = y0 + ly - 1, m1 = x0 + lx - 1, m3 = z0 + lz - 1;
-    T v0 = T(0), v1 = T(0), v2 = T(0), v3 = T(0);
+    V v = {T(0), T(0), T(0), T(0)};
     unsigned char f = 0;
     const bool interior = lx >= 1 && lx <= tx && ly >= 1 && ly <= ty && lz >= 1 && lz <= tz;
     if (m1 >= 0 && m1 < g.X && m2 >= 0 && m2 < g.Y && m3 >= 0 && m3 < g.Z) {
       const long n = g.nid(m1, m2, m3);
-      const T *p = xold + 4 * n;
-      v0 = p[0]; v1 = p[1]; v2 = p[2]; v3 = p[3];
+      v = reinterpret_cast<const V *>(xold)[n];
       if (interior) f = flags[n];  // kinds of the halo are not needed
     }
-    sx[i] = v0; sx[nh + i] = v1; sx[2 * nh + i] = v2; sx[3 * nh + i] = v3;
+    sx[i] = v.x; sx[nh + i] = v.y; sx[2 * nh + i] = v.z; sx[3 * nh + i] = v.w;
     if (interior) sf[__umul24(__umul24(lz - 1, tx) + lx - 1, ty) + ly - 1] = f;
   }
 }
@@ -364,7 +421,7 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
     gv.eyz = sv + 3 * nv;
     gv.ed = sv + 4 * nv;
   }
-  tile_stage<T, TY, TX>(g, flags, xold, sx, sf, tg, x0, y0, z0, ty, tx, tz, nh);
+  tile_stage<T, TY, TX, TZ>(g, flags, xold, sx, sf, tg, x0, y0, z0, ty, tx, tz, nh);
   __syncthreads();
   const int nphase = withp ? 7 : 6;
   for (int ph = 0; ph < nphase; ph++) {
@@ -400,11 +457,164 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
     int m1 = x0 + lx, m2 = y0 + ly, m3 = z0 + lz;
     if (m1 >= g.X || m2 >= g.Y || m3 >= g.Z) continue;
     int li = ((lz + 1) * hx + (lx + 1)) * hy + (ly + 1);
-    T *p = xnew + 4 * g.nid(m1, m2, m3);
-    p[0] = sx[li]; p[1] = sx[nh + li]; p[2] = sx[2 * nh + li]; p[3] = sx[3 * nh + li];
+    using V = typename V4<T>::t;
+    reinterpret_cast<V *>(xnew)[g.nid(m1, m2, m3)] = V{sx[li], sx[nh + li], sx[2 * nh + li], sx[3 * nh + li]};
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// The FP32 velocity sweep of the V-cycle (pressure rows left alone) with the
+// rows' data prefetched.  The colour phases give every thread the same pair
+// of nodes in y: node A (ly = 2kk + p) in the red phases and node B
+// (ly = 2kk + 1 - p) in the black ones, for all three components.  So each
+// thread loads, before the tile is staged, everything its six rows read from
+// global memory: the right-hand sides, the 13 stencil viscosities of each
+// node (+ grad-div cells) and the spacing values.  The phases then only read
+// LDS, and the loads' latency overlaps the staging.  The rows are evaluated in
+// flux form (stk_rows.h: the same coefficients, summed as pair differences,
+// about half the arithmetic of building the 17 coefficients).  One node pair
+// per thread: ceil(ty/2) * tx * tz <= BS, the block size.
+template <typename T>
+struct PfNode {
+  bool ok;             // inside the level and below the upper faces (every velocity row's stencil fits)
+  int m2;              // y index (m1, m3 are the pair's)
+  unsigned n;          // node number
+  T b1, b2, b3;        // right-hand sides of vx, vy, vz
+  T ay0, ay1, by0, by1;
+  Vis<T> vx, vy, vz;
+};
+
+template <typename T, bool UNI>
+__device__ __forceinline__ void pf_load(const Geo<T> &g, const T *__restrict__ b, int m1, int m2, int m3, bool in,
+                                        PfNode<T> &d) {
+  d.m2 = m2;
+  d.ok = in && m1 <= g.X - 2 && m2 <= g.Y - 2 && m3 <= g.Z - 2;
+  // loads without a branch: a node that has no velocity row reads node 0's
+  // neighbourhood (inside every level) and its values are never used
+  d.n = d.ok ? __umul24(m3, g.XY) + __umul24(m1, g.Y) + m2 : 0u;
+  const int my = d.ok ? m2 : 1;
+  const float4 bb = reinterpret_cast<const float4 *>(b)[d.n];
+  d.b1 = bb.y; d.b2 = bb.z; d.b3 = bb.w;
+  if (!UNI) { d.ay0 = g.iAy[my]; d.ay1 = g.iAy[my + 1]; d.by0 = g.iBy[my]; d.by1 = g.iBy[my + 1]; }
+  d.vx = vis_load<T, 1>(g, d.n);
+  d.vy = vis_load<T, 2>(g, d.n);
+  d.vz = vis_load<T, 3>(g, d.n);
+}
+
+// S (sum over the row's slots) and D (its diagonal) in flux form
+template <int C>
+__device__ __forceinline__ void pf_flux(const PfNode<float> &d, const Sp<float> &pxz, float p2, const float *xs,
+                                        float &S, float &D) {
+  const Sp<float> p{pxz.ax0, pxz.ax1, pxz.bx0, pxz.bx1, d.ay0, d.ay1, d.by0, d.by1,
+                    pxz.az0, pxz.az1, pxz.bz0, pxz.bz1};
+  if (C == 1) flux_x<float>(p, d.vx, p2, xs, S, D);
+  else if (C == 2) flux_y<float>(p, d.vy, p2, xs, S, D);
+  else flux_z<float>(p, d.vz, p2, xs, S, D);
+}
+
+// UNI: the level's spacing is uniform along each axis; the rows use the
+// level constants fu (FluxU) instead of the spacing tables.
+template <int TY, int TX, int TZ, bool CH, bool UNI, int BS = 256>
+__global__ void __launch_bounds__(BS) k_vsmooth32(Geo<float> g, const unsigned char *__restrict__ flags,
+                                                   const float *__restrict__ xold, float *__restrict__ xnew,
+                                                   const float *__restrict__ b, TileGeo tg, float v0koef,
+                                                   const TileChains<float> *__restrict__ tcs, FluxU fu) {
+  extern __shared__ unsigned char smem_raw[];
+  float *sx = reinterpret_cast<float *>(smem_raw);
+  const int ty = TY ? TY : tg.ty, tx = TX ? TX : tg.tx, tz = TZ ? TZ : tg.tz, ntx = tg.ntx, nty = tg.nty;
+  const int hy = ty + 2, hx = tx + 2, hz = tz + 2;
+  int by, bx, bz;
+  tile_of_block((int)xcd_block(), nty, ntx, tg.ntz, by, bx, bz);
+  const int bt = (bz * ntx + bx) * nty + by;  // the tile's number (chain lists)
+  const int y0 = tg.oy + by * ty, x0 = tg.ox + bx * tx, z0 = tg.oz + bz * tz;
+  const int nh = hy * hx * hz;
+  int q0 = 0, q1 = 0;
+  if (CH) { q0 = tcs->off[bt]; q1 = tcs->off[bt + 1]; }
+  unsigned char *sf = smem_raw + 4 * nh * sizeof(float);
+  // this thread's node pair and everything its rows read from global memory
+  const int hty = (ty + 1) / 2;
+  const int i = threadIdx.x;
+  const bool act = i < hty * tx * tz;
+  const int q = (int)tdiv<(TY + 1) / 2>(i, tg.dhty);
+  const int kk = i - (int)__umul24(q, hty);
+  const int lz = (int)tdiv<TX>(q, tg.dtx), lx = q - (int)__umul24(lz, tx);
+  const int m1 = x0 + lx, m3 = z0 + lz;
+  const int par = (-m1 - m3 - y0) & 1;  // ly of the red node is 2kk + par
+  const int lyA = 2 * kk + par, lyB = 2 * kk + 1 - par;
+  const bool in13 = act && m1 < g.X && m3 < g.Z;
+  Sp<float> pxz{};
+  if (!UNI) {
+    const bool okxz = in13 && m1 <= g.X - 2 && m3 <= g.Z - 2;
+    const int k1 = okxz ? m1 : 1, k3 = okxz ? m3 : 1;
+    pxz.ax0 = g.iAx[k1]; pxz.ax1 = g.iAx[k1 + 1]; pxz.bx0 = g.iBx[k1]; pxz.bx1 = g.iBx[k1 + 1];
+    pxz.az0 = g.iAz[k3]; pxz.az1 = g.iAz[k3 + 1]; pxz.bz0 = g.iBz[k3]; pxz.bz1 = g.iBz[k3 + 1];
+  }
+  PfNode<float> A, B;
+  pf_load<float, UNI>(g, b, m1, y0 + lyA, m3, in13 && lyA < ty && y0 + lyA < g.Y, A);
+  pf_load<float, UNI>(g, b, m1, y0 + lyB, m3, in13 && lyB < ty && y0 + lyB < g.Y, B);
+  tile_stage<float, TY, TX, TZ, BS>(g, flags, xold, sx, sf, tg, x0, y0, z0, ty, tx, tz, nh);
+  __syncthreads();
+  const int sbase = (int)__umul24(__umul24(lz, tx) + lx, ty);
+  const int lbase = (int)__umul24(__umul24(lz + 1, hx) + lx + 1, hy) + 1;
+  auto phase = [&](auto cc, const PfNode<float> &d, int ly) {
+    constexpr int C = decltype(cc)::value;
+    if (!d.ok || kind_of(sf[sbase + ly], C) != KIND_EQ) return;
+    const int base = lbase + ly;
+    float xs[18];
+    xs[0] = 0.f;
+#pragma unroll
+    for (int k = 1; k <= Slot<C>::ns; k++) {
+      const int li = base + (Slot<C>::d[k][2] * hx + Slot<C>::d[k][0]) * hy + Slot<C>::d[k][1];
+      xs[k] = sx[Slot<C>::d[k][3] * nh + li];
+    }
+    float S, D;
+    if (UNI) {
+      const Vis<float> &w = C == 1 ? d.vx : (C == 2 ? d.vy : d.vz);
+      flux_u<C>(w, fu.g[C - 1], g.p2 != 0.f, xs, S, D);
+    } else {
+      pf_flux<C>(d, pxz, g.p2, xs, S, D);
+    }
+    const float ival = S - (C == 1 ? d.b1 : (C == 2 ? d.b2 : d.b3));
+    sx[C * nh + base] = xs[1] - diag_step(ival, D) * v0koef;
+  };
+  phase(std::integral_constant<int, 1>{}, A, lyA);
+  __syncthreads();
+  phase(std::integral_constant<int, 1>{}, B, lyB);
+  __syncthreads();
+  phase(std::integral_constant<int, 2>{}, A, lyA);
+  __syncthreads();
+  phase(std::integral_constant<int, 2>{}, B, lyB);
+  __syncthreads();
+  phase(std::integral_constant<int, 3>{}, A, lyA);
+  __syncthreads();
+  phase(std::integral_constant<int, 3>{}, B, lyB);
+  __syncthreads();
+  if (CH) {
+    const int *__restrict__ rl = tcs->rl;
+    const int *__restrict__ pl = tcs->pl;
+    const float *__restrict__ cc = tcs->cc;
+    const float *__restrict__ ck = tcs->ck;
+    for (int qq = q0 + threadIdx.x; qq < q1; qq += blockDim.x) {
+      const int r = rl[qq], j = pl[qq];
+      const float K = ck[qq];
+      if (K != K) continue;  // a pressure chain: velocity-only sweep
+      const float v = j >= 0 ? cc[qq] + K * sx[j] : cc[qq];
+      if (r >= 0) sx[r] = v;
+      else xnew[tcs->head[qq]] = v;
+    }
+    __syncthreads();
+  }
+  const int nt = ty * tx * tz;
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+    const int qt = (int)tdiv<TY>(t, tg.dty);
+    const int ly = t - qt * ty, lzt = (int)tdiv<TX>(qt, tg.dtx), lxt = qt - lzt * tx;
+    const int n1 = x0 + lxt, n2 = y0 + ly, n3 = z0 + lzt;
+    if (n1 >= g.X || n2 >= g.Y || n3 >= g.Z) continue;
+    const int li = ((lzt + 1) * hx + (lxt + 1)) * hy + (ly + 1);
+    reinterpret_cast<float4 *>(xnew)[g.nid(n1, n2, n3)] = float4{sx[li], sx[nh + li], sx[2 * nh + li], sx[3 * nh + li]};
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Row operator on equation rows: RES -> y = b - A x (gausmat4's residual
@@ -424,11 +634,11 @@ __global__ void __launch_bounds__(256) k_rowop_tile(Geo<T> g, const unsigned cha
   T *sx = reinterpret_cast<T *>(smem_raw);
   const int ty = TY ? TY : tg.ty, tx = TX ? TX : tg.tx, tz = TZ ? TZ : tg.tz;
   const int hy = ty + 2, hx = tx + 2, nh = hy * hx * (tz + 2);
-  const int bt = (int)xcd_block();
-  const int by = bt % tg.nty, bx = (bt / tg.nty) % tg.ntx, bz = bt / (tg.nty * tg.ntx);
+  int by, bx, bz;
+  tile_of_block((int)xcd_block(), tg.nty, tg.ntx, tg.ntz, by, bx, bz);
   const int y0 = tg.oy + by * ty, x0 = tg.ox + bx * tx, z0 = tg.oz + bz * tz;
   unsigned char *sf = smem_raw + 4 * nh * sizeof(T);
-  tile_stage<T, TY, TX>(g, flags, x, sx, sf, tg, x0, y0, z0, ty, tx, tz, nh);
+  tile_stage<T, TY, TX, TZ>(g, flags, x, sx, sf, tg, x0, y0, z0, ty, tx, tz, nh);
   __syncthreads();
   const int nt = ty * tx * tz;
   for (int i = threadIdx.x; i < nt; i += blockDim.x) {

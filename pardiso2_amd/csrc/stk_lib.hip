@@ -142,6 +142,11 @@ struct Level {
   int *klo[3] = {}, *kcnt[3] = {};  // cells of level-1 owned by each cell of this level, per axis
   int *kpar[3] = {};                // cell of level+1 owning each cell of this level, per axis
   template <typename T> Geo<T> geo(double p2) const;
+  // spacing uniform along each axis (FP32 tables): the FP32 velocity sweep
+  // then uses the level constants fu (set by stk_prepare for the knobs' p2koef)
+  bool uni = false;
+  float ua[3] = {0, 0, 0}, ub[3] = {0, 0, 0};  // the uniform iA, iB per axis (FP32)
+  FluxU fu{};
 };
 
 template <>
@@ -224,6 +229,14 @@ struct stk_ctx {
 namespace {
 
 stk_ctx *g_bound = nullptr;
+
+// the level whose FP32 stencil viscosities a Geo names
+const Level *level_of(const stk_ctx *c, const Geo<float> &g) {
+  for (int l = 0; l <= c->L; l++)
+    if (c->lv[l].e32[0] == g.ec) return &c->lv[l];
+  return nullptr;
+}
+template <typename T> const Level *level_of(const stk_ctx *, const Geo<T> &) { return nullptr; }
 
 // ---------------------------------------------------------------------------
 // index rules (parallel/move3mg.c:3563-3576 etc.) and weights
@@ -449,6 +462,35 @@ void launch_smooth(stk_ctx *c, const int *tile, bool sv, const TileGeo &tg, cons
   const size_t lds = smooth_lds<T>(tile, sv);
   const T v0 = (T)c->kn.v0koef, p0 = (T)c->kn.p0koef;
   const bool t344 = tile[0] == 32 && tile[1] == 4 && tile[2] == 4;
+  if constexpr (std::is_same<T, float>::value) {
+    // the V-cycle's velocity sweep: rows' data prefetched per node pair
+    // (one pair per thread: 256 threads, or 512 for the deeper tiles)
+    const long pairs = (long)((tile[0] + 1) / 2) * tile[1] * tile[2];
+    if (!withp && !sv && pairs <= 512) {
+      const Level *L = level_of(c, g);
+      const bool uni = L && L->uni;
+      const FluxU fu = uni ? L->fu : FluxU{};
+      const bool t348 = tile[0] == 32 && tile[1] == 4 && tile[2] == 8;
+      const bool t384 = tile[0] == 32 && tile[1] == 8 && tile[2] == 4;
+      const dim3 blk2(pairs <= 256 ? 256 : 512);
+#define STK_VSMOOTH(TY, TX, TZ, CHV, BS)                                                                      \
+  do {                                                                                                        \
+    if (uni)                                                                                                  \
+      hipLaunchKernelGGL((k_vsmooth32<TY, TX, TZ, CHV, true, BS>), grid, blk2, lds, c->st, g, flags, xo, xn, b, \
+                         tg, v0, tcs, fu);                                                                    \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_vsmooth32<TY, TX, TZ, CHV, false, BS>), grid, blk2, lds, c->st, g, flags, xo, xn,  \
+                         b, tg, v0, tcs, fu);                                                                 \
+  } while (0)
+      if (t344) { if (tcs) STK_VSMOOTH(32, 4, 4, true, 256); else STK_VSMOOTH(32, 4, 4, false, 256); }
+      else if (t348) { if (tcs) STK_VSMOOTH(32, 4, 8, true, 512); else STK_VSMOOTH(32, 4, 8, false, 512); }
+      else if (t384) { if (tcs) STK_VSMOOTH(32, 8, 4, true, 512); else STK_VSMOOTH(32, 8, 4, false, 512); }
+      else if (pairs <= 256) { if (tcs) STK_VSMOOTH(0, 0, 0, true, 256); else STK_VSMOOTH(0, 0, 0, false, 256); }
+      else { if (tcs) STK_VSMOOTH(0, 0, 0, true, 512); else STK_VSMOOTH(0, 0, 0, false, 512); }
+#undef STK_VSMOOTH
+      return;
+    }
+  }
 #define STK_SMOOTH(...) \
   hipLaunchKernelGGL((k_smooth<__VA_ARGS__>), grid, blk, lds, c->st, g, flags, xo, xn, b, tg, withp, v0, p0, tcs)
   if (sv && tcs) STK_SMOOTH(T, true, 0, 0, 0, true);
@@ -1330,6 +1372,12 @@ int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *
       for (int i = 1; i < nl; i++) iA[i] = 1.0 / (L.g[a][i] - L.g[a][i - 1]);
       for (int i = 1; i + 1 < nl; i++) iB[i] = 1.0 / (L.g[a][i + 1] - L.g[a][i - 1]);
       std::vector<float> fA(iA.begin(), iA.end()), fB(iB.begin(), iB.end());
+      // uniform along this axis: one iA over 1..nl-1 and one iB over 1..nl-2
+      bool u = nl >= 3;
+      for (int i = 2; u && i < nl; i++) u = fA[i] == fA[1];
+      for (int i = 2; u && i + 1 < nl; i++) u = fB[i] == fB[1];
+      L.ua[a] = u ? fA[1] : 0.f;
+      L.ub[a] = u ? fB[1] : 0.f;
       if (upload(&L.iA64[a], iA) || upload(&L.iB64[a], iB) || upload(&L.iA32[a], fA) || upload(&L.iB32[a], fB) ||
           upload(&L.g64[a], L.g[a])) {
         stk_destroy(c);
@@ -1895,6 +1943,8 @@ int stk_prepare_cvm(stk_ctx *c, int cvm) {
                        l == 0 ? c->kn.viscmod : vmc,
                        L.e64[0], L.e64[1], L.e64[2], L.e64[3], L.e64[4], L.e32[0], L.e32[1], L.e32[2], L.e32[3],
                        L.e32[4]);
+    L.uni = L.ua[0] > 0 && L.ua[1] > 0 && L.ua[2] > 0;
+    if (L.uni) L.fu = flux_u_consts(L.ua[0], L.ub[0], L.ua[1], L.ub[1], L.ua[2], L.ub[2], c->kn.p2koef);
     // nd = cell viscosity of the continuity stencil (continerr, move3mg.c:1549)
     hipLaunchKernelGGL((k_convert<float, float>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, L.n, L.e32[0],
                        c->nd + L.off);
@@ -2465,6 +2515,19 @@ int stk_level_tile(const stk_ctx *c, int level, int precision, int out4[4]) {
   const Level &L = c->lv[level];
   const int *t = precision ? tuned_tile<double>(L) : tuned_tile<float>(L);
   for (int a = 0; a < 4; a++) out4[a] = t[a];
+  return STK_OK;
+}
+
+int stk_set_level_tile(stk_ctx *c, int level, int precision, const int tile4[4]) {
+  if (!c || level < 0 || level > c->L || !tile4 || precision < 0 || precision > 1) return STK_EINVAL;
+  const int ty = tile4[0], tx = tile4[1], tz = tile4[2], sv = tile4[3] ? 1 : 0;
+  if (ty < 2 || tx < 1 || tz < 1) return STK_EINVAL;
+  const int t[4] = {ty, tx, tz, sv};
+  const size_t lds = precision ? smooth_lds<double>(t, sv) : smooth_lds<float>(t, sv);
+  if (lds > kLevelLds) return STK_EINVAL;
+  c->graph_ok = false;  // captured launches use the tiles of the time of capture
+  int *d = precision ? c->lv[level].tune64 : c->lv[level].tune32;
+  for (int a = 0; a < 4; a++) d[a] = t[a];
   return STK_OK;
 }
 

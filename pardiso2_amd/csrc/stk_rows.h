@@ -45,10 +45,10 @@ __constant__ signed char kSlot[4][18][4] = {
 
 // Multiply-accumulate: FP32 rows fuse (one rounding), FP64 rows round the
 // product and the sum separately, as the reference's C does.
-__device__ __forceinline__ float mac(float acc, float a, float b) { return __builtin_fmaf(a, b, acc); }
-__device__ __forceinline__ double mac(double acc, double a, double b) { return acc + a * b; }
-__device__ __forceinline__ float msub(float acc, float a, float b) { return __builtin_fmaf(-a, b, acc); }
-__device__ __forceinline__ double msub(double acc, double a, double b) { return acc - a * b; }
+__host__ __device__ __forceinline__ float mac(float acc, float a, float b) { return __builtin_fmaf(a, b, acc); }
+__host__ __device__ __forceinline__ double mac(double acc, double a, double b) { return acc + a * b; }
+__host__ __device__ __forceinline__ float msub(float acc, float a, float b) { return __builtin_fmaf(-a, b, acc); }
+__host__ __device__ __forceinline__ double msub(double acc, double a, double b) { return acc - a * b; }
 
 // Per-level geometry and stencil viscosities in arithmetic type T.
 template <typename T>
@@ -65,7 +65,7 @@ struct Geo {
 // Normal-stress contributions of a cell (sxxcalc/syycalc/szzcalc, coefficient
 // mode): a[0..5] = vx v0, vx v2, vy v0, vy v1, vz v0, vz v4 with weight w.
 template <typename T, int AX>
-__device__ __forceinline__ void snn(T w, T ne, T xk, T yk, T zk, T a[6]) {
+__host__ __device__ __forceinline__ void snn(T w, T ne, T xk, T yk, T zk, T a[6]) {
   if constexpr (std::is_same<T, float>::value) {
     // FP32 rows (the mixed-precision smoother): 4/3 and 2/3 as constants, no divisions
     const float wn = w * ne, f4 = wn * (4.0f / 3.0f), f2 = wn * (2.0f / 3.0f);
@@ -94,111 +94,351 @@ __device__ __forceinline__ void snn(T w, T ne, T xk, T yk, T zk, T a[6]) {
 // Shear-stress contributions (sxycalc/sxzcalc/syzcalc): a[0..3] =
 // first comp lower, first comp upper, second comp lower, second comp upper.
 template <typename T>
-__device__ __forceinline__ void sshear(T w, T ne, T k1, T k2, T a[4]) {
+__host__ __device__ __forceinline__ void sshear(T w, T ne, T k1, T k2, T a[4]) {
   a[0] = -w * ne * T(2) * k1; a[1] = +w * ne * T(2) * k1;
   a[2] = -w * ne * T(2) * k2; a[3] = +w * ne * T(2) * k2;
 }
 
 // grad-div contribution (continadd): same 6 positions as snn.
 template <typename T>
-__device__ __forceinline__ void cadd(T w, T ne, T xk, T yk, T zk, T a[6]) {
+__host__ __device__ __forceinline__ void cadd(T w, T ne, T xk, T yk, T zk, T a[6]) {
   a[0] = -xk * w * ne; a[1] = +xk * w * ne;
   a[2] = -yk * w * ne; a[3] = +yk * w * ne;
   a[4] = -zk * w * ne; a[5] = +zk * w * ne;
 }
 
-// Row builders: (m1,m2,m3) index the spacing tables; v is the node's index
-// in the viscosity arrays (strides g.Y, g.XY), from which every viscosity the
-// row reads lies at offset 0 or +1 per axis.
-// x-momentum row at vx(m1,m2,m3); s[1..17]
+// Spacing values a node's velocity rows read: iA and iB at the node's own
+// index and the next one along each axis (ax0 = iAx[m1], ax1 = iAx[m1+1],
+// bx0 = iBx[m1], bx1 = iBx[m1+1], and likewise in y and z).
 template <typename T>
-__device__ __forceinline__ void row_x(const Geo<T> &g, int m1, int m2, int m3, long v, T s[18]) {
+struct Sp {
+  T ax0, ax1, bx0, bx1, ay0, ay1, by0, by1, az0, az1, bz0, bz1;
+};
+template <typename T>
+__device__ __forceinline__ Sp<T> sp_load(const Geo<T> &g, int m1, int m2, int m3) {
+  return Sp<T>{g.iAx[m1], g.iAx[m1 + 1], g.iBx[m1], g.iBx[m1 + 1], g.iAy[m2], g.iAy[m2 + 1],
+               g.iBy[m2], g.iBy[m2 + 1], g.iAz[m3], g.iAz[m3 + 1], g.iBz[m3], g.iBz[m3 + 1]};
+}
+
+// The viscosities one momentum row reads: the two normal-stress cells (c),
+// the two shear edges along the first (s) and second (t) transverse axis,
+// and the two grad-div cells (d).  Every one lies at offset 0 or +1 per axis
+// from the row's node v (strides g.Y, g.XY).
+template <typename T>
+struct Vis {
+  T c0, c1, s0, s1, t0, t1, d0, d1;
+};
+template <typename T, int C>
+__device__ __forceinline__ Vis<T> vis_load(const Geo<T> &g, long v) {
   const long Y = g.Y, XY = g.XY;
-  const T xkf = T(2) * g.iBx[m1];
-  const T ykf = g.iAy[m2 + 1];
-  const T zkf = g.iAz[m3 + 1];
+  Vis<T> e;
+  // without a grad-div term the d values are not used: they are read from
+  // the cell plane at the c offsets (the same addresses, no extra traffic)
+  // so that no load sits behind a branch
+  const T *ed = g.p2 != T(0) ? g.ed : g.ec;
+  if (C == 1) {
+    e.c0 = g.ec[v + (XY + 1)]; e.c1 = g.ec[v + (XY + Y + 1)];
+    e.s0 = g.exy[v]; e.s1 = g.exy[v + 1]; e.t0 = g.exz[v]; e.t1 = g.exz[v + XY];
+    e.d0 = ed[v + (XY + 1)]; e.d1 = ed[v + (XY + Y + 1)];
+  } else if (C == 2) {
+    e.c0 = g.ec[v + (XY + Y)]; e.c1 = g.ec[v + (XY + Y + 1)];
+    e.s0 = g.exy[v]; e.s1 = g.exy[v + Y]; e.t0 = g.eyz[v]; e.t1 = g.eyz[v + XY];
+    e.d0 = ed[v + (XY + Y)]; e.d1 = ed[v + (XY + Y + 1)];
+  } else {
+    e.c0 = g.ec[v + (Y + 1)]; e.c1 = g.ec[v + (XY + Y + 1)];
+    e.s0 = g.exz[v]; e.s1 = g.exz[v + Y]; e.t0 = g.eyz[v]; e.t1 = g.eyz[v + 1];
+    e.d0 = ed[v + (Y + 1)]; e.d1 = ed[v + (XY + Y + 1)];
+  }
+  return e;
+}
+
+// Row builders from the row's spacing values and viscosities; s[1..17].
+// x-momentum row at vx(m1,m2,m3)
+template <typename T>
+__host__ __device__ __forceinline__ void row_x_v(const Sp<T> &p, const Vis<T> &w, T p2, T s[18]) {
+  const T xkf = T(2) * p.bx0;
+  const T ykf = p.ay1;
+  const T zkf = p.az1;
   T a[6], e[4];
   s[16] = +xkf; s[17] = -xkf;
-  snn<T, 0>(-xkf, g.ec[v + (XY + 1)], g.iAx[m1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+  snn<T, 0>(-xkf, w.c0, p.ax0, p.ay1, p.az1, a);
   s[2] = a[0]; s[1] = a[1]; s[8] = a[2]; s[9] = a[3]; s[12] = a[4]; s[13] = a[5];
-  snn<T, 0>(xkf, g.ec[v + (XY + Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+  snn<T, 0>(xkf, w.c1, p.ax1, p.ay1, p.az1, a);
   s[1] += a[0]; s[3] = a[1]; s[10] = a[2]; s[11] = a[3]; s[14] = a[4]; s[15] = a[5];
-  sshear<T>(-ykf, g.exy[v], g.iBy[m2], g.iBx[m1], e);
+  sshear<T>(-ykf, w.s0, p.by0, p.bx0, e);
   s[1] += e[1]; s[4] = e[0]; s[8] += e[2]; s[10] += e[3];
-  sshear<T>(ykf, g.exy[v + 1], g.iBy[m2 + 1], g.iBx[m1], e);
+  sshear<T>(ykf, w.s1, p.by1, p.bx0, e);
   s[1] += e[0]; s[5] = e[1]; s[9] += e[2]; s[11] += e[3];
-  sshear<T>(-zkf, g.exz[v], g.iBz[m3], g.iBx[m1], e);
+  sshear<T>(-zkf, w.t0, p.bz0, p.bx0, e);
   s[1] += e[1]; s[6] = e[0]; s[12] += e[2]; s[14] += e[3];
-  sshear<T>(zkf, g.exz[v + XY], g.iBz[m3 + 1], g.iBx[m1], e);
+  sshear<T>(zkf, w.t1, p.bz1, p.bx0, e);
   s[1] += e[0]; s[7] = e[1]; s[13] += e[2]; s[15] += e[3];
-  if (g.p2 != T(0)) {
-    cadd<T>(-xkf * g.p2, g.ed[v + (XY + 1)], g.iAx[m1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+  if (p2 != T(0)) {
+    cadd<T>(-xkf * p2, w.d0, p.ax0, p.ay1, p.az1, a);
     s[2] += a[0]; s[1] += a[1]; s[8] += a[2]; s[9] += a[3]; s[12] += a[4]; s[13] += a[5];
-    cadd<T>(xkf * g.p2, g.ed[v + (XY + Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+    cadd<T>(xkf * p2, w.d1, p.ax1, p.ay1, p.az1, a);
     s[1] += a[0]; s[3] += a[1]; s[10] += a[2]; s[11] += a[3]; s[14] += a[4]; s[15] += a[5];
   }
 }
 
 // y-momentum row at vy(m1,m2,m3)
 template <typename T>
-__device__ __forceinline__ void row_y(const Geo<T> &g, int m1, int m2, int m3, long v, T s[18]) {
-  const long Y = g.Y, XY = g.XY;
-  const T xkf = g.iAx[m1 + 1];
-  const T ykf = T(2) * g.iBy[m2];
-  const T zkf = g.iAz[m3 + 1];
+__host__ __device__ __forceinline__ void row_y_v(const Sp<T> &p, const Vis<T> &w, T p2, T s[18]) {
+  const T xkf = p.ax1;
+  const T ykf = T(2) * p.by0;
+  const T zkf = p.az1;
   T a[6], e[4];
   s[16] = +ykf; s[17] = -ykf;
-  snn<T, 1>(-ykf, g.ec[v + (XY + Y)], g.iAx[m1 + 1], g.iAy[m2], g.iAz[m3 + 1], a);
+  snn<T, 1>(-ykf, w.c0, p.ax1, p.ay0, p.az1, a);
   s[8] = a[0]; s[10] = a[1]; s[4] = a[2]; s[1] = a[3]; s[12] = a[4]; s[14] = a[5];
-  snn<T, 1>(ykf, g.ec[v + (XY + Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+  snn<T, 1>(ykf, w.c1, p.ax1, p.ay1, p.az1, a);
   s[9] = a[0]; s[11] = a[1]; s[1] += a[2]; s[5] = a[3]; s[13] = a[4]; s[15] = a[5];
   // sxy(m1,m2,m3,-xkf): vx lower/upper along y, vy lower/upper along x
-  sshear<T>(-xkf, g.exy[v], g.iBy[m2], g.iBx[m1], e);
+  sshear<T>(-xkf, w.s0, p.by0, p.bx0, e);
   s[8] += e[0]; s[9] += e[1]; s[2] = e[2]; s[1] += e[3];
-  sshear<T>(xkf, g.exy[v + Y], g.iBy[m2], g.iBx[m1 + 1], e);
+  sshear<T>(xkf, w.s1, p.by0, p.bx1, e);
   s[10] += e[0]; s[11] += e[1]; s[1] += e[2]; s[3] = e[3];
   // syz(m1,m2,m3,-zkf): vy lower/upper along z, vz lower/upper along y
-  sshear<T>(-zkf, g.eyz[v], g.iBz[m3], g.iBy[m2], e);
+  sshear<T>(-zkf, w.t0, p.bz0, p.by0, e);
   s[6] = e[0]; s[1] += e[1]; s[12] += e[2]; s[13] += e[3];
-  sshear<T>(zkf, g.eyz[v + XY], g.iBz[m3 + 1], g.iBy[m2], e);
+  sshear<T>(zkf, w.t1, p.bz1, p.by0, e);
   s[1] += e[0]; s[7] = e[1]; s[14] += e[2]; s[15] += e[3];
-  if (g.p2 != T(0)) {
-    cadd<T>(-ykf * g.p2, g.ed[v + (XY + Y)], g.iAx[m1 + 1], g.iAy[m2], g.iAz[m3 + 1], a);
+  if (p2 != T(0)) {
+    cadd<T>(-ykf * p2, w.d0, p.ax1, p.ay0, p.az1, a);
     s[8] += a[0]; s[10] += a[1]; s[4] += a[2]; s[1] += a[3]; s[12] += a[4]; s[14] += a[5];
-    cadd<T>(ykf * g.p2, g.ed[v + (XY + Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+    cadd<T>(ykf * p2, w.d1, p.ax1, p.ay1, p.az1, a);
     s[9] += a[0]; s[11] += a[1]; s[1] += a[2]; s[5] += a[3]; s[13] += a[4]; s[15] += a[5];
   }
 }
 
 // z-momentum row at vz(m1,m2,m3)
 template <typename T>
-__device__ __forceinline__ void row_z(const Geo<T> &g, int m1, int m2, int m3, long v, T s[18]) {
-  const long Y = g.Y, XY = g.XY;
-  const T xkf = g.iAx[m1 + 1];
-  const T ykf = g.iAy[m2 + 1];
-  const T zkf = T(2) * g.iBz[m3];
+__host__ __device__ __forceinline__ void row_z_v(const Sp<T> &p, const Vis<T> &w, T p2, T s[18]) {
+  const T xkf = p.ax1;
+  const T ykf = p.ay1;
+  const T zkf = T(2) * p.bz0;
   T a[6], e[4];
   s[16] = +zkf; s[17] = -zkf;
-  snn<T, 2>(-zkf, g.ec[v + (Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3], a);
+  snn<T, 2>(-zkf, w.c0, p.ax1, p.ay1, p.az0, a);
   s[12] = a[0]; s[14] = a[1]; s[8] = a[2]; s[9] = a[3]; s[6] = a[4]; s[1] = a[5];
-  snn<T, 2>(zkf, g.ec[v + (XY + Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+  snn<T, 2>(zkf, w.c1, p.ax1, p.ay1, p.az1, a);
   s[13] = a[0]; s[15] = a[1]; s[10] = a[2]; s[11] = a[3]; s[1] += a[4]; s[7] = a[5];
   // sxz(m1,m2,m3,-xkf): vx lower/upper along z, vz lower/upper along x
-  sshear<T>(-xkf, g.exz[v], g.iBz[m3], g.iBx[m1], e);
+  sshear<T>(-xkf, w.s0, p.bz0, p.bx0, e);
   s[12] += e[0]; s[13] += e[1]; s[2] = e[2]; s[1] += e[3];
-  sshear<T>(xkf, g.exz[v + Y], g.iBz[m3], g.iBx[m1 + 1], e);
+  sshear<T>(xkf, w.s1, p.bz0, p.bx1, e);
   s[14] += e[0]; s[15] += e[1]; s[1] += e[2]; s[3] = e[3];
   // syz(m1,m2,m3,-ykf): vy lower/upper along z, vz lower/upper along y
-  sshear<T>(-ykf, g.eyz[v], g.iBz[m3], g.iBy[m2], e);
+  sshear<T>(-ykf, w.t0, p.bz0, p.by0, e);
   s[8] += e[0]; s[10] += e[1]; s[4] = e[2]; s[1] += e[3];
-  sshear<T>(ykf, g.eyz[v + 1], g.iBz[m3], g.iBy[m2 + 1], e);
+  sshear<T>(ykf, w.t1, p.bz0, p.by1, e);
   s[9] += e[0]; s[11] += e[1]; s[1] += e[2]; s[5] = e[3];
-  if (g.p2 != T(0)) {
-    cadd<T>(-zkf * g.p2, g.ed[v + (Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3], a);
+  if (p2 != T(0)) {
+    cadd<T>(-zkf * p2, w.d0, p.ax1, p.ay1, p.az0, a);
     s[12] += a[0]; s[14] += a[1]; s[8] += a[2]; s[9] += a[3]; s[6] += a[4]; s[1] += a[5];
-    cadd<T>(zkf * g.p2, g.ed[v + (XY + Y + 1)], g.iAx[m1 + 1], g.iAy[m2 + 1], g.iAz[m3 + 1], a);
+    cadd<T>(zkf * p2, w.d1, p.ax1, p.ay1, p.az1, a);
     s[13] += a[0]; s[15] += a[1]; s[10] += a[2]; s[11] += a[3]; s[1] += a[4]; s[7] += a[5];
   }
+}
+
+// ---------------------------------------------------------------------------
+// Flux form of the same velocity rows (the FP32 smoother's).  Every stencil
+// contribution above writes its coefficients in +/- pairs: snn's a[1] = -a[0],
+// a[3] = -a[2], a[5] = -a[4]; sshear's e[1] = -e[0], e[3] = -e[2]; cadd's the
+// same; the pressure pair s[16] = -s[17].  So the row's sum over slots is
+//   S = sum_q s[q] x[q] = sum over pairs (a, b) of c (x[a] - x[b]),
+// and its diagonal s[1] is the sum of the pair coefficients that touch slot 1
+// (+c as first member, -c as second).  Each pair below names the slots the
+// coefficient form assigns (row_x_v / row_y_v / row_z_v), so a sweep can use
+// S and D = s[1] without building the 17 coefficients: about half the
+// arithmetic.  The sums round differently from the coefficient form.
+// xs[q] = x at slot q (q = 1..17).
+template <typename T>
+__host__ __device__ __forceinline__ T fmat(T a, T b, T c) {
+  if constexpr (std::is_same<T, float>::value) return __builtin_fmaf(a, b, c);
+  return a * b + c;
+}
+template <typename T, int A, int B>
+__host__ __device__ __forceinline__ void fpair(T c, const T *xs, T &S, T &D) {
+  S = fmat<T>(c, xs[A] - xs[B], S);
+  if (A == 1) D += c;
+  if (B == 1) D -= c;
+}
+template <typename T, int AX, int A0, int A1, int A2, int A3, int A4, int A5>
+__host__ __device__ __forceinline__ void fsnn(T w, T ne, T xk, T yk, T zk, const T *xs, T &S, T &D) {
+  const T wn = w * ne, f4 = wn * T(4.0 / 3.0), f2 = wn * T(2.0 / 3.0);
+  const T p = AX == 0 ? f4 : f2, q = AX == 1 ? f4 : f2, r = AX == 2 ? f4 : f2;
+  fpair<T, A0, A1>((AX == 0 ? -p : p) * xk, xs, S, D);
+  fpair<T, A2, A3>((AX == 1 ? -q : q) * yk, xs, S, D);
+  fpair<T, A4, A5>((AX == 2 ? -r : r) * zk, xs, S, D);
+}
+template <typename T, int A0, int A1, int A2, int A3>
+__host__ __device__ __forceinline__ void fshear(T w, T ne, T k1, T k2, const T *xs, T &S, T &D) {
+  const T t = -(w * ne) * T(2);
+  fpair<T, A0, A1>(t * k1, xs, S, D);
+  fpair<T, A2, A3>(t * k2, xs, S, D);
+}
+template <typename T, int A0, int A1, int A2, int A3, int A4, int A5>
+__host__ __device__ __forceinline__ void fcadd(T w, T ne, T xk, T yk, T zk, const T *xs, T &S, T &D) {
+  const T t = -(w * ne);
+  fpair<T, A0, A1>(t * xk, xs, S, D);
+  fpair<T, A2, A3>(t * yk, xs, S, D);
+  fpair<T, A4, A5>(t * zk, xs, S, D);
+}
+
+// S and D of the x-momentum row (pairs of row_x_v)
+template <typename T>
+__host__ __device__ __forceinline__ void flux_x(const Sp<T> &p, const Vis<T> &w, T p2, const T *xs, T &S, T &D) {
+  const T xkf = T(2) * p.bx0, ykf = p.ay1, zkf = p.az1;
+  S = T(0); D = T(0);
+  fsnn<T, 0, 2, 1, 8, 9, 12, 13>(-xkf, w.c0, p.ax0, p.ay1, p.az1, xs, S, D);
+  fsnn<T, 0, 1, 3, 10, 11, 14, 15>(xkf, w.c1, p.ax1, p.ay1, p.az1, xs, S, D);
+  fshear<T, 4, 1, 8, 10>(-ykf, w.s0, p.by0, p.bx0, xs, S, D);
+  fshear<T, 1, 5, 9, 11>(ykf, w.s1, p.by1, p.bx0, xs, S, D);
+  fshear<T, 6, 1, 12, 14>(-zkf, w.t0, p.bz0, p.bx0, xs, S, D);
+  fshear<T, 1, 7, 13, 15>(zkf, w.t1, p.bz1, p.bx0, xs, S, D);
+  if (p2 != T(0)) {
+    fcadd<T, 2, 1, 8, 9, 12, 13>(-xkf * p2, w.d0, p.ax0, p.ay1, p.az1, xs, S, D);
+    fcadd<T, 1, 3, 10, 11, 14, 15>(xkf * p2, w.d1, p.ax1, p.ay1, p.az1, xs, S, D);
+  }
+  fpair<T, 16, 17>(xkf, xs, S, D);
+}
+// S and D of the y-momentum row (pairs of row_y_v)
+template <typename T>
+__host__ __device__ __forceinline__ void flux_y(const Sp<T> &p, const Vis<T> &w, T p2, const T *xs, T &S, T &D) {
+  const T xkf = p.ax1, ykf = T(2) * p.by0, zkf = p.az1;
+  S = T(0); D = T(0);
+  fsnn<T, 1, 8, 10, 4, 1, 12, 14>(-ykf, w.c0, p.ax1, p.ay0, p.az1, xs, S, D);
+  fsnn<T, 1, 9, 11, 1, 5, 13, 15>(ykf, w.c1, p.ax1, p.ay1, p.az1, xs, S, D);
+  fshear<T, 8, 9, 2, 1>(-xkf, w.s0, p.by0, p.bx0, xs, S, D);
+  fshear<T, 10, 11, 1, 3>(xkf, w.s1, p.by0, p.bx1, xs, S, D);
+  fshear<T, 6, 1, 12, 13>(-zkf, w.t0, p.bz0, p.by0, xs, S, D);
+  fshear<T, 1, 7, 14, 15>(zkf, w.t1, p.bz1, p.by0, xs, S, D);
+  if (p2 != T(0)) {
+    fcadd<T, 8, 10, 4, 1, 12, 14>(-ykf * p2, w.d0, p.ax1, p.ay0, p.az1, xs, S, D);
+    fcadd<T, 9, 11, 1, 5, 13, 15>(ykf * p2, w.d1, p.ax1, p.ay1, p.az1, xs, S, D);
+  }
+  fpair<T, 16, 17>(ykf, xs, S, D);
+}
+// S and D of the z-momentum row (pairs of row_z_v)
+template <typename T>
+__host__ __device__ __forceinline__ void flux_z(const Sp<T> &p, const Vis<T> &w, T p2, const T *xs, T &S, T &D) {
+  const T xkf = p.ax1, ykf = p.ay1, zkf = T(2) * p.bz0;
+  S = T(0); D = T(0);
+  fsnn<T, 2, 12, 14, 8, 9, 6, 1>(-zkf, w.c0, p.ax1, p.ay1, p.az0, xs, S, D);
+  fsnn<T, 2, 13, 15, 10, 11, 1, 7>(zkf, w.c1, p.ax1, p.ay1, p.az1, xs, S, D);
+  fshear<T, 12, 13, 2, 1>(-xkf, w.s0, p.bz0, p.bx0, xs, S, D);
+  fshear<T, 14, 15, 1, 3>(xkf, w.s1, p.bz0, p.bx1, xs, S, D);
+  fshear<T, 8, 10, 4, 1>(-ykf, w.t0, p.bz0, p.by0, xs, S, D);
+  fshear<T, 9, 11, 1, 5>(ykf, w.t1, p.bz0, p.by1, xs, S, D);
+  if (p2 != T(0)) {
+    fcadd<T, 12, 14, 8, 9, 6, 1>(-zkf * p2, w.d0, p.ax1, p.ay1, p.az0, xs, S, D);
+    fcadd<T, 13, 15, 10, 11, 1, 7>(zkf * p2, w.d1, p.ax1, p.ay1, p.az1, xs, S, D);
+  }
+  fpair<T, 16, 17>(zkf, xs, S, D);
+}
+
+// On a level whose spacing is uniform along each axis, a row's pair
+// coefficients are a viscosity times a constant of the level: the
+// contributions with w = -k and w = +k differ only in sign, so each row kind
+// needs 11 constants (FluxU::g[C-1]): the normal-stress pairs' w (-4/3 or
+// 2/3) k for w = -k (3), the two shear families' -2 w k1, -2 w k2 for
+// w = -k (2 + 2), the grad-div pairs' -w k for w = -k p2 (3), and the
+// pressure pair's k.  Computed once per level on the host.
+struct FluxU {
+  float g[3][11];
+};
+template <int AX, int A0, int A1, int A2, int A3, int A4, int A5>
+__host__ __device__ __forceinline__ void usnn(float ne, const float *G, const float *xs, float &S, float &D) {
+  fpair<float, A0, A1>(ne * G[0], xs, S, D);
+  fpair<float, A2, A3>(ne * G[1], xs, S, D);
+  fpair<float, A4, A5>(ne * G[2], xs, S, D);
+}
+template <int A0, int A1, int A2, int A3>
+__host__ __device__ __forceinline__ void ushear(float ne, const float *G, const float *xs, float &S, float &D) {
+  fpair<float, A0, A1>(ne * G[0], xs, S, D);
+  fpair<float, A2, A3>(ne * G[1], xs, S, D);
+}
+// the x, y and z rows with the pairs of flux_x / flux_y / flux_z; the second
+// contribution of each family enters with its viscosity negated (w = +k)
+template <int C>
+__host__ __device__ __forceinline__ void flux_u(const Vis<float> &w, const float *G, bool p2, const float *xs,
+                                                float &S, float &D) {
+  S = 0.f; D = 0.f;
+  if (C == 1) {
+    usnn<0, 2, 1, 8, 9, 12, 13>(w.c0, G, xs, S, D);
+    usnn<0, 1, 3, 10, 11, 14, 15>(-w.c1, G, xs, S, D);
+    ushear<4, 1, 8, 10>(w.s0, G + 3, xs, S, D);
+    ushear<1, 5, 9, 11>(-w.s1, G + 3, xs, S, D);
+    ushear<6, 1, 12, 14>(w.t0, G + 5, xs, S, D);
+    ushear<1, 7, 13, 15>(-w.t1, G + 5, xs, S, D);
+    if (p2) {
+      usnn<0, 2, 1, 8, 9, 12, 13>(w.d0, G + 7, xs, S, D);
+      usnn<0, 1, 3, 10, 11, 14, 15>(-w.d1, G + 7, xs, S, D);
+    }
+  } else if (C == 2) {
+    usnn<1, 8, 10, 4, 1, 12, 14>(w.c0, G, xs, S, D);
+    usnn<1, 9, 11, 1, 5, 13, 15>(-w.c1, G, xs, S, D);
+    ushear<8, 9, 2, 1>(w.s0, G + 3, xs, S, D);
+    ushear<10, 11, 1, 3>(-w.s1, G + 3, xs, S, D);
+    ushear<6, 1, 12, 13>(w.t0, G + 5, xs, S, D);
+    ushear<1, 7, 14, 15>(-w.t1, G + 5, xs, S, D);
+    if (p2) {
+      usnn<1, 8, 10, 4, 1, 12, 14>(w.d0, G + 7, xs, S, D);
+      usnn<1, 9, 11, 1, 5, 13, 15>(-w.d1, G + 7, xs, S, D);
+    }
+  } else {
+    usnn<2, 12, 14, 8, 9, 6, 1>(w.c0, G, xs, S, D);
+    usnn<2, 13, 15, 10, 11, 1, 7>(-w.c1, G, xs, S, D);
+    ushear<12, 13, 2, 1>(w.s0, G + 3, xs, S, D);
+    ushear<14, 15, 1, 3>(-w.s1, G + 3, xs, S, D);
+    ushear<8, 10, 4, 1>(w.t0, G + 5, xs, S, D);
+    ushear<9, 11, 1, 5>(-w.t1, G + 5, xs, S, D);
+    if (p2) {
+      usnn<2, 12, 14, 8, 9, 6, 1>(w.d0, G + 7, xs, S, D);
+      usnn<2, 13, 15, 10, 11, 1, 7>(-w.d1, G + 7, xs, S, D);
+    }
+  }
+  fpair<float, 16, 17>(G[10], xs, S, D);
+}
+// FluxU of a level from its uniform spacings (iA, iB per axis), in double
+// and rounded once; the constants of the w = -k contributions
+inline FluxU flux_u_consts(double ax, double bx, double ay, double by, double az, double bz, double p2) {
+  FluxU u;
+  auto fill = [&](float *G, int AX, double k, double xk, double yk, double zk, double s1, double s2, double t1,
+                  double t2, double ks, double kt) {
+    const double w = -k;
+    G[0] = (float)(w * (AX == 0 ? -4.0 / 3.0 : 2.0 / 3.0) * xk);
+    G[1] = (float)(w * (AX == 1 ? -4.0 / 3.0 : 2.0 / 3.0) * yk);
+    G[2] = (float)(w * (AX == 2 ? -4.0 / 3.0 : 2.0 / 3.0) * zk);
+    G[3] = (float)(-2.0 * (-ks) * s1); G[4] = (float)(-2.0 * (-ks) * s2);
+    G[5] = (float)(-2.0 * (-kt) * t1); G[6] = (float)(-2.0 * (-kt) * t2);
+    const double wd = -k * p2;
+    G[7] = (float)(-wd * xk); G[8] = (float)(-wd * yk); G[9] = (float)(-wd * zk);
+    G[10] = (float)k;
+  };
+  // x row: xkf = 2 bx, ykf = ay, zkf = az; shears (by, bx) and (bz, bx)
+  fill(u.g[0], 0, 2.0 * bx, ax, ay, az, by, bx, bz, bx, ay, az);
+  // y row: xkf = ax, ykf = 2 by, zkf = az; shears (by, bx) with w = -xkf and (bz, by) with w = -zkf
+  fill(u.g[1], 1, 2.0 * by, ax, ay, az, by, bx, bz, by, ax, az);
+  // z row: xkf = ax, ykf = ay, zkf = 2 bz; shears (bz, bx) with w = -xkf and (bz, by) with w = -ykf
+  fill(u.g[2], 2, 2.0 * bz, ax, ay, az, bz, bx, bz, by, ax, ay);
+  return u;
+}
+
+// The same rows with their values read from the level's tables: (m1,m2,m3)
+// index the spacing tables; v is the node's index in the viscosity arrays.
+template <typename T>
+__device__ __forceinline__ void row_x(const Geo<T> &g, int m1, int m2, int m3, long v, T s[18]) {
+  row_x_v<T>(sp_load<T>(g, m1, m2, m3), vis_load<T, 1>(g, v), g.p2, s);
+}
+template <typename T>
+__device__ __forceinline__ void row_y(const Geo<T> &g, int m1, int m2, int m3, long v, T s[18]) {
+  row_y_v<T>(sp_load<T>(g, m1, m2, m3), vis_load<T, 2>(g, v), g.p2, s);
+}
+template <typename T>
+__device__ __forceinline__ void row_z(const Geo<T> &g, int m1, int m2, int m3, long v, T s[18]) {
+  row_z_v<T>(sp_load<T>(g, m1, m2, m3), vis_load<T, 3>(g, v), g.p2, s);
 }
 
 // continuity row of cell (m1,m2,m3) (upper-corner index)
