@@ -74,6 +74,9 @@ typedef struct {
                         from a process-wide pool for the solve (many contexts, few solving at once:
                         memory follows the solves in flight; a solve waits while the device has no
                         room for another set) */
+  int basis;         /* precision 0 only: 1 stores the flexible-GMRES basis and preconditioned vectors
+                        in FP32 (the orthogonalisation and the update of x run in FP64 on the stored
+                        values; the true residual is FP64), 0 stores them in FP64 */
 } stk_solver_opts;
 
 typedef struct {

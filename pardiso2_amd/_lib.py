@@ -42,7 +42,7 @@ class Opts(C.Structure):
                 ("cycle", C.c_int), ("coarse_viscmod", C.c_int),
                 ("tol", C.c_double), ("maxit", C.c_int), ("verbose", C.c_int),
                 ("schur", C.c_int), ("ksweeps", C.c_int), ("kscale", C.c_double), ("krylov", C.c_int),
-                ("graph", C.c_int), ("history", C.c_int)]
+                ("graph", C.c_int), ("history", C.c_int), ("basis", C.c_int)]
 
 
 class Stats(C.Structure):

@@ -1615,9 +1615,15 @@ __global__ void k_reduce_rows(const double *__restrict__ partial, int nb, int k,
 // Krylov basis kernels: one pass over the vectors for all k directions
 // ---------------------------------------------------------------------------
 constexpr int kMaxDirs = 257;  // restart + 1
-struct PtrsK {
-  const double *p[kMaxDirs];
+// pointers to the Krylov vectors (basis or preconditioned), stored as VT
+template <typename VT>
+struct PtrsKT {
+  const VT *p[kMaxDirs];
 };
+using PtrsK = PtrsKT<double>;
+template <typename VT> struct Pair2;
+template <> struct Pair2<double> { using t = double2; };
+template <> struct Pair2<float> { using t = float2; };
 
 // partial[i*gridDim + block] = sum over the block's slices of w2 * a * V_i
 __global__ void __launch_bounds__(256) k_mdot(long n4, const double *__restrict__ w2, const double *__restrict__ a,
@@ -1652,7 +1658,8 @@ __global__ void __launch_bounds__(256) k_mdot(long n4, const double *__restrict_
 }
 
 // out = a - sum_i coef_i V_i (out may alias a); partial[block] = sum of w2 out^2
-__global__ void __launch_bounds__(256) k_maxpy_norm(long n4, int k, const double *__restrict__ coef, PtrsK V,
+template <typename VT>
+__global__ void __launch_bounds__(256) k_maxpy_norm(long n4, int k, const double *__restrict__ coef, PtrsKT<VT> V,
                                                     const double *a, double *out, const double *__restrict__ w2,
                                                     double *partial) {
   __shared__ double sc[kMaxDirs];
@@ -1662,7 +1669,7 @@ __global__ void __launch_bounds__(256) k_maxpy_norm(long n4, int k, const double
   double ss = 0.0;
   for (long u = (long)blockIdx.x * blockDim.x + threadIdx.x; u < n4; u += (long)gridDim.x * blockDim.x) {
     double v = a[u];
-    for (int i = 0; i < k; i++) v -= sc[i] * V.p[i][u];
+    for (int i = 0; i < k; i++) v -= sc[i] * (double)V.p[i][u];
     out[u] = v;
     ss += w2[u] * v * v;
   }
@@ -1676,8 +1683,10 @@ __global__ void __launch_bounds__(256) k_maxpy_norm(long n4, int k, const double
 // dots, one for the updates per Arnoldi step).  Vectors are read as double2
 // (n4 is a multiple of 4).  Pass 1: partial sums of a_i = <V_i,u>,
 // b_i = <V_i,w> (i < k), <u,w>, <u,u> in the W2 inner product.
+template <typename VT>
 __global__ void __launch_bounds__(256) k_dots2(long n4, const double *__restrict__ w2, const double *__restrict__ u,
-                                               const double *__restrict__ w, PtrsK V, int k, double *partial) {
+                                               const double *__restrict__ w, PtrsKT<VT> V, int k, double *partial) {
+  using P2 = typename Pair2<VT>::t;
   __shared__ double acc[2 * kMaxDirs + 2][4];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nd = 2 * k + 2;
@@ -1708,15 +1717,19 @@ __global__ void __launch_bounds__(256) k_dots2(long n4, const double *__restrict
     if (lane == 0) { acc[2 * k][wave] += suw; acc[2 * k + 1][wave] += suu; }
     for (int i = 0; i < k; i += 2) {
       const bool two = i + 1 < k;
-      const double2 *__restrict__ v0 = reinterpret_cast<const double2 *>(V.p[i]);
-      const double2 *__restrict__ v1 = reinterpret_cast<const double2 *>(V.p[two ? i + 1 : i]);
+      const P2 *__restrict__ v0 = reinterpret_cast<const P2 *>(V.p[i]);
+      const P2 *__restrict__ v1 = reinterpret_cast<const P2 *>(V.p[two ? i + 1 : i]);
       double2 a0[E], a1[E];
 #pragma unroll
       for (int e = 0; e < E; e++) {
         const long p = base + threadIdx.x + 256L * e;
         a0[e] = make_double2(0.0, 0.0);
         a1[e] = a0[e];
-        if (p < n2) { a0[e] = v0[p]; a1[e] = v1[p]; }
+        if (p < n2) {
+          const P2 q0 = v0[p], q1 = v1[p];
+          a0[e] = make_double2((double)q0.x, (double)q0.y);
+          a1[e] = make_double2((double)q1.x, (double)q1.y);
+        }
       }
       double sa0 = 0.0, sb0 = 0.0, sa1 = 0.0, sb1 = 0.0;
 #pragma unroll
@@ -1746,8 +1759,14 @@ __global__ void __launch_bounds__(256) k_dots2(long n4, const double *__restrict
 // Pass 2: u <- (u - sum_i a_i V_i) * su (the reorthogonalised, normalised
 // basis vector v_k), w <- w - sum_i h_i V_i - h_k v_k, partial sums of w2 w^2.
 // coef = [a_0..a_{k-1}, h_0..h_{k-1}, h_k, su]
-__global__ void __launch_bounds__(256) k_update2(long n4, int k, const double *__restrict__ coef, PtrsK V, double *u,
-                                                 double *w, const double *__restrict__ w2, double *partial) {
+// With a VT = float basis, the new basis vector v_k is stored in vout
+// (rounded) and w subtracts h_k times the stored value, so the Arnoldi
+// relation holds for the basis as stored; u keeps its FP64 value.
+template <typename VT>
+__global__ void __launch_bounds__(256) k_update2(long n4, int k, const double *__restrict__ coef, PtrsKT<VT> V,
+                                                 double *u, double *w, const double *__restrict__ w2, double *partial,
+                                                 VT *vout) {
+  using P2 = typename Pair2<VT>::t;
   __shared__ double sc[2 * kMaxDirs + 2];
   __shared__ double red[4];
   for (int i = threadIdx.x; i < 2 * k + 2; i += blockDim.x) sc[i] = coef[i];
@@ -1772,15 +1791,19 @@ __global__ void __launch_bounds__(256) k_update2(long n4, int k, const double *_
     }
     for (int i = 0; i < k; i += 2) {
       const bool two = i + 1 < k;
-      const double2 *__restrict__ v0 = reinterpret_cast<const double2 *>(V.p[i]);
-      const double2 *__restrict__ v1 = reinterpret_cast<const double2 *>(V.p[two ? i + 1 : i]);
+      const P2 *__restrict__ v0 = reinterpret_cast<const P2 *>(V.p[i]);
+      const P2 *__restrict__ v1 = reinterpret_cast<const P2 *>(V.p[two ? i + 1 : i]);
       double2 a0[E], a1[E];
 #pragma unroll
       for (int e = 0; e < E; e++) {
         const long p = base + threadIdx.x + 256L * e;
         a0[e] = make_double2(0.0, 0.0);
         a1[e] = a0[e];
-        if (p < n2) { a0[e] = v0[p]; a1[e] = v1[p]; }
+        if (p < n2) {
+          const P2 q0 = v0[p], q1 = v1[p];
+          a0[e] = make_double2((double)q0.x, (double)q0.y);
+          a1[e] = make_double2((double)q1.x, (double)q1.y);
+        }
       }
       const double c0 = sc[i], d0 = sc[k + i];
 #pragma unroll
@@ -1803,8 +1826,15 @@ __global__ void __launch_bounds__(256) k_update2(long n4, int k, const double *_
       if (p >= n2) continue;
       uu[e].x *= su;
       uu[e].y *= su;
-      wv[e].x -= hk * uu[e].x;
-      wv[e].y -= hk * uu[e].y;
+      if constexpr (std::is_same<VT, float>::value) {
+        const float2 vs = make_float2((float)uu[e].x, (float)uu[e].y);
+        reinterpret_cast<float2 *>(vout)[p] = vs;
+        wv[e].x -= hk * (double)vs.x;
+        wv[e].y -= hk * (double)vs.y;
+      } else {
+        wv[e].x -= hk * uu[e].x;
+        wv[e].y -= hk * uu[e].y;
+      }
       U[p] = uu[e];
       Wv[p] = wv[e];
       const double2 q = W2[p];
@@ -1830,6 +1860,16 @@ template <typename A, typename B>
 __global__ void k_cast(long n, const A *__restrict__ x, B *__restrict__ y) {
   long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (u < n) y[u] = (B)x[u];
+}
+
+// z rounded to FP32: zs = (float)z, and z = (double)zs, so that products
+// formed from z and from its stored FP32 copy agree
+__global__ void k_round32(long n, double *z, float *zs) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float f = (float)z[i];
+  zs[i] = f;
+  z[i] = (double)f;
 }
 
 __global__ void k_scale_to(long n4, double s, const double *__restrict__ x, double *y) {

@@ -195,6 +195,7 @@ struct stk_ctx {
   int nca = 0;
   // outer iteration vectors (finest level)
   double *X = nullptr, *B = nullptr, *R = nullptr, *W2 = nullptr, *T1 = nullptr;
+  double *wu = nullptr, *ww = nullptr, *wz = nullptr;  // FP64 working u, w, z of a solve with an FP32 basis
   std::vector<double *> Zs, Qs;
   double *red = nullptr, *scal = nullptr;
   int nred = 1024;
@@ -1033,6 +1034,7 @@ int ensure_outer(stk_ctx *c, int m) {
   long n4 = 4 * c->lv[0].n;
   if (!c->X) {
     if (dalloc(&c->X, n4) || dalloc(&c->B, n4) || dalloc(&c->R, n4) || dalloc(&c->W2, n4) || dalloc(&c->T1, n4) ||
+        dalloc(&c->wu, n4) || dalloc(&c->ww, n4) || dalloc(&c->wz, n4) ||
         dalloc(&c->red, (size_t)c->nred * 16) || dalloc(&c->scal, 256) ||
         dalloc(&c->kred, (size_t)4 * c->nred * (2 * kMaxDirs + 2)) || dalloc(&c->hcol, 2 * kMaxDirs + 2))
       return STK_ENOMEM;
@@ -1440,6 +1442,7 @@ void stk_destroy(stk_ctx *c) {
   dfree(c->nu); dfree(c->ro); dfree(c->nd); dfree(c->nxx); dfree(c->nxy); dfree(c->nxz); dfree(c->nyz);
   dfree(c->ainv64); dfree(c->ainvP64); dfree(c->ainv32); dfree(c->cact); dfree(c->cinv);
   dfree(c->X); dfree(c->B); dfree(c->R); dfree(c->W2); dfree(c->T1); dfree(c->red); dfree(c->scal);
+  dfree(c->wu); dfree(c->ww); dfree(c->wz);
   dfree(c->kred); dfree(c->hcol); dfree(c->gin); dfree(c->gout);
   if (c->hpin) hipHostFree(c->hpin);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
@@ -1483,9 +1486,13 @@ void stk_default_opts(stk_solver_opts *o, int levels) {
   o->maxit = 800;
   o->schur = 1;
   o->krylov = 1;  // GCR can stagnate when the preconditioned operator is indefinite (sharp strong inclusions)
-  o->ksweeps = 2;
+  // 3 red/black sweeps per K-cycle level and coarse corrections scaled by 1.8 (piecewise-constant
+  // aggregation undershoots by about 2): measured against 2 and 2.0 over random-field, inclusion and
+  // sinker boxes from 33^3 to 257^3 (257^3 spheres: 153 -> 104 outer iterations)
+  o->ksweeps = 3;
   o->graph = 0;  // replay gains ~5% on small levels; captures from several host threads at once are not supported
-  o->kscale = 2.0;  // piecewise-constant aggregation undershoots the coarse correction by about 2
+  o->kscale = 1.8;
+  o->basis = 1;  // FP32 Krylov vectors in mixed precision (half the Gram-Schmidt traffic)
 }
 
 // boundary grammar line projected on every level (parallel/in3mg.c:299-375)
@@ -2150,7 +2157,12 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
                   int &status) {
   const long n4 = 4 * c->lv[0].n;
   const int m = o->restart;
-  const bool fp64 = o->precision == 1;
+  // FP32 storage of the basis and of the preconditioned vectors (mixed
+  // precision): the Krylov vectors' buffers hold FP32 values, u and w stay
+  // FP64 working vectors
+  const bool b32 = o->precision == 0 && o->basis == 1;
+  auto v32 = [&](int i) { return reinterpret_cast<float *>(c->Qs[i]); };
+  auto z32 = [&](int i) { return reinterpret_cast<float *>(c->Zs[i]); };
   // dots and updates: one block per 1024-element slice up to 4 nred blocks
   // (partials of 2j+2 sums per block for the dots)
   const unsigned nbk = (unsigned)std::min<long>(4L * c->nred, std::max<long>(1, (n4 / 2 + 1023) / 1024));
@@ -2176,20 +2188,27 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
   double r = *rel;
   while (r > o->tol && it < o->maxit) {
     const double beta = r * nb;
-    hipLaunchKernelGGL(k_scale_to, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, 1.0 / beta, c->R, c->Qs[0]);
+    hipLaunchKernelGGL(k_scale_to, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, 1.0 / beta, c->R,
+                       b32 ? c->wu : c->Qs[0]);
     std::fill(g.begin(), g.end(), 0.0);
     std::fill(H.begin(), H.end(), 0.0);
     g[0] = beta;
     int j = 0, ncols = 0;
     bool conv = false;
     while (j < m && it < o->maxit) {
-      double *u = c->Qs[j], *Z = c->Zs[j], *W = c->Qs[j + 1];
+      double *u = b32 ? c->wu : c->Qs[j], *Z = b32 ? c->wz : c->Zs[j], *W = b32 ? c->ww : c->Qs[j + 1];
       int prc = apply_precond(c, o, tile, u, Z);
       if (prc) { status = prc; break; }
+      // FP32 storage: A is applied to the stored (rounded) z, so the update x += Z y uses the same vectors
+      if (b32) hipLaunchKernelGGL(k_round32, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, Z, z32(j));
       rowop<double, false>(c, 0, Z, nullptr, W, 1);
       PtrsK P{};
-      for (int i = 0; i < j; i++) P.p[i] = c->Qs[i];
-      hipLaunchKernelGGL(k_dots2, dim3(nbk), dim3(256), 0, c->st, n4, c->W2, u, W, P, j, c->kred);
+      PtrsKT<float> P32{};
+      for (int i = 0; i < j; i++) { P.p[i] = c->Qs[i]; P32.p[i] = v32(i); }
+      if (b32)
+        hipLaunchKernelGGL(k_dots2<float>, dim3(nbk), dim3(256), 0, c->st, n4, c->W2, u, W, P32, j, c->kred);
+      else
+        hipLaunchKernelGGL(k_dots2<double>, dim3(nbk), dim3(256), 0, c->st, n4, c->W2, u, W, P, j, c->kred);
       hipLaunchKernelGGL(k_reduce_rows, dim3(2 * j + 2), dim3(256), 0, c->st, c->kred, (int)nbk, 2 * j + 2, c->hcol);
       HIPCHK(hipMemcpyAsync(c->hpin, c->hcol, (2 * j + 2) * sizeof(double), hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
@@ -2218,7 +2237,12 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
       coef[2 * j + 1] = 1.0 / nu;
       memcpy(c->hpin, coef.data(), (2 * j + 2) * sizeof(double));  // copied before the stream reuses hpin
       HIPCHK(hipMemcpyAsync(c->hcol, c->hpin, (2 * j + 2) * sizeof(double), hipMemcpyHostToDevice, c->st));
-      hipLaunchKernelGGL(k_update2, dim3(nbu), dim3(256), 0, c->st, n4, j, c->hcol, P, u, W, c->W2, c->kred);
+      if (b32)
+        hipLaunchKernelGGL(k_update2<float>, dim3(nbu), dim3(256), 0, c->st, n4, j, c->hcol, P32, u, W, c->W2,
+                           c->kred, v32(j));
+      else
+        hipLaunchKernelGGL(k_update2<double>, dim3(nbu), dim3(256), 0, c->st, n4, j, c->hcol, P, u, W, c->W2,
+                           c->kred, (double *)nullptr);
       hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, c->st, c->kred, (int)nbu, 1, c->hcol + 2 * m + 1);
       HIPCHK(hipMemcpyAsync(c->hpin + 2 * kMaxDirs + 4, c->hcol + 2 * m + 1, sizeof(double), hipMemcpyDeviceToHost,
                             c->st));
@@ -2232,7 +2256,8 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
       it++;
       j++;
       if (hn == 0.0) break;
-      hipLaunchKernelGGL(k_scale_to, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, 1.0 / hn, W, W);
+      // the next candidate: w / |w| (in place, or into the working u)
+      hipLaunchKernelGGL(k_scale_to, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, 1.0 / hn, W, b32 ? u : W);
     }
     if (status) break;
     if (!conv && j > ncols) {
@@ -2251,9 +2276,17 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
     HIPCHK(hipStreamSynchronize(c->st));  // the staging buffer may still feed the last coefficient copy
     memcpy(c->hpin, y.data(), k * sizeof(double));
     HIPCHK(hipMemcpyAsync(c->hcol, c->hpin, k * sizeof(double), hipMemcpyHostToDevice, c->st));
-    PtrsK P{};
-    for (int i = 0; i < k; i++) P.p[i] = c->Zs[i];
-    hipLaunchKernelGGL(k_maxpy_norm, dim3(nbu), dim3(256), 0, c->st, n4, k, c->hcol, P, c->X, c->X, c->W2, c->kred);
+    if (b32) {
+      PtrsKT<float> P{};
+      for (int i = 0; i < k; i++) P.p[i] = z32(i);
+      hipLaunchKernelGGL(k_maxpy_norm<float>, dim3(nbu), dim3(256), 0, c->st, n4, k, c->hcol, P, c->X, c->X, c->W2,
+                         c->kred);
+    } else {
+      PtrsK P{};
+      for (int i = 0; i < k; i++) P.p[i] = c->Zs[i];
+      hipLaunchKernelGGL(k_maxpy_norm<double>, dim3(nbu), dim3(256), 0, c->st, n4, k, c->hcol, P, c->X, c->X, c->W2,
+                         c->kred);
+    }
     true_residual(c);
     double s4[4];
     int rc = blocks_norms(c, c->R, c->B, s4);
@@ -2298,7 +2331,7 @@ int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   if (o->restart < 1 || o->restart > kMaxDirs - 1 || o->maxit < 1 || !(o->tol > 0) || o->cycle < 0 || o->cycle > 1 ||
       o->coarse_viscmod < -1 || o->coarse_viscmod > 2 || o->schur < 0 || o->schur > 2 || o->ksweeps < 0 ||
       o->ksweeps > 16 || !(o->kscale > 0) || o->krylov < 0 || o->krylov > 1 || o->graph < 0 || o->graph > 1 ||
-      o->history < 0 || o->history > 1)
+      o->history < 0 || o->history > 1 || o->basis < 0 || o->basis > 1)
     return STK_EINVAL;
   {
     const int want = (o->coarse_viscmod < 0) ? c->kn.viscmod : o->coarse_viscmod;
