@@ -1,18 +1,25 @@
 #!/usr/bin/env python
 """Stokes solves/s and p99 latency to a 1e-8 FP64 relative residual on one MI355X.
 
-Workload (default): single 129^3-node solve, 6 multigrid levels
-(129/67/36/20/12/8), log10(viscosity) a seeded Gaussian random field
-(correlation length 0.1 box, clipped to [18, 24]), density 3300 +/- 50,
-free-slip box, gravity -z.  Node fields are built on the device from 2x2x2
-markers per cell once; a step is one solve from those resident node fields:
-prepare (viscosity restriction to all levels, stencil viscosities,
-right-hand side, coarsest inverse) + hydrostatic guess + FP64 outer
-iteration with FP32 V-cycle preconditioning until ||W r||/||W b|| <= 1e-8.
+Workload (default): single 257^3-node solve, 7 multigrid levels
+(257/131/68/36/20/12/8), eight seeded spheres of radius 0.03-0.1 box,
+viscosity 1e-3..1e3 times the 1e21 Pa s matrix (log-uniform), density 3200
+in 3300 kg/m^3, 2x2x2 jittered markers per cell, free-slip box, gravity -z.  Node fields are built on the device from 2x2x2 markers per cell
+once; a step is one solve from those resident node fields: prepare
+(viscosity restriction to all levels, stencil viscosities, right-hand side,
+coarsest inverse) + hydrostatic guess + FP64 flexible GMRES with the FP32
+V-cycle / BFBt block preconditioner until ||W r||/||W b|| <= 1e-8.
+--workload randfield --n 129 and --workload sinker --n 33 select the other
+single-solve problems; --workload batched and --workload served the
+64-instance batch and the Poisson request stream.
 
-The line also carries the roofline of the finest-level FP32 smoother sweep
-(64 algorithmic bytes per node and sweep) and the CPU oracle's time for the
-same algorithm on the same problem (bounded sample, single thread).
+The line also carries the roofline of the finest-level FP32 smoother
+instance the timed solve runs (64 algorithmic bytes per node and sweep,
+HIP events on the library stream, HBM bytes from the committed rocprofv3
+counters of that instance), the oracle's residual of the device solution,
+and the CPU baseline: full FP64 solves of the 33^3 instance of the same
+workload by the oracle on all host cores (median over repeats, CPU model and
+sockets recorded) with the device's rate on that same problem beside it.
 """
 from __future__ import annotations
 
@@ -59,57 +66,122 @@ def sweeps_for(L: int):
     return pre, pre
 
 
-def cpu_baseline(pb, pre, post, its_needed: int, budget_s: float, check=None):
-    """The oracle's FP64 GCR (tile-coloured smoother, the device's V-cycle) on
-    the same problem with the host's OpenMP threads, for a bounded number of
-    outer iterations.  ``check`` (a device solution) is first scored by the
-    oracle's own assembled FP64 operator; the relative residual is returned
-    under "oracle_relres"."""
+def host_cpu():
+    """CPU model, socket and physical core count from /proc/cpuinfo, and the
+    CPUs this process may run on."""
+    model, sockets, cores, phys = None, set(), set(), None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                    sockets.add(v)
+                elif k == "core id":
+                    cores.add((phys, v))
+    except OSError:
+        pass
+    return {"cpu_model": model, "sockets": len(sockets) or None, "physical_cores": len(cores) or None,
+            "cpus_allowed": len(os.sched_getaffinity(0)), "nproc": os.cpu_count()}
+
+
+CPU_SAMPLE_N = 33
+
+
+def load_oracle():
+    """The CPU oracle, OpenMP threads bound to the host cores (set before its
+    OpenMP runtime starts)."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import oracle as O
     O.build()
+    return O
+
+
+def cpu_baseline(pb, budget_s: float, repeats: int = 5, device=True):
+    """Measured full FP64 solves on the host cores.
+
+    The oracle's FP64 GCR (velocity V-cycle of the reference's smoother and
+    transfers, mass-matrix Schur scaling; all OpenMP threads) solves ``pb`` to
+    the same weighted relative residual 1e-8 from the same hydrostatic guess,
+    ``repeats`` times (at least 3; fewer than ``repeats`` only when the budget
+    runs out); median and p99 wall clock.  With ``device`` the library solves
+    the same problem the same number of times for a like-for-like rate, and
+    its mixed-precision solution is compared with the oracle's converged one.
+    """
+    host = host_cpu()  # before the OpenMP runtime binds this thread
+    O = load_oracle()
+    pre, post = sweeps_for(pb.levels)
     t0 = time.perf_counter()
     o = O.from_problem(pb)
     setup_s = time.perf_counter() - t0
-    orel = None
-    if check is not None:
-        o.arr("sol0")[: check.size] = check
-        orel = float(o.relres()[0])
-        o.arr("sol0")[:] = 0.0
-    # one iteration to estimate, then as many as fit the budget
-    it, _, _, spi = o.gcr(pre, post, mode=1, tol=1e-30, maxit=1, restart=32)
-    k = int(max(1, min(20, (budget_s - spi) / max(spi, 1e-6))))
-    it, _, hist, spi = o.gcr(pre, post, mode=1, tol=1e-30, maxit=k, restart=32)
-    per_solve = spi * its_needed
-    nt = O.threads()
-    return {"value": 1.0 / per_solve if per_solve > 0 else None, "unit": "solves/s", "cores": nt,
-            "kind": "port", "oracle_relres": orel,
-            "sample": (f"{it} outer iterations ({spi:.3f} s each, wall clock) of the oracle's FP64 GCR + "
-                       f"velocity V-cycle (tile-coloured smoother, mass-matrix Schur scaling) on the same "
-                       f"{pb.n}^3 problem with {nt} OpenMP threads, times the {its_needed} iterations the "
-                       f"FP64 device solve needs; oracle setup {setup_s:.1f} s not counted")}
+    times, its, x_or = [], [], None
+    while len(times) < repeats and (len(times) < 3 or sum(times) < budget_s):
+        ts = time.perf_counter()
+        it, x, hist, _ = o.gcr(pre, post, mode=1, tol=1e-8, maxit=2000, restart=32)
+        times.append(time.perf_counter() - ts)
+        its.append(int(it))
+        x_or = x
+        if it < 0 or hist[-1] > 1e-8:
+            raise RuntimeError(f"oracle solve did not reach 1e-8 ({it} iterations, {hist[-1]:.2e})")
+    med = float(np.median(times))
+    out = {"value": 1.0 / med, "unit": "solves/s", "cores": O.threads(), "kind": "port",
+           "sample": (f"full FP64 solves of the {pb.n}^3 instance of this workload (same generator and seed) by "
+                      f"the oracle's GCR + velocity V-cycle (reference smoother and transfers, mass-matrix Schur "
+                      f"scaling) to weighted relative residual 1e-8, {len(times)} repeats, median wall clock; "
+                      f"oracle setup (assembly, coarse LU) {setup_s:.2f} s not counted; the 129^3 and 257^3 "
+                      f"instances do not fit a bounded CPU run"),
+           "median_s": med, "p99_s": float(np.percentile(times, 99)), "repeats": len(times), "iterations": its,
+           "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES"),
+           **host}
+    if device:
+        from pardiso2_amd import _lib as S
+        ctx = S.from_problem(pb)
+        ctx.autotune(0)
+        opts = S.default_opts(pb.levels, precision=0, pre=pre, post=post, tol=1e-8, maxit=800, restart=128)
+        dts, dits = [], []
+        for _ in range(len(times) + 1):  # the first solve warms the context
+            ts = time.perf_counter()
+            ctx.prepare()
+            ctx.hydrostatic_guess()
+            st = ctx.solve(opts)
+            dts.append(time.perf_counter() - ts)
+            dits.append(st.iterations)
+        out["device_same_sample"] = {"value": 1.0 / float(np.median(dts[1:])), "unit": "solves/s",
+                                     "median_ms": float(np.median(dts[1:])) * 1e3, "iterations": dits[-1],
+                                     "precision": "mixed"}
+        out["device_vs_cpu"] = out["device_same_sample"]["value"] / out["value"]
+        out["device_error_vs_oracle"] = solution_diff(ctx.solution(), x_or, ctx.info[0][0], ctx.info[0][1])
+        ctx.close()
+    return out
 
 
-def solution_error(ctx, x):
-    """Max and RMS difference of solution x from the context's current (FP64)
-    solution, velocities and pressures separately, relative to the FP64
-    solution's max and RMS; pressures compared at a common gauge (cell 2,2,2)."""
+def solution_diff(x, ref, X, Y):
+    """Max and RMS difference of solution x from ref, velocities and
+    pressures separately, relative to ref's max and RMS; pressures compared
+    at a common gauge (cell 2,2,2)."""
     def gauged(v):
-        v = v.copy()
+        v = np.array(v, dtype=np.float64)
         p = v[0::4]
-        X, Y = ctx.info[0][0], ctx.info[0][1]
-        ref = p[2 * X * Y + 2 * Y + 2]
-        v[0::4] = p - ref
+        v[0::4] = p - p[2 * X * Y + 2 * Y + 2]
         return v
-    ctx.pressure_gauge(2, 2, 2, 0.0)
-    ref = ctx.solution()
-    a, b = gauged(x), ref
+    a, b = gauged(x), gauged(ref)
     out = {}
     for name, sl in (("velocity", np.arange(a.size) % 4 != 0), ("pressure", np.arange(a.size) % 4 == 0)):
         d, r = a[sl] - b[sl], b[sl]
         out[name] = {"max_rel": float(np.abs(d).max() / max(np.abs(r).max(), 1e-300)),
                      "rms_rel": float(np.sqrt(np.mean(d * d)) / max(np.sqrt(np.mean(r * r)), 1e-300))}
     return out
+
+
+def solution_error(ctx, x):
+    """Difference of solution x from the context's current (FP64) solution."""
+    ctx.pressure_gauge(2, 2, 2, 0.0)
+    return solution_diff(x, ctx.solution(), ctx.info[0][0], ctx.info[0][1])
 
 
 def smoother_instance(ctx, tile):
@@ -139,12 +211,12 @@ def smoother_roofline(ctx, nodes):
             "ms_per_launch": ms, "bytes_per_launch": SMOOTHER_BYTES_PER_NODE * nodes}
 
 
-def cpu_sample(pb, its_needed, budget_s):
-    pre, post = sweeps_for(pb.levels)
+def cpu_sample(pb, budget_s, device=True):
     try:
-        return cpu_baseline(pb, pre, post, its_needed, budget_s)
+        return cpu_baseline(pb, budget_s, device=device)
     except Exception as e:  # the baseline must not hide the device result
-        return {"value": None, "unit": "solves/s", "cores": 1, "kind": "port", "sample": f"failed: {e}"}
+        return {"value": None, "unit": "solves/s", "cores": None, "kind": "port", "sample": f"failed: {e}",
+                **host_cpu()}
 
 
 def bench_batched(args):
@@ -209,7 +281,8 @@ def bench_batched(args):
         "speedup_vs_fp64_single_instance": (args.instances * args.steps / total) / ref["fp64_serial_solves_per_s"]
         if ref else None,
         "roofline": smoother_roofline(ctxs[big], ctxs[big].n0),
-        "cpu_baseline": None if args.no_cpu else cpu_sample(problems[0], st[0].iterations, args.cpu_budget),
+        "cpu_baseline": None if args.no_cpu else cpu_sample(
+            problems[min(range(len(sizes)), key=lambda i: sizes[i])], args.cpu_budget),
     }
     print(json.dumps(out), flush=True)
     for c in ctxs:
@@ -349,7 +422,7 @@ def main():
         with open(tf) as f:
             tj = json.load(f)
         k = tj.get("kernels", {}).get("smooth32", {})
-        if tj.get("nodes") == n0 and k.get("instance", inst) == inst:
+        if tj.get("nodes") == n0 and inst in k.get("instance", ""):  # counters of this very instance only
             traffic = k.get("hbm_bytes_per_launch")
 
     x_dev = ctx.solution()
@@ -363,13 +436,30 @@ def main():
         log(f"fp64 reference solve: {s64.iterations} iterations, relres {s64.relres:.3e}, {s64.seconds:.2f} s")
         err = solution_error(ctx, x_dev)
 
-    cpu = None
+    cpu, orel = None, None
     log(f"smoother {ms_sm:.4f} ms ({gbs_sm:.0f} GB/s), fp64 residual {ms_res:.4f} ms ({gbs_res:.0f} GB/s)")
     if not args.no_cpu:
+        # the oracle's own residual of the timed solve's solution, on the device's node fields
+        # (rows on the fly, no assembly)
         try:
-            cpu = cpu_baseline(pb, pre, post, fp64_its or its[-1], args.cpu_budget, check=x_dev)
-        except Exception as e:  # the baseline must not hide the device result
-            cpu = {"value": None, "unit": "solves/s", "cores": 1, "kind": "port", "sample": f"failed: {e}"}
+            O = load_oracle()
+            o = O.Oracle(pb.gx, pb.gy, pb.gz, L)
+            k = pb.knobs
+            o.knobs(k["p0koef"], k["p1koef"], k["p2koef"], k["v0koef"], k["v1koef"], k["nubeg"], k["nuend"],
+                    k["viscmod"], pb.gravity)
+            o.bc_lines(pb.bc)
+            f = ctx.fields()
+            o.set_fields(f["nu"], f["ro"], f["nxx"], f["nxy"], f["nxz"], f["nyz"])
+            del f
+            orel = float(o.relres_rows(x_dev)[0])
+            log(f"oracle residual of the device solution: {orel:.3e}")
+            del o
+        except Exception as e:
+            log(f"oracle residual failed: {e}")
+        ctx.close()
+        spb = pb if pb.n <= CPU_SAMPLE_N else make_problem(args.workload, CPU_SAMPLE_N, args.seed)
+        cpu = cpu_sample(spb, args.cpu_budget)
+        log(f"cpu baseline: {cpu.get('value')} solves/s ({cpu.get('sample')})")
 
     out = {
         "metric": "Stokes solves/s to 1e-8 FP64 relative residual",
@@ -394,7 +484,7 @@ def main():
         "solve_ms_device": float(np.mean(solve_s) * 1e3),
         "fp64_iterations": fp64_its,
         "error_vs_fp64_solve": err,
-        "oracle_relres": cpu.get("oracle_relres") if cpu else None,
+        "oracle_relres": orel,
         "smoother_tiles": tiles,
         "roofline": {"bound": "hbm", "kernel": f"{inst}, finest level, tile {tiles[0][:3]}", "achieved": gbs_sm,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs_sm / PEAK_HBM_GBS, "traffic": traffic,

@@ -4,6 +4,7 @@
     python -m pardiso2_amd.rocprof_summary trace <dir> --kernel SUBSTR [--last N]
     python -m pardiso2_amd.rocprof_summary pmc <dir> [--kernel SUBSTR] [--last N]
     python -m pardiso2_amd.rocprof_summary window <dir> --since SUBSTR [--nth K] [--top N] [--by-grid]
+    python -m pardiso2_amd.rocprof_summary traffic <dir> --nodes N --entry NAME:SUBSTR:BYTES_PER_NODE ... [--out F]
 
 `stats` reads the *_kernel_stats.csv that `rocprofv3 --kernel-trace --stats
 --output-format csv` writes; `trace` reads the same run's *_kernel_trace.csv
@@ -15,7 +16,12 @@ number of dispatches and the mean value per dispatch (of the last N);
 SUBSTR to the end (e.g. one timed solve step) and prints per-kernel totals,
 the window's span and the share of it the device spent in kernels (with
 --by-grid, one line per kernel and launch shape, so the levels of a hierarchy
-show apart).
+show apart).  `traffic` reads <dir>/<NAME>_FETCH_SIZE and <dir>/<NAME>_WRITE_SIZE
+(one `--pmc` run each, NAME from the entries and `copy`), scales both counters
+by what they read for k_stream_copy (4*N doubles in, 4*N out per launch) and
+prints, per entry, the HBM bytes per launch of the kernel whose name holds
+SUBSTR (mean of its last --last dispatches, default 10) against its
+algorithmic bytes (BYTES_PER_NODE * N); --out writes the same as JSON.
 """
 from __future__ import annotations
 
@@ -104,7 +110,7 @@ def window(d, since, nth, top, by_grid=False):
         print(f"{cnt:8d} {t / 1e6:10.3f} {t / cnt / 1e3:10.2f} {100 * t / span:7.2f}  {name[:110]}")
 
 
-def pmc(d, kernel, last=0):
+def _pmc_means(d, kernel, last=0):
     per = defaultdict(float)  # (kernel, counter, dispatch) -> value summed over the row's dimensions
     for f in _find(d, "counter_collection.csv"):
         with open(f) as fh:
@@ -123,13 +129,49 @@ def pmc(d, kernel, last=0):
         acc[(name, cn)].append(v)
     if last:
         acc = {key: v[-last:] for key, v in acc.items()}
-    for (name, cn), v in sorted(acc.items()):
+    return acc
+
+
+def pmc(d, kernel, last=0):
+    for (name, cn), v in sorted(_pmc_means(d, kernel, last).items()):
         print(f"{cn:>12} dispatches {len(v):5d} mean {sum(v) / len(v):14.1f}  {name[:100]}")
+
+
+def _counter(d, substr, counter, last):
+    acc = {k: v for k, v in _pmc_means(d, substr, last).items() if k[1] == counter}
+    if len(acc) != 1:
+        raise SystemExit(f"{d}: {len(acc)} kernels named *{substr}* with {counter}: {sorted(k[0] for k in acc)}")
+    (name, _), v = next(iter(acc.items()))
+    return name, sum(v) / len(v)
+
+
+def traffic(d, nodes, entries, last, out):
+    copy_bytes = 4 * nodes * 8.0
+    _, cf = _counter(os.path.join(d, "copy_FETCH_SIZE"), "k_stream_copy", "FETCH_SIZE", last)
+    _, cw = _counter(os.path.join(d, "copy_WRITE_SIZE"), "k_stream_copy", "WRITE_SIZE", last)
+    ff, wf = copy_bytes / (cf * 1024), copy_bytes / (cw * 1024)
+    res = {"nodes": nodes, "fetch_factor": ff, "write_factor": wf, "kernels": {}}
+    print(f"calibration: k_stream_copy moves {copy_bytes / 1024:.1f} KiB each way; FETCH_SIZE reads {cf:.1f} KiB "
+          f"(factor {ff:.5f}), WRITE_SIZE reads {cw:.1f} KiB (factor {wf:.5f})")
+    print(f"{'entry':10} {'FETCH KiB':>12} {'WRITE KiB':>12} {'HBM bytes':>14} {'algorithmic':>14} {'ratio':>6}  instance")
+    for e in entries:
+        key, sub, bpn = e.rsplit(":", 2)
+        name, fk = _counter(os.path.join(d, f"{key}_FETCH_SIZE"), sub, "FETCH_SIZE", last)
+        _, wk = _counter(os.path.join(d, f"{key}_WRITE_SIZE"), sub, "WRITE_SIZE", last)
+        hbm = fk * 1024 * ff + wk * 1024 * wf
+        alg = float(bpn) * nodes
+        res["kernels"][key] = {"instance": name, "fetch_kib": fk, "write_kib": wk, "hbm_bytes_per_launch": hbm,
+                               "algorithmic_bytes": alg}
+        print(f"{key:10} {fk:12.1f} {wk:12.1f} {hbm:14.0f} {alg:14.0f} {hbm / alg:6.3f}  {name[:90]}")
+    if out:
+        import json
+        with open(out, "w") as f:
+            json.dump(res, f, indent=1)
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["stats", "trace", "pmc", "window"])
+    ap.add_argument("mode", choices=["stats", "trace", "pmc", "window", "traffic"])
     ap.add_argument("dir")
     ap.add_argument("--top", type=int, default=25)
     ap.add_argument("--kernel", default=None)
@@ -137,6 +179,9 @@ def main(argv=None):
     ap.add_argument("--since", default=None)
     ap.add_argument("--nth", type=int, default=1)
     ap.add_argument("--by-grid", action="store_true", help="window: one line per kernel and launch shape")
+    ap.add_argument("--nodes", type=int, default=0, help="traffic: finest-level nodes of the run")
+    ap.add_argument("--entry", action="append", default=[], help="traffic: NAME:KERNEL_SUBSTR:BYTES_PER_NODE")
+    ap.add_argument("--out", default=None, help="traffic: JSON output file")
     a = ap.parse_args(argv)
     if a.mode == "stats":
         stats(a.dir, a.top)
@@ -144,6 +189,8 @@ def main(argv=None):
         trace(a.dir, a.kernel or "", a.last)
     elif a.mode == "window":
         window(a.dir, a.since or "", a.nth, a.top, a.by_grid)
+    elif a.mode == "traffic":
+        traffic(a.dir, a.nodes, a.entry, a.last or 10, a.out)
     else:
         pmc(a.dir, a.kernel, a.last)
 

@@ -129,7 +129,8 @@ def case(request, oracle_mod):
     pb.bc = setups.free_slip_lines(n, n, n, gauge=None)
     o = oracle_mod.from_problem(pb)
     g = S.from_problem(pb)
-    yield pb, o, g, Mirror(o, L)
+    d = S.default_opts(L)  # the K-cycle sweeps and coarse-correction scale the solve runs
+    yield pb, o, g, Mirror(o, L, nu=d.ksweeps, scale=d.kscale)
     g.close()
 
 
@@ -152,13 +153,13 @@ def test_kcycle_matches_restatement_and_contracts(case):
     rng = np.random.default_rng(2)
     act = m.K[0].diagonal() != 0
     b = rng.standard_normal(o.info[0][4]) * act
-    x = g.kcycle_apply(b, precision=1)
+    x = g.kcycle_apply(b, precision=1, ksweeps=m.nu, kscale=m.scale)
     xe = m.kcycle(0, b)
     np.testing.assert_allclose(x, xe, rtol=1e-9, atol=1e-9 * np.abs(xe).max())
     b0 = b - b[act].mean() * act
-    x0 = g.kcycle_apply(b0, precision=1)
+    x0 = g.kcycle_apply(b0, precision=1, ksweeps=m.nu, kscale=m.scale)
     assert np.linalg.norm(b0 - m.K[0] @ x0) < 0.5 * np.linalg.norm(b0)
-    x32 = g.kcycle_apply(b, precision=0)
+    x32 = g.kcycle_apply(b, precision=0, ksweeps=m.nu, kscale=m.scale)
     np.testing.assert_allclose(x32, xe, rtol=1e-3, atol=1e-4 * np.abs(xe).max())
 
 
