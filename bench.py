@@ -481,6 +481,9 @@ def main():
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "iterations": its,
         "relres": max(rels),
+        "reference_norms": {k: v for k, v in st.norms.as_dict().items()
+                            if k in ("stoksum", "stoksum1", "stoksum2", "contsum", "contsumb", "bondsum", "maxvxyz",
+                                     "converged")},
         "solve_ms_device": float(np.mean(solve_s) * 1e3),
         "fp64_iterations": fp64_its,
         "error_vs_fp64_solve": err,

@@ -43,12 +43,19 @@ extern "C" {
 typedef struct stk_ctx stk_ctx;
 
 /* Solver knobs with the reference's meaning (parallel/mode.t3c:84-96,
- * parallel/load3mg.c:116-131).  gx/gy/gz are GXKOEF/GYKOEF/GZKOEF. */
+ * parallel/load3mg.c:116-131).  gx/gy/gz are GXKOEF/GYKOEF/GZKOEF.
+ * divvmin/stoksmin/divvmax/stoksmax are DIVVMIN/STOKSMIN/DIVVMAX/STOKSMAX
+ * (parallel/mode.t3c:79-82, parallel/load3mg.c:112-115): the thresholds of
+ * the reference's stop test, reported in stk_norms.converged; divvmax > 0
+ * replaces the velocity scale maxvxyz of the norms.  Zero-initialised
+ * trailing fields (a caller that sets only the first eleven) report
+ * converged = 0 unless every norm is zero. */
 typedef struct {
   double p0koef, p1koef, p2koef, v0koef, v1koef;
   double nubeg, nuend;
   int viscmod;
   double gx, gy, gz;
+  double divvmin, stoksmin, divvmax, stoksmax;
 } stk_knobs;
 
 typedef struct {
@@ -79,6 +86,29 @@ typedef struct {
                         values; the true residual is FP64), 0 stores them in FP64 */
 } stk_solver_opts;
 
+/* The reference's own convergence norms of a finest-level solution
+ * (parallel/move3mg.c:3061-3086, 3090-3238), FP64:
+ *   maxvxyz   RMS of the vx, vy, vz ranges (1e-8 if zero; DIVVMAX if > 0)
+ *   stoksum   RMS momentum error (divergence of the recomputed stresses
+ *             minus pressure gradient minus gravity) over momentum rows
+ *   stoksum1  RMS of that error times h^2/eta, over maxvxyz
+ *   stoksum2  RMS of the product of the two, over maxvxyz
+ *   contsum   RMS of div(v) times the mean cell size over maxvxyz, cells
+ *             with a continuity equation; contsumb: cells with a pressure
+ *             boundary equation
+ *   bondsum   RMS error of the velocity boundary equations
+ * with the counts of rows and cells, the extrema the reference prints
+ * (velocities, pressure, cell viscosity), and converged = the reference's
+ * stop test (parallel/move3mg.c:3282): stoksum2 <= STOKSMIN*STOKSMAX,
+ * stoksum*stoksum1 <= STOKSMIN*STOKSMAX and contsum <= DIVVMIN. */
+typedef struct {
+  double stoksum, stoksum1, stoksum2, contsum, contsumb, bondsum;
+  double stoknum, contnum, contnumb, bondnum;
+  double maxvxyz;
+  double minvx, maxvx, minvy, maxvy, minvz, maxvz, minpr, maxpr, minnu, maxnu;
+  int converged;
+} stk_norms;
+
 typedef struct {
   int iterations;
   int status;
@@ -87,6 +117,7 @@ typedef struct {
   double seconds;    /* wall time of the solve on the device stream */
   double blocks[3];  /* ||r_momentum||, ||W r_continuity||, ||r_boundary|| */
   double hist[256];  /* relative residual after each outer iteration */
+  stk_norms norms;   /* the reference's norms of the final solution */
 } stk_stats;
 
 /* ---- context API ------------------------------------------------------ */
@@ -145,6 +176,17 @@ int stk_get_solution(stk_ctx *ctx, double *sol0_level0);
 int stk_solve(stk_ctx *ctx, const stk_solver_opts *opts, stk_stats *stats);
 /* Relative residual of the current finest solution (FP64, device). */
 int stk_relres(stk_ctx *ctx, double *relres, double blocks[3]);
+/* The reference's convergence norms of the current solution (the post-solve
+ * pass of vpiterate, parallel/move3mg.c:2835-3238; every stk_solve reports
+ * them for its result in stk_stats.norms).  Non-periodic boxes. */
+int stk_norms_eval(stk_ctx *ctx, stk_norms *out);
+/* Stress and strain-rate recompute of the current solution
+ * (parallel/move3mg.c:2908-3046): sig and eps get 6 planes of mgn[0]
+ * doubles each, in the order xx yy zz xy xz yz, node layout of level 0
+ * (sxx..szz at the cell whose upper corner is the node, shear stresses at
+ * the node's edge); zero where the reference's recompute writes nothing.
+ * Either pointer may be NULL. */
+int stk_stress_strain(stk_ctx *ctx, double *sig, double *eps);
 /* Hydrostatic initial guess: v = 0, P integrated up each column through the
  * z-momentum rows (the reference seeds P this way, parallel/in3mg.c:927-938). */
 int stk_hydrostatic_guess(stk_ctx *ctx);

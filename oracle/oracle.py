@@ -68,6 +68,7 @@ def lib():
         L.orc_relres.argtypes = [C.c_void_p, dp]
         L.orc_relres_rows.restype = C.c_double
         L.orc_relres_rows.argtypes = [C.c_void_p, dp, dp]
+        L.orc_post.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.orc_solve.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_double, C.c_int, dp]
         L.orc_band_solve.argtypes = [C.c_void_p, dp, C.POINTER(C.c_long)]
         L.orc_gcr.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
@@ -226,6 +227,19 @@ class Oracle:
         r = lib().orc_relres_rows(self.h, _dp(x), _dp(b))
         return r, b
 
+    def post(self, x, stresses=True):
+        """Post-solve pass on level 0 (orc_post): stress and strain-rate planes
+        (6, n0) each (xx yy zz xy xz yz; None without ``stresses``) and the 20
+        norm sums and extrema."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        n0 = self.info[0][4]
+        assert x.size == 4 * n0
+        out = np.zeros(20)
+        sig = np.zeros((6, n0)) if stresses else None
+        eps = np.zeros((6, n0)) if stresses else None
+        lib().orc_post(self.h, _dp(x), _dp(sig) if stresses else None, _dp(eps) if stresses else None, _dp(out))
+        return sig, eps, out
+
     def set_fields(self, nu, ro, nxx, nxy, nxz, nyz):
         """Finest-level node properties given directly (instead of ronurecalc)."""
         n0 = self.info[0][4]
@@ -286,6 +300,28 @@ def from_problem(pb, L=None):
     if rc:
         raise RuntimeError(f"coarse LU failed rc={rc}")
     return o
+
+
+def post_norms(out, divvmax=0.0):
+    """The reference's norms from orc_post's sums (parallel/move3mg.c:3075-3086,
+    3232-3238): maxvxyz = RMS of the three velocity ranges (1e-8 if not
+    positive, DIVVMAX if that is positive); stoksum = RMS momentum error,
+    stoksum1 = RMS of error*h^2/eta over maxvxyz, stoksum2 = RMS of their
+    product, contsum/contsumb = RMS div(v)*h over maxvxyz on continuity cells /
+    cells with a pressure boundary equation, bondsum = RMS boundary error."""
+    dvx, dvy, dvz = out[11] - out[10], out[13] - out[12], out[15] - out[14]
+    mv = ((dvx * dvx + dvy * dvy + dvz * dvz) / 3.0) ** 0.5
+    if mv <= 0:
+        mv = 1e-8
+    if divvmax > 0:
+        mv = divvmax
+    rms = lambda s, n: (s / n) ** 0.5 if n > 0 else 0.0
+    return {"stoksum": rms(out[0], out[3]), "stoksum1": rms(out[1], out[3]) / mv,
+            "stoksum2": rms(out[2], out[3]) / mv, "contsum": rms(out[4], out[5]) / mv,
+            "contsumb": rms(out[6], out[7]) / mv, "bondsum": rms(out[8], out[9]),
+            "stoknum": out[3], "contnum": out[5], "contnumb": out[7], "bondnum": out[9], "maxvxyz": mv,
+            "minvx": out[10], "maxvx": out[11], "minvy": out[12], "maxvy": out[13], "minvz": out[14],
+            "maxvz": out[15], "minpr": out[16], "maxpr": out[17], "minnu": out[18], "maxnu": out[19]}
 
 
 def threads() -> int:

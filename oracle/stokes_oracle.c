@@ -1646,6 +1646,299 @@ int orc_gcr(orc_t *o, const int *pre, const int *post, int mode, double omega, d
   return it;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Post-solve pass (parallel/move3mg.c:2835-3238) on level 0.                 */
+/* Stress values: sxxcalc/syycalc/szzcalc/sxycalc/sxzcalc/syzcalc in value    */
+/* mode (ynval = 0; move3mg.c:3-131, 132-258, 259-387, 388-519, 520-658,      */
+/* 659-795), strain rate in e, effective viscosity in nuo.                    */
+/* ------------------------------------------------------------------------- */
+static double post_sig(const orc_t *o, int kind, long m1, long m2, long m3, const double *x, double *e,
+                       double *nuo) {
+  const int X = o->mgx[0], Y = o->mgy[0], Z = o->mgz[0];
+  const long XY = o->mgxy[0], p = o->mgp[0];
+  const double *gx = o->mggx[0], *gy = o->mggy[0], *gz = o->mggz[0];
+  long v[8];
+  double nueff, val;
+#define VX(i) x[4 * (i) + 1]
+#define VY(i) x[4 * (i) + 2]
+#define VZ(i) x[4 * (i) + 3]
+  if (kind < 3) {
+    /* cell normal stress, v7 = the node itself (move3mg.c:23-30, 85-101) */
+    double xkf = 1.0 / (gx[m1] - gx[m1 - 1]), ykf = 1.0 / (gy[m2] - gy[m2 - 1]), zkf = 1.0 / (gz[m3] - gz[m3 - 1]);
+    v[0] = p + (m3 - 1) * XY + (m1 - 1) * Y + (m2 - 1); v[1] = v[0] + 1; v[2] = v[0] + Y; v[3] = v[2] + 1;
+    v[4] = v[0] + XY; v[5] = v[4] + 1; v[6] = v[4] + Y; v[7] = v[6] + 1;
+    if (o->xper && m1 == X - 1) v[2] = p + (m3 - 1) * XY + m2 - 1;
+    if (o->yper && m2 == Y - 1) v[1] = p + (m3 - 1) * XY + (m1 - 1) * Y;
+    if (o->zper && m3 == Z - 1) v[4] = p + (m1 - 1) * Y + m2 - 1;
+    nueff = o->nxx[v[7]];
+    if (nueff > o->nuend) nueff = o->nuend;
+    if (nueff < o->nubeg) nueff = o->nubeg;
+    double dvxdx = (VX(v[2]) - VX(v[0])) * xkf;
+    double dvydy = (VY(v[1]) - VY(v[0])) * ykf;
+    double dvzdz = (VZ(v[4]) - VZ(v[0])) * zkf;
+    double d = kind == 0 ? dvxdx : kind == 1 ? dvydy : dvzdz;
+    val = d - (dvxdx + dvydy + dvzdz) / 3.0;
+  } else if (kind == 3) {
+    /* sxy at the node (move3mg.c:400-479) */
+    double xkf = (o->xper && m1 == 0) ? 1.0 / (gx[1] - gx[0] + gx[X - 1] - gx[X - 2]) : 1.0 / (gx[m1 + 1] - gx[m1 - 1]);
+    double ykf = (o->yper && m2 == 0) ? 1.0 / (gy[1] - gy[0] + gy[Y - 1] - gy[Y - 2]) : 1.0 / (gy[m2 + 1] - gy[m2 - 1]);
+    v[0] = p + m3 * XY + (m1 - 1) * Y + (m2 - 1); v[1] = v[0] + 1; v[2] = v[0] + Y; v[3] = v[2] + 1;
+    v[7] = v[3] + XY;
+    if (o->xper && m1 == 0) { v[1] = p + m3 * XY + (X - 2) * Y + m2; v[2] = p + m3 * XY + (m2 - 1); v[3] = v[2] + 1; }
+    if (o->yper && m2 == 0) { v[2] = p + m3 * XY + m1 * Y + Y - 2; v[1] = p + m3 * XY + (m1 - 1) * Y; v[3] = v[1] + Y; }
+    if (o->xper && m1 == 0 && o->yper && m2 == 0) { v[1] = p + m3 * XY + (X - 2) * Y; v[2] = p + m3 * XY + Y - 2; v[3] = p + m3 * XY; }
+    nueff = o->nxy[v[3]];
+    if (nueff > o->nuend) nueff = o->nuend;
+    if (nueff < o->nubeg) nueff = o->nubeg;
+    double dvxdy = (VX(v[3]) - VX(v[2])) * ykf;
+    double dvydx = (VY(v[3]) - VY(v[1])) * xkf;
+    val = dvxdy + dvydx;
+  } else if (kind == 4) {
+    /* sxz at the node (move3mg.c:532-615) */
+    double xkf = (o->xper && m1 == 0) ? 1.0 / (gx[1] - gx[0] + gx[X - 1] - gx[X - 2]) : 1.0 / (gx[m1 + 1] - gx[m1 - 1]);
+    double zkf = (o->zper && m3 == 0) ? 1.0 / (gz[1] - gz[0] + gz[Z - 1] - gz[Z - 2]) : 1.0 / (gz[m3 + 1] - gz[m3 - 1]);
+    v[0] = p + (m3 - 1) * XY + (m1 - 1) * Y + m2; v[2] = v[0] + Y; v[4] = v[0] + XY; v[6] = v[4] + Y;
+    if (o->xper && m1 == 0) { v[4] = p + m3 * XY + (X - 2) * Y + m2; v[2] = p + (m3 - 1) * XY + m2; v[6] = v[2] + XY; }
+    if (o->zper && m3 == 0) { v[2] = p + (Z - 2) * XY + m1 * Y + m2; v[4] = p + (m1 - 1) * Y + m2; v[6] = v[4] + Y; }
+    if (o->xper && m1 == 0 && o->zper && m3 == 0) { v[2] = p + (Z - 2) * XY + m2; v[4] = p + (X - 2) * Y + m2; v[6] = p + m2; }
+    nueff = o->nxz[v[6]];
+    if (nueff > o->nuend) nueff = o->nuend;
+    if (nueff < o->nubeg) nueff = o->nubeg;
+    double dvxdz = (VX(v[6]) - VX(v[2])) * zkf;
+    double dvzdx = (VZ(v[6]) - VZ(v[4])) * xkf;
+    val = dvxdz + dvzdx;
+  } else {
+    /* syz at the node (move3mg.c:671-754) */
+    double ykf = (o->yper && m2 == 0) ? 1.0 / (gy[1] - gy[0] + gy[Y - 1] - gy[Y - 2]) : 1.0 / (gy[m2 + 1] - gy[m2 - 1]);
+    double zkf = (o->zper && m3 == 0) ? 1.0 / (gz[1] - gz[0] + gz[Z - 1] - gz[Z - 2]) : 1.0 / (gz[m3 + 1] - gz[m3 - 1]);
+    v[0] = p + (m3 - 1) * XY + m1 * Y + (m2 - 1); v[1] = v[0] + 1; v[4] = v[0] + XY; v[5] = v[4] + 1;
+    if (o->yper && m2 == 0) { v[1] = p + (m3 - 1) * XY + m1 * Y; v[5] = v[1] + XY; v[4] = v[5] + Y - 2; }
+    if (o->zper && m3 == 0) { v[4] = p + m1 * Y + (m2 - 1); v[5] = v[4] + 1; v[1] = v[5] + (Z - 2) * XY; }
+    if (o->yper && m2 == 0 && o->zper && m3 == 0) { v[5] = p + m1 * Y; v[1] = v[5] + (Z - 2) * XY; v[4] = v[5] + Y - 2; }
+    nueff = o->nyz[v[5]];
+    if (nueff > o->nuend) nueff = o->nuend;
+    if (nueff < o->nubeg) nueff = o->nubeg;
+    double dvydz = (VY(v[5]) - VY(v[1])) * zkf;
+    double dvzdy = (VZ(v[5]) - VZ(v[4])) * ykf;
+    val = dvydz + dvzdy;
+  }
+  if (e) *e = val;
+  if (nuo) *nuo = nueff;
+  return 2.0 * nueff * val;
+}
+
+/* which stress planes hold a value at node (m1,m2,m3) and where it is taken
+   from (the periodic copies of the outer line; move3mg.c:2919-3044) */
+static int post_src(const orc_t *o, int kind, long m1, long m2, long m3, long *s1, long *s2, long *s3) {
+  const int X = o->mgx[0], Y = o->mgy[0], Z = o->mgz[0];
+  *s1 = m1; *s2 = m2; *s3 = m3;
+  if (kind < 3) return m1 > 0 && m2 > 0 && m3 > 0;
+  int pa, pb;     /* periodicity of the two in-plane axes */
+  long *ia, *ib;  /* their indices */
+  long na, nb, nc, kc;
+  if (kind == 3) { pa = o->xper; pb = o->yper; ia = s1; ib = s2; na = X; nb = Y; nc = Z; kc = m3; }
+  else if (kind == 4) { pa = o->xper; pb = o->zper; ia = s1; ib = s3; na = X; nb = Z; nc = Y; kc = m2; }
+  else { pa = o->yper; pb = o->zper; ia = s2; ib = s3; na = Y; nb = Z; nc = X; kc = m1; }
+  if (kc >= nc - 1) return 0;
+  if (pa && pb) {
+    if (*ia == na - 1) *ia = 0;
+    if (*ib == nb - 1) *ib = 0;
+    return 1;
+  }
+  if (pa) {
+    if (*ib <= 0 || *ib >= nb - 1) return 0;
+    if (*ia == na - 1) *ia = 0;
+    return 1;
+  }
+  if (pb) {
+    if (*ia <= 0 || *ia >= na - 1) return 0;
+    if (*ib == nb - 1) *ib = 0;
+    return 1;
+  }
+  return *ia > 0 && *ia < na - 1 && *ib > 0 && *ib < nb - 1;
+}
+
+/* the stored stress plane of kind at node n (zero where the recompute writes nothing) */
+static double post_stored(const orc_t *o, int kind, long n, const double *x) {
+  long q = n - o->mgp[0];
+  long m3 = q / o->mgxy[0], m1 = (q % o->mgxy[0]) / o->mgy[0], m2 = q % o->mgy[0];
+  long s1, s2, s3;
+  if (!post_src(o, kind, m1, m2, m3, &s1, &s2, &s3)) return 0.0;
+  return post_sig(o, kind, s1, s2, s3, x, NULL, NULL);
+}
+
+static double nu48(const orc_t *o, const long *lo, const long *mid, const long *hi) {
+  const float *nu = o->nu;
+  double nueff;
+  if (o->viscmod == 1)
+    nueff = exp((log(nu[lo[0]]) + log(nu[lo[1]]) + log(nu[lo[2]]) + log(nu[lo[3]]) +
+                 10.0 * (log(nu[mid[0]]) + log(nu[mid[1]]) + log(nu[mid[2]]) + log(nu[mid[3]])) + log(nu[hi[0]]) +
+                 log(nu[hi[1]]) + log(nu[hi[2]]) + log(nu[hi[3]])) / 48.0);
+  else if (o->viscmod == 2)
+    nueff = 1.0 / ((1.0 / nu[lo[0]] + 1.0 / nu[lo[1]] + 1.0 / nu[lo[2]] + 1.0 / nu[lo[3]] +
+                    10.0 * (1.0 / nu[mid[0]] + 1.0 / nu[mid[1]] + 1.0 / nu[mid[2]] + 1.0 / nu[mid[3]]) +
+                    1.0 / nu[hi[0]] + 1.0 / nu[hi[1]] + 1.0 / nu[hi[2]] + 1.0 / nu[hi[3]]) / 48.0);
+  else
+    nueff = ((double)nu[lo[0]] + nu[lo[1]] + nu[lo[2]] + nu[lo[3]] +
+             10.0 * ((double)nu[mid[0]] + nu[mid[1]] + nu[mid[2]] + nu[mid[3]]) + nu[hi[0]] + nu[hi[1]] + nu[hi[2]] +
+             nu[hi[3]]) / 48.0;
+  if (nueff > o->nuend) nueff = o->nuend;
+  if (nueff < o->nubeg) nueff = o->nubeg;
+  return nueff;
+}
+
+/* x/y/z-Stokes error (xstokserr/ystokserr/zstokserr with ynerr = 1,
+   move3mg.c:796-905, 1011-1125, 1231-1345): the residual of the momentum
+   equation from the stored stresses, and in *eps that residual times
+   h^2/eta (skf1) */
+static double post_stokes(const orc_t *o, int c, long m1, long m2, long m3, const double *x, double *eps) {
+  const int X = o->mgx[0], Y = o->mgy[0], Z = o->mgz[0];
+  const long XY = o->mgxy[0], p = o->mgp[0];
+  const double *gx = o->mggx[0], *gy = o->mggy[0], *gz = o->mggz[0];
+  const float *ro = o->ro;
+  long v[8];
+  double xkf, ykf, zkf, left, right;
+  v[0] = p + m3 * XY + m1 * Y + m2; v[1] = v[0] + 1; v[2] = v[0] + Y; v[3] = v[2] + 1;
+  v[4] = v[0] + XY; v[5] = v[4] + 1; v[6] = v[4] + Y; v[7] = v[6] + 1;
+#define PR(i) x[4 * (i)]
+  if (c == 1) {
+    xkf = 2.0 / (gx[m1 + 1] - gx[m1 - 1]);
+    if (o->xper && m1 == 0) xkf = 2.0 / (gx[1] - gx[0] + gx[X - 1] - gx[X - 2]);
+    ykf = 1.0 / (gy[m2 + 1] - gy[m2]);
+    zkf = 1.0 / (gz[m3 + 1] - gz[m3]);
+    if (o->xper && m1 == 0) v[5] = p + (m3 + 1) * XY + (X - 1) * Y + m2 + 1;
+    right = -o->gxk * (ro[v[0]] + ro[v[1]] + ro[v[4]] + ro[v[5]]) / 4.0;
+    long lo[4] = {v[0] - Y, v[1] - Y, v[4] - Y, v[5] - Y}, mid[4] = {v[0], v[1], v[4], v[5]},
+         hi[4] = {v[2], v[3], v[6], v[7]};
+    for (int k = 0; k < 4; k++) if (lo[k] < 0) lo[k] = 0;  /* the reference reads before its array here */
+    double nueff = nu48(o, lo, mid, hi);
+    double skf1 = 1.0 / nueff * (1.0 / xkf / xkf + 1.0 / ykf / ykf + 1.0 / zkf / zkf) / 3.0;
+    left = (post_stored(o, 0, v[7], x) - post_stored(o, 0, v[5], x)) * xkf;
+    left += (post_stored(o, 3, v[1], x) - post_stored(o, 3, v[0], x)) * ykf;
+    left += (post_stored(o, 4, v[4], x) - post_stored(o, 4, v[0], x)) * zkf;
+    left -= (PR(v[7]) - PR(v[5])) * xkf;
+    left = left - right;
+    *eps = left * skf1;
+    return left;
+  }
+  if (c == 2) {
+    xkf = 1.0 / (gx[m1 + 1] - gx[m1]);
+    ykf = 2.0 / (gy[m2 + 1] - gy[m2 - 1]);
+    if (o->yper && m2 == 0) ykf = 2.0 / (gy[1] - gy[0] + gy[Y - 1] - gy[Y - 2]);
+    zkf = 1.0 / (gz[m3 + 1] - gz[m3]);
+    if (o->yper && m2 == 0) v[6] = p + (m3 + 1) * XY + (m1 + 1) * Y + Y - 1;
+    right = -o->gyk * (ro[v[0]] + ro[v[2]] + ro[v[4]] + ro[v[6]]) / 4.0;
+    long lo[4] = {v[0] - 1, v[2] - 1, v[4] - 1, v[6] - 1}, mid[4] = {v[0], v[2], v[4], v[6]},
+         hi[4] = {v[1], v[3], v[5], v[7]};
+    for (int k = 0; k < 4; k++) if (lo[k] < 0) lo[k] = 0;
+    double nueff = nu48(o, lo, mid, hi);
+    double skf1 = 1.0 / nueff * (1.0 / xkf / xkf + 1.0 / ykf / ykf + 1.0 / zkf / zkf) / 3.0;
+    left = (post_stored(o, 1, v[7], x) - post_stored(o, 1, v[6], x)) * ykf;
+    left += (post_stored(o, 3, v[2], x) - post_stored(o, 3, v[0], x)) * xkf;
+    left += (post_stored(o, 5, v[4], x) - post_stored(o, 5, v[0], x)) * zkf;
+    left -= (PR(v[7]) - PR(v[6])) * ykf;
+    left = left - right;
+    *eps = left * skf1;
+    return left;
+  }
+  xkf = 1.0 / (gx[m1 + 1] - gx[m1]);
+  ykf = 1.0 / (gy[m2 + 1] - gy[m2]);
+  zkf = 2.0 / (gz[m3 + 1] - gz[m3 - 1]);
+  if (o->zper && m3 == 0) zkf = 2.0 / (gz[1] - gz[0] + gz[Z - 1] - gz[Z - 2]);
+  if (o->zper && m3 == 0) v[3] = p + (Z - 1) * XY + (m1 + 1) * Y + m2 + 1;
+  right = -o->gzk * (ro[v[0]] + ro[v[1]] + ro[v[2]] + ro[v[3]]) / 4.0;
+  long lo[4] = {v[0] - XY, v[1] - XY, v[2] - XY, v[3] - XY}, mid[4] = {v[0], v[1], v[2], v[3]},
+       hi[4] = {v[4], v[5], v[6], v[7]};
+  for (int k = 0; k < 4; k++) if (lo[k] < 0) lo[k] = 0;
+  double nueff = nu48(o, lo, mid, hi);
+  double skf1 = 1.0 / nueff * (1.0 / xkf / xkf + 1.0 / ykf / ykf + 1.0 / zkf / zkf) / 3.0;
+  left = (post_stored(o, 2, v[7], x) - post_stored(o, 2, v[3], x)) * zkf;
+  left += (post_stored(o, 4, v[2], x) - post_stored(o, 4, v[0], x)) * xkf;
+  left += (post_stored(o, 5, v[1], x) - post_stored(o, 5, v[0], x)) * ykf;
+  left -= (PR(v[7]) - PR(v[3])) * zkf;
+  left = left - right;
+  *eps = left * skf1;
+  return left;
+#undef PR
+}
+
+void orc_post(const orc_t *o, const double *x, double *sig, double *eps, double *out) {
+  const int X = o->mgx[0], Y = o->mgy[0], Z = o->mgz[0];
+  const double *gx = o->mggx[0], *gy = o->mggy[0], *gz = o->mggz[0];
+  const long n0 = o->mgn[0];
+  double *part = (double *)calloc((size_t)ORC_POSTN * Z, sizeof(double));
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int m3 = 0; m3 < Z; m3++) {
+    double *pp = part + (size_t)ORC_POSTN * m3;
+    for (int q = 10; q < ORC_POSTN; q += 2) { pp[q] = 1e30; pp[q + 1] = -1e30; }
+    for (int m1 = 0; m1 < X; m1++)
+      for (int m2 = 0; m2 < Y; m2++) {
+        long n = NID(o, 0, m1, m2, m3), r = 4 * n;
+        /* stress and strain-rate planes */
+        for (int k = 0; k < 6; k++) {
+          long s1, s2, s3;
+          double e = 0.0, s = 0.0, nuk = 0.0;
+          if (post_src(o, k, m1, m2, m3, &s1, &s2, &s3)) {
+            s = post_sig(o, k, s1, s2, s3, x, &e, &nuk);
+            if (k < 3) { if (nuk < pp[18]) pp[18] = nuk; if (nuk > pp[19]) pp[19] = nuk; }
+          }
+          if (sig) sig[(size_t)k * n0 + (n - o->mgp[0])] = s;
+          if (eps) eps[(size_t)k * n0 + (n - o->mgp[0])] = e;
+        }
+        /* velocity and pressure extrema (move3mg.c:2858-2900) */
+        if (m1 && m2 && m3) { if (x[r] < pp[16]) pp[16] = x[r]; if (x[r] > pp[17]) pp[17] = x[r]; }
+        if (m2 < Y - 1 && m3 < Z - 1) { if (x[r + 1] < pp[10]) pp[10] = x[r + 1]; if (x[r + 1] > pp[11]) pp[11] = x[r + 1]; }
+        if (m1 < X - 1 && m3 < Z - 1) { if (x[r + 2] < pp[12]) pp[12] = x[r + 2]; if (x[r + 2] > pp[13]) pp[13] = x[r + 2]; }
+        if (m1 < X - 1 && m2 < Y - 1) { if (x[r + 3] < pp[14]) pp[14] = x[r + 3]; if (x[r + 3] > pp[15]) pp[15] = x[r + 3]; }
+        /* continuity (continerr ynerr = 1, move3mg.c:1500-1507; sums 3130-3147) */
+        if (m1 && m2 && m3) {
+          double xkf = 1.0 / (gx[m1] - gx[m1 - 1]), ykf = 1.0 / (gy[m2] - gy[m2 - 1]), zkf = 1.0 / (gz[m3] - gz[m3 - 1]);
+          long v[8];
+          v[0] = NID(o, 0, m1 - 1, m2 - 1, m3 - 1); v[1] = v[0] + 1; v[2] = v[0] + Y; v[4] = v[0] + o->mgxy[0];
+          if (o->xper && m1 == X - 1) v[2] = NID(o, 0, 0, m2 - 1, m3 - 1);
+          if (o->yper && m2 == Y - 1) v[1] = NID(o, 0, m1 - 1, 0, m3 - 1);
+          if (o->zper && m3 == Z - 1) v[4] = NID(o, 0, m1 - 1, m2 - 1, 0);
+          double leftc = (x[4 * v[2] + 1] - x[4 * v[0] + 1]) * xkf + (x[4 * v[1] + 2] - x[4 * v[0] + 2]) * ykf +
+                         (x[4 * v[4] + 3] - x[4 * v[0] + 3]) * zkf;
+          double skf = (1.0 / xkf + 1.0 / ykf + 1.0 / zkf) / 3.0;
+          double cv = (leftc - 0.0) * skf;
+          if (!o->bondm[r]) { pp[4] += cv * cv; pp[5] += 1.0; }
+          else { pp[6] += cv * cv; pp[7] += 1.0; }
+        }
+        /* momentum and velocity boundary rows (move3mg.c:3149-3227) */
+        const int inr[4] = {0, m2 < Y - 1 && m3 < Z - 1, m1 < X - 1 && m3 < Z - 1, m1 < X - 1 && m2 < Y - 1};
+        for (int c = 1; c < 4; c++) {
+          if (!inr[c]) continue;
+          long rr = r + c;
+          if (!o->bondm[rr]) {
+            double e1;
+            double iv = post_stokes(o, c, m1, m2, m3, x, &e1);
+            pp[0] += iv * iv; pp[1] += e1 * e1; pp[2] += e1 * e1 * iv * iv; pp[3] += 1.0;
+          } else {
+            int b = o->bondm[rr];
+            double lb = x[rr] - o->bondv1[2 * b + 0];
+            if (o->bondn1[b]) lb -= o->bondv1[2 * b + 1] * x[o->bondn1[b] - 1];
+            pp[8] += lb * lb; pp[9] += 1.0;
+          }
+        }
+      }
+  }
+  for (int q = 0; q < 10; q++) out[q] = 0.0;
+  for (int q = 10; q < ORC_POSTN; q += 2) { out[q] = 1e30; out[q + 1] = -1e30; }
+  for (int m3 = 0; m3 < Z; m3++) {
+    const double *pp = part + (size_t)ORC_POSTN * m3;
+    for (int q = 0; q < 10; q++) out[q] += pp[q];
+    for (int q = 10; q < ORC_POSTN; q += 2) {
+      if (pp[q] < out[q]) out[q] = pp[q];
+      if (pp[q + 1] > out[q + 1]) out[q + 1] = pp[q + 1];
+    }
+  }
+  free(part);
+#undef VX
+#undef VY
+#undef VZ
+}
+
 /* threads the OpenMP loops of this file use (1 without OpenMP) */
 int orc_threads(void) {
 #ifdef _OPENMP

@@ -121,6 +121,20 @@ void orc_hydrostatic(orc_t *o, double *x);
 int orc_gcr(orc_t *o, const int *pre, const int *post, int mode, double omega, double tol, int maxit,
             int restart, int hydro, double *x_out, double *hist, double *seconds_per_it);
 
+/* Post-solve pass of vpiterate on level 0 for a solution x (4 per node):
+ * stress and strain-rate recompute (parallel/move3mg.c:2908-3046) into
+ * sig/eps (6 planes of mgn[0] each: xx yy zz xy xz yz; either may be NULL;
+ * zero outside the reference's ranges) and the sums of the reference's
+ * convergence norms (parallel/move3mg.c:3090-3238) into out[ORC_POSTN]:
+ *   0 sum ival^2 (momentum error)  1 sum eps^2 (error * h^2/eta)  2 sum eps^2 ival^2
+ *   3 momentum rows  4 sum c^2 (div v * h) over cells with a continuity row
+ *   5 those cells  6 sum c^2 over cells with a pressure boundary equation
+ *   7 those cells  8 sum of squared velocity boundary-equation errors  9 those rows
+ *   10..17 min/max of vx, vy, vz, P (move3mg.c:2858-2900)  18/19 min/max
+ *   of the cell viscosity of the normal stresses (move3mg.c:2926-2938). */
+#define ORC_POSTN 20
+void orc_post(const orc_t *o, const double *x, double *sig, double *eps, double *out);
+
 /* banded direct solver restatement (parallel/gausjomp.c:70-254) */
 int orc_band_solve(orc_t *o, double *x, long *band_out);
 

@@ -26,14 +26,29 @@ EXPORTS = [
     "stk_bind", "matclear", "gausmat", "gausmat2", "pardisomat", "gausmat4", "vprestrict", "vpprolong",
     "ronurestrict", "vpiterate", "stk_kop_apply", "stk_kcycle_apply", "stk_schur_apply",
     "stk_solve_batch", "stk_mem_info", "stk_autotune", "stk_level_tile", "stk_set_level_tile", "stk_tile_chains",
+    "stk_norms_eval", "stk_stress_strain",
 ]
+
+# DIVVMIN, STOKSMIN, DIVVMAX, STOKSMAX of the reference's input (parallel/mode.t3c:79-82)
+REF_STOP = dict(divvmin=9e-3, stoksmin=5e1, divvmax=0.0, stoksmax=3e-3)
 
 
 class Knobs(C.Structure):
     _fields_ = [("p0koef", C.c_double), ("p1koef", C.c_double), ("p2koef", C.c_double),
                 ("v0koef", C.c_double), ("v1koef", C.c_double), ("nubeg", C.c_double),
                 ("nuend", C.c_double), ("viscmod", C.c_int), ("gx", C.c_double), ("gy", C.c_double),
-                ("gz", C.c_double)]
+                ("gz", C.c_double), ("divvmin", C.c_double), ("stoksmin", C.c_double), ("divvmax", C.c_double),
+                ("stoksmax", C.c_double)]
+
+
+class Norms(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("stoksum", "stoksum1", "stoksum2", "contsum", "contsumb", "bondsum",
+                                          "stoknum", "contnum", "contnumb", "bondnum", "maxvxyz", "minvx",
+                                          "maxvx", "minvy", "maxvy", "minvz", "maxvz", "minpr", "maxpr",
+                                          "minnu", "maxnu")] + [("converged", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Opts(C.Structure):
@@ -48,7 +63,7 @@ class Opts(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("status", C.c_int), ("relres", C.c_double),
                 ("relres0", C.c_double), ("seconds", C.c_double), ("blocks", C.c_double * 3),
-                ("hist", C.c_double * 256)]
+                ("hist", C.c_double * 256), ("norms", Norms)]
 
 
 _lib = None
@@ -106,6 +121,8 @@ def lib():
         L.stk_tile_chains.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.stk_kcycle_apply.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, dp]
         L.stk_schur_apply.argtypes = [vp, C.POINTER(Opts), dp, dp]
+        L.stk_norms_eval.argtypes = [vp, C.POINTER(Norms)]
+        L.stk_stress_strain.argtypes = [vp, dp, dp]
         L.matclear.argtypes = [C.c_int, dp]
         L.matclear.restype = None
         L.gausmat.argtypes = [C.c_int, dp, dp]
@@ -188,8 +205,10 @@ class Stokes:
             pass
 
     def knobs(self, p0koef=0.1, p1koef=0.0, p2koef=0.1, v0koef=1.0, v1koef=1.0, nubeg=1.0, nuend=1e4,
-              viscmod=0, gravity=(0.0, 0.0, 0.0)):
-        k = Knobs(p0koef, p1koef, p2koef, v0koef, v1koef, nubeg, nuend, viscmod, *gravity)
+              viscmod=0, gravity=(0.0, 0.0, 0.0), divvmin=REF_STOP["divvmin"], stoksmin=REF_STOP["stoksmin"],
+              divvmax=REF_STOP["divvmax"], stoksmax=REF_STOP["stoksmax"]):
+        k = Knobs(p0koef, p1koef, p2koef, v0koef, v1koef, nubeg, nuend, viscmod, *gravity, divvmin, stoksmin,
+                  divvmax, stoksmax)
         check(lib().stk_set_knobs(self.h, C.byref(k)), "stk_set_knobs")
 
     def bc_lines(self, lines):
@@ -247,6 +266,19 @@ class Stokes:
         b = np.zeros(3)
         check(lib().stk_relres(self.h, C.byref(r), _dp(b)), "stk_relres")
         return r.value, b
+
+    def norms(self):
+        """The reference's convergence norms of the current solution (stk_norms_eval)."""
+        nm = Norms()
+        check(lib().stk_norms_eval(self.h, C.byref(nm)), "stk_norms_eval")
+        return nm.as_dict()
+
+    def stress_strain(self):
+        """Stress and strain-rate planes (6, n0) each: xx yy zz xy xz yz."""
+        sig = np.zeros((6, self.n0))
+        eps = np.zeros((6, self.n0))
+        check(lib().stk_stress_strain(self.h, _dp(sig), _dp(eps)), "stk_stress_strain")
+        return sig, eps
 
     def pressure_gauge(self, px, py, pz, pinit=0.0):
         check(lib().stk_pressure_gauge(self.h, px, py, pz, pinit), "stk_pressure_gauge")
@@ -368,7 +400,7 @@ def from_problem(pb, markers=True, knobs=None):
     if knobs:
         k.update(knobs)
     s.knobs(k["p0koef"], k["p1koef"], k["p2koef"], k["v0koef"], k["v1koef"], k["nubeg"], k["nuend"],
-            k["viscmod"], pb.gravity)
+            k["viscmod"], pb.gravity, **{q: k.get(q, v) for q, v in REF_STOP.items()})
     s.bc_lines(pb.bc)
     if markers:
         s.markers_to_nodes(pb.mx, pb.my, pb.mz, pb.mnu, pb.mro)

@@ -20,6 +20,7 @@
 #include "../../include/pardiso2_amd.h"
 #include "stk_kernels.h"
 #include "stk_markers.h"
+#include "stk_post.h"
 #include "stk_schur.h"
 
 #include <hipcub/hipcub.hpp>
@@ -225,6 +226,8 @@ struct stk_ctx {
   std::vector<double> cb;
   double *cinv = nullptr;           // inverse of ca, kept until a row changes
   bool cinv_ok = false;
+  // post-solve pass: block partials of the norms, stress and strain-rate planes (6 n0 each)
+  double *post_part = nullptr, *sig = nullptr, *epsr = nullptr;
 };
 
 namespace {
@@ -1361,7 +1364,7 @@ int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *
   c->bondm.assign(4 * off, 0);
   c->bondv1.assign(2, 0.f);
   c->bondn1.assign(1, 0);
-  c->kn = stk_knobs{0.10, 0.0, 0.10, 1.0, 1.0, 1.0, 1e4, 0, 0.0, 0.0, 0.0};
+  c->kn = stk_knobs{0.10, 0.0, 0.10, 1.0, 1.0, 1.0, 1e4, 0, 0.0, 0.0, 0.0, 9e-3, 5e1, 0.0, 3e-3};
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; return STK_EHIP; }
   hipEventCreate(&c->ev0);
   hipEventCreate(&c->ev1);
@@ -1443,6 +1446,7 @@ void stk_destroy(stk_ctx *c) {
   dfree(c->ainv64); dfree(c->ainvP64); dfree(c->ainv32); dfree(c->cact); dfree(c->cinv);
   dfree(c->X); dfree(c->B); dfree(c->R); dfree(c->W2); dfree(c->T1); dfree(c->red); dfree(c->scal);
   dfree(c->wu); dfree(c->ww); dfree(c->wz);
+  dfree(c->post_part); dfree(c->sig); dfree(c->epsr);
   dfree(c->kred); dfree(c->hcol); dfree(c->gin); dfree(c->gout);
   if (c->hpin) hipHostFree(c->hpin);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
@@ -2016,6 +2020,31 @@ int stk_relres(stk_ctx *c, double *rel, double blk[3]) {
   return STK_OK;
 }
 
+static PostGeo post_geo(const stk_ctx *c);
+static int post_norms(stk_ctx *c, const double *x, stk_norms *out);
+
+int stk_norms_eval(stk_ctx *c, stk_norms *out) {
+  if (!c || !out) return STK_EINVAL;
+  if (!c->prepared || !c->X || !c->B) return STK_ESTATE;
+  if (c->per[0] || c->per[1] || c->per[2]) return STK_EINVAL;
+  return post_norms(c, c->X, out);
+}
+
+int stk_stress_strain(stk_ctx *c, double *sig, double *eps) {
+  if (!c) return STK_EINVAL;
+  if (!c->prepared || !c->X) return STK_ESTATE;
+  if (c->per[0] || c->per[1] || c->per[2]) return STK_EINVAL;
+  Level &F = c->lv[0];
+  const size_t n6 = (size_t)6 * F.n;
+  if (!c->sig && dalloc(&c->sig, n6)) return STK_ENOMEM;
+  if (!c->epsr && dalloc(&c->epsr, n6)) return STK_ENOMEM;
+  hipLaunchKernelGGL(k_post_stress, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, post_geo(c), c->X, c->sig, c->epsr);
+  if (sig) HIPCHK(hipMemcpyAsync(sig, c->sig, n6 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  if (eps) HIPCHK(hipMemcpyAsync(eps, c->epsr, n6 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return STK_OK;
+}
+
 int stk_pressure_gauge(stk_ctx *c, int px, int py, int pz, double pinit) {
   if (!c || !c->X) return STK_EINVAL;
   Level &F = c->lv[0];
@@ -2299,6 +2328,51 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
 }
 
 // allocations a solve needs, and the preconditioner capture when replay is on
+// ---- post-solve pass: stresses and the reference's norms -------------------
+static PostGeo post_geo(const stk_ctx *c) {
+  const Level &F = c->lv[0];
+  PostGeo g;
+  g.X = F.X; g.Y = F.Y; g.Z = F.Z; g.XY = F.XY;
+  g.iAx = F.iA64[0]; g.iBx = F.iB64[0]; g.iAy = F.iA64[1]; g.iBy = F.iB64[1]; g.iAz = F.iA64[2]; g.iBz = F.iB64[2];
+  g.ec = F.e64[0]; g.exy = F.e64[1]; g.exz = F.e64[2]; g.eyz = F.e64[3];
+  g.nu = c->nu;
+  g.nubeg = c->kn.nubeg; g.nuend = c->kn.nuend; g.vm = c->kn.viscmod;
+  return g;
+}
+
+// norms of the finest-level solution x (move3mg.c:3061-3086, 3090-3238, stop test 3282)
+static int post_norms(stk_ctx *c, const double *x, stk_norms *out) {
+  Level &F = c->lv[0];
+  if (!c->post_part && dalloc(&c->post_part, (size_t)(kPostBlocks + 1) * kPostN)) return STK_ENOMEM;
+  const int nb = (int)std::min<long>(kPostBlocks, (F.n + 255) / 256);
+  hipLaunchKernelGGL(k_post_norms, dim3(nb), dim3(256), 0, c->st, post_geo(c), F.flags, x, c->B, F.nbc, F.bc_r,
+                     F.bc_j, F.bc_negk, F.bc_c, c->post_part);
+  double *dout = c->post_part + (size_t)kPostBlocks * kPostN;
+  hipLaunchKernelGGL(k_post_reduce, dim3(1), dim3(64), 0, c->st, c->post_part, nb, dout);
+  double s[kPostN];
+  HIPCHK(hipMemcpyAsync(s, dout, sizeof(s), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  const double dvx = s[11] - s[10], dvy = s[13] - s[12], dvz = s[15] - s[14];
+  double mv = std::sqrt((dvx * dvx + dvy * dvy + dvz * dvz) / 3.0);
+  if (mv <= 0) mv = 1e-8;
+  if (c->kn.divvmax > 0) mv = c->kn.divvmax;
+  auto rms = [](double sum, double n) { return n > 0 ? std::sqrt(sum / n) : 0.0; };
+  stk_norms &N = *out;
+  N.stoksum = rms(s[0], s[3]);
+  N.stoksum1 = rms(s[1], s[3]) / mv;
+  N.stoksum2 = rms(s[2], s[3]) / mv;
+  N.contsum = rms(s[4], s[5]) / mv;
+  N.contsumb = rms(s[6], s[7]) / mv;
+  N.bondsum = rms(s[8], s[9]);
+  N.stoknum = s[3]; N.contnum = s[5]; N.contnumb = s[7]; N.bondnum = s[9];
+  N.maxvxyz = mv;
+  N.minvx = s[10]; N.maxvx = s[11]; N.minvy = s[12]; N.maxvy = s[13]; N.minvz = s[14]; N.maxvz = s[15];
+  N.minpr = s[16]; N.maxpr = s[17]; N.minnu = s[18]; N.maxnu = s[19];
+  const double lim = c->kn.stoksmin * c->kn.stoksmax;
+  N.converged = !(N.stoksum2 > lim || N.stoksum * N.stoksum1 > lim || N.contsum > c->kn.divvmin);
+  return STK_OK;
+}
+
 static int solve_setup(stk_ctx *c, const stk_solver_opts *o, HistLease &lease) {
   const bool fp64 = o->precision == 1;
   int rc = ensure_vectors(c, !fp64, fp64);
@@ -2418,6 +2492,9 @@ int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   S->relres = std::sqrt(s4[0] + s4[1] + s4[2]) / nb;
   S->blocks[0] = std::sqrt(s4[0]); S->blocks[1] = std::sqrt(s4[1]); S->blocks[2] = std::sqrt(s4[2]);
   if (!S->status) S->status = (S->relres <= o->tol * 1.0000001) ? STK_OK : STK_ENOCONV;
+  if (c->per[0] || c->per[1] || c->per[2]) return S->status;
+  rc = post_norms(c, c->X, &S->norms);
+  if (rc) return rc;
   return S->status;
 }
 
