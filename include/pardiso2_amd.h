@@ -39,6 +39,10 @@ extern "C" {
 #define STK_ENAN -6        /* NaN/Inf in the iteration */
 #define STK_ESTATE -7      /* call order violated (e.g. solve before prepare) */
 #define STK_ENOCTX -8      /* reference-named entry point without stk_bind() */
+#define STK_EBCFORM -9     /* an unknown of the outer layer has no boundary equation: the
+                              reference's walls-on-the-outer-lines layout (its free-slip rows
+                              would read nodes outside the grid); give the outer layer
+                              boundary equations (walls on lines 1 and n-2, ghost copies) */
 
 typedef struct stk_ctx stk_ctx;
 
@@ -269,11 +273,50 @@ int stk_schur_apply(stk_ctx *ctx, const stk_solver_opts *opts, const double *r, 
 /* ---- reference entry points (act on the bound context) ---------------- */
 int stk_bind(stk_ctx *ctx);
 
+/* The caller's copies of the reference's global arrays (declared at
+ * parallel/head3mg.c:59-62, 94-112, 196), in the reference's layout and
+ * types: solver vectors and node properties over all levels (node index
+ * mgp[l] + m3*mgxy[l] + m1*mgy[l] + m2, 4 unknowns P,vx,vy,vz per node),
+ * the row store of gausmat2, the boundary tables (bondv1 holds 2 floats per
+ * equation, equation 0 unused) and the count of boundary equations.  The
+ * level-0 outputs of vpiterate's post-solve pass may be NULL. */
+typedef struct {
+  double *sol0, *fre0, *fre1, *val1;
+  int *lin0, *num0, *pos0;
+  float *nu, *ro, *nd;
+  float *nxx, *nxy, *nxz, *nyz;
+  int *bondm;
+  float *bondv1;
+  int *bondn1;
+  long *bondnum;
+  double *pr;
+  float *vx, *vy, *vz;
+  float *sxx, *syy, *szz, *sxy, *sxz, *syz;
+  float *exx, *eyy, *ezz, *exy, *exz, *eyz;
+} stk_ref_arrays;
+/* Bind the caller's arrays (NULL unbinds; the struct is copied, the arrays
+ * are not).  With arrays bound the reference-named entry points below read
+ * and write them in place, as the reference's own routines do its globals:
+ *   gausmat2   stores a row into val1/lin0/num0/pos0 and fre1 (host only);
+ *   gausmat4   sweeps level mgi of that row store on the device (multicolour
+ *              order: vx, vy, vz rows red then black, continuity rows, then
+ *              boundary rows by dependency) and writes sol0 and fre0 back;
+ *   vprestrict, vpprolong, ronurestrict, pardisomat(0)
+ *              move the levels they read and write between the arrays and
+ *              the device around the device kernel;
+ *   vpiterate  loads nu/ro/nxx..nyz of level 0 and the boundary tables,
+ *              prepares, solves from sol0 and writes sol0, pr, vx, vy, vz
+ *              and the stress and strain-rate planes of level 0 back.
+ * gausmat4 .. vpiterate also need a prepared context selected by stk_bind,
+ * built on the same grid, knobs and boundary equations. */
+int stk_bind_arrays(const stk_ref_arrays *arrays);
+
 /* parallel/gaus3mg_MG.c:2 -- clear an rm x (rm+1) dense matrix */
 void matclear(int rm, double *adr);
 /* parallel/gaus3mg_MG.c:21 -- dense Gauss elimination, no pivoting */
 void gausmat(int rm, double *adr, double *adr1);
-/* parallel/gaus3mg_MG.c:311 -- store one assembled row (un1/ui1 format) */
+/* parallel/gaus3mg_MG.c:311 -- store one assembled row (un1/ui1 format) in
+ * the bound arrays (STK_ENOCTX without them) */
 int gausmat2(int am, long mcmax, long pos0cur1, long *un1, double *ui1);
 /* parallel/gaus3mg_MG.c:64 -- am=1: set one coarsest-level row from its
  * list (duplicate columns add up; a later call for the row replaces it);
@@ -281,7 +324,9 @@ int gausmat2(int am, long mcmax, long pos0cur1, long *un1, double *ui1);
  * solution at sol0[4*(mgp[L]+node)+c] of the bound context. */
 int pardisomat(int am, long mcmax, long mcmax2, long pos0cur1, int tid, long *un1, double *ui1);
 /* parallel/move3mg.c:3855 (prototype parallel/head3mg.c:350-377) -- am
- * smoother sweeps on level mgi, then fre0 = fre1 - A sol0 on [mcmin,mcmax] */
+ * smoother sweeps on level mgi, then fre0 = fre1 - A sol0 on [mcmin,mcmax]:
+ * over the bound arrays' row store if arrays are bound, else over the bound
+ * context's matrix-free rows and its own sol0/fre1/fre0 copies */
 int gausmat4(int am, long mcmax, long mcmin, int mgi);
 /* parallel/move3mg.c:3966 -- residual restriction to level mgi */
 void vprestrict(int mgi);

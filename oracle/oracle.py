@@ -52,6 +52,7 @@ def lib():
         L.orc_assemble.argtypes = [C.c_void_p]
         L.orc_get_row.argtypes = [C.c_void_p, C.c_long, ip, dp, dp]
         L.orc_gausmat4.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_gausmat4_colored.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_residual.argtypes = [C.c_void_p, C.c_int]
         L.orc_sweep_tiles.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_eq_box.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
@@ -179,6 +180,12 @@ class Oracle:
 
     def gausmat4(self, am, l):
         lib().orc_gausmat4(self.h, am, l)
+
+    def gausmat4_colored(self, am, l):
+        """gausmat4 in the device's multicolour order over the row store."""
+        rc = lib().orc_gausmat4_colored(self.h, am, l)
+        if rc:
+            raise RuntimeError("boundary rows form a cycle")
 
     def residual(self, l):
         lib().orc_residual(self.h, l)

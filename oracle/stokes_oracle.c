@@ -949,6 +949,59 @@ void orc_gausmat4(orc_t *o, int am, int l) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* gausmat4 over the assembled row store in the multicolour order the device  */
+/* runs on a caller's arrays: per sweep the momentum rows of vx, vy, vz, each */
+/* red then black by (m1+m2+m3)&1, then all continuity rows, then the         */
+/* boundary rows in dependency order (a row after the boundary row it copies). */
+/* Rows of one phase read no unknown of the same phase (momentum rows couple */
+/* face neighbours, continuity rows only velocities), so any order inside a  */
+/* phase gives the same result; then fre0 = fre1 - A sol0 (move3mg.c:3935-3957). */
+/* ------------------------------------------------------------------------- */
+static int bc_row_depth(const orc_t *o, long r, int lim) {
+  int d = 0;
+  while (lim-- > 0) {
+    long j = -1;
+    for (int k = 0; k < o->num0[r]; k++)
+      if (o->lin0[o->pos0[r] + k] != r) j = o->lin0[o->pos0[r] + k];
+    if (j < 0 || !o->bondm[j]) return d;
+    r = j;
+    d++;
+  }
+  return -1;
+}
+
+int orc_gausmat4_colored(orc_t *o, int am, int l) {
+  const long r0 = 4 * o->mgp[l], r1 = 4 * (o->mgp[l] + o->mgn[l]);
+  int maxd = 0;
+  for (long r = r0; r < r1; r++)
+    if (o->num0[r] && o->bondm[r]) {
+      int d = bc_row_depth(o, r, 64);
+      if (d < 0) return -1;
+      if (d > maxd) maxd = d;
+    }
+  for (int s = 0; s < am; s++) {
+    for (int c = 1; c < 4; c++)
+      for (int col = 0; col < 2; col++)
+        for (int m3 = 0; m3 < o->mgz[l]; m3++)
+          for (int m1 = 0; m1 < o->mgx[l]; m1++)
+            for (int m2 = 0; m2 < o->mgy[l]; m2++) {
+              if (((m1 + m2 + m3) & 1) != col) continue;
+              long r = 4 * NID(o, l, m1, m2, m3) + c;
+              if (o->num0[r] && !o->bondm[r]) row_update(o, r, o->sol0);
+            }
+    if (!o->vonly)
+      for (long r = r0; r < r1; r += 4)
+        if (o->num0[r] && !o->bondm[r]) row_update(o, r, o->sol0);
+    for (int d = 0; d <= maxd; d++)
+      for (long r = r0; r < r1; r++)
+        if (o->num0[r] && o->bondm[r] && !(o->vonly && !(r & 3)) && bc_row_depth(o, r, 64) == d)
+          row_update(o, r, o->sol0);
+  }
+  orc_residual(o, l);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Coloured tile-hybrid sweep: the same gausmat4 row updates in the order the */
 /* GPU smoother uses.  Nodes are grouped in ty x tx x tz tiles (y fastest)   */
 /* laid from the lower corner of the equation-row box (orc_eq_box).          */
@@ -1312,6 +1365,7 @@ int orc_coarse_solve(orc_t *o) {
 static void smooth(orc_t *o, int am, int l, int mode, int ty, int tx, int tz) {
   if (am <= 0) { orc_residual(o, l); return; }
   if (mode == 0) orc_gausmat4(o, am, l);
+  else if (mode == 2) orc_gausmat4_colored(o, am, l);
   else orc_sweep_tiles(o, am, l, ty, tx, tz);
 }
 

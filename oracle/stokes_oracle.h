@@ -107,6 +107,11 @@ int orc_coarse_solve(orc_t *o);
 int orc_dense_lu(int n, double *a, int *piv);
 void orc_dense_lu_solve(int n, const double *lu, const int *piv, double *b);
 
+/* gausmat4 in the multicolour order of the device's sweep over a caller's
+ * row store (velocity rows by component and colour, continuity rows,
+ * boundary rows by dependency depth); -1 if boundary rows form a cycle */
+int orc_gausmat4_colored(orc_t *o, int am, int l);
+
 /* V-cycle (canonical shape of mg/move3mgold.c:2486-2600, fixes for K1/K2) */
 void orc_vcycle(orc_t *o, const int *pre, const int *post, int mode, int ty, int tx, int tz);
 double orc_relres(orc_t *o, double *blocks3);

@@ -15,7 +15,8 @@ LIBPATH = os.path.join(HERE, "libpardiso2_amd.so")
 
 ERRORS = {0: "ok", -1: "invalid argument", -2: "device out of memory", -3: "HIP error",
           -4: "coarsest matrix singular", -5: "not converged", -6: "NaN/Inf",
-          -7: "call order", -8: "no bound context"}
+          -7: "call order", -8: "no bound context",
+          -9: "an outer-layer unknown has no boundary equation (walls on the outer lines)"}
 
 EXPORTS = [
     "stk_create", "stk_destroy", "stk_level_info", "stk_set_knobs", "stk_default_opts", "stk_bc_line",
@@ -26,7 +27,7 @@ EXPORTS = [
     "stk_bind", "matclear", "gausmat", "gausmat2", "pardisomat", "gausmat4", "vprestrict", "vpprolong",
     "ronurestrict", "vpiterate", "stk_kop_apply", "stk_kcycle_apply", "stk_schur_apply",
     "stk_solve_batch", "stk_mem_info", "stk_autotune", "stk_level_tile", "stk_set_level_tile", "stk_tile_chains",
-    "stk_norms_eval", "stk_stress_strain",
+    "stk_norms_eval", "stk_stress_strain", "stk_bind_arrays",
 ]
 
 # DIVVMIN, STOKSMIN, DIVVMAX, STOKSMAX of the reference's input (parallel/mode.t3c:79-82)
@@ -49,6 +50,43 @@ class Norms(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _ref_fields():
+    dp, fp, ip = C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_int)
+    return ([(k, dp) for k in ("sol0", "fre0", "fre1", "val1")] + [(k, ip) for k in ("lin0", "num0", "pos0")] +
+            [(k, fp) for k in ("nu", "ro", "nd", "nxx", "nxy", "nxz", "nyz")] +
+            [("bondm", ip), ("bondv1", fp), ("bondn1", ip), ("bondnum", C.POINTER(C.c_long)), ("pr", dp)] +
+            [(k, fp) for k in ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz", "exx", "eyy", "ezz",
+                               "exy", "exz", "eyz")])
+
+
+class RefArrays(C.Structure):
+    """stk_ref_arrays: pointers to a caller's copies of the reference's globals."""
+    _fields_ = _ref_fields()
+
+
+def bind_arrays(arrays):
+    """Bind a dict of numpy arrays (names of stk_ref_arrays, reference dtypes:
+    float64 for sol0/fre0/fre1/val1/pr, int32 for lin0/num0/pos0/bondm/bondn1,
+    float32 for the node properties, bondv1 and the post-solve outputs, int64
+    for bondnum) as the caller's reference arrays; None unbinds.  The arrays
+    must stay alive while bound."""
+    if arrays is None:
+        check(lib().stk_bind_arrays(None), "stk_bind_arrays")
+        return None
+    want = {C.POINTER(C.c_double): np.float64, C.POINTER(C.c_float): np.float32, C.POINTER(C.c_int): np.int32,
+            C.POINTER(C.c_long): np.int64}
+    ra = RefArrays()
+    for name, typ in RefArrays._fields_:
+        a = arrays.get(name)
+        if a is None:
+            continue
+        if a.dtype != want[typ] or not a.flags.c_contiguous:
+            raise ValueError(f"{name}: need a contiguous {np.dtype(want[typ])} array")
+        setattr(ra, name, a.ctypes.data_as(typ))
+    check(lib().stk_bind_arrays(C.byref(ra)), "stk_bind_arrays")
+    return ra
 
 
 class Opts(C.Structure):
@@ -122,6 +160,7 @@ def lib():
         L.stk_kcycle_apply.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, dp]
         L.stk_schur_apply.argtypes = [vp, C.POINTER(Opts), dp, dp]
         L.stk_norms_eval.argtypes = [vp, C.POINTER(Norms)]
+        L.stk_bind_arrays.argtypes = [C.POINTER(RefArrays)]
         L.stk_stress_strain.argtypes = [vp, dp, dp]
         L.matclear.argtypes = [C.c_int, dp]
         L.matclear.restype = None

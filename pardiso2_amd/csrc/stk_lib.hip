@@ -19,6 +19,7 @@
 
 #include "../../include/pardiso2_amd.h"
 #include "stk_kernels.h"
+#include "stk_compat.h"
 #include "stk_markers.h"
 #include "stk_post.h"
 #include "stk_schur.h"
@@ -148,6 +149,11 @@ struct Level {
   bool uni = false;
   float ua[3] = {0, 0, 0}, ub[3] = {0, 0, 0};  // the uniform iA, iB per axis (FP32)
   FluxU fu{};
+  // a caller's row store of this level (gausmat4 on bound arrays): ELL rows, state, boundary rows by depth
+  int *ell_cnt = nullptr, *ell_col = nullptr, *ell_bcrows = nullptr;
+  double *ell_val = nullptr, *ell_x = nullptr, *ell_f = nullptr, *ell_f0 = nullptr;
+  unsigned char *ell_isbc = nullptr;
+  float *ell_nd = nullptr;
 };
 
 template <>
@@ -233,6 +239,8 @@ struct stk_ctx {
 namespace {
 
 stk_ctx *g_bound = nullptr;
+stk_ref_arrays g_arr{};  // the caller's reference arrays (stk_bind_arrays)
+bool g_arr_set = false;
 
 // the level whose FP32 stencil viscosities a Geo names
 const Level *level_of(const stk_ctx *c, const Geo<float> &g) {
@@ -1438,6 +1446,8 @@ void stk_destroy(stk_ctx *c) {
     for (int q = 0; q < 7; q++) { dfree(L.k64[q]); dfree(L.k32[q]); }
     dfree(L.kc64); dfree(L.kc32);
     dfree(L.kx32); dfree(L.kb32); dfree(L.kr32); dfree(L.kx64); dfree(L.kb64); dfree(L.kr64);
+    dfree(L.ell_cnt); dfree(L.ell_col); dfree(L.ell_bcrows); dfree(L.ell_val); dfree(L.ell_x); dfree(L.ell_f);
+    dfree(L.ell_f0); dfree(L.ell_isbc); dfree(L.ell_nd);
     for (int a = 0; a < 3; a++) { dfree(L.klo[a]); dfree(L.kcnt[a]); dfree(L.kpar[a]); }
   }
   dfree(c->dinv64); dfree(c->bfy64); dfree(c->bfw64); dfree(c->dinv32); dfree(c->bfy32); dfree(c->bfw32);
@@ -1830,7 +1840,7 @@ static int build_rows(stk_ctx *c) {
               if (!eq_inside(L, m1, m2, m3, cc)) {
                 fprintf(stderr, "pardiso2_amd: level %d unknown (%d,%d,%d,%d) needs a boundary equation\n", l, m1,
                         m2, m3, cc);
-                return STK_EINVAL;
+                return STK_EBCFORM;
               }
               f |= KIND_EQ << (2 * cc);
             }
@@ -2749,6 +2759,18 @@ int stk_bind(stk_ctx *c) {
   return STK_OK;
 }
 
+int stk_bind_arrays(const stk_ref_arrays *a) {
+  if (!a) {
+    g_arr = stk_ref_arrays{};
+    g_arr_set = false;
+    return STK_OK;
+  }
+  if (!a->sol0 || !a->fre0 || !a->fre1 || !a->val1 || !a->lin0 || !a->num0 || !a->pos0) return STK_EINVAL;
+  g_arr = *a;
+  g_arr_set = true;
+  return STK_OK;
+}
+
 // ---- reference-named entry points ----------------------------------------
 
 // parallel/gaus3mg_MG.c:2-17
@@ -2784,28 +2806,20 @@ void gausmat(int rm, double *adr, double *adr1) {
   }
 }
 
-// parallel/gaus3mg_MG.c:311-365: store the row in the bound context's
-// ELL row store (57 slots per node) and its right-hand side in fre1
+// parallel/gaus3mg_MG.c:311-365: store the row in the caller's row store
+// (fre1[mcmax] = ui1[0]; entries from position pos0cur1 on)
 int gausmat2(int am, long mcmax, long pos0cur1, long *un1, double *ui1) {
   (void)am;
-  stk_ctx *c = g_bound;
-  if (!c) return STK_ENOCTX;
-  if (mcmax < 0 || mcmax >= 4 * c->ntot || pos0cur1 < 0 || pos0cur1 + un1[0] > 57 * c->ntot) return STK_EINVAL;
-  if (c->hval1.empty()) {
-    c->hval1.assign(57 * c->ntot, 0.0);
-    c->hlin0.assign(57 * c->ntot, 0);
-    c->hnum0.assign(4 * c->ntot, 0);
-    c->hpos0.assign(4 * c->ntot, 0);
-    c->hfre1.assign(4 * c->ntot, 0.0);
-  }
-  c->hfre1[mcmax] = ui1[0];
-  c->hpos0[mcmax] = (int)pos0cur1;
-  c->hnum0[mcmax] = 0;
+  if (!g_arr_set) return STK_ENOCTX;
+  if (mcmax < 0 || pos0cur1 < 0 || !un1 || !ui1) return STK_EINVAL;
+  g_arr.fre1[mcmax] = ui1[0];
+  g_arr.pos0[mcmax] = (int)pos0cur1;
+  g_arr.num0[mcmax] = 0;
   for (long m2 = 1; m2 <= un1[0]; m2++) {
-    c->hlin0[pos0cur1] = (int)un1[m2];
-    c->hval1[pos0cur1] = ui1[m2];
+    g_arr.lin0[pos0cur1] = (int)un1[m2];
+    g_arr.val1[pos0cur1] = ui1[m2];
     pos0cur1++;
-    c->hnum0[mcmax]++;
+    g_arr.num0[mcmax]++;
   }
   return 0;
 }
@@ -2841,6 +2855,7 @@ int pardisomat(int am, long mcmax, long mcmax2, long pos0cur1, int tid, long *un
     int rc = ensure_ref(c, c->L);
     if (!rc) {
       HIPCHK(hipMemcpyAsync(C.ref[0], dx, (size_t)n * 8, hipMemcpyDeviceToDevice, c->st));
+      if (g_arr_set) HIPCHK(hipMemcpyAsync(g_arr.sol0 + 4 * C.off, dx, (size_t)n * 8, hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
     }
     dfree(db); dfree(dx);
@@ -2868,19 +2883,136 @@ int pardisomat(int am, long mcmax, long mcmax2, long pos0cur1, int tid, long *un
   return 0;
 }
 
-// parallel/move3mg.c:3855-3960 on the bound context: sweeps on level mgi,
-// then fre0 = fre1 - A sol0 (rows [mcmin, mcmax] are the level's rows)
+// gausmat4 on the caller's row store of level l (move3mg.c:3855-3960):
+// rows gathered into ELL form, am multicolour sweeps on the device, then
+// fre0 = fre1 - A sol0 on the rows [mcmin, mcmax] of the level
+static int ell_gausmat4(stk_ctx *c, int am, long mcmax, long mcmin, int l) {
+  Level &L = c->lv[l];
+  const long n4 = 4 * L.n, base = 4 * L.off;
+  std::vector<int> cnt(n4, 0), col((size_t)n4 * kEllMax, 0);
+  std::vector<double> val((size_t)n4 * kEllMax, 0.0);
+  std::vector<unsigned char> isbc(n4, 0);
+  for (long r = 0; r < n4; r++) {
+    const long R = base + r;
+    const int k = g_arr.num0[R];
+    if (k < 0 || k > kEllMax) return STK_EINVAL;
+    cnt[r] = k;
+    isbc[r] = g_arr.bondm && g_arr.bondm[R] != 0;
+    for (int q = 0; q < k; q++) {
+      const long j = (long)g_arr.lin0[g_arr.pos0[R] + q] - base;
+      if (j < 0 || j >= n4) return STK_EINVAL;  // a row of this level reads only this level
+      col[(size_t)r * kEllMax + q] = (int)j;
+      val[(size_t)r * kEllMax + q] = g_arr.val1[g_arr.pos0[R] + q];
+    }
+  }
+  // boundary rows by dependency depth: a row after the boundary row it copies
+  auto partner = [&](long r) -> long {
+    long j = -1;
+    for (int q = 0; q < cnt[r]; q++)
+      if (col[(size_t)r * kEllMax + q] != r) j = col[(size_t)r * kEllMax + q];
+    return j;
+  };
+  std::vector<int> depth(n4, -1);
+  int maxd = -1;
+  for (long r = 0; r < n4; r++) {
+    if (!cnt[r] || !isbc[r]) continue;
+    int d = 0;
+    long q = r;
+    for (;;) {
+      const long j = partner(q);
+      if (j < 0 || !isbc[j] || !cnt[j]) break;
+      if (++d > 64) return STK_EINVAL;  // boundary rows copying each other in a cycle
+      q = j;
+    }
+    depth[r] = d;
+    maxd = std::max(maxd, d);
+  }
+  std::vector<int> bcrows, bcoff(1, 0);
+  for (int d = 0; d <= maxd; d++) {
+    for (long r = 0; r < n4; r++)
+      if (depth[r] == d) bcrows.push_back((int)r);
+    bcoff.push_back((int)bcrows.size());
+  }
+  if (!L.ell_cnt) {
+    if (dalloc(&L.ell_cnt, n4) || dalloc(&L.ell_col, (size_t)n4 * kEllMax) || dalloc(&L.ell_bcrows, n4) ||
+        dalloc(&L.ell_val, (size_t)n4 * kEllMax) || dalloc(&L.ell_x, n4) || dalloc(&L.ell_f, n4) ||
+        dalloc(&L.ell_f0, n4) || dalloc(&L.ell_isbc, n4) || dalloc(&L.ell_nd, L.n))
+      return STK_ENOMEM;
+  }
+  H2D(c, L.ell_cnt, cnt.data(), n4 * sizeof(int));
+  H2D(c, L.ell_col, col.data(), col.size() * sizeof(int));
+  H2D(c, L.ell_val, val.data(), val.size() * sizeof(double));
+  H2D(c, L.ell_isbc, isbc.data(), n4);
+  if (!bcrows.empty()) H2D(c, L.ell_bcrows, bcrows.data(), bcrows.size() * sizeof(int));
+  H2D(c, L.ell_x, g_arr.sol0 + base, n4 * 8);
+  H2D(c, L.ell_f, g_arr.fre1 + base, n4 * 8);
+  H2D(c, L.ell_f0, g_arr.fre0 + base, n4 * 8);
+  if (g_arr.nd) H2D(c, L.ell_nd, g_arr.nd + L.off, L.n * sizeof(float));
+  else HIPCHK(hipMemsetAsync(L.ell_nd, 0, L.n * sizeof(float), c->st));
+  const unsigned nb = blocks(L.n, 256);
+  for (int sw = 0; sw < am; sw++) {
+    for (int cc = 1; cc < 4; cc++)
+      for (int colr = 0; colr < 2; colr++)
+        hipLaunchKernelGGL(k_ell_phase, dim3(nb), dim3(256), 0, c->st, L.n, L.Y, L.XY, cc, colr, L.ell_cnt,
+                           L.ell_col, L.ell_val, L.ell_f, L.ell_isbc, c->kn.v0koef, L.ell_x);
+    hipLaunchKernelGGL(k_ell_pphase, dim3(nb), dim3(256), 0, c->st, L.n, L.ell_cnt, L.ell_col, L.ell_val, L.ell_f,
+                       L.ell_isbc, L.ell_nd, c->kn.p0koef, L.ell_x);
+    for (int d = 0; d <= maxd; d++) {
+      const int nr = bcoff[d + 1] - bcoff[d];
+      if (nr)
+        hipLaunchKernelGGL(k_ell_bc, dim3(blocks(nr, 256)), dim3(256), 0, c->st, nr, L.ell_bcrows + bcoff[d],
+                           L.ell_cnt, L.ell_col, L.ell_val, L.ell_f, L.ell_x);
+    }
+  }
+  const long r0 = std::max(mcmin - base, 0L), r1 = std::min(mcmax - base + 1, n4);
+  if (r1 > r0)
+    hipLaunchKernelGGL(k_ell_res, dim3(blocks(r1 - r0, 256)), dim3(256), 0, c->st, r0, r1, L.ell_cnt, L.ell_col,
+                       L.ell_val, L.ell_f, L.ell_x, L.ell_f0);
+  HIPCHK(hipGetLastError());
+  D2H(c, g_arr.sol0 + base, L.ell_x, n4 * 8);
+  D2H(c, g_arr.fre0 + base, L.ell_f0, n4 * 8);
+  return STK_OK;
+}
+
+// level l of a bound all-levels array <-> the context's copy of it
+static int arr_up(stk_ctx *c, int which, int l, const double *h) {
+  int rc = ensure_ref(c, l);
+  if (rc) return rc;
+  H2D(c, c->lv[l].ref[which], h + 4 * c->lv[l].off, 4 * c->lv[l].n * 8);
+  return STK_OK;
+}
+static int arr_down(stk_ctx *c, int which, int l, double *h) {
+  D2H(c, h + 4 * c->lv[l].off, c->lv[l].ref[which], 4 * c->lv[l].n * 8);
+  return STK_OK;
+}
+
+// parallel/move3mg.c:3855-3960: sweeps on level mgi, then fre0 = fre1 - A sol0
+// (rows [mcmin, mcmax]): over the bound arrays' row store, else the bound
+// context's matrix-free rows on its own copies
 int gausmat4(int am, long mcmax, long mcmin, int mgi) {
-  (void)mcmax; (void)mcmin;
   stk_ctx *c = g_bound;
   if (!c) return STK_ENOCTX;
+  if (mgi < 0 || mgi > c->L || am < 0) return STK_EINVAL;
+  if (g_arr_set) return ell_gausmat4(c, am, mcmax, mcmin, mgi);
   return stk_smooth(c, mgi, am, kTileDefault[0], kTileDefault[1], kTileDefault[2]);
+}
+
+static int vprestrict_arrays(stk_ctx *c, int mgi) {
+  int rc = arr_up(c, 2, mgi - 1, g_arr.fre0) || arr_up(c, 1, mgi, g_arr.fre1) || arr_up(c, 0, mgi, g_arr.sol0);
+  if (rc) return STK_ENOMEM;
+  restrict_to<double>(c, mgi, c->lv[mgi - 1].ref[2], c->lv[mgi].ref[1], c->lv[mgi].ref[0], 1);
+  HIPCHK(hipGetLastError());
+  return arr_down(c, 1, mgi, g_arr.fre1) || arr_down(c, 0, mgi, g_arr.sol0);
 }
 
 // parallel/move3mg.c:3966-4354
 void vprestrict(int mgi) {
   stk_ctx *c = g_bound;
   if (!c || !c->prepared || mgi < 1 || mgi > c->L) return;
+  if (g_arr_set) {
+    (void)vprestrict_arrays(c, mgi);
+    return;
+  }
   if (ensure_ref(c, mgi) || ensure_ref(c, mgi - 1)) return;
   restrict_to<double>(c, mgi, c->lv[mgi - 1].ref[2], c->lv[mgi].ref[1], c->lv[mgi].ref[0], 1);
   hipStreamSynchronize(c->st);
@@ -2899,36 +3031,107 @@ int stk_vrestrict32(stk_ctx *c, int l) {
   return hipGetLastError() == hipSuccess ? STK_OK : STK_EHIP;
 }
 
+static int vpprolong_arrays(stk_ctx *c, int mgi, int ynreset) {
+  int rc = arr_up(c, 0, mgi, g_arr.sol0) || arr_up(c, 0, mgi - 1, g_arr.sol0);
+  if (rc) return STK_ENOMEM;
+  prolong_from<double>(c, mgi, c->lv[mgi].ref[0], c->lv[mgi - 1].ref[0], 1, ynreset);
+  HIPCHK(hipGetLastError());
+  return arr_down(c, 0, mgi - 1, g_arr.sol0);
+}
+
 // parallel/move3mg.c:3535-3850
 void vpprolong(int mgi, int ynreset) {
   stk_ctx *c = g_bound;
   if (!c || !c->prepared || mgi < 1 || mgi > c->L) return;
+  if (g_arr_set) {
+    (void)vpprolong_arrays(c, mgi, ynreset);
+    return;
+  }
   if (ensure_ref(c, mgi) || ensure_ref(c, mgi - 1)) return;
   prolong_from<double>(c, mgi, c->lv[mgi].ref[0], c->lv[mgi - 1].ref[0], 1, ynreset);
   hipStreamSynchronize(c->st);
+}
+
+static int ronurestrict_run(stk_ctx *c, int mgi) {
+  Level &F = c->lv[mgi - 1], &C = c->lv[mgi];
+  const bool arr = g_arr_set && g_arr.nu && g_arr.ro;
+  if (arr) {
+    H2D(c, c->nu + F.off, g_arr.nu + F.off, F.n * sizeof(float));
+    H2D(c, c->ro + F.off, g_arr.ro + F.off, F.n * sizeof(float));
+  }
+  CompTab t;
+  for (int a = 0; a < 3; a++) t.ax[a] = C.ax[3][a].tab();
+  hipLaunchKernelGGL(k_ronurestrict, dim3(blocks(C.n, 256)), dim3(256), 0, c->st, F.X, F.Y, C.X, C.Y, C.Z, t,
+                     c->nu + F.off, c->ro + F.off, c->nu + C.off, c->ro + C.off, c->kn.viscmod);
+  HIPCHK(hipGetLastError());
+  if (arr) {
+    D2H(c, g_arr.nu + C.off, c->nu + C.off, C.n * sizeof(float));
+    D2H(c, g_arr.ro + C.off, c->ro + C.off, C.n * sizeof(float));
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  return STK_OK;
 }
 
 // parallel/move3mg.c:3387-3528
 void ronurestrict(int mgi) {
   stk_ctx *c = g_bound;
   if (!c || mgi < 1 || mgi > c->L) return;
-  Level &F = c->lv[mgi - 1], &C = c->lv[mgi];
-  CompTab t;
-  for (int a = 0; a < 3; a++) t.ax[a] = C.ax[3][a].tab();
-  hipLaunchKernelGGL(k_ronurestrict, dim3(blocks(C.n, 256)), dim3(256), 0, c->st, F.X, F.Y, C.X, C.Y, C.Z, t,
-                     c->nu + F.off, c->ro + F.off, c->nu + C.off, c->ro + C.off, c->kn.viscmod);
-  hipStreamSynchronize(c->st);
+  (void)ronurestrict_run(c, mgi);
 }
 
-// parallel/move3mg.c:1791-3382: prepare and solve the bound context
+// the reference's post-solve reload into the bound arrays (move3mg.c:2858-2900
+// without the pressure shift, then the stress recompute 2908-3046)
+static int vpiterate_outputs(stk_ctx *c) {
+  Level &F = c->lv[0];
+  const double *x = g_arr.sol0;
+  for (int m3 = 0; m3 < F.Z; m3++)
+    for (int m1 = 0; m1 < F.X; m1++)
+      for (int m2 = 0; m2 < F.Y; m2++) {
+        const long n = (long)m3 * F.XY + (long)m1 * F.Y + m2;
+        if (g_arr.pr && m1 && m2 && m3) g_arr.pr[n] = x[4 * n];
+        if (g_arr.vx && m2 < F.Y - 1 && m3 < F.Z - 1) g_arr.vx[n] = (float)x[4 * n + 1];
+        if (g_arr.vy && m1 < F.X - 1 && m3 < F.Z - 1) g_arr.vy[n] = (float)x[4 * n + 2];
+        if (g_arr.vz && m1 < F.X - 1 && m2 < F.Y - 1) g_arr.vz[n] = (float)x[4 * n + 3];
+      }
+  float *sp[6] = {g_arr.sxx, g_arr.syy, g_arr.szz, g_arr.sxy, g_arr.sxz, g_arr.syz};
+  float *ep[6] = {g_arr.exx, g_arr.eyy, g_arr.ezz, g_arr.exy, g_arr.exz, g_arr.eyz};
+  bool any = false;
+  for (int k = 0; k < 6; k++) any = any || sp[k] || ep[k];
+  if (!any || c->per[0] || c->per[1] || c->per[2]) return STK_OK;
+  std::vector<double> sig(6 * (size_t)F.n), eps(6 * (size_t)F.n);
+  int rc = stk_stress_strain(c, sig.data(), eps.data());
+  if (rc) return rc;
+  for (int k = 0; k < 6; k++)
+    for (long n = 0; n < F.n; n++) {
+      if (sp[k]) sp[k][n] = (float)sig[(size_t)k * F.n + n];
+      if (ep[k]) ep[k][n] = (float)eps[(size_t)k * F.n + n];
+    }
+  return STK_OK;
+}
+
+// parallel/move3mg.c:1791-3382: prepare and solve the bound context; with
+// bound arrays from their level-0 fields, boundary tables and sol0
 void vpiterate(int m0, int f0) {
   (void)m0; (void)f0;
   stk_ctx *c = g_bound;
   if (!c) return;
-  if (!c->prepared && stk_prepare(c)) return;
+  if (g_arr_set) {
+    const long n0 = c->lv[0].n;
+    if (g_arr.nu && g_arr.ro && g_arr.nxx && g_arr.nxy && g_arr.nxz && g_arr.nyz &&
+        stk_set_fields(c, g_arr.nu, g_arr.ro, g_arr.nxx, g_arr.nxy, g_arr.nxz, g_arr.nyz))
+      return;
+    if (g_arr.bondm && g_arr.bondv1 && g_arr.bondn1 && g_arr.bondnum &&
+        stk_set_bc_tables(c, g_arr.bondm, (int)*g_arr.bondnum, g_arr.bondv1, g_arr.bondn1))
+      return;
+    (void)n0;
+    if (stk_prepare(c) || stk_set_solution(c, g_arr.sol0)) return;
+  } else if (!c->prepared && stk_prepare(c)) {
+    return;
+  }
   stk_solver_opts o;
   stk_default_opts(&o, c->L);
   stk_solve(c, &o, nullptr);
+  if (g_arr_set && !stk_get_solution(c, g_arr.sol0)) (void)vpiterate_outputs(c);
 }
 
 }  // extern "C"

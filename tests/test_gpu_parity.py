@@ -303,7 +303,7 @@ def test_coarse_solve_matches_dense_lu(oracle_mod):
         g.close()
 
 
-def test_pardisomat_and_gausmat2_entry_points(oracle_mod):
+def test_pardisomat_entry_point(oracle_mod):
     """Rows handed over in the reference's un1/ui1 format (pardisomat(1,..)),
     then pardisomat(0,..) solves the coarsest system on the device and stores
     the solution at sol0[4*(mgp[L]+node)+c]."""
@@ -335,7 +335,6 @@ def test_pardisomat_and_gausmat2_entry_points(oracle_mod):
                     ui1[k + 1] = float(v)
                 ui1[0] = rhs[i]
             assert S.lib().pardisomat(1, i, r, 0, 0, un1, ui1) == 0
-            assert S.lib().gausmat2(1, r, 57 * (r // 4) + (0, 6, 23, 40)[r % 4], un1, ui1) == 0
         assert S.lib().pardisomat(0, nL, 0, 0, 0, un1, ui1) == 0
         o.arr("fre1")[bL:bL + nL] = rhs
         o.coarse_solve()
