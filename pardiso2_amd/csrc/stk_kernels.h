@@ -1847,6 +1847,179 @@ __global__ void __launch_bounds__(256) k_update2(long n4, int k, const double *_
   if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// ---------------------------------------------------------------------------
+// The same two passes for an FP32 basis, four basis vectors per step and
+// 16-byte loads (float4 of the basis, double4 of u, w, W2): each thread
+// keeps E element quads of W2 u and W2 w in registers across the k basis
+// vectors.  The eight dots of a step (4 vectors x {u, w}) are reduced over
+// the wave by halving: at each of the first three exchanges a lane keeps
+// half of its sums and adds the partner's copy of them (xor 32, 16, 8), so
+// 10 shuffles give the wave total of quantity lane >> 3 in every lane.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum8(const double v[8], int lane) {
+  const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+  double k4[4], k2[2];
+#pragma unroll
+  for (int j = 0; j < 4; j++) k4[j] = (b5 ? v[4 + j] : v[j]) + __shfl_xor(b5 ? v[j] : v[4 + j], 32);
+#pragma unroll
+  for (int j = 0; j < 2; j++) k2[j] = (b4 ? k4[2 + j] : k4[j]) + __shfl_xor(b4 ? k4[j] : k4[2 + j], 16);
+  double k1 = (b3 ? k2[1] : k2[0]) + __shfl_xor(b3 ? k2[0] : k2[1], 8);
+  k1 += __shfl_xor(k1, 4);
+  k1 += __shfl_xor(k1, 2);
+  k1 += __shfl_xor(k1, 1);
+  return k1;
+}
+
+constexpr int kGsE = 2;  // element quads per thread per slice
+
+// partial[i * gridDim + block], i < 2k+2: <V_i, W2 u> (i < k), <V_i, W2 w>
+// (k + i), <u, W2 w>, <u, W2 u>
+__global__ void __launch_bounds__(256) k_dots2f(long n4, const double *__restrict__ w2, const double *__restrict__ u,
+                                                const double *__restrict__ w, PtrsKT<float> V, int k,
+                                                double *partial) {
+  __shared__ double acc[2 * kMaxDirs + 2][4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nd = 2 * k + 2;
+  for (int i = threadIdx.x; i < (2 * kMaxDirs + 2) * 4; i += blockDim.x) acc[i >> 2][i & 3] = 0.0;
+  __syncthreads();
+  constexpr int E = kGsE;
+  const long nq = n4 >> 2;
+  const double4 *__restrict__ W2 = reinterpret_cast<const double4 *>(w2);
+  const double4 *__restrict__ U = reinterpret_cast<const double4 *>(u);
+  const double4 *__restrict__ Wv = reinterpret_cast<const double4 *>(w);
+  for (long base = (long)blockIdx.x * 256 * E; base < nq; base += (long)gridDim.x * 256 * E) {
+    double wu[E][4], ww[E][4];
+    double suw = 0.0, suu = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const long p = base + threadIdx.x + 256L * e;
+      double4 q = make_double4(0.0, 0.0, 0.0, 0.0), uu = q, wv = q;
+      if (p < nq) { q = W2[p]; uu = U[p]; wv = Wv[p]; }
+      wu[e][0] = q.x * uu.x; wu[e][1] = q.y * uu.y; wu[e][2] = q.z * uu.z; wu[e][3] = q.w * uu.w;
+      ww[e][0] = q.x * wv.x; ww[e][1] = q.y * wv.y; ww[e][2] = q.z * wv.z; ww[e][3] = q.w * wv.w;
+      suw += wu[e][0] * wv.x + wu[e][1] * wv.y + wu[e][2] * wv.z + wu[e][3] * wv.w;
+      suu += wu[e][0] * uu.x + wu[e][1] * uu.y + wu[e][2] * uu.z + wu[e][3] * uu.w;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      suw += __shfl_xor(suw, off);
+      suu += __shfl_xor(suu, off);
+    }
+    if (lane == 0) { acc[2 * k][wave] += suw; acc[2 * k + 1][wave] += suu; }
+    for (int i = 0; i < k; i += 4) {
+      float4 a[4][E];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const float4 *__restrict__ vt = reinterpret_cast<const float4 *>(V.p[min(i + t, k - 1)]);
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const long p = base + threadIdx.x + 256L * e;
+          a[t][e] = p < nq ? vt[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+      }
+      double sv[8];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        double su = 0.0, sw = 0.0;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const double ax = a[t][e].x, ay = a[t][e].y, az = a[t][e].z, aw = a[t][e].w;
+          su += wu[e][0] * ax + wu[e][1] * ay + wu[e][2] * az + wu[e][3] * aw;
+          sw += ww[e][0] * ax + ww[e][1] * ay + ww[e][2] * az + ww[e][3] * aw;
+        }
+        sv[2 * t] = su;
+        sv[2 * t + 1] = sw;
+      }
+      const double tot = wave_sum8(sv, lane);
+      if ((lane & 7) == 0) {
+        const int q = lane >> 3, t = q >> 1;
+        if (i + t < k) acc[(q & 1) ? k + i + t : i + t][wave] += tot;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nd; i += blockDim.x)
+    partial[(long)i * gridDim.x + blockIdx.x] = (acc[i][0] + acc[i][1]) + (acc[i][2] + acc[i][3]);
+}
+
+// u <- (u - sum_i a_i V_i) su stored rounded in vout (and in u with keep_u),
+// w <- w - sum_i b_i V_i - h_k vout, partial sums of W2 w^2 (coef = [a, b,
+// h_k, su] as k_update2); each element subtracts the terms in basis order
+__global__ void __launch_bounds__(256) k_update2f(long n4, int k, const double *__restrict__ coef, PtrsKT<float> V,
+                                                  double *u, double *w, const double *__restrict__ w2,
+                                                  double *partial, float *vout, int keep_u) {
+  __shared__ double sc[2 * kMaxDirs + 2];
+  __shared__ double red[4];
+  for (int i = threadIdx.x; i < 2 * k + 2; i += blockDim.x) sc[i] = coef[i];
+  __syncthreads();
+  const double hk = sc[2 * k], su = sc[2 * k + 1];
+  constexpr int E = kGsE;
+  const long nq = n4 >> 2;
+  double4 *U = reinterpret_cast<double4 *>(u);
+  double4 *Wv = reinterpret_cast<double4 *>(w);
+  const double4 *__restrict__ W2 = reinterpret_cast<const double4 *>(w2);
+  double ss = 0.0;
+  for (long base = (long)blockIdx.x * 256 * E; base < nq; base += (long)gridDim.x * 256 * E) {
+    double uu[E][4], wv[E][4];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const long p = base + threadIdx.x + 256L * e;
+      double4 a = make_double4(0.0, 0.0, 0.0, 0.0), b = a;
+      if (p < nq) { a = U[p]; b = Wv[p]; }
+      uu[e][0] = a.x; uu[e][1] = a.y; uu[e][2] = a.z; uu[e][3] = a.w;
+      wv[e][0] = b.x; wv[e][1] = b.y; wv[e][2] = b.z; wv[e][3] = b.w;
+    }
+    for (int i = 0; i < k; i += 4) {
+      const int nt = min(4, k - i);
+      float4 a[4][E];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const float4 *__restrict__ vt = reinterpret_cast<const float4 *>(V.p[min(i + t, k - 1)]);
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const long p = base + threadIdx.x + 256L * e;
+          a[t][e] = p < nq ? vt[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        if (t >= nt) break;
+        const double c = sc[i + t], d = sc[k + i + t];
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const double av[4] = {a[t][e].x, a[t][e].y, a[t][e].z, a[t][e].w};
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            uu[e][q] -= c * av[q];
+            wv[e][q] -= d * av[q];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const long p = base + threadIdx.x + 256L * e;
+      if (p >= nq) continue;
+      float vs[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        uu[e][q] *= su;
+        vs[q] = (float)uu[e][q];
+        wv[e][q] -= hk * (double)vs[q];
+      }
+      reinterpret_cast<float4 *>(vout)[p] = make_float4(vs[0], vs[1], vs[2], vs[3]);
+      if (keep_u) U[p] = make_double4(uu[e][0], uu[e][1], uu[e][2], uu[e][3]);
+      Wv[p] = make_double4(wv[e][0], wv[e][1], wv[e][2], wv[e][3]);
+      const double4 q = W2[p];
+      ss += q.x * wv[e][0] * wv[e][0] + q.y * wv[e][1] * wv[e][1] + q.z * wv[e][2] * wv[e][2] +
+            q.w * wv[e][3] * wv[e][3];
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 // x = v (a fill kernel instead of a memset node, so the preconditioner can be
 // captured in a HIP graph)
 template <typename T>

@@ -2204,8 +2204,10 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
   auto z32 = [&](int i) { return reinterpret_cast<float *>(c->Zs[i]); };
   // dots and updates: one block per 1024-element slice up to 4 nred blocks
   // (partials of 2j+2 sums per block for the dots)
-  const unsigned nbk = (unsigned)std::min<long>(4L * c->nred, std::max<long>(1, (n4 / 2 + 1023) / 1024));
-  const unsigned nbu = (unsigned)std::min<long>(4L * c->nred, std::max<long>(1, (n4 / 2 + 1023) / 1024));
+  const long slice = b32 ? 256L * kGsE : 1024L;  // quads (FP32 basis) or double2 pairs per block slice
+  const long units = b32 ? n4 / 4 : n4 / 2;
+  const unsigned nbk = (unsigned)std::min<long>(4L * c->nred, std::max<long>(1, (units + slice - 1) / slice));
+  const unsigned nbu = nbk;
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), hh(2 * m + 2), y(m), coef(2 * m + 2);
   auto h = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
   auto givens = [&](int col) {
@@ -2245,7 +2247,7 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
       PtrsKT<float> P32{};
       for (int i = 0; i < j; i++) { P.p[i] = c->Qs[i]; P32.p[i] = v32(i); }
       if (b32)
-        hipLaunchKernelGGL(k_dots2<float>, dim3(nbk), dim3(256), 0, c->st, n4, c->W2, u, W, P32, j, c->kred);
+        hipLaunchKernelGGL(k_dots2f, dim3(nbk), dim3(256), 0, c->st, n4, c->W2, u, W, P32, j, c->kred);
       else
         hipLaunchKernelGGL(k_dots2<double>, dim3(nbk), dim3(256), 0, c->st, n4, c->W2, u, W, P, j, c->kred);
       hipLaunchKernelGGL(k_reduce_rows, dim3(2 * j + 2), dim3(256), 0, c->st, c->kred, (int)nbk, 2 * j + 2, c->hcol);
@@ -2277,8 +2279,9 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
       memcpy(c->hpin, coef.data(), (2 * j + 2) * sizeof(double));  // copied before the stream reuses hpin
       HIPCHK(hipMemcpyAsync(c->hcol, c->hpin, (2 * j + 2) * sizeof(double), hipMemcpyHostToDevice, c->st));
       if (b32)
-        hipLaunchKernelGGL(k_update2<float>, dim3(nbu), dim3(256), 0, c->st, n4, j, c->hcol, P32, u, W, c->W2,
-                           c->kred, v32(j));
+        // the FP64 v_j is not read again (the next candidate overwrites u), only its stored FP32 copy
+        hipLaunchKernelGGL(k_update2f, dim3(nbu), dim3(256), 0, c->st, n4, j, c->hcol, P32, u, W, c->W2, c->kred,
+                           v32(j), 0);
       else
         hipLaunchKernelGGL(k_update2<double>, dim3(nbu), dim3(256), 0, c->st, n4, j, c->hcol, P, u, W, c->W2,
                            c->kred, (double *)nullptr);

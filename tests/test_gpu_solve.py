@@ -114,8 +114,8 @@ def test_error_paths_return_codes():
         fp = S._fp
         assert S.lib().stk_set_fields(g.h, fp(bad), fp(ok), fp(ok), fp(ok), fp(ok), fp(ok)) == -1
         g.set_fields(ok * 1e21, ok * 3300, ok * 1e21, ok * 1e21, ok * 1e21, ok * 1e21)
-        # no boundary equations: wall rows would leave the grid
-        assert S.lib().stk_prepare(g.h) == -1
+        # no boundary equations: the outer layer's rows would leave the grid (STK_EBCFORM)
+        assert S.lib().stk_prepare(g.h) == -9
     finally:
         g.close()
     L = S.lib()
