@@ -79,8 +79,9 @@ typedef struct {
   int ksweeps;       /* red/black Gauss-Seidel sweeps before and after each K-cycle level */
   double kscale;     /* coarse-correction scale of the aggregation K-cycle */
   int krylov;        /* outer Krylov method: 0 GCR, 1 flexible GMRES */
-  int graph;         /* 1: replay the preconditioner from a captured HIP graph (one host thread
-                        per process capturing at a time; stk_solve_batch launches eagerly), 0: eager */
+  int graph;         /* 1: replay the preconditioner from a captured HIP graph (captured on the
+                        first solve of a shape; captures, and the library's device allocations and
+                        releases, are serialised across the process's threads), 0: eager */
   int history;       /* 0: the context keeps its Krylov vectors between solves; 1: it borrows them
                         from a process-wide pool for the solve (many contexts, few solving at once:
                         memory follows the solves in flight; a solve waits while the device has no

@@ -41,8 +41,17 @@ namespace {
     }                                              \
   } while (0)
 
+// Graph capture and the runtime calls that can wait on other streams
+// (device allocation and release, pinned host memory, stream, event and
+// graph lifetime) exclude each other across host threads: a release that
+// synchronises the device while another thread's stream is capturing breaks
+// that capture.  Recursive, so a holder may allocate.
+std::recursive_mutex g_rt_mu;
+using RtLock = std::lock_guard<std::recursive_mutex>;
+
 template <typename T>
 int dalloc(T **p, size_t n) {
+  RtLock lk(g_rt_mu);
   *p = nullptr;
   if (n == 0) n = 1;
   if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) return STK_ENOMEM;
@@ -51,6 +60,7 @@ int dalloc(T **p, size_t n) {
 
 template <typename T>
 void dfree(T *&p) {
+  RtLock lk(g_rt_mu);
   if (p) (void)hipFree(p);
   p = nullptr;
 }
@@ -1049,6 +1059,7 @@ int ensure_outer(stk_ctx *c, int m) {
         dalloc(&c->red, (size_t)c->nred * 16) || dalloc(&c->scal, 256) ||
         dalloc(&c->kred, (size_t)4 * c->nred * (2 * kMaxDirs + 2)) || dalloc(&c->hcol, 2 * kMaxDirs + 2))
       return STK_ENOMEM;
+    RtLock lk(g_rt_mu);
     if (hipHostMalloc((void **)&c->hpin, (2 * kMaxDirs + 8) * sizeof(double), hipHostMallocDefault) != hipSuccess)
       return STK_ENOMEM;
     hipMemsetAsync(c->X, 0, n4 * 8, c->st);
@@ -1373,9 +1384,15 @@ int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *
   c->bondv1.assign(2, 0.f);
   c->bondn1.assign(1, 0);
   c->kn = stk_knobs{0.10, 0.0, 0.10, 1.0, 1.0, 1.0, 1e4, 0, 0.0, 0.0, 0.0, 9e-3, 5e1, 0.0, 3e-3};
-  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; return STK_EHIP; }
-  hipEventCreate(&c->ev0);
-  hipEventCreate(&c->ev1);
+  {
+    RtLock lk(g_rt_mu);
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; return STK_EHIP; }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+      (void)hipStreamDestroy(c->st);
+      delete c;
+      return STK_EHIP;
+    }
+  }
   // per-level geometry tables
   for (int l = 0; l <= levels; l++) {
     Level &L = c->lv[l];
@@ -1458,6 +1475,7 @@ void stk_destroy(stk_ctx *c) {
   dfree(c->wu); dfree(c->ww); dfree(c->wz);
   dfree(c->post_part); dfree(c->sig); dfree(c->epsr);
   dfree(c->kred); dfree(c->hcol); dfree(c->gin); dfree(c->gout);
+  RtLock lk(g_rt_mu);
   if (c->hpin) hipHostFree(c->hpin);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
   if (c->graph) hipGraphDestroy(c->graph);
@@ -1504,7 +1522,7 @@ void stk_default_opts(stk_solver_opts *o, int levels) {
   // aggregation undershoots by about 2): measured against 2 and 2.0 over random-field, inclusion and
   // sinker boxes from 33^3 to 257^3 (257^3 spheres: 153 -> 104 outer iterations)
   o->ksweeps = 3;
-  o->graph = 0;  // replay gains ~5% on small levels; captures from several host threads at once are not supported
+  o->graph = 0;  // replay gains ~5% on small levels
   o->kscale = 1.8;
   o->basis = 1;  // FP32 Krylov vectors in mixed precision (half the Gram-Schmidt traffic)
 }
@@ -1620,7 +1638,10 @@ int stk_markers_to_nodes(stk_ctx *c, long nmark, const float *mx, const float *m
   if (!rc) rc = markers_run(c, nmark, mx, my, mz, mnu, mro, nukoef, b);
   for (int k = 0; k < 5; k++) { dfree(b.in[k]); dfree(b.so[k]); }
   dfree(b.key); dfree(b.val); dfree(b.skey); dfree(b.sval); dfree(b.start);
-  if (b.tmp) hipFree(b.tmp);
+  if (b.tmp) {
+    RtLock lk(g_rt_mu);
+    (void)hipFree(b.tmp);
+  }
   return rc;
 }
 
@@ -2149,19 +2170,39 @@ static int ensure_graph(stk_ctx *c, const stk_solver_opts *o, const int *tile) {
   if (c->graph_ok && same_shape(c->gkey, *o)) return STK_OK;
   const long n4 = 4 * c->lv[0].n;
   if (!c->gin && (dalloc(&c->gin, n4) || dalloc(&c->gout, n4))) return STK_ENOMEM;
-  if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-  if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+  c->graph_ok = false;
+  {
+    RtLock lk(g_rt_mu);
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+  }
   // the tile sweeps' chain lists exist before capture, so the graph uses them
   for (int l = 0; l < c->L; l++) {
     Level &L = c->lv[l];
     (void)tile_bc(c, L, tile ? tile : (o->precision == 1 ? tuned_tile<double>(L) : tuned_tile<float>(L)));
   }
   HIPCHK(hipStreamSynchronize(c->st));
-  HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  // one capture at a time in the process, and no allocation or release on
+  // another thread while it runs; the preconditioner only launches kernels
+  RtLock lk(g_rt_mu);
+  hipError_t e = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    fprintf(stderr, "pardiso2_amd: hipStreamBeginCapture: %s\n", hipGetErrorString(e));
+    return STK_EHIP;
+  }
   if (o->precision == 1) precond<double>(c, o, c->gin, c->gout, tile);
   else precond<float>(c, o, c->gin, c->gout, tile);
-  HIPCHK(hipStreamEndCapture(c->st, &c->graph));
-  HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  hipGraph_t gr = nullptr;
+  e = hipStreamEndCapture(c->st, &gr);  // always ends the capture, so the stream leaves capture mode
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    fprintf(stderr, "pardiso2_amd: preconditioner capture: %s\n", hipGetErrorString(e));
+    if (gr) (void)hipGraphDestroy(gr);
+    c->gexec = nullptr;
+    return STK_EHIP;
+  }
+  c->graph = gr;
   c->gkey = *o;
   c->graph_ok = true;
   return STK_OK;

@@ -139,7 +139,7 @@ class Server:
                  keep_solutions: bool = False, mem_info=None, factory=None):
         self.workers = workers
         self.opts_kw = dict(opts_kw or {})
-        self.opts_kw["graph"] = 0  # workers launch eagerly (no concurrent stream captures)
+        self.opts_kw.setdefault("graph", 0)  # 1: each context replays its captured preconditioner
         self.restart = self.opts_kw.get("restart", 128)
         self.pool = ContextPool(self.restart, factory)
         self.reserve = reserve_bytes

@@ -43,3 +43,60 @@ def test_graph_replay_matches_eager(precision):
         np.testing.assert_array_equal(out[0][1], out[1][1])
     finally:
         g.close()
+
+
+def test_concurrent_captures_from_two_threads():
+    """Two host threads each capture and replay their own context's
+    preconditioner at the same time, while a third allocates and releases
+    contexts: every solve equals the same context's eager solve bit for bit."""
+    import threading
+    probs = [setups.random_field(n, setups.coarsest_levels(n), seed=s, lo=19.0, hi=23.0, field_n=33)
+             for n, s in ((33, 1), (25, 2))]
+    ctxs = [S.from_problem(pb) for pb in probs]
+    try:
+        eager = []
+        for g, pb in zip(ctxs, probs):
+            g.hydrostatic_guess()
+            st = g.solve(S.default_opts(pb.levels, graph=0))
+            eager.append((st.iterations, g.solution()))
+        out = [[] for _ in ctxs]
+        errs = []
+        stop = threading.Event()
+
+        def solver(i):
+            try:
+                g, pb = ctxs[i], probs[i]
+                for _ in range(3):
+                    g.prepare()
+                    g.hydrostatic_guess()
+                    st = g.solve(S.default_opts(pb.levels, graph=1))
+                    out[i].append((st.status, st.iterations, g.solution()))
+            except Exception as e:  # reported below
+                errs.append(e)
+
+        def churn():
+            try:
+                pb = setups.sinker(17, setups.coarsest_levels(17))
+                while not stop.is_set():
+                    S.from_problem(pb).close()
+            except Exception as e:
+                errs.append(e)
+
+        th = [threading.Thread(target=solver, args=(i,)) for i in range(len(ctxs))]
+        ch = threading.Thread(target=churn)
+        ch.start()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        stop.set()
+        ch.join()
+        assert not errs, errs
+        for (ie, xe), res in zip(eager, out):
+            assert len(res) == 3
+            for status, it, x in res:
+                assert status == 0 and it == ie
+                np.testing.assert_array_equal(x, xe)
+    finally:
+        for g in ctxs:
+            g.close()
