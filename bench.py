@@ -89,6 +89,7 @@ def host_cpu():
 
 
 CPU_SAMPLE_N = 33
+HOST = host_cpu()  # before any OpenMP runtime binds this process's threads
 
 
 def load_oracle():
@@ -113,7 +114,7 @@ def cpu_baseline(pb, budget_s: float, repeats: int = 5, device=True):
     the same problem the same number of times for a like-for-like rate, and
     its mixed-precision solution is compared with the oracle's converged one.
     """
-    host = host_cpu()  # before the OpenMP runtime binds this thread
+    host = HOST
     O = load_oracle()
     pre, post = sweeps_for(pb.levels)
     t0 = time.perf_counter()
@@ -127,7 +128,8 @@ def cpu_baseline(pb, budget_s: float, repeats: int = 5, device=True):
         its.append(int(it))
         x_or = x
         if it < 0 or hist[-1] > 1e-8:
-            raise RuntimeError(f"oracle solve did not reach 1e-8 ({it} iterations, {hist[-1]:.2e})")
+            raise RuntimeError(f"the oracle's GCR (mass-matrix Schur scaling) stalls on this {pb.n}^3 problem: "
+                               f"{hist[-1]:.2e} after {it} iterations, no full CPU solve to 1e-8")
     med = float(np.median(times))
     out = {"value": 1.0 / med, "unit": "solves/s", "cores": O.threads(), "kind": "port",
            "sample": (f"full FP64 solves of the {pb.n}^3 instance of this workload (same generator and seed) by "
@@ -215,8 +217,7 @@ def cpu_sample(pb, budget_s, device=True):
     try:
         return cpu_baseline(pb, budget_s, device=device)
     except Exception as e:  # the baseline must not hide the device result
-        return {"value": None, "unit": "solves/s", "cores": None, "kind": "port", "sample": f"failed: {e}",
-                **host_cpu()}
+        return {"value": None, "unit": "solves/s", "cores": None, "kind": "port", "sample": f"failed: {e}", **HOST}
 
 
 def bench_batched(args):

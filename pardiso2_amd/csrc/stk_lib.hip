@@ -2402,7 +2402,7 @@ static int post_norms(stk_ctx *c, const double *x, stk_norms *out) {
   hipLaunchKernelGGL(k_post_norms, dim3(nb), dim3(256), 0, c->st, post_geo(c), F.flags, x, c->B, F.nbc, F.bc_r,
                      F.bc_j, F.bc_negk, F.bc_c, c->post_part);
   double *dout = c->post_part + (size_t)kPostBlocks * kPostN;
-  hipLaunchKernelGGL(k_post_reduce, dim3(1), dim3(64), 0, c->st, c->post_part, nb, dout);
+  hipLaunchKernelGGL(k_post_reduce, dim3(kPostN), dim3(256), 0, c->st, c->post_part, nb, dout);
   double s[kPostN];
   HIPCHK(hipMemcpyAsync(s, dout, sizeof(s), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));

@@ -291,16 +291,21 @@ __global__ void __launch_bounds__(256) k_post_norms(PostGeo g, const unsigned ch
   }
 }
 
-// partials of nb blocks -> out[kPostN], one thread per quantity, blocks in order
-__global__ void k_post_reduce(const double *__restrict__ partial, int nb, double *out) {
-  const int q = threadIdx.x;
-  if (q >= kPostN) return;
-  double v = partial[q];
-  for (int k = 1; k < nb; k++) {
-    const double o = partial[(long)k * kPostN + q];
-    v = q < 10 ? v + o : ((q & 1) ? fmax(v, o) : fmin(v, o));
+// partials of nb blocks -> out[kPostN]: one 256-thread block per quantity,
+// thread t folds blocks t, t+256, ... in order, then a fixed tree
+__global__ void __launch_bounds__(256) k_post_reduce(const double *__restrict__ partial, int nb, double *out) {
+  __shared__ double red[256];
+  const int q = blockIdx.x, t = threadIdx.x;
+  auto op = [q](double a, double b) { return q < 10 ? a + b : ((q & 1) ? fmax(a, b) : fmin(a, b)); };
+  double v = q < 10 ? 0.0 : ((q & 1) ? -1e30 : 1e30);
+  for (int k = t; k < nb; k += 256) v = op(v, partial[(long)k * kPostN + q]);
+  red[t] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) red[t] = op(red[t], red[t + s]);
+    __syncthreads();
   }
-  out[q] = v;
+  if (t == 0) out[q] = red[0];
 }
 
 }  // namespace stk
