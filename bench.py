@@ -426,6 +426,21 @@ def main():
         if tj.get("nodes") == n0 and inst in k.get("instance", ""):  # counters of this very instance only
             traffic = k.get("hbm_bytes_per_launch")
 
+    # one marker-advection step (RK4) of the problem's own markers through the
+    # timed solution: host coordinates in and out, so PCIe is included
+    adv = None
+    if pb.mx is not None and len(pb.mx) > 0:
+        mx, my, mz = (np.array(a, dtype=np.float32) for a in (pb.mx, pb.my, pb.mz))
+        vmax = float(np.abs(ctx.solution()[1::4]).max())
+        dt_adv = 0.01 * float(pb.gx[-1]) / max(vmax, 1e-300)
+        ts = time.perf_counter()
+        ctx.move_markers(mx, my, mz, timestep=dt_adv, want_eii=False)
+        t_adv = time.perf_counter() - ts
+        adv = {"markers": int(len(mx)), "ms": t_adv * 1e3, "markers_per_s": len(mx) / t_adv,
+               "note": "one RK4 step incl. host<->device copies of the coordinates (the kernel alone: profiles/)"}
+        log(f"marker advection: {len(mx)} markers in {t_adv * 1e3:.1f} ms")
+        del mx, my, mz
+
     x_dev = ctx.solution()
     fp64_its, err = None, None
     if not args.no_fp64_ref:
@@ -496,6 +511,7 @@ def main():
         "residual_fp64": {"achieved": gbs_res, "unit": "GB/s", "frac": gbs_res / PEAK_HBM_GBS,
                           "ms_per_launch": ms_res},
         "cpu_baseline": cpu,
+        "marker_advection": adv,
     }
     print(json.dumps(out), flush=True)
 

@@ -192,6 +192,22 @@ int stk_norms_eval(stk_ctx *ctx, stk_norms *out);
  * the node's edge); zero where the reference's recompute writes nothing.
  * Either pointer may be NULL. */
 int stk_stress_strain(stk_ctx *ctx, double *sig, double *eps);
+/* One marker-advection step through the current solution: movemark
+ * (parallel/mark3mg.c:174-629) with the velocity and strain-rate
+ * interpolations vxyzcalc1 and epscalc1 (parallel/mark3mg.c:2963, 3244),
+ * walled boxes.  Velocities and strain rates are read as the reference's
+ * FP32 node arrays hold them.  mx/my/mz (marker coordinates) are updated in
+ * place; markmod 1: one Euler step, else RK4 (the knob markmod); outgrid as
+ * the reference's knob (1: markers outside the box stay, 2: they keep only
+ * their motion across the faces they lie beyond).  Optional (NULL):
+ * marke, the accumulated strain, grows by the RK4 average of the strain-rate
+ * invariant times timestep where positive; markk is set to 0 for a normal
+ * marker that leaves the box; markt, the marker types (< 50 normal, >= 100
+ * immobile; NULL: all normal); eii receives the strain-rate invariant at the
+ * start position (the reference's val1).  The water-marker branch and the
+ * immobile-marker reset, both disabled in the reference, are not built. */
+int stk_move_markers(stk_ctx *ctx, long nmark, float *mx, float *my, float *mz, float *marke, float *markk,
+                     const int *markt, float *eii, double timestep, int markmod, int outgrid);
 /* Hydrostatic initial guess: v = 0, P integrated up each column through the
  * z-momentum rows (the reference seeds P this way, parallel/in3mg.c:927-938). */
 int stk_hydrostatic_guess(stk_ctx *ctx);

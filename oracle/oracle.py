@@ -70,6 +70,9 @@ def lib():
         L.orc_relres_rows.restype = C.c_double
         L.orc_relres_rows.argtypes = [C.c_void_p, dp, dp]
         L.orc_post.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.orc_move_markers.argtypes = [C.c_void_p, dp, dp, C.c_long, fp, fp, fp, fp, fp, ip, fp, C.c_double,
+                                       C.c_int, C.c_int]
+        L.orc_interp.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_double, C.c_double, dp, dp]
         L.orc_solve.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_double, C.c_int, dp]
         L.orc_band_solve.argtypes = [C.c_void_p, dp, C.POINTER(C.c_long)]
         L.orc_gcr.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
@@ -246,6 +249,32 @@ class Oracle:
         eps = np.zeros((6, n0)) if stresses else None
         lib().orc_post(self.h, _dp(x), _dp(sig) if stresses else None, _dp(eps) if stresses else None, _dp(out))
         return sig, eps, out
+
+    def interp(self, x, eps, p):
+        """Velocity (3) and strain rates (6) at point p from solution x and the
+        strain-rate planes eps (6, n0) (vxyzcalc1 / epscalc1)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        v, e = np.zeros(3), np.zeros(6)
+        lib().orc_interp(self.h, _dp(x), _dp(eps), float(p[0]), float(p[1]), float(p[2]), _dp(v), _dp(e))
+        return v, e
+
+    def move_markers(self, x, eps, mx, my, mz, marke=None, markk=None, markt=None, timestep=1.0, markmod=0,
+                     outgrid=0, want_eii=True):
+        """One movemark step of the markers (arrays updated in place, float32);
+        returns the strain-rate invariant at the start positions."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        n = len(mx)
+        for a in (mx, my, mz, marke, markk):
+            assert a is None or (a.dtype == np.float32 and a.flags.c_contiguous and len(a) == n)
+        eii = np.zeros(n, np.float32) if want_eii else None
+        mt = None if markt is None else np.ascontiguousarray(markt, dtype=np.int32)
+        opt = lambda a, f: f(a) if a is not None else None
+        lib().orc_move_markers(self.h, _dp(x), _dp(eps), n, _fp(mx), _fp(my), _fp(mz), opt(marke, _fp),
+                               opt(markk, _fp), opt(mt, _ip), opt(eii, _fp), float(timestep), int(markmod),
+                               int(outgrid))
+        return eii
 
     def set_fields(self, nu, ro, nxx, nxy, nxz, nyz):
         """Finest-level node properties given directly (instead of ronurecalc)."""

@@ -1993,6 +1993,168 @@ void orc_post(const orc_t *o, const double *x, double *sig, double *eps, double 
 #undef VZ
 }
 
+/* ------------------------------------------------------------------------- */
+/* Marker advection: movemark (parallel/mark3mg.c:174-629) with vxyzcalc1     */
+/* (:2963-3236) and epscalc1 (:3244-3760), walled boxes (the periodic wraps   */
+/* are not restated).  Along each axis a quantity sits on grid lines or on    */
+/* cell centres, and the reference brackets the point with one of five rules */
+/* (names below); weights are 1 - |distance| / spacing, accumulated z outer, */
+/* x, y inner, as the reference's loops run.                                 */
+/* ------------------------------------------------------------------------- */
+enum { AX_VN, AX_VC, AX_CELL, AX_SN, AX_SC };
+/* AX_VN   velocity component along its own axis: lines m, m+1 (m clamped to
+           0..n-2), the point itself                                (:3004-3007)
+   AX_VC   velocity across an axis: centres of cells below and above (:3008-3023)
+   AX_CELL normal strain rates: cell centres m, m+1 (1..n-2)        (:3272-3285)
+   AX_SN   shear strain rates along a line axis: lines 1..n-3, point clamped
+           to the clamped line                                      (:3390-3398)
+   AX_SC   shear strain rates across: centres, the lower bracket clamped from
+           below at 1 to 0 with the point on centre 0               (:3411-3427) */
+typedef struct { long i0; double w0, w1; } axw_t;
+
+static axw_t axis_weights(const double *g, int n, long m, double p, int kind) {
+  axw_t r;
+  double P = p, d, t0, t1;
+  long i0 = m;
+  if (kind == AX_VN) {
+    if (i0 < 0) i0 = 0;
+    if (i0 > n - 2) i0 = n - 2;
+    d = g[i0 + 1] - g[i0];
+    t0 = (g[i0] - p) / d; t1 = (g[i0 + 1] - p) / d;
+  } else if (kind == AX_SN) {
+    if (i0 < 1) { i0 = 1; P = g[i0]; }
+    if (i0 > n - 3) { i0 = n - 3; P = g[i0 + 1]; }
+    d = g[i0 + 1] - g[i0];
+    t0 = (g[i0] - P) / d; t1 = (g[i0 + 1] - P) / d;
+  } else if (kind == AX_CELL) {
+    if (P > (g[i0] + g[i0 + 1]) / 2.0) i0 += 1;
+    if (i0 < 1) { i0 = 1; P = (g[i0] + g[i0 - 1]) / 2.0; }
+    if (i0 > n - 2) { i0 = n - 2; P = (g[i0] + g[i0 + 1]) / 2.0; }
+    d = (g[i0 + 1] - g[i0 - 1]) / 2.0;
+    t0 = ((g[i0] + g[i0 - 1]) / 2.0 - P) / d; t1 = ((g[i0 + 1] + g[i0]) / 2.0 - P) / d;
+  } else {  /* AX_VC, AX_SC */
+    if (P < (g[i0] + g[i0 + 1]) / 2.0) i0 -= 1;
+    if (kind == AX_VC ? i0 < 0 : i0 < 1) { i0 = 0; P = (g[i0] + g[i0 + 1]) / 2.0; }
+    if (i0 > n - 3) { i0 = n - 3; P = (g[i0 + 1] + g[i0 + 2]) / 2.0; }
+    d = (g[i0 + 2] - g[i0]) / 2.0;
+    t0 = ((g[i0] + g[i0 + 1]) / 2.0 - P) / d; t1 = ((g[i0 + 1] + g[i0 + 2]) / 2.0 - P) / d;
+  }
+  r.i0 = i0;
+  r.w0 = 1.0 - (-t0);
+  r.w1 = 1.0 - t1;
+  return r;
+}
+
+/* sum over the 8 bracketing nodes of val(node) * wx * wy * wz, z outer, x, y inner */
+static double interp8(const orc_t *o, axw_t ax, axw_t ay, axw_t az, const double *vals, int stride, int comp,
+                      int as_float) {
+  const long Y = o->mgy[0], XY = o->mgxy[0];
+  double acc = 0.0;
+  for (int kz = 0; kz < 2; kz++) {
+    const double wz = kz ? az.w1 : az.w0;
+    for (int kx = 0; kx < 2; kx++) {
+      const double wx = kx ? ax.w1 : ax.w0;
+      for (int ky = 0; ky < 2; ky++) {
+        const double wy = ky ? ay.w1 : ay.w0;
+        const long n = (az.i0 + kz) * XY + (ax.i0 + kx) * Y + (ay.i0 + ky);
+        double v = vals[(size_t)n * stride + comp];
+        if (as_float) v = (double)(float)v;
+        acc += v * (wx * wy * wz);
+      }
+    }
+  }
+  return acc;
+}
+
+void orc_interp(const orc_t *o, const double *x, const double *eps, double px, double py, double pz, double *v,
+                double *e) {
+  const int X = o->mgx[0], Y = o->mgy[0], Z = o->mgz[0];
+  const double *gx = o->mggx[0], *gy = o->mggy[0], *gz = o->mggz[0];
+  const long n0 = o->mgn[0];
+  const double xs = gx[X - 1] - gx[0], ys = gy[Y - 1] - gy[0], zs = gz[Z - 1] - gz[0];
+  /* out of grid: onto the box (:2975-2977, :3255-3257) */
+  px = px < 0 ? 0 : (px > xs ? xs : px);
+  py = py < 0 ? 0 : (py > ys ? ys : py);
+  pz = pz < 0 ? 0 : (pz > zs ? zs : pz);
+  const long m1 = serch(gx, X, px), m2 = serch(gy, Y, py), m3 = serch(gz, Z, pz);
+  if (v) {
+    v[0] = interp8(o, axis_weights(gx, X, m1, px, AX_VN), axis_weights(gy, Y, m2, py, AX_VC),
+                   axis_weights(gz, Z, m3, pz, AX_VC), x, 4, 1, 1);
+    v[1] = interp8(o, axis_weights(gx, X, m1, px, AX_VC), axis_weights(gy, Y, m2, py, AX_VN),
+                   axis_weights(gz, Z, m3, pz, AX_VC), x, 4, 2, 1);
+    v[2] = interp8(o, axis_weights(gx, X, m1, px, AX_VC), axis_weights(gy, Y, m2, py, AX_VC),
+                   axis_weights(gz, Z, m3, pz, AX_VN), x, 4, 3, 1);
+  }
+  if (e) {
+    const axw_t cx = axis_weights(gx, X, m1, px, AX_CELL), cy = axis_weights(gy, Y, m2, py, AX_CELL),
+                cz = axis_weights(gz, Z, m3, pz, AX_CELL);
+    for (int k = 0; k < 3; k++) e[k] = interp8(o, cx, cy, cz, eps + (size_t)k * n0, 1, 0, 1);
+    e[3] = interp8(o, axis_weights(gx, X, m1, px, AX_SN), axis_weights(gy, Y, m2, py, AX_SN),
+                   axis_weights(gz, Z, m3, pz, AX_SC), eps + 3 * (size_t)n0, 1, 0, 1);
+    e[4] = interp8(o, axis_weights(gx, X, m1, px, AX_SN), axis_weights(gy, Y, m2, py, AX_SC),
+                   axis_weights(gz, Z, m3, pz, AX_SN), eps + 4 * (size_t)n0, 1, 0, 1);
+    e[5] = interp8(o, axis_weights(gx, X, m1, px, AX_SC), axis_weights(gy, Y, m2, py, AX_SN),
+                   axis_weights(gz, Z, m3, pz, AX_SN), eps + 5 * (size_t)n0, 1, 0, 1);
+  }
+}
+
+/* second invariant of the strain rate (movemark, mark3mg.c:229): sqrt of
+   (exx^2+eyy^2+ezz^2)/2 + exy^2+exz^2+eyz^2; pow(., 0.5) there, sqrt here */
+static double eii_of(const double *e) {
+  return sqrt(0.5 * (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) + e[3] * e[3] + e[4] * e[4] + e[5] * e[5]);
+}
+
+void orc_move_markers(const orc_t *o, const double *x, const double *eps, long nmark, float *mx, float *my,
+                      float *mz, float *marke, float *markk, const int *markt, float *eii, double timestep,
+                      int markmod, int outgrid) {
+  const int X = o->mgx[0], Y = o->mgy[0], Z = o->mgz[0];
+  const double xs = o->mggx[0][X - 1] - o->mggx[0][0], ys = o->mggy[0][Y - 1] - o->mggy[0][0],
+               zs = o->mggz[0][Z - 1] - o->mggz[0][0];
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < nmark; i++) {
+    const int t = markt ? markt[i] : 0;
+    const int inside = mx[i] >= 0 && my[i] >= 0 && mz[i] >= 0 && (double)mx[i] <= xs && (double)my[i] <= ys &&
+                       (double)mz[i] <= zs;
+    if (!(inside || (outgrid != 1 && (t < 50 || t >= 100)))) continue;
+    const int wantE = marke && marke[i] > 0;
+    double v1[3], e[6], ee1, ee0, vv[3];
+    orc_interp(o, x, eps, mx[i], my[i], mz[i], v1, e);
+    ee1 = eii_of(e);
+    if (eii) eii[i] = (float)ee1;
+    if (markmod == 1) {
+      vv[0] = v1[0]; vv[1] = v1[1]; vv[2] = v1[2];
+      ee0 = ee1;
+    } else {
+      double v2[3], v3[3], v4[3], ee2 = 0, ee3 = 0, ee4 = 0;
+      orc_interp(o, x, eps, (double)(mx[i] + v1[0] * timestep / 2.0), (double)(my[i] + v1[1] * timestep / 2.0),
+                 (double)(mz[i] + v1[2] * timestep / 2.0), v2, wantE ? e : NULL);
+      if (wantE) ee2 = eii_of(e);
+      orc_interp(o, x, eps, (double)(mx[i] + v2[0] * timestep / 2.0), (double)(my[i] + v2[1] * timestep / 2.0),
+                 (double)(mz[i] + v2[2] * timestep / 2.0), v3, wantE ? e : NULL);
+      if (wantE) ee3 = eii_of(e);
+      orc_interp(o, x, eps, (double)(mx[i] + v3[0] * timestep), (double)(my[i] + v3[1] * timestep),
+                 (double)(mz[i] + v3[2] * timestep), v4, wantE ? e : NULL);
+      if (wantE) ee4 = eii_of(e);
+      for (int k = 0; k < 3; k++) vv[k] = (v1[k] + 2.0 * v2[k] + 2.0 * v3[k] + v4[k]) / 6.0;
+      ee0 = (ee1 + 2.0 * ee2 + 2.0 * ee3 + ee4) / 6.0;
+    }
+    if (outgrid == 2) {  /* orthogonal motion only outside the box (mark3mg.c:451-455) */
+      if (mx[i] < 0 || (double)mx[i] > xs) vv[1] = vv[2] = 0;
+      if (my[i] < 0 || (double)my[i] > ys) vv[0] = vv[2] = 0;
+      if (mz[i] < 0 || (double)mz[i] > zs) vv[0] = vv[1] = 0;
+    }
+    mx[i] += timestep * vv[0];
+    my[i] += timestep * vv[1];
+    mz[i] += timestep * vv[2];
+    if (t < 100) {
+      if (wantE) marke[i] += (float)(ee0 * timestep);
+      if (markk && (mx[i] < 0 || my[i] < 0 || mz[i] < 0 || (double)mx[i] > xs || (double)my[i] > ys ||
+                    (double)mz[i] > zs))
+        markk[i] = 0;
+    }
+  }
+}
+
 /* threads the OpenMP loops of this file use (1 without OpenMP) */
 int orc_threads(void) {
 #ifdef _OPENMP

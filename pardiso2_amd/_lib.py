@@ -27,7 +27,7 @@ EXPORTS = [
     "stk_bind", "matclear", "gausmat", "gausmat2", "pardisomat", "gausmat4", "vprestrict", "vpprolong",
     "ronurestrict", "vpiterate", "stk_kop_apply", "stk_kcycle_apply", "stk_schur_apply",
     "stk_solve_batch", "stk_mem_info", "stk_autotune", "stk_level_tile", "stk_set_level_tile", "stk_tile_chains",
-    "stk_norms_eval", "stk_stress_strain", "stk_bind_arrays",
+    "stk_norms_eval", "stk_stress_strain", "stk_bind_arrays", "stk_move_markers",
 ]
 
 # DIVVMIN, STOKSMIN, DIVVMAX, STOKSMAX of the reference's input (parallel/mode.t3c:79-82)
@@ -161,6 +161,7 @@ def lib():
         L.stk_schur_apply.argtypes = [vp, C.POINTER(Opts), dp, dp]
         L.stk_norms_eval.argtypes = [vp, C.POINTER(Norms)]
         L.stk_bind_arrays.argtypes = [C.POINTER(RefArrays)]
+        L.stk_move_markers.argtypes = [vp, C.c_long, fp, fp, fp, fp, fp, ip, fp, C.c_double, C.c_int, C.c_int]
         L.stk_stress_strain.argtypes = [vp, dp, dp]
         L.matclear.argtypes = [C.c_int, dp]
         L.matclear.restype = None
@@ -318,6 +319,23 @@ class Stokes:
         eps = np.zeros((6, self.n0))
         check(lib().stk_stress_strain(self.h, _dp(sig), _dp(eps)), "stk_stress_strain")
         return sig, eps
+
+    def move_markers(self, mx, my, mz, marke=None, markk=None, markt=None, timestep=1.0, markmod=0, outgrid=0,
+                     want_eii=True):
+        """One marker-advection step through the current solution
+        (stk_move_markers): float32 arrays updated in place; returns the
+        strain-rate invariant at the start positions."""
+        n = len(mx)
+        for a in (mx, my, mz, marke, markk):
+            if a is not None and (a.dtype != np.float32 or not a.flags.c_contiguous or len(a) != n):
+                raise ValueError("marker arrays: contiguous float32 of equal length")
+        eii = np.zeros(n, np.float32) if want_eii else None
+        mt = None if markt is None else np.ascontiguousarray(markt, dtype=np.int32)
+        opt = lambda a, f: f(a) if a is not None else None
+        check(lib().stk_move_markers(self.h, n, _fp(mx), _fp(my), _fp(mz), opt(marke, _fp), opt(markk, _fp),
+                                     opt(mt, _ip), opt(eii, _fp), float(timestep), int(markmod), int(outgrid)),
+              "stk_move_markers")
+        return eii
 
     def pressure_gauge(self, px, py, pz, pinit=0.0):
         check(lib().stk_pressure_gauge(self.h, px, py, pz, pinit), "stk_pressure_gauge")

@@ -1,0 +1,210 @@
+// stk_advect.h -- marker advection on the device (gfx950): one step of
+// movemark (parallel/mark3mg.c:174-629) per marker, with the velocity and
+// strain-rate interpolations vxyzcalc1 (mark3mg.c:2963-3236) and epscalc1
+// (mark3mg.c:3244-3760), walled boxes.
+//
+// One thread per marker.  The staggered quantities are read from FP32 node
+// planes, as the reference's arrays hold them (k_adv_planes rounds the FP64
+// solution and strain rates once per step), and every interpolation weight
+// and sum follows the reference's operation order, so a marker lands where
+// the oracle's restatement puts it bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace stk {
+
+struct AdvGeo {
+  int X, Y, Z, XY;
+  const double *gx, *gy, *gz;  // grid lines of level 0
+  double xs, ys, zs;           // box sizes (last line minus first)
+  long n0;
+};
+
+// the rules by which an axis brackets a point (mark3mg.c:3004-3023,
+// 3272-3285, 3390-3398, 3411-3427): a velocity component along its own
+// axis (lines, index clamped), across an axis (centres, point clamped to the
+// outer centres), normal strain rates (cell centres 1..n-1), shear strain
+// rates along a line axis (lines 1..n-2, point clamped to the line) and
+// across (centres; the lower bracket clamped from 1 down to 0)
+enum { kAxVN = 0, kAxVC = 1, kAxCell = 2, kAxSN = 3, kAxSC = 4 };
+
+struct AxW {
+  int i0;
+  double w0, w1;
+};
+
+// cell search of m1serch/m2serch/m3serch (mark3mg.c:1138-1205)
+__device__ __forceinline__ int adv_serch(const double *__restrict__ g, int n, double p) {
+  int a = 0, b = n - 1;
+  do {
+    const int m = (a + b) / 2;
+    if (g[m] > p) b = m;
+    else a = m;
+  } while (b - a > 1);
+  return a > n - 2 ? n - 2 : a;
+}
+
+template <int K>
+__device__ __forceinline__ AxW adv_axis(const double *__restrict__ g, int n, int m, double p) {
+  double P = p, d, t0, t1;
+  int i0 = m;
+  if (K == kAxVN) {
+    if (i0 < 0) i0 = 0;
+    if (i0 > n - 2) i0 = n - 2;
+    d = g[i0 + 1] - g[i0];
+    t0 = (g[i0] - p) / d;
+    t1 = (g[i0 + 1] - p) / d;
+  } else if (K == kAxSN) {
+    if (i0 < 1) { i0 = 1; P = g[i0]; }
+    if (i0 > n - 3) { i0 = n - 3; P = g[i0 + 1]; }
+    d = g[i0 + 1] - g[i0];
+    t0 = (g[i0] - P) / d;
+    t1 = (g[i0 + 1] - P) / d;
+  } else if (K == kAxCell) {
+    if (P > (g[i0] + g[i0 + 1]) / 2.0) i0 += 1;
+    if (i0 < 1) { i0 = 1; P = (g[i0] + g[i0 - 1]) / 2.0; }
+    if (i0 > n - 2) { i0 = n - 2; P = (g[i0] + g[i0 + 1]) / 2.0; }
+    d = (g[i0 + 1] - g[i0 - 1]) / 2.0;
+    t0 = ((g[i0] + g[i0 - 1]) / 2.0 - P) / d;
+    t1 = ((g[i0 + 1] + g[i0]) / 2.0 - P) / d;
+  } else {
+    if (P < (g[i0] + g[i0 + 1]) / 2.0) i0 -= 1;
+    if (K == kAxVC ? i0 < 0 : i0 < 1) { i0 = 0; P = (g[i0] + g[i0 + 1]) / 2.0; }
+    if (i0 > n - 3) { i0 = n - 3; P = (g[i0 + 1] + g[i0 + 2]) / 2.0; }
+    d = (g[i0 + 2] - g[i0]) / 2.0;
+    t0 = ((g[i0] + g[i0 + 1]) / 2.0 - P) / d;
+    t1 = ((g[i0 + 1] + g[i0 + 2]) / 2.0 - P) / d;
+  }
+  AxW r;
+  r.i0 = i0;
+  r.w0 = 1.0 - (-t0);
+  r.w1 = 1.0 - t1;
+  return r;
+}
+
+// vx, vy, vz planes (3 n0) from the solution and, with eps, the six
+// strain-rate planes (6 n0), rounded to FP32
+__global__ void k_adv_planes(long n0, const double *__restrict__ x, const double *__restrict__ eps, float *v3,
+                             float *e6) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= n0) return;
+  const double4 q = reinterpret_cast<const double4 *>(x)[n];
+  v3[n] = (float)q.y;
+  v3[n0 + n] = (float)q.z;
+  v3[2 * n0 + n] = (float)q.w;
+  if (eps)
+    for (int k = 0; k < 6; k++) e6[k * n0 + n] = (float)eps[k * n0 + n];
+}
+
+// sum over the 8 bracketing nodes, z outer, x, y inner, of the node value
+// times (wx * wy) * wz
+__device__ __forceinline__ double adv_sum8(const AdvGeo &g, AxW ax, AxW ay, AxW az, const float *__restrict__ v) {
+  double acc = 0.0;
+#pragma unroll
+  for (int kz = 0; kz < 2; kz++) {
+    const double wz = kz ? az.w1 : az.w0;
+#pragma unroll
+    for (int kx = 0; kx < 2; kx++) {
+      const double wx = kx ? ax.w1 : ax.w0;
+#pragma unroll
+      for (int ky = 0; ky < 2; ky++) {
+        const double wy = ky ? ay.w1 : ay.w0;
+        const long n = (long)(az.i0 + kz) * g.XY + (long)(ax.i0 + kx) * g.Y + (ay.i0 + ky);
+        acc += (double)v[n] * (wx * wy * wz);
+      }
+    }
+  }
+  return acc;
+}
+
+// velocity (v, if non-null) and strain rates (e, if non-null) at a point
+__device__ __forceinline__ void adv_interp(const AdvGeo &g, const float *__restrict__ v3,
+                                           const float *__restrict__ e6, double px, double py, double pz, double *v,
+                                           double *e) {
+  px = px < 0 ? 0 : (px > g.xs ? g.xs : px);
+  py = py < 0 ? 0 : (py > g.ys ? g.ys : py);
+  pz = pz < 0 ? 0 : (pz > g.zs ? g.zs : pz);
+  const int m1 = adv_serch(g.gx, g.X, px), m2 = adv_serch(g.gy, g.Y, py), m3 = adv_serch(g.gz, g.Z, pz);
+  if (v) {
+    v[0] = adv_sum8(g, adv_axis<kAxVN>(g.gx, g.X, m1, px), adv_axis<kAxVC>(g.gy, g.Y, m2, py),
+                    adv_axis<kAxVC>(g.gz, g.Z, m3, pz), v3);
+    v[1] = adv_sum8(g, adv_axis<kAxVC>(g.gx, g.X, m1, px), adv_axis<kAxVN>(g.gy, g.Y, m2, py),
+                    adv_axis<kAxVC>(g.gz, g.Z, m3, pz), v3 + g.n0);
+    v[2] = adv_sum8(g, adv_axis<kAxVC>(g.gx, g.X, m1, px), adv_axis<kAxVC>(g.gy, g.Y, m2, py),
+                    adv_axis<kAxVN>(g.gz, g.Z, m3, pz), v3 + 2 * g.n0);
+  }
+  if (e) {
+    const AxW cx = adv_axis<kAxCell>(g.gx, g.X, m1, px), cy = adv_axis<kAxCell>(g.gy, g.Y, m2, py),
+              cz = adv_axis<kAxCell>(g.gz, g.Z, m3, pz);
+    for (int k = 0; k < 3; k++) e[k] = adv_sum8(g, cx, cy, cz, e6 + (long)k * g.n0);
+    e[3] = adv_sum8(g, adv_axis<kAxSN>(g.gx, g.X, m1, px), adv_axis<kAxSN>(g.gy, g.Y, m2, py),
+                    adv_axis<kAxSC>(g.gz, g.Z, m3, pz), e6 + 3 * g.n0);
+    e[4] = adv_sum8(g, adv_axis<kAxSN>(g.gx, g.X, m1, px), adv_axis<kAxSC>(g.gy, g.Y, m2, py),
+                    adv_axis<kAxSN>(g.gz, g.Z, m3, pz), e6 + 4 * g.n0);
+    e[5] = adv_sum8(g, adv_axis<kAxSC>(g.gx, g.X, m1, px), adv_axis<kAxSN>(g.gy, g.Y, m2, py),
+                    adv_axis<kAxSN>(g.gz, g.Z, m3, pz), e6 + 5 * g.n0);
+  }
+}
+
+// second invariant of the strain rate (mark3mg.c:229)
+__device__ __forceinline__ double adv_eii(const double *e) {
+  return sqrt(0.5 * (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) + e[3] * e[3] + e[4] * e[4] + e[5] * e[5]);
+}
+
+// one movemark step per marker (mark3mg.c:208-520): the velocity at the
+// marker (Euler, markmod 1) or the RK4 combination of four stages, the
+// strain-rate invariant at the start (eii) and, for markers with
+// accumulated strain, its RK4 average added times dt; outgrid 2 keeps only
+// the motion across the faces an outside marker lies beyond; a normal
+// marker (type < 100) leaving the box gets markk = 0.  e6 may be null when
+// neither eii nor marke is asked for (the strain rates are then not read).
+__global__ void __launch_bounds__(256) k_move_markers(AdvGeo g, const float *__restrict__ v3,
+                                                      const float *__restrict__ e6, long nmark, float *mx,
+                                                      float *my, float *mz, float *marke, float *markk,
+                                                      const int *__restrict__ markt, float *eii, double dt,
+                                                      int markmod, int outgrid) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= nmark) return;
+  const int t = markt ? markt[i] : 0;
+  const float fx = mx[i], fy = my[i], fz = mz[i];
+  const bool inside = fx >= 0 && fy >= 0 && fz >= 0 && (double)fx <= g.xs && (double)fy <= g.ys && (double)fz <= g.zs;
+  if (!(inside || (outgrid != 1 && (t < 50 || t >= 100)))) return;
+  const bool wantE = e6 && marke && marke[i] > 0;
+  double v1[3], e[6], vv[3];
+  adv_interp(g, v3, e6, fx, fy, fz, v1, e6 ? e : nullptr);
+  const double ee1 = e6 ? adv_eii(e) : 0.0;
+  if (eii) eii[i] = (float)ee1;
+  double ee0;
+  if (markmod == 1) {
+    vv[0] = v1[0]; vv[1] = v1[1]; vv[2] = v1[2];
+    ee0 = ee1;
+  } else {
+    double v2[3], w3[3], v4[3], ee2 = 0, ee3 = 0, ee4 = 0;
+    adv_interp(g, v3, e6, fx + v1[0] * dt / 2.0, fy + v1[1] * dt / 2.0, fz + v1[2] * dt / 2.0, v2,
+               wantE ? e : nullptr);
+    if (wantE) ee2 = adv_eii(e);
+    adv_interp(g, v3, e6, fx + v2[0] * dt / 2.0, fy + v2[1] * dt / 2.0, fz + v2[2] * dt / 2.0, w3,
+               wantE ? e : nullptr);
+    if (wantE) ee3 = adv_eii(e);
+    adv_interp(g, v3, e6, fx + w3[0] * dt, fy + w3[1] * dt, fz + w3[2] * dt, v4, wantE ? e : nullptr);
+    if (wantE) ee4 = adv_eii(e);
+    for (int k = 0; k < 3; k++) vv[k] = (v1[k] + 2.0 * v2[k] + 2.0 * w3[k] + v4[k]) / 6.0;
+    ee0 = (ee1 + 2.0 * ee2 + 2.0 * ee3 + ee4) / 6.0;
+  }
+  if (outgrid == 2) {
+    if (fx < 0 || (double)fx > g.xs) vv[1] = vv[2] = 0;
+    if (fy < 0 || (double)fy > g.ys) vv[0] = vv[2] = 0;
+    if (fz < 0 || (double)fz > g.zs) vv[0] = vv[1] = 0;
+  }
+  const float nx = fx + dt * vv[0], ny = fy + dt * vv[1], nz = fz + dt * vv[2];
+  mx[i] = nx;
+  my[i] = ny;
+  mz[i] = nz;
+  if (t < 100) {
+    if (wantE) marke[i] = marke[i] + (float)(ee0 * dt);
+    if (markk && (nx < 0 || ny < 0 || nz < 0 || (double)nx > g.xs || (double)ny > g.ys || (double)nz > g.zs))
+      markk[i] = 0.0f;
+  }
+}
+
+}  // namespace stk

@@ -19,6 +19,7 @@
 
 #include "../../include/pardiso2_amd.h"
 #include "stk_kernels.h"
+#include "stk_advect.h"
 #include "stk_compat.h"
 #include "stk_markers.h"
 #include "stk_post.h"
@@ -2074,6 +2075,63 @@ int stk_stress_strain(stk_ctx *c, double *sig, double *eps) {
   if (eps) HIPCHK(hipMemcpyAsync(eps, c->epsr, n6 * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   return STK_OK;
+}
+
+int stk_move_markers(stk_ctx *c, long nmark, float *mx, float *my, float *mz, float *marke, float *markk,
+                     const int *markt, float *eii, double timestep, int markmod, int outgrid) {
+  if (!c || nmark < 0 || (nmark && (!mx || !my || !mz)) || outgrid < 0 || outgrid > 2) return STK_EINVAL;
+  if (!c->prepared || !c->X) return STK_ESTATE;
+  if (c->per[0] || c->per[1] || c->per[2]) return STK_EINVAL;
+  if (nmark == 0) return STK_OK;
+  Level &F = c->lv[0];
+  const size_t n6 = (size_t)6 * F.n;
+  // FP32 planes of vx, vy, vz and, when an invariant or strain is asked for, of the strain rates
+  const bool need_e = eii || marke;
+  if (need_e) {
+    if (!c->sig && dalloc(&c->sig, n6)) return STK_ENOMEM;
+    if (!c->epsr && dalloc(&c->epsr, n6)) return STK_ENOMEM;
+    hipLaunchKernelGGL(k_post_stress, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, post_geo(c), c->X, c->sig,
+                       c->epsr);
+  }
+  float *planes = nullptr;
+  if (dalloc(&planes, (size_t)(need_e ? 9 : 3) * F.n)) return STK_ENOMEM;
+  hipLaunchKernelGGL(k_adv_planes, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.n, c->X,
+                     need_e ? c->epsr : nullptr, planes, planes + 3 * F.n);
+  float *d[5] = {};
+  int *dt = nullptr;
+  float *de = nullptr;
+  float *h[5] = {mx, my, mz, marke, markk};
+  int rc = STK_OK;
+  for (int k = 0; k < 5 && !rc; k++)
+    if (h[k]) {
+      rc = dalloc(&d[k], nmark);
+      if (!rc) rc = hipMemcpyAsync(d[k], h[k], nmark * sizeof(float), hipMemcpyHostToDevice, c->st) ? STK_EHIP : 0;
+    }
+  if (!rc && markt) {
+    rc = dalloc(&dt, nmark);
+    if (!rc) rc = hipMemcpyAsync(dt, markt, nmark * sizeof(int), hipMemcpyHostToDevice, c->st) ? STK_EHIP : 0;
+  }
+  if (!rc && eii) rc = dalloc(&de, nmark);
+  if (!rc) {
+    AdvGeo g;
+    g.X = F.X; g.Y = F.Y; g.Z = F.Z; g.XY = F.XY;
+    g.gx = F.g64[0]; g.gy = F.g64[1]; g.gz = F.g64[2];
+    g.xs = F.g[0][F.X - 1] - F.g[0][0]; g.ys = F.g[1][F.Y - 1] - F.g[1][0]; g.zs = F.g[2][F.Z - 1] - F.g[2][0];
+    g.n0 = F.n;
+    hipLaunchKernelGGL(k_move_markers, dim3(blocks(nmark, 256)), dim3(256), 0, c->st, g, planes,
+                       need_e ? planes + 3 * F.n : nullptr, nmark, d[0], d[1], d[2], d[3], d[4], dt, de, timestep,
+                       markmod, outgrid);
+    for (int k = 0; k < 5 && !rc; k++)
+      if (h[k]) rc = hipMemcpyAsync(h[k], d[k], nmark * sizeof(float), hipMemcpyDeviceToHost, c->st) ? STK_EHIP : 0;
+    if (!rc && eii) rc = hipMemcpyAsync(eii, de, nmark * sizeof(float), hipMemcpyDeviceToHost, c->st) ? STK_EHIP : 0;
+    if (!rc) rc = hipStreamSynchronize(c->st) ? STK_EHIP : 0;
+  }
+  (void)hipStreamSynchronize(c->st);
+  for (int k = 0; k < 5; k++) dfree(d[k]);
+  dfree(dt);
+  dfree(de);
+  dfree(planes);
+  return rc;
 }
 
 int stk_pressure_gauge(stk_ctx *c, int px, int py, int pz, double pinit) {
