@@ -13,12 +13,25 @@
 
 namespace stk {
 
+// bins of the cell search: an axis [g[0], g[n-1]] cut into nb equal bins,
+// bin b holding the largest line index a <= n-2 with g[a] <= the bin's start
+struct AdvBins {
+  const int *t;
+  int nb;
+  double g0, inv;  // first line, nb / (g[n-1] - g[0])
+};
+
 struct AdvGeo {
   int X, Y, Z, XY;
   const double *gx, *gy, *gz;  // grid lines of level 0
   double xs, ys, zs;           // box sizes (last line minus first)
   long n0;
+  AdvBins bx, by, bz;
 };
+
+// bins per axis: 8 per cell, so a search takes a step or two on grids whose
+// spacing varies by less than 8x
+constexpr int kAdvBinsPerCell = 8;
 
 // the rules by which an axis brackets a point (mark3mg.c:3004-3023,
 // 3272-3285, 3390-3398, 3411-3427): a velocity component along its own
@@ -33,15 +46,18 @@ struct AxW {
   double w0, w1;
 };
 
-// cell search of m1serch/m2serch/m3serch (mark3mg.c:1138-1205)
-__device__ __forceinline__ int adv_serch(const double *__restrict__ g, int n, double p) {
-  int a = 0, b = n - 1;
-  do {
-    const int m = (a + b) / 2;
-    if (g[m] > p) b = m;
-    else a = m;
-  } while (b - a > 1);
-  return a > n - 2 ? n - 2 : a;
+// cell search of m1serch/m2serch/m3serch (mark3mg.c:1138-1205).  Their
+// bisection over strictly increasing lines returns the largest a <= n-2 with
+// g[a] <= p, 0 below g[0] and n-2 for a NaN; here the bin of p gives a start
+// and short walks up and down end on that same index whatever the start.
+__device__ __forceinline__ int adv_serch(const double *__restrict__ g, int n, const AdvBins &B, double p) {
+  if (p != p) return n - 2;
+  const double f = (p - B.g0) * B.inv;
+  const int b = f <= 0.0 ? 0 : (f >= (double)(B.nb - 1) ? B.nb - 1 : (int)f);
+  int a = B.t[b];
+  while (a < n - 2 && g[a + 1] <= p) a++;
+  while (a > 0 && g[a] > p) a--;
+  return a;
 }
 
 template <int K>
@@ -124,25 +140,31 @@ __device__ __forceinline__ void adv_interp(const AdvGeo &g, const float *__restr
   px = px < 0 ? 0 : (px > g.xs ? g.xs : px);
   py = py < 0 ? 0 : (py > g.ys ? g.ys : py);
   pz = pz < 0 ? 0 : (pz > g.zs ? g.zs : pz);
-  const int m1 = adv_serch(g.gx, g.X, px), m2 = adv_serch(g.gy, g.Y, py), m3 = adv_serch(g.gz, g.Z, pz);
+  const int m1 = adv_serch(g.gx, g.X, g.bx, px), m2 = adv_serch(g.gy, g.Y, g.by, py),
+            m3 = adv_serch(g.gz, g.Z, g.bz, pz);
+  // each bracketing serves two of the three sums that use it
   if (v) {
-    v[0] = adv_sum8(g, adv_axis<kAxVN>(g.gx, g.X, m1, px), adv_axis<kAxVC>(g.gy, g.Y, m2, py),
-                    adv_axis<kAxVC>(g.gz, g.Z, m3, pz), v3);
-    v[1] = adv_sum8(g, adv_axis<kAxVC>(g.gx, g.X, m1, px), adv_axis<kAxVN>(g.gy, g.Y, m2, py),
-                    adv_axis<kAxVC>(g.gz, g.Z, m3, pz), v3 + g.n0);
-    v[2] = adv_sum8(g, adv_axis<kAxVC>(g.gx, g.X, m1, px), adv_axis<kAxVC>(g.gy, g.Y, m2, py),
-                    adv_axis<kAxVN>(g.gz, g.Z, m3, pz), v3 + 2 * g.n0);
+    const AxW nx = adv_axis<kAxVN>(g.gx, g.X, m1, px), ny = adv_axis<kAxVN>(g.gy, g.Y, m2, py),
+              nz = adv_axis<kAxVN>(g.gz, g.Z, m3, pz);
+    const AxW cx = adv_axis<kAxVC>(g.gx, g.X, m1, px), cy = adv_axis<kAxVC>(g.gy, g.Y, m2, py),
+              cz = adv_axis<kAxVC>(g.gz, g.Z, m3, pz);
+    v[0] = adv_sum8(g, nx, cy, cz, v3);
+    v[1] = adv_sum8(g, cx, ny, cz, v3 + g.n0);
+    v[2] = adv_sum8(g, cx, cy, nz, v3 + 2 * g.n0);
   }
   if (e) {
-    const AxW cx = adv_axis<kAxCell>(g.gx, g.X, m1, px), cy = adv_axis<kAxCell>(g.gy, g.Y, m2, py),
-              cz = adv_axis<kAxCell>(g.gz, g.Z, m3, pz);
-    for (int k = 0; k < 3; k++) e[k] = adv_sum8(g, cx, cy, cz, e6 + (long)k * g.n0);
-    e[3] = adv_sum8(g, adv_axis<kAxSN>(g.gx, g.X, m1, px), adv_axis<kAxSN>(g.gy, g.Y, m2, py),
-                    adv_axis<kAxSC>(g.gz, g.Z, m3, pz), e6 + 3 * g.n0);
-    e[4] = adv_sum8(g, adv_axis<kAxSN>(g.gx, g.X, m1, px), adv_axis<kAxSC>(g.gy, g.Y, m2, py),
-                    adv_axis<kAxSN>(g.gz, g.Z, m3, pz), e6 + 4 * g.n0);
-    e[5] = adv_sum8(g, adv_axis<kAxSC>(g.gx, g.X, m1, px), adv_axis<kAxSN>(g.gy, g.Y, m2, py),
-                    adv_axis<kAxSN>(g.gz, g.Z, m3, pz), e6 + 5 * g.n0);
+    {
+      const AxW cx = adv_axis<kAxCell>(g.gx, g.X, m1, px), cy = adv_axis<kAxCell>(g.gy, g.Y, m2, py),
+                cz = adv_axis<kAxCell>(g.gz, g.Z, m3, pz);
+      for (int k = 0; k < 3; k++) e[k] = adv_sum8(g, cx, cy, cz, e6 + (long)k * g.n0);
+    }
+    const AxW nx = adv_axis<kAxSN>(g.gx, g.X, m1, px), ny = adv_axis<kAxSN>(g.gy, g.Y, m2, py),
+              nz = adv_axis<kAxSN>(g.gz, g.Z, m3, pz);
+    const AxW cx = adv_axis<kAxSC>(g.gx, g.X, m1, px), cy = adv_axis<kAxSC>(g.gy, g.Y, m2, py),
+              cz = adv_axis<kAxSC>(g.gz, g.Z, m3, pz);
+    e[3] = adv_sum8(g, nx, ny, cz, e6 + 3 * g.n0);
+    e[4] = adv_sum8(g, nx, cy, nz, e6 + 4 * g.n0);
+    e[5] = adv_sum8(g, cx, ny, nz, e6 + 5 * g.n0);
   }
 }
 

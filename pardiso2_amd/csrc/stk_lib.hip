@@ -245,6 +245,7 @@ struct stk_ctx {
   bool cinv_ok = false;
   // post-solve pass: block partials of the norms, stress and strain-rate planes (6 n0 each)
   double *post_part = nullptr, *sig = nullptr, *epsr = nullptr;
+  int *adv_bins = nullptr;  // cell-search bins of the three axes (marker advection)
 };
 
 namespace {
@@ -1347,6 +1348,14 @@ int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *
   if (!out || levels < 0 || levels >= MAXLV || nx < 5 || ny < 5 || nz < 5 || !gx || !gy || !gz)
     return STK_EINVAL;
   if (xper || yper || zper) return STK_EINVAL;  // periodic boxes: not supported by the device path
+  {
+    // grid lines strictly increasing (every spacing is a divisor)
+    const double *gl[3] = {gx, gy, gz};
+    const int nl[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; a++)
+      for (int i = 0; i + 1 < nl[a]; i++)
+        if (!(gl[a][i + 1] > gl[a][i])) return STK_EINVAL;
+  }
   stk_ctx *c = new stk_ctx();
   c->L = levels;
   {
@@ -1474,7 +1483,7 @@ void stk_destroy(stk_ctx *c) {
   dfree(c->ainv64); dfree(c->ainvP64); dfree(c->ainv32); dfree(c->cact); dfree(c->cinv);
   dfree(c->X); dfree(c->B); dfree(c->R); dfree(c->W2); dfree(c->T1); dfree(c->red); dfree(c->scal);
   dfree(c->wu); dfree(c->ww); dfree(c->wz);
-  dfree(c->post_part); dfree(c->sig); dfree(c->epsr);
+  dfree(c->post_part); dfree(c->sig); dfree(c->epsr); dfree(c->adv_bins);
   dfree(c->kred); dfree(c->hcol); dfree(c->gin); dfree(c->gout);
   RtLock lk(g_rt_mu);
   if (c->hpin) hipHostFree(c->hpin);
@@ -2112,12 +2121,41 @@ int stk_move_markers(stk_ctx *c, long nmark, float *mx, float *my, float *mz, fl
     if (!rc) rc = hipMemcpyAsync(dt, markt, nmark * sizeof(int), hipMemcpyHostToDevice, c->st) ? STK_EHIP : 0;
   }
   if (!rc && eii) rc = dalloc(&de, nmark);
+  // cell-search bins, built once per context from the level-0 lines
+  const int nl[3] = {F.X, F.Y, F.Z};
+  const int nbin[3] = {kAdvBinsPerCell * (F.X - 1), kAdvBinsPerCell * (F.Y - 1), kAdvBinsPerCell * (F.Z - 1)};
+  if (!rc && !c->adv_bins) {
+    std::vector<int> t;
+    for (int a = 0; a < 3; a++) {
+      const double *gl = F.g[a].data();
+      const double h = (gl[nl[a] - 1] - gl[0]) / nbin[a];
+      int i = 0;
+      for (int b = 0; b < nbin[a]; b++) {
+        const double s0 = gl[0] + b * h;
+        while (i < nl[a] - 2 && gl[i + 1] <= s0) i++;
+        t.push_back(i);
+      }
+    }
+    rc = dalloc(&c->adv_bins, t.size());
+    if (!rc) rc = hipMemcpy(c->adv_bins, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice) ? STK_EHIP : 0;
+    if (rc) {
+      dfree(c->adv_bins);
+      c->adv_bins = nullptr;
+    }
+  }
   if (!rc) {
     AdvGeo g;
     g.X = F.X; g.Y = F.Y; g.Z = F.Z; g.XY = F.XY;
     g.gx = F.g64[0]; g.gy = F.g64[1]; g.gz = F.g64[2];
     g.xs = F.g[0][F.X - 1] - F.g[0][0]; g.ys = F.g[1][F.Y - 1] - F.g[1][0]; g.zs = F.g[2][F.Z - 1] - F.g[2][0];
     g.n0 = F.n;
+    AdvBins *bn[3] = {&g.bx, &g.by, &g.bz};
+    for (int a = 0, o = 0; a < 3; o += nbin[a], a++) {
+      bn[a]->t = c->adv_bins + o;
+      bn[a]->nb = nbin[a];
+      bn[a]->g0 = F.g[a][0];
+      bn[a]->inv = nbin[a] / (F.g[a][nl[a] - 1] - F.g[a][0]);
+    }
     hipLaunchKernelGGL(k_move_markers, dim3(blocks(nmark, 256)), dim3(256), 0, c->st, g, planes,
                        need_e ? planes + 3 * F.n : nullptr, nmark, d[0], d[1], d[2], d[3], d[4], dt, de, timestep,
                        markmod, outgrid);

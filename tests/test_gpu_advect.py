@@ -24,10 +24,14 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def _markers(rng, n, size):
+def _markers(rng, n, size, lines=None):
     p = rng.uniform(-0.03, 1.03, (n, 3)) * size
     p[: n // 8] = rng.random((n // 8, 3)) * size * 0.02  # near the lower corner
     p[n // 8: n // 4, 0] = 0.0                            # on a wall
+    if lines is not None:                                 # on grid lines (as float32 holds them)
+        k = slice(n // 4, n // 4 + n // 8)
+        for a in range(3):
+            p[k, a] = rng.choice(lines[a], n // 8)
     return [np.ascontiguousarray(p[:, a], dtype=np.float32) for a in range(3)]
 
 
@@ -46,7 +50,8 @@ def test_step_is_bit_identical_to_the_oracle(oracle_mod, markmod, outgrid):
         size = np.array([pb.gx[-1], pb.gy[-1], pb.gz[-1]])
         rng = np.random.default_rng(10 + markmod + 3 * outgrid)
         nm = 20000
-        dev = _markers(rng, nm, size)
+        lines = (pb.gx, pb.gy, pb.gz)
+        dev = _markers(rng, nm, size, lines)
         marke = rng.uniform(-0.5, 1.0, nm).astype(np.float32)
         markk = np.full(nm, 300.0, np.float32)
         markt = rng.choice([0, 60, 120], nm).astype(np.int32)
@@ -63,7 +68,7 @@ def test_step_is_bit_identical_to_the_oracle(oracle_mod, markmod, outgrid):
         np.testing.assert_array_equal(marke, marke_o)
         np.testing.assert_array_equal(markk, markk_o)
         np.testing.assert_array_equal(eii, eii_o)
-        moved = np.abs(dev[0] - _markers(np.random.default_rng(10 + markmod + 3 * outgrid), nm, size)[0]).max()
+        moved = np.abs(dev[0] - _markers(np.random.default_rng(10 + markmod + 3 * outgrid), nm, size, lines)[0]).max()
         assert moved > 0
         _say(f"markmod {markmod} outgrid {outgrid}: {nm} markers bit-identical, {int((markk == 0).sum())} left "
              f"the box, max eii {eii.max():.3e}")
@@ -107,5 +112,50 @@ def test_rk4_step_of_a_linear_flow_on_the_device():
             k = A @ p + B
             want = p + dt * (k + dt / 2 * (A @ k) + dt ** 2 / 6 * (A @ A @ k) + dt ** 3 / 24 * (A @ A @ A @ k))
             np.testing.assert_allclose([mx[i], my[i], mz[i]], want, rtol=0, atol=2e-6)
+    finally:
+        g.close()
+
+
+def test_step_on_a_refined_grid_is_bit_identical_to_the_oracle(oracle_mod):
+    """Lines 20x denser over the middle third of each axis: the cell search
+    walks over many lines per bin; an arbitrary velocity field."""
+    N = 29
+    grids = []
+    for s in (1.0, 0.8, 1.3):
+        h = np.ones(N - 1)
+        h[(N - 1) // 3: 2 * (N - 1) // 3] = 0.05
+        grids.append(np.concatenate([[0.0], np.cumsum(h / h.sum() * s)]))
+    g = S.Stokes(*grids, 1)
+    try:
+        g.knobs(p0koef=1.0, p1koef=1.0, p2koef=0.0, nubeg=1e-3, nuend=1e3, viscmod=0)
+        g.bc_lines(setups.free_slip_lines(N, N, N, gauge=None))
+        one = np.ones(g.n0, np.float32)
+        g.set_fields(one, one * 0, one, one, one, one)
+        g.prepare()
+        rng = np.random.default_rng(5)
+        x = rng.standard_normal(4 * g.n0)
+        g.set_solution(x)
+        _, eps = g.stress_strain()
+        o = oracle_mod.Oracle(*grids, 1)
+        size = np.array([gg[-1] for gg in grids])
+        nm = 30000
+        dev = _markers(rng, nm, size, grids)
+        ref = [a.copy() for a in dev]
+        marke = np.full(nm, 0.25, np.float32)
+        marke_o = marke.copy()
+        eii = g.move_markers(*dev, marke=marke, timestep=0.01)
+        eii_o = o.move_markers(x, eps, *ref, marke=marke_o, timestep=0.01)
+        for a, b, name in zip(dev, ref, "xyz"):
+            np.testing.assert_array_equal(a, b, err_msg=name)
+        np.testing.assert_array_equal(marke, marke_o)
+        np.testing.assert_array_equal(eii, eii_o)
+        # the velocity-only step (no strain rates read) moves them the same way
+        dev2 = _markers(np.random.default_rng(6), nm, size, grids)
+        ref2 = [a.copy() for a in dev2]
+        assert g.move_markers(*dev2, timestep=0.01, want_eii=False) is None
+        o.move_markers(x, eps, *ref2, timestep=0.01)
+        for a, b, name in zip(dev2, ref2, "xyz"):
+            np.testing.assert_array_equal(a, b, err_msg=name)
+        _say(f"refined grid: {nm} markers bit-identical, with and without strain")
     finally:
         g.close()

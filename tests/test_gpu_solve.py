@@ -120,6 +120,12 @@ def test_error_paths_return_codes():
         g.close()
     L = S.lib()
     assert L.gausmat4(1, 0, 0, 0) in (0, -8)
+    # grid lines that do not increase strictly are refused
+    for k in (4, len(pb.gx) - 1):
+        gx = pb.gx.copy()
+        gx[k] = gx[k - 1]
+        with pytest.raises(RuntimeError):
+            S.Stokes(gx, pb.gy, pb.gz, pb.levels)
 
 
 def test_solve_reports_nonconvergence_instead_of_exiting():
