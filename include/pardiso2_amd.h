@@ -129,7 +129,8 @@ typedef struct {
 
 /* Level hierarchy from the fine grid lines (restates gridcheck,
  * parallel/load3mg.c:2011-2195): mg[l] = (mg[l-1]-5)/2+5, coarse lines keep
- * the three outer lines per side.  levels = coarsest level index (multinum). */
+ * the three outer lines per side.  levels = coarsest level index (multinum).
+ * Grid lines must increase strictly (STK_EINVAL otherwise). */
 int stk_create(stk_ctx **ctx, int nx, int ny, int nz, int levels, const double *gx,
                const double *gy, const double *gz, int xperiodic, int yperiodic, int zperiodic);
 void stk_destroy(stk_ctx *ctx);
