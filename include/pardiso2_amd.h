@@ -130,7 +130,13 @@ typedef struct {
 /* Level hierarchy from the fine grid lines (restates gridcheck,
  * parallel/load3mg.c:2011-2195): mg[l] = (mg[l-1]-5)/2+5, coarse lines keep
  * the three outer lines per side.  levels = coarsest level index (multinum).
- * Grid lines must increase strictly (STK_EINVAL otherwise). */
+ * Grid lines must increase strictly (STK_EINVAL otherwise).  A periodic axis
+ * (the reference's xperiodic/yperiodic/zperiodic) identifies its last grid
+ * line with line 0: the stencils wrap as the reference's periodic branches
+ * do (parallel/move3mg.c:27-34, 400-429, 834-847, 1481-1483), the normal
+ * velocity on the last line needs the copy equation of a periodic boundary
+ * line (shift magnitude > 1, parallel/in3mg.c:339-341), and marker averaging
+ * folds the last line into line 0 (parallel/mark3mg.c:893-955, 1032-1111). */
 int stk_create(stk_ctx **ctx, int nx, int ny, int nz, int levels, const double *gx,
                const double *gy, const double *gz, int xperiodic, int yperiodic, int zperiodic);
 void stk_destroy(stk_ctx *ctx);
@@ -184,14 +190,16 @@ int stk_solve(stk_ctx *ctx, const stk_solver_opts *opts, stk_stats *stats);
 int stk_relres(stk_ctx *ctx, double *relres, double blocks[3]);
 /* The reference's convergence norms of the current solution (the post-solve
  * pass of vpiterate, parallel/move3mg.c:2835-3238; every stk_solve reports
- * them for its result in stk_stats.norms).  Non-periodic boxes. */
+ * them for its result in stk_stats.norms), periodic copies included. */
 int stk_norms_eval(stk_ctx *ctx, stk_norms *out);
 /* Stress and strain-rate recompute of the current solution
  * (parallel/move3mg.c:2908-3046): sig and eps get 6 planes of mgn[0]
  * doubles each, in the order xx yy zz xy xz yz, node layout of level 0
  * (sxx..szz at the cell whose upper corner is the node, shear stresses at
  * the node's edge); zero where the reference's recompute writes nothing.
- * Either pointer may be NULL. */
+ * On a periodic axis the shear stresses of the last line are line 0's and
+ * the normal ones of line 0 the last cell's (move3mg.c:2933-3044,
+ * 3321-3379).  Either pointer may be NULL. */
 int stk_stress_strain(stk_ctx *ctx, double *sig, double *eps);
 /* One marker-advection step through the current solution: movemark
  * (parallel/mark3mg.c:174-629) with the velocity and strain-rate
@@ -206,7 +214,9 @@ int stk_stress_strain(stk_ctx *ctx, double *sig, double *eps);
  * marker that leaves the box; markt, the marker types (< 50 normal, >= 100
  * immobile; NULL: all normal); eii receives the strain-rate invariant at the
  * start position (the reference's val1).  The water-marker branch and the
- * immobile-marker reset, both disabled in the reference, are not built. */
+ * immobile-marker reset, both disabled in the reference, are not built.
+ * Periodic boxes: STK_EINVAL (the periodic interpolation branches of
+ * mark3mg.c:2992-3760 are not built). */
 int stk_move_markers(stk_ctx *ctx, long nmark, float *mx, float *my, float *mz, float *marke, float *markk,
                      const int *markt, float *eii, double timestep, int markmod, int outgrid);
 /* Hydrostatic initial guess: v = 0, P integrated up each column through the

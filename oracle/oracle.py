@@ -324,7 +324,7 @@ def from_problem(pb, L=None):
     """Oracle instance for a setups.Problem: hierarchy, knobs, BCs, markers,
     restricted properties, assembled operator and factored coarsest level."""
     L = pb.levels if L is None else L
-    o = Oracle(pb.gx, pb.gy, pb.gz, L)
+    o = Oracle(pb.gx, pb.gy, pb.gz, L, periodic=getattr(pb, "periodic", (0, 0, 0)))
     k = pb.knobs
     o.knobs(k["p0koef"], k["p1koef"], k["p2koef"], k["v0koef"], k["v1koef"], k["nubeg"], k["nuend"],
             k["viscmod"], pb.gravity)

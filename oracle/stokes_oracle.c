@@ -218,8 +218,11 @@ int orc_bc_line(orc_t *o, int var, int m10, int m11, int m20, int m21, int m30, 
 /* Marker material values are given per marker (the outputs of viscalc and   */
 /* dencalc1, mark3mg.c:1418-1498).  Cell search follows m1serch/m2serch/      */
 /* m3serch (mark3mg.c:1138-1205) via prcalc1 (mark3mg.c:1285-1304).           */
-/* Non-periodic boxes only (the periodic folds of mark3mg.c:893-955 are not  */
-/* restated here).                                                            */
+/* Periodic axes: the sums of the last grid line are added to line 0 after  */
+/* the marker loop (mark3mg.c:893-955) and line 0's values are copied back   */
+/* to the last line after normalising (mark3mg.c:1032-1111); each property  */
+/* folds along the axes whose grid lines it sits on (nu, ro: all; nxy: x, y; */
+/* nxz: x, z; nyz: y, z; nxx: none).                                          */
 /* ------------------------------------------------------------------------- */
 static long serch(const double *g, int n, double x) {
   long a = 0, b = n - 1, m;
@@ -233,7 +236,6 @@ static long serch(const double *g, int n, double x) {
 
 int orc_ronurecalc(orc_t *o, long nmark, const float *mx, const float *my, const float *mz,
                    const float *mnu_in, const float *mro_in, double nukoef) {
-  if (o->xper || o->yper || o->zper) return -1;
   const int nx = o->mgx[0], ny = o->mgy[0], nz = o->mgz[0];
   const long nn = o->mgn[0];
   const double *gx = o->mggx[0], *gy = o->mggy[0], *gz = o->mggz[0];
@@ -316,6 +318,29 @@ int orc_ronurecalc(orc_t *o, long nmark, const float *mx, const float *my, const
       }
     }
   }
+  /* periodic folds, in the reference's order x, y, z (mark3mg.c:893-955) */
+  const long nxy = (long)nx * ny;
+  if (o->xper)
+    for (long m3 = 0; m3 < nz; m3++)
+      for (long m2 = 0; m2 < ny; m2++) {
+        long a = m3 * nxy + m2, b = a + (long)(nx - 1) * ny;
+        wro[a] += wro[b]; wnu[a] += wnu[b]; wxy[a] += wxy[b]; wxz[a] += wxz[b];
+        aro[a] += aro[b]; anu[a] += anu[b]; axy[a] += axy[b]; axz[a] += axz[b];
+      }
+  if (o->yper)
+    for (long m3 = 0; m3 < nz; m3++)
+      for (long m1 = 0; m1 < nx; m1++) {
+        long a = m3 * nxy + m1 * ny, b = a + ny - 1;
+        wro[a] += wro[b]; wnu[a] += wnu[b]; wxy[a] += wxy[b]; wyz[a] += wyz[b];
+        aro[a] += aro[b]; anu[a] += anu[b]; axy[a] += axy[b]; ayz[a] += ayz[b];
+      }
+  if (o->zper)
+    for (long m1 = 0; m1 < nx; m1++)
+      for (long m2 = 0; m2 < ny; m2++) {
+        long a = m1 * ny + m2, b = a + (long)(nz - 1) * nxy;
+        wro[a] += wro[b]; wnu[a] += wnu[b]; wxz[a] += wxz[b]; wyz[a] += wyz[b];
+        aro[a] += aro[b]; anu[a] += anu[b]; axz[a] += axz[b]; ayz[a] += ayz[b];
+      }
   /* normalise, back-transform, clamp: mark3mg.c:962-1031 */
   for (long m3 = 0; m3 < nn; m3++) {
     if (wnu[m3] > 0) {
@@ -355,6 +380,30 @@ int orc_ronurecalc(orc_t *o, long nmark, const float *mx, const float *my, const
     if (o->nyz[m3] < o->nubeg) o->nyz[m3] = o->nubeg;
     if (o->nyz[m3] > o->nuend) o->nyz[m3] = o->nuend;
   }
+  /* the last line of a periodic axis takes line 0's values (mark3mg.c:1032-1111) */
+  if (o->xper)
+    for (long m3 = 0; m3 < nz; m3++)
+      for (long m2 = 0; m2 < ny; m2++) {
+        long a = m3 * nxy + m2, b = a + (long)(nx - 1) * ny;
+        o->ro[b] = o->ro[a]; o->nu[b] = o->nu[a]; o->nxy[b] = o->nxy[a]; o->nxz[b] = o->nxz[a];
+      }
+  if (o->yper)
+    for (long m3 = 0; m3 < nz; m3++)
+      for (long m1 = 0; m1 < nx; m1++) {
+        long a = m3 * nxy + m1 * ny, b = a + ny - 1;
+        o->ro[b] = o->ro[a]; o->nu[b] = o->nu[a]; o->nxy[b] = o->nxy[a]; o->nyz[b] = o->nyz[a];
+      }
+  if (o->zper)
+    for (long m1 = 0; m1 < nx; m1++)
+      for (long m2 = 0; m2 < ny; m2++) {
+        long a = m1 * ny + m2, b = a + (long)(nz - 1) * nxy;
+        o->ro[b] = o->ro[a]; o->nu[b] = o->nu[a]; o->nxz[b] = o->nxz[a]; o->nyz[b] = o->nyz[a];
+      }
+  if (o->xper && o->zper)
+    for (long m2 = 0; m2 < ny; m2++) {
+      const long b3[3] = {m2 + (long)(nx - 1) * ny, m2 + (long)(nz - 1) * nxy, m2 + (long)(nz - 1) * nxy + (long)(nx - 1) * ny};
+      for (int k = 0; k < 3; k++) { o->ro[b3[k]] = o->ro[m2]; o->nu[b3[k]] = o->nu[m2]; o->nxz[b3[k]] = o->nxz[m2]; }
+    }
   free(anu); free(aro); free(axx); free(axy); free(axz); free(ayz); free(w);
   return 0;
 }
@@ -1928,13 +1977,23 @@ void orc_post(const orc_t *o, const double *x, double *sig, double *eps, double 
     for (int m1 = 0; m1 < X; m1++)
       for (int m2 = 0; m2 < Y; m2++) {
         long n = NID(o, 0, m1, m2, m3), r = 4 * n;
-        /* stress and strain-rate planes */
+        /* stress and strain-rate planes; on line 0 of a periodic axis the  */
+        /* normal ones are copied from the last line after the solve        */
+        /* (move3mg.c:3321-3379, x then y then z)                            */
         for (int k = 0; k < 6; k++) {
           long s1, s2, s3;
           double e = 0.0, s = 0.0, nuk = 0.0;
           if (post_src(o, k, m1, m2, m3, &s1, &s2, &s3)) {
             s = post_sig(o, k, s1, s2, s3, x, &e, &nuk);
             if (k < 3) { if (nuk < pp[18]) pp[18] = nuk; if (nuk > pp[19]) pp[19] = nuk; }
+          }
+          if (k < 3) {
+            long c1 = (o->xper && m1 == 0) ? X - 1 : m1, c2 = (o->yper && m2 == 0) ? Y - 1 : m2;
+            long c3 = (o->zper && m3 == 0) ? Z - 1 : m3;
+            if ((c1 != m1 || c2 != m2 || c3 != m3)) {
+              e = 0.0; s = 0.0;
+              if (post_src(o, k, c1, c2, c3, &s1, &s2, &s3)) s = post_sig(o, k, s1, s2, s3, x, &e, NULL);
+            }
           }
           if (sig) sig[(size_t)k * n0 + (n - o->mgp[0])] = s;
           if (eps) eps[(size_t)k * n0 + (n - o->mgp[0])] = e;

@@ -218,12 +218,13 @@ def default_opts(levels: int, **kw) -> Opts:
 class Stokes:
     """One Stokes problem resident on the GPU (a stk_ctx)."""
 
-    def __init__(self, gx, gy, gz, levels: int):
+    def __init__(self, gx, gy, gz, levels: int, periodic=(0, 0, 0)):
         self.levels = levels
+        self.periodic = tuple(int(bool(v)) for v in periodic)
         self._g = [np.ascontiguousarray(a, dtype=np.float64) for a in (gx, gy, gz)]
         h = C.c_void_p()
         check(lib().stk_create(C.byref(h), len(gx), len(gy), len(gz), levels, _dp(self._g[0]),
-                               _dp(self._g[1]), _dp(self._g[2]), 0, 0, 0), "stk_create")
+                               _dp(self._g[1]), _dp(self._g[2]), *self.periodic), "stk_create")
         self.h = h
         self.info = []
         for l in range(levels + 1):
@@ -452,7 +453,7 @@ def mem_info():
 def from_problem(pb, markers=True, knobs=None):
     """Stokes context for a setups.Problem: hierarchy, knobs, boundary lines,
     node fields from the markers (on the device), prepared operators."""
-    s = Stokes(pb.gx, pb.gy, pb.gz, pb.levels)
+    s = Stokes(pb.gx, pb.gy, pb.gz, pb.levels, periodic=getattr(pb, "periodic", (0, 0, 0)))
     k = dict(pb.knobs)
     if knobs:
         k.update(knobs)

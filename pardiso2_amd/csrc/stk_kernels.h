@@ -62,10 +62,18 @@ __device__ __forceinline__ double clampnu(double e, double nb, double ne) {
   return e;
 }
 
-__device__ __forceinline__ double mean8(const float *nu, long v0, int Y, int XY, int vm) {
+// 8-node mean of the cell whose lower corner is node v0.  On the last cell
+// of a periodic axis the reference swaps one node of the octet for its
+// line-0 copy (x: v[2], y: v[1], z: v[4]; move3mg.c:27-29), and so does
+// this (wx, wy, wz: the replacement nodes, or -1).
+__device__ __forceinline__ double mean8(const float *nu, long v0, int Y, int XY, int vm, long wx = -1,
+                                        long wy = -1, long wz = -1) {
   long v[8];
   v[0] = v0; v[1] = v0 + 1; v[2] = v0 + Y; v[3] = v[2] + 1;
   v[4] = v0 + XY; v[5] = v[4] + 1; v[6] = v[4] + Y; v[7] = v[6] + 1;
+  if (wx >= 0) v[2] = wx;
+  if (wy >= 0) v[1] = wy;
+  if (wz >= 0) v[4] = wz;
   if (vm == 1) {
     double s = log((double)nu[v[0]]);
     for (int k = 1; k < 8; k++) s = s + log((double)nu[v[k]]);
@@ -92,7 +100,7 @@ __global__ void k_visc_prep(int X, int Y, int Z, int level0, const float *__rest
                             const float *__restrict__ nxz, const float *__restrict__ nyz,
                             double nubeg, double nuend, int vm, double *e64c, double *e64xy,
                             double *e64xz, double *e64yz, double *e64d, float *e32c, float *e32xy,
-                            float *e32xz, float *e32yz, float *e32d) {
+                            float *e32xz, float *e32yz, float *e32d, int per) {
   long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
   const int XY = X * Y;
   if (n >= (long)XY * Z) return;
@@ -101,7 +109,10 @@ __global__ void k_visc_prep(int X, int Y, int Z, int level0, const float *__rest
   double ec = nubeg, ed = nubeg, exy = nubeg, exz = nubeg, eyz = nubeg;
   if (m1 >= 1 && m2 >= 1 && m3 >= 1) {
     long v0 = n - XY - Y - 1;
-    ed = clampnu(mean8(nu, v0, Y, XY, vm), nubeg, nuend);
+    const long wx = ((per & 1) && m1 == X - 1) ? (long)(m3 - 1) * XY + (m2 - 1) : -1;
+    const long wy = ((per & 2) && m2 == Y - 1) ? (long)(m3 - 1) * XY + (long)(m1 - 1) * Y : -1;
+    const long wz = ((per & 4) && m3 == Z - 1) ? (long)(m1 - 1) * Y + (m2 - 1) : -1;
+    ed = clampnu(mean8(nu, v0, Y, XY, vm, wx, wy, wz), nubeg, nuend);
     ec = level0 ? clampnu((double)nxx[n], nubeg, nuend) : ed;
   }
   if (level0) {
@@ -118,24 +129,59 @@ __global__ void k_visc_prep(int X, int Y, int Z, int level0, const float *__rest
   e32d[n] = (float)ed;
 }
 
+// Periodic copies of the stencil viscosities (after k_visc_prep): a cell
+// plane at index 0 of a periodic axis holds the value of its cell n-1, an
+// edge plane at index n-1 the value of its edge on line 0 (pwrap / vwrap),
+// so rows next to the wrap read the wrapped stress points at the offsets
+// they use elsewhere.  Sources are never copies themselves.
+__global__ void k_visc_wrap(int X, int Y, int Z, int per, double *e64c, double *e64xy, double *e64xz,
+                            double *e64yz, double *e64d, float *e32c, float *e32xy, float *e32xz, float *e32yz,
+                            float *e32d) {
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XY = X * Y;
+  if (n >= (long)XY * Z) return;
+  int m1, m2, m3;
+  decode(n, X, Y, XY, m1, m2, m3);
+  const int c1 = (per & 1) ? pwrap(m1, X) : m1, c2 = (per & 2) ? pwrap(m2, Y) : m2, c3 = (per & 4) ? pwrap(m3, Z) : m3;
+  const long sc = (long)c3 * XY + (long)c1 * Y + c2;
+  if (sc != n) {
+    e64c[n] = e64c[sc]; e64d[n] = e64d[sc];
+    e32c[n] = e32c[sc]; e32d[n] = e32d[sc];
+  }
+  const int v1 = (per & 1) ? vwrap(m1, X) : m1, v2 = (per & 2) ? vwrap(m2, Y) : m2, v3 = (per & 4) ? vwrap(m3, Z) : m3;
+  const long se = (long)v3 * XY + (long)v1 * Y + v2;
+  if (se != n) {
+    e64xy[n] = e64xy[se]; e64xz[n] = e64xz[se]; e64yz[n] = e64yz[se];
+    e32xy[n] = e32xy[se]; e32xz[n] = e32xz[se]; e32yz[n] = e32yz[se];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Finest-level right-hand side: gravity terms of the momentum rows
 // (move3mg.c:848, 1061, 1281), zero for continuity, the constant of
 // boundary equations (move3mg.c:1710-1712).
 // ---------------------------------------------------------------------------
 __global__ void k_rhs(int X, int Y, int Z, const unsigned char *__restrict__ flags,
-                      const float *__restrict__ ro, double gx, double gy, double gz, double *b) {
+                      const float *__restrict__ ro, double gx, double gy, double gz, double *b, int per) {
   long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
   const int XY = X * Y;
   if (n >= (long)XY * Z) return;
+  int m1, m2, m3;
+  decode(n, X, Y, XY, m1, m2, m3);
   unsigned char f = flags[n];
   double *bn = b + 4 * n;
+  // rows on line 0 of a periodic axis take the fourth density node from the
+  // last line (the node the reference also moves into the pressure slot:
+  // x row v[5], y row v[6], z row v[3]; move3mg.c:842, 1056, 1276)
+  const long wx = ((per & 1) && m1 == 0) ? n + XY + 1 + (long)(X - 1) * Y : n + XY + 1;
+  const long wy = ((per & 2) && m2 == 0) ? n + XY + Y + (Y - 1) : n + XY + Y;
+  const long wz = ((per & 4) && m3 == 0) ? n + Y + 1 + (long)(Z - 1) * XY : n + Y + 1;
   if (kind_of(f, 0) != KIND_BC) bn[0] = 0.0;
-  if (kind_of(f, 1) == KIND_EQ) bn[1] = -gx * (ro[n] + ro[n + 1] + ro[n + XY] + ro[n + XY + 1]) / 4.0;
+  if (kind_of(f, 1) == KIND_EQ) bn[1] = -gx * (ro[n] + ro[n + 1] + ro[n + XY] + ro[wx]) / 4.0;
   else if (kind_of(f, 1) == KIND_NONE) bn[1] = 0.0;
-  if (kind_of(f, 2) == KIND_EQ) bn[2] = -gy * (ro[n] + ro[n + Y] + ro[n + XY] + ro[n + XY + Y]) / 4.0;
+  if (kind_of(f, 2) == KIND_EQ) bn[2] = -gy * (ro[n] + ro[n + Y] + ro[n + XY] + ro[wy]) / 4.0;
   else if (kind_of(f, 2) == KIND_NONE) bn[2] = 0.0;
-  if (kind_of(f, 3) == KIND_EQ) bn[3] = -gz * (ro[n] + ro[n + 1] + ro[n + Y] + ro[n + Y + 1]) / 4.0;
+  if (kind_of(f, 3) == KIND_EQ) bn[3] = -gz * (ro[n] + ro[n + 1] + ro[n + Y] + ro[wz]) / 4.0;
   else if (kind_of(f, 3) == KIND_NONE) bn[3] = 0.0;
 }
 
@@ -318,6 +364,39 @@ __global__ void k_bc_compose(BcChain ch, const int *__restrict__ idx, const T *_
 
 // Stage a tile and its one-node halo of (P, vx, vy, vz) in LDS, component-
 // major (zeros outside the level), and the unknown kinds of the interior.
+// On a periodic axis a position outside the level's lines, or on a line
+// whose unknown is a copy, holds the unknown it names (vwrap for the
+// velocities, pwrap for P, stk_rows.h), so the rows next to the wrap read
+// what the reference's periodic stencils read; such positions in the halo
+// hold the values of xold like every other halo position.
+template <typename T>
+struct StagePos {
+  bool lev;         // inside the level (interior kinds are staged only there)
+  bool ld;          // a value is staged (else zeros)
+  unsigned nv, np;  // node of the velocities and of P
+};
+template <typename T>
+__device__ __forceinline__ StagePos<T> stage_pos(const Geo<T> &g, int m1, int m2, int m3, bool any) {
+  StagePos<T> s;
+  s.lev = any && m1 >= 0 && m1 < g.X && m2 >= 0 && m2 < g.Y && m3 >= 0 && m3 < g.Z;
+  if (!g.per) {
+    s.ld = s.lev;
+    s.nv = s.np = s.lev ? __umul24(m3, g.XY) + __umul24(m1, g.Y) + m2 : 0u;
+    return s;
+  }
+  const int per = g.per;
+  const bool ox = (per & 1) ? (m1 >= -1 && m1 <= g.X) : (m1 >= 0 && m1 < g.X);
+  const bool oy = (per & 2) ? (m2 >= -1 && m2 <= g.Y) : (m2 >= 0 && m2 < g.Y);
+  const bool oz = (per & 4) ? (m3 >= -1 && m3 <= g.Z) : (m3 >= 0 && m3 < g.Z);
+  s.ld = any && ox && oy && oz;
+  const int a1 = (per & 1) ? vwrap(m1, g.X) : m1, b1 = (per & 1) ? pwrap(m1, g.X) : m1;
+  const int a2 = (per & 2) ? vwrap(m2, g.Y) : m2, b2 = (per & 2) ? pwrap(m2, g.Y) : m2;
+  const int a3 = (per & 4) ? vwrap(m3, g.Z) : m3, b3 = (per & 4) ? pwrap(m3, g.Z) : m3;
+  s.nv = s.ld ? __umul24(a3, g.XY) + __umul24(a1, g.Y) + a2 : 0u;
+  s.np = s.ld ? __umul24(b3, g.XY) + __umul24(b1, g.Y) + b2 : 0u;
+  return s;
+}
+
 template <typename T, int TY, int TX, int TZ = 0, int BS = 256>
 __device__ __forceinline__ void tile_stage(const Geo<T> &g, const unsigned char *__restrict__ flags,
                                            const T *__restrict__ xold, T *sx, unsigned char *sf, const TileGeo &tg,
@@ -332,31 +411,35 @@ __device__ __forceinline__ void tile_stage(const Geo<T> &g, const unsigned char 
     // around a load, so no wait between them)
     constexpr int kIt = (kNh + BS - 1) / BS;
     V v[kIt];
+    T pv[kIt];
     unsigned char f[kIt];
-    bool inl[kIt], intr[kIt];
+    bool ld[kIt], lev[kIt], intr[kIt];
 #pragma unroll
     for (int k = 0; k < kIt; k++) {
       const int i = k * BS + (int)threadIdx.x;
       const int q = i / (TY + 2);
       const int ly = i - q * (TY + 2), lz = q / (TX + 2), lx = q - lz * (TX + 2);
       const int m2 = y0 + ly - 1, m1 = x0 + lx - 1, m3 = z0 + lz - 1;
-      inl[k] = i < kNh && m1 >= 0 && m1 < g.X && m2 >= 0 && m2 < g.Y && m3 >= 0 && m3 < g.Z;
+      const StagePos<T> sp = stage_pos<T>(g, m1, m2, m3, i < kNh);
+      ld[k] = sp.ld;
+      lev[k] = sp.lev;
       intr[k] = i < kNh && lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && lz >= 1 && lz <= TZ;
-      const unsigned n = inl[k] ? __umul24(m3, g.XY) + __umul24(m1, g.Y) + m2 : 0u;
-      v[k] = reinterpret_cast<const V *>(xold)[n];
-      f[k] = flags[n];
+      v[k] = reinterpret_cast<const V *>(xold)[sp.nv];
+      pv[k] = g.per ? xold[4 * (size_t)sp.np] : T(0);
+      f[k] = flags[sp.lev ? __umul24(m3, g.XY) + __umul24(m1, g.Y) + m2 : 0u];
     }
 #pragma unroll
     for (int k = 0; k < kIt; k++) {
       const int i = k * BS + (int)threadIdx.x;
       if (i >= kNh) break;
       const V z = {T(0), T(0), T(0), T(0)};
-      const V w = inl[k] ? v[k] : z;
+      V w = ld[k] ? v[k] : z;
+      if (g.per && ld[k]) w.x = pv[k];
       sx[i] = w.x; sx[kNh + i] = w.y; sx[2 * kNh + i] = w.z; sx[3 * kNh + i] = w.w;
       if (intr[k]) {
         const int q = i / (TY + 2);
         const int ly = i - q * (TY + 2), lz = q / (TX + 2), lx = q - lz * (TX + 2);
-        sf[((lz - 1) * TX + lx - 1) * TY + ly - 1] = inl[k] ? f[k] : (unsigned char)0;
+        sf[((lz - 1) * TX + lx - 1) * TY + ly - 1] = lev[k] ? f[k] : (unsigned char)0;
       }
     }
     return;
@@ -368,14 +451,27 @@ __device__ __forceinline__ void tile_stage(const Geo<T> &g, const unsigned char 
     V v = {T(0), T(0), T(0), T(0)};
     unsigned char f = 0;
     const bool interior = lx >= 1 && lx <= tx && ly >= 1 && ly <= ty && lz >= 1 && lz <= tz;
-    if (m1 >= 0 && m1 < g.X && m2 >= 0 && m2 < g.Y && m3 >= 0 && m3 < g.Z) {
-      const long n = g.nid(m1, m2, m3);
-      v = reinterpret_cast<const V *>(xold)[n];
-      if (interior) f = flags[n];  // kinds of the halo are not needed
+    const StagePos<T> sp = stage_pos<T>(g, m1, m2, m3, true);
+    if (sp.ld) {
+      v = reinterpret_cast<const V *>(xold)[sp.nv];
+      if (g.per) v.x = xold[4 * (size_t)sp.np];
     }
+    if (interior && sp.lev) f = flags[g.nid(m1, m2, m3)];  // kinds of the halo are not needed
     sx[i] = v.x; sx[nh + i] = v.y; sx[2 * nh + i] = v.z; sx[3 * nh + i] = v.w;
     if (interior) sf[__umul24(__umul24(lz - 1, tx) + lx - 1, ty) + ly - 1] = f;
   }
+}
+
+// Write-back value of a node of a tile: on a periodic level the unknowns a
+// node does not hold (kind none) are written as zeros, not as the copies
+// the staging put there.
+template <typename V>
+__device__ __forceinline__ V wb_mask(V o, unsigned char f) {
+  if (kind_of(f, 0) == KIND_NONE) o.x = 0;
+  if (kind_of(f, 1) == KIND_NONE) o.y = 0;
+  if (kind_of(f, 2) == KIND_NONE) o.z = 0;
+  if (kind_of(f, 3) == KIND_NONE) o.w = 0;
+  return o;
 }
 
 // SV: the tile's stencil viscosities (nodes [x0, x0+tx] x [y0, y0+ty] x
@@ -458,7 +554,9 @@ __global__ void __launch_bounds__(256) k_smooth(Geo<T> g, const unsigned char *_
     if (m1 >= g.X || m2 >= g.Y || m3 >= g.Z) continue;
     int li = ((lz + 1) * hx + (lx + 1)) * hy + (ly + 1);
     using V = typename V4<T>::t;
-    reinterpret_cast<V *>(xnew)[g.nid(m1, m2, m3)] = V{sx[li], sx[nh + li], sx[2 * nh + li], sx[3 * nh + li]};
+    V o = V{sx[li], sx[nh + li], sx[2 * nh + li], sx[3 * nh + li]};
+    if (g.per) o = wb_mask(o, sf[i]);
+    reinterpret_cast<V *>(xnew)[g.nid(m1, m2, m3)] = o;
   }
 }
 
@@ -612,7 +710,9 @@ __global__ void __launch_bounds__(BS) k_vsmooth32(Geo<float> g, const unsigned c
     const int n1 = x0 + lxt, n2 = y0 + ly, n3 = z0 + lzt;
     if (n1 >= g.X || n2 >= g.Y || n3 >= g.Z) continue;
     const int li = ((lzt + 1) * hx + (lxt + 1)) * hy + (ly + 1);
-    reinterpret_cast<float4 *>(xnew)[g.nid(n1, n2, n3)] = float4{sx[li], sx[nh + li], sx[2 * nh + li], sx[3 * nh + li]};
+    float4 o = float4{sx[li], sx[nh + li], sx[2 * nh + li], sx[3 * nh + li]};
+    if (g.per) o = wb_mask(o, sf[t]);
+    reinterpret_cast<float4 *>(xnew)[g.nid(n1, n2, n3)] = o;
   }
 }
 
@@ -1252,7 +1352,8 @@ __global__ void k_dense_rows(Geo<double> g, const unsigned char *__restrict__ fl
   double s[18];
   int ns = row_coef<double>(g, c, m1, m2, m3, s);
   for (int q = 1; q <= ns; q++) {
-    long uq = 4 * (n + (long)kSlot[c][q][2] * g.XY + (long)kSlot[c][q][0] * g.Y + kSlot[c][q][1]) + kSlot[c][q][3];
+    const int cq = kSlot[c][q][3];
+    long uq = 4 * unode(g, m1 + kSlot[c][q][0], m2 + kSlot[c][q][1], m3 + kSlot[c][q][2], cq) + cq;
     row[uq] += s[q];
   }
 }
@@ -1488,7 +1589,8 @@ __global__ void k_dense_rows_c(Geo<double> g, const unsigned char *__restrict__ 
   double s[18];
   const int ns = row_coef<double>(g, c, m1, m2, m3, s);
   for (int q = 1; q <= ns; q++) {
-    const long uq = 4 * (n + (long)kSlot[c][q][2] * g.XY + (long)kSlot[c][q][0] * g.Y + kSlot[c][q][1]) + kSlot[c][q][3];
+    const int cq = kSlot[c][q][3];
+    const long uq = 4 * unode(g, m1 + kSlot[c][q][0], m2 + kSlot[c][q][1], m3 + kSlot[c][q][2], cq) + cq;
     const int j = idx[uq];
     if (j >= 0) row[j] += s[q];
   }
@@ -1557,7 +1659,7 @@ __global__ void k_vrhs(Geo<double> g, const unsigned char *__restrict__ flags, c
     double acc = rr[c];
 #pragma unroll
     for (int q = 16; q <= 17; q++) {
-      const long uq = 4 * (n + (long)kSlot[c][q][2] * g.XY + (long)kSlot[c][q][0] * g.Y + kSlot[c][q][1]);
+      const long uq = 4 * unode(g, m1 + kSlot[c][q][0], m2 + kSlot[c][q][1], m3 + kSlot[c][q][2], 0);
       acc -= (q == 16 ? kf : -kf) * zp[uq];
     }
     o[c] = (T)acc;

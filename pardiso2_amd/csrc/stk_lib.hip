@@ -94,6 +94,7 @@ struct AxisDev {
 
 struct Level {
   int X = 0, Y = 0, Z = 0, XY = 0;
+  int per = 0;  // periodic axes (bit 0 x, bit 1 y, bit 2 z), the same on every level
   long n = 0, off = 0;
   std::vector<double> g[3];
   // device geometry
@@ -174,6 +175,7 @@ Geo<double> Level::geo<double>(double p2) const {
   g.iAx = iA64[0]; g.iBx = iB64[0]; g.iAy = iA64[1]; g.iBy = iB64[1]; g.iAz = iA64[2]; g.iBz = iB64[2];
   g.ec = e64[0]; g.exy = e64[1]; g.exz = e64[2]; g.eyz = e64[3]; g.ed = e64[4];
   g.p2 = p2;
+  g.per = per;
   return g;
 }
 template <>
@@ -183,6 +185,7 @@ Geo<float> Level::geo<float>(double p2) const {
   g.iAx = iA32[0]; g.iBx = iB32[0]; g.iAy = iA32[1]; g.iBy = iB32[1]; g.iAz = iA32[2]; g.iBz = iB32[2];
   g.ec = e32[0]; g.exy = e32[1]; g.exz = e32[2]; g.eyz = e32[3]; g.ed = e32[4];
   g.p2 = (float)p2;
+  g.per = per;
   return g;
 }
 
@@ -399,6 +402,7 @@ bool comp_exists(const Level &L, int m1, int m2, int m3, int c) {
 
 // equation rows must keep their stencil inside the grid (the reference has
 // no guard either: boundary equations cover those unknowns)
+// (on a periodic axis the stencil wraps: velocity rows on lines 0..n-2)
 bool eq_inside(const Level &L, int m1, int m2, int m3, int c) {
   if (c == 0) return true;
   int lo1 = 1, lo2 = 1, lo3 = 1;
@@ -406,6 +410,9 @@ bool eq_inside(const Level &L, int m1, int m2, int m3, int c) {
   if (c == 1) h1 = L.X - 2;
   if (c == 2) h2 = L.Y - 2;
   if (c == 3) h3 = L.Z - 2;
+  if (L.per & 1) { lo1 = 0; h1 = L.X - 2; }
+  if (L.per & 2) { lo2 = 0; h2 = L.Y - 2; }
+  if (L.per & 4) { lo3 = 0; h3 = L.Z - 2; }
   return m1 >= lo1 && m1 <= h1 && m2 >= lo2 && m2 <= h2 && m3 >= lo3 && m3 <= h3;
 }
 
@@ -605,7 +612,8 @@ void smooth(stk_ctx *c, int l, int sweeps, const int *tile, int withp, bool whol
   Level &L = c->lv[l];
   const bool tuned = !tile;
   if (!tile) tile = tuned_tile<T>(L);
-  if (whole_level && sweeps > 0 && 4 * (size_t)L.n * sizeof(T) <= kLevelLds) {
+  // (the whole-level kernels read neighbours by plain offsets: not on periodic levels)
+  if (whole_level && sweeps > 0 && !L.per && 4 * (size_t)L.n * sizeof(T) <= kLevelLds) {
     Geo<T> g = L.geo<T>(c->kn.p2koef);
     const LevelDiv ld{make_fdiv(L.Y), make_fdiv(L.X), make_fdiv((L.Y + 1) / 2)};
     const float *coef = std::is_same<T, float>::value ? L.lc32 : nullptr;
@@ -779,11 +787,13 @@ template <typename T> KPlanesC<T> kplanes(const Level &L);
 template <> KPlanesC<float> kplanes<float>(const Level &L) {
   KPlanesC<float> k;
   for (int q = 0; q < 7; q++) k.p[q] = L.k32[q];
+  k.per = L.per;
   return k;
 }
 template <> KPlanesC<double> kplanes<double>(const Level &L) {
   KPlanesC<double> k;
   for (int q = 0; q < 7; q++) k.p[q] = L.k64[q];
+  k.per = L.per;
   return k;
 }
 template <typename T> T *kx(Level &L);
@@ -871,7 +881,7 @@ void kcycle(stk_ctx *c, int l, int nu, T scale) {
         const int col = q == 0 ? c1 : 1 - c1;
         if (kc)
           hipLaunchKernelGGL(k_kgs_c<T>, dim3(blocks(kcolour_n(V), 256)), dim3(256), 0, c->st, V.X, V.Y, V.Z, dhy,
-                             dx, kc, kb<T>(V), kx<T>(V), col);
+                             dx, kc, kb<T>(V), kx<T>(V), col, V.per);
         else
           hipLaunchKernelGGL(k_kgs<T>, dim3(nb), dim3(256), 0, c->st, V.X, V.Y, V.Z, k, kb<T>(V), kx<T>(V), col);
       }
@@ -900,10 +910,10 @@ void schur_bfbt(stk_ctx *c, const stk_solver_opts *o, const double *r, double *z
   hipLaunchKernelGGL(k_fill<T>, dim3(nb), dim3(256), 0, c->st, F.n, T(0), kx<T>(F));
   kcycle<T>(c, 0, nu, sc);
   hipLaunchKernelGGL(k_bf_grad<T>, dim3(nb), dim3(256), 0, c->st, F.X, F.Y, F.Z, iBt<T>(F, 0), iBt<T>(F, 1),
-                     iBt<T>(F, 2), dinvt<T>(c), kx<T>(F), bfy<T>(c));
+                     iBt<T>(F, 2), dinvt<T>(c), kx<T>(F), bfy<T>(c), F.per);
   rowop<T, false>(c, 0, bfy<T>(c), nullptr, bfw<T>(c), 0);
   hipLaunchKernelGGL(k_bf_div<T>, dim3(nb), dim3(256), 0, c->st, F.X, F.Y, F.Z, F.flags, iAt<T>(F, 0),
-                     iAt<T>(F, 1), iAt<T>(F, 2), dinvt<T>(c), bfw<T>(c), kb<T>(F));
+                     iAt<T>(F, 1), iAt<T>(F, 2), dinvt<T>(c), bfw<T>(c), kb<T>(F), F.per);
   hipLaunchKernelGGL(k_fill<T>, dim3(nb), dim3(256), 0, c->st, F.n, T(0), kx<T>(F));
   kcycle<T>(c, 0, nu, sc);
   hipLaunchKernelGGL(k_bf_scatter<T>, dim3(nb), dim3(256), 0, c->st, F.n, F.flags, kx<T>(F), sign, z);
@@ -1229,7 +1239,7 @@ static int autotune_levels(stk_ctx *c, int reps) {
   const int ns = (int)(sizeof(shape) / sizeof(shape[0]));
   for (int l = 0; l <= c->L; l++) {
     Level &L = c->lv[l];
-    if (4 * (size_t)L.n * sizeof(T) <= kLevelLds) continue;  // smoothed by the whole-level kernel
+    if (!L.per && 4 * (size_t)L.n * sizeof(T) <= kLevelLds) continue;  // smoothed by the whole-level kernel
     // fixed shapes, plus tile heights that split the y extent of the equation-row box evenly
     std::vector<std::array<int, 4>> cand;
     auto add = [&](int ty, int tx, int tz) {
@@ -1296,9 +1306,13 @@ static int markers_run(stk_ctx *c, long nmark, const float *mx, const float *my,
   // every start[0..ncell] is written (no entry needs clearing first)
   hipLaunchKernelGGL(k_cell_start, dim3(blocks(nmark + 1, 256)), dim3(256), 0, c->st, nmark, b.skey, ncell + 1,
                      b.start);
-  hipLaunchKernelGGL(k_markers_gather, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, F.g64[0],
-                     F.g64[1], F.g64[2], b.start, b.so[0], b.so[1], b.so[2], b.so[3], b.so[4], c->kn.nubeg,
-                     c->kn.nuend, nukoef, c->kn.viscmod, c->nu, c->ro, c->nxx, c->nxy, c->nxz, c->nyz);
+  const MGather q{F.X, F.Y, F.Z, F.g64[0], F.g64[1], F.g64[2], b.start, b.so[0], b.so[1], b.so[2], b.so[3], b.so[4],
+                  c->kn.nubeg, c->kn.nuend, nukoef, c->kn.viscmod};
+  hipLaunchKernelGGL(k_markers_gather, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, q, F.per, c->nu, c->ro, c->nxx,
+                     c->nxy, c->nxz, c->nyz);
+  if (F.per)
+    hipLaunchKernelGGL(k_markers_wrap, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, F.per, c->nu,
+                       c->ro, c->nxy, c->nxz, c->nyz);
   HIPCHK(hipStreamSynchronize(c->st));
   HIPCHK(hipGetLastError());
   c->fields_set = true;
@@ -1347,7 +1361,6 @@ int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *
                const double *gz, int xper, int yper, int zper) {
   if (!out || levels < 0 || levels >= MAXLV || nx < 5 || ny < 5 || nz < 5 || !gx || !gy || !gz)
     return STK_EINVAL;
-  if (xper || yper || zper) return STK_EINVAL;  // periodic boxes: not supported by the device path
   {
     // grid lines strictly increasing (every spacing is a divisor)
     const double *gl[3] = {gx, gy, gz};
@@ -1358,6 +1371,8 @@ int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *
   }
   stk_ctx *c = new stk_ctx();
   c->L = levels;
+  c->per[0] = xper ? 1 : 0; c->per[1] = yper ? 1 : 0; c->per[2] = zper ? 1 : 0;
+  const int pmask = c->per[0] | (c->per[1] << 1) | (c->per[2] << 2);
   {
     int dev = 0, lds = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -1376,6 +1391,7 @@ int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *
       for (int a = 0; a < 3; a++) dims[a] = (dims[a] - 5) / 2 + 5;
     }
     L.X = dims[0]; L.Y = dims[1]; L.Z = dims[2]; L.XY = L.X * L.Y;
+    L.per = pmask;
     L.n = (long)L.XY * L.Z;
     L.off = off;
     off += L.n;
@@ -1411,11 +1427,22 @@ int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *
       std::vector<double> iA(nl, 0.0), iB(nl, 0.0);
       for (int i = 1; i < nl; i++) iA[i] = 1.0 / (L.g[a][i] - L.g[a][i - 1]);
       for (int i = 1; i + 1 < nl; i++) iB[i] = 1.0 / (L.g[a][i + 1] - L.g[a][i - 1]);
+      if (c->per[a]) {
+        // across the wrap: cell 0 is cell nl-1, and the spacing at line 0
+        // (line nl-1) spans lines nl-2 and 1 (move3mg.c:17-19 with the
+        // periodic cell, 400-403, 834), summed in the reference's order
+        const std::vector<double> &gg = L.g[a];
+        iA[0] = iA[nl - 1];
+        iB[0] = 1.0 / (gg[1] - gg[0] + gg[nl - 1] - gg[nl - 2]);
+        iB[nl - 1] = iB[0];
+      }
       std::vector<float> fA(iA.begin(), iA.end()), fB(iB.begin(), iB.end());
       // uniform along this axis: one iA over 1..nl-1 and one iB over 1..nl-2
+      // (and over the wrap entries of a periodic axis)
       bool u = nl >= 3;
       for (int i = 2; u && i < nl; i++) u = fA[i] == fA[1];
       for (int i = 2; u && i + 1 < nl; i++) u = fB[i] == fB[1];
+      if (u && c->per[a]) u = fA[0] == fA[1] && fB[0] == fB[1] && fB[nl - 1] == fB[1];
       L.ua[a] = u ? fA[1] : 0.f;
       L.ub[a] = u ? fB[1] : 0.f;
       if (upload(&L.iA64[a], iA) || upload(&L.iB64[a], iB) || upload(&L.iA32[a], fA) || upload(&L.iB32[a], fB) ||
@@ -1803,8 +1830,16 @@ static int build_schur(stk_ctx *c) {
     m3 = (int)(n / C.XY); m1 = (int)((n % C.XY) / C.Y); m2 = (int)(n % C.Y);
     A[(size_t)i * N + i] += h[0][n];
     dmean += h[0][n];
-    const long nbh[6] = {m1 > 0 ? n - C.Y : -1, m1 < C.X - 1 ? n + C.Y : -1, m2 > 0 ? n - 1 : -1,
-                         m2 < C.Y - 1 ? n + 1 : -1, m3 > 0 ? n - C.XY : -1, m3 < C.Z - 1 ? n + C.XY : -1};
+    long nbh[6] = {m1 > 0 ? n - C.Y : -1, m1 < C.X - 1 ? n + C.Y : -1, m2 > 0 ? n - 1 : -1,
+                   m2 < C.Y - 1 ? n + 1 : -1, m3 > 0 ? n - C.XY : -1, m3 < C.Z - 1 ? n + C.XY : -1};
+    // periodic axes: cells 1 and n-1 are neighbours across the wrap
+    const int mm[3] = {m1, m2, m3}, nn[3] = {C.X, C.Y, C.Z};
+    const long st[3] = {C.Y, 1, C.XY};
+    for (int a = 0; a < 3; a++) {
+      if (!((C.per >> a) & 1)) continue;
+      if (mm[a] == 1) nbh[2 * a] = n + (long)(nn[a] - 2) * st[a];
+      if (mm[a] == nn[a] - 1) nbh[2 * a + 1] = n - (long)(nn[a] - 2) * st[a];
+    }
     for (int q = 0; q < 6; q++)
       if (nbh[q] >= 0 && idx[nbh[q]] >= 0) A[(size_t)i * N + idx[nbh[q]]] += h[q + 1][n];
     I[(size_t)i * N + i] = 1.0;
@@ -1994,14 +2029,17 @@ int stk_prepare_cvm(stk_ctx *c, int cvm) {
                        c->nu + L.off, c->nxx, c->nxy, c->nxz, c->nyz, c->kn.nubeg, c->kn.nuend,
                        l == 0 ? c->kn.viscmod : vmc,
                        L.e64[0], L.e64[1], L.e64[2], L.e64[3], L.e64[4], L.e32[0], L.e32[1], L.e32[2], L.e32[3],
-                       L.e32[4]);
+                       L.e32[4], L.per);
+    if (L.per)
+      hipLaunchKernelGGL(k_visc_wrap, dim3(blocks(L.n, 256)), dim3(256), 0, c->st, L.X, L.Y, L.Z, L.per, L.e64[0],
+                         L.e64[1], L.e64[2], L.e64[3], L.e64[4], L.e32[0], L.e32[1], L.e32[2], L.e32[3], L.e32[4]);
     L.uni = L.ua[0] > 0 && L.ua[1] > 0 && L.ua[2] > 0;
     if (L.uni) L.fu = flux_u_consts(L.ua[0], L.ub[0], L.ua[1], L.ub[1], L.ua[2], L.ub[2], c->kn.p2koef);
     // nd = cell viscosity of the continuity stencil (continerr, move3mg.c:1549)
     hipLaunchKernelGGL((k_convert<float, float>), dim3(blocks(L.n, 256)), dim3(256), 0, c->st, L.n, L.e32[0],
                        c->nd + L.off);
     // rows of the FP32 whole-level smoother
-    if (4 * (size_t)L.n * sizeof(float) <= kLevelLds) {
+    if (!L.per && 4 * (size_t)L.n * sizeof(float) <= kLevelLds) {
       if (!L.lc32 && dalloc(&L.lc32, (size_t)54 * L.n)) return STK_ENOMEM;
       hipLaunchKernelGGL(k_level_coef, dim3(blocks(L.n, 256)), dim3(256), 0, c->st, L.geo<float>(c->kn.p2koef),
                          L.flags, L.lc32);
@@ -2019,7 +2057,7 @@ int stk_prepare_cvm(stk_ctx *c, int cvm) {
   if (rc) return rc;
   Level &F = c->lv[0];
   hipLaunchKernelGGL(k_rhs, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, F.flags, c->ro,
-                     c->kn.gx, c->kn.gy, c->kn.gz, c->B);
+                     c->kn.gx, c->kn.gy, c->kn.gz, c->B, F.per);
   if (F.nbc) hipLaunchKernelGGL(k_bc_rhs, dim3(blocks(F.nbc, 256)), dim3(256), 0, c->st, F.nbc, F.bc_r, F.bc_c, c->B);
   hipLaunchKernelGGL(k_weights, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, F.flags, F.e64[0],
                      F.g64[0], F.g64[1], F.g64[2], c->W2);
@@ -2067,14 +2105,12 @@ static int post_norms(stk_ctx *c, const double *x, stk_norms *out);
 int stk_norms_eval(stk_ctx *c, stk_norms *out) {
   if (!c || !out) return STK_EINVAL;
   if (!c->prepared || !c->X || !c->B) return STK_ESTATE;
-  if (c->per[0] || c->per[1] || c->per[2]) return STK_EINVAL;
   return post_norms(c, c->X, out);
 }
 
 int stk_stress_strain(stk_ctx *c, double *sig, double *eps) {
   if (!c) return STK_EINVAL;
   if (!c->prepared || !c->X) return STK_ESTATE;
-  if (c->per[0] || c->per[1] || c->per[2]) return STK_EINVAL;
   Level &F = c->lv[0];
   const size_t n6 = (size_t)6 * F.n;
   if (!c->sig && dalloc(&c->sig, n6)) return STK_ENOMEM;
@@ -2487,6 +2523,7 @@ static PostGeo post_geo(const stk_ctx *c) {
   g.ec = F.e64[0]; g.exy = F.e64[1]; g.exz = F.e64[2]; g.eyz = F.e64[3];
   g.nu = c->nu;
   g.nubeg = c->kn.nubeg; g.nuend = c->kn.nuend; g.vm = c->kn.viscmod;
+  g.per = F.per;
   return g;
 }
 
@@ -2642,7 +2679,6 @@ int stk_solve(stk_ctx *c, const stk_solver_opts *o, stk_stats *st) {
   S->relres = std::sqrt(s4[0] + s4[1] + s4[2]) / nb;
   S->blocks[0] = std::sqrt(s4[0]); S->blocks[1] = std::sqrt(s4[1]); S->blocks[2] = std::sqrt(s4[2]);
   if (!S->status) S->status = (S->relres <= o->tol * 1.0000001) ? STK_OK : STK_ENOCONV;
-  if (c->per[0] || c->per[1] || c->per[2]) return S->status;
   rc = post_norms(c, c->X, &S->norms);
   if (rc) return rc;
   return S->status;
@@ -2701,7 +2737,7 @@ int stk_smooth_ex(stk_ctx *c, int l, int sweeps, int ty, int tx, int tz, int pre
   const bool staged = tz < 0;
   if (staged) tz = -tz;
   const size_t tsize = precision ? sizeof(double) : sizeof(float);
-  if (whole && 4 * (size_t)c->lv[l].n * tsize > kLevelLds) return STK_EINVAL;
+  if (whole && (4 * (size_t)c->lv[l].n * tsize > kLevelLds || c->lv[l].per)) return STK_EINVAL;
   if (!whole && (ty < 1 || tx < 1 || tz < 1 || (size_t)(ty + 2) * (tx + 2) * (tz + 2) * 32 > 65536)) return STK_EINVAL;
   int rc = ensure_ref(c, l) || ensure_vectors(c, !precision, true);
   if (rc) return STK_ENOMEM;
@@ -3233,11 +3269,24 @@ static int vpiterate_outputs(stk_ctx *c) {
         if (g_arr.vy && m1 < F.X - 1 && m3 < F.Z - 1) g_arr.vy[n] = (float)x[4 * n + 2];
         if (g_arr.vz && m1 < F.X - 1 && m2 < F.Y - 1) g_arr.vz[n] = (float)x[4 * n + 3];
       }
+  // pressure of line 0 of a periodic axis from the last line (move3mg.c:3321-3379, x, y, z in turn)
+  if (g_arr.pr) {
+    double *pr = g_arr.pr;
+    if (c->per[0])
+      for (int m3 = 0; m3 < F.Z; m3++)
+        for (int m2 = 0; m2 < F.Y; m2++) pr[(long)m3 * F.XY + m2] = pr[(long)m3 * F.XY + (long)(F.X - 1) * F.Y + m2];
+    if (c->per[1])
+      for (int m3 = 0; m3 < F.Z; m3++)
+        for (int m1 = 0; m1 < F.X; m1++) pr[(long)m3 * F.XY + (long)m1 * F.Y] = pr[(long)m3 * F.XY + (long)m1 * F.Y + F.Y - 1];
+    if (c->per[2])
+      for (int m1 = 0; m1 < F.X; m1++)
+        for (int m2 = 0; m2 < F.Y; m2++) pr[(long)m1 * F.Y + m2] = pr[(long)(F.Z - 1) * F.XY + (long)m1 * F.Y + m2];
+  }
   float *sp[6] = {g_arr.sxx, g_arr.syy, g_arr.szz, g_arr.sxy, g_arr.sxz, g_arr.syz};
   float *ep[6] = {g_arr.exx, g_arr.eyy, g_arr.ezz, g_arr.exy, g_arr.exz, g_arr.eyz};
   bool any = false;
   for (int k = 0; k < 6; k++) any = any || sp[k] || ep[k];
-  if (!any || c->per[0] || c->per[1] || c->per[2]) return STK_OK;
+  if (!any) return STK_OK;
   std::vector<double> sig(6 * (size_t)F.n), eps(6 * (size_t)F.n);
   int rc = stk_stress_strain(c, sig.data(), eps.data());
   if (rc) return rc;

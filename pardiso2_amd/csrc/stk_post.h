@@ -8,7 +8,10 @@
 // syzcalc operation for operation; the momentum errors take the stresses of
 // their neighbour cells and edges as the reference's stored arrays hold them
 // (zero where the recompute writes nothing), so the norms need no stored
-// stress planes.  Non-periodic boxes only.
+// stress planes.  Periodic axes follow the reference's branches: the
+// stencils wrap (move3mg.c:27-29, 400-429, 834-847, 1481-1483) and a shear
+// stress on the last line of a periodic axis is the one on line 0
+// (move3mg.c:2933-3044).
 #pragma once
 #include <hip/hip_runtime.h>
 #include "stk_kernels.h"
@@ -25,7 +28,8 @@ struct PostGeo {
   const double *ec, *exy, *exz, *eyz;               // clamped nxx, nxy, nxz, nyz of level 0
   const float *nu;                                   // node viscosity of level 0
   double nubeg, nuend;
-  int vm;  // viscmod
+  int vm;   // viscmod
+  int per;  // periodic axes (bit 0 x, bit 1 y, bit 2 z)
   __device__ __forceinline__ long nid(int m1, int m2, int m3) const { return (long)m3 * XY + (long)m1 * Y + m2; }
 };
 
@@ -43,29 +47,47 @@ __device__ __forceinline__ double post_sn(const PostGeo &g, const double *__rest
   const long n = g.nid(a1, a2, a3);
   const long v0 = n - g.XY - g.Y - 1;
   const double xkf = g.iAx[a1], ykf = g.iAy[a2], zkf = g.iAz[a3];
-  const double dvxdx = (x[4 * (v0 + g.Y) + 1] - x[4 * v0 + 1]) * xkf;
-  const double dvydy = (x[4 * (v0 + 1) + 2] - x[4 * v0 + 2]) * ykf;
-  const double dvzdz = (x[4 * (v0 + g.XY) + 3] - x[4 * v0 + 3]) * zkf;
+  // upper faces; on the last cell of a periodic axis the one on line 0
+  const long vxu = ((g.per & 1) && a1 == g.X - 1) ? g.nid(0, a2 - 1, a3 - 1) : v0 + g.Y;
+  const long vyu = ((g.per & 2) && a2 == g.Y - 1) ? g.nid(a1 - 1, 0, a3 - 1) : v0 + 1;
+  const long vzu = ((g.per & 4) && a3 == g.Z - 1) ? g.nid(a1 - 1, a2 - 1, 0) : v0 + g.XY;
+  const double dvxdx = (x[4 * vxu + 1] - x[4 * v0 + 1]) * xkf;
+  const double dvydy = (x[4 * vyu + 2] - x[4 * v0 + 2]) * ykf;
+  const double dvzdz = (x[4 * vzu + 3] - x[4 * v0 + 3]) * zkf;
   const double d = AX == 0 ? dvxdx : (AX == 1 ? dvydy : dvzdz);
   const double e = d - (dvxdx + dvydy + dvzdz) / 3.0;
   if (eo) *eo = e;
   return 2.0 * g.ec[n] * e;
 }
 
+// Where the recompute stores a shear stress of the plane with in-plane axes
+// (ia, ib) and normal axis kc (move3mg.c:2933-3044): below the last line of
+// kc; in-plane indices 1 .. n-2 on walled axes, and on a periodic axis any
+// line, the last one holding the value of line 0 (so the position is moved
+// there).  False: the plane holds zero at this node.
+__device__ __forceinline__ bool post_shear_at(int &ia, int &ib, int kc, int na, int nb, int nc, bool pa, bool pb) {
+  if (kc >= nc - 1) return false;
+  if (!pa && (ia < 1 || ia > na - 2)) return false;
+  if (!pb && (ib < 1 || ib > nb - 2)) return false;
+  if (pa && ia == na - 1) ia = 0;
+  if (pb && ib == nb - 1) ib = 0;
+  return true;
+}
+
 // Shear stresses at node (a1, a2, a3): sxycalc (move3mg.c:400-479),
-// sxzcalc (532-615), syzcalc (671-754), value mode.  Ranges of the
-// recompute (move3mg.c:2979-2983, 3017-3021, 3040-3044): both in-plane
-// indices in 1 .. n-2, the third below its last line.
+// sxzcalc (532-615), syzcalc (671-754), value mode, as the recompute stores
+// them (post_shear_at); the lower neighbours wrap on periodic axes.
 __device__ __forceinline__ double post_sxy(const PostGeo &g, const double *__restrict__ x, int a1, int a2, int a3,
                                            double *eo = nullptr) {
-  if (a1 < 1 || a1 > g.X - 2 || a2 < 1 || a2 > g.Y - 2 || a3 > g.Z - 2) {
+  if (!post_shear_at(a1, a2, a3, g.X, g.Y, g.Z, g.per & 1, g.per & 2)) {
     if (eo) *eo = 0.0;
     return 0.0;
   }
   const long n = g.nid(a1, a2, a3);
+  const int l1 = (g.per & 1) ? vwrap(a1 - 1, g.X) : a1 - 1, l2 = (g.per & 2) ? vwrap(a2 - 1, g.Y) : a2 - 1;
   const double xkf = g.iBx[a1], ykf = g.iBy[a2];
-  const double dvxdy = (x[4 * n + 1] - x[4 * (n - 1) + 1]) * ykf;
-  const double dvydx = (x[4 * n + 2] - x[4 * (n - g.Y) + 2]) * xkf;
+  const double dvxdy = (x[4 * n + 1] - x[4 * g.nid(a1, l2, a3) + 1]) * ykf;
+  const double dvydx = (x[4 * n + 2] - x[4 * g.nid(l1, a2, a3) + 2]) * xkf;
   const double e = dvxdy + dvydx;
   if (eo) *eo = e;
   return 2.0 * g.exy[n] * e;
@@ -73,14 +95,15 @@ __device__ __forceinline__ double post_sxy(const PostGeo &g, const double *__res
 
 __device__ __forceinline__ double post_sxz(const PostGeo &g, const double *__restrict__ x, int a1, int a2, int a3,
                                            double *eo = nullptr) {
-  if (a1 < 1 || a1 > g.X - 2 || a3 < 1 || a3 > g.Z - 2 || a2 > g.Y - 2) {
+  if (!post_shear_at(a1, a3, a2, g.X, g.Z, g.Y, g.per & 1, g.per & 4)) {
     if (eo) *eo = 0.0;
     return 0.0;
   }
   const long n = g.nid(a1, a2, a3);
+  const int l1 = (g.per & 1) ? vwrap(a1 - 1, g.X) : a1 - 1, l3 = (g.per & 4) ? vwrap(a3 - 1, g.Z) : a3 - 1;
   const double xkf = g.iBx[a1], zkf = g.iBz[a3];
-  const double dvxdz = (x[4 * n + 1] - x[4 * (n - g.XY) + 1]) * zkf;
-  const double dvzdx = (x[4 * n + 3] - x[4 * (n - g.Y) + 3]) * xkf;
+  const double dvxdz = (x[4 * n + 1] - x[4 * g.nid(a1, a2, l3) + 1]) * zkf;
+  const double dvzdx = (x[4 * n + 3] - x[4 * g.nid(l1, a2, a3) + 3]) * xkf;
   const double e = dvxdz + dvzdx;
   if (eo) *eo = e;
   return 2.0 * g.exz[n] * e;
@@ -88,14 +111,15 @@ __device__ __forceinline__ double post_sxz(const PostGeo &g, const double *__res
 
 __device__ __forceinline__ double post_syz(const PostGeo &g, const double *__restrict__ x, int a1, int a2, int a3,
                                            double *eo = nullptr) {
-  if (a2 < 1 || a2 > g.Y - 2 || a3 < 1 || a3 > g.Z - 2 || a1 > g.X - 2) {
+  if (!post_shear_at(a2, a3, a1, g.Y, g.Z, g.X, g.per & 2, g.per & 4)) {
     if (eo) *eo = 0.0;
     return 0.0;
   }
   const long n = g.nid(a1, a2, a3);
+  const int l2 = (g.per & 2) ? vwrap(a2 - 1, g.Y) : a2 - 1, l3 = (g.per & 4) ? vwrap(a3 - 1, g.Z) : a3 - 1;
   const double ykf = g.iBy[a2], zkf = g.iBz[a3];
-  const double dvydz = (x[4 * n + 2] - x[4 * (n - g.XY) + 2]) * zkf;
-  const double dvzdy = (x[4 * n + 3] - x[4 * (n - 1) + 3]) * ykf;
+  const double dvydz = (x[4 * n + 2] - x[4 * g.nid(a1, a2, l3) + 2]) * zkf;
+  const double dvzdy = (x[4 * n + 3] - x[4 * g.nid(a1, l2, a3) + 3]) * ykf;
   const double e = dvydz + dvzdy;
   if (eo) *eo = e;
   return 2.0 * g.eyz[n] * e;
@@ -136,38 +160,50 @@ template <int C>
 __device__ __forceinline__ double post_err(const PostGeo &g, const double *__restrict__ x,
                                            const double *__restrict__ b, int m1, int m2, int m3, double *eps) {
   const long n = g.nid(m1, m2, m3), Y = g.Y, XY = g.XY;
-  const long v0 = n, v1 = n + 1, v2 = n + Y, v3 = n + Y + 1, v4 = n + XY, v5 = n + XY + 1, v6 = n + XY + Y,
-             v7 = n + XY + Y + 1;
+  long v0 = n, v1 = n + 1, v2 = n + Y, v3 = n + Y + 1, v4 = n + XY, v5 = n + XY + 1, v6 = n + XY + Y,
+       v7 = n + XY + Y + 1;
+  // a row on line 0 of its periodic axis: the lower cell is the last one
+  // (x: v5, y: v6, z: v3 moved, move3mg.c:842, 1056, 1276), which also
+  // enters the viscosity lines as the reference has it
+  const bool wrap = ((g.per >> (C - 1)) & 1) && (C == 1 ? m1 : (C == 2 ? m2 : m3)) == 0;
   double xkf, ykf, zkf, left, ne;
+  // the reference reads the viscosity lines by plain offsets, clamped at the array start
+  auto lo0 = [](long v) { return v < 0 ? 0L : v; };
   if (C == 1) {
+    if (wrap) v5 = g.nid(g.X - 1, m2 + 1, m3 + 1);
     xkf = 2.0 * g.iBx[m1];
     ykf = g.iAy[m2 + 1];
     zkf = g.iAz[m3 + 1];
-    left = (post_sn<0>(g, x, m1 + 1, m2 + 1, m3 + 1) - post_sn<0>(g, x, m1, m2 + 1, m3 + 1)) * xkf;
+    left = (post_sn<0>(g, x, m1 + 1, m2 + 1, m3 + 1) - post_sn<0>(g, x, wrap ? g.X - 1 : m1, m2 + 1, m3 + 1)) * xkf;
     left += (post_sxy(g, x, m1, m2 + 1, m3) - post_sxy(g, x, m1, m2, m3)) * ykf;
     left += (post_sxz(g, x, m1, m2, m3 + 1) - post_sxz(g, x, m1, m2, m3)) * zkf;
     left -= (x[4 * v7] - x[4 * v5]) * xkf;
-    const long lo[4] = {v0 - Y, v1 - Y, v4 - Y, v5 - Y}, mid[4] = {v0, v1, v4, v5}, hi[4] = {v2, v3, v6, v7};
+    const long lo[4] = {lo0(v0 - Y), lo0(v1 - Y), lo0(v4 - Y), lo0(v5 - Y)}, mid[4] = {v0, v1, v4, v5},
+               hi[4] = {v2, v3, v6, v7};
     ne = post_nu48(g, lo, mid, hi);
   } else if (C == 2) {
+    if (wrap) v6 = g.nid(m1 + 1, g.Y - 1, m3 + 1);
     xkf = g.iAx[m1 + 1];
     ykf = 2.0 * g.iBy[m2];
     zkf = g.iAz[m3 + 1];
-    left = (post_sn<1>(g, x, m1 + 1, m2 + 1, m3 + 1) - post_sn<1>(g, x, m1 + 1, m2, m3 + 1)) * ykf;
+    left = (post_sn<1>(g, x, m1 + 1, m2 + 1, m3 + 1) - post_sn<1>(g, x, m1 + 1, wrap ? g.Y - 1 : m2, m3 + 1)) * ykf;
     left += (post_sxy(g, x, m1 + 1, m2, m3) - post_sxy(g, x, m1, m2, m3)) * xkf;
     left += (post_syz(g, x, m1, m2, m3 + 1) - post_syz(g, x, m1, m2, m3)) * zkf;
     left -= (x[4 * v7] - x[4 * v6]) * ykf;
-    const long lo[4] = {v0 - 1, v2 - 1, v4 - 1, v6 - 1}, mid[4] = {v0, v2, v4, v6}, hi[4] = {v1, v3, v5, v7};
+    const long lo[4] = {lo0(v0 - 1), lo0(v2 - 1), lo0(v4 - 1), lo0(v6 - 1)}, mid[4] = {v0, v2, v4, v6},
+               hi[4] = {v1, v3, v5, v7};
     ne = post_nu48(g, lo, mid, hi);
   } else {
+    if (wrap) v3 = g.nid(m1 + 1, m2 + 1, g.Z - 1);
     xkf = g.iAx[m1 + 1];
     ykf = g.iAy[m2 + 1];
     zkf = 2.0 * g.iBz[m3];
-    left = (post_sn<2>(g, x, m1 + 1, m2 + 1, m3 + 1) - post_sn<2>(g, x, m1 + 1, m2 + 1, m3)) * zkf;
+    left = (post_sn<2>(g, x, m1 + 1, m2 + 1, m3 + 1) - post_sn<2>(g, x, m1 + 1, m2 + 1, wrap ? g.Z - 1 : m3)) * zkf;
     left += (post_sxz(g, x, m1 + 1, m2, m3) - post_sxz(g, x, m1, m2, m3)) * xkf;
     left += (post_syz(g, x, m1, m2 + 1, m3) - post_syz(g, x, m1, m2, m3)) * ykf;
     left -= (x[4 * v7] - x[4 * v3]) * zkf;
-    const long lo[4] = {v0 - XY, v1 - XY, v2 - XY, v3 - XY}, mid[4] = {v0, v1, v2, v3}, hi[4] = {v4, v5, v6, v7};
+    const long lo[4] = {lo0(v0 - XY), lo0(v1 - XY), lo0(v2 - XY), lo0(v3 - XY)}, mid[4] = {v0, v1, v2, v3},
+               hi[4] = {v4, v5, v6, v7};
     ne = post_nu48(g, lo, mid, hi);
   }
   left = left - b[4 * n + C];
@@ -177,7 +213,9 @@ __device__ __forceinline__ double post_err(const PostGeo &g, const double *__res
 }
 
 // Stress and strain-rate planes (xx yy zz xy xz yz, n each) of the
-// recompute; zero outside its ranges.
+// recompute; zero outside its ranges.  On line 0 of a periodic axis the
+// normal stresses and strain rates are those of the last cell, as the
+// reference copies them after the solve (move3mg.c:3321-3379).
 __global__ void __launch_bounds__(256) k_post_stress(PostGeo g, const double *__restrict__ x, double *sig,
                                                      double *eps) {
   const long N = (long)g.XY * g.Z;
@@ -186,9 +224,11 @@ __global__ void __launch_bounds__(256) k_post_stress(PostGeo g, const double *__
   int m1, m2, m3;
   decode(n, g.X, g.Y, g.XY, m1, m2, m3);
   double e;
-  sig[0 * N + n] = post_sn<0>(g, x, m1, m2, m3, &e); eps[0 * N + n] = e;
-  sig[1 * N + n] = post_sn<1>(g, x, m1, m2, m3, &e); eps[1 * N + n] = e;
-  sig[2 * N + n] = post_sn<2>(g, x, m1, m2, m3, &e); eps[2 * N + n] = e;
+  const int c1 = (g.per & 1) ? pwrap(m1, g.X) : m1, c2 = (g.per & 2) ? pwrap(m2, g.Y) : m2;
+  const int c3 = (g.per & 4) ? pwrap(m3, g.Z) : m3;
+  sig[0 * N + n] = post_sn<0>(g, x, c1, c2, c3, &e); eps[0 * N + n] = e;
+  sig[1 * N + n] = post_sn<1>(g, x, c1, c2, c3, &e); eps[1 * N + n] = e;
+  sig[2 * N + n] = post_sn<2>(g, x, c1, c2, c3, &e); eps[2 * N + n] = e;
   sig[3 * N + n] = post_sxy(g, x, m1, m2, m3, &e); eps[3 * N + n] = e;
   sig[4 * N + n] = post_sxz(g, x, m1, m2, m3, &e); eps[4 * N + n] = e;
   sig[5 * N + n] = post_syz(g, x, m1, m2, m3, &e); eps[5 * N + n] = e;
@@ -227,8 +267,11 @@ __global__ void __launch_bounds__(256) k_post_norms(PostGeo g, const unsigned ch
       // continerr, ynerr = 1 (move3mg.c:1500-1507)
       const double xkf = g.iAx[m1], ykf = g.iAy[m2], zkf = g.iAz[m3];
       const long v0 = n - g.XY - Y - 1;
-      const double leftc = (x[4 * (v0 + Y) + 1] - x[4 * v0 + 1]) * xkf + (x[4 * (v0 + 1) + 2] - x[4 * v0 + 2]) * ykf +
-                           (x[4 * (v0 + g.XY) + 3] - x[4 * v0 + 3]) * zkf;
+      const long vxu = ((g.per & 1) && m1 == X - 1) ? g.nid(0, m2 - 1, m3 - 1) : v0 + Y;
+      const long vyu = ((g.per & 2) && m2 == Y - 1) ? g.nid(m1 - 1, 0, m3 - 1) : v0 + 1;
+      const long vzu = ((g.per & 4) && m3 == Z - 1) ? g.nid(m1 - 1, m2 - 1, 0) : v0 + g.XY;
+      const double leftc = (x[4 * vxu + 1] - x[4 * v0 + 1]) * xkf + (x[4 * vyu + 2] - x[4 * v0 + 2]) * ykf +
+                           (x[4 * vzu + 3] - x[4 * v0 + 3]) * zkf;
       const double skf = (1.0 / xkf + 1.0 / ykf + 1.0 / zkf) / 3.0;
       const double cv = (leftc - 0.0) * skf;
       if (kind_of(f, 0) != KIND_BC) { a[4] += cv * cv; a[5] += 1.0; }

@@ -57,10 +57,36 @@ struct Geo {
   const T *iAx, *iBx, *iAy, *iBy, *iAz, *iBz;
   const T *ec, *exy, *exz, *eyz, *ed;  // stencil viscosities (clamped)
   T p2;                                // grad-div weight (p2koef)
+  int per;                             // periodic axes: bit 0 x, bit 1 y, bit 2 z
   __device__ __forceinline__ long nid(int m1, int m2, int m3) const {
     return (long)m3 * XY + (long)m1 * Y + m2;
   }
 };
+
+// Periodic axes (the reference's xperiodic/yperiodic/zperiodic branches,
+// parallel/move3mg.c:27-34, 400-429, 834-847, 1481-1483) amount to one index
+// map per axis of n lines: grid line n-1 is line 0 again (its normal
+// velocity carries a copy equation), so velocities live on lines 0..n-2 and
+// pressure cells on 1..n-1, and a stencil index i outside those ranges names
+// the unknown at
+//   velocity:  vwrap(i) = i mod (n-1)        (-1 -> n-2, n-1 -> 0, n -> 1)
+//   pressure:  pwrap(i) = (i-1) mod (n-1) + 1 (0 -> n-1, n -> 1)
+// for i in [-1, n].  Stress points follow the same maps: a cell (indexed by
+// its upper corner, like P) by pwrap, an edge on a grid line by vwrap.
+__host__ __device__ __forceinline__ int vwrap(int i, int n) {
+  return i < 0 ? i + (n - 1) : (i >= n - 1 ? i - (n - 1) : i);
+}
+__host__ __device__ __forceinline__ int pwrap(int i, int n) {
+  return i < 1 ? i + (n - 1) : (i > n - 1 ? i - (n - 1) : i);
+}
+// node that holds unknown c read at stencil position (m1, m2, m3)
+template <typename T>
+__device__ __forceinline__ long unode(const Geo<T> &g, int m1, int m2, int m3, int c) {
+  if (g.per & 1) m1 = c ? vwrap(m1, g.X) : pwrap(m1, g.X);
+  if (g.per & 2) m2 = c ? vwrap(m2, g.Y) : pwrap(m2, g.Y);
+  if (g.per & 4) m3 = c ? vwrap(m3, g.Z) : pwrap(m3, g.Z);
+  return g.nid(m1, m2, m3);
+}
 
 // Normal-stress contributions of a cell (sxxcalc/syycalc/szzcalc, coefficient
 // mode): a[0..5] = vx v0, vx v2, vy v0, vy v1, vz v0, vz v4 with weight w.

@@ -16,7 +16,30 @@
 namespace stk {
 
 template <typename T> struct KPlanes { T *p[7]; };  // diag, -x, +x, -y, +y, -z, +z
-template <typename T> struct KPlanesC { const T *p[7]; };
+template <typename T> struct KPlanesC {
+  const T *p[7];
+  int per;  // periodic axes of the level (bit 0 x, bit 1 y, bit 2 z)
+};
+
+// Neighbour offset of a cell across its lower (LO) or upper face along an
+// axis of nn lines (cells 1 .. nn-1) with stride st; false: no neighbour.
+// On a periodic axis cells 1 and nn-1 are neighbours across the wrap (the
+// face velocity on line 0), and a Gauss-Seidel half-sweep (GS) takes that
+// coupling only when the two cells have different colours (nn odd), so no
+// cell reads a value its own half-sweep writes.
+template <bool LO, bool GS>
+__device__ __forceinline__ bool knb(int m, int nn, bool per, long st, long &off) {
+  if (per && m == (LO ? 1 : nn - 1)) {
+    if (GS && !(nn & 1)) return false;
+    off = LO ? (long)(nn - 2) * st : -(long)(nn - 2) * st;
+    return true;
+  }
+  if (LO ? m > 0 : m < nn - 1) {
+    off = LO ? -st : st;
+    return true;
+  }
+  return false;
+}
 
 // 1 / |diagonal| of the level's momentum equations; 0 on other unknowns
 __global__ void k_bf_dinv(Geo<double> g, const unsigned char *__restrict__ flags, double *d64, float *d32) {
@@ -53,9 +76,13 @@ __global__ void k_bf_kop(Geo<double> g, const unsigned char *__restrict__ flags,
     const int mm[3] = {m1, m2, m3};
     const double *iA[3] = {g.iAx, g.iAy, g.iAz};
     const double *iB[3] = {g.iBx, g.iBy, g.iBz};
+    const int nn[3] = {g.X, g.Y, g.Z};
     for (int a = 0; a < 3; a++) {
       const int m = mm[a];
-      const double ap = iA[a][m] * dinv[4 * (base + step[a]) + 1 + a] * 2.0 * iB[a][m];
+      // on a periodic axis the +face of the last cell is the velocity on line 0
+      // (the coupling across the wrap is left out of K: its diagonal keeps both faces)
+      const long up = ((g.per >> a) & 1) && m == nn[a] - 1 ? base - (long)(nn[a] - 2) * step[a] : base + step[a];
+      const double ap = iA[a][m] * dinv[4 * up + 1 + a] * 2.0 * iB[a][m];
       const double am = iA[a][m] * dinv[4 * base + 1 + a] * 2.0 * iB[a][m - 1];
       kc[0] += ap + am;
       kc[1 + 2 * a] = -am;
@@ -72,7 +99,8 @@ __global__ void k_bf_kop(Geo<double> g, const unsigned char *__restrict__ flags,
 // non-equation unknowns 0); p is a cell vector (one value per node)
 template <typename T>
 __global__ void k_bf_grad(int X, int Y, int Z, const T *__restrict__ iBx, const T *__restrict__ iBy,
-                          const T *__restrict__ iBz, const T *__restrict__ dinv, const T *__restrict__ p, T *y) {
+                          const T *__restrict__ iBz, const T *__restrict__ dinv, const T *__restrict__ p, T *y,
+                          int per) {
   long n = xcd_block() * blockDim.x + threadIdx.x;
   const int XY = X * Y;
   if (n >= (long)XY * Z) return;
@@ -82,11 +110,14 @@ __global__ void k_bf_grad(int X, int Y, int Z, const T *__restrict__ iBx, const 
   const long up = n + XY + Y + 1;  // cell (m1+1, m2+1, m3+1)
   const long stp[3] = {(long)Y, 1L, (long)XY};
   const int mm[3] = {m1, m2, m3};
+  const int nn[3] = {X, Y, Z};
   const T *iB[3] = {iBx, iBy, iBz};
   for (int a = 0; a < 3; a++) {
     const T d = dinv[4 * n + 1 + a];
+    // the lower cell of a velocity on line 0 of a periodic axis is cell n-1
+    const long lo = ((per >> a) & 1) && mm[a] == 0 ? up + (long)(nn[a] - 2) * stp[a] : up - stp[a];
     T v = T(0);
-    if (d != T(0)) v = d * (T(2) * iB[a][mm[a]]) * (p[up - stp[a]] - p[up]);
+    if (d != T(0)) v = d * (T(2) * iB[a][mm[a]]) * (p[lo] - p[up]);
     y[4 * n + 1 + a] = v;
   }
 }
@@ -95,7 +126,7 @@ __global__ void k_bf_grad(int X, int Y, int Z, const T *__restrict__ iBx, const 
 template <typename T>
 __global__ void k_bf_div(int X, int Y, int Z, const unsigned char *__restrict__ flags, const T *__restrict__ iAx,
                          const T *__restrict__ iAy, const T *__restrict__ iAz, const T *__restrict__ dinv,
-                         const T *__restrict__ w, T *t) {
+                         const T *__restrict__ w, T *t, int per) {
   long n = xcd_block() * blockDim.x + threadIdx.x;
   const int XY = X * Y;
   if (n >= (long)XY * Z) return;
@@ -105,10 +136,13 @@ __global__ void k_bf_div(int X, int Y, int Z, const unsigned char *__restrict__ 
   const long base = n - XY - Y - 1;
   const long stp[3] = {(long)Y, 1L, (long)XY};
   const int mm[3] = {m1, m2, m3};
+  const int nn[3] = {X, Y, Z};
   const T *iA[3] = {iAx, iAy, iAz};
   T acc = T(0);
   for (int a = 0; a < 3; a++) {
-    const long up = 4 * (base + stp[a]) + 1 + a, lo = 4 * base + 1 + a;
+    // the +face of the last cell of a periodic axis is the velocity on line 0
+    const long un = ((per >> a) & 1) && mm[a] == nn[a] - 1 ? base - (long)(nn[a] - 2) * stp[a] : base + stp[a];
+    const long up = 4 * un + 1 + a, lo = 4 * base + 1 + a;
     acc += iA[a][mm[a]] * (dinv[up] * w[up] - dinv[lo] * w[lo]);
   }
   t[n] = acc;
@@ -179,12 +213,13 @@ __device__ __forceinline__ bool kgs_at(int X, int Y, int Z, const KPlanesC<T> &k
   const T d = k.p[0][n];
   if (d == T(0)) return false;
   T acc = b[n];
-  if (m1 > 0) acc -= k.p[1][n] * x[n - Y];
-  if (m1 < X - 1) acc -= k.p[2][n] * x[n + Y];
-  if (m2 > 0) acc -= k.p[3][n] * x[n - 1];
-  if (m2 < Y - 1) acc -= k.p[4][n] * x[n + 1];
-  if (m3 > 0) acc -= k.p[5][n] * x[n - XY];
-  if (m3 < Z - 1) acc -= k.p[6][n] * x[n + XY];
+  long o;
+  if (knb<true, true>(m1, X, k.per & 1, Y, o)) acc -= k.p[1][n] * x[n + o];
+  if (knb<false, true>(m1, X, k.per & 1, Y, o)) acc -= k.p[2][n] * x[n + o];
+  if (knb<true, true>(m2, Y, k.per & 2, 1, o)) acc -= k.p[3][n] * x[n + o];
+  if (knb<false, true>(m2, Y, k.per & 2, 1, o)) acc -= k.p[4][n] * x[n + o];
+  if (knb<true, true>(m3, Z, k.per & 4, XY, o)) acc -= k.p[5][n] * x[n + o];
+  if (knb<false, true>(m3, Z, k.per & 4, XY, o)) acc -= k.p[6][n] * x[n + o];
   out = acc / d;
   return true;
 }
@@ -230,7 +265,7 @@ __device__ __forceinline__ void kcell_of(unsigned i, int colour, int X, int Y, F
 
 template <typename T>
 __global__ void k_kgs_c(int X, int Y, int Z, FDiv dhy, FDiv dx, const T *__restrict__ kc, const T *__restrict__ b,
-                        T *x, int colour) {
+                        T *x, int colour, int per) {
   const unsigned i = (unsigned)(xcd_block() * blockDim.x + threadIdx.x);
   const unsigned ncol = (unsigned)((Y + 1) / 2) * X * Z;
   if (i >= ncol) return;
@@ -243,12 +278,13 @@ __global__ void k_kgs_c(int X, int Y, int Z, FDiv dhy, FDiv dx, const T *__restr
   const T d = p[0];
   if (d == T(0)) return;
   T acc = b[n];
-  if (m1 > 0) acc -= p[2 * ncol] * x[n - Y];
-  if (m1 < X - 1) acc -= p[4 * (size_t)ncol] * x[n + Y];
-  if (m2 > 0) acc -= p[6 * (size_t)ncol] * x[n - 1];
-  if (m2 < Y - 1) acc -= p[8 * (size_t)ncol] * x[n + 1];
-  if (m3 > 0) acc -= p[10 * (size_t)ncol] * x[n - XY];
-  if (m3 < Z - 1) acc -= p[12 * (size_t)ncol] * x[n + XY];
+  long o;
+  if (knb<true, true>(m1, X, per & 1, Y, o)) acc -= p[2 * ncol] * x[n + o];
+  if (knb<false, true>(m1, X, per & 1, Y, o)) acc -= p[4 * (size_t)ncol] * x[n + o];
+  if (knb<true, true>(m2, Y, per & 2, 1, o)) acc -= p[6 * (size_t)ncol] * x[n + o];
+  if (knb<false, true>(m2, Y, per & 2, 1, o)) acc -= p[8 * (size_t)ncol] * x[n + o];
+  if (knb<true, true>(m3, Z, per & 4, XY, o)) acc -= p[10 * (size_t)ncol] * x[n + o];
+  if (knb<false, true>(m3, Z, per & 4, XY, o)) acc -= p[12 * (size_t)ncol] * x[n + o];
   x[n] = acc / d;
 }
 
@@ -277,12 +313,13 @@ __device__ __forceinline__ T kres_val(int X, int Y, int Z, const KPlanesC<T> &k,
   int m1, m2, m3;
   decode(n, X, Y, XY, m1, m2, m3);
   T acc = b[n] - d * x[n];
-  if (m1 > 0) acc -= k.p[1][n] * x[n - Y];
-  if (m1 < X - 1) acc -= k.p[2][n] * x[n + Y];
-  if (m2 > 0) acc -= k.p[3][n] * x[n - 1];
-  if (m2 < Y - 1) acc -= k.p[4][n] * x[n + 1];
-  if (m3 > 0) acc -= k.p[5][n] * x[n - XY];
-  if (m3 < Z - 1) acc -= k.p[6][n] * x[n + XY];
+  long o;
+  if (knb<true, false>(m1, X, k.per & 1, Y, o)) acc -= k.p[1][n] * x[n + o];
+  if (knb<false, false>(m1, X, k.per & 1, Y, o)) acc -= k.p[2][n] * x[n + o];
+  if (knb<true, false>(m2, Y, k.per & 2, 1, o)) acc -= k.p[3][n] * x[n + o];
+  if (knb<false, false>(m2, Y, k.per & 2, 1, o)) acc -= k.p[4][n] * x[n + o];
+  if (knb<true, false>(m3, Z, k.per & 4, XY, o)) acc -= k.p[5][n] * x[n + o];
+  if (knb<false, false>(m3, Z, k.per & 4, XY, o)) acc -= k.p[6][n] * x[n + o];
   return acc;
 }
 

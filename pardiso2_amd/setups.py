@@ -230,6 +230,7 @@ class Problem:
     gravity: Tuple[float, float, float] = (0.0, 0.0, -9.81)
     knobs: dict = dataclasses.field(default_factory=dict)
     name: str = ""
+    periodic: Tuple[int, int, int] = (0, 0, 0)
 
     @property
     def nodes(self) -> int:
@@ -340,3 +341,48 @@ def served_request(n: int, seed: int, decades: float, knobs=None) -> Problem:
     le = np.clip(21.0 + 0.25 * decades * s, 21.0 - 0.5 * decades, 21.0 + 0.5 * decades)
     mro = np.where(s > 0, 3200.0, 3300.0)
     return _box(n, L, 10.0 ** le, mro, mx, my, mz, f"served{n}", knobs)
+
+
+# Boundary block of parallel/init_inclusion.t3c:43-75 (the reference's simple
+# shear setup, periodic in x and y): pressure fixed in cell (2,2,2); on the z
+# walls (outer lines) vz = 0 and the first/last vx cell centres held at +1/-1,
+# vy at 0; the last x line of vx and the last y line of vy copy line 0.
+PERIODIC_SHEAR_BC = """
+ P     2   2     2   2     2   2     0       0   0 0 0
+ Vx    0   x     0   y-1   0   0     1       0   0 0 0
+ Vy    0   x-1   0   y     0   0     0       0   0 0 0
+ Vz    0   x-1   0   y-1   0   0     0       0   0 0 0
+ Vx    0   x     0   y-1   z-1 z-1  -1       0   0 0 0
+ Vy    0   x-1   0   y     z-1 z-1   0       0   0 0 0
+ Vz    0   x-1   0   y-1   z   z     0       0   0 0 0
+ Vy    0   x-1   y   y     1   z-2   0       1   +0 -100 +0
+ Vx    x   x     0   y-1   1   z-2   0       1  -100  +0 +0
+"""
+
+
+def periodic_shear(n: int = 33, L: int = None, nz: int = None, seed: int = 5, count: int = 1, r: float = 0.2,
+                   eta_m: float = 1.0, eta_in: float = 1e3, zsize: float = 1.16, jitter: float = 0.25,
+                   knobs=None) -> Problem:
+    """The reference's periodic simple-shear box (parallel/init_inclusion.t3c):
+    x and y periodic over [0, 1], z walls at 0 and ``zsize`` driving vx = +1 /
+    -1, no gravity, ``count`` seeded spherical inclusions of radius ``r``
+    (viscosity ``eta_in`` in an ``eta_m`` matrix; count 0 leaves the box
+    isoviscous)."""
+    nz = n if nz is None else nz
+    L = coarsest_levels(min(n, nz)) if L is None else L
+    gx = uniform_grid(n, 1.0)
+    gz = uniform_grid(nz, zsize)
+    mx, my, mz = markers_regular(gx, gx, gz, jitter=jitter, seed=seed)
+    rng = np.random.default_rng(seed)
+    mnu = np.full(mx.shape, eta_m)
+    mro = np.full(mx.shape, 3000.0)
+    for k in range(count):
+        c = np.array([0.5, 0.5, 0.5 * zsize]) if k == 0 else rng.uniform([0.2, 0.2, 0.2 * zsize], [0.8, 0.8, 0.8 * zsize])
+        ins = (mx - c[0]) ** 2 + (my - c[1]) ** 2 + (mz - c[2]) ** 2 < r * r
+        mnu[ins] = eta_in
+    k = dict(DEFAULT_KNOBS, nubeg=min(eta_m, eta_in) * 1e-3, nuend=max(eta_m, eta_in) * 1e3)
+    if knobs:
+        k.update(knobs)
+    return Problem(n=n, levels=L, gx=gx, gy=gx.copy(), gz=gz, bc=parse_bc_lines(PERIODIC_SHEAR_BC, n, n, nz),
+                   mx=mx, my=my, mz=mz, mnu=mnu.astype(np.float32), mro=mro.astype(np.float32),
+                   gravity=(0.0, 0.0, 0.0), knobs=k, name=f"pshear{n}", periodic=(1, 1, 0))
