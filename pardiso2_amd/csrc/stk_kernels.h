@@ -405,41 +405,38 @@ __device__ __forceinline__ void tile_stage(const Geo<T> &g, const unsigned char 
   const int hy = ty + 2, hx = tx + 2;
   constexpr int kNh = (TY && TX && TZ) ? (TY + 2) * (TX + 2) * (TZ + 2) : 0;
   if constexpr (kNh > 0) {
-    // shape compiled in: every load of the thread's share is issued before
-    // the first LDS store, from a clamped (always valid) address, and the
-    // values outside the level are replaced by zeros afterwards (no branch
-    // around a load, so no wait between them)
+    // shape compiled in (walled levels only; periodic levels run the
+    // run-time-shape instances): every load of the thread's share is issued
+    // before the first LDS store, from a clamped (always valid) address, and
+    // the values outside the level are replaced by zeros afterwards (no
+    // branch around a load, so no wait between them)
     constexpr int kIt = (kNh + BS - 1) / BS;
     V v[kIt];
-    T pv[kIt];
     unsigned char f[kIt];
-    bool ld[kIt], lev[kIt], intr[kIt];
+    bool inl[kIt], intr[kIt];
 #pragma unroll
     for (int k = 0; k < kIt; k++) {
       const int i = k * BS + (int)threadIdx.x;
       const int q = i / (TY + 2);
       const int ly = i - q * (TY + 2), lz = q / (TX + 2), lx = q - lz * (TX + 2);
       const int m2 = y0 + ly - 1, m1 = x0 + lx - 1, m3 = z0 + lz - 1;
-      const StagePos<T> sp = stage_pos<T>(g, m1, m2, m3, i < kNh);
-      ld[k] = sp.ld;
-      lev[k] = sp.lev;
+      inl[k] = i < kNh && m1 >= 0 && m1 < g.X && m2 >= 0 && m2 < g.Y && m3 >= 0 && m3 < g.Z;
       intr[k] = i < kNh && lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && lz >= 1 && lz <= TZ;
-      v[k] = reinterpret_cast<const V *>(xold)[sp.nv];
-      pv[k] = g.per ? xold[4 * (size_t)sp.np] : T(0);
-      f[k] = flags[sp.lev ? __umul24(m3, g.XY) + __umul24(m1, g.Y) + m2 : 0u];
+      const unsigned n = inl[k] ? __umul24(m3, g.XY) + __umul24(m1, g.Y) + m2 : 0u;
+      v[k] = reinterpret_cast<const V *>(xold)[n];
+      f[k] = flags[n];
     }
 #pragma unroll
     for (int k = 0; k < kIt; k++) {
       const int i = k * BS + (int)threadIdx.x;
       if (i >= kNh) break;
       const V z = {T(0), T(0), T(0), T(0)};
-      V w = ld[k] ? v[k] : z;
-      if (g.per && ld[k]) w.x = pv[k];
+      const V w = inl[k] ? v[k] : z;
       sx[i] = w.x; sx[kNh + i] = w.y; sx[2 * kNh + i] = w.z; sx[3 * kNh + i] = w.w;
       if (intr[k]) {
         const int q = i / (TY + 2);
         const int ly = i - q * (TY + 2), lz = q / (TX + 2), lx = q - lz * (TX + 2);
-        sf[((lz - 1) * TX + lx - 1) * TY + ly - 1] = lev[k] ? f[k] : (unsigned char)0;
+        sf[((lz - 1) * TX + lx - 1) * TY + ly - 1] = inl[k] ? f[k] : (unsigned char)0;
       }
     }
     return;

@@ -492,7 +492,8 @@ void launch_smooth(stk_ctx *c, const int *tile, bool sv, const TileGeo &tg, cons
   const dim3 grid(tg.nty * tg.ntx * tg.ntz), blk(256);
   const size_t lds = smooth_lds<T>(tile, sv);
   const T v0 = (T)c->kn.v0koef, p0 = (T)c->kn.p0koef;
-  const bool t344 = tile[0] == 32 && tile[1] == 4 && tile[2] == 4;
+  // the compiled-shape instances stage walled levels only
+  const bool t344 = tile[0] == 32 && tile[1] == 4 && tile[2] == 4 && !g.per;
   if constexpr (std::is_same<T, float>::value) {
     // the V-cycle's velocity sweep: rows' data prefetched per node pair
     // (one pair per thread: 256 threads, or 512 for the deeper tiles)
@@ -501,8 +502,8 @@ void launch_smooth(stk_ctx *c, const int *tile, bool sv, const TileGeo &tg, cons
       const Level *L = level_of(c, g);
       const bool uni = L && L->uni;
       const FluxU fu = uni ? L->fu : FluxU{};
-      const bool t348 = tile[0] == 32 && tile[1] == 4 && tile[2] == 8;
-      const bool t384 = tile[0] == 32 && tile[1] == 8 && tile[2] == 4;
+      const bool t348 = tile[0] == 32 && tile[1] == 4 && tile[2] == 8 && !g.per;
+      const bool t384 = tile[0] == 32 && tile[1] == 8 && tile[2] == 4 && !g.per;
       const dim3 blk2(pairs <= 256 ? 256 : 512);
 #define STK_VSMOOTH(TY, TX, TZ, CHV, BS)                                                                      \
   do {                                                                                                        \
@@ -652,8 +653,14 @@ template <typename T, bool RES>
 void rowop_rows(stk_ctx *c, const Level &L, const T *x, const T *b, T *y, int withp) {
   const int full[6] = {0, 0, 0, L.Y, L.X, L.Z};
   const TileGeo tg = make_tile(32, 4, 4, full);
-  hipLaunchKernelGGL((k_rowop_tile<T, RES, 32, 4, 4>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256),
-                     smooth_lds<T>(kTileDefault4, false), c->st, L.geo<T>(c->kn.p2koef), L.flags, x, b, y, tg, withp);
+  if (L.per)  // the run-time-shape instance stages periodic levels
+    hipLaunchKernelGGL((k_rowop_tile<T, RES>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256),
+                       smooth_lds<T>(kTileDefault4, false), c->st, L.geo<T>(c->kn.p2koef), L.flags, x, b, y, tg,
+                       withp);
+  else
+    hipLaunchKernelGGL((k_rowop_tile<T, RES, 32, 4, 4>), dim3(tg.nty * tg.ntx * tg.ntz), dim3(256),
+                       smooth_lds<T>(kTileDefault4, false), c->st, L.geo<T>(c->kn.p2koef), L.flags, x, b, y, tg,
+                       withp);
 }
 
 template <typename T, bool RES>

@@ -223,6 +223,42 @@ def test_vpiterate_solves_from_the_callers_arrays(oracle_mod):
         g.close()
 
 
+def test_vpiterate_on_the_periodic_shear_box(oracle_mod):
+    """The reference's own setup, periodic in x and y (parallel/init_inclusion.t3c),
+    solved by vpiterate from the caller's arrays; pr on line 0 of each
+    periodic axis is the last line's (move3mg.c:3321-3379), and the stress
+    planes are the reference's periodic ones."""
+    pb = setups.periodic_shear(N, count=1, r=0.25)
+    o = oracle_mod.from_problem(pb)
+    g = S.from_problem(pb)
+    try:
+        g.bind()
+        a = _caller_arrays(o)
+        a["sol0"][:] = 0.0
+        n0 = o.info[0][4]
+        a["pr"] = np.zeros(n0)
+        for k in ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz", "exx", "eyy", "ezz", "exy", "exz",
+                  "eyz"):
+            a[k] = np.zeros(n0, np.float32)
+        S.bind_arrays(a)
+        S.lib().vpiterate(0, 0)
+        x = a["sol0"][:4 * n0]
+        rel = o.relres_rows(x)[0]
+        _say(f"vpiterate on the periodic shear box: oracle relres {rel:.2e}")
+        assert rel <= 1.0000001e-8
+        pr = a["pr"].reshape(N, N, N)  # [m3, m1, m2]
+        np.testing.assert_array_equal(pr[1:, 0, 1:], pr[1:, N - 1, 1:])
+        np.testing.assert_array_equal(pr[1:, :, 0], pr[1:, :, N - 1])
+        sig, eps = g.stress_strain()
+        for k, name in enumerate(("sxx", "syy", "szz", "sxy", "sxz", "syz")):
+            np.testing.assert_array_equal(a[name], sig[k].astype(np.float32), err_msg=name)
+        sxx = a["sxx"].reshape(N, N, N)
+        np.testing.assert_array_equal(sxx[1:, 0, 1:-1], sxx[1:, N - 1, 1:-1])
+    finally:
+        S.bind_arrays(None)
+        g.close()
+
+
 def test_walls_on_the_outer_lines_are_refused_with_their_own_code():
     """The reference's layout with the walls on grid lines 0 and n-1 leaves
     outer-layer velocities without a boundary equation, whose rows would
