@@ -162,6 +162,28 @@ def cpu_baseline(pb, budget_s: float, repeats: int = 5, device=True):
     return out
 
 
+def banded_baseline(pb, repeats: int = 3):
+    """The reference's secondary CPU path: the banded direct solve gausmat3
+    (parallel/gausjomp.c:70-254, rows pipelined over all host threads as
+    there), restated by the oracle, on ``pb``; median of ``repeats`` full
+    solves, the oracle's relative residual of the result."""
+    O = load_oracle()
+    o = O.from_problem(pb)
+    times, band, rc, x = [], 0, 0, None
+    for _ in range(repeats):
+        ts = time.perf_counter()
+        rc, x, band = o.band_solve(threads=0)
+        times.append(time.perf_counter() - ts)
+        if rc:
+            raise RuntimeError(f"banded solve failed ({rc})")
+    rel = float(o.relres_rows(x)[0])
+    med = float(np.median(times))
+    return {"value": 1.0 / med, "unit": "solves/s", "cores": O.threads(), "kind": "port",
+            "sample": (f"full banded direct FP64 solves (gausmat3, band {band}) of the {pb.n}^3 instance of this "
+                       f"workload, {len(times)} repeats, median wall clock, assembly not counted"),
+            "median_s": med, "n": pb.n, "band": int(band), "unknowns": 4 * pb.n ** 3, "relres": rel}
+
+
 def solution_diff(x, ref, X, Y):
     """Max and RMS difference of solution x from ref, velocities and
     pressures separately, relative to ref's max and RMS; pressures compared
@@ -348,6 +370,8 @@ def main():
     ap.add_argument("--maxit", type=int, default=800)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--banded-n", type=int, default=17,
+                    help="grid of the banded direct CPU solve (gausmat3) beside the CPU baseline; 0: skip")
     ap.add_argument("--no-fp64-ref", action="store_true")
     ap.add_argument("--kernel", choices=sorted(KERNELS), default=None,
                     help="only launch one finest-level kernel --reps times (for counter runs) and exit")
@@ -452,7 +476,7 @@ def main():
         log(f"fp64 reference solve: {s64.iterations} iterations, relres {s64.relres:.3e}, {s64.seconds:.2f} s")
         err = solution_error(ctx, x_dev)
 
-    cpu, orel = None, None
+    cpu, orel, banded = None, None, None
     log(f"smoother {ms_sm:.4f} ms ({gbs_sm:.0f} GB/s), fp64 residual {ms_res:.4f} ms ({gbs_res:.0f} GB/s)")
     if not args.no_cpu:
         # the oracle's own residual of the timed solve's solution, on the device's node fields
@@ -476,6 +500,13 @@ def main():
         spb = pb if pb.n <= CPU_SAMPLE_N else make_problem(args.workload, CPU_SAMPLE_N, args.seed)
         cpu = cpu_sample(spb, args.cpu_budget)
         log(f"cpu baseline: {cpu.get('value')} solves/s ({cpu.get('sample')})")
+        if args.banded_n > 0:
+            try:
+                banded = banded_baseline(make_problem(args.workload, args.banded_n, args.seed))
+                log(f"cpu banded direct: {banded['median_s']:.2f} s per solve at {args.banded_n}^3 "
+                    f"(band {banded['band']}, relres {banded['relres']:.1e})")
+            except Exception as e:
+                banded = {"value": None, "error": str(e)}
 
     out = {
         "metric": "Stokes solves/s to 1e-8 FP64 relative residual",
@@ -511,6 +542,7 @@ def main():
         "residual_fp64": {"achieved": gbs_res, "unit": "GB/s", "frac": gbs_res / PEAK_HBM_GBS,
                           "ms_per_launch": ms_res},
         "cpu_baseline": cpu,
+        "cpu_banded_direct": banded,
         "marker_advection": adv,
     }
     print(json.dumps(out), flush=True)

@@ -75,6 +75,7 @@ def lib():
         L.orc_interp.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_double, C.c_double, dp, dp]
         L.orc_solve.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_double, C.c_int, dp]
         L.orc_band_solve.argtypes = [C.c_void_p, dp, C.POINTER(C.c_long)]
+        L.orc_band_solve_mt.argtypes = [C.c_void_p, dp, C.POINTER(C.c_long), C.c_int]
         L.orc_gcr.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                               dp, dp, dp]
         L.orc_hydrostatic.argtypes = [C.c_void_p, dp]
@@ -299,10 +300,15 @@ class Oracle:
                            C.byref(spi))
         return it, x, hist[: max(it, 0) + 1], spi.value
 
-    def band_solve(self):
+    def band_solve(self, threads=None):
+        """Banded direct solve (gausmat3): serial, or with the reference's
+        thread structure when ``threads`` is given (0: the OpenMP default)."""
         x = np.zeros(4 * self.info[0][4])
         bw = C.c_long(0)
-        rc = lib().orc_band_solve(self.h, _dp(x), C.byref(bw))
+        if threads is None:
+            rc = lib().orc_band_solve(self.h, _dp(x), C.byref(bw))
+        else:
+            rc = lib().orc_band_solve_mt(self.h, _dp(x), C.byref(bw), int(threads))
         return rc, x, bw.value
 
 

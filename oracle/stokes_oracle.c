@@ -1598,6 +1598,87 @@ int orc_band_solve(orc_t *o, double *x, long *band_out) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* The same banded elimination with the reference's thread structure          */
+/* (parallel/gausjomp.c:117-215): rows go round-robin to the threads, each    */
+/* thread accumulates its row in a private dense buffer and, before           */
+/* eliminating with an earlier row, waits until that row is finished (the     */
+/* reference's pos0[m2] == -1 flag, here one ready byte per row with          */
+/* acquire/release ordering).  Every row sees the same finished rows in the   */
+/* same order as the serial restatement, so the result is bit-identical to    */
+/* orc_band_solve whatever the thread count; back substitution on one        */
+/* thread (gausjomp.c:219-250).  threads <= 0: the OpenMP default.            */
+/* ------------------------------------------------------------------------- */
+int orc_band_solve_mt(orc_t *o, double *x, long *band_out, int threads) {
+  long n = 4 * o->mgn[0];
+  long rightn = 0;
+  for (long r = 0; r < n; r++)
+    for (int k = 0; k < o->num0[r]; k++) {
+      long d = (long)o->lin0[o->pos0[r] + k] - r;
+      if (d > rightn) rightn = d;
+    }
+  if (band_out) *band_out = rightn;
+  if (rightn < 1) rightn = 1;
+  double *U = (double *)calloc((size_t)n * rightn, sizeof(double));
+  double *f = (double *)malloc(sizeof(double) * n);
+  unsigned char *ready = (unsigned char *)calloc((size_t)n, 1);
+  int nt = threads > 0 ? threads : omp_get_max_threads();
+  double *bufs = (double *)calloc((size_t)nt * (size_t)(n + rightn + 1), sizeof(double));
+  if (!U || !f || !ready || !bufs) { free(U); free(f); free(ready); free(bufs); return -2; }
+  for (long r = 0; r < n; r++) f[r] = o->fre1[r];
+  int err = 0;
+#pragma omp parallel num_threads(nt)
+  {
+    const int t = omp_get_thread_num(), tn = omp_get_num_threads();
+    double *buf = bufs + (size_t)t * (size_t)(n + rightn + 1);
+    for (long m1 = t; m1 < n; m1 += tn) {
+      if (o->num0[m1] && !__atomic_load_n(&err, __ATOMIC_RELAXED)) {
+        long m6 = m1;
+        for (int k = 0; k < o->num0[m1]; k++) {
+          long m4 = o->lin0[o->pos0[m1] + k];
+          buf[m4] += o->val1[o->pos0[m1] + k];
+          if (m4 < m6) m6 = m4;
+        }
+        for (long m2 = m6; m2 < m1; m2++)
+          if (buf[m2]) {
+            double ival = buf[m2];
+            buf[m2] = 0;
+            while (!__atomic_load_n(&ready[m2], __ATOMIC_ACQUIRE)) {
+              if (__atomic_load_n(&err, __ATOMIC_RELAXED)) break;
+            }
+            if (!o->num0[m2]) { __atomic_store_n(&err, -3, __ATOMIC_RELAXED); break; }
+            const double *u = U + m2 * rightn;
+            for (long m3 = m2 + 1; m3 <= m2 + rightn; m3++) buf[m3] -= ival * u[m3 - m2 - 1];
+            f[m1] -= f[m2] * ival;
+          }
+        double ival = buf[m1];
+        buf[m1] = 0;
+        if (!ival) __atomic_store_n(&err, -3, __ATOMIC_RELAXED);
+        double *u = U + m1 * rightn;
+        for (long m2 = m1 + 1; m2 <= m1 + rightn; m2++) {
+          if (ival) u[m2 - m1 - 1] = buf[m2] / ival;
+          buf[m2] = 0;
+        }
+        if (ival) f[m1] /= ival;
+      }
+      __atomic_store_n(&ready[m1], 1, __ATOMIC_RELEASE);
+    }
+  }
+  if (!err)
+    for (long m1 = n - 1; m1 >= 0; m1--) {
+      if (o->num0[m1]) {
+        const double *u = U + m1 * rightn;
+        double ival = f[m1];
+        for (long m2 = m1 + 1; m2 < m1 + 1 + rightn && m2 < n; m2++) ival -= u[m2 - m1 - 1] * x[m2];
+        x[m1] = ival;
+      } else {
+        x[m1] = 0;
+      }
+    }
+  free(U); free(f); free(ready); free(bufs);
+  return err;
+}
+
+/* ------------------------------------------------------------------------- */
 /* FP64 outer iteration on the assembled finest operator: GCR with the right  */
 /* preconditioner z_p = omega*eta_cell*r_p (continuity rows), pressure        */
 /* boundary rows by substitution, z_v = one velocity-block V-cycle of the     */

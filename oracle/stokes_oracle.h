@@ -157,6 +157,9 @@ void orc_interp(const orc_t *o, const double *x, const double *eps, double px, d
 
 /* banded direct solver restatement (parallel/gausjomp.c:70-254) */
 int orc_band_solve(orc_t *o, double *x, long *band_out);
+/* the same with the reference's thread structure (rows round-robin, each
+   waiting for the rows it eliminates with); bit-identical to orc_band_solve */
+int orc_band_solve_mt(orc_t *o, double *x, long *band_out, int threads);
 
 int orc_threads(void);
 

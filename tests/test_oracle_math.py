@@ -124,6 +124,20 @@ def test_band_solver_matches_dense_direct(oracle_mod):
     np.testing.assert_allclose(xb, xd, rtol=1e-7, atol=1e-9 * scale)
 
 
+@pytest.mark.parametrize("threads", [1, 3, 4])
+def test_pipelined_band_solver_is_bit_identical(oracle_mod, threads):
+    # the reference's thread structure (rows round-robin, each waiting for the
+    # rows it eliminates with) repeats the serial elimination exactly
+    pb = setups.sinker(11, 1, eta_in=1e22, eta_out=1e20)
+    pb.bc = setups.free_slip_lines(11, 11, 11, gauge=(2, 2, 2))
+    o = oracle_mod.from_problem(pb)
+    rc, xs, bw = o.band_solve()
+    rc2, xm, bw2 = o.band_solve(threads=threads)
+    assert rc == rc2 == 0 and bw == bw2
+    np.testing.assert_array_equal(xm, xs)
+    assert o.relres_rows(xm)[0] < 1e-12
+
+
 def test_exact_solution_is_fixed_point_of_smoother(oracle_mod):
     pb, o = _small_gauged(oracle_mod)
     A, b = _assembled_dense(o)
