@@ -1,0 +1,585 @@
+/*
+ * heat_oracle.c -- CPU restatement of the reference's heat-equation
+ * multigrid (parallel/heat3mg.c), test infrastructure only: the checker the
+ * HIP library's heat solve is compared against.  Nothing in the product
+ * library links this file.
+ *
+ * The heat solve runs on the Stokes hierarchy of an orc_t (grid lines,
+ * levels 0..Lt, Lt <= L) with its own node arrays: FP32 ro, cp, kt on all
+ * levels, FP32 temperature tk, start-of-step temperature tk0 and source term
+ * tk1 on level 0 (parallel/head3mg.c:95-104), FP64 solver vectors and a
+ * 7-slot row store per node (pos0 = 7 * node, parallel/heat3mg.c:427).
+ */
+#include <math.h>
+#include <omp.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "stokes_oracle.h"
+
+#define NIDH(o, l, m1, m2, m3) ((o)->mgp[l] + (long)(m3) * (o)->mgxy[l] + (long)(m1) * (o)->mgy[l] + (m2))
+
+struct orh_s {
+  orc_t *o;
+  int Lt;
+  long ntot;
+  int *tbondm; /* ntot, 0 = no T boundary equation (the reference's bondm[nodenum8 + node]) */
+  int nbond, cap;
+  float *bv1; /* 2*(cap+1): constant, coefficient */
+  int *bn1;   /* cap+1: partner node + 1 (0: none) */
+  float *ro, *cp, *kt;    /* ntot */
+  float *tk, *tk0, *tk1;  /* mgn[0] */
+  double *sol0, *sol1, *fre0, *fre1; /* ntot */
+  double *val1; /* 7*ntot */
+  int *lin0;    /* 7*ntot */
+  int *num0;    /* ntot */
+  double timestep, t0koef, t1koef;
+};
+
+orh_t *orh_create(orc_t *o, int Lt) {
+  if (!o || Lt < 0 || Lt > o->L) return NULL;
+  orh_t *h = (orh_t *)calloc(1, sizeof(orh_t));
+  h->o = o;
+  h->Lt = Lt;
+  h->ntot = o->mgp[Lt] + o->mgn[Lt];
+  const long nt = h->ntot, n0 = o->mgn[0];
+  h->tbondm = (int *)calloc(nt, sizeof(int));
+  h->cap = 1024;
+  h->bv1 = (float *)calloc(2 * (h->cap + 1), sizeof(float));
+  h->bn1 = (int *)calloc(h->cap + 1, sizeof(int));
+  h->ro = (float *)calloc(nt, sizeof(float));
+  h->cp = (float *)calloc(nt, sizeof(float));
+  h->kt = (float *)calloc(nt, sizeof(float));
+  h->tk = (float *)calloc(n0, sizeof(float));
+  h->tk0 = (float *)calloc(n0, sizeof(float));
+  h->tk1 = (float *)calloc(n0, sizeof(float));
+  h->sol0 = (double *)calloc(nt, sizeof(double));
+  h->sol1 = (double *)calloc(nt, sizeof(double));
+  h->fre0 = (double *)calloc(nt, sizeof(double));
+  h->fre1 = (double *)calloc(nt, sizeof(double));
+  h->val1 = (double *)calloc(7 * nt, sizeof(double));
+  h->lin0 = (int *)calloc(7 * nt, sizeof(int));
+  h->num0 = (int *)calloc(nt, sizeof(int));
+  h->timestep = 1.0;
+  h->t0koef = 1.0;
+  h->t1koef = 1.0;
+  return h;
+}
+
+void orh_destroy(orh_t *h) {
+  if (!h) return;
+  free(h->tbondm); free(h->bv1); free(h->bn1);
+  free(h->ro); free(h->cp); free(h->kt); free(h->tk); free(h->tk0); free(h->tk1);
+  free(h->sol0); free(h->sol1); free(h->fre0); free(h->fre1);
+  free(h->val1); free(h->lin0); free(h->num0);
+  free(h);
+}
+
+void orh_set(orh_t *h, double timestep, double t0koef, double t1koef) {
+  h->timestep = timestep;
+  h->t0koef = t0koef;
+  h->t1koef = t1koef;
+}
+
+void *orh_array(orh_t *h, const char *n) {
+  if (!strcmp(n, "tbondm")) return h->tbondm;
+  if (!strcmp(n, "bv1")) return h->bv1;
+  if (!strcmp(n, "bn1")) return h->bn1;
+  if (!strcmp(n, "ro")) return h->ro;
+  if (!strcmp(n, "cp")) return h->cp;
+  if (!strcmp(n, "kt")) return h->kt;
+  if (!strcmp(n, "tk")) return h->tk;
+  if (!strcmp(n, "tk0")) return h->tk0;
+  if (!strcmp(n, "tk1")) return h->tk1;
+  if (!strcmp(n, "sol0")) return h->sol0;
+  if (!strcmp(n, "fre0")) return h->fre0;
+  if (!strcmp(n, "fre1")) return h->fre1;
+  if (!strcmp(n, "val1")) return h->val1;
+  if (!strcmp(n, "lin0")) return h->lin0;
+  if (!strcmp(n, "num0")) return h->num0;
+  return NULL;
+}
+
+int orh_nbond(const orh_t *h) { return h->nbond; }
+
+/* ------------------------------------------------------------------------- */
+/* T boundary equations on levels 0..Lt: parallel/in3mg.c:302-324 (index     */
+/* ranges kept 3 outer lines per side), 369-385 (case 4, T):                  */
+/* T_n = const + koef * T_{n + shift}, |shift| > 1 periodic (in3mg.c:339-341). */
+/* ------------------------------------------------------------------------- */
+static int lvl_index_h(int m, int nfine, int nlev, long mgs) {
+  int r = (m - 2) / (int)mgs + 2;
+  if (m < 3) r = m;
+  if (m > nfine - 4) r = nlev - (nfine - m);
+  return r;
+}
+
+int orh_bc_line(orh_t *h, int m10, int m11, int m20, int m21, int m30, int m31, float cnst, float koef, int dm1,
+                int dm2, int dm3) {
+  const orc_t *o = h->o;
+  for (int l = 0; l <= h->Lt; l++) {
+    int m1min = lvl_index_h(m10, o->mgx[0], o->mgx[l], o->mgs[l]);
+    int m1max = lvl_index_h(m11, o->mgx[0], o->mgx[l], o->mgs[l]);
+    int m2min = lvl_index_h(m20, o->mgy[0], o->mgy[l], o->mgs[l]);
+    int m2max = lvl_index_h(m21, o->mgy[0], o->mgy[l], o->mgs[l]);
+    int m3min = lvl_index_h(m30, o->mgz[0], o->mgz[l], o->mgs[l]);
+    int m3max = lvl_index_h(m31, o->mgz[0], o->mgz[l], o->mgs[l]);
+    for (int m3 = m3min; m3 <= m3max; m3++)
+      for (int m1 = m1min; m1 <= m1max; m1++)
+        for (int m2 = m2min; m2 <= m2max; m2++) {
+          long m4 = NIDH(o, l, m1, m2, m3);
+          int sx = dm1, sy = dm2, sz = dm3;
+          if (abs(dm1) > 1) sx = -(o->mgx[l] - 1);
+          if (abs(dm2) > 1) sy = -(o->mgy[l] - 1);
+          if (abs(dm3) > 1) sz = -(o->mgz[l] - 1);
+          long m6 = m4 + (long)sz * o->mgxy[l] + (long)sx * o->mgy[l] + sy;
+          if (h->nbond + 1 > h->cap) {
+            int nc = h->cap * 2;
+            h->bv1 = (float *)realloc(h->bv1, 2 * (size_t)(nc + 1) * sizeof(float));
+            h->bn1 = (int *)realloc(h->bn1, (size_t)(nc + 1) * sizeof(int));
+            h->cap = nc;
+          }
+          h->nbond++;
+          h->tbondm[m4] = h->nbond;
+          h->bv1[2 * h->nbond + 0] = cnst;
+          if (dm1 || dm2 || dm3) {
+            h->bv1[2 * h->nbond + 1] = koef;
+            h->bn1[h->nbond] = (int)(m6 + 1);
+          } else {
+            h->bv1[2 * h->nbond + 1] = 0;
+            h->bn1[h->nbond] = 0;
+          }
+        }
+  }
+  return h->nbond;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Ro, Cp, kt restriction to level l: parallel/heat3mg.c:1235-1330.  Every   */
+/* fine node adds to the 8 coarse nodes of its cell (index rules 1254-1272,  */
+/* weights 1-|d|/dx per axis, 1282-1302), then each coarse value is divided  */
+/* by its weight sum.  FP32 arrays, sums in double then stored as float.     */
+/* ------------------------------------------------------------------------- */
+static void cell_lo(int m, int nf, int nc, int *lo) {
+  int r = (m - 2) / 2 + 2;
+  if (m < 3) r = m;
+  if (m > nf - 4) r = nc - (nf - m);
+  if (r > nc - 2) r = nc - 2;
+  *lo = r;
+}
+
+void orh_rocprestrict(orh_t *h, int l) {
+  const orc_t *o = h->o;
+  const int f = l - 1;
+  double *S = h->sol1;
+  for (long m1 = o->mgp[l]; m1 < o->mgp[l] + o->mgn[l]; m1++) {
+    h->cp[m1] = 0; h->kt[m1] = 0; h->ro[m1] = 0; S[m1] = 0;
+  }
+  for (int m3 = 0; m3 < o->mgz[f]; m3++)
+    for (int m1 = 0; m1 < o->mgx[f]; m1++)
+      for (int m2 = 0; m2 < o->mgy[f]; m2++) {
+        long m4 = NIDH(o, f, m1, m2, m3);
+        int m10min, m20min, m30min;
+        cell_lo(m1, o->mgx[f], o->mgx[l], &m10min);
+        cell_lo(m2, o->mgy[f], o->mgy[l], &m20min);
+        cell_lo(m3, o->mgz[f], o->mgz[l], &m30min);
+        double dx = o->mggx[l][m10min + 1] - o->mggx[l][m10min];
+        double dy = o->mggy[l][m20min + 1] - o->mggy[l][m20min];
+        double dz = o->mggz[l][m30min + 1] - o->mggz[l][m30min];
+        for (int m30 = m30min; m30 <= m30min + 1; m30++) {
+          double ddz = (o->mggz[l][m30] - o->mggz[f][m3]) / dz;
+          ddz = 1.0 - fabs(ddz);
+          for (int m10 = m10min; m10 <= m10min + 1; m10++) {
+            double ddx = (o->mggx[l][m10] - o->mggx[f][m1]) / dx;
+            ddx = 1.0 - fabs(ddx);
+            for (int m20 = m20min; m20 <= m20min + 1; m20++) {
+              double ddy = (o->mggy[l][m20] - o->mggy[f][m2]) / dy;
+              ddy = 1.0 - fabs(ddy);
+              long m40 = NIDH(o, l, m10, m20, m30);
+              double swt = ddx * ddy * ddz;
+              h->cp[m40] += h->cp[m4] * swt;
+              h->kt[m40] += h->kt[m4] * swt;
+              h->ro[m40] += h->ro[m4] * swt;
+              S[m40] += swt;
+            }
+          }
+        }
+      }
+  for (long m1 = o->mgp[l]; m1 < o->mgp[l] + o->mgn[l]; m1++) {
+    h->cp[m1] /= S[m1];
+    h->kt[m1] /= S[m1];
+    h->ro[m1] /= S[m1];
+    S[m1] = 0;
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Row of node (m1, m2, m3) of level l: the implicit heat equation           */
+/* heaterr(ynerr = 1), parallel/heat3mg.c:195-249 -- slots: 1 the node        */
+/* (rho cp / dt plus the six face conductances, added x-, x+, y-, y+, z-, z+), */
+/* then -k_face xkf / dx for x-, x+, y-, y+, z-, z+ (k_face the two-node      */
+/* mean); x wraps on a periodic x axis (heat3mg.c:21, 36) -- or the boundary  */
+/* equation tbonderr (heat3mg.c:869-893).  ui1[0]: tk1 + tk0 rho cp / dt on   */
+/* level 0, 0 on coarser levels.                                              */
+/* ------------------------------------------------------------------------- */
+static double rocp_of(const orh_t *h, long v) {
+  double r = (float)(h->ro[v] * h->cp[v]);
+  if (r < 2e+6) r = 2e+6;
+  return r;
+}
+
+int orh_row(orh_t *h, int l, int m1, int m2, int m3, long *un1, double *ui1) {
+  const orc_t *o = h->o;
+  long v3 = NIDH(o, l, m1, m2, m3);
+  int b = h->tbondm[v3];
+  if (b) {
+    un1[0] = 1;
+    ui1[0] = h->bv1[2 * b + 0];
+    un1[1] = v3;
+    ui1[1] = 1.0;
+    if (h->bn1[b]) {
+      un1[0] += 1;
+      un1[un1[0]] = h->bn1[b] - 1;
+      ui1[un1[0]] = -h->bv1[2 * b + 1];
+    }
+    return 1;
+  }
+  const double *gx = o->mggx[l], *gy = o->mggy[l], *gz = o->mggz[l];
+  const int X = o->mgx[l];
+  double xkf;
+  if (o->xper && m1 == 0) xkf = 2.0 / (gx[1] + gx[X - 1] - gx[X - 2]);
+  else xkf = 2.0 / (gx[m1 + 1] - gx[m1 - 1]);
+  double ykf = 2.0 / (gy[m2 + 1] - gy[m2 - 1]);
+  double zkf = 2.0 / (gz[m3 + 1] - gz[m3 - 1]);
+  long v[7];
+  v[3] = v3;
+  v[0] = v3 - o->mgxy[l];
+  if (o->xper && m1 == 0) v[1] = NIDH(o, l, X - 2, m2, m3);
+  else v[1] = v3 - o->mgy[l];
+  v[2] = v3 - 1;
+  v[4] = v3 + 1;
+  v[5] = v3 + o->mgy[l];
+  v[6] = v3 + o->mgxy[l];
+  const float *kt = h->kt;
+  double rocpkf = rocp_of(h, v3), ival;
+  if (l == 0) ui1[0] = h->tk1[v3 - o->mgp[0]] + h->tk0[v3 - o->mgp[0]] / h->timestep * rocpkf;
+  else ui1[0] = 0;
+  ui1[1] = rocpkf / h->timestep;
+  un1[1] = v3;
+  if (o->xper && m1 == 0) ival = (float)(kt[v[1]] + kt[v[3]]) / 2.0 * xkf / (gx[X - 1] - gx[X - 2]);
+  else ival = (float)(kt[v[1]] + kt[v[3]]) / 2.0 * xkf / (gx[m1] - gx[m1 - 1]);
+  ui1[2] = -ival; un1[2] = v[1];
+  ui1[1] += ival;
+  ival = (float)(kt[v[3]] + kt[v[5]]) / 2.0 * xkf / (gx[m1 + 1] - gx[m1]);
+  ui1[1] += ival;
+  ui1[3] = -ival; un1[3] = v[5];
+  ival = (float)(kt[v[2]] + kt[v[3]]) / 2.0 * ykf / (gy[m2] - gy[m2 - 1]);
+  ui1[4] = -ival; un1[4] = v[2];
+  ui1[1] += ival;
+  ival = (float)(kt[v[3]] + kt[v[4]]) / 2.0 * ykf / (gy[m2 + 1] - gy[m2]);
+  ui1[1] += ival;
+  ui1[5] = -ival; un1[5] = v[4];
+  ival = (float)(kt[v[0]] + kt[v[3]]) / 2.0 * zkf / (gz[m3] - gz[m3 - 1]);
+  ui1[6] = -ival; un1[6] = v[0];
+  ui1[1] += ival;
+  ival = (float)(kt[v[3]] + kt[v[6]]) / 2.0 * zkf / (gz[m3 + 1] - gz[m3]);
+  ui1[1] += ival;
+  ui1[7] = -ival; un1[7] = v[6];
+  un1[0] = 7;
+  return 0;
+}
+
+/* rows of levels 0..Lt into the 7-slot store (parallel/heat3mg.c:416-451) */
+void orh_assemble(orh_t *h) {
+  const orc_t *o = h->o;
+  for (int l = 0; l <= h->Lt; l++) {
+#pragma omp parallel for schedule(static)
+    for (int m3 = 0; m3 < o->mgz[l]; m3++) {
+      long un1[16];
+      double ui1[16];
+      for (int m1 = 0; m1 < o->mgx[l]; m1++)
+        for (int m2 = 0; m2 < o->mgy[l]; m2++) {
+          long n = NIDH(o, l, m1, m2, m3);
+          orh_row(h, l, m1, m2, m3, un1, ui1);
+          h->fre1[n] = ui1[0];
+          h->num0[n] = (int)un1[0];
+          for (int k = 0; k < un1[0]; k++) {
+            h->lin0[7 * n + k] = (int)un1[k + 1];
+            h->val1[7 * n + k] = ui1[k + 1];
+          }
+        }
+    }
+  }
+}
+
+static inline void row_update(orh_t *h, long m1) {
+  double ival = -h->fre1[m1];
+  double ival1 = h->val1[7 * m1];
+  for (int m2 = 0; m2 < h->num0[m1]; m2++) ival += h->val1[7 * m1 + m2] * h->sol0[h->lin0[7 * m1 + m2]];
+  h->sol0[m1] -= ival / ival1 * h->t0koef;
+}
+
+/* residual fre0 = fre1 - A sol0 on level l (parallel/heat3mg.c:1101-1117) */
+void orh_residual(orh_t *h, int l) {
+  const orc_t *o = h->o;
+  const long a = o->mgp[l], e = o->mgp[l] + o->mgn[l];
+#pragma omp parallel for schedule(static)
+  for (long m1 = a; m1 < e; m1++)
+    if (h->num0[m1]) {
+      double ival = h->fre1[m1];
+      for (int m2 = 0; m2 < h->num0[m1]; m2++) ival -= h->val1[7 * m1 + m2] * h->sol0[h->lin0[7 * m1 + m2]];
+      h->fre0[m1] = ival;
+    }
+}
+
+/* gausmatt (parallel/heat3mg.c:1057-1120): am Gauss-Seidel sweeps over the   */
+/* level's rows in index order (the reference's loop on one thread), then    */
+/* the residual                                                               */
+void orh_gausmatt(orh_t *h, int am, int l) {
+  const orc_t *o = h->o;
+  const long a = o->mgp[l], e = o->mgp[l] + o->mgn[l];
+  for (int s = 0; s < am; s++)
+    for (long m1 = a; m1 < e; m1++)
+      if (h->num0[m1]) row_update(h, m1);
+  orh_residual(h, l);
+}
+
+static int tbc_depth(const orh_t *h, long r, int lim) {
+  int d = 0;
+  while (d <= lim) {
+    int b = h->tbondm[r];
+    if (!b || !h->bn1[b]) return d;
+    long j = h->bn1[b] - 1;
+    if (!h->tbondm[j]) return d;
+    r = j;
+    d++;
+  }
+  return -1;
+}
+
+/* The same row updates in the device's multicolour order: heat-equation    */
+/* rows of colour (m1+m2+m3)&1 = 0, then of colour 1, then the boundary rows */
+/* by dependency depth (a row's partner first); -1 if boundary rows cycle.   */
+int orh_gausmatt_colored(orh_t *h, int am, int l) {
+  const orc_t *o = h->o;
+  const long a = o->mgp[l], e = o->mgp[l] + o->mgn[l];
+  int maxd = 0;
+  for (long r = a; r < e; r++)
+    if (h->tbondm[r]) {
+      int d = tbc_depth(h, r, 64);
+      if (d < 0) return -1;
+      if (d > maxd) maxd = d;
+    }
+  for (int s = 0; s < am; s++) {
+    for (int col = 0; col < 2; col++)
+#pragma omp parallel for schedule(static)
+      for (int m3 = 0; m3 < o->mgz[l]; m3++)
+        for (int m1 = 0; m1 < o->mgx[l]; m1++)
+          for (int m2 = 0; m2 < o->mgy[l]; m2++) {
+            if (((m1 + m2 + m3) & 1) != col) continue;
+            long n = NIDH(o, l, m1, m2, m3);
+            if (h->num0[n] && !h->tbondm[n]) row_update(h, n);
+          }
+    for (int d = 0; d <= maxd; d++)
+      for (long r = a; r < e; r++)
+        if (h->tbondm[r] && tbc_depth(h, r, 64) == d) row_update(h, r);
+  }
+  orh_residual(h, l);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Residual restriction to level l (parallel/heat3mg.c:1122-1232): every     */
+/* fine residual adds to the coarse nodes of its cell without a boundary     */
+/* equation, weights 1 -+ d/dx per axis (1170-1192); then division by the    */
+/* weight sums, coarse solution 0.                                            */
+/* ------------------------------------------------------------------------- */
+void orh_trestrict(orh_t *h, int l) {
+  const orc_t *o = h->o;
+  const int f = l - 1;
+  for (long m1 = o->mgp[l]; m1 < o->mgp[l] + o->mgn[l]; m1++) { h->fre1[m1] = 0; h->sol1[m1] = 0; }
+  for (int m3 = 0; m3 < o->mgz[f]; m3++)
+    for (int m1 = 0; m1 < o->mgx[f]; m1++)
+      for (int m2 = 0; m2 < o->mgy[f]; m2++) {
+        long m4 = NIDH(o, f, m1, m2, m3);
+        int m10min, m20min, m30min;
+        cell_lo(m1, o->mgx[f], o->mgx[l], &m10min);
+        cell_lo(m2, o->mgy[f], o->mgy[l], &m20min);
+        cell_lo(m3, o->mgz[f], o->mgz[l], &m30min);
+        double dx = o->mggx[l][m10min + 1] - o->mggx[l][m10min];
+        double dy = o->mggy[l][m20min + 1] - o->mggy[l][m20min];
+        double dz = o->mggz[l][m30min + 1] - o->mggz[l][m30min];
+        for (int m30 = m30min; m30 <= m30min + 1; m30++) {
+          double ddz = (o->mggz[l][m30] - o->mggz[f][m3]) / dz;
+          if (m30 == m30min) ddz = -ddz;
+          ddz = 1.0 - ddz;
+          for (int m10 = m10min; m10 <= m10min + 1; m10++) {
+            double ddx = (o->mggx[l][m10] - o->mggx[f][m1]) / dx;
+            if (m10 == m10min) ddx = -ddx;
+            ddx = 1.0 - ddx;
+            for (int m20 = m20min; m20 <= m20min + 1; m20++) {
+              double ddy = (o->mggy[l][m20] - o->mggy[f][m2]) / dy;
+              if (m20 == m20min) ddy = -ddy;
+              ddy = 1.0 - ddy;
+              long m40 = NIDH(o, l, m10, m20, m30);
+              if (!h->tbondm[m40]) {
+                double swt = ddx * ddy * ddz;
+                h->fre1[m40] += h->fre0[m4] * swt;
+                h->sol1[m40] += swt;
+              }
+            }
+          }
+        }
+      }
+  for (long m1 = o->mgp[l]; m1 < o->mgp[l] + o->mgn[l]; m1++) {
+    if (h->sol1[m1]) {
+      h->fre1[m1] /= h->sol1[m1];
+      h->sol1[m1] = 0;
+    }
+    h->sol0[m1] = 0;
+  }
+}
+
+/* Correction prolongation from level l (parallel/heat3mg.c:954-1050): every */
+/* fine node without a boundary equation adds t1koef * sum w sol0_c over the */
+/* 8 coarse nodes of its cell, weights as the restriction's, unnormalised.   */
+void orh_tprolong(orh_t *h, int l, int ynreset) {
+  const orc_t *o = h->o;
+  const int f = l - 1;
+#pragma omp parallel for schedule(static)
+  for (int m3 = 0; m3 < o->mgz[f]; m3++)
+    for (int m1 = 0; m1 < o->mgx[f]; m1++)
+      for (int m2 = 0; m2 < o->mgy[f]; m2++) {
+        long m4 = NIDH(o, f, m1, m2, m3);
+        if (h->tbondm[m4]) continue;
+        int m10min, m20min, m30min;
+        cell_lo(m1, o->mgx[f], o->mgx[l], &m10min);
+        cell_lo(m2, o->mgy[f], o->mgy[l], &m20min);
+        cell_lo(m3, o->mgz[f], o->mgz[l], &m30min);
+        double dx = o->mggx[l][m10min + 1] - o->mggx[l][m10min];
+        double dy = o->mggy[l][m20min + 1] - o->mggy[l][m20min];
+        double dz = o->mggz[l][m30min + 1] - o->mggz[l][m30min];
+        if (ynreset) h->sol0[m4] = 0;
+        for (int m30 = m30min; m30 <= m30min + 1; m30++) {
+          double ddz = (o->mggz[l][m30] - o->mggz[f][m3]) / dz;
+          if (m30 == m30min) ddz = -ddz;
+          ddz = 1.0 - ddz;
+          for (int m10 = m10min; m10 <= m10min + 1; m10++) {
+            double ddx = (o->mggx[l][m10] - o->mggx[f][m1]) / dx;
+            if (m10 == m10min) ddx = -ddx;
+            ddx = 1.0 - ddx;
+            for (int m20 = m20min; m20 <= m20min + 1; m20++) {
+              double ddy = (o->mggy[l][m20] - o->mggy[f][m2]) / dy;
+              if (m20 == m20min) ddy = -ddy;
+              ddy = 1.0 - ddy;
+              long m40 = NIDH(o, l, m10, m20, m30);
+              double swt = ddx * ddy * ddz;
+              h->sol0[m4] += h->sol0[m40] * swt * h->t1koef;
+            }
+          }
+        }
+      }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Errors of the level-0 temperature tk (parallel/heat3mg.c:528-565):        */
+/* heatsum = RMS over heat-equation nodes of heaterr(ynerr = 2) * timestep   */
+/* (heat3mg.c:251-287) on the FP32 tk, tk0, tk1 as the reference's, and     */
+/* bondsum = RMS of the boundary equations' errors on the FP64 solution      */
+/* (tbonderr ynerr = 1, 854-867).                                             */
+/* ------------------------------------------------------------------------- */
+void orh_err(orh_t *h, double *heatsum, double *bondsum) {
+  const orc_t *o = h->o;
+  const int X = o->mgx[0];
+  const double *gx = o->mggx[0], *gy = o->mggy[0], *gz = o->mggz[0];
+  double hs = 0, hn = 0, bs = 0, bn = 0;
+  const float *tk = h->tk, *tk0 = h->tk0, *tk1 = h->tk1, *kt = h->kt;
+  for (int m3 = 0; m3 < o->mgz[0]; m3++)
+    for (int m1 = 0; m1 < X; m1++)
+      for (int m2 = 0; m2 < o->mgy[0]; m2++) {
+        long n = NIDH(o, 0, m1, m2, m3);
+        int b = h->tbondm[n];
+        if (b) {
+          double lb = h->sol0[n] - h->bv1[2 * b + 0];
+          if (h->bn1[b]) lb -= h->bv1[2 * b + 1] * h->sol0[h->bn1[b] - 1];
+          bs += lb * lb;
+          bn += 1.0;
+          continue;
+        }
+        double xkf;
+        if (o->xper && m1 == 0) xkf = 2.0 / (gx[1] + gx[X - 1] - gx[X - 2]);
+        else xkf = 2.0 / (gx[m1 + 1] - gx[m1 - 1]);
+        double ykf = 2.0 / (gy[m2 + 1] - gy[m2 - 1]);
+        double zkf = 2.0 / (gz[m3 + 1] - gz[m3 - 1]);
+        long v[7];
+        v[3] = n;
+        v[0] = n - o->mgxy[0];
+        v[1] = (o->xper && m1 == 0) ? NIDH(o, 0, X - 2, m2, m3) : n - o->mgy[0];
+        v[2] = n - 1; v[4] = n + 1; v[5] = n + o->mgy[0]; v[6] = n + o->mgxy[0];
+        double rocpkf = rocp_of(h, n), ival, ival1;
+        ival1 = (tk[n] - tk0[n]) / h->timestep;
+        if (o->xper && m1 == 0) ival = (float)(kt[v[1]] + kt[v[3]]) / 2.0 * xkf / (gx[X - 1] - gx[X - 2]) / rocpkf;
+        else ival = (float)(kt[v[1]] + kt[v[3]]) / 2.0 * xkf / (gx[m1] - gx[m1 - 1]) / rocpkf;
+        ival1 += (tk[v[3]] - tk[v[1]]) * ival;
+        ival = (float)(kt[v[3]] + kt[v[5]]) / 2.0 * xkf / (gx[m1 + 1] - gx[m1]) / rocpkf;
+        ival1 -= (tk[v[5]] - tk[v[3]]) * ival;
+        ival = (float)(kt[v[2]] + kt[v[3]]) / 2.0 * ykf / (gy[m2] - gy[m2 - 1]) / rocpkf;
+        ival1 += (tk[v[3]] - tk[v[2]]) * ival;
+        ival = (float)(kt[v[3]] + kt[v[4]]) / 2.0 * ykf / (gy[m2 + 1] - gy[m2]) / rocpkf;
+        ival1 -= (tk[v[4]] - tk[v[3]]) * ival;
+        ival = (float)(kt[v[0]] + kt[v[3]]) / 2.0 * zkf / (gz[m3] - gz[m3 - 1]) / rocpkf;
+        ival1 += (tk[v[3]] - tk[v[0]]) * ival;
+        ival = (float)(kt[v[3]] + kt[v[6]]) / 2.0 * zkf / (gz[m3 + 1] - gz[m3]) / rocpkf;
+        ival1 -= (tk[v[6]] - tk[v[3]]) * ival;
+        ival1 -= tk1[n] / rocpkf;
+        ival1 *= h->timestep;
+        hs += ival1 * ival1;
+        hn += 1.0;
+      }
+  *heatsum = hn > 0 ? sqrt(hs / hn) : 0.0;
+  *bondsum = bn > 0 ? sqrt(bs / bn) : 0.0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* One implicit temperature step (titerate, parallel/heat3mg.c:384-590):     */
+/* ro/cp/kt restricted to levels 1..Lt, rows assembled, sol0 = tk, then      */
+/* groups of multicyct V-cycles (up: smooth pre[l] + restrict, down: smooth  */
+/* post[l] + prolong, level 0 not smoothed after the last prolongation,      */
+/* heat3mg.c:463-492; Lt = 0: pre[0] sweeps) until heatsum <= heatmin or     */
+/* maxit groups; tk = sol0 after each group.  colored: the device's sweep    */
+/* order.  Returns the number of groups (negative: not converged).            */
+/* ------------------------------------------------------------------------- */
+int orh_step(orh_t *h, const int *pre, const int *post, int multicyct, double heatmin, int maxit, int colored,
+             double *heatsum, double *bondsum) {
+  const orc_t *o = h->o;
+  for (int l = 1; l <= h->Lt; l++) orh_rocprestrict(h, l);
+  orh_assemble(h);
+  for (long m1 = 0; m1 < o->mgn[0]; m1++) h->sol0[m1] = h->tk[m1];
+  double hs = 0, bs = 0;
+  int it = 0;
+  do {
+    for (int c = 0; c < multicyct; c++) {
+      if (h->Lt > 0) {
+        for (int l = 0; l < h->Lt; l++) {
+          if (colored) orh_gausmatt_colored(h, pre[l], l);
+          else orh_gausmatt(h, pre[l], l);
+          orh_trestrict(h, l + 1);
+        }
+        for (int l = h->Lt; l > 0; l--) {
+          if (colored) orh_gausmatt_colored(h, post[l], l);
+          else orh_gausmatt(h, post[l], l);
+          orh_tprolong(h, l, 0);
+        }
+      } else {
+        if (colored) orh_gausmatt_colored(h, pre[0], 0);
+        else orh_gausmatt(h, pre[0], 0);
+      }
+    }
+    for (long m1 = 0; m1 < o->mgn[0]; m1++) h->tk[m1] = (float)h->sol0[m1];
+    orh_err(h, &hs, &bs);
+    it++;
+  } while (hs > heatmin && it < maxit);
+  if (heatsum) *heatsum = hs;
+  if (bondsum) *bondsum = bs;
+  return hs > heatmin ? -it : it;
+}

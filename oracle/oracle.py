@@ -80,6 +80,26 @@ def lib():
                               dp, dp, dp]
         L.orc_hydrostatic.argtypes = [C.c_void_p, dp]
         L.orc_weights.argtypes = [C.c_void_p, dp]
+        # heat-equation multigrid (heat_oracle.c)
+        L.orh_create.restype = C.c_void_p
+        L.orh_create.argtypes = [C.c_void_p, C.c_int]
+        L.orh_destroy.argtypes = [C.c_void_p]
+        L.orh_set.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.orh_array.restype = C.c_void_p
+        L.orh_array.argtypes = [C.c_void_p, C.c_char_p]
+        L.orh_nbond.restype = C.c_int
+        L.orh_nbond.argtypes = [C.c_void_p]
+        L.orh_bc_line.argtypes = [C.c_void_p] + [C.c_int] * 6 + [C.c_float, C.c_float] + [C.c_int] * 3
+        L.orh_rocprestrict.argtypes = [C.c_void_p, C.c_int]
+        L.orh_row.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_long), dp]
+        L.orh_assemble.argtypes = [C.c_void_p]
+        L.orh_residual.argtypes = [C.c_void_p, C.c_int]
+        L.orh_gausmatt.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orh_gausmatt_colored.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orh_trestrict.argtypes = [C.c_void_p, C.c_int]
+        L.orh_tprolong.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orh_err.argtypes = [C.c_void_p, dp, dp]
+        L.orh_step.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_double, C.c_int, C.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -155,7 +175,10 @@ class Oracle:
         return C.cast(p, C.POINTER(C.c_int))[0]
 
     def bc_lines(self, lines):
+        """Boundary lines of the Stokes unknowns (T lines, variable 4, belong to Heat)."""
         for ln in lines:
+            if int(ln[0]) == 4:
+                continue
             lib().orc_bc_line(self.h, *[int(v) for v in ln[:7]], float(ln[7]), float(ln[8]),
                               *[int(v) for v in ln[9:12]])
 
@@ -364,6 +387,103 @@ def post_norms(out, divvmax=0.0):
             "stoknum": out[3], "contnum": out[5], "contnumb": out[7], "bondnum": out[9], "maxvxyz": mv,
             "minvx": out[10], "maxvx": out[11], "minvy": out[12], "maxvy": out[13], "minvz": out[14],
             "maxvz": out[15], "minpr": out[16], "maxpr": out[17], "minnu": out[18], "maxnu": out[19]}
+
+
+class Heat:
+    """The heat-equation multigrid of the reference (parallel/heat3mg.c) on
+    levels 0..Lt of an Oracle's hierarchy: T boundary lines, FP32 ro/cp/kt
+    (all levels), tk/tk0/tk1 (level 0), 7-slot rows, gausmatt, trestrict,
+    tprolong, rocprestrict and the titerate step."""
+
+    def __init__(self, oracle: "Oracle", Lt: int):
+        self.o = oracle
+        self.Lt = Lt
+        self.h = lib().orh_create(oracle.h, Lt)
+        if not self.h:
+            raise ValueError("bad heat levels")
+        self.ntot = oracle.info[Lt][5] + oracle.info[Lt][4]
+        self.n0 = oracle.info[0][4]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orh_destroy(self.h)
+            self.h = None
+
+    def set(self, timestep, t0koef=1.0, t1koef=1.0):
+        lib().orh_set(self.h, timestep, t0koef, t1koef)
+
+    def arr(self, name):
+        p = lib().orh_array(self.h, name.encode())
+        n = self.n0 if name in ("tk", "tk0", "tk1") else self.ntot
+        if name in ("val1", "lin0"):
+            n *= 7
+        if name in ("bv1", "bn1"):
+            n = (2 if name == "bv1" else 1) * (lib().orh_nbond(self.h) + 1)
+        ct = {"tbondm": C.c_int, "bn1": C.c_int, "lin0": C.c_int, "num0": C.c_int, "bv1": C.c_float,
+              "ro": C.c_float, "cp": C.c_float, "kt": C.c_float, "tk": C.c_float, "tk0": C.c_float,
+              "tk1": C.c_float}.get(name, C.c_double)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), shape=(n,))
+
+    def bc_lines(self, lines):
+        """Boundary lines of variable T (4) in the setups.BcLine form; others are skipped."""
+        for ln in lines:
+            if int(ln[0]) != 4:
+                continue
+            lib().orh_bc_line(self.h, *[int(v) for v in ln[1:7]], float(ln[7]), float(ln[8]),
+                              *[int(v) for v in ln[9:12]])
+
+    def set_fields(self, ro, cp, kt, tk, tk1):
+        """Level-0 properties, temperature at the step start (also tk0) and source term."""
+        n0 = self.n0
+        for name, v in (("ro", ro), ("cp", cp), ("kt", kt)):
+            self.arr(name)[:n0] = np.asarray(v, dtype=np.float32)[:n0]
+        self.arr("tk")[:] = np.asarray(tk, dtype=np.float32)[:n0]
+        self.arr("tk0")[:] = np.asarray(tk, dtype=np.float32)[:n0]
+        self.arr("tk1")[:] = np.asarray(tk1, dtype=np.float32)[:n0]
+
+    def rocprestrict(self, l):
+        lib().orh_rocprestrict(self.h, l)
+
+    def row(self, l, m1, m2, m3):
+        un1 = (C.c_long * 16)()
+        ui1 = (C.c_double * 16)()
+        lib().orh_row(self.h, l, m1, m2, m3, un1, ui1)
+        k = un1[0]
+        return np.array(un1[1:k + 1]), np.array(ui1[1:k + 1]), ui1[0]
+
+    def assemble(self):
+        lib().orh_assemble(self.h)
+
+    def residual(self, l):
+        lib().orh_residual(self.h, l)
+
+    def gausmatt(self, am, l, colored=False):
+        if colored:
+            if lib().orh_gausmatt_colored(self.h, am, l):
+                raise RuntimeError("cyclic T boundary equations")
+        else:
+            lib().orh_gausmatt(self.h, am, l)
+
+    def trestrict(self, l):
+        lib().orh_trestrict(self.h, l)
+
+    def tprolong(self, l, ynreset=0):
+        lib().orh_tprolong(self.h, l, ynreset)
+
+    def err(self):
+        a, b = C.c_double(), C.c_double()
+        lib().orh_err(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def step(self, pre, post, multicyct=1, heatmin=1e-4, maxit=200, colored=True):
+        p = np.zeros(12, np.int32)
+        q = np.zeros(12, np.int32)
+        p[:len(pre)] = pre
+        q[:len(post)] = post
+        a, b = C.c_double(), C.c_double()
+        it = lib().orh_step(self.h, _ip(p), _ip(q), multicyct, heatmin, maxit, 1 if colored else 0, C.byref(a),
+                            C.byref(b))
+        return int(it), a.value, b.value
 
 
 def threads() -> int:

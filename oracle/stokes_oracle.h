@@ -163,6 +163,29 @@ int orc_band_solve_mt(orc_t *o, double *x, long *band_out, int threads);
 
 int orc_threads(void);
 
+/* ---- heat-equation multigrid (parallel/heat3mg.c), heat_oracle.c ---------- */
+typedef struct orh_s orh_t;
+/* heat solve on levels 0..Lt of o's hierarchy (Lt: the reference's multinumt) */
+orh_t *orh_create(orc_t *o, int Lt);
+void orh_destroy(orh_t *h);
+void orh_set(orh_t *h, double timestep, double t0koef, double t1koef);
+void *orh_array(orh_t *h, const char *name);
+int orh_nbond(const orh_t *h);
+/* T boundary line of the reference grammar (parallel/in3mg.c:302-385, var T) */
+int orh_bc_line(orh_t *h, int m10, int m11, int m20, int m21, int m30, int m31, float cnst, float koef, int dm1,
+                int dm2, int dm3);
+void orh_rocprestrict(orh_t *h, int l);                                  /* heat3mg.c:1235-1330 */
+int orh_row(orh_t *h, int l, int m1, int m2, int m3, long *un1, double *ui1); /* heaterr / tbonderr */
+void orh_assemble(orh_t *h);
+void orh_residual(orh_t *h, int l);
+void orh_gausmatt(orh_t *h, int am, int l);                              /* heat3mg.c:1057-1120 */
+int orh_gausmatt_colored(orh_t *h, int am, int l);
+void orh_trestrict(orh_t *h, int l);                                     /* heat3mg.c:1122-1232 */
+void orh_tprolong(orh_t *h, int l, int ynreset);                         /* heat3mg.c:954-1050 */
+void orh_err(orh_t *h, double *heatsum, double *bondsum);                /* heat3mg.c:528-565 */
+int orh_step(orh_t *h, const int *pre, const int *post, int multicyct, double heatmin, int maxit, int colored,
+             double *heatsum, double *bondsum);                          /* titerate, heat3mg.c:384-590 */
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,7 +24,7 @@ from typing import List, Sequence, Tuple
 
 import numpy as np
 
-VARS = {"P": 0, "Vx": 1, "Vy": 2, "Vz": 3}
+VARS = {"P": 0, "Vx": 1, "Vy": 2, "Vz": 3, "T": 4}
 
 BcLine = Tuple[int, int, int, int, int, int, int, float, float, int, int, int]
 
@@ -40,8 +40,9 @@ def parse_index(tok: str, nmax: int) -> int:
 def parse_bc_lines(text: str, nx: int, ny: int, nz: int) -> List[BcLine]:
     """Parse boundary-equation lines in the reference grammar (in3mg.c:240-290).
 
-    Comment lines start with '/', the block ends at '~'.  Only P/Vx/Vy/Vz
-    equations are solver boundary rows; other variables are rejected.
+    Comment lines start with '/', the block ends at '~'.  P/Vx/Vy/Vz lines
+    are the Stokes solve's boundary rows, T (4) lines the heat solve's
+    (in3mg.c:243-247); other variables are rejected.
     """
     out: List[BcLine] = []
     for raw in text.splitlines():
@@ -386,3 +387,68 @@ def periodic_shear(n: int = 33, L: int = None, nz: int = None, seed: int = 5, co
     return Problem(n=n, levels=L, gx=gx, gy=gx.copy(), gz=gz, bc=parse_bc_lines(PERIODIC_SHEAR_BC, n, n, nz),
                    mx=mx, my=my, mz=mz, mnu=mnu.astype(np.float32), mro=mro.astype(np.float32),
                    gravity=(0.0, 0.0, 0.0), knobs=k, name=f"pshear{n}", periodic=(1, 1, 0))
+
+
+# ---------------------------------------------------------------------------
+# Heat equation (parallel/heat3mg.c): node properties, temperature and
+# boundary lines of variable T in the reference grammar.
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class HeatProblem:
+    n: int
+    levels: int
+    gx: np.ndarray
+    gy: np.ndarray
+    gz: np.ndarray
+    bc: List[BcLine]
+    ro: np.ndarray   # FP32 node arrays of level 0, node order (m3, m1, m2)
+    cp: np.ndarray
+    kt: np.ndarray
+    tk: np.ndarray   # temperature at the start of the step (K)
+    tk1: np.ndarray  # heat source term of the step (the reference's tk1, W/m^3)
+    timestep: float
+    periodic: Tuple[int, int, int] = (0, 0, 0)
+    name: str = ""
+
+
+def heat_bc_text(t_top: float, t_bottom: float, xperiodic: bool = False) -> str:
+    """T boundary lines: fixed temperatures on the z faces, zero flux on the
+    other faces (copy of the inner neighbour), or x periodic (the last x line
+    copies line 0, in3mg.c:339-341)."""
+    lines = [f" T 0 x 0 y 0 0 {t_top} 0", f" T 0 x 0 y z z {t_bottom} 0"]
+    if xperiodic:
+        lines.append(" T x x 0 y 1 z-1 0 1 -100 0 0")
+    else:
+        lines += [" T 0 0 0 y 1 z-1 0 1 +1 0 0", " T x x 0 y 1 z-1 0 1 -1 0 0"]
+    lines += [" T 0 x 0 0 1 z-1 0 1 0 +1 0", " T 0 x y y 1 z-1 0 1 0 -1 0"]
+    return "\n".join(lines) + "\n"
+
+
+def heat_box(n: int = 65, L: int = None, seed: int = 4, timestep: float = 1e4, xperiodic: bool = False,
+             kt_lo: float = 1.0, kt_hi: float = 5.0, corr: float = 0.15) -> HeatProblem:
+    """Unit box, one implicit temperature step: conductivity a seeded Gaussian
+    random field in [kt_lo, kt_hi] W/m/K, rho 3000-3300 kg/m^3, cp 1000-1200
+    J/kg/K, temperature (scaled to [0, 1]) 0 on top (z = 0) to 1 at the
+    bottom with a seeded perturbation, a source term 1e-6 scaled by a second
+    field.  timestep 1e4 s makes diffusion dominate each cell (k dt /
+    (rho cp h^2) ~ 40 at 65^3, ~650 at 257^3), so the step needs the
+    multigrid.  The temperature is kept of order one: the reference's error
+    measure reads the FP32 temperature, whose rounding sets its floor."""
+    L = coarsest_levels(n) if L is None else L
+    g = uniform_grid(n, 1.0)
+    zz, xx, yy = np.meshgrid(g, g, g, indexing="ij")  # node order (m3, m1, m2)
+    f1 = gaussian_random_field(33, corr, seed)
+    f2 = gaussian_random_field(33, corr, seed + 1)
+    u1 = np.clip(0.5 + 0.25 * sample_trilinear(f1, xx.ravel(), yy.ravel(), zz.ravel()), 0.0, 1.0)
+    u2 = np.clip(0.5 + 0.25 * sample_trilinear(f2, xx.ravel(), yy.ravel(), zz.ravel()), 0.0, 1.0)
+    rng = np.random.default_rng(seed)
+    kt = kt_lo + (kt_hi - kt_lo) * u1
+    ro = 3000.0 + 300.0 * u2
+    cp = rng.uniform(1000.0, 1200.0, kt.shape)
+    tk = zz.ravel() + 0.02 * (u1 - 0.5)
+    tk1 = 1e-6 * (0.5 + u2)
+    bc = parse_bc_lines(heat_bc_text(0.0, 1.0, xperiodic), n, n, n)
+    return HeatProblem(n=n, levels=L, gx=g, gy=g.copy(), gz=g.copy(), bc=bc, ro=ro.astype(np.float32),
+                       cp=cp.astype(np.float32), kt=kt.astype(np.float32), tk=tk.astype(np.float32),
+                       tk1=tk1.astype(np.float32), timestep=timestep,
+                       periodic=(1 if xperiodic else 0, 0, 0), name=f"heat{n}")

@@ -1,0 +1,174 @@
+"""The oracle's heat-equation multigrid (parallel/heat3mg.c) pinned by
+closed forms (CPU).
+
+- Rows (heaterr ynerr = 1, heat3mg.c:195-249): on a uniform grid with
+  constant conductivity every face conductance is k/h^2 and the diagonal
+  rho cp/dt + 6k/h^2, with rho cp clamped below at 2e6 (heat3mg.c:46-47);
+  the periodic x wrap (heat3mg.c:21, 36) gives the same row on line 0.
+- A temperature linear in z with no source is a steady exact state of the
+  discrete step (Dirichlet top and bottom, zero-flux or periodic sides).
+- Layered conductivity k(z): the piecewise-linear T(z) whose flux
+  (k_face dT/dz, k_face the mean of the two nodes) is constant is exact;
+  the same profile with the face means shifted by one line is not.
+- Transfers (heat3mg.c:954-1330): restriction of a constant residual is the
+  constant (normalised), prolongation reproduces a linear coarse field on
+  the fine nodes, property restriction reproduces a linear field.
+- The titerate step (heat3mg.c:384-590) converges to the direct solution of
+  its level-0 rows, in the reference's sweep order and in the device's.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+from pardiso2_amd import setups
+
+
+def _heat(oracle_mod, n, Lt, bc_text, periodic=(0, 0, 0), fields=None, timestep=1e4):
+    g = setups.uniform_grid(n, 1.0)
+    o = oracle_mod.Oracle(g, g.copy(), g.copy(), max(Lt, 1), periodic=periodic)
+    h = oracle_mod.Heat(o, Lt)
+    h.bc_lines(setups.parse_bc_lines(bc_text, n, n, n))
+    N = n ** 3
+    ro, cp, kt, tk, tk1 = fields if fields else (np.full(N, 3000.0), np.full(N, 1000.0), np.full(N, 3.0),
+                                                  np.zeros(N), np.zeros(N))
+    h.set_fields(ro, cp, kt, tk, tk1)
+    h.set(timestep)
+    return g, o, h
+
+
+def test_row_closed_form_and_periodic_wrap(oracle_mod):
+    n, k, dt = 9, 3.0, 1e4
+    g, o, h = _heat(oracle_mod, n, 0, setups.heat_bc_text(0.0, 1.0, xperiodic=True), periodic=(1, 0, 0),
+                    fields=(np.full(n ** 3, 1500.0), np.full(n ** 3, 1000.0), np.full(n ** 3, k), np.full(n ** 3, 0.25),
+                            np.full(n ** 3, 2.0)))
+    hh = g[1] - g[0]
+    R = 2e6  # 1500 * 1000 is clamped up to 2e6
+    for m1 in (0, 3):  # line 0 wraps across the periodic x axis
+        idx, val, rhs = h.row(0, m1, 4, 4)
+        assert len(idx) == 7
+        np.testing.assert_allclose(val[0], R / dt + 6 * k / hh ** 2, rtol=1e-13)
+        np.testing.assert_allclose(val[1:], -k / hh ** 2, rtol=1e-12)
+        np.testing.assert_allclose(rhs, 2.0 + 0.25 * R / dt, rtol=1e-13)
+        nb = {int(v) for v in idx[1:]}
+        xm = (n - 2) if m1 == 0 else m1 - 1
+        assert 4 * n * n + xm * n + 4 in nb  # x- neighbour (wrapped on line 0)
+
+
+@pytest.mark.parametrize("xper", [False, True])
+def test_linear_temperature_is_steady_exact(oracle_mod, xper):
+    n = 9  # spacing 1/8: T = z is exact in FP32
+    N = n ** 3
+    g = setups.uniform_grid(n, 1.0)
+    tk = np.broadcast_to(g[:, None, None], (n, n, n)).reshape(-1)  # T = z
+    kxy = np.random.default_rng(2).uniform(1.0, 4.0, (n, n)).astype(np.float32)
+    kt = np.broadcast_to(kxy[None], (n, n, n)).reshape(-1)  # varies across z lines only
+    fields = (np.full(N, 3100.0), np.full(N, 1100.0), kt, tk, np.zeros(N))
+    g, o, h = _heat(oracle_mod, n, 0, setups.heat_bc_text(0.0, 1.0, xperiodic=xper),
+                    periodic=(int(xper), 0, 0), fields=fields)
+    h.assemble()
+    h.arr("sol0")[:N] = tk
+    h.residual(0)
+    r = h.arr("fre0")[:N]
+    b = h.arr("fre1")[:N]
+    assert np.abs(r).max() < 1e-12 * np.abs(b).max()
+
+
+def _layered(n, shift):
+    rng = np.random.default_rng(3)
+    kz = rng.uniform(1.0, 6.0, n).astype(np.float32)  # conductivity per z node line
+    g = setups.uniform_grid(n, 1.0)
+    h = g[1] - g[0]
+    kf = (kz[:-1] + kz[1:]).astype(np.float64) / 2.0  # face between lines m3 and m3+1 (FP32 sum)
+    if shift:
+        kf = np.roll(kf, 1)
+    q = 0.37
+    T = np.concatenate([[0.0], np.cumsum(q * h / kf)])
+    return kz, T
+
+
+@pytest.mark.parametrize("shift", [False, True])
+def test_layered_conductivity_flux_form(oracle_mod, shift):
+    n = 13
+    N = n ** 3
+    kz, T = _layered(n, shift)
+    kt = np.broadcast_to(kz[:, None, None], (n, n, n)).reshape(-1)
+    tk = np.broadcast_to(T[:, None, None], (n, n, n)).reshape(-1)
+    bc = setups.heat_bc_text(float(np.float32(T[0])), float(np.float32(T[-1])))
+    # a step so long that the rho cp / dt term vanishes: the rows are the steady flux balance
+    g, o, h = _heat(oracle_mod, n, 0, bc, fields=(np.full(N, 3000.0), np.full(N, 1000.0), kt, tk, np.zeros(N)),
+                    timestep=1e30)
+    h.assemble()
+    h.arr("sol0")[:N] = tk
+    h.residual(0)
+    r = h.arr("fre0")[:N].reshape(n, n, n)[1:-1, 1:-1, 1:-1]
+    scale = np.abs(h.arr("val1")).max() * np.abs(T).max()
+    if shift:
+        assert np.abs(r).max() > 1e-6 * scale
+    else:
+        assert np.abs(r).max() < 1e-12 * scale
+
+
+def test_transfers_reproduce_constants_and_linear_fields(oracle_mod):
+    n = 17
+    N = n ** 3
+    g, o, h = _heat(oracle_mod, n, 1, setups.heat_bc_text(0.0, 1.0))
+    h.assemble()
+    i1 = o.info[1]
+    b1, n1 = i1[5], i1[4]
+    # restriction: a constant fine residual gives the constant on every coarse node without a T equation
+    h.arr("fre0")[:N] = 2.5
+    h.trestrict(1)
+    tb = h.arr("tbondm")[b1:b1 + n1]
+    f1 = h.arr("fre1")[b1:b1 + n1]
+    np.testing.assert_allclose(f1[tb == 0], 2.5, rtol=1e-14)
+    assert np.all(f1[tb != 0] == 0)
+    # prolongation of a coarse field linear in x, y, z onto fine nodes without a T equation
+    X, Y, Z = i1[:3]
+    gc = [o.grid(1, a) for a in range(3)]
+    cz, cx, cy = np.meshgrid(gc[2], gc[0], gc[1], indexing="ij")
+    lin = lambda x, y, z: 0.3 + 1.5 * x - 0.7 * y + 2.0 * z
+    h.arr("sol0")[b1:b1 + n1] = lin(cx, cy, cz).reshape(-1)
+    h.arr("sol0")[:N] = 0.0
+    h.tprolong(1, 0)
+    fz, fx, fy = np.meshgrid(g, g, g, indexing="ij")
+    fine = h.arr("sol0")[:N]
+    tbf = h.arr("tbondm")[:N]
+    np.testing.assert_allclose(fine[tbf == 0], lin(fx, fy, fz).reshape(-1)[tbf == 0], rtol=1e-12, atol=1e-12)
+    # property restriction of a linear conductivity: exact where the coarse node's two spacings are equal
+    # (the 17-line axis coarsens to lines 0 1 2 4 6 ... 14 15 16: coarse lines 3..7 are symmetric)
+    h.arr("kt")[:N] = lin(fx, fy, fz).reshape(-1)
+    h.rocprestrict(1)
+    kc = h.arr("kt")[b1:b1 + n1].reshape(Z, X, Y)
+    s = slice(3, -3)
+    np.testing.assert_allclose(kc[s, s, s], lin(cx, cy, cz)[s, s, s], rtol=2e-6)
+    assert np.abs(kc[2, 5, 5] - lin(cx, cy, cz)[2, 5, 5]) > 1e-3  # uneven spacings: the hat mean shifts
+
+
+def _direct(h, N):
+    num0, lin0, val1 = h.arr("num0")[:N], h.arr("lin0"), h.arr("val1")
+    rows, cols, vals = [], [], []
+    for r in range(N):
+        for k in range(num0[r]):
+            rows.append(r)
+            cols.append(lin0[7 * r + k])
+            vals.append(val1[7 * r + k])
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
+    return spl.spsolve(A.tocsc(), h.arr("fre1")[:N].copy())
+
+
+@pytest.mark.parametrize("colored,xper", [(False, False), (True, False), (True, True)])
+def test_multigrid_step_converges_to_direct_solution(oracle_mod, colored, xper):
+    n, Lt = 17, 2
+    hp = setups.heat_box(n, Lt, xperiodic=xper)
+    g = hp.gx
+    o = oracle_mod.Oracle(g, g.copy(), g.copy(), Lt, periodic=hp.periodic)
+    h = oracle_mod.Heat(o, Lt)
+    h.bc_lines(hp.bc)
+    h.set_fields(hp.ro, hp.cp, hp.kt, hp.tk, hp.tk1)
+    h.set(hp.timestep)
+    it, hs, bs = h.step([3, 3, 3], [0, 3, 3], heatmin=1e-6, maxit=60, colored=colored)
+    assert 0 < it < 60, (it, hs)
+    N = n ** 3
+    x = _direct(h, N)
+    np.testing.assert_allclose(h.arr("tk"), x.astype(np.float32), rtol=0, atol=1e-5)
