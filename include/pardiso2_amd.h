@@ -146,7 +146,8 @@ int stk_set_knobs(stk_ctx *ctx, const stk_knobs *k);
 void stk_default_opts(stk_solver_opts *o, int levels);
 
 /* One boundary-equation line of the reference grammar, projected on every
- * level (restates parallel/in3mg.c:234-390).  var: 0 P, 1 vx, 2 vy, 3 vz. */
+ * level (restates parallel/in3mg.c:234-390).  var: 0 P, 1 vx, 2 vy, 3 vz,
+ * 4 T (the temperature rows of the heat solve, in3mg.c:369-385). */
 int stk_bc_line(stk_ctx *ctx, int var, int m10, int m11, int m20, int m21, int m30, int m31,
                 float cnst, float koef, int dm1, int dm2, int dm3);
 /* Boundary tables in the reference's own layout (parallel/head3mg.c:110-112):
@@ -231,6 +232,73 @@ int stk_hydrostatic_guess(stk_ctx *ctx);
 int stk_time_kernel(stk_ctx *ctx, int which, int level, int reps, double *ms_avg);
 /* Shift P so that P(px,py,pz) = pinit (parallel/move3mg.c:2835-2856). */
 int stk_pressure_gauge(stk_ctx *ctx, int px, int py, int pz, double pinit);
+
+/* ---- temperature step (parallel/heat3mg.c) ---------------------------- */
+/* The implicit heat-equation step of titerate on levels 0..levels of the
+ * context's hierarchy: 7-point rows of heaterr (heat3mg.c:195-249, rho cp
+ * clamped below at 2e6, face conductivities the two-node means, x wraps on
+ * an x-periodic box), boundary rows of the var 4 lines, Gauss-Seidel in two
+ * colours and boundary passes, residual restriction and correction
+ * prolongation (trestrict, tprolong), property restriction (rocprestrict).
+ * Every node without a T boundary row needs its six neighbours on the grid
+ * (STK_EINVAL otherwise), i.e. T rows on every face except x line 0 of an
+ * x-periodic box. */
+typedef struct {
+  int levels;          /* multinumt: coarse levels under the finest (0: Gauss-Seidel alone) */
+  int cycles;          /* multicyct: V-cycles per iteration */
+  int pre[12];         /* multittt[0][l]: sweeps on level l going down (level 0: every iteration) */
+  int post[12];        /* multittt[1][l]: sweeps on level l >= 1 going up */
+  double heatmin;      /* HEATMIN: iterate until the RMS heat-equation error (K) is at most this */
+  double t0koef;       /* relaxation of a sweep's update */
+  double t1koef;       /* weight of the prolonged correction */
+  double maxtkstep;    /* largest temperature change (K) a sub-step may take from the source term */
+  int maxit;           /* iterations per sub-step before STK_ENOCONV (the reference has no cap) */
+} stk_heat_opts;
+
+typedef struct {
+  int substeps;        /* sub-steps taken (0: timestep or the sub-step limit was 0) */
+  int iterations;      /* iterations over all sub-steps */
+  int converged;       /* 1 if every sub-step reached heatmin */
+  double heatsum, heatnum;  /* RMS heat-equation error of the last iteration, its node count */
+  double bondsum, bondnum;  /* RMS boundary-row error, its row count */
+  double tkstep;       /* sub-step limit maxtkstep / largest rate (1e30 without a source) */
+  double tkmin, tkmax; /* temperature range after the step */
+  double seconds;      /* wall time of the step */
+} stk_heat_stats;
+
+/* The reference's shipped values: levels 0, cycles 1, pre {10}, post {0},
+ * heatmin 1e-4, t0koef = t1koef = 1, maxtkstep 25 K, maxit 2500 (cyc2max). */
+void stk_heat_default_opts(stk_heat_opts *o);
+/* Level-0 properties (FP32, mgn[0] each): density ro, heat capacity cp,
+ * conductivity kt, temperature tk, source term ht (tk1: W/m^3, the right part
+ * of heaterr, heat3mg.c:54-62, 183). */
+int stk_heat_set_fields(stk_ctx *ctx, const float *ro, const float *cp, const float *kt, const float *tk,
+                        const float *ht);
+/* ro, cp, kt of every level (ntot each, restricted by the last step or
+ * stk_heat_prepare); tk and tk0 of level 0.  Any pointer may be NULL. */
+int stk_heat_get_fields(stk_ctx *ctx, float *ro, float *cp, float *kt, float *tk, float *tk0);
+/* The whole temperature step (titerate, parallel/heat3mg.c:292-616): the
+ * sub-step limit from the largest heating rate (331-382), rocprestrict on
+ * levels 1..levels, then sub-steps, each from tk0 = tk: groups of `cycles`
+ * V-cycles, tk = sol0 and the errors after each group, until heatsum <=
+ * heatmin.  STK_ENOCONV if a sub-step reached maxit first (the step still
+ * runs to its end).  st may be NULL. */
+int stk_heat_titerate(stk_ctx *ctx, double timestep, const stk_heat_opts *o, stk_heat_stats *st);
+/* The pieces, for inspection.  prepare: rocprestrict on levels 1..levels,
+ * then the start of one sub-step of `timestep` (tk0 = tk, sol0 = tk, the
+ * level-0 right-hand side fre1).  gausmatt: `sweeps` sweeps on level l, then
+ * its residual fre0 (heat3mg.c:1057-1120).  trestrict: level l-1's fre0 into
+ * level l's fre1, sol0 = 0 there (1122-1232).  tprolong: level l's sol0 into
+ * level l-1 (954-1050).  err: tk = (float) sol0, then the RMS errors
+ * (heat3mg.c:507-553). */
+int stk_heat_prepare(stk_ctx *ctx, int levels, double timestep, double t0koef, double t1koef);
+int stk_heat_gausmatt(stk_ctx *ctx, int level, int sweeps);
+int stk_heat_trestrict(stk_ctx *ctx, int level);
+int stk_heat_tprolong(stk_ctx *ctx, int level, int ynreset);
+int stk_heat_err(stk_ctx *ctx, double *heatsum, double *bondsum);
+/* sol0 / fre1 / fre0 (STK_SOL0, STK_FRE1, STK_FRE0) of the heat solve on
+ * level l (mgn[l] doubles): dir 0 device to host, 1 host to device. */
+int stk_heat_vec(stk_ctx *ctx, int which, int level, double *host, int dir);
 
 /* ---- reference arrays on the context (host <-> device) ---------------- */
 #define STK_SOL0 0

@@ -549,12 +549,12 @@ void orh_err(orh_t *h, double *heatsum, double *bondsum) {
 /* maxit groups; tk = sol0 after each group.  colored: the device's sweep    */
 /* order.  Returns the number of groups (negative: not converged).            */
 /* ------------------------------------------------------------------------- */
-int orh_step(orh_t *h, const int *pre, const int *post, int multicyct, double heatmin, int maxit, int colored,
-             double *heatsum, double *bondsum) {
+static int orh_substep(orh_t *h, const int *pre, const int *post, int multicyct, double heatmin, int maxit,
+                       int colored, double *heatsum, double *bondsum) {
   const orc_t *o = h->o;
-  for (int l = 1; l <= h->Lt; l++) orh_rocprestrict(h, l);
-  orh_assemble(h);
+  for (long m1 = 0; m1 < o->mgn[0]; m1++) h->tk0[m1] = h->tk[m1];
   for (long m1 = 0; m1 < o->mgn[0]; m1++) h->sol0[m1] = h->tk[m1];
+  orh_assemble(h);
   double hs = 0, bs = 0;
   int it = 0;
   do {
@@ -582,4 +582,60 @@ int orh_step(orh_t *h, const int *pre, const int *post, int multicyct, double he
   if (heatsum) *heatsum = hs;
   if (bondsum) *bondsum = bs;
   return hs > heatmin ? -it : it;
+}
+
+int orh_step(orh_t *h, const int *pre, const int *post, int multicyct, double heatmin, int maxit, int colored,
+             double *heatsum, double *bondsum) {
+  for (int l = 1; l <= h->Lt; l++) orh_rocprestrict(h, l);
+  return orh_substep(h, pre, post, multicyct, heatmin, maxit, colored, heatsum, bondsum);
+}
+
+/* ------------------------------------------------------------------------- */
+/* The whole thermal step of titerate (parallel/heat3mg.c:313-616): the      */
+/* largest heating rate |tk1 / rho cp| over the heat-equation nodes           */
+/* (heaterr ynerr = 0 with the source term tk1 alone, 54-62 and 183-187)     */
+/* limits the sub-step to maxtkstep / rate (373-382); nothing is done for a  */
+/* zero step (387); ro/cp/kt restricted once (397); sub-steps of             */
+/* min(timestep, limit), the last one cut to what is left (403-405), each    */
+/* the iteration of orh_step from tk0 = tk (408-411) until heatsum <=        */
+/* heatmin.  out[8]: sub-steps, iterations (all sub-steps), heatsum and      */
+/* bondsum of the last iteration, the sub-step limit, tk min and max after,  */
+/* and 1 if every sub-step converged.  Returns the sub-step count.           */
+/* ------------------------------------------------------------------------- */
+int orh_titerate(orh_t *h, double timestep, double maxtkstep, const int *pre, const int *post, int multicyct,
+                 double heatmin, int maxit, int colored, double *out) {
+  const orc_t *o = h->o;
+  double rate = 0, tkstep = 1e+30;
+  for (long n = 0; n < o->mgn[0]; n++)
+    if (!h->tbondm[n]) {
+      double v = fabs(h->tk1[n] / rocp_of(h, n));
+      if (v > rate) rate = v;
+    }
+  if (rate) tkstep = maxtkstep / rate;
+  int sub = 0, its = 0, conv = 1;
+  double hs = 0, bs = 0;
+  if (timestep != 0 && tkstep != 0) {
+    for (int l = 1; l <= h->Lt; l++) orh_rocprestrict(h, l);
+    double tsum = 0, tsum0 = timestep;
+    do {
+      double dt = tsum0 < tkstep ? tsum0 : tkstep;
+      if (dt > tsum0 - tsum) dt = tsum0 - tsum;
+      h->timestep = dt;
+      int it = orh_substep(h, pre, post, multicyct, heatmin, maxit, colored, &hs, &bs);
+      if (it < 0) conv = 0;
+      its += it < 0 ? -it : it;
+      sub++;
+      tsum += dt;
+    } while (tsum < tsum0);
+    h->timestep = tsum0;
+  }
+  double mn = 1e30, mx = -1e30;
+  for (long n = 0; n < o->mgn[0]; n++) {
+    if (h->tk[n] < mn) mn = h->tk[n];
+    if (h->tk[n] > mx) mx = h->tk[n];
+  }
+  if (out) {
+    out[0] = sub; out[1] = its; out[2] = hs; out[3] = bs; out[4] = tkstep; out[5] = mn; out[6] = mx; out[7] = conv;
+  }
+  return sub;
 }

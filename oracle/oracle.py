@@ -100,6 +100,8 @@ def lib():
         L.orh_tprolong.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orh_err.argtypes = [C.c_void_p, dp, dp]
         L.orh_step.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_double, C.c_int, C.c_int, dp, dp]
+        L.orh_titerate.argtypes = [C.c_void_p, C.c_double, C.c_double, ip, ip, C.c_int, C.c_double, C.c_int,
+                                   C.c_int, dp]
         _lib = L
     return _lib
 
@@ -484,6 +486,18 @@ class Heat:
         it = lib().orh_step(self.h, _ip(p), _ip(q), multicyct, heatmin, maxit, 1 if colored else 0, C.byref(a),
                             C.byref(b))
         return int(it), a.value, b.value
+
+    def titerate(self, timestep, maxtkstep, pre, post, multicyct=1, heatmin=1e-4, maxit=200, colored=True):
+        """Whole thermal step with the sub-step limit; returns a dict of the step's figures."""
+        p = np.zeros(12, np.int32)
+        q = np.zeros(12, np.int32)
+        p[:len(pre)] = pre
+        q[:len(post)] = post
+        out = np.zeros(8)
+        lib().orh_titerate(self.h, timestep, maxtkstep, _ip(p), _ip(q), multicyct, heatmin, maxit,
+                           1 if colored else 0, _dp(out))
+        keys = ("substeps", "iterations", "heatsum", "bondsum", "tkstep", "tkmin", "tkmax", "converged")
+        return {k: (int(v) if k in ("substeps", "iterations", "converged") else float(v)) for k, v in zip(keys, out)}
 
 
 def threads() -> int:

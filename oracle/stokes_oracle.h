@@ -185,6 +185,9 @@ void orh_tprolong(orh_t *h, int l, int ynreset);                         /* heat
 void orh_err(orh_t *h, double *heatsum, double *bondsum);                /* heat3mg.c:528-565 */
 int orh_step(orh_t *h, const int *pre, const int *post, int multicyct, double heatmin, int maxit, int colored,
              double *heatsum, double *bondsum);                          /* titerate, heat3mg.c:384-590 */
+/* the whole thermal step with the sub-step limit (heat3mg.c:313-616); out[8] */
+int orh_titerate(orh_t *h, double timestep, double maxtkstep, const int *pre, const int *post, int multicyct,
+                 double heatmin, int maxit, int colored, double *out);
 
 #ifdef __cplusplus
 }

@@ -104,6 +104,22 @@ class Stats(C.Structure):
                 ("hist", C.c_double * 256), ("norms", Norms)]
 
 
+class HeatOpts(C.Structure):
+    _fields_ = [("levels", C.c_int), ("cycles", C.c_int), ("pre", C.c_int * 12), ("post", C.c_int * 12),
+                ("heatmin", C.c_double), ("t0koef", C.c_double), ("t1koef", C.c_double),
+                ("maxtkstep", C.c_double), ("maxit", C.c_int)]
+
+
+class HeatStats(C.Structure):
+    _fields_ = [("substeps", C.c_int), ("iterations", C.c_int), ("converged", C.c_int),
+                ("heatsum", C.c_double), ("heatnum", C.c_double), ("bondsum", C.c_double),
+                ("bondnum", C.c_double), ("tkstep", C.c_double), ("tkmin", C.c_double), ("tkmax", C.c_double),
+                ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -163,6 +179,17 @@ def lib():
         L.stk_bind_arrays.argtypes = [C.POINTER(RefArrays)]
         L.stk_move_markers.argtypes = [vp, C.c_long, fp, fp, fp, fp, fp, ip, fp, C.c_double, C.c_int, C.c_int]
         L.stk_stress_strain.argtypes = [vp, dp, dp]
+        L.stk_heat_default_opts.argtypes = [C.POINTER(HeatOpts)]
+        L.stk_heat_default_opts.restype = None
+        L.stk_heat_set_fields.argtypes = [vp] + [fp] * 5
+        L.stk_heat_get_fields.argtypes = [vp] + [fp] * 5
+        L.stk_heat_titerate.argtypes = [vp, C.c_double, C.POINTER(HeatOpts), C.POINTER(HeatStats)]
+        L.stk_heat_prepare.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double]
+        L.stk_heat_gausmatt.argtypes = [vp, C.c_int, C.c_int]
+        L.stk_heat_trestrict.argtypes = [vp, C.c_int]
+        L.stk_heat_tprolong.argtypes = [vp, C.c_int, C.c_int]
+        L.stk_heat_err.argtypes = [vp, dp, dp]
+        L.stk_heat_vec.argtypes = [vp, C.c_int, C.c_int, dp, C.c_int]
         L.matclear.argtypes = [C.c_int, dp]
         L.matclear.restype = None
         L.gausmat.argtypes = [C.c_int, dp, dp]
@@ -210,6 +237,21 @@ def default_opts(levels: int, **kw) -> Opts:
         elif k == "tile":
             for i, x in enumerate(v):
                 o.tile[i] = int(x)
+        else:
+            setattr(o, k, v)
+    return o
+
+
+def heat_default_opts(**kw) -> HeatOpts:
+    o = HeatOpts()
+    lib().stk_heat_default_opts(C.byref(o))
+    for k, v in kw.items():
+        if k in ("pre", "post"):
+            arr = getattr(o, k)
+            for i in range(12):
+                arr[i] = 0
+            for i, x in enumerate(v):
+                arr[i] = int(x)
         else:
             setattr(o, k, v)
     return o
@@ -421,8 +463,62 @@ class Stokes:
         check(lib().stk_schur_apply(self.h, C.byref(o), _dp(r), _dp(z)), "stk_schur_apply")
         return z
 
+    # ---- temperature step (parallel/heat3mg.c) ----
+    def heat_set_fields(self, ro, cp, kt, tk, ht):
+        a = [np.ascontiguousarray(v, dtype=np.float32)[: self.n0] for v in (ro, cp, kt, tk, ht)]
+        check(lib().stk_heat_set_fields(self.h, *[_fp(v) for v in a]), "stk_heat_set_fields")
+
+    def heat_fields(self):
+        """ro, cp, kt of every level; tk, tk0 of level 0."""
+        out = [np.zeros(self.ntot, np.float32) for _ in range(3)] + [np.zeros(self.n0, np.float32) for _ in range(2)]
+        check(lib().stk_heat_get_fields(self.h, *[_fp(v) for v in out]), "stk_heat_get_fields")
+        return dict(zip(("ro", "cp", "kt", "tk", "tk0"), out))
+
+    def heat_titerate(self, timestep, opts: HeatOpts = None, raise_on_fail=True, **kw):
+        o = opts if opts is not None else heat_default_opts(**kw)
+        st = HeatStats()
+        rc = lib().stk_heat_titerate(self.h, timestep, C.byref(o), C.byref(st))
+        if rc and (raise_on_fail or rc != -5):
+            check(rc, "stk_heat_titerate")
+        return st
+
+    def heat_prepare(self, levels, timestep, t0koef=1.0, t1koef=1.0):
+        check(lib().stk_heat_prepare(self.h, levels, timestep, t0koef, t1koef), "stk_heat_prepare")
+
+    def heat_gausmatt(self, level, sweeps):
+        check(lib().stk_heat_gausmatt(self.h, level, sweeps), "stk_heat_gausmatt")
+
+    def heat_trestrict(self, level):
+        check(lib().stk_heat_trestrict(self.h, level), "stk_heat_trestrict")
+
+    def heat_tprolong(self, level, ynreset=0):
+        check(lib().stk_heat_tprolong(self.h, level, ynreset), "stk_heat_tprolong")
+
+    def heat_err(self):
+        a, b = C.c_double(), C.c_double()
+        check(lib().stk_heat_err(self.h, C.byref(a), C.byref(b)), "stk_heat_err")
+        return a.value, b.value
+
+    def heat_vec_get(self, which, level):
+        out = np.zeros(self.info[level][4])
+        check(lib().stk_heat_vec(self.h, which, level, _dp(out), 0), "stk_heat_vec")
+        return out
+
+    def heat_vec_set(self, which, level, x):
+        a = np.ascontiguousarray(x, dtype=np.float64)
+        assert a.size == self.info[level][4]
+        check(lib().stk_heat_vec(self.h, which, level, _dp(a), 1), "stk_heat_vec")
+
 
 SOL0, FRE1, FRE0 = 0, 1, 2
+
+
+def from_heat_problem(hp):
+    """Context holding a setups.HeatProblem: hierarchy, T boundary lines, level-0 fields."""
+    s = Stokes(hp.gx, hp.gy, hp.gz, hp.levels, periodic=hp.periodic)
+    s.bc_lines(hp.bc)
+    s.heat_set_fields(hp.ro, hp.cp, hp.kt, hp.tk, hp.tk1)
+    return s
 
 
 def solve_batch(contexts, opts=None, threads=16, prepare=False):

@@ -1,0 +1,357 @@
+// stk_heat.h -- the implicit temperature step on the device (restates the
+// heat-equation multigrid of parallel/heat3mg.c: heaterr 6-290, titerate
+// 292-616, tprolong 954-1050, gausmatt 1057-1120, trestrict 1122-1232,
+// rocprestrict 1235-1330).
+//
+// Rows are not stored: a node's 7-point row is rebuilt from the FP32
+// conductivity kt and the clamped FP32 rho*cp wherever it is needed, in the
+// reference's operation order (face mean as an FP32 sum, then / 2.0 * kf /
+// spacing in FP64; diagonal rho cp / dt plus the faces in the order x-, x+,
+// y-, y+, z-, z+), so every coefficient equals the reference's assembled
+// one bit for bit.  A sweep reads 8 bytes of temperature, 8 of right-hand
+// side and 8 of properties per node instead of a 56-byte row.
+//
+// Gauss-Seidel runs in two colours of (m1 + m2 + m3) parity -- the 7-point
+// rows of one colour read only the other colour -- then the boundary rows by
+// dependency depth.  On an x-periodic level with an even line count the
+// rows on line 0 read line X-2 of their own colour; they run first, so they
+// read it before its update, as the reference's loop order does.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace stk {
+
+struct HeatGeo {
+  int X, Y, Z, XY;
+  int xper;                 // x wraps: line 0's x- neighbour is line X-2 (heat3mg.c:21, 36)
+  const double *kf[3];      // 2 / (g[m+1] - g[m-1]) per line (x line 0 of a periodic axis: the wrapped span)
+  const double *dm[3];      // g[m] - g[m-1] (x line 0 periodic: g[X-1] - g[X-2])
+  const double *dp[3];      // g[m+1] - g[m]
+};
+
+// the row of heat-equation node n = (m1, m2, m3): neighbours x-, x+, y-, y+,
+// z-, z+, their conductances (the negated off-diagonals) and the diagonal
+// (heat3mg.c:195-249)
+struct HeatRow {
+  long nb[6];
+  double c[6];
+  double d;
+};
+
+__device__ __forceinline__ void heat_row(const HeatGeo &g, const float *__restrict__ kt, float rcp, double dt,
+                                         int m1, int m2, int m3, long n, HeatRow &h) {
+  h.nb[0] = (g.xper && m1 == 0) ? n + (long)(g.X - 2) * g.Y : n - g.Y;
+  h.nb[1] = n + g.Y;
+  h.nb[2] = n - 1;
+  h.nb[3] = n + 1;
+  h.nb[4] = n - g.XY;
+  h.nb[5] = n + g.XY;
+  const float k0 = kt[n];
+  const int mm[3] = {m1, m2, m3};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float sm = kt[h.nb[2 * a]] + k0;      // FP32 sums, as the reference's float arrays add
+    const float sp = k0 + kt[h.nb[2 * a + 1]];
+    h.c[2 * a] = (double)sm / 2.0 * g.kf[a][mm[a]] / g.dm[a][mm[a]];
+    h.c[2 * a + 1] = (double)sp / 2.0 * g.kf[a][mm[a]] / g.dp[a][mm[a]];
+  }
+  double d = (double)rcp / dt;
+#pragma unroll
+  for (int k = 0; k < 6; k++) d += h.c[k];
+  h.d = d;
+}
+
+// rho cp of a node, FP32 product clamped below at 2e6 (heat3mg.c:46-47)
+__global__ void k_heat_rcp(long n, const float *__restrict__ ro, const float *__restrict__ cp, float *rcp) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float r = ro[i] * cp[i];
+  rcp[i] = r < 2e+6f ? 2e+6f : r;
+}
+
+// rocprestrict (heat3mg.c:1235-1330): every coarse node gathers the fine
+// nodes of the 8 cells around it, (z, x, y) in the reference's scan order,
+// weights (1 - |d|) per axis multiplied x * y * z; FP32 accumulators of FP64
+// increments and an FP64 weight sum, as the reference's float arrays; then
+// the coarse rho cp
+__global__ void k_heat_props(int Xf, int Yf, int Xc, int Yc, int Zc, CompTab t, const float *__restrict__ rof,
+                             const float *__restrict__ cpf, const float *__restrict__ ktf, float *roc, float *cpc,
+                             float *ktc, float *rcpc) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XYc = Xc * Yc, XYf = Xf * Yf;
+  if (n >= (long)XYc * Zc) return;
+  int cx, cy, cz;
+  decode(n, Xc, Yc, XYc, cx, cy, cz);
+  float acp = 0.f, akt = 0.f, aro = 0.f;
+  double ws = 0.0;
+  for (int a = t.ax[2].cbeg[cz]; a < t.ax[2].cbeg[cz + 1]; a++) {
+    const int k = t.ax[2].cfi[a];
+    const double wz = t.ax[2].cw64[a];
+    for (int bq = t.ax[0].cbeg[cx]; bq < t.ax[0].cbeg[cx + 1]; bq++) {
+      const int i = t.ax[0].cfi[bq];
+      const double wx = t.ax[0].cw64[bq];
+      for (int e = t.ax[1].cbeg[cy]; e < t.ax[1].cbeg[cy + 1]; e++) {
+        const int j = t.ax[1].cfi[e];
+        const double wy = t.ax[1].cw64[e];
+        const long nf = (long)k * XYf + (long)i * Yf + j;
+        const double swt = wx * wy * wz;
+        acp = (float)((double)acp + cpf[nf] * swt);
+        akt = (float)((double)akt + ktf[nf] * swt);
+        aro = (float)((double)aro + rof[nf] * swt);
+        ws += swt;
+      }
+    }
+  }
+  const float c = (float)((double)acp / ws), k = (float)((double)akt / ws), r = (float)((double)aro / ws);
+  cpc[n] = c;
+  ktc[n] = k;
+  roc[n] = r;
+  const float rc = r * c;
+  rcpc[n] = rc < 2e+6f ? 2e+6f : rc;
+}
+
+// block reduction of up to 4 values (sum, sum, min, max) over 256 threads
+__device__ __forceinline__ void heat_block_red(double v0, double v1, double v2, double v3, double *out) {
+  __shared__ double s[4][256];
+  const int t = threadIdx.x;
+  s[0][t] = v0; s[1][t] = v1; s[2][t] = v2; s[3][t] = v3;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      s[0][t] += s[0][t + w];
+      s[1][t] += s[1][t + w];
+      s[2][t] = fmin(s[2][t], s[2][t + w]);
+      s[3][t] = fmax(s[3][t], s[3][t + w]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[4 * blockIdx.x + 0] = s[0][0];
+    out[4 * blockIdx.x + 1] = s[1][0];
+    out[4 * blockIdx.x + 2] = s[2][0];
+    out[4 * blockIdx.x + 3] = s[3][0];
+  }
+}
+
+// largest heating rate |tk1 / rho cp| over the heat-equation nodes (heaterr
+// ynerr = 0 with the source term alone, heat3mg.c:54-62, 183-187; the
+// titerate pass 331-365); per block in out[4 b + 3]
+__global__ void __launch_bounds__(256) k_heat_rate(long n, const unsigned char *__restrict__ isbc,
+                                                   const float *__restrict__ tk1, const float *__restrict__ rcp,
+                                                   double *out) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < n && !isbc[i]) v = fabs(tk1[i] / (double)rcp[i]);
+  heat_block_red(0.0, 0.0, 0.0, v, out);
+}
+
+// start of a sub-step (heat3mg.c:408-416, 429-451): tk0 = tk, sol0 = tk, and
+// the level-0 right-hand side of the heat rows, tk1 + tk0 / dt * rho cp
+// (heat3mg.c:204); boundary rows take their constant from k_heat_bcc
+__global__ void k_heat_rhs(long n, const unsigned char *__restrict__ isbc, const float *__restrict__ tk,
+                           float *tk0, const float *__restrict__ tk1, const float *__restrict__ rcp, double dt,
+                           double *x, double *b) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float t = tk[i];
+  tk0[i] = t;
+  x[i] = (double)t;
+  b[i] = isbc[i] ? 0.0 : (double)tk1[i] + (double)t / dt * (double)rcp[i];
+}
+
+__global__ void k_heat_bcc(int nbc, const int *__restrict__ r, const double *__restrict__ c, double *b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbc) b[r[i]] = c[i];
+}
+
+// one colour of a Gauss-Seidel sweep over the heat rows on lines
+// m1lo <= m1 < m1hi (gausmatt's update, heat3mg.c:1083-1099: the row's
+// products summed from -fre1 in slot order, sol0 -= sum / diagonal * t0koef)
+__global__ void __launch_bounds__(256) k_heat_gs(HeatGeo g, int col, int m1lo, int m1hi,
+                                                 const unsigned char *__restrict__ isbc,
+                                                 const float *__restrict__ kt, const float *__restrict__ rcp,
+                                                 double dt, double t0, const double *__restrict__ b, double *x) {
+  const int hy = (g.Y + 1) >> 1, nx = m1hi - m1lo;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)hy * nx * g.Z) return;
+  const int k = (int)(t % hy);
+  const long q = t / hy;
+  const int m1 = m1lo + (int)(q % nx), m3 = (int)(q / nx);
+  const int m2 = 2 * k + ((m1 + m3 + col) & 1);
+  if (m2 >= g.Y) return;
+  const long n = (long)m3 * g.XY + (long)m1 * g.Y + m2;
+  if (isbc[n]) return;
+  HeatRow h;
+  heat_row(g, kt, rcp[n], dt, m1, m2, m3, n, h);
+  const double xn = x[n];
+  double iv = -b[n];
+  iv += h.d * xn;
+#pragma unroll
+  for (int s = 0; s < 6; s++) iv += (-h.c[s]) * x[h.nb[s]];
+  x[n] = xn - iv / h.d * t0;
+}
+
+// boundary rows lo <= i < hi of one dependency depth: x_r = c + koef x_j
+// through the same update (tbonderr rows, heat3mg.c:869-893)
+__global__ void k_heat_bcpass(int lo, int hi, const int *__restrict__ r, const int *__restrict__ j,
+                              const double *__restrict__ negk, const double *__restrict__ b, double *x, double t0) {
+  const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= hi) return;
+  const int rr = r[i], jj = j[i];
+  const double xr = x[rr];
+  double iv = -b[rr];
+  iv += 1.0 * xr;
+  if (jj >= 0) iv += negk[i] * x[jj];
+  x[rr] = xr - iv / 1.0 * t0;
+}
+
+// residual fre0 = fre1 - A sol0 of the heat rows (heat3mg.c:1101-1117)
+__global__ void __launch_bounds__(256) k_heat_res(HeatGeo g, const unsigned char *__restrict__ isbc,
+                                                  const float *__restrict__ kt, const float *__restrict__ rcp,
+                                                  double dt, const double *__restrict__ b,
+                                                  const double *__restrict__ x, double *r) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= (long)g.XY * g.Z) return;
+  if (isbc[n]) return;
+  int m1, m2, m3;
+  decode(n, g.X, g.Y, g.XY, m1, m2, m3);
+  HeatRow h;
+  heat_row(g, kt, rcp[n], dt, m1, m2, m3, n, h);
+  double iv = b[n];
+  iv -= h.d * x[n];
+#pragma unroll
+  for (int s = 0; s < 6; s++) iv -= (-h.c[s]) * x[h.nb[s]];
+  r[n] = iv;
+}
+
+__global__ void k_heat_bcres(int nbc, const int *__restrict__ r, const int *__restrict__ j,
+                             const double *__restrict__ negk, const double *__restrict__ b,
+                             const double *__restrict__ x, double *res) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbc) return;
+  const int rr = r[i], jj = j[i];
+  double iv = b[rr];
+  iv -= 1.0 * x[rr];
+  if (jj >= 0) iv -= negk[i] * x[jj];
+  res[rr] = iv;
+}
+
+// trestrict (heat3mg.c:1122-1232): a coarse node without a boundary row
+// gathers the fine residuals of the 8 cells around it in scan order, weights
+// (1 -+ d) per axis multiplied x * y * z, and divides by the weight sum; the
+// coarse boundary rows get 0, the coarse solution 0
+__global__ void __launch_bounds__(256) k_heat_restrict(int Xf, int Yf, int Xc, int Yc, int Zc, CompTab t,
+                                                       const unsigned char *__restrict__ isbc_c,
+                                                       const double *__restrict__ rf, double *bc, double *xc) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XYc = Xc * Yc, XYf = Xf * Yf;
+  if (n >= (long)XYc * Zc) return;
+  xc[n] = 0.0;
+  if (isbc_c[n]) { bc[n] = 0.0; return; }
+  int cx, cy, cz;
+  decode(n, Xc, Yc, XYc, cx, cy, cz);
+  double acc = 0.0, ws = 0.0;
+  for (int a = t.ax[2].cbeg[cz]; a < t.ax[2].cbeg[cz + 1]; a++) {
+    const long oz = (long)t.ax[2].cfi[a] * XYf;
+    const double wz = t.ax[2].cw64[a];
+    for (int bq = t.ax[0].cbeg[cx]; bq < t.ax[0].cbeg[cx + 1]; bq++) {
+      const long ox = oz + (long)t.ax[0].cfi[bq] * Yf;
+      const double wx = t.ax[0].cw64[bq];
+      for (int e = t.ax[1].cbeg[cy]; e < t.ax[1].cbeg[cy + 1]; e++) {
+        const double swt = wx * t.ax[1].cw64[e] * wz;
+        acc += rf[ox + t.ax[1].cfi[e]] * swt;
+        ws += swt;
+      }
+    }
+  }
+  bc[n] = ws != 0.0 ? acc / ws : acc;
+}
+
+// tprolong (heat3mg.c:954-1050): a fine node without a boundary row adds
+// sol0_c * w * t1koef of the 8 coarse nodes of its cell, (z, x, y) order
+__global__ void __launch_bounds__(256) k_heat_prolong(int Xf, int Yf, int Zf, int Xc, int Yc, CompTab t,
+                                                      const unsigned char *__restrict__ isbc_f,
+                                                      const double *__restrict__ xc, double *xf, double t1,
+                                                      int ynreset) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XYf = Xf * Yf, XYc = Xc * Yc;
+  if (n >= (long)XYf * Zf) return;
+  if (isbc_f[n]) return;
+  int m1, m2, m3;
+  decode(n, Xf, Yf, XYf, m1, m2, m3);
+  const int lx = t.ax[0].lo[m1], ly = t.ax[1].lo[m2], lz = t.ax[2].lo[m3];
+  double v = ynreset ? 0.0 : xf[n];
+  for (int dz = 0; dz < 2; dz++) {
+    const double wz = t.ax[2].w64[2 * m3 + dz];
+    for (int dx = 0; dx < 2; dx++) {
+      const double wx = t.ax[0].w64[2 * m1 + dx];
+      for (int dy = 0; dy < 2; dy++) {
+        const double swt = wx * t.ax[1].w64[2 * m2 + dy] * wz;
+        v += xc[(long)(lz + dz) * XYc + (long)(lx + dx) * Yc + (ly + dy)] * swt * t1;
+      }
+    }
+  }
+  xf[n] = v;
+}
+
+// tk = (float) sol0 (heat3mg.c:507-512)
+__global__ void k_heat_tk(long n, const double *__restrict__ x, float *tk) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) tk[i] = (float)x[i];
+}
+
+// heat-equation error of the FP32 temperature (heaterr ynerr = 2,
+// heat3mg.c:251-287, times the sub-step, 540-548) squared and summed per
+// block, with the node count and the tk min / max: out[4 b + 0..3]
+__global__ void __launch_bounds__(256) k_heat_err(HeatGeo g, const unsigned char *__restrict__ isbc,
+                                                  const float *__restrict__ kt, const float *__restrict__ rcp,
+                                                  const float *__restrict__ tk, const float *__restrict__ tk0,
+                                                  const float *__restrict__ tk1, double dt, double *out) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  double sq = 0.0, cnt = 0.0, mn = 1e30, mx = -1e30;
+  if (n < (long)g.XY * g.Z) {
+    const float t3 = tk[n];
+    mn = mx = (double)t3;
+    if (!isbc[n]) {
+      int m1, m2, m3;
+      decode(n, g.X, g.Y, g.XY, m1, m2, m3);
+      const double rc = (double)rcp[n];
+      const long nb[6] = {(g.xper && m1 == 0) ? n + (long)(g.X - 2) * g.Y : n - g.Y, n + g.Y, n - 1, n + 1,
+                          n - g.XY, n + g.XY};
+      const int mm[3] = {m1, m2, m3};
+      const float k0 = kt[n];
+      double e = (double)(t3 - tk0[n]) / dt;
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        const float sm = kt[nb[2 * a]] + k0, sp = k0 + kt[nb[2 * a + 1]];
+        const double cm = (double)sm / 2.0 * g.kf[a][mm[a]] / g.dm[a][mm[a]] / rc;
+        e += (double)(t3 - tk[nb[2 * a]]) * cm;
+        const double cp = (double)sp / 2.0 * g.kf[a][mm[a]] / g.dp[a][mm[a]] / rc;
+        e -= (double)(tk[nb[2 * a + 1]] - t3) * cp;
+      }
+      e -= (double)tk1[n] / rc;
+      e *= dt;
+      sq = e * e;
+      cnt = 1.0;
+    }
+  }
+  heat_block_red(sq, cnt, mn, mx, out);
+}
+
+// boundary-row errors on the FP64 solution (tbonderr ynerr = 1,
+// heat3mg.c:854-867), squared and summed per block
+__global__ void __launch_bounds__(256) k_heat_bonderr(int nbc, const int *__restrict__ r, const int *__restrict__ j,
+                                                      const double *__restrict__ c, const double *__restrict__ kf,
+                                                      const double *__restrict__ x, double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double sq = 0.0, cnt = 0.0;
+  if (i < nbc) {
+    double lb = x[r[i]] - c[i];
+    if (j[i] >= 0) lb -= kf[i] * x[j[i]];
+    sq = lb * lb;
+    cnt = 1.0;
+  }
+  heat_block_red(sq, cnt, 0.0, 0.0, out);
+}
+
+}  // namespace stk

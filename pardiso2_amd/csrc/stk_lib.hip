@@ -21,6 +21,7 @@
 #include "stk_kernels.h"
 #include "stk_advect.h"
 #include "stk_compat.h"
+#include "stk_heat.h"
 #include "stk_markers.h"
 #include "stk_post.h"
 #include "stk_schur.h"
@@ -189,6 +190,39 @@ Geo<float> Level::geo<float>(double p2) const {
   return g;
 }
 
+// Temperature solve on levels 0..Lt of the hierarchy: per-level spacing
+// tables of the heat rows, boundary-row flags and the boundary rows by
+// dependency depth
+struct HeatLevel {
+  double *kf[3] = {}, *dm[3] = {}, *dp[3] = {};
+  unsigned char *isbc = nullptr;
+  int nbc = 0;
+  std::vector<int> bcoff;  // depth d: rows bcoff[d] .. bcoff[d+1]-1
+  int *bc_r = nullptr, *bc_j = nullptr;  // row and partner (-1: none), level-local nodes
+  double *bc_negk = nullptr, *bc_k = nullptr, *bc_c = nullptr;
+  bool split0 = false;  // x-periodic with an even line count: line 0 of a colour first
+  void release() {
+    for (int a = 0; a < 3; a++) { dfree(kf[a]); dfree(dm[a]); dfree(dp[a]); }
+    dfree(isbc); dfree(bc_r); dfree(bc_j); dfree(bc_negk); dfree(bc_k); dfree(bc_c);
+    nbc = 0;
+    bcoff.clear();
+    split0 = false;
+  }
+};
+
+struct HeatState {
+  HeatLevel lv[MAXLV];
+  int built = -1;                       // levels 0..built have their tables
+  float *ro = nullptr, *cp = nullptr, *kt = nullptr, *rcp = nullptr;  // every level
+  float *tk = nullptr, *tk0 = nullptr, *tk1 = nullptr;                // level 0
+  double *x = nullptr, *b = nullptr, *r = nullptr;                    // sol0, fre1, fre0 of every level
+  double *part = nullptr;               // block partials of the reductions
+  std::vector<double> hpart;
+  bool fields = false;
+  int Lt = -1;                          // levels of the current sub-step (-1: none started)
+  double dt = 1.0, t0 = 1.0, t1 = 1.0;
+};
+
 }  // namespace
 
 struct stk_ctx {
@@ -249,6 +283,13 @@ struct stk_ctx {
   // post-solve pass: block partials of the norms, stress and strain-rate planes (6 n0 each)
   double *post_part = nullptr, *sig = nullptr, *epsr = nullptr;
   int *adv_bins = nullptr;  // cell-search bins of the three axes (marker advection)
+  // temperature boundary equations in the reference's layout (bondm[nodenum8 + node],
+  // parallel/in3mg.c:369-385): tbondm[ntot] (0 = none), tbv1[2*(n+1)], tbn1[n+1] (node + 1)
+  std::vector<int> tbondm;
+  std::vector<float> tbv1;
+  std::vector<int> tbn1;
+  bool heat_dirty = true;
+  HeatState *heat = nullptr;
 };
 
 namespace {
@@ -1362,6 +1403,316 @@ static int scratch_base(unsigned char **out) {
   return STK_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Temperature solve (parallel/heat3mg.c) on levels 0..Lt of the hierarchy
+// ---------------------------------------------------------------------------
+namespace {
+
+void heat_free(HeatState *h) {
+  if (!h) return;
+  for (int l = 0; l < MAXLV; l++) h->lv[l].release();
+  dfree(h->ro); dfree(h->cp); dfree(h->kt); dfree(h->rcp);
+  dfree(h->tk); dfree(h->tk0); dfree(h->tk1);
+  dfree(h->x); dfree(h->b); dfree(h->r); dfree(h->part);
+  delete h;
+}
+
+int heat_alloc(stk_ctx *c) {
+  if (c->heat) return STK_OK;
+  HeatState *h = new HeatState();
+  const long nt = c->ntot, n0 = c->lv[0].n;
+  // every level's rows read all of its level-0 sized partial buffer at most
+  const long np = 4 * (2 * (long)blocks(n0, 256) + 2);
+  if (dalloc(&h->ro, nt) || dalloc(&h->cp, nt) || dalloc(&h->kt, nt) || dalloc(&h->rcp, nt) ||
+      dalloc(&h->tk, n0) || dalloc(&h->tk0, n0) || dalloc(&h->tk1, n0) || dalloc(&h->x, nt) ||
+      dalloc(&h->b, nt) || dalloc(&h->r, nt) || dalloc(&h->part, np)) {
+    heat_free(h);
+    return STK_ENOMEM;
+  }
+  h->hpart.assign(np, 0.0);
+  for (double *p : {h->x, h->b, h->r}) HIPCHK(hipMemsetAsync(p, 0, nt * sizeof(double), c->st));
+  for (float *p : {h->ro, h->cp, h->kt, h->rcp}) HIPCHK(hipMemsetAsync(p, 0, nt * sizeof(float), c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->heat = h;
+  return STK_OK;
+}
+
+HeatGeo heat_geo(const stk_ctx *c, int l) {
+  const Level &L = c->lv[l];
+  const HeatLevel &H = c->heat->lv[l];
+  HeatGeo g;
+  g.X = L.X; g.Y = L.Y; g.Z = L.Z; g.XY = L.XY;
+  g.xper = c->per[0];
+  for (int a = 0; a < 3; a++) { g.kf[a] = H.kf[a]; g.dm[a] = H.dm[a]; g.dp[a] = H.dp[a]; }
+  return g;
+}
+
+// Tables of levels 0..Lt: spacings of the heat rows (heat3mg.c:21-24 and the
+// denominators of 216-243), boundary flags, and the boundary rows sorted by
+// dependency depth (a row after its partner).  Every node without a
+// boundary row must have its six neighbours on the grid (x line 0 of an
+// x-periodic box wraps to line X-2): the reference reads outside the level
+// otherwise.
+int heat_build(stk_ctx *c, int Lt) {
+  HeatState *h = c->heat;
+  if (!c->heat_dirty && h->built >= Lt) return STK_OK;
+  for (int l = 0; l < MAXLV; l++) h->lv[l].release();
+  h->built = -1;
+  for (int l = 0; l <= Lt; l++) {
+    const Level &L = c->lv[l];
+    HeatLevel &H = h->lv[l];
+    const int dims[3] = {L.X, L.Y, L.Z};
+    for (int a = 0; a < 3; a++) {
+      const int n = dims[a];
+      const std::vector<double> &g = L.g[a];
+      std::vector<double> kf(n, 1.0), dm(n, 1.0), dp(n, 1.0);
+      for (int m = 1; m + 1 < n; m++) {
+        kf[m] = 2.0 / (g[m + 1] - g[m - 1]);
+        dm[m] = g[m] - g[m - 1];
+        dp[m] = g[m + 1] - g[m];
+      }
+      if (a == 0 && c->per[0]) {
+        kf[0] = 2.0 / (g[1] + g[n - 1] - g[n - 2]);
+        dm[0] = g[n - 1] - g[n - 2];
+        dp[0] = g[1] - g[0];
+      }
+      if (upload(&H.kf[a], kf) || upload(&H.dm[a], dm) || upload(&H.dp[a], dp)) return STK_ENOMEM;
+    }
+    std::vector<unsigned char> isbc(L.n, 0);
+    std::vector<int> rows;
+    for (int m3 = 0; m3 < L.Z; m3++)
+      for (int m1 = 0; m1 < L.X; m1++)
+        for (int m2 = 0; m2 < L.Y; m2++) {
+          const long n = (long)m3 * L.XY + (long)m1 * L.Y + m2;
+          if (c->tbondm[L.off + n]) {
+            isbc[n] = 1;
+            rows.push_back((int)n);
+            continue;
+          }
+          const bool x_ok = (m1 >= 1 || c->per[0]) && m1 <= L.X - 2;
+          if (!x_ok || m2 < 1 || m2 > L.Y - 2 || m3 < 1 || m3 > L.Z - 2) return STK_EINVAL;
+        }
+    // partner and depth of every boundary row
+    const int nb = (int)rows.size();
+    std::vector<int> part(nb, -1), depth(nb, 0);
+    for (int i = 0; i < nb; i++) {
+      const int b = c->tbondm[L.off + rows[i]];
+      if (c->tbn1[b]) {
+        const long j = (long)c->tbn1[b] - 1 - L.off;
+        if (j < 0 || j >= L.n) return STK_EINVAL;
+        part[i] = (int)j;
+      }
+    }
+    for (int i = 0; i < nb; i++) {
+      long r = rows[i];
+      int d = 0;
+      for (;;) {
+        const int b = c->tbondm[L.off + r];
+        if (!c->tbn1[b]) break;
+        const long j = (long)c->tbn1[b] - 1 - L.off;
+        if (!isbc[j]) break;
+        if (++d > 64) return STK_EINVAL;  // boundary rows that refer to each other in a cycle
+        r = j;
+      }
+      depth[i] = d;
+    }
+    std::vector<int> order(nb);
+    for (int i = 0; i < nb; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return depth[a] < depth[b]; });
+    std::vector<int> br(nb), bj(nb);
+    std::vector<double> bnk(nb), bk(nb), bcv(nb);
+    const int maxd = nb ? depth[order[nb - 1]] : -1;
+    H.bcoff.assign(maxd + 2, nb);
+    for (int d = 0, q = 0; d <= maxd; d++) {
+      while (q < nb && depth[order[q]] < d) q++;
+      H.bcoff[d] = q;
+    }
+    for (int q = 0; q < nb; q++) {
+      const int i = order[q];
+      const int b = c->tbondm[L.off + rows[i]];
+      br[q] = rows[i];
+      bj[q] = part[i];
+      bk[q] = (double)c->tbv1[2 * b + 1];
+      bnk[q] = -(double)c->tbv1[2 * b + 1];
+      bcv[q] = (double)c->tbv1[2 * b];
+    }
+    H.nbc = nb;
+    if (upload(&H.isbc, isbc) || upload(&H.bc_r, br) || upload(&H.bc_j, bj) || upload(&H.bc_negk, bnk) ||
+        upload(&H.bc_k, bk) || upload(&H.bc_c, bcv))
+      return STK_ENOMEM;
+    H.split0 = c->per[0] && (L.X % 2 == 0);
+  }
+  h->built = Lt;
+  c->heat_dirty = false;
+  return STK_OK;
+}
+
+// am Gauss-Seidel sweeps of level l: colour 0, colour 1, boundary rows by depth
+void heat_gs(stk_ctx *c, int l, int sweeps) {
+  HeatState *h = c->heat;
+  const Level &L = c->lv[l];
+  const HeatLevel &H = h->lv[l];
+  const HeatGeo g = heat_geo(c, l);
+  const float *kt = h->kt + L.off, *rcp = h->rcp + L.off;
+  const double *b = h->b + L.off;
+  double *x = h->x + L.off;
+  const int hy = (L.Y + 1) / 2;
+  for (int s = 0; s < sweeps; s++) {
+    for (int col = 0; col < 2; col++) {
+      const int cut = H.split0 ? 1 : 0;
+      if (cut)
+        hipLaunchKernelGGL(k_heat_gs, dim3(blocks((long)hy * L.Z, 256)), dim3(256), 0, c->st, g, col, 0, 1,
+                           H.isbc, kt, rcp, h->dt, h->t0, b, x);
+      hipLaunchKernelGGL(k_heat_gs, dim3(blocks((long)hy * (L.X - cut) * L.Z, 256)), dim3(256), 0, c->st, g, col,
+                         cut, L.X, H.isbc, kt, rcp, h->dt, h->t0, b, x);
+    }
+    for (size_t d = 0; d + 1 < H.bcoff.size(); d++) {
+      const int lo = H.bcoff[d], hi = H.bcoff[d + 1];
+      if (hi > lo)
+        hipLaunchKernelGGL(k_heat_bcpass, dim3(blocks(hi - lo, 256)), dim3(256), 0, c->st, lo, hi, H.bc_r, H.bc_j,
+                           H.bc_negk, b, x, h->t0);
+    }
+  }
+}
+
+void heat_res(stk_ctx *c, int l) {
+  HeatState *h = c->heat;
+  const Level &L = c->lv[l];
+  const HeatLevel &H = h->lv[l];
+  hipLaunchKernelGGL(k_heat_res, dim3(blocks(L.n, 256)), dim3(256), 0, c->st, heat_geo(c, l), H.isbc,
+                     h->kt + L.off, h->rcp + L.off, h->dt, h->b + L.off, h->x + L.off, h->r + L.off);
+  if (H.nbc)
+    hipLaunchKernelGGL(k_heat_bcres, dim3(blocks(H.nbc, 256)), dim3(256), 0, c->st, H.nbc, H.bc_r, H.bc_j,
+                       H.bc_negk, h->b + L.off, h->x + L.off, h->r + L.off);
+}
+
+CompTab comp_tab(const Level &C, int kind) {
+  CompTab t;
+  for (int a = 0; a < 3; a++) t.ax[a] = C.ax[kind][a].tab();
+  return t;
+}
+
+void heat_restrict(stk_ctx *c, int l) {
+  HeatState *h = c->heat;
+  const Level &F = c->lv[l - 1], &C = c->lv[l];
+  hipLaunchKernelGGL(k_heat_restrict, dim3(blocks(C.n, 256)), dim3(256), 0, c->st, F.X, F.Y, C.X, C.Y, C.Z,
+                     comp_tab(C, 0), h->lv[l].isbc, h->r + F.off, h->b + C.off, h->x + C.off);
+}
+
+void heat_prolong(stk_ctx *c, int l, int ynreset) {
+  HeatState *h = c->heat;
+  const Level &F = c->lv[l - 1], &C = c->lv[l];
+  hipLaunchKernelGGL(k_heat_prolong, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.X, F.Y, F.Z, C.X, C.Y,
+                     comp_tab(C, 0), h->lv[l - 1].isbc, h->x + C.off, h->x + F.off, h->t1, ynreset);
+}
+
+void heat_props(stk_ctx *c, int Lt) {
+  HeatState *h = c->heat;
+  for (int l = 1; l <= Lt; l++) {
+    const Level &F = c->lv[l - 1], &C = c->lv[l];
+    hipLaunchKernelGGL(k_heat_props, dim3(blocks(C.n, 256)), dim3(256), 0, c->st, F.X, F.Y, C.X, C.Y, C.Z,
+                       comp_tab(C, 3), h->ro + F.off, h->cp + F.off, h->kt + F.off, h->ro + C.off, h->cp + C.off,
+                       h->kt + C.off, h->rcp + C.off);
+  }
+}
+
+// tk0 = tk, sol0 = tk, level-0 right-hand side for a sub-step of dt
+void heat_start(stk_ctx *c, double dt) {
+  HeatState *h = c->heat;
+  const Level &F = c->lv[0];
+  const HeatLevel &H = h->lv[0];
+  h->dt = dt;
+  hipLaunchKernelGGL(k_heat_rhs, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.n, H.isbc, h->tk, h->tk0, h->tk1,
+                     h->rcp, dt, h->x, h->b);
+  if (H.nbc)
+    hipLaunchKernelGGL(k_heat_bcc, dim3(blocks(H.nbc, 256)), dim3(256), 0, c->st, H.nbc, H.bc_r, H.bc_c, h->b);
+}
+
+// sums of nb block partials (4 per block) in block order
+int heat_partials(stk_ctx *c, int nb, double out4[4]) {
+  HeatState *h = c->heat;
+  D2H(c, h->hpart.data(), h->part, (size_t)4 * nb * sizeof(double));
+  out4[0] = out4[1] = 0.0;
+  out4[2] = 1e30;
+  out4[3] = -1e30;
+  for (int i = 0; i < nb; i++) {
+    out4[0] += h->hpart[4 * i];
+    out4[1] += h->hpart[4 * i + 1];
+    out4[2] = std::min(out4[2], h->hpart[4 * i + 2]);
+    out4[3] = std::max(out4[3], h->hpart[4 * i + 3]);
+  }
+  return STK_OK;
+}
+
+// tk = (float) sol0, then the reference's errors (heat3mg.c:507-553)
+int heat_errs(stk_ctx *c, double *hs, double *bs, double *hn, double *bn, double *tmin, double *tmax) {
+  HeatState *h = c->heat;
+  const Level &F = c->lv[0];
+  const HeatLevel &H = h->lv[0];
+  hipLaunchKernelGGL(k_heat_tk, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.n, h->x, h->tk);
+  const int nb = (int)blocks(F.n, 256);
+  hipLaunchKernelGGL(k_heat_err, dim3(nb), dim3(256), 0, c->st, heat_geo(c, 0), H.isbc, h->kt, h->rcp, h->tk,
+                     h->tk0, h->tk1, h->dt, h->part);
+  double e[4];
+  int rc = heat_partials(c, nb, e);
+  if (rc) return rc;
+  *hs = e[1] > 0 ? std::sqrt(e[0] / e[1]) : 0.0;
+  if (hn) *hn = e[1];
+  if (tmin) *tmin = e[2];
+  if (tmax) *tmax = e[3];
+  double q[4] = {0, 0, 0, 0};
+  if (H.nbc) {
+    const int nq = (int)blocks(H.nbc, 256);
+    hipLaunchKernelGGL(k_heat_bonderr, dim3(nq), dim3(256), 0, c->st, H.nbc, H.bc_r, H.bc_j, H.bc_c, H.bc_k, h->x,
+                       h->part);
+    rc = heat_partials(c, nq, q);
+    if (rc) return rc;
+  }
+  *bs = q[1] > 0 ? std::sqrt(q[0] / q[1]) : 0.0;
+  if (bn) *bn = q[1];
+  return STK_OK;
+}
+
+// largest heating rate |tk1 / rho cp| over the level-0 heat rows
+int heat_rate(stk_ctx *c, double *rate) {
+  HeatState *h = c->heat;
+  const Level &F = c->lv[0];
+  const int nb = (int)blocks(F.n, 256);
+  hipLaunchKernelGGL(k_heat_rate, dim3(nb), dim3(256), 0, c->st, F.n, h->lv[0].isbc, h->tk1, h->rcp, h->part);
+  double e[4];
+  int rc = heat_partials(c, nb, e);
+  *rate = e[3] > 0 ? e[3] : 0.0;
+  return rc;
+}
+
+// one V-cycle (heat3mg.c:463-492): up, smooth pre[l] and restrict the
+// residual; down, smooth post[l] and prolong; level 0 is not smoothed after
+// the last prolongation; one level: pre[0] sweeps
+void heat_cycle(stk_ctx *c, const stk_heat_opts *o) {
+  const int Lt = o->levels;
+  if (Lt > 0) {
+    for (int l = 0; l < Lt; l++) {
+      heat_gs(c, l, o->pre[l]);
+      heat_res(c, l);
+      heat_restrict(c, l + 1);
+    }
+    for (int l = Lt; l > 0; l--) {
+      heat_gs(c, l, o->post[l]);
+      heat_prolong(c, l, 0);
+    }
+  } else {
+    heat_gs(c, 0, o->pre[0]);
+  }
+}
+
+int heat_ready(stk_ctx *c, int Lt) {
+  if (!c || !c->heat || !c->heat->fields) return STK_ESTATE;
+  if (Lt < 0 || Lt > c->L) return STK_EINVAL;
+  return heat_build(c, Lt);
+}
+
+}  // namespace
+
 extern "C" {
 
 int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *gx, const double *gy,
@@ -1416,6 +1767,9 @@ int stk_create(stk_ctx **out, int nx, int ny, int nz, int levels, const double *
   c->bondm.assign(4 * off, 0);
   c->bondv1.assign(2, 0.f);
   c->bondn1.assign(1, 0);
+  c->tbondm.assign(off, 0);
+  c->tbv1.assign(2, 0.f);
+  c->tbn1.assign(1, 0);
   c->kn = stk_knobs{0.10, 0.0, 0.10, 1.0, 1.0, 1.0, 1e4, 0, 0.0, 0.0, 0.0, 9e-3, 5e1, 0.0, 3e-3};
   {
     RtLock lk(g_rt_mu);
@@ -1518,6 +1872,8 @@ void stk_destroy(stk_ctx *c) {
   dfree(c->X); dfree(c->B); dfree(c->R); dfree(c->W2); dfree(c->T1); dfree(c->red); dfree(c->scal);
   dfree(c->wu); dfree(c->ww); dfree(c->wz);
   dfree(c->post_part); dfree(c->sig); dfree(c->epsr); dfree(c->adv_bins);
+  heat_free(c->heat);
+  c->heat = nullptr;
   dfree(c->kred); dfree(c->hcol); dfree(c->gin); dfree(c->gout);
   RtLock lk(g_rt_mu);
   if (c->hpin) hipHostFree(c->hpin);
@@ -1574,7 +1930,7 @@ void stk_default_opts(stk_solver_opts *o, int levels) {
 // boundary grammar line projected on every level (parallel/in3mg.c:299-375)
 int stk_bc_line(stk_ctx *c, int var, int m10, int m11, int m20, int m21, int m30, int m31, float cnst,
                 float koef, int dm1, int dm2, int dm3) {
-  if (!c || var < 0 || var > 3) return STK_EINVAL;
+  if (!c || var < 0 || var > 4) return STK_EINVAL;
   const Level &F = c->lv[0];
   long mgs = 1;
   for (int l = 0; l <= c->L; l++, mgs *= 2) {
@@ -1592,6 +1948,15 @@ int stk_bc_line(stk_ctx *c, int var, int m10, int m11, int m20, int m21, int m30
           if (std::abs(dm2) > 1) sy = -(L.Y - 1);
           if (std::abs(dm3) > 1) sz = -(L.Z - 1);
           long m6 = m4 + (long)sz * L.XY + (long)sx * L.Y + sy;
+          if (var == 4) {
+            // temperature rows (in3mg.c:369-385): partner node + 1
+            int nt = (int)c->tbn1.size();
+            c->tbondm[m4] = nt;
+            c->tbv1.push_back(cnst);
+            c->tbv1.push_back((dm1 || dm2 || dm3) ? koef : 0.f);
+            c->tbn1.push_back((dm1 || dm2 || dm3) ? (int)(m6 + 1) : 0);
+            continue;
+          }
           int nb = (int)c->bondn1.size();
           c->bondm[m4 * 4 + var] = nb;
           c->bondv1.push_back(cnst);
@@ -1603,6 +1968,10 @@ int stk_bc_line(stk_ctx *c, int var, int m10, int m11, int m20, int m21, int m30
             c->bondn1.push_back(0);
           }
         }
+  }
+  if (var == 4) {
+    c->heat_dirty = true;
+    return STK_OK;
   }
   c->prepared = false;
   c->rows_dirty = true;
@@ -3328,6 +3697,177 @@ void vpiterate(int m0, int f0) {
   stk_default_opts(&o, c->L);
   stk_solve(c, &o, nullptr);
   if (g_arr_set && !stk_get_solution(c, g_arr.sol0)) (void)vpiterate_outputs(c);
+}
+
+// ---- temperature step (parallel/heat3mg.c) --------------------------------
+
+void stk_heat_default_opts(stk_heat_opts *o) {
+  if (!o) return;
+  memset(o, 0, sizeof *o);
+  o->levels = 0;
+  o->cycles = 1;
+  o->pre[0] = 10;
+  o->heatmin = 1e-4;
+  o->t0koef = 1.0;
+  o->t1koef = 1.0;
+  o->maxtkstep = 25.0;
+  o->maxit = 2500;
+}
+
+int stk_heat_set_fields(stk_ctx *c, const float *ro, const float *cp, const float *kt, const float *tk,
+                        const float *ht) {
+  if (!c || !ro || !cp || !kt || !tk || !ht) return STK_EINVAL;
+  const long n0 = c->lv[0].n;
+  for (long i = 0; i < n0; i++)
+    if (!std::isfinite(ro[i]) || !std::isfinite(cp[i]) || !std::isfinite(kt[i]) || !std::isfinite(tk[i]) ||
+        !std::isfinite(ht[i]))
+      return STK_EINVAL;
+  int rc = heat_alloc(c);
+  if (rc) return rc;
+  HeatState *h = c->heat;
+  H2D(c, h->ro, ro, n0 * 4);
+  H2D(c, h->cp, cp, n0 * 4);
+  H2D(c, h->kt, kt, n0 * 4);
+  H2D(c, h->tk, tk, n0 * 4);
+  H2D(c, h->tk0, tk, n0 * 4);
+  H2D(c, h->tk1, ht, n0 * 4);
+  hipLaunchKernelGGL(k_heat_rcp, dim3(blocks(n0, 256)), dim3(256), 0, c->st, n0, h->ro, h->cp, h->rcp);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->st));
+  h->fields = true;
+  h->Lt = -1;
+  return STK_OK;
+}
+
+int stk_heat_get_fields(stk_ctx *c, float *ro, float *cp, float *kt, float *tk, float *tk0) {
+  if (!c || !c->heat) return STK_ESTATE;
+  HeatState *h = c->heat;
+  const long n0 = c->lv[0].n;
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (ro) D2H(c, ro, h->ro, c->ntot * 4);
+  if (cp) D2H(c, cp, h->cp, c->ntot * 4);
+  if (kt) D2H(c, kt, h->kt, c->ntot * 4);
+  if (tk) D2H(c, tk, h->tk, n0 * 4);
+  if (tk0) D2H(c, tk0, h->tk0, n0 * 4);
+  return STK_OK;
+}
+
+int stk_heat_prepare(stk_ctx *c, int levels, double timestep, double t0koef, double t1koef) {
+  int rc = heat_ready(c, levels);
+  if (rc) return rc;
+  if (!(timestep > 0) || !std::isfinite(t0koef) || !std::isfinite(t1koef)) return STK_EINVAL;
+  HeatState *h = c->heat;
+  h->t0 = t0koef;
+  h->t1 = t1koef;
+  heat_props(c, levels);
+  heat_start(c, timestep);
+  h->Lt = levels;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->st));
+  return STK_OK;
+}
+
+int stk_heat_gausmatt(stk_ctx *c, int level, int sweeps) {
+  if (!c || !c->heat || c->heat->Lt < 0) return STK_ESTATE;
+  if (level < 0 || level > c->heat->Lt || sweeps < 0) return STK_EINVAL;
+  heat_gs(c, level, sweeps);
+  heat_res(c, level);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->st));
+  return STK_OK;
+}
+
+int stk_heat_trestrict(stk_ctx *c, int level) {
+  if (!c || !c->heat || c->heat->Lt < 0) return STK_ESTATE;
+  if (level < 1 || level > c->heat->Lt) return STK_EINVAL;
+  heat_restrict(c, level);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->st));
+  return STK_OK;
+}
+
+int stk_heat_tprolong(stk_ctx *c, int level, int ynreset) {
+  if (!c || !c->heat || c->heat->Lt < 0) return STK_ESTATE;
+  if (level < 1 || level > c->heat->Lt) return STK_EINVAL;
+  heat_prolong(c, level, ynreset ? 1 : 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->st));
+  return STK_OK;
+}
+
+int stk_heat_err(stk_ctx *c, double *heatsum, double *bondsum) {
+  if (!c || !c->heat || c->heat->Lt < 0) return STK_ESTATE;
+  double hs = 0, bs = 0;
+  int rc = heat_errs(c, &hs, &bs, nullptr, nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  HIPCHK(hipGetLastError());
+  if (heatsum) *heatsum = hs;
+  if (bondsum) *bondsum = bs;
+  return STK_OK;
+}
+
+int stk_heat_vec(stk_ctx *c, int which, int level, double *host, int dir) {
+  if (!c || !c->heat) return STK_ESTATE;
+  if (!host || level < 0 || level > c->L || which < 0 || which > 2) return STK_EINVAL;
+  HeatState *h = c->heat;
+  const Level &L = c->lv[level];
+  double *d = (which == STK_SOL0 ? h->x : which == STK_FRE1 ? h->b : h->r) + L.off;
+  if (dir) H2D(c, d, host, L.n * sizeof(double));
+  else D2H(c, host, d, L.n * sizeof(double));
+  return STK_OK;
+}
+
+int stk_heat_titerate(stk_ctx *c, double timestep, const stk_heat_opts *o, stk_heat_stats *st) {
+  if (!o) return STK_EINVAL;
+  int rc = heat_ready(c, o->levels);
+  if (rc) return rc;
+  if (!std::isfinite(timestep) || timestep < 0 || o->cycles < 1 || o->maxit < 1 || !(o->heatmin >= 0) ||
+      !std::isfinite(o->t0koef) || !std::isfinite(o->t1koef) || !(o->maxtkstep >= 0))
+    return STK_EINVAL;
+  for (int l = 0; l <= o->levels; l++)
+    if (o->pre[l] < 0 || o->post[l] < 0) return STK_EINVAL;
+  const auto t_start = std::chrono::steady_clock::now();
+  HeatState *h = c->heat;
+  h->t0 = o->t0koef;
+  h->t1 = o->t1koef;
+  stk_heat_stats S;
+  memset(&S, 0, sizeof S);
+  S.converged = 1;
+  double rate = 0;
+  rc = heat_rate(c, &rate);
+  if (rc) return rc;
+  S.tkstep = 1e+30;
+  if (rate) S.tkstep = o->maxtkstep / rate;
+  if (timestep != 0 && S.tkstep != 0) {
+    heat_props(c, o->levels);
+    h->Lt = o->levels;
+    double tsum = 0;
+    const double tsum0 = timestep;
+    do {
+      double dt = std::min(tsum0, S.tkstep);
+      if (dt > tsum0 - tsum) dt = tsum0 - tsum;
+      heat_start(c, dt);
+      int it = 0;
+      double hs = 0;
+      do {
+        for (int k = 0; k < o->cycles; k++) heat_cycle(c, o);
+        HIPCHK(hipGetLastError());
+        rc = heat_errs(c, &hs, &S.bondsum, &S.heatnum, &S.bondnum, &S.tkmin, &S.tkmax);
+        if (rc) return rc;
+        if (!std::isfinite(hs)) return STK_ENAN;
+        it++;
+      } while (hs > o->heatmin && it < o->maxit);
+      if (hs > o->heatmin) S.converged = 0;
+      S.heatsum = hs;
+      S.iterations += it;
+      S.substeps++;
+      tsum += dt;
+    } while (tsum < tsum0);
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  S.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  if (st) *st = S;
+  return S.converged ? STK_OK : STK_ENOCONV;
 }
 
 }  // extern "C"

@@ -1,0 +1,198 @@
+"""The temperature step on the device against the oracle (parallel/heat3mg.c:
+heaterr 6-290, titerate 292-616, tprolong 954-1050, gausmatt 1057-1120,
+trestrict 1122-1232, rocprestrict 1235-1330).
+
+The device rebuilds every row from kt and rho cp in the reference's
+operation order, so the property restriction, the right-hand side, the
+sweeps (the oracle's coloured order), the residuals and both transfers are
+compared bit for bit.  The RMS errors are block-reduced on the device (a
+different summation order): 1e-12 relative.  Whole steps then agree to the
+bit in the temperature and in the iteration count.
+
+Boxes: setups.heat_box (random conductivity, fixed T on the z faces, zero
+flux or x periodic on the sides), 33^3 walled and x-periodic (levels
+33/19/12/8: on the even levels 2 and 3 the periodic rows of x line 0 read
+line X-2 of their own colour, and run before it).  Line counts of the form
+2^k + 1 keep the reference's coarsening rule consistent (an even count
+puts fine lines outside the coarse cell the rule assigns them).
+"""
+import numpy as np
+import pytest
+
+from pardiso2_amd import _lib as S
+from pardiso2_amd import setups
+
+pytestmark = pytest.mark.gpu
+
+SOL0, FRE1, FRE0 = 0, 1, 2
+
+
+def _pair(oracle_mod, n=33, Lt=3, xper=False, timestep=1e4, seed=4):
+    hp = setups.heat_box(n, max(Lt, 1), seed=seed, xperiodic=xper, timestep=timestep)
+    o = oracle_mod.Oracle(hp.gx, hp.gy, hp.gz, Lt, periodic=hp.periodic)
+    h = oracle_mod.Heat(o, Lt)
+    h.bc_lines(hp.bc)
+    h.set_fields(hp.ro, hp.cp, hp.kt, hp.tk, hp.tk1)
+    h.set(hp.timestep)
+    d = S.from_heat_problem(hp)
+    return hp, o, h, d
+
+
+def _lvl(o, l):
+    i = o.info[l]
+    return i[5], i[5] + i[4]
+
+
+@pytest.mark.parametrize("xper,n", [(False, 33), (True, 33)])
+def test_property_restriction_and_rhs_bit_identical(oracle_mod, xper, n):
+    hp, o, h, d = _pair(oracle_mod, n=n, xper=xper)
+    Lt = 3
+    for l in range(1, Lt + 1):
+        h.rocprestrict(l)
+    h.assemble()
+    d.heat_prepare(Lt, hp.timestep)
+    f = d.heat_fields()
+    nt = _lvl(o, Lt)[1]
+    for name in ("ro", "cp", "kt"):
+        np.testing.assert_array_equal(f[name][:nt], h.arr(name)[:nt], err_msg=name)
+    np.testing.assert_array_equal(d.heat_vec_get(FRE1, 0), h.arr("fre1")[: hp.n ** 3])
+    np.testing.assert_array_equal(d.heat_vec_get(SOL0, 0), hp.tk.astype(np.float32).astype(np.float64))
+
+
+@pytest.mark.parametrize("xper,n", [(False, 33), (True, 33)])
+def test_sweeps_transfers_bit_identical(oracle_mod, xper, n):
+    hp, o, h, d = _pair(oracle_mod, n=n, xper=xper)
+    Lt = 3
+    N = hp.n ** 3
+    for l in range(1, Lt + 1):
+        h.rocprestrict(l)
+    h.assemble()
+    h.arr("sol0")[:N] = hp.tk.astype(np.float32)
+    d.heat_prepare(Lt, hp.timestep)
+    # level 0: sweeps + residual
+    h.gausmatt(3, 0, colored=True)
+    d.heat_gausmatt(0, 3)
+    np.testing.assert_array_equal(d.heat_vec_get(SOL0, 0), h.arr("sol0")[:N])
+    np.testing.assert_array_equal(d.heat_vec_get(FRE0, 0), h.arr("fre0")[:N])
+    # down the levels: restrict, sweep the coarse level
+    for l in range(1, Lt + 1):
+        a, e = _lvl(o, l)
+        h.trestrict(l)
+        d.heat_trestrict(l)
+        np.testing.assert_array_equal(d.heat_vec_get(FRE1, l), h.arr("fre1")[a:e], err_msg=f"fre1 {l}")
+        assert not d.heat_vec_get(SOL0, l).any()
+        h.gausmatt(2, l, colored=True)
+        d.heat_gausmatt(l, 2)
+        np.testing.assert_array_equal(d.heat_vec_get(SOL0, l), h.arr("sol0")[a:e], err_msg=f"sol0 {l}")
+        np.testing.assert_array_equal(d.heat_vec_get(FRE0, l), h.arr("fre0")[a:e], err_msg=f"fre0 {l}")
+    # up: prolong each correction
+    for l in range(Lt, 0, -1):
+        a, e = _lvl(o, l - 1)
+        h.tprolong(l, 0)
+        d.heat_tprolong(l, 0)
+        np.testing.assert_array_equal(d.heat_vec_get(SOL0, l - 1), h.arr("sol0")[a:e], err_msg=f"prolong {l}")
+    # errors of the FP32 temperature tk = (float) sol0 (the device's err sets it, as the reference's loop does)
+    h.arr("tk")[:] = h.arr("sol0")[:N].astype(np.float32)
+    hs, bs = h.err()
+    dhs, dbs = d.heat_err()
+    np.testing.assert_array_equal(d.heat_fields()["tk"], h.arr("tk"))
+    np.testing.assert_allclose(dhs, hs, rtol=1e-12)
+    np.testing.assert_allclose(dbs, bs, rtol=1e-12, atol=1e-300)
+
+
+def test_prolong_with_reset_and_random_coarse_field(oracle_mod):
+    hp, o, h, d = _pair(oracle_mod)
+    h.assemble()
+    d.heat_prepare(3, hp.timestep)
+    rng = np.random.default_rng(7)
+    a, e = _lvl(o, 1)
+    xc = rng.standard_normal(e - a)
+    N = hp.n ** 3
+    x0 = rng.standard_normal(N)
+    h.arr("sol0")[a:e] = xc
+    h.arr("sol0")[:N] = x0
+    d.heat_vec_set(SOL0, 1, xc)
+    d.heat_vec_set(SOL0, 0, x0)
+    h.tprolong(1, 1)
+    d.heat_tprolong(1, 1)
+    np.testing.assert_array_equal(d.heat_vec_get(SOL0, 0), h.arr("sol0")[:N])
+
+
+@pytest.mark.parametrize("xper,n,Lt", [(False, 33, 3), (True, 33, 3), (False, 33, 0)])
+def test_titerate_matches_oracle(oracle_mod, xper, n, Lt):
+    hp, o, h, d = _pair(oracle_mod, n=n, Lt=Lt, xper=xper)
+    pre = [3] * (Lt + 1) if Lt else [10]
+    post = [0] + [3] * Lt
+    heatmin = 2e-5 if Lt else 5e-3
+    ref = h.titerate(hp.timestep, 25.0, pre, post, heatmin=heatmin, maxit=400)
+    st = d.heat_titerate(hp.timestep, levels=Lt, pre=pre, post=post, heatmin=heatmin, maxit=400)
+    assert ref["converged"] == 1 and st.converged == 1
+    assert (st.substeps, st.iterations) == (ref["substeps"], ref["iterations"])
+    np.testing.assert_array_equal(d.heat_fields()["tk"], h.arr("tk"))
+    np.testing.assert_allclose(st.heatsum, ref["heatsum"], rtol=1e-12)
+    np.testing.assert_allclose(st.bondsum, ref["bondsum"], rtol=1e-12, atol=1e-300)
+    assert (st.tkmin, st.tkmax) == (ref["tkmin"], ref["tkmax"])
+    assert st.heatsum <= heatmin
+
+
+def test_titerate_substeps(oracle_mod):
+    # a strong source: the rate limit maxtkstep / max|tk1 / rho cp| cuts the step into sub-steps
+    hp, o, h, d = _pair(oracle_mod, n=17, Lt=2)
+    tk1 = (hp.tk1 * 2e4).astype(np.float32)
+    h.set_fields(hp.ro, hp.cp, hp.kt, hp.tk, tk1)
+    d.heat_set_fields(hp.ro, hp.cp, hp.kt, hp.tk, tk1)
+    rate = np.max(np.abs(tk1 / np.maximum(hp.ro.astype(np.float32) * hp.cp.astype(np.float32), 2e6)))
+    maxtk = float(rate * hp.timestep / 2.5)
+    ref = h.titerate(hp.timestep, maxtk, [3, 3, 3], [0, 3, 3], heatmin=1e-5, maxit=100)
+    st = d.heat_titerate(hp.timestep, levels=2, pre=[3, 3, 3], post=[0, 3, 3], heatmin=1e-5, maxit=100,
+                         maxtkstep=maxtk)
+    assert ref["substeps"] == 3 and st.substeps == 3
+    assert st.iterations == ref["iterations"]
+    np.testing.assert_allclose(st.tkstep, ref["tkstep"], rtol=1e-15)
+    np.testing.assert_array_equal(d.heat_fields()["tk"], h.arr("tk"))
+
+
+def test_zero_step_and_errors(oracle_mod):
+    hp, o, h, d = _pair(oracle_mod, n=17, Lt=2)
+    st = d.heat_titerate(0.0, levels=2, pre=[3, 3, 3], post=[0, 3, 3])
+    assert st.substeps == 0
+    np.testing.assert_array_equal(d.heat_fields()["tk"], hp.tk.astype(np.float32))
+    with pytest.raises(S.StkError):
+        d.heat_titerate(hp.timestep, levels=9)
+    # a face node without a T row
+    g = setups.uniform_grid(17, 1.0)
+    s = S.Stokes(g, g, g, 2)
+    s.bc_lines(setups.parse_bc_lines(" T 0 x 0 y 0 0 0 0\n T 0 x 0 y z z 1 0\n", 17, 17, 17))
+    s.heat_set_fields(hp.ro, hp.cp, hp.kt, hp.tk, hp.tk1)
+    with pytest.raises(S.StkError, match="invalid"):
+        s.heat_titerate(hp.timestep, levels=2)
+    # nothing set
+    s2 = S.Stokes(g, g, g, 2)
+    with pytest.raises(S.StkError):
+        s2.heat_titerate(hp.timestep)
+
+
+def test_titerate_large_box_converges(oracle_mod):
+    # 129^3, 6 levels (diffusion number ~150 per cell: the error of the FP32 temperature
+    # bottoms out near 1e-5, heatmin sits above it); the oracle evaluates the reference's
+    # error on the device's temperature
+    n, Lt = 129, 5
+    hp = setups.heat_box(n, Lt)
+    d = S.from_heat_problem(hp)
+    pre = [3] * (Lt + 1)
+    post = [0] + [3] * Lt
+    st = d.heat_titerate(hp.timestep, levels=Lt, pre=pre, post=post, heatmin=1e-4, maxit=100)
+    assert st.converged, st.as_dict()
+    o = oracle_mod.Oracle(hp.gx, hp.gy, hp.gz, Lt)
+    h = oracle_mod.Heat(o, Lt)
+    h.bc_lines(hp.bc)
+    f = d.heat_fields()
+    h.set_fields(hp.ro, hp.cp, hp.kt, hp.tk, hp.tk1)
+    h.arr("tk")[:] = f["tk"]
+    h.arr("tk0")[:] = f["tk0"]
+    h.set(hp.timestep)
+    h.assemble()
+    h.arr("sol0")[: n ** 3] = d.heat_vec_get(SOL0, 0)
+    hs, bs = h.err()
+    np.testing.assert_allclose(st.heatsum, hs, rtol=1e-10)
+    assert hs <= 1e-4

@@ -172,3 +172,31 @@ def test_multigrid_step_converges_to_direct_solution(oracle_mod, colored, xper):
     N = n ** 3
     x = _direct(h, N)
     np.testing.assert_allclose(h.arr("tk"), x.astype(np.float32), rtol=0, atol=1e-5)
+
+
+def test_titerate_substep_count(oracle_mod):
+    # the sub-step limit maxtkstep / max|tk1 / rho cp| (heat3mg.c:331-382, 401-405): a step 2.5 limits
+    # long is cut into 0.4 + 0.4 + 0.2; no source: one sub-step; a zero step: nothing done
+    hp = setups.heat_box(17, 2)
+    g = hp.gx
+
+    def run(tk1, maxtk, timestep):
+        o = oracle_mod.Oracle(g, g.copy(), g.copy(), 2)
+        h = oracle_mod.Heat(o, 2)
+        h.bc_lines(hp.bc)
+        h.set_fields(hp.ro, hp.cp, hp.kt, hp.tk, tk1)
+        h.set(hp.timestep)
+        return h, h.titerate(timestep, maxtk, [3, 3, 3], [0, 3, 3], heatmin=1e-5, maxit=100)
+
+    tk1 = (hp.tk1 * 2e4).astype(np.float32)
+    heat_rows = run(tk1, 25.0, 0.0)[0].arr("tbondm")[: 17 ** 3] == 0
+    rcp = np.maximum(hp.ro.astype(np.float32) * hp.cp.astype(np.float32), np.float32(2e6))
+    rate = np.max(np.abs(tk1 / rcp.astype(np.float64))[heat_rows])
+    h, r = run(tk1, float(rate * hp.timestep / 2.5), hp.timestep)
+    assert r["substeps"] == 3 and r["converged"]
+    np.testing.assert_allclose(r["tkstep"], hp.timestep / 2.5, rtol=1e-6)
+    h, r = run(np.zeros_like(tk1), 25.0, hp.timestep)
+    assert r["substeps"] == 1 and r["tkstep"] == 1e30
+    h, r = run(tk1, 25.0, 0.0)
+    assert r["substeps"] == 0
+    np.testing.assert_array_equal(h.arr("tk"), hp.tk.astype(np.float32))
