@@ -13,6 +13,11 @@ V-cycle / BFBt block preconditioner until ||W r||/||W b|| <= 1e-8.
 single-solve problems; --workload batched and --workload served the
 64-instance batch and the Poisson request stream.
 
+The line also carries one implicit temperature step (titerate, parallel/heat3mg.c) of the
+257^3 heat box (random conductivity, fixed temperatures on the z faces, 6 levels, V(3,3),
+to heatmin 1e-4): its time, the roofline of its colour pass, and the oracle's step at 65^3 on
+the host cores beside the device's on that box.
+
 The line also carries the roofline of the finest-level FP32 smoother
 instance the timed solve runs (64 algorithmic bytes per node and sweep,
 HIP events on the library stream, HBM bytes from the committed rocprofv3
@@ -40,6 +45,10 @@ RESIDUAL_BYTES_PER_NODE = 112.0  # FP64 residual: x(32) + b(32) + 4 viscosities(
 
 
 KERNELS = {"smooth32": 0, "smooth64": 1, "apply64": 2, "res32": 3, "res64": 4, "copy": 5}
+HEAT_KERNELS = {"heatgs": 0, "heatres": 2, "heatcopy": 3}
+# one colour pass of the temperature sweep (k_heat_gs over the finest level): x (8) and kt (4) of every
+# node read, b (8), rho cp (4) and the boundary flag (1) of the colour's half read, x (8) of that half written
+HEAT_GS_BYTES_PER_NODE = 12.0 + 21.0 / 2
 
 
 def log(msg):
@@ -242,6 +251,94 @@ def cpu_sample(pb, budget_s, device=True):
         return {"value": None, "unit": "solves/s", "cores": None, "kind": "port", "sample": f"failed: {e}", **HOST}
 
 
+def heat_levels(n: int) -> int:
+    """Coarse levels of the temperature V-cycle: the hierarchy down to 12 lines per axis (the
+    reference's cycle diverges on these boxes with its 8-line level below the 12-line one)."""
+    from pardiso2_amd import setups
+    L = setups.coarsest_levels(n)
+    return L - 1 if L >= 2 else L
+
+
+def heat_setup(n: int):
+    from pardiso2_amd import _lib as S, setups
+    Lt = heat_levels(n)
+    hp = setups.heat_box(n, setups.coarsest_levels(n))
+    return hp, Lt, S.from_heat_problem(hp)
+
+
+def heat_opts(Lt: int):
+    return dict(levels=Lt, pre=[3] * (Lt + 1), post=[0] + [3] * Lt, heatmin=1e-4, maxit=400)
+
+
+def bench_heat(args):
+    """One implicit temperature step (titerate) of the heat box on the device, the roofline of its
+    colour pass, and the oracle's step on the 65^3 box on the host cores beside the device's."""
+    hp, Lt, ctx = heat_setup(args.heat_n)
+    n0 = ctx.n0
+    ko = heat_opts(Lt)
+
+    def step(c, h):
+        c.heat_set_fields(h.ro, h.cp, h.kt, h.tk, h.tk1)
+        return c.heat_titerate(h.timestep, **ko)
+
+    st = step(ctx, hp)
+    times, its = [], []
+    for _ in range(args.heat_steps):
+        st = step(ctx, hp)
+        times.append(st.seconds)
+        its.append(st.iterations)
+    ms_gs = ctx.heat_time(0, 0, reps=50)
+    gbs = HEAT_GS_BYTES_PER_NODE * n0 / (ms_gs * 1e-3) / 1e9
+    ms_sweep = ctx.heat_time(1, 0, reps=20)
+    traffic = None
+    tf = os.path.join(HERE, "profiles", f"heat{args.heat_n}_colour_pass_pmc.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            tj = json.load(f)
+        if tj.get("nodes") == n0:
+            traffic = tj.get("kernels", {}).get("heatgs", {}).get("hbm_bytes_per_launch")
+    log(f"temperature step {args.heat_n}^3: {np.median(times) * 1e3:.1f} ms, {its} iterations, heatsum "
+        f"{st.heatsum:.2e}; colour pass {ms_gs:.4f} ms ({gbs:.0f} GB/s)")
+    ctx.close()
+    out = {"workload": f"heat box {args.heat_n}^3, {Lt + 1} levels, V(3,3), heatmin 1e-4, timestep 1e4 s",
+           "ms_per_step": float(np.median(times) * 1e3), "steps_per_s": float(1.0 / np.median(times)),
+           "iterations": its, "heatsum": st.heatsum, "bondsum": st.bondsum, "substeps": st.substeps,
+           "roofline": {"bound": "hbm", "kernel": "k_heat_gs (one colour, finest level)", "achieved": gbs,
+                        "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS, "traffic": traffic,
+                        "ms_per_launch": ms_gs, "bytes_per_launch": HEAT_GS_BYTES_PER_NODE * n0},
+           "sweep_ms": ms_sweep, "cpu_baseline": None}
+    if not args.no_cpu:
+        try:
+            n = 65
+            hp65, Lt65, c65 = heat_setup(n)
+            k65 = heat_opts(Lt65)
+            c65.heat_set_fields(hp65.ro, hp65.cp, hp65.kt, hp65.tk, hp65.tk1)
+            d65 = c65.heat_titerate(hp65.timestep, **k65)
+            c65.close()
+            O = load_oracle()
+            o = O.Oracle(hp65.gx, hp65.gy, hp65.gz, hp65.levels)
+            h = O.Heat(o, Lt65)
+            h.bc_lines(hp65.bc)
+            ts = []
+            for _ in range(3):
+                h.set_fields(hp65.ro, hp65.cp, hp65.kt, hp65.tk, hp65.tk1)
+                h.set(hp65.timestep)
+                t0 = time.perf_counter()
+                r = h.titerate(hp65.timestep, 25.0, k65["pre"], k65["post"], heatmin=1e-4, maxit=400)
+                ts.append(time.perf_counter() - t0)
+            med = float(np.median(ts))
+            out["cpu_baseline"] = {"value": 1.0 / med, "unit": "temperature steps/s", "cores": O.threads(),
+                                   "kind": "port", "sample": f"oracle titerate, heat box 65^3, {Lt65 + 1} levels, "
+                                   f"same cycle and heatmin, {r['iterations']} iterations, median of 3",
+                                   "device_same_problem_steps_per_s": 1.0 / d65.seconds,
+                                   "device_iterations": d65.iterations}
+            log(f"temperature step cpu baseline 65^3: {med * 1e3:.1f} ms ({r['iterations']} iterations), "
+                f"device {d65.seconds * 1e3:.1f} ms ({d65.iterations})")
+        except Exception as e:
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    return out
+
+
 def bench_batched(args):
     """64 independent instances, sizes drawn from {33,49,65,97}^3, 1-4 seeded
     inclusions each (contrast 1e0..1e6); a step prepares and solves all of
@@ -373,7 +470,9 @@ def main():
     ap.add_argument("--banded-n", type=int, default=17,
                     help="grid of the banded direct CPU solve (gausmat3) beside the CPU baseline; 0: skip")
     ap.add_argument("--no-fp64-ref", action="store_true")
-    ap.add_argument("--kernel", choices=sorted(KERNELS), default=None,
+    ap.add_argument("--heat-n", type=int, default=257, help="temperature step box (0 skips it)")
+    ap.add_argument("--heat-steps", type=int, default=3)
+    ap.add_argument("--kernel", choices=sorted(KERNELS) + sorted(HEAT_KERNELS), default=None,
                     help="only launch one finest-level kernel --reps times (for counter runs) and exit")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--instances", type=int, default=64, help="batched workload: number of instances")
@@ -395,6 +494,12 @@ def main():
         return bench_served(args)
 
     from pardiso2_amd import _lib as S
+    if args.kernel in HEAT_KERNELS:  # counter and trace runs of the temperature kernels (heat box, --heat-n)
+        hp, Lt, hctx = heat_setup(args.heat_n)
+        hctx.heat_titerate(hp.timestep, **heat_opts(Lt))
+        ms = hctx.heat_time(HEAT_KERNELS[args.kernel], 0, reps=args.reps)
+        log(f"kernel {args.kernel}: {args.reps} launches, {ms:.4f} ms each, {hctx.n0} nodes")
+        return
     pb = make_problem(args.workload, args.n, args.seed)
     L = pb.levels
     pre, post = sweeps_for(L)
@@ -476,6 +581,14 @@ def main():
         log(f"fp64 reference solve: {s64.iterations} iterations, relres {s64.relres:.3e}, {s64.seconds:.2f} s")
         err = solution_error(ctx, x_dev)
 
+    heat = None
+    if args.heat_n > 0:
+        try:
+            heat = bench_heat(args)
+        except Exception as e:
+            log(f"temperature step failed: {e}")
+            heat = {"error": str(e)}
+
     cpu, orel, banded = None, None, None
     log(f"smoother {ms_sm:.4f} ms ({gbs_sm:.0f} GB/s), fp64 residual {ms_res:.4f} ms ({gbs_res:.0f} GB/s)")
     if not args.no_cpu:
@@ -544,6 +657,7 @@ def main():
         "cpu_baseline": cpu,
         "cpu_banded_direct": banded,
         "marker_advection": adv,
+        "temperature_step": heat,
     }
     print(json.dumps(out), flush=True)
 

@@ -299,6 +299,12 @@ int stk_heat_err(stk_ctx *ctx, double *heatsum, double *bondsum);
 /* sol0 / fre1 / fre0 (STK_SOL0, STK_FRE1, STK_FRE0) of the heat solve on
  * level l (mgn[l] doubles): dir 0 device to host, 1 host to device. */
 int stk_heat_vec(stk_ctx *ctx, int which, int level, double *host, int dir);
+/* Average device time (ms) of one launch on level `level` after
+ * stk_heat_prepare or a step, HIP events on the library stream: which 0 one
+ * colour of a sweep (k_heat_gs over the whole level), 1 a whole sweep (both
+ * colours and the boundary passes), 2 the residual, 3 a plain copy of
+ * mgn[level] doubles (8 B read + 8 B written per node). */
+int stk_heat_time(stk_ctx *ctx, int which, int level, int reps, double *ms_avg);
 
 /* ---- reference arrays on the context (host <-> device) ---------------- */
 #define STK_SOL0 0

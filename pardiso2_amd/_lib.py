@@ -190,6 +190,7 @@ def lib():
         L.stk_heat_tprolong.argtypes = [vp, C.c_int, C.c_int]
         L.stk_heat_err.argtypes = [vp, dp, dp]
         L.stk_heat_vec.argtypes = [vp, C.c_int, C.c_int, dp, C.c_int]
+        L.stk_heat_time.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
         L.matclear.argtypes = [C.c_int, dp]
         L.matclear.restype = None
         L.gausmat.argtypes = [C.c_int, dp, dp]
@@ -503,6 +504,11 @@ class Stokes:
         out = np.zeros(self.info[level][4])
         check(lib().stk_heat_vec(self.h, which, level, _dp(out), 0), "stk_heat_vec")
         return out
+
+    def heat_time(self, which, level=0, reps=20):
+        ms = C.c_double()
+        check(lib().stk_heat_time(self.h, which, level, reps, C.byref(ms)), "stk_heat_time")
+        return ms.value
 
     def heat_vec_set(self, which, level, x):
         a = np.ascontiguousarray(x, dtype=np.float64)

@@ -16,18 +16,64 @@
 // dependency depth.  On an x-periodic level with an even line count the
 // rows on line 0 read line X-2 of their own colour; they run first, so they
 // read it before its update, as the reference's loop order does.
-#pragma once
-#include <hip/hip_runtime.h>
+//
+// Blocks of the node kernels are renumbered by xcd_block (each XCD owns a
+// contiguous slab of lines, so the x and z neighbour lines a block reads
+// were read by blocks of the same L2).
+//
+// Layout: the solve's arrays hold every y line split by parity, the nodes
+// of even m2 first (position m2/2) and those of odd m2 after them (position
+// (Y+1)/2 + m2/2); lines, planes and levels are where the reference puts
+// them.  One colour of a line is then one contiguous run, its y neighbours
+// the other run of the same line, its x and z neighbours the same run of
+// the next lines: a colour pass reads and writes only what it uses, in
+// whole cache lines.  The boundary converts to and from the reference's
+// order.
 
 namespace stk {
 
+// unsigned n / d for n < 2^31 as a multiply and shift: m = ceil(2^(32+s) / d)
+// with 2^s >= d (m < 2^33, so n m stays below 2^64; the error term
+// n (m d - 2^(32+s)) / (d 2^(32+s)) stays below 1/d)
+struct HDiv {
+  unsigned long long m;
+  int s;
+};
+__host__ __device__ __forceinline__ unsigned hdiv(unsigned n, const HDiv &f) {
+  return (unsigned)(((unsigned long long)n * f.m) >> (32 + f.s));
+}
+inline HDiv make_hdiv(unsigned d) {
+  HDiv f;
+  f.s = 0;
+  while ((1ull << f.s) < d) f.s++;
+  const unsigned long long p = 1ull << (32 + f.s);
+  f.m = (p + d - 1) / d;
+  return f;
+}
+
 struct HeatGeo {
   int X, Y, Z, XY;
+  int hy;                   // (Y + 1) / 2: position of the first odd-m2 node of a line
+  HDiv fY, fX;           // / Y, / X
   int xper;                 // x wraps: line 0's x- neighbour is line X-2 (heat3mg.c:21, 36)
   const double *kf[3];      // 2 / (g[m+1] - g[m-1]) per line (x line 0 of a periodic axis: the wrapped span)
   const double *dm[3];      // g[m] - g[m-1] (x line 0 periodic: g[X-1] - g[X-2])
   const double *dp[3];      // g[m+1] - g[m]
 };
+
+
+
+// index of node (m1, m2, m3) in the split-line layout
+__host__ __device__ __forceinline__ long hidx(int X, int Y, int hy, int m1, int m2, int m3) {
+  return ((long)m3 * X + m1) * Y + ((m2 & 1) ? hy : 0) + (m2 >> 1);
+}
+__device__ __forceinline__ void hdecode(const HeatGeo &g, long n, int &m1, int &m2, int &m3) {
+  const unsigned L = hdiv((unsigned)n, g.fY);
+  const int pos = (int)(n - (long)L * g.Y);
+  m2 = pos < g.hy ? 2 * pos : 2 * (pos - g.hy) + 1;
+  m3 = (int)hdiv(L, g.fX);
+  m1 = (int)L - m3 * g.X;
+}
 
 // the row of heat-equation node n = (m1, m2, m3): neighbours x-, x+, y-, y+,
 // z-, z+, their conductances (the negated off-diagonals) and the diagonal
@@ -42,8 +88,11 @@ __device__ __forceinline__ void heat_row(const HeatGeo &g, const float *__restri
                                          int m1, int m2, int m3, long n, HeatRow &h) {
   h.nb[0] = (g.xper && m1 == 0) ? n + (long)(g.X - 2) * g.Y : n - g.Y;
   h.nb[1] = n + g.Y;
-  h.nb[2] = n - 1;
-  h.nb[3] = n + 1;
+  // y neighbours: the other parity run of the same line
+  const long k = m2 >> 1;
+  const long base = n - ((m2 & 1) ? g.hy : 0) - k;
+  h.nb[2] = (m2 & 1) ? base + k : base + g.hy + k - 1;
+  h.nb[3] = (m2 & 1) ? base + k + 1 : base + g.hy + k;
   h.nb[4] = n - g.XY;
   h.nb[5] = n + g.XY;
   const float k0 = kt[n];
@@ -52,8 +101,10 @@ __device__ __forceinline__ void heat_row(const HeatGeo &g, const float *__restri
   for (int a = 0; a < 3; a++) {
     const float sm = kt[h.nb[2 * a]] + k0;      // FP32 sums, as the reference's float arrays add
     const float sp = k0 + kt[h.nb[2 * a + 1]];
-    h.c[2 * a] = (double)sm / 2.0 * g.kf[a][mm[a]] / g.dm[a][mm[a]];
-    h.c[2 * a + 1] = (double)sp / 2.0 * g.kf[a][mm[a]] / g.dp[a][mm[a]];
+    const int m = mm[a];
+    const double kf = g.kf[a][m];
+    h.c[2 * a] = (double)sm / 2.0 * kf / g.dm[a][m];
+    h.c[2 * a + 1] = (double)sp / 2.0 * kf / g.dp[a][m];
   }
   double d = (double)rcp / dt;
 #pragma unroll
@@ -77,11 +128,12 @@ __global__ void k_heat_rcp(long n, const float *__restrict__ ro, const float *__
 __global__ void k_heat_props(int Xf, int Yf, int Xc, int Yc, int Zc, CompTab t, const float *__restrict__ rof,
                              const float *__restrict__ cpf, const float *__restrict__ ktf, float *roc, float *cpc,
                              float *ktc, float *rcpc) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const int XYc = Xc * Yc, XYf = Xf * Yf;
-  if (n >= (long)XYc * Zc) return;
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int XYc = Xc * Yc, XYf = Xf * Yf, hyf = (Yf + 1) / 2;
+  if (q >= (long)XYc * Zc) return;
   int cx, cy, cz;
-  decode(n, Xc, Yc, XYc, cx, cy, cz);
+  decode(q, Xc, Yc, XYc, cx, cy, cz);
+  const long n = hidx(Xc, Yc, (Yc + 1) / 2, cx, cy, cz);
   float acp = 0.f, akt = 0.f, aro = 0.f;
   double ws = 0.0;
   for (int a = t.ax[2].cbeg[cz]; a < t.ax[2].cbeg[cz + 1]; a++) {
@@ -93,7 +145,7 @@ __global__ void k_heat_props(int Xf, int Yf, int Xc, int Yc, int Zc, CompTab t, 
       for (int e = t.ax[1].cbeg[cy]; e < t.ax[1].cbeg[cy + 1]; e++) {
         const int j = t.ax[1].cfi[e];
         const double wy = t.ax[1].cw64[e];
-        const long nf = (long)k * XYf + (long)i * Yf + j;
+        const long nf = (long)k * XYf + (long)i * Yf + ((j & 1) ? hyf : 0) + (j >> 1);
         const double swt = wx * wy * wz;
         acp = (float)((double)acp + cpf[nf] * swt);
         akt = (float)((double)akt + ktf[nf] * swt);
@@ -165,21 +217,25 @@ __global__ void k_heat_bcc(int nbc, const int *__restrict__ r, const double *__r
 }
 
 // one colour of a Gauss-Seidel sweep over the heat rows on lines
-// m1lo <= m1 < m1hi (gausmatt's update, heat3mg.c:1083-1099: the row's
-// products summed from -fre1 in slot order, sol0 -= sum / diagonal * t0koef)
-__global__ void __launch_bounds__(256) k_heat_gs(HeatGeo g, int col, int m1lo, int m1hi,
+// m1lo <= m1 < m1lo + nx (gausmatt's update, heat3mg.c:1083-1099: the row's
+// products summed from -fre1 in slot order, sol0 -= sum / diagonal *
+// t0koef).  Thread t: line t / hy of the range, k = t mod hy the k-th node of
+// the line's run of this colour.
+__global__ void __launch_bounds__(256) k_heat_gs(HeatGeo g, int col, int m1lo, int nx, HDiv fhy, HDiv fnx,
                                                  const unsigned char *__restrict__ isbc,
                                                  const float *__restrict__ kt, const float *__restrict__ rcp,
-                                                 double dt, double t0, const double *__restrict__ b, double *x) {
-  const int hy = (g.Y + 1) >> 1, nx = m1hi - m1lo;
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= (long)hy * nx * g.Z) return;
-  const int k = (int)(t % hy);
-  const long q = t / hy;
-  const int m1 = m1lo + (int)(q % nx), m3 = (int)(q / nx);
-  const int m2 = 2 * k + ((m1 + m3 + col) & 1);
-  if (m2 >= g.Y) return;
-  const long n = (long)m3 * g.XY + (long)m1 * g.Y + m2;
+                                                 double dt, double t0, const double *__restrict__ b,
+                                                 double *x) {
+  const unsigned t = (unsigned)xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= (unsigned)g.hy * nx * g.Z) return;
+  const unsigned L = hdiv(t, fhy);
+  const int k = (int)(t - L * g.hy);
+  const int m3 = (int)hdiv(L, fnx);
+  const int m1 = m1lo + (int)L - m3 * nx;
+  const int p = (m1 + m3 + col) & 1;
+  if (p && k >= (g.Y >> 1)) return;
+  const int m2 = 2 * k + p;
+  const long n = ((long)m3 * g.X + m1) * g.Y + (p ? g.hy : 0) + k;
   if (isbc[n]) return;
   HeatRow h;
   heat_row(g, kt, rcp[n], dt, m1, m2, m3, n, h);
@@ -205,16 +261,41 @@ __global__ void k_heat_bcpass(int lo, int hi, const int *__restrict__ r, const i
   x[rr] = xr - iv / 1.0 * t0;
 }
 
+// All depths of a level's boundary rows in one workgroup (levels with few
+// of them): depth after depth, a barrier between (it orders the global
+// writes of one depth before the reads of the next within the workgroup).
+constexpr int kHeatBcDepths = 8;
+constexpr int kHeatBcSmall = 2048;
+struct HeatBcDepths {
+  int nd;
+  int off[kHeatBcDepths + 1];
+};
+__global__ void __launch_bounds__(1024) k_heat_bc_small(HeatBcDepths D, const int *__restrict__ r,
+                                                        const int *__restrict__ j, const double *__restrict__ negk,
+                                                        const double *__restrict__ b, double *x, double t0) {
+  for (int d = 0; d < D.nd; d++) {
+    for (int i = D.off[d] + threadIdx.x; i < D.off[d + 1]; i += blockDim.x) {
+      const int rr = r[i], jj = j[i];
+      const double xr = x[rr];
+      double iv = -b[rr];
+      iv += 1.0 * xr;
+      if (jj >= 0) iv += negk[i] * x[jj];
+      x[rr] = xr - iv / 1.0 * t0;
+    }
+    __syncthreads();
+  }
+}
+
 // residual fre0 = fre1 - A sol0 of the heat rows (heat3mg.c:1101-1117)
 __global__ void __launch_bounds__(256) k_heat_res(HeatGeo g, const unsigned char *__restrict__ isbc,
                                                   const float *__restrict__ kt, const float *__restrict__ rcp,
                                                   double dt, const double *__restrict__ b,
                                                   const double *__restrict__ x, double *r) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long n = xcd_block() * blockDim.x + threadIdx.x;
   if (n >= (long)g.XY * g.Z) return;
   if (isbc[n]) return;
   int m1, m2, m3;
-  decode(n, g.X, g.Y, g.XY, m1, m2, m3);
+  hdecode(g, n, m1, m2, m3);
   HeatRow h;
   heat_row(g, kt, rcp[n], dt, m1, m2, m3, n, h);
   double iv = b[n];
@@ -243,13 +324,14 @@ __global__ void k_heat_bcres(int nbc, const int *__restrict__ r, const int *__re
 __global__ void __launch_bounds__(256) k_heat_restrict(int Xf, int Yf, int Xc, int Yc, int Zc, CompTab t,
                                                        const unsigned char *__restrict__ isbc_c,
                                                        const double *__restrict__ rf, double *bc, double *xc) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const int XYc = Xc * Yc, XYf = Xf * Yf;
-  if (n >= (long)XYc * Zc) return;
+  const long q = xcd_block() * blockDim.x + threadIdx.x;
+  const int XYc = Xc * Yc, XYf = Xf * Yf, hyf = (Yf + 1) / 2;
+  if (q >= (long)XYc * Zc) return;
+  int cx, cy, cz;
+  decode(q, Xc, Yc, XYc, cx, cy, cz);
+  const long n = hidx(Xc, Yc, (Yc + 1) / 2, cx, cy, cz);
   xc[n] = 0.0;
   if (isbc_c[n]) { bc[n] = 0.0; return; }
-  int cx, cy, cz;
-  decode(n, Xc, Yc, XYc, cx, cy, cz);
   double acc = 0.0, ws = 0.0;
   for (int a = t.ax[2].cbeg[cz]; a < t.ax[2].cbeg[cz + 1]; a++) {
     const long oz = (long)t.ax[2].cfi[a] * XYf;
@@ -259,7 +341,8 @@ __global__ void __launch_bounds__(256) k_heat_restrict(int Xf, int Yf, int Xc, i
       const double wx = t.ax[0].cw64[bq];
       for (int e = t.ax[1].cbeg[cy]; e < t.ax[1].cbeg[cy + 1]; e++) {
         const double swt = wx * t.ax[1].cw64[e] * wz;
-        acc += rf[ox + t.ax[1].cfi[e]] * swt;
+        const int j = t.ax[1].cfi[e];
+        acc += rf[ox + ((j & 1) ? hyf : 0) + (j >> 1)] * swt;
         ws += swt;
       }
     }
@@ -273,12 +356,13 @@ __global__ void __launch_bounds__(256) k_heat_prolong(int Xf, int Yf, int Zf, in
                                                       const unsigned char *__restrict__ isbc_f,
                                                       const double *__restrict__ xc, double *xf, double t1,
                                                       int ynreset) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const int XYf = Xf * Yf, XYc = Xc * Yc;
-  if (n >= (long)XYf * Zf) return;
-  if (isbc_f[n]) return;
+  const long q = xcd_block() * blockDim.x + threadIdx.x;
+  const int XYf = Xf * Yf, XYc = Xc * Yc, hyc = (Yc + 1) / 2;
+  if (q >= (long)XYf * Zf) return;
   int m1, m2, m3;
-  decode(n, Xf, Yf, XYf, m1, m2, m3);
+  decode(q, Xf, Yf, XYf, m1, m2, m3);
+  const long n = hidx(Xf, Yf, (Yf + 1) / 2, m1, m2, m3);
+  if (isbc_f[n]) return;
   const int lx = t.ax[0].lo[m1], ly = t.ax[1].lo[m2], lz = t.ax[2].lo[m3];
   double v = ynreset ? 0.0 : xf[n];
   for (int dz = 0; dz < 2; dz++) {
@@ -287,7 +371,8 @@ __global__ void __launch_bounds__(256) k_heat_prolong(int Xf, int Yf, int Zf, in
       const double wx = t.ax[0].w64[2 * m1 + dx];
       for (int dy = 0; dy < 2; dy++) {
         const double swt = wx * t.ax[1].w64[2 * m2 + dy] * wz;
-        v += xc[(long)(lz + dz) * XYc + (long)(lx + dx) * Yc + (ly + dy)] * swt * t1;
+        const int j = ly + dy;
+        v += xc[(long)(lz + dz) * XYc + (long)(lx + dx) * Yc + ((j & 1) ? hyc : 0) + (j >> 1)] * swt * t1;
       }
     }
   }
@@ -307,26 +392,30 @@ __global__ void __launch_bounds__(256) k_heat_err(HeatGeo g, const unsigned char
                                                   const float *__restrict__ kt, const float *__restrict__ rcp,
                                                   const float *__restrict__ tk, const float *__restrict__ tk0,
                                                   const float *__restrict__ tk1, double dt, double *out) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long n = xcd_block() * blockDim.x + threadIdx.x;
   double sq = 0.0, cnt = 0.0, mn = 1e30, mx = -1e30;
   if (n < (long)g.XY * g.Z) {
     const float t3 = tk[n];
     mn = mx = (double)t3;
     if (!isbc[n]) {
       int m1, m2, m3;
-      decode(n, g.X, g.Y, g.XY, m1, m2, m3);
+      hdecode(g, n, m1, m2, m3);
       const double rc = (double)rcp[n];
-      const long nb[6] = {(g.xper && m1 == 0) ? n + (long)(g.X - 2) * g.Y : n - g.Y, n + g.Y, n - 1, n + 1,
+      const long k = m2 >> 1, base = n - ((m2 & 1) ? g.hy : 0) - k;
+      const long nb[6] = {(g.xper && m1 == 0) ? n + (long)(g.X - 2) * g.Y : n - g.Y, n + g.Y,
+                          (m2 & 1) ? base + k : base + g.hy + k - 1, (m2 & 1) ? base + k + 1 : base + g.hy + k,
                           n - g.XY, n + g.XY};
       const int mm[3] = {m1, m2, m3};
       const float k0 = kt[n];
       double e = (double)(t3 - tk0[n]) / dt;
 #pragma unroll
       for (int a = 0; a < 3; a++) {
+        const int m = mm[a];
+        const double kf = g.kf[a][m];
         const float sm = kt[nb[2 * a]] + k0, sp = k0 + kt[nb[2 * a + 1]];
-        const double cm = (double)sm / 2.0 * g.kf[a][mm[a]] / g.dm[a][mm[a]] / rc;
+        const double cm = (double)sm / 2.0 * kf / g.dm[a][m] / rc;
         e += (double)(t3 - tk[nb[2 * a]]) * cm;
-        const double cp = (double)sp / 2.0 * g.kf[a][mm[a]] / g.dp[a][mm[a]] / rc;
+        const double cp = (double)sp / 2.0 * kf / g.dp[a][m] / rc;
         e -= (double)(tk[nb[2 * a + 1]] - t3) * cp;
       }
       e -= (double)tk1[n] / rc;

@@ -1437,11 +1437,34 @@ int heat_alloc(stk_ctx *c) {
   return STK_OK;
 }
 
+// level arrays between the reference's node order and the split-line
+// layout of the temperature solve (even m2 of a line first, then odd m2)
+template <typename T>
+void to_split(const T *src, T *dst, int X, int Y, int Z) {
+  const int hy = (Y + 1) / 2;
+  for (long L = 0; L < (long)X * Z; L++)
+    for (int m2 = 0; m2 < Y; m2++) dst[L * Y + ((m2 & 1) ? hy : 0) + (m2 >> 1)] = src[L * Y + m2];
+}
+template <typename T>
+void from_split(const T *src, T *dst, int X, int Y, int Z) {
+  const int hy = (Y + 1) / 2;
+  for (long L = 0; L < (long)X * Z; L++)
+    for (int m2 = 0; m2 < Y; m2++) dst[L * Y + m2] = src[L * Y + ((m2 & 1) ? hy : 0) + (m2 >> 1)];
+}
+inline long split_pos(const Level &L, long n) {
+  const long line = n / L.Y;
+  const int m2 = (int)(n - line * L.Y);
+  return line * L.Y + ((m2 & 1) ? (L.Y + 1) / 2 : 0) + (m2 >> 1);
+}
+
 HeatGeo heat_geo(const stk_ctx *c, int l) {
   const Level &L = c->lv[l];
   const HeatLevel &H = c->heat->lv[l];
   HeatGeo g;
   g.X = L.X; g.Y = L.Y; g.Z = L.Z; g.XY = L.XY;
+  g.hy = (L.Y + 1) / 2;
+  g.fY = make_hdiv((unsigned)L.Y);
+  g.fX = make_hdiv((unsigned)L.X);
   g.xper = c->per[0];
   for (int a = 0; a < 3; a++) { g.kf[a] = H.kf[a]; g.dm[a] = H.dm[a]; g.dp[a] = H.dp[a]; }
   return g;
@@ -1461,6 +1484,7 @@ int heat_build(stk_ctx *c, int Lt) {
   for (int l = 0; l <= Lt; l++) {
     const Level &L = c->lv[l];
     HeatLevel &H = h->lv[l];
+    if (L.n >= (1l << 31)) return STK_EINVAL;  // node indices of the sweeps are 31-bit
     const int dims[3] = {L.X, L.Y, L.Z};
     for (int a = 0; a < 3; a++) {
       const int n = dims[a];
@@ -1478,7 +1502,7 @@ int heat_build(stk_ctx *c, int Lt) {
       }
       if (upload(&H.kf[a], kf) || upload(&H.dm[a], dm) || upload(&H.dp[a], dp)) return STK_ENOMEM;
     }
-    std::vector<unsigned char> isbc(L.n, 0);
+    std::vector<unsigned char> isbc(L.n, 0);  // reference order here, split-line on the device
     std::vector<int> rows;
     for (int m3 = 0; m3 < L.Z; m3++)
       for (int m1 = 0; m1 < L.X; m1++)
@@ -1530,14 +1554,16 @@ int heat_build(stk_ctx *c, int Lt) {
     for (int q = 0; q < nb; q++) {
       const int i = order[q];
       const int b = c->tbondm[L.off + rows[i]];
-      br[q] = rows[i];
-      bj[q] = part[i];
+      br[q] = (int)split_pos(L, rows[i]);
+      bj[q] = part[i] < 0 ? -1 : (int)split_pos(L, part[i]);
       bk[q] = (double)c->tbv1[2 * b + 1];
       bnk[q] = -(double)c->tbv1[2 * b + 1];
       bcv[q] = (double)c->tbv1[2 * b];
     }
     H.nbc = nb;
-    if (upload(&H.isbc, isbc) || upload(&H.bc_r, br) || upload(&H.bc_j, bj) || upload(&H.bc_negk, bnk) ||
+    std::vector<unsigned char> isbc_s(L.n);
+    to_split(isbc.data(), isbc_s.data(), L.X, L.Y, L.Z);
+    if (upload(&H.isbc, isbc_s) || upload(&H.bc_r, br) || upload(&H.bc_j, bj) || upload(&H.bc_negk, bnk) ||
         upload(&H.bc_k, bk) || upload(&H.bc_c, bcv))
       return STK_ENOMEM;
     H.split0 = c->per[0] && (L.X % 2 == 0);
@@ -1556,21 +1582,29 @@ void heat_gs(stk_ctx *c, int l, int sweeps) {
   const float *kt = h->kt + L.off, *rcp = h->rcp + L.off;
   const double *b = h->b + L.off;
   double *x = h->x + L.off;
-  const int hy = (L.Y + 1) / 2;
+  const int hy = (L.Y + 1) / 2, cut = H.split0 ? 1 : 0;
+  const HDiv fhy = make_hdiv((unsigned)hy), f1 = make_hdiv(1u), fn = make_hdiv((unsigned)(L.X - cut));
   for (int s = 0; s < sweeps; s++) {
     for (int col = 0; col < 2; col++) {
-      const int cut = H.split0 ? 1 : 0;
       if (cut)
-        hipLaunchKernelGGL(k_heat_gs, dim3(blocks((long)hy * L.Z, 256)), dim3(256), 0, c->st, g, col, 0, 1,
+        hipLaunchKernelGGL(k_heat_gs, dim3(blocks((long)hy * L.Z, 256)), dim3(256), 0, c->st, g, col, 0, 1, fhy, f1,
                            H.isbc, kt, rcp, h->dt, h->t0, b, x);
       hipLaunchKernelGGL(k_heat_gs, dim3(blocks((long)hy * (L.X - cut) * L.Z, 256)), dim3(256), 0, c->st, g, col,
-                         cut, L.X, H.isbc, kt, rcp, h->dt, h->t0, b, x);
+                         cut, L.X - cut, fhy, fn, H.isbc, kt, rcp, h->dt, h->t0, b, x);
     }
-    for (size_t d = 0; d + 1 < H.bcoff.size(); d++) {
-      const int lo = H.bcoff[d], hi = H.bcoff[d + 1];
-      if (hi > lo)
-        hipLaunchKernelGGL(k_heat_bcpass, dim3(blocks(hi - lo, 256)), dim3(256), 0, c->st, lo, hi, H.bc_r, H.bc_j,
-                           H.bc_negk, b, x, h->t0);
+    const int nd = (int)H.bcoff.size() - 1;
+    if (H.nbc && H.nbc <= kHeatBcSmall && nd <= kHeatBcDepths) {
+      HeatBcDepths D;
+      D.nd = nd;
+      for (int d = 0; d <= nd; d++) D.off[d] = H.bcoff[d];
+      hipLaunchKernelGGL(k_heat_bc_small, dim3(1), dim3(1024), 0, c->st, D, H.bc_r, H.bc_j, H.bc_negk, b, x, h->t0);
+    } else {
+      for (int d = 0; d < nd; d++) {
+        const int lo = H.bcoff[d], hi = H.bcoff[d + 1];
+        if (hi > lo)
+          hipLaunchKernelGGL(k_heat_bcpass, dim3(blocks(hi - lo, 256)), dim3(256), 0, c->st, lo, hi, H.bc_r,
+                             H.bc_j, H.bc_negk, b, x, h->t0);
+      }
     }
   }
 }
@@ -3725,12 +3759,14 @@ int stk_heat_set_fields(stk_ctx *c, const float *ro, const float *cp, const floa
   int rc = heat_alloc(c);
   if (rc) return rc;
   HeatState *h = c->heat;
-  H2D(c, h->ro, ro, n0 * 4);
-  H2D(c, h->cp, cp, n0 * 4);
-  H2D(c, h->kt, kt, n0 * 4);
-  H2D(c, h->tk, tk, n0 * 4);
-  H2D(c, h->tk0, tk, n0 * 4);
-  H2D(c, h->tk1, ht, n0 * 4);
+  const Level &F = c->lv[0];
+  std::vector<float> sp(n0);
+  const std::pair<const float *, float *> up[6] = {{ro, h->ro}, {cp, h->cp}, {kt, h->kt},
+                                                   {tk, h->tk}, {tk, h->tk0}, {ht, h->tk1}};
+  for (const auto &u : up) {
+    to_split(u.first, sp.data(), F.X, F.Y, F.Z);
+    H2D(c, u.second, sp.data(), n0 * 4);
+  }
   hipLaunchKernelGGL(k_heat_rcp, dim3(blocks(n0, 256)), dim3(256), 0, c->st, n0, h->ro, h->cp, h->rcp);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->st));
@@ -3742,13 +3778,26 @@ int stk_heat_set_fields(stk_ctx *c, const float *ro, const float *cp, const floa
 int stk_heat_get_fields(stk_ctx *c, float *ro, float *cp, float *kt, float *tk, float *tk0) {
   if (!c || !c->heat) return STK_ESTATE;
   HeatState *h = c->heat;
-  const long n0 = c->lv[0].n;
   HIPCHK(hipStreamSynchronize(c->st));
-  if (ro) D2H(c, ro, h->ro, c->ntot * 4);
-  if (cp) D2H(c, cp, h->cp, c->ntot * 4);
-  if (kt) D2H(c, kt, h->kt, c->ntot * 4);
-  if (tk) D2H(c, tk, h->tk, n0 * 4);
-  if (tk0) D2H(c, tk0, h->tk0, n0 * 4);
+  std::vector<float> sp;
+  const std::pair<float *, const float *> dn[3] = {{ro, h->ro}, {cp, h->cp}, {kt, h->kt}};
+  for (const auto &d : dn) {
+    if (!d.first) continue;
+    for (int l = 0; l <= c->L; l++) {
+      const Level &L = c->lv[l];
+      sp.resize(L.n);
+      D2H(c, sp.data(), d.second + L.off, L.n * 4);
+      from_split(sp.data(), d.first + L.off, L.X, L.Y, L.Z);
+    }
+  }
+  const Level &F = c->lv[0];
+  const std::pair<float *, const float *> d0[2] = {{tk, h->tk}, {tk0, h->tk0}};
+  for (const auto &d : d0) {
+    if (!d.first) continue;
+    sp.resize(F.n);
+    D2H(c, sp.data(), d.second, F.n * 4);
+    from_split(sp.data(), d.first, F.X, F.Y, F.Z);
+  }
   return STK_OK;
 }
 
@@ -3812,8 +3861,47 @@ int stk_heat_vec(stk_ctx *c, int which, int level, double *host, int dir) {
   HeatState *h = c->heat;
   const Level &L = c->lv[level];
   double *d = (which == STK_SOL0 ? h->x : which == STK_FRE1 ? h->b : h->r) + L.off;
-  if (dir) H2D(c, d, host, L.n * sizeof(double));
-  else D2H(c, host, d, L.n * sizeof(double));
+  std::vector<double> sp(L.n);
+  if (dir) {
+    to_split(host, sp.data(), L.X, L.Y, L.Z);
+    H2D(c, d, sp.data(), L.n * sizeof(double));
+  } else {
+    D2H(c, sp.data(), d, L.n * sizeof(double));
+    from_split(sp.data(), host, L.X, L.Y, L.Z);
+  }
+  return STK_OK;
+}
+
+int stk_heat_time(stk_ctx *c, int which, int level, int reps, double *ms_avg) {
+  if (!c || !c->heat || c->heat->Lt < 0) return STK_ESTATE;
+  if (level < 0 || level > c->heat->Lt || reps < 1 || !ms_avg || which < 0 || which > 3) return STK_EINVAL;
+  HeatState *h = c->heat;
+  const Level &L = c->lv[level];
+  const HeatLevel &H = h->lv[level];
+  const HeatGeo g = heat_geo(c, level);
+  const int hy = (L.Y + 1) / 2;
+  auto launch = [&]() {
+    if (which == 0) {
+      hipLaunchKernelGGL(k_heat_gs, dim3(blocks((long)hy * L.X * L.Z, 256)), dim3(256), 0, c->st, g, 0, 0, L.X,
+                         make_hdiv((unsigned)hy), make_hdiv((unsigned)L.X), H.isbc, h->kt + L.off, h->rcp + L.off,
+                         h->dt, h->t0, h->b + L.off, h->x + L.off);
+    } else if (which == 1) {
+      heat_gs(c, level, 1);
+    } else if (which == 2) {
+      heat_res(c, level);
+    } else {
+      hipLaunchKernelGGL(k_stream_copy, dim3(blocks(L.n, 256)), dim3(256), 0, c->st, L.n, h->b + L.off,
+                         h->r + L.off);
+    }
+  };
+  launch();
+  HIPCHK(hipEventRecord(c->ev0, c->st));
+  for (int i = 0; i < reps; i++) launch();
+  HIPCHK(hipEventRecord(c->ev1, c->st));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *ms_avg = ms / reps;
   return STK_OK;
 }
 

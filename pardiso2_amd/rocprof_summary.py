@@ -5,6 +5,7 @@
     python -m pardiso2_amd.rocprof_summary pmc <dir> [--kernel SUBSTR] [--last N]
     python -m pardiso2_amd.rocprof_summary window <dir> --since SUBSTR [--nth K] [--top N] [--by-grid]
     python -m pardiso2_amd.rocprof_summary traffic <dir> --nodes N --entry NAME:SUBSTR:BYTES_PER_NODE ... [--out F]
+        [--copy-per-node K]
 
 `stats` reads the *_kernel_stats.csv that `rocprofv3 --kernel-trace --stats
 --output-format csv` writes; `trace` reads the same run's *_kernel_trace.csv
@@ -18,7 +19,8 @@ the window's span and the share of it the device spent in kernels (with
 --by-grid, one line per kernel and launch shape, so the levels of a hierarchy
 show apart).  `traffic` reads <dir>/<NAME>_FETCH_SIZE and <dir>/<NAME>_WRITE_SIZE
 (one `--pmc` run each, NAME from the entries and `copy`), scales both counters
-by what they read for k_stream_copy (4*N doubles in, 4*N out per launch) and
+by what they read for k_stream_copy (K*N doubles in, K*N out per launch, K = 4
+by default) and
 prints, per entry, the HBM bytes per launch of the kernel whose name holds
 SUBSTR (mean of its last --last dispatches, default 10) against its
 algorithmic bytes (BYTES_PER_NODE * N); --out writes the same as JSON.
@@ -145,8 +147,8 @@ def _counter(d, substr, counter, last):
     return name, sum(v) / len(v)
 
 
-def traffic(d, nodes, entries, last, out):
-    copy_bytes = 4 * nodes * 8.0
+def traffic(d, nodes, entries, last, out, copy_per_node=4):
+    copy_bytes = copy_per_node * nodes * 8.0
     _, cf = _counter(os.path.join(d, "copy_FETCH_SIZE"), "k_stream_copy", "FETCH_SIZE", last)
     _, cw = _counter(os.path.join(d, "copy_WRITE_SIZE"), "k_stream_copy", "WRITE_SIZE", last)
     ff, wf = copy_bytes / (cf * 1024), copy_bytes / (cw * 1024)
@@ -182,6 +184,8 @@ def main(argv=None):
     ap.add_argument("--nodes", type=int, default=0, help="traffic: finest-level nodes of the run")
     ap.add_argument("--entry", action="append", default=[], help="traffic: NAME:KERNEL_SUBSTR:BYTES_PER_NODE")
     ap.add_argument("--out", default=None, help="traffic: JSON output file")
+    ap.add_argument("--copy-per-node", type=int, default=4,
+                    help="traffic: doubles per node the calibrating k_stream_copy moves each way")
     a = ap.parse_args(argv)
     if a.mode == "stats":
         stats(a.dir, a.top)
@@ -190,7 +194,7 @@ def main(argv=None):
     elif a.mode == "window":
         window(a.dir, a.since or "", a.nth, a.top, a.by_grid)
     elif a.mode == "traffic":
-        traffic(a.dir, a.nodes, a.entry, a.last or 10, a.out)
+        traffic(a.dir, a.nodes, a.entry, a.last or 10, a.out, a.copy_per_node)
     else:
         pmc(a.dir, a.kernel, a.last)
 
