@@ -274,6 +274,13 @@ void stk_heat_default_opts(stk_heat_opts *o);
  * of heaterr, heat3mg.c:54-62, 183). */
 int stk_heat_set_fields(stk_ctx *ctx, const float *ro, const float *cp, const float *kt, const float *tk,
                         const float *ht);
+/* T boundary equations from the reference's own tables (parallel/head3mg.c:
+ * 110-112, in3mg.c:369-385): tbondm = bondm + nodenum8, one entry per node of
+ * every level (0 = none), bondv1[2*(nbond+1)] and bondn1[nbond+1] (partner
+ * node + 1, 0 = none) shared with the Stokes rows.  Replaces the T rows of
+ * earlier stk_bc_line(var 4) calls. */
+int stk_heat_set_bc_tables(stk_ctx *ctx, const int *tbondm, int nbond, const float *bondv1,
+                           const int *bondn1);
 /* ro, cp, kt of every level (ntot each, restricted by the last step or
  * stk_heat_prepare); tk and tk0 of level 0.  Any pointer may be NULL. */
 int stk_heat_get_fields(stk_ctx *ctx, float *ro, float *cp, float *kt, float *tk, float *tk0);

@@ -183,6 +183,7 @@ def lib():
         L.stk_heat_default_opts.restype = None
         L.stk_heat_set_fields.argtypes = [vp] + [fp] * 5
         L.stk_heat_get_fields.argtypes = [vp] + [fp] * 5
+        L.stk_heat_set_bc_tables.argtypes = [vp, ip, C.c_int, fp, ip]
         L.stk_heat_titerate.argtypes = [vp, C.c_double, C.POINTER(HeatOpts), C.POINTER(HeatStats)]
         L.stk_heat_prepare.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double]
         L.stk_heat_gausmatt.argtypes = [vp, C.c_int, C.c_int]
@@ -468,6 +469,14 @@ class Stokes:
     def heat_set_fields(self, ro, cp, kt, tk, ht):
         a = [np.ascontiguousarray(v, dtype=np.float32)[: self.n0] for v in (ro, cp, kt, tk, ht)]
         check(lib().stk_heat_set_fields(self.h, *[_fp(v) for v in a]), "stk_heat_set_fields")
+
+    def heat_set_bc_tables(self, tbondm, nbond, bondv1, bondn1):
+        """T boundary rows from the reference's tables (tbondm = bondm + nodenum8, ntot entries)."""
+        tb = np.ascontiguousarray(tbondm, dtype=np.int32)
+        assert tb.size == self.ntot
+        v1 = np.ascontiguousarray(bondv1, dtype=np.float32)
+        n1 = np.ascontiguousarray(bondn1, dtype=np.int32)
+        check(lib().stk_heat_set_bc_tables(self.h, _ip(tb), int(nbond), _fp(v1), _ip(n1)), "stk_heat_set_bc_tables")
 
     def heat_fields(self):
         """ro, cp, kt of every level; tk, tk0 of level 0."""

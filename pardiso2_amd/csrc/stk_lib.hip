@@ -3775,6 +3775,27 @@ int stk_heat_set_fields(stk_ctx *c, const float *ro, const float *cp, const floa
   return STK_OK;
 }
 
+int stk_heat_set_bc_tables(stk_ctx *c, const int *tbondm, int nbond, const float *bondv1, const int *bondn1) {
+  if (!c || !tbondm || nbond < 0 || (nbond && (!bondv1 || !bondn1))) return STK_EINVAL;
+  std::vector<int> tb(c->ntot, 0);
+  std::vector<float> v1(2, 0.f);
+  std::vector<int> n1(1, 0);
+  for (long n = 0; n < c->ntot; n++) {
+    const int b = tbondm[n];
+    if (!b) continue;
+    if (b < 0 || b > nbond || bondn1[b] < 0 || bondn1[b] > c->ntot) return STK_EINVAL;
+    tb[n] = (int)n1.size();
+    v1.push_back(bondv1[2 * b]);
+    v1.push_back(bondn1[b] ? bondv1[2 * b + 1] : 0.f);
+    n1.push_back(bondn1[b]);
+  }
+  c->tbondm.swap(tb);
+  c->tbv1.swap(v1);
+  c->tbn1.swap(n1);
+  c->heat_dirty = true;
+  return STK_OK;
+}
+
 int stk_heat_get_fields(stk_ctx *c, float *ro, float *cp, float *kt, float *tk, float *tk0) {
   if (!c || !c->heat) return STK_ESTATE;
   HeatState *h = c->heat;

@@ -196,3 +196,26 @@ def test_titerate_large_box_converges(oracle_mod):
     hs, bs = h.err()
     np.testing.assert_allclose(st.heatsum, hs, rtol=1e-10)
     assert hs <= 1e-4
+
+
+def test_boundary_tables_in_the_reference_layout(oracle_mod):
+    # the oracle's T tables (bondm + nodenum8 over levels 0..Lt, bondv1, bondn1) handed over as the
+    # reference keeps them give the same step as the boundary lines
+    hp, o, h, d = _pair(oracle_mod, n=33, Lt=3, xper=True)
+    nt = d.ntot
+    tb = np.zeros(nt, np.int32)
+    na = h.arr("tbondm").size
+    tb[:na] = h.arr("tbondm")
+    nb = oracle_mod.lib().orh_nbond(h.h)
+    s = S.Stokes(hp.gx, hp.gy, hp.gz, hp.levels, periodic=hp.periodic)
+    s.heat_set_bc_tables(tb, nb, h.arr("bv1"), h.arr("bn1"))
+    s.heat_set_fields(hp.ro, hp.cp, hp.kt, hp.tk, hp.tk1)
+    kw = dict(levels=3, pre=[3] * 4, post=[0, 3, 3, 3], heatmin=2e-5, maxit=100)
+    a = s.heat_titerate(hp.timestep, **kw)
+    b = d.heat_titerate(hp.timestep, **kw)
+    assert a.iterations == b.iterations
+    np.testing.assert_array_equal(s.heat_fields()["tk"], d.heat_fields()["tk"])
+    with pytest.raises(S.StkError):
+        bad = tb.copy()
+        bad[np.nonzero(bad)[0][0]] = nb + 5
+        s.heat_set_bc_tables(bad, nb, h.arr("bv1"), h.arr("bn1"))
