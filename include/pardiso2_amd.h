@@ -216,8 +216,11 @@ int stk_stress_strain(stk_ctx *ctx, double *sig, double *eps);
  * immobile; NULL: all normal); eii receives the strain-rate invariant at the
  * start position (the reference's val1).  The water-marker branch and the
  * immobile-marker reset, both disabled in the reference, are not built.
- * Periodic boxes: STK_EINVAL (the periodic interpolation branches of
- * mark3mg.c:2992-3760 are not built). */
+ * Periodic axes (mark3mg.c:337-485, 2992-3760): the interpolations pair the
+ * centres or cells across the wrap as the reference does, and the RK stage
+ * points and the moved markers wrap by one period; STK_EINVAL after the step
+ * if a point is still outside then (moved more than a period: the reference
+ * exits). */
 int stk_move_markers(stk_ctx *ctx, long nmark, float *mx, float *my, float *mz, float *marke, float *markk,
                      const int *markt, float *eii, double timestep, int markmod, int outgrid);
 /* Hydrostatic initial guess: v = 0, P integrated up each column through the

@@ -297,9 +297,9 @@ class Oracle:
         eii = np.zeros(n, np.float32) if want_eii else None
         mt = None if markt is None else np.ascontiguousarray(markt, dtype=np.int32)
         opt = lambda a, f: f(a) if a is not None else None
-        lib().orc_move_markers(self.h, _dp(x), _dp(eps), n, _fp(mx), _fp(my), _fp(mz), opt(marke, _fp),
-                               opt(markk, _fp), opt(mt, _ip), opt(eii, _fp), float(timestep), int(markmod),
-                               int(outgrid))
+        self.wraps_out = lib().orc_move_markers(self.h, _dp(x), _dp(eps), n, _fp(mx), _fp(my), _fp(mz),
+                                                opt(marke, _fp), opt(markk, _fp), opt(mt, _ip), opt(eii, _fp),
+                                                float(timestep), int(markmod), int(outgrid))
         return eii
 
     def set_fields(self, nu, ro, nxx, nxy, nxz, nyz):

@@ -2135,51 +2135,88 @@ void orc_post(const orc_t *o, const double *x, double *sig, double *eps, double 
 
 /* ------------------------------------------------------------------------- */
 /* Marker advection: movemark (parallel/mark3mg.c:174-629) with vxyzcalc1     */
-/* (:2963-3236) and epscalc1 (:3244-3760), walled boxes (the periodic wraps   */
-/* are not restated).  Along each axis a quantity sits on grid lines or on    */
-/* cell centres, and the reference brackets the point with one of five rules */
-/* (names below); weights are 1 - |distance| / spacing, accumulated z outer, */
-/* x, y inner, as the reference's loops run.                                 */
+/* (:2963-3236) and epscalc1 (:3244-3760), walled and periodic boxes.  Along  */
+/* each axis a quantity sits on grid lines or on cell centres, and the        */
+/* reference brackets the point with one of five rules (names below);        */
+/* weights are 1 - |distance| / spacing, accumulated z outer, x, y inner, as */
+/* the reference's loops run.                                                */
 /* ------------------------------------------------------------------------- */
 enum { AX_VN, AX_VC, AX_CELL, AX_SN, AX_SC };
 /* AX_VN   velocity component along its own axis: lines m, m+1 (m clamped to
            0..n-2), the point itself                                (:3004-3007)
-   AX_VC   velocity across an axis: centres of cells below and above (:3008-3023)
-   AX_CELL normal strain rates: cell centres m, m+1 (1..n-2)        (:3272-3285)
+   AX_VC   velocity across an axis: centres of cells below and above (:3008-3023);
+           periodic: below centre 0 or above centre n-3 the pair is (n-2, 0)
+           across the wrap, the point shifted by the box size (:2992-3001)
+   AX_CELL normal strain rates: cell centres m, m+1 (1..n-2)        (:3272-3285);
+           periodic: below centre 1 or above centre n-2 the pair is (n-1, 1)
+           across the wrap (:3282-3291)
    AX_SN   shear strain rates along a line axis: lines 1..n-3, point clamped
-           to the clamped line                                      (:3390-3398)
+           to the clamped line                                      (:3390-3398);
+           periodic: lines 0..n-2 (:3385-3389)
    AX_SC   shear strain rates across: centres, the lower bracket clamped from
-           below at 1 to 0 with the point on centre 0               (:3411-3427) */
-typedef struct { long i0; double w0, w1; } axw_t;
+           below at 1 to 0 with the point on centre 0               (:3411-3427);
+           periodic: below centre 0 the pair is (n-2, 0) across the wrap, the
+           rest as walled (the wrap test there is > n-2, never true) (:3410-3416)
+   Across the wrap the second node's distance is taken from the point minus
+   the box size (:3031, 3044, 3336-3351).  dvar selects the spacing the vz
+   interpolation takes across a periodic x wrap, (g[1] + g[n-1] - g[n-2]) / 2
+   (:3192), where every other wrap takes (g[1] - g[0] + g[n-1] - g[n-2]) / 2. */
+typedef struct { long i0, i1; double w0, w1; } axw_t;
 
-static axw_t axis_weights(const double *g, int n, long m, double p, int kind) {
+static axw_t axis_weights_p(const double *g, int n, long m, double p, int kind, int per, double size, int dvar,
+                            int ylow) {
   axw_t r;
   double P = p, d, t0, t1;
-  long i0 = m;
+  long i0 = m, i1;
+  int wrap = 0;
   if (kind == AX_VN) {
     if (i0 < 0) i0 = 0;
     if (i0 > n - 2) i0 = n - 2;
+    i1 = i0 + 1;
     d = g[i0 + 1] - g[i0];
-    t0 = (g[i0] - p) / d; t1 = (g[i0 + 1] - p) / d;
+    t0 = (g[i0] - p) / d; t1 = (g[i1] - p) / d;
   } else if (kind == AX_SN) {
-    if (i0 < 1) { i0 = 1; P = g[i0]; }
-    if (i0 > n - 3) { i0 = n - 3; P = g[i0 + 1]; }
+    if (per) {
+      if (i0 < 0) { i0 = 0; P = ylow ? g[0] : 0.0; }
+      if (i0 > n - 2) { i0 = n - 2; P = g[i0 + 1]; }
+    } else {
+      if (i0 < 1) { i0 = 1; P = g[i0]; }
+      if (i0 > n - 3) { i0 = n - 3; P = g[i0 + 1]; }
+    }
+    i1 = i0 + 1;
     d = g[i0 + 1] - g[i0];
-    t0 = (g[i0] - P) / d; t1 = (g[i0 + 1] - P) / d;
+    t0 = (g[i0] - P) / d; t1 = (g[i1] - P) / d;
   } else if (kind == AX_CELL) {
     if (P > (g[i0] + g[i0 + 1]) / 2.0) i0 += 1;
-    if (i0 < 1) { i0 = 1; P = (g[i0] + g[i0 - 1]) / 2.0; }
-    if (i0 > n - 2) { i0 = n - 2; P = (g[i0] + g[i0 + 1]) / 2.0; }
-    d = (g[i0 + 1] - g[i0 - 1]) / 2.0;
-    t0 = ((g[i0] + g[i0 - 1]) / 2.0 - P) / d; t1 = ((g[i0 + 1] + g[i0]) / 2.0 - P) / d;
+    if (per && (i0 < 1 || i0 > n - 2)) {
+      if (i0 < 1) P = size + p;
+      i0 = n - 1; i1 = 1; wrap = 1;
+      d = (g[1] - g[0] + g[n - 1] - g[n - 2]) / 2.0;
+    } else {
+      if (i0 < 1) { i0 = 1; P = (g[i0] + g[i0 - 1]) / 2.0; }
+      if (i0 > n - 2) { i0 = n - 2; P = (g[i0] + g[i0 + 1]) / 2.0; }
+      i1 = i0 + 1;
+      d = (g[i0 + 1] - g[i0 - 1]) / 2.0;
+    }
+    t0 = ((g[i0] + g[i0 - 1]) / 2.0 - P) / d;
+    t1 = ((g[i1] + g[i1 - 1]) / 2.0 - (wrap ? P - size : P)) / d;
   } else {  /* AX_VC, AX_SC */
     if (P < (g[i0] + g[i0 + 1]) / 2.0) i0 -= 1;
-    if (kind == AX_VC ? i0 < 0 : i0 < 1) { i0 = 0; P = (g[i0] + g[i0 + 1]) / 2.0; }
-    if (i0 > n - 3) { i0 = n - 3; P = (g[i0 + 1] + g[i0 + 2]) / 2.0; }
-    d = (g[i0 + 2] - g[i0]) / 2.0;
-    t0 = ((g[i0] + g[i0 + 1]) / 2.0 - P) / d; t1 = ((g[i0 + 1] + g[i0 + 2]) / 2.0 - P) / d;
+    if (per && (i0 < 0 || i0 > (kind == AX_VC ? n - 3 : n - 2))) {
+      if (i0 < 0) P = size + p;
+      i0 = n - 2; i1 = 0; wrap = 1;
+      d = dvar ? (g[1] + g[n - 1] - g[n - 2]) / 2.0 : (g[1] - g[0] + g[n - 1] - g[n - 2]) / 2.0;
+    } else {
+      if (kind == AX_VC ? i0 < 0 : i0 < 1) { i0 = 0; P = (g[i0] + g[i0 + 1]) / 2.0; }
+      if (i0 > n - 3) { i0 = n - 3; P = (g[i0 + 1] + g[i0 + 2]) / 2.0; }
+      i1 = i0 + 1;
+      d = (g[i0 + 2] - g[i0]) / 2.0;
+    }
+    t0 = ((g[i0] + g[i0 + 1]) / 2.0 - P) / d;
+    t1 = ((g[i1] + g[i1 + 1]) / 2.0 - (wrap ? P - size : P)) / d;
   }
   r.i0 = i0;
+  r.i1 = i1;
   r.w0 = 1.0 - (-t0);
   r.w1 = 1.0 - t1;
   return r;
@@ -2196,7 +2233,7 @@ static double interp8(const orc_t *o, axw_t ax, axw_t ay, axw_t az, const double
       const double wx = kx ? ax.w1 : ax.w0;
       for (int ky = 0; ky < 2; ky++) {
         const double wy = ky ? ay.w1 : ay.w0;
-        const long n = (az.i0 + kz) * XY + (ax.i0 + kx) * Y + (ay.i0 + ky);
+        const long n = (kz ? az.i1 : az.i0) * XY + (kx ? ax.i1 : ax.i0) * Y + (ky ? ay.i1 : ay.i0);
         double v = vals[(size_t)n * stride + comp];
         if (as_float) v = (double)(float)v;
         acc += v * (wx * wy * wz);
@@ -2212,30 +2249,33 @@ void orc_interp(const orc_t *o, const double *x, const double *eps, double px, d
   const double *gx = o->mggx[0], *gy = o->mggy[0], *gz = o->mggz[0];
   const long n0 = o->mgn[0];
   const double xs = gx[X - 1] - gx[0], ys = gy[Y - 1] - gy[0], zs = gz[Z - 1] - gz[0];
+  const int px_ = o->xper, py_ = o->yper, pz_ = o->zper;
   /* out of grid: onto the box (:2975-2977, :3255-3257) */
   px = px < 0 ? 0 : (px > xs ? xs : px);
   py = py < 0 ? 0 : (py > ys ? ys : py);
   pz = pz < 0 ? 0 : (pz > zs ? zs : pz);
   const long m1 = serch(gx, X, px), m2 = serch(gy, Y, py), m3 = serch(gz, Z, pz);
+#define AW(g, n, m, p, k, per, sz, dv, yl) axis_weights_p(g, n, m, p, k, per, sz, dv, yl)
   if (v) {
-    v[0] = interp8(o, axis_weights(gx, X, m1, px, AX_VN), axis_weights(gy, Y, m2, py, AX_VC),
-                   axis_weights(gz, Z, m3, pz, AX_VC), x, 4, 1, 1);
-    v[1] = interp8(o, axis_weights(gx, X, m1, px, AX_VC), axis_weights(gy, Y, m2, py, AX_VN),
-                   axis_weights(gz, Z, m3, pz, AX_VC), x, 4, 2, 1);
-    v[2] = interp8(o, axis_weights(gx, X, m1, px, AX_VC), axis_weights(gy, Y, m2, py, AX_VC),
-                   axis_weights(gz, Z, m3, pz, AX_VN), x, 4, 3, 1);
+    v[0] = interp8(o, AW(gx, X, m1, px, AX_VN, px_, xs, 0, 0), AW(gy, Y, m2, py, AX_VC, py_, ys, 0, 0),
+                   AW(gz, Z, m3, pz, AX_VC, pz_, zs, 0, 0), x, 4, 1, 1);
+    v[1] = interp8(o, AW(gx, X, m1, px, AX_VC, px_, xs, 0, 0), AW(gy, Y, m2, py, AX_VN, py_, ys, 0, 0),
+                   AW(gz, Z, m3, pz, AX_VC, pz_, zs, 0, 0), x, 4, 2, 1);
+    v[2] = interp8(o, AW(gx, X, m1, px, AX_VC, px_, xs, 1, 0), AW(gy, Y, m2, py, AX_VC, py_, ys, 0, 0),
+                   AW(gz, Z, m3, pz, AX_VN, pz_, zs, 0, 0), x, 4, 3, 1);
   }
   if (e) {
-    const axw_t cx = axis_weights(gx, X, m1, px, AX_CELL), cy = axis_weights(gy, Y, m2, py, AX_CELL),
-                cz = axis_weights(gz, Z, m3, pz, AX_CELL);
+    const axw_t cx = AW(gx, X, m1, px, AX_CELL, px_, xs, 0, 0), cy = AW(gy, Y, m2, py, AX_CELL, py_, ys, 0, 0),
+                cz = AW(gz, Z, m3, pz, AX_CELL, pz_, zs, 0, 0);
     for (int k = 0; k < 3; k++) e[k] = interp8(o, cx, cy, cz, eps + (size_t)k * n0, 1, 0, 1);
-    e[3] = interp8(o, axis_weights(gx, X, m1, px, AX_SN), axis_weights(gy, Y, m2, py, AX_SN),
-                   axis_weights(gz, Z, m3, pz, AX_SC), eps + 3 * (size_t)n0, 1, 0, 1);
-    e[4] = interp8(o, axis_weights(gx, X, m1, px, AX_SN), axis_weights(gy, Y, m2, py, AX_SC),
-                   axis_weights(gz, Z, m3, pz, AX_SN), eps + 4 * (size_t)n0, 1, 0, 1);
-    e[5] = interp8(o, axis_weights(gx, X, m1, px, AX_SC), axis_weights(gy, Y, m2, py, AX_SN),
-                   axis_weights(gz, Z, m3, pz, AX_SN), eps + 5 * (size_t)n0, 1, 0, 1);
+    e[3] = interp8(o, AW(gx, X, m1, px, AX_SN, px_, xs, 0, 0), AW(gy, Y, m2, py, AX_SN, py_, ys, 0, 1),
+                   AW(gz, Z, m3, pz, AX_SC, pz_, zs, 0, 0), eps + 3 * (size_t)n0, 1, 0, 1);
+    e[4] = interp8(o, AW(gx, X, m1, px, AX_SN, px_, xs, 0, 0), AW(gy, Y, m2, py, AX_SC, py_, ys, 0, 0),
+                   AW(gz, Z, m3, pz, AX_SN, pz_, zs, 0, 0), eps + 4 * (size_t)n0, 1, 0, 1);
+    e[5] = interp8(o, AW(gx, X, m1, px, AX_SC, px_, xs, 0, 0), AW(gy, Y, m2, py, AX_SN, py_, ys, 0, 1),
+                   AW(gz, Z, m3, pz, AX_SN, pz_, zs, 0, 0), eps + 5 * (size_t)n0, 1, 0, 1);
   }
+#undef AW
 }
 
 /* second invariant of the strain rate (movemark, mark3mg.c:229): sqrt of
@@ -2244,13 +2284,31 @@ static double eii_of(const double *e) {
   return sqrt(0.5 * (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) + e[3] * e[3] + e[4] * e[4] + e[5] * e[5]);
 }
 
-void orc_move_markers(const orc_t *o, const double *x, const double *eps, long nmark, float *mx, float *my,
-                      float *mz, float *marke, float *markk, const int *markt, float *eii, double timestep,
-                      int markmod, int outgrid) {
+/* periodic wrap of an RK stage position (mark3mg.c:337-355): one period
+   back into [0, size]; the z test compares with the x size as the reference
+   does (:353); 1 if the point is still outside (the reference exits) */
+static int wrap_stage(double *P, double size, double cmp) {
+  if (*P < 0) *P = size + *P;
+  if (*P > cmp) *P = *P - size;
+  return *P < 0 || *P > size;
+}
+
+/* the same for a marker coordinate after the step (:468-485) */
+static int wrap_mark(float *m, double size) {
+  if (*m < 0) *m = (float)(size + *m);
+  if (*m > size) *m = (float)(*m - size);
+  return *m < 0 || *m > size;
+}
+
+int orc_move_markers(const orc_t *o, const double *x, const double *eps, long nmark, float *mx, float *my,
+                     float *mz, float *marke, float *markk, const int *markt, float *eii, double timestep,
+                     int markmod, int outgrid) {
   const int X = o->mgx[0], Y = o->mgy[0], Z = o->mgz[0];
   const double xs = o->mggx[0][X - 1] - o->mggx[0][0], ys = o->mggy[0][Y - 1] - o->mggy[0][0],
                zs = o->mggz[0][Z - 1] - o->mggz[0][0];
-#pragma omp parallel for schedule(static)
+  const int xp = o->xper, yp = o->yper, zp = o->zper;
+  long bad = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad)
   for (long i = 0; i < nmark; i++) {
     const int t = markt ? markt[i] : 0;
     const int inside = mx[i] >= 0 && my[i] >= 0 && mz[i] >= 0 && (double)mx[i] <= xs && (double)my[i] <= ys &&
@@ -2265,27 +2323,46 @@ void orc_move_markers(const orc_t *o, const double *x, const double *eps, long n
       vv[0] = v1[0]; vv[1] = v1[1]; vv[2] = v1[2];
       ee0 = ee1;
     } else {
-      double v2[3], v3[3], v4[3], ee2 = 0, ee3 = 0, ee4 = 0;
-      orc_interp(o, x, eps, (double)(mx[i] + v1[0] * timestep / 2.0), (double)(my[i] + v1[1] * timestep / 2.0),
-                 (double)(mz[i] + v1[2] * timestep / 2.0), v2, wantE ? e : NULL);
-      if (wantE) ee2 = eii_of(e);
-      orc_interp(o, x, eps, (double)(mx[i] + v2[0] * timestep / 2.0), (double)(my[i] + v2[1] * timestep / 2.0),
-                 (double)(mz[i] + v2[2] * timestep / 2.0), v3, wantE ? e : NULL);
-      if (wantE) ee3 = eii_of(e);
-      orc_interp(o, x, eps, (double)(mx[i] + v3[0] * timestep), (double)(my[i] + v3[1] * timestep),
-                 (double)(mz[i] + v3[2] * timestep), v4, wantE ? e : NULL);
-      if (wantE) ee4 = eii_of(e);
-      for (int k = 0; k < 3; k++) vv[k] = (v1[k] + 2.0 * v2[k] + 2.0 * v3[k] + v4[k]) / 6.0;
-      ee0 = (ee1 + 2.0 * ee2 + 2.0 * ee3 + ee4) / 6.0;
+      double vs[4][3], es[4] = {ee1, 0, 0, 0};
+      for (int k = 0; k < 3; k++) vs[0][k] = v1[k];
+      for (int s = 1; s < 4; s++) {
+        /* stage points (:333-335, 363-365, 393-395): x + v dt / 2, x + v dt / 2, x + v dt */
+        double P[3];
+        for (int k = 0; k < 3; k++) {
+          const double mk = k == 0 ? mx[i] : (k == 1 ? my[i] : mz[i]);
+          P[k] = s < 3 ? mk + vs[s - 1][k] * timestep / 2.0 : mk + vs[s - 1][k] * timestep;
+        }
+        if (xp) bad += wrap_stage(&P[0], xs, xs);
+        if (yp) bad += wrap_stage(&P[1], ys, ys);
+        if (zp) bad += wrap_stage(&P[2], zs, xs);
+        orc_interp(o, x, eps, P[0], P[1], P[2], vs[s], wantE ? e : NULL);
+        if (wantE) es[s] = eii_of(e);
+      }
+      for (int k = 0; k < 3; k++) vv[k] = (vs[0][k] + 2.0 * vs[1][k] + 2.0 * vs[2][k] + vs[3][k]) / 6.0;
+      ee0 = (es[0] + 2.0 * es[1] + 2.0 * es[2] + es[3]) / 6.0;
     }
-    if (outgrid == 2) {  /* orthogonal motion only outside the box (mark3mg.c:451-455) */
-      if (mx[i] < 0 || (double)mx[i] > xs) vv[1] = vv[2] = 0;
-      if (my[i] < 0 || (double)my[i] > ys) vv[0] = vv[2] = 0;
-      if (mz[i] < 0 || (double)mz[i] > zs) vv[0] = vv[1] = 0;
+    if (outgrid == 2) {  /* orthogonal motion only outside the box (mark3mg.c:433-457) */
+      if (xp) {
+        if (my[i] < 0 || (double)my[i] > ys) vv[0] = vv[2] = 0;
+        if (mz[i] < 0 || (double)mz[i] > zs) vv[0] = vv[1] = 0;
+      } else if (yp) {
+        if (mx[i] < 0 || (double)mx[i] > xs) vv[1] = vv[2] = 0;
+        if (mz[i] < 0 || (double)mz[i] > zs) vv[0] = vv[1] = 0;
+      } else if (zp) {
+        if (mx[i] < 0 || (double)mx[i] > xs) vv[1] = vv[2] = 0;
+        if (my[i] < 0 || (double)my[i] > zs) vv[0] = vv[2] = 0;
+      } else {
+        if (mx[i] < 0 || (double)mx[i] > xs) vv[1] = vv[2] = 0;
+        if (my[i] < 0 || (double)my[i] > ys) vv[0] = vv[2] = 0;
+        if (mz[i] < 0 || (double)mz[i] > zs) vv[0] = vv[1] = 0;
+      }
     }
     mx[i] += timestep * vv[0];
     my[i] += timestep * vv[1];
     mz[i] += timestep * vv[2];
+    if (xp) bad += wrap_mark(&mx[i], xs);
+    if (yp) bad += wrap_mark(&my[i], ys);
+    if (zp) bad += wrap_mark(&mz[i], zs);
     if (t < 100) {
       if (wantE) marke[i] += (float)(ee0 * timestep);
       if (markk && (mx[i] < 0 || my[i] < 0 || mz[i] < 0 || (double)mx[i] > xs || (double)my[i] > ys ||
@@ -2293,6 +2370,7 @@ void orc_move_markers(const orc_t *o, const double *x, const double *eps, long n
         markk[i] = 0;
     }
   }
+  return (int)(bad > 0x7fffffff ? 0x7fffffff : bad);
 }
 
 /* threads the OpenMP loops of this file use (1 without OpenMP) */

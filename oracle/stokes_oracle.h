@@ -141,16 +141,19 @@ int orc_gcr(orc_t *o, const int *pre, const int *post, int mode, double omega, d
 void orc_post(const orc_t *o, const double *x, double *sig, double *eps, double *out);
 
 /* Marker advection (movemark, parallel/mark3mg.c:174-629) through the
- * interpolations vxyzcalc1 (:2963) and epscalc1 (:3244), walled boxes.
+ * interpolations vxyzcalc1 (:2963) and epscalc1 (:3244), walled and periodic
+ * boxes (periodic axes wrap the RK stage points and the moved markers;
+ * returns the number of wraps that left a point outside, where the reference
+ * exits, 0 normally).
  * x: level-0 solution (4 per node); eps: the six strain-rate planes of
  * orc_post.  Velocities and strain rates are read as the reference's FP32
  * arrays hold them.  marke (accumulated strain), markk (set to 0 for a
  * normal marker leaving the box), markt (types: < 50 normal, >= 100
  * immobile), eii (strain-rate invariant at the start position) may be NULL.
  * markmod 1: one Euler step, else RK4; outgrid as the reference's knob. */
-void orc_move_markers(const orc_t *o, const double *x, const double *eps, long nmark, float *mx, float *my,
-                      float *mz, float *marke, float *markk, const int *markt, float *eii, double timestep,
-                      int markmod, int outgrid);
+int orc_move_markers(const orc_t *o, const double *x, const double *eps, long nmark, float *mx, float *my,
+                     float *mz, float *marke, float *markk, const int *markt, float *eii, double timestep,
+                     int markmod, int outgrid);
 /* the two interpolations alone: v[3] at (px,py,pz); e[6] the strain rates there */
 void orc_interp(const orc_t *o, const double *x, const double *eps, double px, double py, double pz, double *v,
                 double *e);

@@ -1,7 +1,7 @@
 // stk_advect.h -- marker advection on the device (gfx950): one step of
 // movemark (parallel/mark3mg.c:174-629) per marker, with the velocity and
 // strain-rate interpolations vxyzcalc1 (mark3mg.c:2963-3236) and epscalc1
-// (mark3mg.c:3244-3760), walled boxes.
+// (mark3mg.c:3244-3760), walled and periodic boxes.
 //
 // One thread per marker.  The staggered quantities are read from FP32 node
 // planes, as the reference's arrays hold them (k_adv_planes rounds the FP64
@@ -27,6 +27,8 @@ struct AdvGeo {
   double xs, ys, zs;           // box sizes (last line minus first)
   long n0;
   AdvBins bx, by, bz;
+  int px, py, pz;              // periodic axes
+  int *wrap_out;               // set when a wrapped point stays outside (the reference exits there)
 };
 
 // bins per axis: 8 per cell, so a search takes a step or two on grids whose
@@ -38,11 +40,18 @@ constexpr int kAdvBinsPerCell = 8;
 // axis (lines, index clamped), across an axis (centres, point clamped to the
 // outer centres), normal strain rates (cell centres 1..n-1), shear strain
 // rates along a line axis (lines 1..n-2, point clamped to the line) and
-// across (centres; the lower bracket clamped from 1 down to 0)
+// across (centres; the lower bracket clamped from 1 down to 0).  On a
+// periodic axis (mark3mg.c:2992-3001, 3282-3291, 3385-3389, 3410-3416):
+// across-axis velocities pair centres n-2 and 0 over the wrap below centre 0
+// and above centre n-3; normal rates pair cells n-1 and 1 below cell 1 and
+// above cell n-2; shear rates along the axis take lines 0..n-2; shear rates
+// across it wrap only below centre 0 (the reference's test above is > n-2,
+// never true), the rest as walled.  Over the wrap the second node's
+// distance is taken from the point minus the box size.
 enum { kAxVN = 0, kAxVC = 1, kAxCell = 2, kAxSN = 3, kAxSC = 4 };
 
 struct AxW {
-  int i0;
+  int i0, i1;
   double w0, w1;
 };
 
@@ -60,39 +69,72 @@ __device__ __forceinline__ int adv_serch(const double *__restrict__ g, int n, co
   return a;
 }
 
+// per: periodic axis; size: its box size; dvar: the spacing over an x wrap
+// of the vz interpolation, (g[1] + g[n-1] - g[n-2]) / 2 (mark3mg.c:3192),
+// where every other wrap takes (g[1] - g[0] + g[n-1] - g[n-2]) / 2; ylow:
+// the low clamp of a periodic line axis at g[0] (the y axis, :3399) rather
+// than at 0 (x and z, :3387, :3503)
 template <int K>
-__device__ __forceinline__ AxW adv_axis(const double *__restrict__ g, int n, int m, double p) {
+__device__ __forceinline__ AxW adv_axis(const double *__restrict__ g, int n, int m, double p, int per = 0,
+                                        double size = 0.0, int dvar = 0, int ylow = 0) {
   double P = p, d, t0, t1;
-  int i0 = m;
+  int i0 = m, i1;
+  bool wrap = false;
   if (K == kAxVN) {
     if (i0 < 0) i0 = 0;
     if (i0 > n - 2) i0 = n - 2;
+    i1 = i0 + 1;
     d = g[i0 + 1] - g[i0];
     t0 = (g[i0] - p) / d;
-    t1 = (g[i0 + 1] - p) / d;
+    t1 = (g[i1] - p) / d;
   } else if (K == kAxSN) {
-    if (i0 < 1) { i0 = 1; P = g[i0]; }
-    if (i0 > n - 3) { i0 = n - 3; P = g[i0 + 1]; }
+    if (per) {
+      if (i0 < 0) { i0 = 0; P = ylow ? g[0] : 0.0; }
+      if (i0 > n - 2) { i0 = n - 2; P = g[i0 + 1]; }
+    } else {
+      if (i0 < 1) { i0 = 1; P = g[i0]; }
+      if (i0 > n - 3) { i0 = n - 3; P = g[i0 + 1]; }
+    }
+    i1 = i0 + 1;
     d = g[i0 + 1] - g[i0];
     t0 = (g[i0] - P) / d;
-    t1 = (g[i0 + 1] - P) / d;
+    t1 = (g[i1] - P) / d;
   } else if (K == kAxCell) {
     if (P > (g[i0] + g[i0 + 1]) / 2.0) i0 += 1;
-    if (i0 < 1) { i0 = 1; P = (g[i0] + g[i0 - 1]) / 2.0; }
-    if (i0 > n - 2) { i0 = n - 2; P = (g[i0] + g[i0 + 1]) / 2.0; }
-    d = (g[i0 + 1] - g[i0 - 1]) / 2.0;
+    if (per && (i0 < 1 || i0 > n - 2)) {
+      if (i0 < 1) P = size + p;
+      i0 = n - 1;
+      i1 = 1;
+      wrap = true;
+      d = (g[1] - g[0] + g[n - 1] - g[n - 2]) / 2.0;
+    } else {
+      if (i0 < 1) { i0 = 1; P = (g[i0] + g[i0 - 1]) / 2.0; }
+      if (i0 > n - 2) { i0 = n - 2; P = (g[i0] + g[i0 + 1]) / 2.0; }
+      i1 = i0 + 1;
+      d = (g[i0 + 1] - g[i0 - 1]) / 2.0;
+    }
     t0 = ((g[i0] + g[i0 - 1]) / 2.0 - P) / d;
-    t1 = ((g[i0 + 1] + g[i0]) / 2.0 - P) / d;
+    t1 = ((g[i1] + g[i1 - 1]) / 2.0 - (wrap ? P - size : P)) / d;
   } else {
     if (P < (g[i0] + g[i0 + 1]) / 2.0) i0 -= 1;
-    if (K == kAxVC ? i0 < 0 : i0 < 1) { i0 = 0; P = (g[i0] + g[i0 + 1]) / 2.0; }
-    if (i0 > n - 3) { i0 = n - 3; P = (g[i0 + 1] + g[i0 + 2]) / 2.0; }
-    d = (g[i0 + 2] - g[i0]) / 2.0;
+    if (per && (i0 < 0 || i0 > (K == kAxVC ? n - 3 : n - 2))) {
+      if (i0 < 0) P = size + p;
+      i0 = n - 2;
+      i1 = 0;
+      wrap = true;
+      d = dvar ? (g[1] + g[n - 1] - g[n - 2]) / 2.0 : (g[1] - g[0] + g[n - 1] - g[n - 2]) / 2.0;
+    } else {
+      if (K == kAxVC ? i0 < 0 : i0 < 1) { i0 = 0; P = (g[i0] + g[i0 + 1]) / 2.0; }
+      if (i0 > n - 3) { i0 = n - 3; P = (g[i0 + 1] + g[i0 + 2]) / 2.0; }
+      i1 = i0 + 1;
+      d = (g[i0 + 2] - g[i0]) / 2.0;
+    }
     t0 = ((g[i0] + g[i0 + 1]) / 2.0 - P) / d;
-    t1 = ((g[i0 + 1] + g[i0 + 2]) / 2.0 - P) / d;
+    t1 = ((g[i1] + g[i1 + 1]) / 2.0 - (wrap ? P - size : P)) / d;
   }
   AxW r;
   r.i0 = i0;
+  r.i1 = i1;
   r.w0 = 1.0 - (-t0);
   r.w1 = 1.0 - t1;
   return r;
@@ -125,7 +167,7 @@ __device__ __forceinline__ double adv_sum8(const AdvGeo &g, AxW ax, AxW ay, AxW 
 #pragma unroll
       for (int ky = 0; ky < 2; ky++) {
         const double wy = ky ? ay.w1 : ay.w0;
-        const long n = (long)(az.i0 + kz) * g.XY + (long)(ax.i0 + kx) * g.Y + (ay.i0 + ky);
+        const long n = (long)(kz ? az.i1 : az.i0) * g.XY + (long)(kx ? ax.i1 : ax.i0) * g.Y + (ky ? ay.i1 : ay.i0);
         acc += (double)v[n] * (wx * wy * wz);
       }
     }
@@ -142,26 +184,31 @@ __device__ __forceinline__ void adv_interp(const AdvGeo &g, const float *__restr
   pz = pz < 0 ? 0 : (pz > g.zs ? g.zs : pz);
   const int m1 = adv_serch(g.gx, g.X, g.bx, px), m2 = adv_serch(g.gy, g.Y, g.by, py),
             m3 = adv_serch(g.gz, g.Z, g.bz, pz);
-  // each bracketing serves two of the three sums that use it
+  // each bracketing serves two of the three sums that use it (the vz sum
+  // takes its own x bracketing on a periodic x axis: another spacing there)
   if (v) {
     const AxW nx = adv_axis<kAxVN>(g.gx, g.X, m1, px), ny = adv_axis<kAxVN>(g.gy, g.Y, m2, py),
               nz = adv_axis<kAxVN>(g.gz, g.Z, m3, pz);
-    const AxW cx = adv_axis<kAxVC>(g.gx, g.X, m1, px), cy = adv_axis<kAxVC>(g.gy, g.Y, m2, py),
-              cz = adv_axis<kAxVC>(g.gz, g.Z, m3, pz);
+    const AxW cx = adv_axis<kAxVC>(g.gx, g.X, m1, px, g.px, g.xs),
+              cy = adv_axis<kAxVC>(g.gy, g.Y, m2, py, g.py, g.ys),
+              cz = adv_axis<kAxVC>(g.gz, g.Z, m3, pz, g.pz, g.zs);
     v[0] = adv_sum8(g, nx, cy, cz, v3);
     v[1] = adv_sum8(g, cx, ny, cz, v3 + g.n0);
-    v[2] = adv_sum8(g, cx, cy, nz, v3 + 2 * g.n0);
+    const AxW cxz = g.px ? adv_axis<kAxVC>(g.gx, g.X, m1, px, 1, g.xs, 1) : cx;
+    v[2] = adv_sum8(g, cxz, cy, nz, v3 + 2 * g.n0);
   }
   if (e) {
     {
-      const AxW cx = adv_axis<kAxCell>(g.gx, g.X, m1, px), cy = adv_axis<kAxCell>(g.gy, g.Y, m2, py),
-                cz = adv_axis<kAxCell>(g.gz, g.Z, m3, pz);
+      const AxW cx = adv_axis<kAxCell>(g.gx, g.X, m1, px, g.px, g.xs),
+                cy = adv_axis<kAxCell>(g.gy, g.Y, m2, py, g.py, g.ys),
+                cz = adv_axis<kAxCell>(g.gz, g.Z, m3, pz, g.pz, g.zs);
       for (int k = 0; k < 3; k++) e[k] = adv_sum8(g, cx, cy, cz, e6 + (long)k * g.n0);
     }
-    const AxW nx = adv_axis<kAxSN>(g.gx, g.X, m1, px), ny = adv_axis<kAxSN>(g.gy, g.Y, m2, py),
-              nz = adv_axis<kAxSN>(g.gz, g.Z, m3, pz);
-    const AxW cx = adv_axis<kAxSC>(g.gx, g.X, m1, px), cy = adv_axis<kAxSC>(g.gy, g.Y, m2, py),
-              cz = adv_axis<kAxSC>(g.gz, g.Z, m3, pz);
+    const AxW nx = adv_axis<kAxSN>(g.gx, g.X, m1, px, g.px), ny = adv_axis<kAxSN>(g.gy, g.Y, m2, py, g.py, 0.0, 0, 1),
+              nz = adv_axis<kAxSN>(g.gz, g.Z, m3, pz, g.pz);
+    const AxW cx = adv_axis<kAxSC>(g.gx, g.X, m1, px, g.px, g.xs),
+              cy = adv_axis<kAxSC>(g.gy, g.Y, m2, py, g.py, g.ys),
+              cz = adv_axis<kAxSC>(g.gz, g.Z, m3, pz, g.pz, g.zs);
     e[3] = adv_sum8(g, nx, ny, cz, e6 + 3 * g.n0);
     e[4] = adv_sum8(g, nx, cy, nz, e6 + 4 * g.n0);
     e[5] = adv_sum8(g, cx, ny, nz, e6 + 5 * g.n0);
@@ -197,28 +244,73 @@ __global__ void __launch_bounds__(256) k_move_markers(AdvGeo g, const float *__r
   const double ee1 = e6 ? adv_eii(e) : 0.0;
   if (eii) eii[i] = (float)ee1;
   double ee0;
+  bool out = false;
   if (markmod == 1) {
     vv[0] = v1[0]; vv[1] = v1[1]; vv[2] = v1[2];
     ee0 = ee1;
   } else {
-    double v2[3], w3[3], v4[3], ee2 = 0, ee3 = 0, ee4 = 0;
-    adv_interp(g, v3, e6, fx + v1[0] * dt / 2.0, fy + v1[1] * dt / 2.0, fz + v1[2] * dt / 2.0, v2,
-               wantE ? e : nullptr);
-    if (wantE) ee2 = adv_eii(e);
-    adv_interp(g, v3, e6, fx + v2[0] * dt / 2.0, fy + v2[1] * dt / 2.0, fz + v2[2] * dt / 2.0, w3,
-               wantE ? e : nullptr);
-    if (wantE) ee3 = adv_eii(e);
-    adv_interp(g, v3, e6, fx + w3[0] * dt, fy + w3[1] * dt, fz + w3[2] * dt, v4, wantE ? e : nullptr);
-    if (wantE) ee4 = adv_eii(e);
-    for (int k = 0; k < 3; k++) vv[k] = (v1[k] + 2.0 * v2[k] + 2.0 * w3[k] + v4[k]) / 6.0;
-    ee0 = (ee1 + 2.0 * ee2 + 2.0 * ee3 + ee4) / 6.0;
+    double vs[3][3], es[4] = {ee1, 0.0, 0.0, 0.0};
+    const double *prev = v1;
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      // stage points (mark3mg.c:333-335, 363-365, 393-395), wrapped on periodic axes (:337-355;
+      // the z test compares with the x size, as the reference's does)
+      double P[3];
+      if (s < 2) {
+        P[0] = fx + prev[0] * dt / 2.0; P[1] = fy + prev[1] * dt / 2.0; P[2] = fz + prev[2] * dt / 2.0;
+      } else {
+        P[0] = fx + prev[0] * dt; P[1] = fy + prev[1] * dt; P[2] = fz + prev[2] * dt;
+      }
+      const double sz[3] = {g.xs, g.ys, g.zs}, cmp[3] = {g.xs, g.ys, g.xs};
+      const int pr[3] = {g.px, g.py, g.pz};
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+        if (pr[a]) {
+          if (P[a] < 0) P[a] = sz[a] + P[a];
+          if (P[a] > cmp[a]) P[a] = P[a] - sz[a];
+          out |= P[a] < 0 || P[a] > sz[a];
+        }
+      adv_interp(g, v3, e6, P[0], P[1], P[2], vs[s], wantE ? e : nullptr);
+      if (wantE) es[s + 1] = adv_eii(e);
+      prev = vs[s];
+    }
+    for (int k = 0; k < 3; k++) vv[k] = (v1[k] + 2.0 * vs[0][k] + 2.0 * vs[1][k] + vs[2][k]) / 6.0;
+    ee0 = (es[0] + 2.0 * es[1] + 2.0 * es[2] + es[3]) / 6.0;
   }
-  if (outgrid == 2) {
-    if (fx < 0 || (double)fx > g.xs) vv[1] = vv[2] = 0;
-    if (fy < 0 || (double)fy > g.ys) vv[0] = vv[2] = 0;
-    if (fz < 0 || (double)fz > g.zs) vv[0] = vv[1] = 0;
+  if (outgrid == 2) {  // orthogonal motion only (mark3mg.c:433-457): one periodic branch, the first set
+    if (g.px) {
+      if (fy < 0 || (double)fy > g.ys) vv[0] = vv[2] = 0;
+      if (fz < 0 || (double)fz > g.zs) vv[0] = vv[1] = 0;
+    } else if (g.py) {
+      if (fx < 0 || (double)fx > g.xs) vv[1] = vv[2] = 0;
+      if (fz < 0 || (double)fz > g.zs) vv[0] = vv[1] = 0;
+    } else if (g.pz) {
+      if (fx < 0 || (double)fx > g.xs) vv[1] = vv[2] = 0;
+      if (fy < 0 || (double)fy > g.zs) vv[0] = vv[2] = 0;
+    } else {
+      if (fx < 0 || (double)fx > g.xs) vv[1] = vv[2] = 0;
+      if (fy < 0 || (double)fy > g.ys) vv[0] = vv[2] = 0;
+      if (fz < 0 || (double)fz > g.zs) vv[0] = vv[1] = 0;
+    }
   }
-  const float nx = fx + dt * vv[0], ny = fy + dt * vv[1], nz = fz + dt * vv[2];
+  float nx = fx + dt * vv[0], ny = fy + dt * vv[1], nz = fz + dt * vv[2];
+  // periodic wrap of the moved marker (mark3mg.c:468-485)
+  if (g.px) {
+    if (nx < 0) nx = (float)(g.xs + nx);
+    if (nx > g.xs) nx = (float)(nx - g.xs);
+    out |= nx < 0 || nx > g.xs;
+  }
+  if (g.py) {
+    if (ny < 0) ny = (float)(g.ys + ny);
+    if (ny > g.ys) ny = (float)(ny - g.ys);
+    out |= ny < 0 || ny > g.ys;
+  }
+  if (g.pz) {
+    if (nz < 0) nz = (float)(g.zs + nz);
+    if (nz > g.zs) nz = (float)(nz - g.zs);
+    out |= nz < 0 || nz > g.zs;
+  }
+  if (out && g.wrap_out) g.wrap_out[0] = 1;
   mx[i] = nx;
   my[i] = ny;
   mz[i] = nz;

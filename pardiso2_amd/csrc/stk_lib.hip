@@ -2536,7 +2536,6 @@ int stk_move_markers(stk_ctx *c, long nmark, float *mx, float *my, float *mz, fl
                      const int *markt, float *eii, double timestep, int markmod, int outgrid) {
   if (!c || nmark < 0 || (nmark && (!mx || !my || !mz)) || outgrid < 0 || outgrid > 2) return STK_EINVAL;
   if (!c->prepared || !c->X) return STK_ESTATE;
-  if (c->per[0] || c->per[1] || c->per[2]) return STK_EINVAL;
   if (nmark == 0) return STK_OK;
   Level &F = c->lv[0];
   const size_t n6 = (size_t)6 * F.n;
@@ -2567,6 +2566,12 @@ int stk_move_markers(stk_ctx *c, long nmark, float *mx, float *my, float *mz, fl
     if (!rc) rc = hipMemcpyAsync(dt, markt, nmark * sizeof(int), hipMemcpyHostToDevice, c->st) ? STK_EHIP : 0;
   }
   if (!rc && eii) rc = dalloc(&de, nmark);
+  int *wrap_out = nullptr;
+  const bool periodic = c->per[0] || c->per[1] || c->per[2];
+  if (!rc && periodic) {
+    rc = dalloc(&wrap_out, 1);
+    if (!rc) rc = hipMemsetAsync(wrap_out, 0, sizeof(int), c->st) ? STK_EHIP : 0;
+  }
   // cell-search bins, built once per context from the level-0 lines
   const int nl[3] = {F.X, F.Y, F.Z};
   const int nbin[3] = {kAdvBinsPerCell * (F.X - 1), kAdvBinsPerCell * (F.Y - 1), kAdvBinsPerCell * (F.Z - 1)};
@@ -2595,6 +2600,8 @@ int stk_move_markers(stk_ctx *c, long nmark, float *mx, float *my, float *mz, fl
     g.gx = F.g64[0]; g.gy = F.g64[1]; g.gz = F.g64[2];
     g.xs = F.g[0][F.X - 1] - F.g[0][0]; g.ys = F.g[1][F.Y - 1] - F.g[1][0]; g.zs = F.g[2][F.Z - 1] - F.g[2][0];
     g.n0 = F.n;
+    g.px = c->per[0]; g.py = c->per[1]; g.pz = c->per[2];
+    g.wrap_out = wrap_out;
     AdvBins *bn[3] = {&g.bx, &g.by, &g.bz};
     for (int a = 0, o = 0; a < 3; o += nbin[a], a++) {
       bn[a]->t = c->adv_bins + o;
@@ -2608,13 +2615,18 @@ int stk_move_markers(stk_ctx *c, long nmark, float *mx, float *my, float *mz, fl
     for (int k = 0; k < 5 && !rc; k++)
       if (h[k]) rc = hipMemcpyAsync(h[k], d[k], nmark * sizeof(float), hipMemcpyDeviceToHost, c->st) ? STK_EHIP : 0;
     if (!rc && eii) rc = hipMemcpyAsync(eii, de, nmark * sizeof(float), hipMemcpyDeviceToHost, c->st) ? STK_EHIP : 0;
+    int out = 0;
+    if (!rc && wrap_out) rc = hipMemcpyAsync(&out, wrap_out, sizeof(int), hipMemcpyDeviceToHost, c->st) ? STK_EHIP : 0;
     if (!rc) rc = hipStreamSynchronize(c->st) ? STK_EHIP : 0;
+    // a point still outside after one period's wrap: the reference stops there (mark3mg.c:340, 471)
+    if (!rc && out) rc = STK_EINVAL;
   }
   (void)hipStreamSynchronize(c->st);
   for (int k = 0; k < 5; k++) dfree(d[k]);
   dfree(dt);
   dfree(de);
   dfree(planes);
+  dfree(wrap_out);
   return rc;
 }
 

@@ -306,3 +306,72 @@ def test_periodic_stresses_match_oracle(oracle_mod):
         np.testing.assert_array_equal(s[:-1, X - 1, 1:-1], s[:-1, 0, 1:-1])
     finally:
         d.close()
+
+
+@pytest.mark.parametrize("markmod,outgrid", [(0, 0), (1, 0), (0, 2)])
+def test_periodic_advection_bit_identical(oracle_mod, markmod, outgrid):
+    """Markers through the solved periodic shear box (movemark's periodic
+    branches, mark3mg.c:337-485, 2992-3760): interpolations across the x and
+    y wraps, RK stage points and moved markers wrapped by one period;
+    positions, strain, markk and the invariant bit-identical to the oracle's
+    step on the same velocities and strain rates."""
+    pb, o, d = _shear_pair(oracle_mod, n=33, count=1, r=0.2)
+    try:
+        st = d.solve(tol=1e-8)
+        assert st.status == 0
+        x = d.solution()
+        _, eps = d.stress_strain()
+        size = np.array([pb.gx[-1], pb.gy[-1], pb.gz[-1]])
+        rng = np.random.default_rng(40 + markmod + 3 * outgrid)
+        nm = 20000
+        p = rng.random((nm, 3)) * size
+        p[: nm // 4, 0] = rng.choice([0.0, 1.0], nm // 4) * size[0] + rng.uniform(-0.02, 0.02, nm // 4) * size[0]
+        p[: nm // 4, 0] = np.mod(p[: nm // 4, 0], size[0])                  # near the x wrap
+        p[nm // 4: nm // 2, 1] = np.mod(rng.uniform(-0.03, 0.03, nm // 4) * size[1], size[1])  # near the y wrap
+        k = slice(nm // 2, nm // 2 + nm // 8)                               # on grid lines
+        for a, gl in enumerate((pb.gx, pb.gy, pb.gz)):
+            p[k, a] = rng.choice(gl, nm // 8)
+        if outgrid == 2:
+            p[-nm // 16:, 2] = rng.uniform(-0.03, 0.0, nm // 16) * size[2]   # below the z wall
+        dev = [np.ascontiguousarray(p[:, a], dtype=np.float32) for a in range(3)]
+        ref = [a.copy() for a in dev]
+        marke = rng.uniform(-0.5, 1.0, nm).astype(np.float32)
+        markk = np.full(nm, 300.0, np.float32)
+        markt = rng.choice([0, 60, 120], nm).astype(np.int32)
+        marke_o, markk_o = marke.copy(), markk.copy()
+        vmax = np.abs(x[1::4]).max()
+        dt = 0.04 * size[0] / vmax
+        eii = d.move_markers(*dev, marke=marke, markk=markk, markt=markt, timestep=dt, markmod=markmod,
+                             outgrid=outgrid)
+        eii_o = o.move_markers(x, eps, *ref, marke=marke_o, markk=markk_o, markt=markt, timestep=dt,
+                               markmod=markmod, outgrid=outgrid)
+        assert o.wraps_out == 0
+        for a, b, name in zip(dev, ref, "xyz"):
+            np.testing.assert_array_equal(a, b, err_msg=name)
+        np.testing.assert_array_equal(marke, marke_o)
+        np.testing.assert_array_equal(markk, markk_o)
+        np.testing.assert_array_equal(eii, eii_o)
+        crossed = np.abs(dev[0] - p[:, 0].astype(np.float32)) > 0.5 * size[0]
+        assert crossed.sum() > 10, crossed.sum()
+        print(f"[periodic advect] markmod {markmod} outgrid {outgrid}: {nm} markers bit-identical, "
+              f"{int(crossed.sum())} crossed the x wrap", flush=True)
+    finally:
+        d.close()
+
+
+def test_periodic_advection_past_one_period_is_refused(oracle_mod):
+    """A step longer than the period leaves points outside after one wrap:
+    the reference exits there; the device finishes the step and returns
+    STK_EINVAL."""
+    pb, o, d = _shear_pair(oracle_mod, n=17, count=0)
+    try:
+        st = d.solve(tol=1e-8)
+        assert st.status == 0
+        x = d.solution()
+        vmax = np.abs(x[1::4]).max()
+        m = [np.array([0.5 * pb.gx[-1]], np.float32), np.array([0.5], np.float32),
+             np.array([0.9 * pb.gz[-1]], np.float32)]
+        with pytest.raises(S.StkError, match="invalid"):
+            d.move_markers(*m, timestep=3.0 * pb.gx[-1] / vmax, want_eii=False)
+    finally:
+        d.close()
