@@ -285,8 +285,19 @@ int stk_heat_set_fields(stk_ctx *ctx, const float *ro, const float *cp, const fl
 int stk_heat_set_bc_tables(stk_ctx *ctx, const int *tbondm, int nbond, const float *bondv1,
                            const int *bondn1);
 /* ro, cp, kt of every level (ntot each, restricted by the last step or
- * stk_heat_prepare); tk and tk0 of level 0.  Any pointer may be NULL. */
-int stk_heat_get_fields(stk_ctx *ctx, float *ro, float *cp, float *kt, float *tk, float *tk0);
+ * stk_heat_prepare); tk, tk0 and the source term tk1 of level 0.  Any
+ * pointer may be NULL. */
+int stk_heat_get_fields(stk_ctx *ctx, float *ro, float *cp, float *kt, float *tk, float *tk0, float *tk1);
+/* The right part of heaterr (ynerr = 0, parallel/heat3mg.c:54-187) from the
+ * context's current Stokes solution: tk1 = ht (stk_heat_set_fields) plus the
+ * adiabatic term (adiabyn 2: et tk v.grad P, the cells' pressure gradient at
+ * the node; adiabyn 1: et tk ro v.g with the knobs' gravity) and the
+ * frictional heating (frictyn 1: sigma:epsilon from the recomputed stresses
+ * and strain rates), both where ro > 2e3, with the velocity vxyzcalc1
+ * interpolates at the node.  et: the thermal expansion (mgn[0] floats; may be
+ * NULL when adiabyn is 0).  Call it after the solve and before
+ * stk_heat_titerate, as titerate evaluates it at the start of the step. */
+int stk_heat_sources(stk_ctx *ctx, const float *et, int adiabyn, int frictyn);
 /* The whole temperature step (titerate, parallel/heat3mg.c:292-616): the
  * sub-step limit from the largest heating rate (331-382), rocprestrict on
  * levels 1..levels, then sub-steps, each from tk0 = tk: groups of `cycles`

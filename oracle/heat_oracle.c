@@ -482,6 +482,95 @@ void orh_tprolong(orh_t *h, int l, int ynreset) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* The right part of heaterr (ynerr = 0, parallel/heat3mg.c:54-187) on the   */
+/* level-0 heat-equation nodes: tk1 = ht, plus the adiabatic term (adiabyn 2: */
+/* et tk0 v.grad(P) with the pressure gradient of the 8 cells around the node */
+/* at the node's relative position, :100-115; adiabyn 1: et tk0 ro v.g,      */
+/* :131-134) and the frictional heating (frictyn: the normal stress-strain   */
+/* products of the 8 cells and twice the shear ones of the node's 6 edges,  */
+/* :140-175), both where ro > 2e3, with the velocity vxyzcalc1 interpolates  */
+/* at the node (orc_interp on the FP32 velocities of x).  The reference's   */
+/* types: FP32 ht, et, ro, tk0, stresses and strain rates (sig, eps: 6      */
+/* planes each, xx yy zz xy xz yz, rounded to FP32 here), FP64 pressure     */
+/* (x[4 n]).  Boundary-row nodes keep tk1 = ht.                             */
+/* ------------------------------------------------------------------------- */
+void orh_sources(orh_t *h, const double *x, const double *sig, const double *eps, const float *ht, const float *et,
+                 int adiabyn, int frictyn, double gxk, double gyk, double gzk) {
+  const orc_t *o = h->o;
+  const int X = o->mgx[0], Y = o->mgy[0], Z = o->mgz[0];
+  const long XY = o->mgxy[0], n0 = o->mgn[0];
+  const double *gx = o->mggx[0], *gy = o->mggy[0], *gz = o->mggz[0];
+#define F32(a, k, i) ((float)(a)[(size_t)(k) * n0 + (i)])
+#pragma omp parallel for schedule(static)
+  for (int m3 = 0; m3 < Z; m3++)
+    for (int m1 = 0; m1 < X; m1++)
+      for (int m2 = 0; m2 < Y; m2++) {
+        const long v3 = NIDH(o, 0, m1, m2, m3);
+        if (h->tbondm[v3]) { h->tk1[v3] = ht[v3]; continue; }
+        double ival = 0;
+        ival += ht[v3];
+        double xkf;
+        if (o->xper && m1 == 0) xkf = 2.0 / (gx[1] + gx[X - 1] - gx[X - 2]);
+        else xkf = 2.0 / (gx[m1 + 1] - gx[m1 - 1]);
+        const double ykf = 2.0 / (gy[m2 + 1] - gy[m2 - 1]);
+        const double zkf = 2.0 / (gz[m3 + 1] - gz[m3 - 1]);
+        long v[7];
+        v[3] = v3;
+        v[0] = v3 - XY;
+        v[1] = (o->xper && m1 == 0) ? NIDH(o, 0, X - 2, m2, m3) : v3 - Y;
+        v[2] = v3 - 1;
+        v[4] = v3 + 1;
+        v[5] = v3 + Y;
+        v[6] = v3 + XY;
+        long p[8];
+        double rx = 0, ry = 0, rz = 0;
+        if (adiabyn == 2 || frictyn) {
+          p[0] = v3; p[1] = p[0] + 1; p[2] = p[0] + Y; p[3] = p[2] + 1;
+          p[4] = p[0] + XY; p[5] = p[4] + 1; p[6] = p[4] + Y; p[7] = p[6] + 1;
+          if (o->xper && m1 == 0) rx = (gx[X - 1] - gx[X - 2]) / 2.0 * xkf;
+          else rx = (gx[m1] - gx[m1 - 1]) / 2.0 * xkf;
+          ry = (gy[m2] - gy[m2 - 1]) / 2.0 * ykf;
+          rz = (gz[m3] - gz[m3 - 1]) / 2.0 * zkf;
+        }
+        double vv[3] = {0, 0, 0};
+        if (adiabyn) orc_interp(o, x, NULL, gx[m1], gy[m2], gz[m3], vv, NULL);
+        const float r = h->ro[v3], tk0 = h->tk[v3];
+        if (adiabyn == 2 && r > 2e+3) {
+#define PR(k) x[4 * p[k]]
+          const double dpdx = xkf * ((1.0 - ry) * (1.0 - rz) * (PR(2) - PR(0)) + ry * (1.0 - rz) * (PR(3) - PR(1)) +
+                                     (1.0 - ry) * rz * (PR(6) - PR(4)) + ry * rz * (PR(7) - PR(5)));
+          const double dpdy = ykf * ((1.0 - rx) * (1.0 - rz) * (PR(1) - PR(0)) + rx * (1.0 - rz) * (PR(3) - PR(2)) +
+                                     (1.0 - rx) * rz * (PR(5) - PR(4)) + rx * rz * (PR(7) - PR(6)));
+          const double dpdz = zkf * ((1.0 - rx) * (1.0 - ry) * (PR(4) - PR(0)) + rx * (1.0 - ry) * (PR(6) - PR(2)) +
+                                     (1.0 - rx) * ry * (PR(5) - PR(1)) + rx * ry * (PR(7) - PR(3)));
+#undef PR
+          ival += et[v3] * tk0 * (vv[0] * dpdx + vv[1] * dpdy + vv[2] * dpdz);
+        }
+        if (adiabyn == 1 && r > 2e+3) ival += et[v3] * tk0 * r * (vv[0] * gxk + vv[1] * gyk + vv[2] * gzk);
+        if (frictyn && r > 2e+3) {
+#define NRM(k) (F32(sig, 0, p[k]) * F32(eps, 0, p[k]) + F32(sig, 1, p[k]) * F32(eps, 1, p[k]) + \
+                F32(sig, 2, p[k]) * F32(eps, 2, p[k]))
+          double epsval;
+          epsval = (1.0 - rx) * (1.0 - ry) * (1.0 - rz) * NRM(0);
+          epsval += (1.0 - rx) * (ry) * (1.0 - rz) * NRM(1);
+          epsval += (rx) * (1.0 - ry) * (1.0 - rz) * NRM(2);
+          epsval += (rx) * (ry) * (1.0 - rz) * NRM(3);
+          epsval += (1.0 - rx) * (1.0 - ry) * (rz) * NRM(4);
+          epsval += (1.0 - rx) * (ry) * (rz) * NRM(5);
+          epsval += (rx) * (1.0 - ry) * (rz) * NRM(6);
+          epsval += (rx) * (ry) * (rz) * NRM(7);
+#undef NRM
+          epsval += 2.0 * ((1.0 - rz) * (F32(sig, 3, v[0]) * F32(eps, 3, v[0])) + rz * (F32(sig, 3, v3) * F32(eps, 3, v3)));
+          epsval += 2.0 * ((1.0 - ry) * (F32(sig, 4, v[2]) * F32(eps, 4, v[2])) + ry * (F32(sig, 4, v3) * F32(eps, 4, v3)));
+          epsval += 2.0 * ((1.0 - rx) * (F32(sig, 5, v[1]) * F32(eps, 5, v[1])) + rx * (F32(sig, 5, v3) * F32(eps, 5, v3)));
+          ival += epsval;
+        }
+        h->tk1[v3] = (float)ival;
+      }
+#undef F32
+}
+
+/* ------------------------------------------------------------------------- */
 /* Errors of the level-0 temperature tk (parallel/heat3mg.c:528-565):        */
 /* heatsum = RMS over heat-equation nodes of heaterr(ynerr = 2) * timestep   */
 /* (heat3mg.c:251-287) on the FP32 tk, tk0, tk1 as the reference's, and     */

@@ -100,6 +100,8 @@ def lib():
         L.orh_tprolong.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orh_err.argtypes = [C.c_void_p, dp, dp]
         L.orh_step.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_double, C.c_int, C.c_int, dp, dp]
+        L.orh_sources.argtypes = [C.c_void_p, dp, dp, dp, fp, fp, C.c_int, C.c_int, C.c_double, C.c_double,
+                                  C.c_double]
         L.orh_titerate.argtypes = [C.c_void_p, C.c_double, C.c_double, ip, ip, C.c_int, C.c_double, C.c_int,
                                    C.c_int, dp]
         _lib = L
@@ -486,6 +488,17 @@ class Heat:
         it = lib().orh_step(self.h, _ip(p), _ip(q), multicyct, heatmin, maxit, 1 if colored else 0, C.byref(a),
                             C.byref(b))
         return int(it), a.value, b.value
+
+    def sources(self, x, sig, eps, ht, et, adiabyn, frictyn, gravity=(0.0, 0.0, 0.0)):
+        """tk1 = heaterr's right part (radiogenic ht + adiabatic + frictional heating) from a Stokes
+        solution x and its stress / strain-rate planes (6, n0); the temperature tk is tk0."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        sig = np.ascontiguousarray(sig, dtype=np.float64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        ht = np.ascontiguousarray(ht, dtype=np.float32)
+        et = np.ascontiguousarray(et, dtype=np.float32)
+        lib().orh_sources(self.h, _dp(x), _dp(sig), _dp(eps), _fp(ht), _fp(et), int(adiabyn), int(frictyn),
+                          *[float(v) for v in gravity])
 
     def titerate(self, timestep, maxtkstep, pre, post, multicyct=1, heatmin=1e-4, maxit=200, colored=True):
         """Whole thermal step with the sub-step limit; returns a dict of the step's figures."""

@@ -188,6 +188,11 @@ void orh_tprolong(orh_t *h, int l, int ynreset);                         /* heat
 void orh_err(orh_t *h, double *heatsum, double *bondsum);                /* heat3mg.c:528-565 */
 int orh_step(orh_t *h, const int *pre, const int *post, int multicyct, double heatmin, int maxit, int colored,
              double *heatsum, double *bondsum);                          /* titerate, heat3mg.c:384-590 */
+/* the right part of heaterr (ynerr = 0, heat3mg.c:54-187) into tk1 from a
+   Stokes solution x (4 per node), its stress and strain-rate planes (6 n0
+   each) and FP32 ht, et; gxk..gzk the gravity (GXKOEF..GZKOEF) */
+void orh_sources(orh_t *h, const double *x, const double *sig, const double *eps, const float *ht, const float *et,
+                 int adiabyn, int frictyn, double gxk, double gyk, double gzk);
 /* the whole thermal step with the sub-step limit (heat3mg.c:313-616); out[8] */
 int orh_titerate(orh_t *h, double timestep, double maxtkstep, const int *pre, const int *post, int multicyct,
                  double heatmin, int maxit, int colored, double *out);

@@ -182,7 +182,8 @@ def lib():
         L.stk_heat_default_opts.argtypes = [C.POINTER(HeatOpts)]
         L.stk_heat_default_opts.restype = None
         L.stk_heat_set_fields.argtypes = [vp] + [fp] * 5
-        L.stk_heat_get_fields.argtypes = [vp] + [fp] * 5
+        L.stk_heat_get_fields.argtypes = [vp] + [fp] * 6
+        L.stk_heat_sources.argtypes = [vp, fp, C.c_int, C.c_int]
         L.stk_heat_set_bc_tables.argtypes = [vp, ip, C.c_int, fp, ip]
         L.stk_heat_titerate.argtypes = [vp, C.c_double, C.POINTER(HeatOpts), C.POINTER(HeatStats)]
         L.stk_heat_prepare.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double]
@@ -479,10 +480,16 @@ class Stokes:
         check(lib().stk_heat_set_bc_tables(self.h, _ip(tb), int(nbond), _fp(v1), _ip(n1)), "stk_heat_set_bc_tables")
 
     def heat_fields(self):
-        """ro, cp, kt of every level; tk, tk0 of level 0."""
-        out = [np.zeros(self.ntot, np.float32) for _ in range(3)] + [np.zeros(self.n0, np.float32) for _ in range(2)]
+        """ro, cp, kt of every level; tk, tk0, tk1 of level 0."""
+        out = [np.zeros(self.ntot, np.float32) for _ in range(3)] + [np.zeros(self.n0, np.float32) for _ in range(3)]
         check(lib().stk_heat_get_fields(self.h, *[_fp(v) for v in out]), "stk_heat_get_fields")
-        return dict(zip(("ro", "cp", "kt", "tk", "tk0"), out))
+        return dict(zip(("ro", "cp", "kt", "tk", "tk0", "tk1"), out))
+
+    def heat_sources(self, et=None, adiabyn=0, frictyn=0):
+        """tk1 = ht + heaterr's adiabatic and frictional terms from the current Stokes solution."""
+        e = None if et is None else np.ascontiguousarray(et, dtype=np.float32)[: self.n0]
+        check(lib().stk_heat_sources(self.h, _fp(e) if e is not None else None, int(adiabyn), int(frictyn)),
+              "stk_heat_sources")
 
     def heat_titerate(self, timestep, opts: HeatOpts = None, raise_on_fail=True, **kw):
         o = opts if opts is not None else heat_default_opts(**kw)

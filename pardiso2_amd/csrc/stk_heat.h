@@ -443,4 +443,92 @@ __global__ void __launch_bounds__(256) k_heat_bonderr(int nbc, const int *__rest
   heat_block_red(sq, cnt, 0.0, 0.0, out);
 }
 
+// The right part of heaterr (ynerr = 0, heat3mg.c:54-187) on the level-0
+// heat rows: tk1 = ht, plus the adiabatic term (adiabyn 2: et tk0 v.grad P,
+// the pressure gradient of the 8 cells around the node at the node's
+// relative position, :100-115; adiabyn 1: et tk0 ro v.g, :131-134) and the
+// frictional heating (frictyn: the cells' normal stress-strain products and
+// twice the edges' shear ones, :140-175), both where ro > 2e3.  v at the node
+// as vxyzcalc1 interpolates it (adv_interp on the FP32 velocity planes), the
+// pressure from the FP64 solution, stresses and strain rates rounded to FP32
+// as the reference's arrays hold them; FP32 products and sums where the
+// reference multiplies float arrays.  Natural node order for the Stokes
+// fields, the split-line layout for the temperature arrays.
+__global__ void __launch_bounds__(256) k_heat_sources(HeatGeo hg, AdvGeo ag, const unsigned char *__restrict__ isbc,
+                                                      const double *__restrict__ x, const float *__restrict__ v3,
+                                                      const double *__restrict__ sig,
+                                                      const double *__restrict__ eps, const float *__restrict__ ro,
+                                                      const float *__restrict__ tk, const float *__restrict__ ht,
+                                                      const float *__restrict__ et, float *tk1, int adiabyn,
+                                                      int frictyn, double gxk, double gyk, double gzk) {
+  const long n = xcd_block() * blockDim.x + threadIdx.x;
+  const long n0 = (long)hg.XY * hg.Z;
+  if (n >= n0) return;
+  int m1, m2, m3;
+  decode(n, hg.X, hg.Y, hg.XY, m1, m2, m3);
+  const long hn = hidx(hg.X, hg.Y, hg.hy, m1, m2, m3);
+  if (isbc[hn]) return;
+  double ival = 0.0;
+  ival += ht[hn];
+  const double xkf = hg.kf[0][m1], ykf = hg.kf[1][m2], zkf = hg.kf[2][m3];
+  const long Y = hg.Y, XY = hg.XY;
+  const long v0 = n - XY, v1 = (hg.xper && m1 == 0) ? n + (long)(hg.X - 2) * Y : n - Y, v2 = n - 1;
+  long p[8];
+  double rx = 0.0, ry = 0.0, rz = 0.0;
+  if (adiabyn == 2 || frictyn) {
+    p[0] = n; p[1] = n + 1; p[2] = n + Y; p[3] = p[2] + 1;
+    p[4] = n + XY; p[5] = p[4] + 1; p[6] = p[4] + Y; p[7] = p[6] + 1;
+    rx = hg.dm[0][m1] / 2.0 * xkf;
+    ry = hg.dm[1][m2] / 2.0 * ykf;
+    rz = hg.dm[2][m3] / 2.0 * zkf;
+  }
+  double vv[3] = {0.0, 0.0, 0.0};
+  if (adiabyn) adv_interp(ag, v3, nullptr, ag.gx[m1], ag.gy[m2], ag.gz[m3], vv, nullptr);
+  const float r = ro[hn], t0 = tk[hn];
+  if (adiabyn == 2 && r > 2e+3) {
+    double P[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) P[k] = x[4 * p[k]];
+    const double dpdx = xkf * ((1.0 - ry) * (1.0 - rz) * (P[2] - P[0]) + ry * (1.0 - rz) * (P[3] - P[1]) +
+                               (1.0 - ry) * rz * (P[6] - P[4]) + ry * rz * (P[7] - P[5]));
+    const double dpdy = ykf * ((1.0 - rx) * (1.0 - rz) * (P[1] - P[0]) + rx * (1.0 - rz) * (P[3] - P[2]) +
+                               (1.0 - rx) * rz * (P[5] - P[4]) + rx * rz * (P[7] - P[6]));
+    const double dpdz = zkf * ((1.0 - rx) * (1.0 - ry) * (P[4] - P[0]) + rx * (1.0 - ry) * (P[6] - P[2]) +
+                               (1.0 - rx) * ry * (P[5] - P[1]) + rx * ry * (P[7] - P[3]));
+    const float ek = et[hn] * t0;
+    ival += (double)ek * (vv[0] * dpdx + vv[1] * dpdy + vv[2] * dpdz);
+  }
+  if (adiabyn == 1 && r > 2e+3) {
+    const float ekr = et[hn] * t0 * r;
+    ival += (double)ekr * (vv[0] * gxk + vv[1] * gyk + vv[2] * gzk);
+  }
+  if (frictyn && r > 2e+3) {
+    float q[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const float a = (float)sig[p[k]] * (float)eps[p[k]];
+      const float b = (float)sig[n0 + p[k]] * (float)eps[n0 + p[k]];
+      const float c = (float)sig[2 * n0 + p[k]] * (float)eps[2 * n0 + p[k]];
+      q[k] = a + b + c;
+    }
+    double e;
+    e = (1.0 - rx) * (1.0 - ry) * (1.0 - rz) * (double)q[0];
+    e += (1.0 - rx) * (ry) * (1.0 - rz) * (double)q[1];
+    e += (rx) * (1.0 - ry) * (1.0 - rz) * (double)q[2];
+    e += (rx) * (ry) * (1.0 - rz) * (double)q[3];
+    e += (1.0 - rx) * (1.0 - ry) * (rz) * (double)q[4];
+    e += (1.0 - rx) * (ry) * (rz) * (double)q[5];
+    e += (rx) * (1.0 - ry) * (rz) * (double)q[6];
+    e += (rx) * (ry) * (rz) * (double)q[7];
+    const float sxy0 = (float)sig[3 * n0 + v0] * (float)eps[3 * n0 + v0], sxy3 = (float)sig[3 * n0 + n] * (float)eps[3 * n0 + n];
+    const float sxz2 = (float)sig[4 * n0 + v2] * (float)eps[4 * n0 + v2], sxz3 = (float)sig[4 * n0 + n] * (float)eps[4 * n0 + n];
+    const float syz1 = (float)sig[5 * n0 + v1] * (float)eps[5 * n0 + v1], syz3 = (float)sig[5 * n0 + n] * (float)eps[5 * n0 + n];
+    e += 2.0 * ((1.0 - rz) * (double)sxy0 + rz * (double)sxy3);
+    e += 2.0 * ((1.0 - ry) * (double)sxz2 + ry * (double)sxz3);
+    e += 2.0 * ((1.0 - rx) * (double)syz1 + rx * (double)syz3);
+    ival += e;
+  }
+  tk1[hn] = (float)ival;
+}
+
 }  // namespace stk

@@ -215,6 +215,7 @@ struct HeatState {
   int built = -1;                       // levels 0..built have their tables
   float *ro = nullptr, *cp = nullptr, *kt = nullptr, *rcp = nullptr;  // every level
   float *tk = nullptr, *tk0 = nullptr, *tk1 = nullptr;                // level 0
+  float *ht = nullptr;                  // the caller's source term (tk1 = ht + heating terms)
   double *x = nullptr, *b = nullptr, *r = nullptr;                    // sol0, fre1, fre0 of every level
   double *part = nullptr;               // block partials of the reductions
   std::vector<double> hpart;
@@ -1412,7 +1413,7 @@ void heat_free(HeatState *h) {
   if (!h) return;
   for (int l = 0; l < MAXLV; l++) h->lv[l].release();
   dfree(h->ro); dfree(h->cp); dfree(h->kt); dfree(h->rcp);
-  dfree(h->tk); dfree(h->tk0); dfree(h->tk1);
+  dfree(h->tk); dfree(h->tk0); dfree(h->tk1); dfree(h->ht);
   dfree(h->x); dfree(h->b); dfree(h->r); dfree(h->part);
   delete h;
 }
@@ -1424,7 +1425,7 @@ int heat_alloc(stk_ctx *c) {
   // every level's rows read all of its level-0 sized partial buffer at most
   const long np = 4 * (2 * (long)blocks(n0, 256) + 2);
   if (dalloc(&h->ro, nt) || dalloc(&h->cp, nt) || dalloc(&h->kt, nt) || dalloc(&h->rcp, nt) ||
-      dalloc(&h->tk, n0) || dalloc(&h->tk0, n0) || dalloc(&h->tk1, n0) || dalloc(&h->x, nt) ||
+      dalloc(&h->tk, n0) || dalloc(&h->tk0, n0) || dalloc(&h->tk1, n0) || dalloc(&h->ht, n0) || dalloc(&h->x, nt) ||
       dalloc(&h->b, nt) || dalloc(&h->r, nt) || dalloc(&h->part, np)) {
     heat_free(h);
     return STK_ENOMEM;
@@ -2532,6 +2533,48 @@ int stk_stress_strain(stk_ctx *c, double *sig, double *eps) {
   return STK_OK;
 }
 
+// interpolation geometry of level 0 (vxyzcalc1 / epscalc1); the cell-search
+// bins are built once per context from the level-0 lines
+static int adv_geo(stk_ctx *c, AdvGeo &g) {
+  const Level &F = c->lv[0];
+  const int nl[3] = {F.X, F.Y, F.Z};
+  const int nbin[3] = {kAdvBinsPerCell * (F.X - 1), kAdvBinsPerCell * (F.Y - 1), kAdvBinsPerCell * (F.Z - 1)};
+  if (!c->adv_bins) {
+    std::vector<int> t;
+    for (int a = 0; a < 3; a++) {
+      const double *gl = F.g[a].data();
+      const double h = (gl[nl[a] - 1] - gl[0]) / nbin[a];
+      int i = 0;
+      for (int b = 0; b < nbin[a]; b++) {
+        const double s0 = gl[0] + b * h;
+        while (i < nl[a] - 2 && gl[i + 1] <= s0) i++;
+        t.push_back(i);
+      }
+    }
+    int rc = dalloc(&c->adv_bins, t.size());
+    if (!rc) rc = hipMemcpy(c->adv_bins, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice) ? STK_EHIP : 0;
+    if (rc) {
+      dfree(c->adv_bins);
+      c->adv_bins = nullptr;
+      return rc;
+    }
+  }
+  g.X = F.X; g.Y = F.Y; g.Z = F.Z; g.XY = F.XY;
+  g.gx = F.g64[0]; g.gy = F.g64[1]; g.gz = F.g64[2];
+  g.xs = F.g[0][F.X - 1] - F.g[0][0]; g.ys = F.g[1][F.Y - 1] - F.g[1][0]; g.zs = F.g[2][F.Z - 1] - F.g[2][0];
+  g.n0 = F.n;
+  g.px = c->per[0]; g.py = c->per[1]; g.pz = c->per[2];
+  g.wrap_out = nullptr;
+  AdvBins *bn[3] = {&g.bx, &g.by, &g.bz};
+  for (int a = 0, o = 0; a < 3; o += nbin[a], a++) {
+    bn[a]->t = c->adv_bins + o;
+    bn[a]->nb = nbin[a];
+    bn[a]->g0 = F.g[a][0];
+    bn[a]->inv = nbin[a] / (F.g[a][nl[a] - 1] - F.g[a][0]);
+  }
+  return STK_OK;
+}
+
 int stk_move_markers(stk_ctx *c, long nmark, float *mx, float *my, float *mz, float *marke, float *markk,
                      const int *markt, float *eii, double timestep, int markmod, int outgrid) {
   if (!c || nmark < 0 || (nmark && (!mx || !my || !mz)) || outgrid < 0 || outgrid > 2) return STK_EINVAL;
@@ -2572,43 +2615,10 @@ int stk_move_markers(stk_ctx *c, long nmark, float *mx, float *my, float *mz, fl
     rc = dalloc(&wrap_out, 1);
     if (!rc) rc = hipMemsetAsync(wrap_out, 0, sizeof(int), c->st) ? STK_EHIP : 0;
   }
-  // cell-search bins, built once per context from the level-0 lines
-  const int nl[3] = {F.X, F.Y, F.Z};
-  const int nbin[3] = {kAdvBinsPerCell * (F.X - 1), kAdvBinsPerCell * (F.Y - 1), kAdvBinsPerCell * (F.Z - 1)};
-  if (!rc && !c->adv_bins) {
-    std::vector<int> t;
-    for (int a = 0; a < 3; a++) {
-      const double *gl = F.g[a].data();
-      const double h = (gl[nl[a] - 1] - gl[0]) / nbin[a];
-      int i = 0;
-      for (int b = 0; b < nbin[a]; b++) {
-        const double s0 = gl[0] + b * h;
-        while (i < nl[a] - 2 && gl[i + 1] <= s0) i++;
-        t.push_back(i);
-      }
-    }
-    rc = dalloc(&c->adv_bins, t.size());
-    if (!rc) rc = hipMemcpy(c->adv_bins, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice) ? STK_EHIP : 0;
-    if (rc) {
-      dfree(c->adv_bins);
-      c->adv_bins = nullptr;
-    }
-  }
+  AdvGeo g;
+  if (!rc) rc = adv_geo(c, g);
   if (!rc) {
-    AdvGeo g;
-    g.X = F.X; g.Y = F.Y; g.Z = F.Z; g.XY = F.XY;
-    g.gx = F.g64[0]; g.gy = F.g64[1]; g.gz = F.g64[2];
-    g.xs = F.g[0][F.X - 1] - F.g[0][0]; g.ys = F.g[1][F.Y - 1] - F.g[1][0]; g.zs = F.g[2][F.Z - 1] - F.g[2][0];
-    g.n0 = F.n;
-    g.px = c->per[0]; g.py = c->per[1]; g.pz = c->per[2];
     g.wrap_out = wrap_out;
-    AdvBins *bn[3] = {&g.bx, &g.by, &g.bz};
-    for (int a = 0, o = 0; a < 3; o += nbin[a], a++) {
-      bn[a]->t = c->adv_bins + o;
-      bn[a]->nb = nbin[a];
-      bn[a]->g0 = F.g[a][0];
-      bn[a]->inv = nbin[a] / (F.g[a][nl[a] - 1] - F.g[a][0]);
-    }
     hipLaunchKernelGGL(k_move_markers, dim3(blocks(nmark, 256)), dim3(256), 0, c->st, g, planes,
                        need_e ? planes + 3 * F.n : nullptr, nmark, d[0], d[1], d[2], d[3], d[4], dt, de, timestep,
                        markmod, outgrid);
@@ -3773,8 +3783,8 @@ int stk_heat_set_fields(stk_ctx *c, const float *ro, const float *cp, const floa
   HeatState *h = c->heat;
   const Level &F = c->lv[0];
   std::vector<float> sp(n0);
-  const std::pair<const float *, float *> up[6] = {{ro, h->ro}, {cp, h->cp}, {kt, h->kt},
-                                                   {tk, h->tk}, {tk, h->tk0}, {ht, h->tk1}};
+  const std::pair<const float *, float *> up[7] = {{ro, h->ro}, {cp, h->cp}, {kt, h->kt}, {tk, h->tk},
+                                                   {tk, h->tk0}, {ht, h->tk1}, {ht, h->ht}};
   for (const auto &u : up) {
     to_split(u.first, sp.data(), F.X, F.Y, F.Z);
     H2D(c, u.second, sp.data(), n0 * 4);
@@ -3808,7 +3818,55 @@ int stk_heat_set_bc_tables(stk_ctx *c, const int *tbondm, int nbond, const float
   return STK_OK;
 }
 
-int stk_heat_get_fields(stk_ctx *c, float *ro, float *cp, float *kt, float *tk, float *tk0) {
+int stk_heat_sources(stk_ctx *c, const float *et, int adiabyn, int frictyn) {
+  if (!c || !c->heat || !c->heat->fields) return STK_ESTATE;
+  if (adiabyn < 0 || adiabyn > 2 || frictyn < 0 || frictyn > 1 || (adiabyn && !et)) return STK_EINVAL;
+  if ((adiabyn || frictyn) && (!c->prepared || !c->X)) return STK_ESTATE;
+  int rc = heat_build(c, std::max(c->heat->built, 0));
+  if (rc) return rc;
+  HeatState *h = c->heat;
+  const Level &F = c->lv[0];
+  float *et_d = nullptr, *v3 = nullptr;
+  AdvGeo ag;
+  memset(&ag, 0, sizeof ag);
+  if (dalloc(&et_d, F.n)) return STK_ENOMEM;
+  if (et) {
+    std::vector<float> sp(F.n);
+    to_split(et, sp.data(), F.X, F.Y, F.Z);
+    rc = hipMemcpyAsync(et_d, sp.data(), F.n * sizeof(float), hipMemcpyHostToDevice, c->st) ? STK_EHIP : 0;
+    if (!rc) rc = hipStreamSynchronize(c->st) ? STK_EHIP : 0;
+  } else {
+    rc = hipMemsetAsync(et_d, 0, F.n * sizeof(float), c->st) ? STK_EHIP : 0;
+  }
+  if (!rc && adiabyn) {
+    rc = dalloc(&v3, (size_t)3 * F.n);
+    if (!rc) rc = adv_geo(c, ag);
+    if (!rc)
+      hipLaunchKernelGGL(k_adv_planes, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, F.n, c->X, nullptr, v3,
+                         nullptr);
+  }
+  if (!rc && frictyn) {
+    const size_t n6 = (size_t)6 * F.n;
+    if (!c->sig && dalloc(&c->sig, n6)) rc = STK_ENOMEM;
+    if (!rc && !c->epsr && dalloc(&c->epsr, n6)) rc = STK_ENOMEM;
+    if (!rc)
+      hipLaunchKernelGGL(k_post_stress, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, post_geo(c), c->X, c->sig,
+                         c->epsr);
+  }
+  if (!rc) {
+    hipLaunchKernelGGL(k_heat_sources, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, heat_geo(c, 0), ag,
+                       h->lv[0].isbc, c->X, v3, c->sig, c->epsr, h->ro, h->tk, h->ht, et_d, h->tk1, adiabyn,
+                       frictyn, c->kn.gx, c->kn.gy, c->kn.gz);
+    rc = hipGetLastError() ? STK_EHIP : 0;
+    if (!rc) rc = hipStreamSynchronize(c->st) ? STK_EHIP : 0;
+  }
+  (void)hipStreamSynchronize(c->st);
+  dfree(et_d);
+  dfree(v3);
+  return rc;
+}
+
+int stk_heat_get_fields(stk_ctx *c, float *ro, float *cp, float *kt, float *tk, float *tk0, float *tk1) {
   if (!c || !c->heat) return STK_ESTATE;
   HeatState *h = c->heat;
   HIPCHK(hipStreamSynchronize(c->st));
@@ -3824,7 +3882,7 @@ int stk_heat_get_fields(stk_ctx *c, float *ro, float *cp, float *kt, float *tk, 
     }
   }
   const Level &F = c->lv[0];
-  const std::pair<float *, const float *> d0[2] = {{tk, h->tk}, {tk0, h->tk0}};
+  const std::pair<float *, const float *> d0[3] = {{tk, h->tk}, {tk0, h->tk0}, {tk1, h->tk1}};
   for (const auto &d : d0) {
     if (!d.first) continue;
     sp.resize(F.n);

@@ -219,3 +219,49 @@ def test_boundary_tables_in_the_reference_layout(oracle_mod):
         bad = tb.copy()
         bad[np.nonzero(bad)[0][0]] = nb + 5
         s.heat_set_bc_tables(bad, nb, h.arr("bv1"), h.arr("bn1"))
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_sources_from_the_stokes_solution(oracle_mod, periodic):
+    """heaterr's right part (heat3mg.c:54-187) from the context's own solved Stokes problem -- the
+    velocity interpolated at the nodes, the cells' pressure gradient, the recomputed stresses and
+    strain rates -- bit-identical to the oracle's on the same solution and planes, for the sinker and
+    the periodic shear box, with densities below and above the 2e3 threshold."""
+    n = 33
+    pb = setups.periodic_shear(n, count=1, r=0.2) if periodic else setups.sinker(n, setups.coarsest_levels(n), seed=1)
+    d = S.from_problem(pb)
+    try:
+        st = d.solve(tol=1e-8)
+        assert st.status == 0
+        x = d.solution()
+        sig, eps = d.stress_strain()
+        N = n ** 3
+        rng = np.random.default_rng(8)
+        ro = rng.uniform(1500.0, 3500.0, N).astype(np.float32)
+        cp = np.full(N, 1000.0, np.float32)
+        kt = rng.uniform(1.0, 4.0, N).astype(np.float32)
+        tk = rng.uniform(0.5, 1.5, N).astype(np.float32)
+        # the heating terms scale with the solution (SI sinker: v ~ 1e-18 m/s), so ht stays below them
+        vmax = np.abs(x[1::4]).max()
+        ht = rng.uniform(0.0, 1e-3 * 3e-5 * 3000.0 * 9.81 * vmax, N).astype(np.float32)
+        et = rng.uniform(1e-5, 5e-5, N).astype(np.float32)
+        lines = setups.parse_bc_lines(setups.heat_bc_text(0.0, 1.0, xperiodic=periodic), n, n, n)
+        d.bc_lines(lines)
+        d.heat_set_fields(ro, cp, kt, tk, ht)
+        o = oracle_mod.Oracle(pb.gx, pb.gy, pb.gz, 0, periodic=pb.periodic)
+        h = oracle_mod.Heat(o, 0)
+        h.bc_lines(lines)
+        h.set_fields(ro, cp, kt, tk, ht)
+        heat = h.arr("tbondm")[:N] == 0
+        for adiab, frict in ((1, 0), (2, 0), (0, 1), (2, 1)):
+            d.heat_sources(et, adiab, frict)
+            h.sources(x, sig, eps, ht, et, adiab, frict, pb.gravity)
+            a, b = d.heat_fields()["tk1"], h.arr("tk1")
+            np.testing.assert_array_equal(a[heat], b[heat], err_msg=f"adiabyn {adiab} frictyn {frict}")
+            if adiab == 2 or frict or any(pb.gravity):  # (no gravity in the shear box: v.g = 0)
+                assert np.abs(a[heat] - ht[heat]).max() > 0  # the terms are there
+        # idempotent: the terms are added to ht, not to the last tk1
+        d.heat_sources(et, 2, 1)
+        np.testing.assert_array_equal(d.heat_fields()["tk1"][heat], b[heat])
+    finally:
+        d.close()

@@ -200,3 +200,61 @@ def test_titerate_substep_count(oracle_mod):
     h, r = run(tk1, 25.0, 0.0)
     assert r["substeps"] == 0
     np.testing.assert_array_equal(h.arr("tk"), hp.tk.astype(np.float32))
+
+
+def _src_setup(oracle_mod, n=11, seed=5):
+    rng = np.random.default_rng(seed)
+    h = rng.uniform(0.7, 1.3, n - 1)
+    g = np.concatenate([[0.0], np.cumsum(h / h.sum())])
+    o = oracle_mod.Oracle(g, g.copy(), g.copy(), 0)
+    H = oracle_mod.Heat(o, 0)
+    H.bc_lines(setups.parse_bc_lines(setups.heat_bc_text(0.0, 1.0), n, n, n))
+    N = n ** 3
+    ro = np.full(N, 3300.0, np.float32)
+    ro[: N // 3] = 1500.0  # below the 2e3 threshold: no heating terms there
+    tk = rng.uniform(0.5, 1.5, N).astype(np.float32)
+    H.set_fields(ro, np.full(N, 1000.0), np.full(N, 3.0), tk, np.zeros(N))
+    return g, o, H, ro, tk, N
+
+
+def test_sources_closed_forms(oracle_mod):
+    """heaterr's right part (heat3mg.c:54-187): with a constant velocity, a pressure linear in x, y, z at
+    the cell centres and constant stresses and strain rates, the adiabatic terms are et tk0 v.grad P
+    (adiabyn 2) and et tk0 ro v.g (adiabyn 1), the frictional term the full contraction sig : eps,
+    all where ro > 2e3 only."""
+    g, o, H, ro, tk, N = _src_setup(oracle_mod)
+    n = len(g)
+    v0 = np.array([0.3, -0.2, 0.5])
+    a = np.array([2.0, -1.0, 0.5])
+    c = np.concatenate([[0.0], 0.5 * (g[:-1] + g[1:])])  # centre of cell m (upper corner m)
+    m3, m1, m2 = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    x = np.zeros(4 * N)
+    x[0::4] = (a[0] * c[m1] + a[1] * c[m2] + a[2] * c[m3]).ravel()
+    for k in range(3):
+        x[1 + k::4] = v0[k]
+    sv = np.array([1.5, -0.5, 2.0, 0.25, -0.75, 0.5])
+    ev = np.array([0.2, 0.1, -0.3, 0.05, 0.4, -0.15])
+    sig = np.repeat(sv[:, None], N, axis=1)
+    eps = np.repeat(ev[:, None], N, axis=1)
+    ht = np.random.default_rng(6).uniform(0.0, 1e-6, N).astype(np.float32)
+    et = np.full(N, 3e-5, np.float32)
+    grav = (0.0, 0.0, 9.81)
+    heat = H.arr("tbondm")[:N] == 0
+    hot = heat & (ro > 2e3)
+    v32 = v0.astype(np.float32).astype(np.float64)
+    f32 = lambda z: z.astype(np.float32).astype(np.float64)
+    contraction = float(np.sum(f32(sv[:3]) * f32(ev[:3])) + 2 * np.sum(f32(sv[3:]) * f32(ev[3:])))
+    for adiab, frict in ((0, 0), (1, 0), (2, 0), (0, 1), (2, 1)):
+        H.sources(x, sig, eps, ht, et, adiab, frict, grav)
+        tk1 = H.arr("tk1").astype(np.float64)
+        want = ht.astype(np.float64).copy()
+        etk = (et * tk).astype(np.float64)
+        if adiab == 1:
+            want[hot] += (et * tk * ro).astype(np.float64)[hot] * float(v32 @ np.array(grav))
+        if adiab == 2:
+            want[hot] += etk[hot] * float(v32 @ a)
+        if frict:
+            want[hot] += contraction
+        np.testing.assert_allclose(tk1[heat], want[heat].astype(np.float32), rtol=2e-6, atol=1e-12,
+                                   err_msg=f"adiabyn {adiab} frictyn {frict}")
+        np.testing.assert_array_equal(tk1[~heat], ht[~heat])
