@@ -284,6 +284,20 @@ int stk_heat_set_fields(stk_ctx *ctx, const float *ro, const float *cp, const fl
  * earlier stk_bc_line(var 4) calls. */
 int stk_heat_set_bc_tables(stk_ctx *ctx, const int *tbondm, int nbond, const float *bondv1,
                            const int *bondn1);
+/* Marker temperatures after the step (titerate, parallel/heat3mg.c:618-950),
+ * from the step's new and starting node temperatures: fluid markers (type 50..99)
+ * inside take the new temperature; normal markers (type < 50) strictly inside
+ * with markk <= 0 take the old one (tdeep where y > zdeep and colder); with
+ * heatdif, the subgrid numerical diffusion toward the nodes (the exponent
+ * -heatdif k/(rho cp) 2 (1/dx^2+1/dy^2+1/dz^2) timestep clamped at -150, the
+ * boundary cells capped at the nodes' largest temperature) scattered to the
+ * nodes and taken back from every marker (antidiffusion); then the nodal
+ * change.  markt may be NULL (all normal); markk (FP32) is updated in place.
+ * The node sums run over the markers of each node's 8 cells in cell order,
+ * markers of a cell in input order: the reference's single-thread order for
+ * markers stored cell by cell.  exp is the device's (ULPs from the host's). */
+int stk_heat_markers(stk_ctx *ctx, long nmark, const float *mx, const float *my, const float *mz,
+                     const int *markt, float *markk, double heatdif, double timestep, double zdeep, double tdeep);
 /* ro, cp, kt of every level (ntot each, restricted by the last step or
  * stk_heat_prepare); tk, tk0 and the source term tk1 of level 0.  Any
  * pointer may be NULL. */

@@ -29,6 +29,7 @@ struct orh_s {
   int *bn1;   /* cap+1: partner node + 1 (0: none) */
   float *ro, *cp, *kt;    /* ntot */
   float *tk, *tk0, *tk1;  /* mgn[0] */
+  float *tk2;             /* mgn[0]: the temperature at the start of the whole step (heat3mg.c:377-381) */
   double *sol0, *sol1, *fre0, *fre1; /* ntot */
   double *val1; /* 7*ntot */
   int *lin0;    /* 7*ntot */
@@ -53,6 +54,7 @@ orh_t *orh_create(orc_t *o, int Lt) {
   h->tk = (float *)calloc(n0, sizeof(float));
   h->tk0 = (float *)calloc(n0, sizeof(float));
   h->tk1 = (float *)calloc(n0, sizeof(float));
+  h->tk2 = (float *)calloc(n0, sizeof(float));
   h->sol0 = (double *)calloc(nt, sizeof(double));
   h->sol1 = (double *)calloc(nt, sizeof(double));
   h->fre0 = (double *)calloc(nt, sizeof(double));
@@ -69,7 +71,7 @@ orh_t *orh_create(orc_t *o, int Lt) {
 void orh_destroy(orh_t *h) {
   if (!h) return;
   free(h->tbondm); free(h->bv1); free(h->bn1);
-  free(h->ro); free(h->cp); free(h->kt); free(h->tk); free(h->tk0); free(h->tk1);
+  free(h->ro); free(h->cp); free(h->kt); free(h->tk); free(h->tk0); free(h->tk1); free(h->tk2);
   free(h->sol0); free(h->sol1); free(h->fre0); free(h->fre1);
   free(h->val1); free(h->lin0); free(h->num0);
   free(h);
@@ -91,6 +93,7 @@ void *orh_array(orh_t *h, const char *n) {
   if (!strcmp(n, "tk")) return h->tk;
   if (!strcmp(n, "tk0")) return h->tk0;
   if (!strcmp(n, "tk1")) return h->tk1;
+  if (!strcmp(n, "tk2")) return h->tk2;
   if (!strcmp(n, "sol0")) return h->sol0;
   if (!strcmp(n, "fre0")) return h->fre0;
   if (!strcmp(n, "fre1")) return h->fre1;
@@ -107,6 +110,18 @@ int orh_nbond(const orh_t *h) { return h->nbond; }
 /* ranges kept 3 outer lines per side), 369-385 (case 4, T):                  */
 /* T_n = const + koef * T_{n + shift}, |shift| > 1 periodic (in3mg.c:339-341). */
 /* ------------------------------------------------------------------------- */
+/* cell search of m1serch/m2serch/m3serch (mark3mg.c:1138-1205): bisection
+   for the largest a <= n-2 with g[a] <= x */
+static long serch_h(const double *g, int n, double x) {
+  long a = 0, b = n - 1, m;
+  do {
+    m = (a + b) / 2;
+    if (g[m] > x) b = m; else a = m;
+  } while ((b - a) > 1);
+  if (a > n - 2) a = n - 2;
+  return a;
+}
+
 static int lvl_index_h(int m, int nfine, int nlev, long mgs) {
   int r = (m - 2) / (int)mgs + 2;
   if (m < 3) r = m;
@@ -571,6 +586,128 @@ void orh_sources(orh_t *h, const double *x, const double *sig, const double *eps
 }
 
 /* ------------------------------------------------------------------------- */
+/* Marker temperatures after the step (titerate, parallel/heat3mg.c:618-950),  */
+/* markers in index order (the reference's loops on one thread):              */
+/*  - tkcalc1 (mark3mg.c:1212-1278): the marker's cell by serch, trilinear   */
+/*    node weights 1 - |d| (z outer, x, y inner), tk and tk2 interpolated;   */
+/*  - fluid markers (50 <= type < 100) inside take the new temperature;      */
+/*  - normal markers (type < 50) strictly inside: markk <= 0 takes the old   */
+/*    one (tdeep where y > zdeep and it is colder); with heatdif, the        */
+/*    boundary-cell cap at the nodes' largest tk, the numerical-diffusion    */
+/*    change dtk1 = (Told - markk) (1 - exp(-heatdif kt/(rho cp) 2 (1/dx^2 + */
+/*    1/dy^2 + 1/dz^2) dt)) (the exponent clamped at -150), added to the     */
+/*    marker and scattered to the 8 nodes with the trilinear weights (Sol1:  */
+/*    dtk1 w, Sol0: w); then the nodal change Tnew - Told;                    */
+/*  - with heatdif: Sol1 / Sol0 at the nodes, and each normal marker loses   */
+/*    sum w Sol1 over its nodes with Sol1 != 0 (antidiffusion).              */
+/* FP32 markk, node fields as the reference's arrays hold them.  exp is the  */
+/* host libm's.                                                               */
+/* ------------------------------------------------------------------------- */
+static void tkcalc(const orh_t *h, double x, double y, double z, long *wn, double *wi, double *tn, double *to) {
+  const orc_t *o = h->o;
+  const double *gx = o->mggx[0], *gy = o->mggy[0], *gz = o->mggz[0];
+  const long m10 = serch_h(gx, o->mgx[0], x), m20 = serch_h(gy, o->mgy[0], y), m30 = serch_h(gz, o->mgz[0], z);
+  wn[0] = m10; wn[1] = m20; wn[2] = m30;
+  const double dx = gx[m10 + 1] - gx[m10], dy = gy[m20 + 1] - gy[m20], dz = gz[m30 + 1] - gz[m30];
+  wi[0] = dx; wi[1] = dy; wi[2] = dz;
+  double a = 0, b = 0;
+  int n1 = 0;
+  for (long m3 = m30; m3 <= m30 + 1; m3++) {
+    double ddz = (gz[m3] - z) / dz;
+    ddz = 1.0 - fabs(ddz);
+    for (long m1 = m10; m1 <= m10 + 1; m1++) {
+      double ddx = (gx[m1] - x) / dx;
+      ddx = 1.0 - fabs(ddx);
+      for (long m2 = m20; m2 <= m20 + 1; m2++) {
+        double ddy = (gy[m2] - y) / dy;
+        ddy = 1.0 - fabs(ddy);
+        const double swt = ddx * ddy * ddz;
+        const long m4 = m3 * o->mgxy[0] + m1 * o->mgy[0] + m2;
+        wi[3 + n1] = swt;
+        wn[3 + n1] = m4;
+        a += swt * h->tk[m4];
+        b += swt * h->tk2[m4];
+        n1++;
+      }
+    }
+  }
+  *tn = a;
+  *to = b;
+}
+
+void orh_markers(orh_t *h, long nmark, const float *mx, const float *my, const float *mz, const int *markt,
+                 float *markk, double heatdif, double timestep, double zdeep, double tdeep) {
+  const orc_t *o = h->o;
+  const int X = o->mgx[0], Y = o->mgy[0], Z = o->mgz[0];
+  const double xs = o->mggx[0][X - 1], ys = o->mggy[0][Y - 1], zs = o->mggz[0][Z - 1];
+  const long n0 = o->mgn[0];
+  double *S0 = (double *)calloc(n0, sizeof(double)), *S1 = (double *)calloc(n0, sizeof(double));
+  long wn[11];
+  double wi[11], tn, to;
+  for (long i = 0; i < nmark; i++)
+    if (markt[i] >= 50 && markt[i] < 100 && mx[i] > 0 && my[i] > 0 && mz[i] > 0 && (double)mx[i] < xs &&
+        (double)my[i] < ys && (double)mz[i] < zs) {
+      tkcalc(h, mx[i], my[i], mz[i], wn, wi, &tn, &to);
+      markk[i] = (float)tn;
+    }
+  for (long i = 0; i < nmark; i++) {
+    if (!(markt[i] < 50 && mx[i] > 0 && my[i] > 0 && mz[i] > 0 && (double)mx[i] < xs && (double)my[i] < ys &&
+          (double)mz[i] < zs))
+      continue;
+    tkcalc(h, mx[i], my[i], mz[i], wn, wi, &tn, &to);
+    if (markk[i] <= 0) {
+      markk[i] = (float)to;
+      if (my[i] > zdeep && markk[i] < tdeep) markk[i] = (float)tdeep;
+    }
+    if (heatdif) {
+      double mkt = 0, mcp = 0, mro = 0, mtk = h->tk[wn[3]];
+      for (int n1 = 0; n1 < 8; n1++) {
+        const long m4 = wn[3 + n1];
+        const double ival = wi[3 + n1];
+        mkt += h->kt[m4] * ival;
+        mcp += h->cp[m4] * ival;
+        mro += h->ro[m4] * ival;
+        if (h->tk[m4] > mtk) mtk = h->tk[m4];
+      }
+      if ((wn[0] <= 0 || wn[0] >= X - 2 || wn[1] <= 0 || wn[1] >= Y - 2 || wn[2] <= 0 || wn[2] >= Z - 2) &&
+          (markk[i] + tn - to) > mtk)
+        markk[i] = (float)(mtk - tn + to);
+      const double dtk = to - (double)markk[i];
+      double rocpkf = mro * mcp;
+      if (rocpkf < 2e+6) rocpkf = 2e+6;
+      double dtk1 = -heatdif * mkt / rocpkf * 2.0 * (1.0 / wi[0] / wi[0] + 1.0 / wi[1] / wi[1] + 1.0 / wi[2] / wi[2]) *
+                    timestep;
+      if (dtk1 < -150.0) dtk1 = -150.0;
+      dtk1 = dtk * (1.0 - exp(dtk1));
+      markk[i] += (float)dtk1;
+      for (int n1 = 0; n1 < 8; n1++) {
+        const long m4 = wn[3 + n1];
+        const double ival = wi[3 + n1];
+        S1[m4] += dtk1 * ival;
+        S0[m4] += ival;
+      }
+    }
+    markk[i] += (float)(tn - to);
+  }
+  if (heatdif) {
+    for (long n = 0; n < n0; n++)
+      if (S0[n]) S1[n] /= S0[n];
+    for (long i = 0; i < nmark; i++) {
+      if (!(markt[i] < 50 && mx[i] > 0 && my[i] > 0 && mz[i] > 0 && (double)mx[i] < xs && (double)my[i] < ys &&
+            (double)mz[i] < zs))
+        continue;
+      tkcalc(h, mx[i], my[i], mz[i], wn, wi, &tn, &to);
+      for (int n1 = 0; n1 < 8; n1++) {
+        const long m4 = wn[3 + n1];
+        if (S1[m4]) markk[i] -= (float)(S1[m4] * wi[3 + n1]);
+      }
+    }
+  }
+  free(S0);
+  free(S1);
+}
+
+/* ------------------------------------------------------------------------- */
 /* Errors of the level-0 temperature tk (parallel/heat3mg.c:528-565):        */
 /* heatsum = RMS over heat-equation nodes of heaterr(ynerr = 2) * timestep   */
 /* (heat3mg.c:251-287) on the FP32 tk, tk0, tk1 as the reference's, and     */
@@ -704,6 +841,7 @@ int orh_titerate(orh_t *h, double timestep, double maxtkstep, const int *pre, co
   int sub = 0, its = 0, conv = 1;
   double hs = 0, bs = 0;
   if (timestep != 0 && tkstep != 0) {
+    for (long n = 0; n < o->mgn[0]; n++) h->tk2[n] = h->tk[n];
     for (int l = 1; l <= h->Lt; l++) orh_rocprestrict(h, l);
     double tsum = 0, tsum0 = timestep;
     do {

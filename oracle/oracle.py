@@ -102,6 +102,8 @@ def lib():
         L.orh_step.argtypes = [C.c_void_p, ip, ip, C.c_int, C.c_double, C.c_int, C.c_int, dp, dp]
         L.orh_sources.argtypes = [C.c_void_p, dp, dp, dp, fp, fp, C.c_int, C.c_int, C.c_double, C.c_double,
                                   C.c_double]
+        L.orh_markers.argtypes = [C.c_void_p, C.c_long, fp, fp, fp, ip, fp, C.c_double, C.c_double, C.c_double,
+                                  C.c_double]
         L.orh_titerate.argtypes = [C.c_void_p, C.c_double, C.c_double, ip, ip, C.c_int, C.c_double, C.c_int,
                                    C.c_int, dp]
         _lib = L
@@ -418,14 +420,14 @@ class Heat:
 
     def arr(self, name):
         p = lib().orh_array(self.h, name.encode())
-        n = self.n0 if name in ("tk", "tk0", "tk1") else self.ntot
+        n = self.n0 if name in ("tk", "tk0", "tk1", "tk2") else self.ntot
         if name in ("val1", "lin0"):
             n *= 7
         if name in ("bv1", "bn1"):
             n = (2 if name == "bv1" else 1) * (lib().orh_nbond(self.h) + 1)
         ct = {"tbondm": C.c_int, "bn1": C.c_int, "lin0": C.c_int, "num0": C.c_int, "bv1": C.c_float,
               "ro": C.c_float, "cp": C.c_float, "kt": C.c_float, "tk": C.c_float, "tk0": C.c_float,
-              "tk1": C.c_float}.get(name, C.c_double)
+              "tk1": C.c_float, "tk2": C.c_float}.get(name, C.c_double)
         return np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), shape=(n,))
 
     def bc_lines(self, lines):
@@ -444,6 +446,7 @@ class Heat:
         self.arr("tk")[:] = np.asarray(tk, dtype=np.float32)[:n0]
         self.arr("tk0")[:] = np.asarray(tk, dtype=np.float32)[:n0]
         self.arr("tk1")[:] = np.asarray(tk1, dtype=np.float32)[:n0]
+        self.arr("tk2")[:] = np.asarray(tk, dtype=np.float32)[:n0]
 
     def rocprestrict(self, l):
         lib().orh_rocprestrict(self.h, l)
@@ -499,6 +502,14 @@ class Heat:
         et = np.ascontiguousarray(et, dtype=np.float32)
         lib().orh_sources(self.h, _dp(x), _dp(sig), _dp(eps), _fp(ht), _fp(et), int(adiabyn), int(frictyn),
                           *[float(v) for v in gravity])
+
+    def markers(self, mx, my, mz, markt, markk, heatdif=1.0, timestep=1.0, zdeep=1e30, tdeep=0.0):
+        """Marker temperatures after the step (markk updated in place, float32)."""
+        for a in (mx, my, mz, markk):
+            assert a.dtype == np.float32 and a.flags.c_contiguous
+        mt = np.ascontiguousarray(markt, dtype=np.int32)
+        lib().orh_markers(self.h, len(mx), _fp(mx), _fp(my), _fp(mz), _ip(mt), _fp(markk), float(heatdif),
+                          float(timestep), float(zdeep), float(tdeep))
 
     def titerate(self, timestep, maxtkstep, pre, post, multicyct=1, heatmin=1e-4, maxit=200, colored=True):
         """Whole thermal step with the sub-step limit; returns a dict of the step's figures."""

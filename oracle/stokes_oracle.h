@@ -193,6 +193,10 @@ int orh_step(orh_t *h, const int *pre, const int *post, int multicyct, double he
    each) and FP32 ht, et; gxk..gzk the gravity (GXKOEF..GZKOEF) */
 void orh_sources(orh_t *h, const double *x, const double *sig, const double *eps, const float *ht, const float *et,
                  int adiabyn, int frictyn, double gxk, double gyk, double gzk);
+/* marker temperatures after the step (heat3mg.c:618-950), markers in index
+   order; tk2 (array "tk2") is the temperature at the start of the step */
+void orh_markers(orh_t *h, long nmark, const float *mx, const float *my, const float *mz, const int *markt,
+                 float *markk, double heatdif, double timestep, double zdeep, double tdeep);
 /* the whole thermal step with the sub-step limit (heat3mg.c:313-616); out[8] */
 int orh_titerate(orh_t *h, double timestep, double maxtkstep, const int *pre, const int *post, int multicyct,
                  double heatmin, int maxit, int colored, double *out);

@@ -184,6 +184,8 @@ def lib():
         L.stk_heat_set_fields.argtypes = [vp] + [fp] * 5
         L.stk_heat_get_fields.argtypes = [vp] + [fp] * 6
         L.stk_heat_sources.argtypes = [vp, fp, C.c_int, C.c_int]
+        L.stk_heat_markers.argtypes = [vp, C.c_long, fp, fp, fp, ip, fp, C.c_double, C.c_double, C.c_double,
+                                       C.c_double]
         L.stk_heat_set_bc_tables.argtypes = [vp, ip, C.c_int, fp, ip]
         L.stk_heat_titerate.argtypes = [vp, C.c_double, C.POINTER(HeatOpts), C.POINTER(HeatStats)]
         L.stk_heat_prepare.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double]
@@ -490,6 +492,17 @@ class Stokes:
         e = None if et is None else np.ascontiguousarray(et, dtype=np.float32)[: self.n0]
         check(lib().stk_heat_sources(self.h, _fp(e) if e is not None else None, int(adiabyn), int(frictyn)),
               "stk_heat_sources")
+
+    def heat_markers(self, mx, my, mz, markt, markk, heatdif=1.0, timestep=1.0, zdeep=1e30, tdeep=0.0):
+        """Marker temperatures after the step (markk, float32, updated in place)."""
+        n = len(mx)
+        for a in (mx, my, mz, markk):
+            if a.dtype != np.float32 or not a.flags.c_contiguous or len(a) != n:
+                raise ValueError("marker arrays: contiguous float32 of equal length")
+        mt = None if markt is None else np.ascontiguousarray(markt, dtype=np.int32)
+        check(lib().stk_heat_markers(self.h, n, _fp(mx), _fp(my), _fp(mz), _ip(mt) if mt is not None else None,
+                                     _fp(markk), float(heatdif), float(timestep), float(zdeep), float(tdeep)),
+              "stk_heat_markers")
 
     def heat_titerate(self, timestep, opts: HeatOpts = None, raise_on_fail=True, **kw):
         o = opts if opts is not None else heat_default_opts(**kw)

@@ -531,4 +531,182 @@ __global__ void __launch_bounds__(256) k_heat_sources(HeatGeo hg, AdvGeo ag, con
   tk1[hn] = (float)ival;
 }
 
+// ---- marker temperatures after the step (heat3mg.c:618-950) --------------
+// tkcalc1 (mark3mg.c:1212-1278): the marker's cell by the cell search,
+// trilinear node weights 1 - |d| per axis multiplied x * y * z, z outer,
+// x, y inner; tk (new) and tk2 (the step's start) interpolated
+struct HeatMark {
+  int c1, c2, c3;
+  double dx, dy, dz;
+  double tn, to;
+};
+__device__ __forceinline__ double tk_w(const double *g, int m, double p, double d) {
+  double t = (g[m] - p) / d;
+  return 1.0 - fabs(t);
+}
+__device__ __forceinline__ HeatMark heat_tkcalc(const HeatGeo &hg, const AdvGeo &ag, const float *__restrict__ tk,
+                                               const float *__restrict__ tk2, double x, double y, double z) {
+  HeatMark r;
+  r.c1 = adv_serch(ag.gx, ag.X, ag.bx, x);
+  r.c2 = adv_serch(ag.gy, ag.Y, ag.by, y);
+  r.c3 = adv_serch(ag.gz, ag.Z, ag.bz, z);
+  r.dx = ag.gx[r.c1 + 1] - ag.gx[r.c1];
+  r.dy = ag.gy[r.c2 + 1] - ag.gy[r.c2];
+  r.dz = ag.gz[r.c3 + 1] - ag.gz[r.c3];
+  double a = 0.0, b = 0.0;
+  for (int m3 = r.c3; m3 <= r.c3 + 1; m3++) {
+    const double ddz = tk_w(ag.gz, m3, z, r.dz);
+    for (int m1 = r.c1; m1 <= r.c1 + 1; m1++) {
+      const double ddx = tk_w(ag.gx, m1, x, r.dx);
+      for (int m2 = r.c2; m2 <= r.c2 + 1; m2++) {
+        const double ddy = tk_w(ag.gy, m2, y, r.dy);
+        const double swt = ddx * ddy * ddz;
+        const long h = hidx(hg.X, hg.Y, hg.hy, m1, m2, m3);
+        a += swt * tk[h];
+        b += swt * tk2[h];
+      }
+    }
+  }
+  r.tn = a;
+  r.to = b;
+  return r;
+}
+
+__device__ __forceinline__ bool mark_inside(const AdvGeo &ag, float x, float y, float z) {
+  return x > 0 && y > 0 && z > 0 && (double)x < ag.gx[ag.X - 1] && (double)y < ag.gy[ag.Y - 1] &&
+         (double)z < ag.gz[ag.Z - 1];
+}
+
+// per marker: fluid markers (50 <= type < 100) inside take the new
+// temperature; normal ones (type < 50) strictly inside: markk <= 0 takes the
+// old one (tdeep where y > zdeep and it is colder), the numerical diffusion
+// (heatdif: the boundary-cell cap at the 8 nodes' largest tk, dtk1 =
+// (Told - markk)(1 - exp(-heatdif kt/(rho cp) 2 (1/dx^2+1/dy^2+1/dz^2) dt)),
+// the exponent clamped at -150, added to the marker; dtk1 kept for the
+// nodes' gather), then the nodal change Tnew - Told
+__global__ void __launch_bounds__(256) k_heat_mark_a(HeatGeo hg, AdvGeo ag, long nm, const float *__restrict__ mx,
+                                                     const float *__restrict__ my, const float *__restrict__ mz,
+                                                     const int *__restrict__ markt, float *markk,
+                                                     const float *__restrict__ tk, const float *__restrict__ tk2,
+                                                     const float *__restrict__ kt, const float *__restrict__ cp,
+                                                     const float *__restrict__ ro, double heatdif, double dt,
+                                                     double zdeep, double tdeep, unsigned char *flag, double *dtk1o) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  flag[i] = 0;
+  const int t = markt ? markt[i] : 0;
+  const float fx = mx[i], fy = my[i], fz = mz[i];
+  if (!mark_inside(ag, fx, fy, fz) || t >= 100) return;
+  const HeatMark r = heat_tkcalc(hg, ag, tk, tk2, fx, fy, fz);
+  if (t >= 50) {
+    markk[i] = (float)r.tn;
+    return;
+  }
+  float mk = markk[i];
+  if (mk <= 0) {
+    mk = (float)r.to;
+    if (fy > zdeep && mk < tdeep) mk = (float)tdeep;
+  }
+  if (heatdif != 0.0) {
+    double mkt = 0.0, mcp = 0.0, mro = 0.0;
+    double mtk = tk[hidx(hg.X, hg.Y, hg.hy, r.c1, r.c2, r.c3)];
+    for (int m3 = r.c3; m3 <= r.c3 + 1; m3++) {
+      const double ddz = tk_w(ag.gz, m3, fz, r.dz);
+      for (int m1 = r.c1; m1 <= r.c1 + 1; m1++) {
+        const double ddx = tk_w(ag.gx, m1, fx, r.dx);
+        for (int m2 = r.c2; m2 <= r.c2 + 1; m2++) {
+          const double w = ddx * tk_w(ag.gy, m2, fy, r.dy) * ddz;
+          const long h = hidx(hg.X, hg.Y, hg.hy, m1, m2, m3);
+          mkt += kt[h] * w;
+          mcp += cp[h] * w;
+          mro += ro[h] * w;
+          mtk = fmax(mtk, (double)tk[h]);
+        }
+      }
+    }
+    if ((r.c1 <= 0 || r.c1 >= ag.X - 2 || r.c2 <= 0 || r.c2 >= ag.Y - 2 || r.c3 <= 0 || r.c3 >= ag.Z - 2) &&
+        ((double)mk + r.tn - r.to) > mtk)
+      mk = (float)(mtk - r.tn + r.to);
+    const double dtk = r.to - (double)mk;
+    double rocpkf = mro * mcp;
+    if (rocpkf < 2e+6) rocpkf = 2e+6;
+    double d1 = -heatdif * mkt / rocpkf * 2.0 * (1.0 / r.dx / r.dx + 1.0 / r.dy / r.dy + 1.0 / r.dz / r.dz) * dt;
+    if (d1 < -150.0) d1 = -150.0;
+    d1 = dtk * (1.0 - exp(d1));
+    mk = mk + (float)d1;
+    dtk1o[i] = d1;
+    flag[i] = 1;
+  }
+  mk = mk + (float)(r.tn - r.to);
+  markk[i] = mk;
+}
+
+// per node: Sol0 = sum w, Sol1 = sum dtk1 w over the diffusing markers of
+// the 8 cells around it (cells in (z, x, y) order, markers of a cell in the
+// sorted order -- the input order for markers stored cell by cell, the
+// reference's single-thread accumulation order), then Sol1 / Sol0 (natural
+// node order)
+__global__ void __launch_bounds__(256) k_heat_mark_gather(AdvGeo ag, const int *__restrict__ start,
+                                                          const int *__restrict__ sidx,
+                                                          const float *__restrict__ mx, const float *__restrict__ my,
+                                                          const float *__restrict__ mz,
+                                                          const unsigned char *__restrict__ flag,
+                                                          const double *__restrict__ dtk1, double *sol1) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= ag.n0) return;
+  int m1, m2, m3;
+  decode(n, ag.X, ag.Y, ag.XY, m1, m2, m3);
+  double s0 = 0.0, s1 = 0.0;
+  for (int c3 = m3 - 1; c3 <= m3; c3++) {
+    if (c3 < 0 || c3 > ag.Z - 2) continue;
+    for (int c1 = m1 - 1; c1 <= m1; c1++) {
+      if (c1 < 0 || c1 > ag.X - 2) continue;
+      for (int c2 = m2 - 1; c2 <= m2; c2++) {
+        if (c2 < 0 || c2 > ag.Y - 2) continue;
+        const int cell = (c3 * (ag.X - 1) + c1) * (ag.Y - 1) + c2;
+        for (int k = start[cell]; k < start[cell + 1]; k++) {
+          const int i = sidx[k];
+          if (!flag[i]) continue;
+          const double ddz = tk_w(ag.gz, m3, mz[i], ag.gz[c3 + 1] - ag.gz[c3]);
+          const double ddx = tk_w(ag.gx, m1, mx[i], ag.gx[c1 + 1] - ag.gx[c1]);
+          const double ddy = tk_w(ag.gy, m2, my[i], ag.gy[c2 + 1] - ag.gy[c2]);
+          const double w = ddx * ddy * ddz;
+          s1 += dtk1[i] * w;
+          s0 += w;
+        }
+      }
+    }
+  }
+  sol1[n] = s0 != 0.0 ? s1 / s0 : s1;
+}
+
+// antidiffusion: a normal marker inside loses sum w Sol1 over its nodes with
+// Sol1 != 0 (heat3mg.c:795-830)
+__global__ void __launch_bounds__(256) k_heat_mark_anti(AdvGeo ag, long nm, const float *__restrict__ mx,
+                                                        const float *__restrict__ my, const float *__restrict__ mz,
+                                                        const int *__restrict__ markt, float *markk,
+                                                        const double *__restrict__ sol1) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  const int t = markt ? markt[i] : 0;
+  const float fx = mx[i], fy = my[i], fz = mz[i];
+  if (t >= 50 || !mark_inside(ag, fx, fy, fz)) return;
+  const int c1 = adv_serch(ag.gx, ag.X, ag.bx, fx), c2 = adv_serch(ag.gy, ag.Y, ag.by, fy),
+            c3 = adv_serch(ag.gz, ag.Z, ag.bz, fz);
+  const double dx = ag.gx[c1 + 1] - ag.gx[c1], dy = ag.gy[c2 + 1] - ag.gy[c2], dz = ag.gz[c3 + 1] - ag.gz[c3];
+  float mk = markk[i];
+  for (int m3 = c3; m3 <= c3 + 1; m3++) {
+    const double ddz = tk_w(ag.gz, m3, fz, dz);
+    for (int m1 = c1; m1 <= c1 + 1; m1++) {
+      const double ddx = tk_w(ag.gx, m1, fx, dx);
+      for (int m2 = c2; m2 <= c2 + 1; m2++) {
+        const double w = ddx * tk_w(ag.gy, m2, fy, dy) * ddz;
+        const double s = sol1[(long)m3 * ag.XY + (long)m1 * ag.Y + m2];
+        if (s != 0.0) mk = mk - (float)(s * w);
+      }
+    }
+  }
+  markk[i] = mk;
+}
+
 }  // namespace stk

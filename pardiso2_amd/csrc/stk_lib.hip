@@ -216,6 +216,7 @@ struct HeatState {
   float *ro = nullptr, *cp = nullptr, *kt = nullptr, *rcp = nullptr;  // every level
   float *tk = nullptr, *tk0 = nullptr, *tk1 = nullptr;                // level 0
   float *ht = nullptr;                  // the caller's source term (tk1 = ht + heating terms)
+  float *tk2 = nullptr;                 // the temperature at the start of the last step
   double *x = nullptr, *b = nullptr, *r = nullptr;                    // sol0, fre1, fre0 of every level
   double *part = nullptr;               // block partials of the reductions
   std::vector<double> hpart;
@@ -1413,7 +1414,7 @@ void heat_free(HeatState *h) {
   if (!h) return;
   for (int l = 0; l < MAXLV; l++) h->lv[l].release();
   dfree(h->ro); dfree(h->cp); dfree(h->kt); dfree(h->rcp);
-  dfree(h->tk); dfree(h->tk0); dfree(h->tk1); dfree(h->ht);
+  dfree(h->tk); dfree(h->tk0); dfree(h->tk1); dfree(h->ht); dfree(h->tk2);
   dfree(h->x); dfree(h->b); dfree(h->r); dfree(h->part);
   delete h;
 }
@@ -1425,7 +1426,8 @@ int heat_alloc(stk_ctx *c) {
   // every level's rows read all of its level-0 sized partial buffer at most
   const long np = 4 * (2 * (long)blocks(n0, 256) + 2);
   if (dalloc(&h->ro, nt) || dalloc(&h->cp, nt) || dalloc(&h->kt, nt) || dalloc(&h->rcp, nt) ||
-      dalloc(&h->tk, n0) || dalloc(&h->tk0, n0) || dalloc(&h->tk1, n0) || dalloc(&h->ht, n0) || dalloc(&h->x, nt) ||
+      dalloc(&h->tk, n0) || dalloc(&h->tk0, n0) || dalloc(&h->tk1, n0) || dalloc(&h->ht, n0) ||
+      dalloc(&h->tk2, n0) || dalloc(&h->x, nt) ||
       dalloc(&h->b, nt) || dalloc(&h->r, nt) || dalloc(&h->part, np)) {
     heat_free(h);
     return STK_ENOMEM;
@@ -3783,8 +3785,8 @@ int stk_heat_set_fields(stk_ctx *c, const float *ro, const float *cp, const floa
   HeatState *h = c->heat;
   const Level &F = c->lv[0];
   std::vector<float> sp(n0);
-  const std::pair<const float *, float *> up[7] = {{ro, h->ro}, {cp, h->cp}, {kt, h->kt}, {tk, h->tk},
-                                                   {tk, h->tk0}, {ht, h->tk1}, {ht, h->ht}};
+  const std::pair<const float *, float *> up[8] = {{ro, h->ro}, {cp, h->cp}, {kt, h->kt}, {tk, h->tk},
+                                                   {tk, h->tk0}, {ht, h->tk1}, {ht, h->ht}, {tk, h->tk2}};
   for (const auto &u : up) {
     to_split(u.first, sp.data(), F.X, F.Y, F.Z);
     H2D(c, u.second, sp.data(), n0 * 4);
@@ -3863,6 +3865,76 @@ int stk_heat_sources(stk_ctx *c, const float *et, int adiabyn, int frictyn) {
   (void)hipStreamSynchronize(c->st);
   dfree(et_d);
   dfree(v3);
+  return rc;
+}
+
+int stk_heat_markers(stk_ctx *c, long nmark, const float *mx, const float *my, const float *mz, const int *markt,
+                     float *markk, double heatdif, double timestep, double zdeep, double tdeep) {
+  if (!c || !c->heat || !c->heat->fields) return STK_ESTATE;
+  if (nmark < 0 || (nmark && (!mx || !my || !mz || !markk)) || !std::isfinite(heatdif) || !std::isfinite(timestep))
+    return STK_EINVAL;
+  if (nmark == 0) return STK_OK;
+  if (nmark >= (1l << 31)) return STK_EINVAL;
+  int rc = heat_build(c, std::max(c->heat->built, 0));
+  if (rc) return rc;
+  HeatState *h = c->heat;
+  const Level &F = c->lv[0];
+  const int ncell = (F.X - 1) * (F.Y - 1) * (F.Z - 1);
+  AdvGeo ag;
+  rc = adv_geo(c, ag);
+  if (rc) return rc;
+  float *d[4] = {};
+  int *dt = nullptr, *key = nullptr, *val = nullptr, *skey = nullptr, *sval = nullptr, *start = nullptr;
+  unsigned char *flag = nullptr;
+  double *dtk1 = nullptr, *sol1 = nullptr;
+  void *tmp = nullptr;
+  size_t tmpb = 0;
+  const float *hsrc[4] = {mx, my, mz, markk};
+  for (int k = 0; k < 4 && !rc; k++) {
+    rc = dalloc(&d[k], nmark);
+    if (!rc) rc = hipMemcpyAsync(d[k], hsrc[k], nmark * sizeof(float), hipMemcpyHostToDevice, c->st) ? STK_EHIP : 0;
+  }
+  if (!rc && markt) {
+    rc = dalloc(&dt, nmark);
+    if (!rc) rc = hipMemcpyAsync(dt, markt, nmark * sizeof(int), hipMemcpyHostToDevice, c->st) ? STK_EHIP : 0;
+  }
+  if (!rc && (dalloc(&flag, nmark) || dalloc(&dtk1, nmark))) rc = STK_ENOMEM;
+  if (!rc)
+    hipLaunchKernelGGL(k_heat_mark_a, dim3(blocks(nmark, 256)), dim3(256), 0, c->st, heat_geo(c, 0), ag, nmark, d[0],
+                       d[1], d[2], dt, d[3], h->tk, h->tk2, h->kt, h->cp, h->ro, heatdif, timestep, zdeep, tdeep, flag,
+                       dtk1);
+  if (!rc && heatdif != 0.0) {
+    // markers by cell (stable radix sort of the cell keys), then the nodes' gather
+    if (dalloc(&key, nmark) || dalloc(&val, nmark) || dalloc(&skey, nmark) || dalloc(&sval, nmark) ||
+        dalloc(&start, (size_t)ncell + 2) || dalloc(&sol1, F.n))
+      rc = STK_ENOMEM;
+    if (!rc) {
+      hipLaunchKernelGGL(k_marker_cells, dim3(blocks(nmark, 256)), dim3(256), 0, c->st, nmark, d[0], d[1], d[2],
+                         F.g64[0], F.g64[1], F.g64[2], F.X, F.Y, F.Z, key, val);
+      rc = hipcub::DeviceRadixSort::SortPairs(nullptr, tmpb, key, skey, val, sval, (int)nmark, 0, 32, c->st)
+               ? STK_EHIP : 0;
+      if (!rc) rc = dalloc((unsigned char **)&tmp, tmpb);
+      if (!rc)
+        rc = hipcub::DeviceRadixSort::SortPairs(tmp, tmpb, key, skey, val, sval, (int)nmark, 0, 32, c->st)
+                 ? STK_EHIP : 0;
+    }
+    if (!rc) {
+      hipLaunchKernelGGL(k_cell_start, dim3(blocks(nmark + 1, 256)), dim3(256), 0, c->st, nmark, skey, ncell + 1,
+                         start);
+      hipLaunchKernelGGL(k_heat_mark_gather, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, ag, start, sval, d[0],
+                         d[1], d[2], flag, dtk1, sol1);
+      hipLaunchKernelGGL(k_heat_mark_anti, dim3(blocks(nmark, 256)), dim3(256), 0, c->st, ag, nmark, d[0], d[1],
+                         d[2], dt, d[3], sol1);
+    }
+  }
+  if (!rc) rc = hipGetLastError() ? STK_EHIP : 0;
+  if (!rc) rc = hipMemcpyAsync(markk, d[3], nmark * sizeof(float), hipMemcpyDeviceToHost, c->st) ? STK_EHIP : 0;
+  if (!rc) rc = hipStreamSynchronize(c->st) ? STK_EHIP : 0;
+  (void)hipStreamSynchronize(c->st);
+  for (int k = 0; k < 4; k++) dfree(d[k]);
+  dfree(dt); dfree(key); dfree(val); dfree(skey); dfree(sval); dfree(start); dfree(flag); dfree(dtk1); dfree(sol1);
+  unsigned char *tb = (unsigned char *)tmp;
+  dfree(tb);
   return rc;
 }
 
@@ -4018,6 +4090,8 @@ int stk_heat_titerate(stk_ctx *c, double timestep, const stk_heat_opts *o, stk_h
   S.tkstep = 1e+30;
   if (rate) S.tkstep = o->maxtkstep / rate;
   if (timestep != 0 && S.tkstep != 0) {
+    // the step's starting temperature, for the markers' update (heat3mg.c:377-381)
+    HIPCHK(hipMemcpyAsync(h->tk2, h->tk, c->lv[0].n * sizeof(float), hipMemcpyDeviceToDevice, c->st));
     heat_props(c, o->levels);
     h->Lt = o->levels;
     double tsum = 0;

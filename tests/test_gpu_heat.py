@@ -265,3 +265,51 @@ def test_sources_from_the_stokes_solution(oracle_mod, periodic):
         np.testing.assert_array_equal(d.heat_fields()["tk1"][heat], b[heat])
     finally:
         d.close()
+
+
+def _cell_major_markers(g, per_cell, rng):
+    n = len(g)
+    out = []
+    for k in range(n - 1):
+        for i in range(n - 1):
+            for j in range(n - 1):
+                u = rng.uniform(0.05, 0.95, (per_cell, 3))
+                out.append(np.stack([g[i] + u[:, 0] * (g[i + 1] - g[i]), g[j] + u[:, 1] * (g[j + 1] - g[j]),
+                                     g[k] + u[:, 2] * (g[k + 1] - g[k])], axis=1))
+    p = np.concatenate(out)
+    return [np.ascontiguousarray(p[:, a], dtype=np.float32) for a in range(3)]
+
+
+@pytest.mark.parametrize("heatdif", [0.0, 1.0])
+def test_marker_temperatures_after_the_step(oracle_mod, heatdif):
+    """titerate's marker update (heat3mg.c:618-950) after a temperature step, markers stored cell by
+    cell (the node sums then run in the reference's order): bit-identical to the oracle without
+    diffusion; with it, equal except where exp differs by an ULP between the device and the host
+    (at most 2 FP32 ULPs, almost all bit-identical); shuffled markers agree to the sums' rounding."""
+    hp, o, h, d = _pair(oracle_mod, n=17, Lt=2)
+    kw = dict(levels=2, pre=[3, 3, 3], post=[0, 3, 3], heatmin=1e-5, maxit=100)
+    ref = h.titerate(hp.timestep, 25.0, kw["pre"], kw["post"], heatmin=kw["heatmin"], maxit=kw["maxit"])
+    st = d.heat_titerate(hp.timestep, **kw)
+    assert st.iterations == ref["iterations"]
+    rng = np.random.default_rng(11)
+    mx, my, mz = _cell_major_markers(hp.gx, 3, rng)
+    M = len(mx)
+    t = rng.choice([0, 0, 0, 60, 120], M).astype(np.int32)
+    markk = rng.uniform(0.0, 1.0, M).astype(np.float32)
+    markk[rng.random(M) < 0.1] = -1.0
+    a, b = markk.copy(), markk.copy()
+    args = dict(heatdif=heatdif, timestep=hp.timestep, zdeep=0.5, tdeep=0.9)
+    d.heat_markers(mx, my, mz, t, a, **args)
+    h.markers(mx, my, mz, t, b, **args)
+    if heatdif == 0.0:
+        np.testing.assert_array_equal(a, b)
+    else:
+        ulp = np.spacing(np.abs(b).astype(np.float32))
+        assert np.all(np.abs(a - b) <= 2 * ulp), np.abs(a - b).max()
+        assert np.mean(a == b) > 0.99
+    assert np.abs(a - markk).max() > 0
+    # shuffled markers: the node sums run in another order
+    perm = rng.permutation(M)
+    a2 = markk[perm].copy()
+    d.heat_markers(mx[perm].copy(), my[perm].copy(), mz[perm].copy(), t[perm].copy(), a2, **args)
+    np.testing.assert_allclose(a2, b[perm], rtol=0, atol=1e-6)

@@ -258,3 +258,86 @@ def test_sources_closed_forms(oracle_mod):
         np.testing.assert_allclose(tk1[heat], want[heat].astype(np.float32), rtol=2e-6, atol=1e-12,
                                    err_msg=f"adiabyn {adiab} frictyn {frict}")
         np.testing.assert_array_equal(tk1[~heat], ht[~heat])
+
+
+def _mk_setup(oracle_mod, n=9, seed=9):
+    rng = np.random.default_rng(seed)
+    g = setups.uniform_grid(n, 1.0)
+    o = oracle_mod.Oracle(g, g.copy(), g.copy(), 0)
+    H = oracle_mod.Heat(o, 0)
+    N = n ** 3
+    zz, xx, yy = np.meshgrid(g, g, g, indexing="ij")
+    T0 = (0.2 + 0.5 * xx - 0.3 * yy + 0.8 * zz).ravel()   # linear: interpolated exactly
+    dT = 0.05
+    H.set_fields(np.full(N, 3000.0), np.full(N, 1000.0), np.full(N, 3.0), T0, np.zeros(N))
+    H.arr("tk")[:] = (T0 + dT).astype(np.float32)  # the step's result: a uniform change
+    return g, o, H, T0, dT, rng
+
+
+def _mk_markers(g, rng, per_cell=2):
+    n = len(g)
+    c = []
+    for k in range(n - 1):
+        for i in range(n - 1):
+            for j in range(n - 1):
+                for _ in range(per_cell):
+                    c.append((g[i] + rng.uniform(0.1, 0.9) * (g[i + 1] - g[i]), g[j] + rng.uniform(0.1, 0.9) * (g[j + 1] - g[j]),
+                              g[k] + rng.uniform(0.1, 0.9) * (g[k + 1] - g[k])))
+    p = np.array(c)
+    return [np.ascontiguousarray(p[:, a], dtype=np.float32) for a in range(3)]
+
+
+def _lin(p):
+    return 0.2 + 0.5 * p[0].astype(np.float64) - 0.3 * p[1] + 0.8 * p[2]
+
+
+def test_marker_temperatures_closed_forms(oracle_mod):
+    """titerate's marker update (heat3mg.c:618-950) with linear node temperatures: markers in
+    equilibrium with the grid gain the nodal change; fluid markers and new markers (markk <= 0) take
+    the interpolated new / old temperature (tdeep below zdeep); a uniform offset of all markers
+    survives the numerical diffusion (the antidiffusion returns the mean of what diffused); the
+    boundary cells cap a marker at the nodes' largest temperature."""
+    g, o, H, T0, dT, rng = _mk_setup(oracle_mod)
+    mx, my, mz = _mk_markers(g, rng)
+    M = len(mx)
+    Told = _lin((mx, my, mz))
+    # in equilibrium, with and without diffusion
+    for hd in (0.0, 1.0):
+        markk = Told.astype(np.float32)
+        H.markers(mx, my, mz, np.zeros(M, np.int32), markk, heatdif=hd, timestep=1e9)
+        np.testing.assert_allclose(markk, Told + dT, rtol=0, atol=2e-6)
+    # fluid markers take the new temperature, new markers the old one (or tdeep below zdeep)
+    t = np.where(np.arange(M) % 3 == 0, 60, 0).astype(np.int32)
+    markk = np.where(np.arange(M) % 3 == 1, -1.0, Told).astype(np.float32)
+    H.markers(mx, my, mz, t, markk, heatdif=0.0, zdeep=0.5, tdeep=0.9)
+    fluid, new = t == 60, (np.arange(M) % 3 == 1)
+    np.testing.assert_allclose(markk[fluid], Told[fluid] + dT, atol=2e-6)
+    deep = new & (my > 0.5) & (Told < 0.9)
+    np.testing.assert_allclose(markk[deep], 0.9 + dT, atol=2e-6)
+    np.testing.assert_allclose(markk[new & ~deep], Told[new & ~deep] + dT, atol=2e-6)
+    # a uniform offset survives diffusion + antidiffusion; an alternating one decays
+    for off, keeps in ((0.01, True), (None, False)):
+        d = np.full(M, 0.01) if off else np.where(np.arange(M) % 2 == 0, 0.01, -0.01)
+        markk = (Told + d).astype(np.float32)
+        H.markers(mx, my, mz, np.zeros(M, np.int32), markk, heatdif=1.0, timestep=1e3)
+        got = markk - (Told + dT)
+        # (markers whose nodes no boundary-cell marker reaches: those are capped, see below)
+        inner = (mx > g[2]) & (mx < g[-3]) & (my > g[2]) & (my < g[-3]) & (mz > g[2]) & (mz < g[-3])
+        if keeps:
+            np.testing.assert_allclose(got[inner], 0.01, atol=1e-5)
+        else:
+            # subgrid differences relax by exp(-K), K = heatdif kt / (rho cp) 2 (3 / h^2) dt
+            h = g[1] - g[0]
+            K = 1.0 * 3.0 / 3e6 * 2.0 * (3.0 / h ** 2) * 1e3
+            r = np.abs(got[inner]) / 0.01
+            assert np.all(np.abs(r - np.exp(-K)) < 0.1), (r.min(), r.max(), np.exp(-K))
+    # boundary cap
+    markk = Told.astype(np.float32)
+    k = int(np.argmin(mx))  # a marker in the first x cell
+    markk[k] = 50.0
+    H.markers(mx, my, mz, np.zeros(M, np.int32), markk, heatdif=1.0, timestep=1e-30)
+    assert markk[k] < T0.max() + dT + 1e-5
+    markk = Told.astype(np.float32)
+    markk[k] = 50.0
+    H.markers(mx, my, mz, np.zeros(M, np.int32), markk, heatdif=0.0)
+    assert markk[k] > 49.0  # no cap without diffusion
