@@ -214,7 +214,8 @@ int stk_stress_strain(stk_ctx *ctx, double *sig, double *eps);
  * invariant times timestep where positive; markk is set to 0 for a normal
  * marker that leaves the box; markt, the marker types (< 50 normal, >= 100
  * immobile; NULL: all normal); eii receives the strain-rate invariant at the
- * start position (the reference's val1).  The water-marker branch and the
+ * start position (the reference's val1) of every marker the step moves (the
+ * others keep their entries).  The water-marker branch and the
  * immobile-marker reset, both disabled in the reference, are not built.
  * Periodic axes (mark3mg.c:337-485, 2992-3760): the interpolations pair the
  * centres or cells across the wrap as the reference does, and the RK stage

@@ -369,15 +369,17 @@ class Stokes:
         return sig, eps
 
     def move_markers(self, mx, my, mz, marke=None, markk=None, markt=None, timestep=1.0, markmod=0, outgrid=0,
-                     want_eii=True):
+                     want_eii=True, eii=None):
         """One marker-advection step through the current solution
         (stk_move_markers): float32 arrays updated in place; returns the
-        strain-rate invariant at the start positions."""
+        strain-rate invariant at the start positions (into eii when given:
+        entries of markers the step skips keep their values)."""
         n = len(mx)
-        for a in (mx, my, mz, marke, markk):
+        for a in (mx, my, mz, marke, markk, eii):
             if a is not None and (a.dtype != np.float32 or not a.flags.c_contiguous or len(a) != n):
                 raise ValueError("marker arrays: contiguous float32 of equal length")
-        eii = np.zeros(n, np.float32) if want_eii else None
+        if eii is None:
+            eii = np.zeros(n, np.float32) if want_eii else None
         mt = None if markt is None else np.ascontiguousarray(markt, dtype=np.int32)
         opt = lambda a, f: f(a) if a is not None else None
         check(lib().stk_move_markers(self.h, n, _fp(mx), _fp(my), _fp(mz), opt(marke, _fp), opt(markk, _fp),

@@ -2610,7 +2610,11 @@ int stk_move_markers(stk_ctx *c, long nmark, float *mx, float *my, float *mz, fl
     rc = dalloc(&dt, nmark);
     if (!rc) rc = hipMemcpyAsync(dt, markt, nmark * sizeof(int), hipMemcpyHostToDevice, c->st) ? STK_EHIP : 0;
   }
-  if (!rc && eii) rc = dalloc(&de, nmark);
+  // eii of the markers the step skips keeps the caller's values
+  if (!rc && eii) {
+    rc = dalloc(&de, nmark);
+    if (!rc) rc = hipMemcpyAsync(de, eii, nmark * sizeof(float), hipMemcpyHostToDevice, c->st) ? STK_EHIP : 0;
+  }
   int *wrap_out = nullptr;
   const bool periodic = c->per[0] || c->per[1] || c->per[2];
   if (!rc && periodic) {

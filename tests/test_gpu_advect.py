@@ -159,3 +159,28 @@ def test_step_on_a_refined_grid_is_bit_identical_to_the_oracle(oracle_mod):
         _say(f"refined grid: {nm} markers bit-identical, with and without strain")
     finally:
         g.close()
+
+
+def test_skipped_markers_keep_their_eii():
+    """Markers the step does not move (outside fluid markers under outgrid 2) keep the caller's eii
+    entries: the device buffer starts from the caller's array, not from whatever it last held."""
+    n = 17
+    pb = setups.sinker(n, setups.coarsest_levels(n), seed=1)
+    g = S.from_problem(pb)
+    try:
+        g.hydrostatic_guess()
+        assert g.solve(precision=0, tol=1e-8).status == 0
+        m = 512
+        rng = np.random.default_rng(3)
+        p = [np.ascontiguousarray(rng.uniform(0.05, 0.95, m) * v, dtype=np.float32)
+             for v in (pb.gx[-1], pb.gy[-1], pb.gz[-1])]
+        p[2][: m // 2] = -0.01  # below the box
+        t = np.full(m, 60, np.int32)  # fluid: not moved while outside
+        sentinel = np.full(m, -7.0, np.float32)
+        for _ in range(3):  # reused device memory must not leak into the skipped entries
+            e = sentinel.copy()
+            g.move_markers(*[a.copy() for a in p], markt=t, timestep=1e-3, outgrid=2, eii=e)
+            np.testing.assert_array_equal(e[: m // 2], -7.0)
+            assert np.all(e[m // 2:] >= 0)
+    finally:
+        g.close()
