@@ -525,8 +525,9 @@ template <> const int *tuned_tile<float>(const Level &L) { return L.tune32[0] ? 
 template <> const int *tuned_tile<double>(const Level &L) { return L.tune64[0] ? L.tune64 : kTileDefault4; }
 
 // One launch of the tile smoother.  The 32x4x4 tile (the default, and the
-// usual choice on the finest levels) runs an instance with its shape compiled
-// in: the rows' LDS offsets become immediates and index divisions shifts.
+// usual choice on the finest levels) and the other tiles the autotuner picks
+// on large levels run instances with their shape compiled in: the rows' LDS
+// offsets become immediates and index divisions shifts.
 // tcs: the tile chains of this shape (the sweep sets the boundary rows), or
 // nullptr (a separate chain pass follows).
 template <typename T>
@@ -547,6 +548,8 @@ void launch_smooth(stk_ctx *c, const int *tile, bool sv, const TileGeo &tg, cons
       const FluxU fu = uni ? L->fu : FluxU{};
       const bool t348 = tile[0] == 32 && tile[1] == 4 && tile[2] == 8 && !g.per;
       const bool t384 = tile[0] == 32 && tile[1] == 8 && tile[2] == 4 && !g.per;
+      const bool t642 = tile[0] == 64 && tile[1] == 4 && tile[2] == 2 && !g.per;
+      const bool t624 = tile[0] == 64 && tile[1] == 2 && tile[2] == 4 && !g.per;
       const dim3 blk2(pairs <= 256 ? 256 : 512);
 #define STK_VSMOOTH(TY, TX, TZ, CHV, BS)                                                                      \
   do {                                                                                                        \
@@ -560,6 +563,8 @@ void launch_smooth(stk_ctx *c, const int *tile, bool sv, const TileGeo &tg, cons
       if (t344) { if (tcs) STK_VSMOOTH(32, 4, 4, true, 256); else STK_VSMOOTH(32, 4, 4, false, 256); }
       else if (t348) { if (tcs) STK_VSMOOTH(32, 4, 8, true, 512); else STK_VSMOOTH(32, 4, 8, false, 512); }
       else if (t384) { if (tcs) STK_VSMOOTH(32, 8, 4, true, 512); else STK_VSMOOTH(32, 8, 4, false, 512); }
+      else if (t642) { if (tcs) STK_VSMOOTH(64, 4, 2, true, 256); else STK_VSMOOTH(64, 4, 2, false, 256); }
+      else if (t624) { if (tcs) STK_VSMOOTH(64, 2, 4, true, 256); else STK_VSMOOTH(64, 2, 4, false, 256); }
       else if (pairs <= 256) { if (tcs) STK_VSMOOTH(0, 0, 0, true, 256); else STK_VSMOOTH(0, 0, 0, false, 256); }
       else { if (tcs) STK_VSMOOTH(0, 0, 0, true, 512); else STK_VSMOOTH(0, 0, 0, false, 512); }
 #undef STK_VSMOOTH
@@ -1270,7 +1275,17 @@ static double time_smooth(stk_ctx *c, int l, const int *tile, int reps) {
   const bool sv = tile[3] && smooth_lds<T>(tile, true) <= kLevelLds;
   Geo<T> g = L.geo<T>(c->kn.p2koef);
   T *x0 = lvx<T>(L), *x1 = lvxo<T>(L);
-  auto launch = [&]() { launch_smooth<T>(c, tile, sv, tg, g, L.flags, x0, x1, lvb<T>(L), 0); };
+  // as the V-cycle runs a sweep of this shape: boundary rows set inside the
+  // tiles (chains composed once per smoothing call) where the shape allows,
+  // else a chain pass after the sweep
+  const bool inplace = tile_bc(c, L, tile);
+  if (inplace)
+    hipLaunchKernelGGL(k_bc_compose<T>, dim3(blocks(L.nbc, 256)), dim3(256), 0, c->st, L.chain(), L.bt_idx,
+                       lvb<T>(L), chain_c<T>(L), chain_k<T>(L), 1);
+  auto launch = [&]() {
+    launch_smooth<T>(c, tile, sv, tg, g, L.flags, x0, x1, lvb<T>(L), 0, inplace ? tile_chains<T>(L) : nullptr);
+    if (!inplace) bc_passes<T>(c, L, lvb<T>(L), x1, 1);
+  };
   launch();
   hipEventRecord(c->ev0, c->st);
   for (int i = 0; i < reps; i++) launch();
