@@ -175,7 +175,9 @@ __device__ __forceinline__ double adv_sum8(const AdvGeo &g, AxW ax, AxW ay, AxW 
   return acc;
 }
 
-// velocity (v, if non-null) and strain rates (e, if non-null) at a point
+// velocity (v, if non-null) and strain rates (e, if non-null) at a point;
+// PER = false compiles the walled box only (the periodic branches fold away)
+template <bool PER = true>
 __device__ __forceinline__ void adv_interp(const AdvGeo &g, const float *__restrict__ v3,
                                            const float *__restrict__ e6, double px, double py, double pz, double *v,
                                            double *e) {
@@ -186,29 +188,30 @@ __device__ __forceinline__ void adv_interp(const AdvGeo &g, const float *__restr
             m3 = adv_serch(g.gz, g.Z, g.bz, pz);
   // each bracketing serves two of the three sums that use it (the vz sum
   // takes its own x bracketing on a periodic x axis: another spacing there)
+  const int px_ = PER ? g.px : 0, py_ = PER ? g.py : 0, pz_ = PER ? g.pz : 0;
   if (v) {
     const AxW nx = adv_axis<kAxVN>(g.gx, g.X, m1, px), ny = adv_axis<kAxVN>(g.gy, g.Y, m2, py),
               nz = adv_axis<kAxVN>(g.gz, g.Z, m3, pz);
-    const AxW cx = adv_axis<kAxVC>(g.gx, g.X, m1, px, g.px, g.xs),
-              cy = adv_axis<kAxVC>(g.gy, g.Y, m2, py, g.py, g.ys),
-              cz = adv_axis<kAxVC>(g.gz, g.Z, m3, pz, g.pz, g.zs);
+    const AxW cx = adv_axis<kAxVC>(g.gx, g.X, m1, px, px_, g.xs),
+              cy = adv_axis<kAxVC>(g.gy, g.Y, m2, py, py_, g.ys),
+              cz = adv_axis<kAxVC>(g.gz, g.Z, m3, pz, pz_, g.zs);
     v[0] = adv_sum8(g, nx, cy, cz, v3);
     v[1] = adv_sum8(g, cx, ny, cz, v3 + g.n0);
-    const AxW cxz = g.px ? adv_axis<kAxVC>(g.gx, g.X, m1, px, 1, g.xs, 1) : cx;
+    const AxW cxz = px_ ? adv_axis<kAxVC>(g.gx, g.X, m1, px, 1, g.xs, 1) : cx;
     v[2] = adv_sum8(g, cxz, cy, nz, v3 + 2 * g.n0);
   }
   if (e) {
     {
-      const AxW cx = adv_axis<kAxCell>(g.gx, g.X, m1, px, g.px, g.xs),
-                cy = adv_axis<kAxCell>(g.gy, g.Y, m2, py, g.py, g.ys),
-                cz = adv_axis<kAxCell>(g.gz, g.Z, m3, pz, g.pz, g.zs);
+      const AxW cx = adv_axis<kAxCell>(g.gx, g.X, m1, px, px_, g.xs),
+                cy = adv_axis<kAxCell>(g.gy, g.Y, m2, py, py_, g.ys),
+                cz = adv_axis<kAxCell>(g.gz, g.Z, m3, pz, pz_, g.zs);
       for (int k = 0; k < 3; k++) e[k] = adv_sum8(g, cx, cy, cz, e6 + (long)k * g.n0);
     }
-    const AxW nx = adv_axis<kAxSN>(g.gx, g.X, m1, px, g.px), ny = adv_axis<kAxSN>(g.gy, g.Y, m2, py, g.py, 0.0, 0, 1),
-              nz = adv_axis<kAxSN>(g.gz, g.Z, m3, pz, g.pz);
-    const AxW cx = adv_axis<kAxSC>(g.gx, g.X, m1, px, g.px, g.xs),
-              cy = adv_axis<kAxSC>(g.gy, g.Y, m2, py, g.py, g.ys),
-              cz = adv_axis<kAxSC>(g.gz, g.Z, m3, pz, g.pz, g.zs);
+    const AxW nx = adv_axis<kAxSN>(g.gx, g.X, m1, px, px_), ny = adv_axis<kAxSN>(g.gy, g.Y, m2, py, py_, 0.0, 0, 1),
+              nz = adv_axis<kAxSN>(g.gz, g.Z, m3, pz, pz_);
+    const AxW cx = adv_axis<kAxSC>(g.gx, g.X, m1, px, px_, g.xs),
+              cy = adv_axis<kAxSC>(g.gy, g.Y, m2, py, py_, g.ys),
+              cz = adv_axis<kAxSC>(g.gz, g.Z, m3, pz, pz_, g.zs);
     e[3] = adv_sum8(g, nx, ny, cz, e6 + 3 * g.n0);
     e[4] = adv_sum8(g, nx, cy, nz, e6 + 4 * g.n0);
     e[5] = adv_sum8(g, cx, ny, nz, e6 + 5 * g.n0);
@@ -227,7 +230,8 @@ __device__ __forceinline__ double adv_eii(const double *e) {
 // the motion across the faces an outside marker lies beyond; a normal
 // marker (type < 100) leaving the box gets markk = 0.  e6 may be null when
 // neither eii nor marke is asked for (the strain rates are then not read).
-__global__ void __launch_bounds__(256) k_move_markers(AdvGeo g, const float *__restrict__ v3,
+template <bool PER>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_move_markers(AdvGeo g, const float *__restrict__ v3,
                                                       const float *__restrict__ e6, long nmark, float *mx,
                                                       float *my, float *mz, float *marke, float *markk,
                                                       const int *__restrict__ markt, float *eii, double dt,
@@ -240,7 +244,7 @@ __global__ void __launch_bounds__(256) k_move_markers(AdvGeo g, const float *__r
   if (!(inside || (outgrid != 1 && (t < 50 || t >= 100)))) return;
   const bool wantE = e6 && marke && marke[i] > 0;
   double v1[3], e[6], vv[3];
-  adv_interp(g, v3, e6, fx, fy, fz, v1, e6 ? e : nullptr);
+  adv_interp<PER>(g, v3, e6, fx, fy, fz, v1, e6 ? e : nullptr);
   const double ee1 = e6 ? adv_eii(e) : 0.0;
   if (eii) eii[i] = (float)ee1;
   double ee0;
@@ -249,42 +253,40 @@ __global__ void __launch_bounds__(256) k_move_markers(AdvGeo g, const float *__r
     vv[0] = v1[0]; vv[1] = v1[1]; vv[2] = v1[2];
     ee0 = ee1;
   } else {
-    double vs[3][3], es[4] = {ee1, 0.0, 0.0, 0.0};
-    const double *prev = v1;
+    // stage points (mark3mg.c:333-335, 363-365, 393-395), wrapped on periodic axes (:337-355;
+    // the z test compares with the x size, as the reference's does)
+    auto stage = [&](const double *vin, bool half, double *vout) -> double {
+      double P[3] = {half ? fx + vin[0] * dt / 2.0 : fx + vin[0] * dt, half ? fy + vin[1] * dt / 2.0 : fy + vin[1] * dt,
+                     half ? fz + vin[2] * dt / 2.0 : fz + vin[2] * dt};
+      if (PER) {
+        const double sz[3] = {g.xs, g.ys, g.zs}, cmp[3] = {g.xs, g.ys, g.xs};
+        const int pr[3] = {g.px, g.py, g.pz};
 #pragma unroll
-    for (int s = 0; s < 3; s++) {
-      // stage points (mark3mg.c:333-335, 363-365, 393-395), wrapped on periodic axes (:337-355;
-      // the z test compares with the x size, as the reference's does)
-      double P[3];
-      if (s < 2) {
-        P[0] = fx + prev[0] * dt / 2.0; P[1] = fy + prev[1] * dt / 2.0; P[2] = fz + prev[2] * dt / 2.0;
-      } else {
-        P[0] = fx + prev[0] * dt; P[1] = fy + prev[1] * dt; P[2] = fz + prev[2] * dt;
+        for (int a = 0; a < 3; a++)
+          if (pr[a]) {
+            if (P[a] < 0) P[a] = sz[a] + P[a];
+            if (P[a] > cmp[a]) P[a] = P[a] - sz[a];
+            out |= P[a] < 0 || P[a] > sz[a];
+          }
       }
-      const double sz[3] = {g.xs, g.ys, g.zs}, cmp[3] = {g.xs, g.ys, g.xs};
-      const int pr[3] = {g.px, g.py, g.pz};
-#pragma unroll
-      for (int a = 0; a < 3; a++)
-        if (pr[a]) {
-          if (P[a] < 0) P[a] = sz[a] + P[a];
-          if (P[a] > cmp[a]) P[a] = P[a] - sz[a];
-          out |= P[a] < 0 || P[a] > sz[a];
-        }
-      adv_interp(g, v3, e6, P[0], P[1], P[2], vs[s], wantE ? e : nullptr);
-      if (wantE) es[s + 1] = adv_eii(e);
-      prev = vs[s];
-    }
-    for (int k = 0; k < 3; k++) vv[k] = (v1[k] + 2.0 * vs[0][k] + 2.0 * vs[1][k] + vs[2][k]) / 6.0;
-    ee0 = (es[0] + 2.0 * es[1] + 2.0 * es[2] + es[3]) / 6.0;
+      adv_interp<PER>(g, v3, e6, P[0], P[1], P[2], vout, wantE ? e : nullptr);
+      return wantE ? adv_eii(e) : 0.0;
+    };
+    double v2[3], w3[3], v4[3];
+    const double ee2 = stage(v1, true, v2);
+    const double ee3 = stage(v2, true, w3);
+    const double ee4 = stage(w3, false, v4);
+    for (int k = 0; k < 3; k++) vv[k] = (v1[k] + 2.0 * v2[k] + 2.0 * w3[k] + v4[k]) / 6.0;
+    ee0 = (ee1 + 2.0 * ee2 + 2.0 * ee3 + ee4) / 6.0;
   }
   if (outgrid == 2) {  // orthogonal motion only (mark3mg.c:433-457): one periodic branch, the first set
-    if (g.px) {
+    if (PER && g.px) {
       if (fy < 0 || (double)fy > g.ys) vv[0] = vv[2] = 0;
       if (fz < 0 || (double)fz > g.zs) vv[0] = vv[1] = 0;
-    } else if (g.py) {
+    } else if (PER && g.py) {
       if (fx < 0 || (double)fx > g.xs) vv[1] = vv[2] = 0;
       if (fz < 0 || (double)fz > g.zs) vv[0] = vv[1] = 0;
-    } else if (g.pz) {
+    } else if (PER && g.pz) {
       if (fx < 0 || (double)fx > g.xs) vv[1] = vv[2] = 0;
       if (fy < 0 || (double)fy > g.zs) vv[0] = vv[2] = 0;
     } else {
@@ -295,22 +297,22 @@ __global__ void __launch_bounds__(256) k_move_markers(AdvGeo g, const float *__r
   }
   float nx = fx + dt * vv[0], ny = fy + dt * vv[1], nz = fz + dt * vv[2];
   // periodic wrap of the moved marker (mark3mg.c:468-485)
-  if (g.px) {
+  if (PER && g.px) {
     if (nx < 0) nx = (float)(g.xs + nx);
     if (nx > g.xs) nx = (float)(nx - g.xs);
     out |= nx < 0 || nx > g.xs;
   }
-  if (g.py) {
+  if (PER && g.py) {
     if (ny < 0) ny = (float)(g.ys + ny);
     if (ny > g.ys) ny = (float)(ny - g.ys);
     out |= ny < 0 || ny > g.ys;
   }
-  if (g.pz) {
+  if (PER && g.pz) {
     if (nz < 0) nz = (float)(g.zs + nz);
     if (nz > g.zs) nz = (float)(nz - g.zs);
     out |= nz < 0 || nz > g.zs;
   }
-  if (out && g.wrap_out) g.wrap_out[0] = 1;
+  if (PER && out && g.wrap_out) g.wrap_out[0] = 1;
   mx[i] = nx;
   my[i] = ny;
   mz[i] = nz;

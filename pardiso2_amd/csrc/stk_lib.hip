@@ -2640,9 +2640,14 @@ int stk_move_markers(stk_ctx *c, long nmark, float *mx, float *my, float *mz, fl
   if (!rc) rc = adv_geo(c, g);
   if (!rc) {
     g.wrap_out = wrap_out;
-    hipLaunchKernelGGL(k_move_markers, dim3(blocks(nmark, 256)), dim3(256), 0, c->st, g, planes,
-                       need_e ? planes + 3 * F.n : nullptr, nmark, d[0], d[1], d[2], d[3], d[4], dt, de, timestep,
-                       markmod, outgrid);
+    if (g.px || g.py || g.pz)
+      hipLaunchKernelGGL(k_move_markers<true>, dim3(blocks(nmark, 256)), dim3(256), 0, c->st, g, planes,
+                         need_e ? planes + 3 * F.n : nullptr, nmark, d[0], d[1], d[2], d[3], d[4], dt, de, timestep,
+                         markmod, outgrid);
+    else
+      hipLaunchKernelGGL(k_move_markers<false>, dim3(blocks(nmark, 256)), dim3(256), 0, c->st, g, planes,
+                         need_e ? planes + 3 * F.n : nullptr, nmark, d[0], d[1], d[2], d[3], d[4], dt, de, timestep,
+                         markmod, outgrid);
     for (int k = 0; k < 5 && !rc; k++)
       if (h[k]) rc = hipMemcpyAsync(h[k], d[k], nmark * sizeof(float), hipMemcpyDeviceToHost, c->st) ? STK_EHIP : 0;
     if (!rc && eii) rc = hipMemcpyAsync(eii, de, nmark * sizeof(float), hipMemcpyDeviceToHost, c->st) ? STK_EHIP : 0;

@@ -85,7 +85,7 @@ def trace(d, kernel, last):
           f"median us {ds[len(ds) // 2] / 1e3:.3f}  max us {ds[-1] / 1e3:.3f}")
 
 
-def window(d, since, nth, top, by_grid=False):
+def window(d, since, nth, top, by_grid=False, through=None):
     rows = []
     for f in _find(d, "kernel_trace.csv"):
         with open(f) as fh:
@@ -99,6 +99,10 @@ def window(d, since, nth, top, by_grid=False):
     if len(marks) < nth:
         raise SystemExit(f"fewer than {nth} dispatches of {since}")
     rows = rows[marks[-nth]:]
+    if through:  # up to and including the first later dispatch of this kernel
+        ends = [i for i, r in enumerate(rows) if i > 0 and through in r[2]]
+        if ends:
+            rows = rows[:ends[0] + 1]
     span = rows[-1][1] - rows[0][0]
     per = defaultdict(lambda: [0, 0.0])
     for t0, t1, name in rows:
@@ -179,6 +183,7 @@ def main(argv=None):
     ap.add_argument("--kernel", default=None)
     ap.add_argument("--last", type=int, default=0)
     ap.add_argument("--since", default=None)
+    ap.add_argument("--through", default=None, help="window: end with the first later dispatch of this kernel")
     ap.add_argument("--nth", type=int, default=1)
     ap.add_argument("--by-grid", action="store_true", help="window: one line per kernel and launch shape")
     ap.add_argument("--nodes", type=int, default=0, help="traffic: finest-level nodes of the run")
@@ -192,7 +197,7 @@ def main(argv=None):
     elif a.mode == "trace":
         trace(a.dir, a.kernel or "", a.last)
     elif a.mode == "window":
-        window(a.dir, a.since or "", a.nth, a.top, a.by_grid)
+        window(a.dir, a.since or "", a.nth, a.top, a.by_grid, a.through)
     elif a.mode == "traffic":
         traffic(a.dir, a.nodes, a.entry, a.last or 10, a.out, a.copy_per_node)
     else:

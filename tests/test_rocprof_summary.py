@@ -22,6 +22,11 @@ def test_window_from_the_kth_last_marker(tmp_path, capsys):
     R.window(str(tmp_path), "mark", 2, 10)
     out = capsys.readouterr().out
     assert "window: 5 dispatches" in out
+    # ended with the first later dispatch of "a": the last marker and a (50 ns of a 50 ns span)
+    R.window(str(tmp_path), "mark", 1, 10, through="a")
+    out = capsys.readouterr().out
+    assert "window: 2 dispatches" in out
+    assert "(100.0 %)" in out
 
 
 def test_trace_last_dispatches(tmp_path, capsys):
