@@ -1969,6 +1969,16 @@ __device__ __forceinline__ double wave_sum8(const double v[8], int lane) {
   return k1;
 }
 
+// a basis vector's 16 bytes, loaded non-temporally: each pass streams the
+// FP32 basis once (j vectors of 4 n0 values, far beyond the caches), and
+// keeping it out of the caches leaves them to u, w and W2 (3 % of a 257^3
+// solve)
+typedef float f4v_nt __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_basis(const float4 *p) {
+  const f4v_nt v = __builtin_nontemporal_load(reinterpret_cast<const f4v_nt *>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 constexpr int kGsE = 2;  // element quads per thread per slice
 
 // partial[i * gridDim + block], i < 2k+2: <V_i, W2 u> (i < k), <V_i, W2 w>
@@ -2012,7 +2022,7 @@ __global__ void __launch_bounds__(256) k_dots2f(long n4, const double *__restric
 #pragma unroll
         for (int e = 0; e < E; e++) {
           const long p = base + threadIdx.x + 256L * e;
-          a[t][e] = p < nq ? vt[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+          a[t][e] = p < nq ? ld_basis(vt + p) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
       }
       double sv[8];
@@ -2076,7 +2086,7 @@ __global__ void __launch_bounds__(256) k_update2f(long n4, int k, const double *
 #pragma unroll
         for (int e = 0; e < E; e++) {
           const long p = base + threadIdx.x + 256L * e;
-          a[t][e] = p < nq ? vt[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+          a[t][e] = p < nq ? ld_basis(vt + p) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
       }
 #pragma unroll
