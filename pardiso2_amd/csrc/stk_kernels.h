@@ -2146,6 +2146,18 @@ __global__ void k_cast(long n, const A *__restrict__ x, B *__restrict__ y) {
 
 // z rounded to FP32: zs = (float)z, and z = (double)zs, so that products
 // formed from z and from its stored FP32 copy agree
+// The preconditioner's last pass in mixed precision with FP32 storage: the
+// velocity slots from the FP32 V-cycle, the pressure slots (already in z)
+// rounded to FP32, each value stored in z and in the FP32 copy zs -- what
+// k_combine followed by k_round32 leave, in one pass.
+__global__ void k_combine_round(long n4, const float *__restrict__ xv, double *z, float *zs) {
+  long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u >= n4) return;
+  const float f = (u & 3) == 0 ? (float)z[u] : xv[u];
+  zs[u] = f;
+  z[u] = (double)f;
+}
+
 __global__ void k_round32(long n, double *z, float *zs) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= n) return;

@@ -976,7 +976,7 @@ void schur_bfbt(stk_ctx *c, const stk_solver_opts *o, const double *r, double *z
 
 // z = M^{-1} r : pressure by the Schur approximation, velocity by one V-cycle
 template <typename T>
-void precond(stk_ctx *c, const stk_solver_opts *o, const double *r, double *z, const int *tile) {
+void precond(stk_ctx *c, const stk_solver_opts *o, const double *r, double *z, const int *tile, float *z32 = nullptr) {
   Level &F = c->lv[0];
   long n4 = 4 * F.n;
   if (o->schur >= 1)
@@ -987,6 +987,12 @@ void precond(stk_ctx *c, const stk_solver_opts *o, const double *r, double *z, c
   Geo<double> g = F.geo<double>(c->kn.p2koef);
   hipLaunchKernelGGL(k_vrhs<T>, dim3(blocks(F.n, 256)), dim3(256), 0, c->st, g, F.flags, r, z, lvb<T>(F));
   vcycle_v<T>(c, o, tile);
+  if constexpr (std::is_same<T, float>::value) {
+    if (z32) {  // z rounded to FP32 and its FP32 copy, in the same pass
+      hipLaunchKernelGGL(k_combine_round, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, lvx<float>(F), z, z32);
+      return;
+    }
+  }
   hipLaunchKernelGGL(k_combine<T>, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, lvx<T>(F), z);
 }
 
@@ -2799,7 +2805,10 @@ static int ensure_graph(stk_ctx *c, const stk_solver_opts *o, const int *tile) {
 }
 
 // z = M^-1 r, replayed from the captured graph or launched eagerly
-static int apply_precond(stk_ctx *c, const stk_solver_opts *o, const int *tile, const double *r, double *z) {
+// z32: in mixed precision with FP32 storage, the FP32 copy of z to write
+// (z itself then holds the rounded values)
+static int apply_precond(stk_ctx *c, const stk_solver_opts *o, const int *tile, const double *r, double *z,
+                         float *z32 = nullptr) {
   const long n4 = 4 * c->lv[0].n;
   if (o->graph) {
     HIPCHK(hipMemcpyAsync(c->gin, r, n4 * 8, hipMemcpyDeviceToDevice, c->st));
@@ -2808,8 +2817,10 @@ static int apply_precond(stk_ctx *c, const stk_solver_opts *o, const int *tile, 
   } else if (o->precision == 1) {
     precond<double>(c, o, r, z, tile);
   } else {
-    precond<float>(c, o, r, z, tile);
+    precond<float>(c, o, r, z, tile, z32);
   }
+  if (z32 && o->graph)  // the captured preconditioner ends with k_combine: round here
+    hipLaunchKernelGGL(k_round32, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, z, z32);
   return STK_OK;
 }
 
@@ -2869,10 +2880,9 @@ static int fgmres(stk_ctx *c, const stk_solver_opts *o, const int *tile, double 
     bool conv = false;
     while (j < m && it < o->maxit) {
       double *u = b32 ? c->wu : c->Qs[j], *Z = b32 ? c->wz : c->Zs[j], *W = b32 ? c->ww : c->Qs[j + 1];
-      int prc = apply_precond(c, o, tile, u, Z);
-      if (prc) { status = prc; break; }
       // FP32 storage: A is applied to the stored (rounded) z, so the update x += Z y uses the same vectors
-      if (b32) hipLaunchKernelGGL(k_round32, dim3(blocks(n4, 256)), dim3(256), 0, c->st, n4, Z, z32(j));
+      int prc = apply_precond(c, o, tile, u, Z, b32 ? z32(j) : nullptr);
+      if (prc) { status = prc; break; }
       rowop<double, false>(c, 0, Z, nullptr, W, 1);
       PtrsK P{};
       PtrsKT<float> P32{};
