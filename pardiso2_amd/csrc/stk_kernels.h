@@ -1979,7 +1979,7 @@ __device__ __forceinline__ float4 ld_basis(const float4 *p) {
   return make_float4(v.x, v.y, v.z, v.w);
 }
 
-constexpr int kGsE = 2;  // element quads per thread per slice
+constexpr int kGsE = 4;  // element quads per thread per slice (4: 1.4 % faster 257^3 solves than 2)
 
 // partial[i * gridDim + block], i < 2k+2: <V_i, W2 u> (i < k), <V_i, W2 w>
 // (k + i), <u, W2 w>, <u, W2 u>
